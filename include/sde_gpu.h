@@ -1,0 +1,201 @@
+/*
+ * sde_gpu.h -- C-ABI of the MI355X implementation of SDE's imaging hot path
+ * (libsdegpu.so).
+ *
+ * Part 1: drop-in replacements for the pixel-level Fortran routines that the
+ * reference's VISGRID / VISDEGRI / IMGFFT / IMGGRIDC / GRDUWT / GRDRUWT call.
+ * They keep the Fortran-77 calling convention of the reference exactly: every
+ * argument by reference, symbol = lower-case name + '_', COMPLEX = two
+ * interleaved floats, LOGICAL = 4-byte int (non-zero = .TRUE.), arrays
+ * column-major and 1-based as seen from Fortran, one hidden trailing size_t
+ * length per CHARACTER*(*) argument.  A call is synchronous: its results are
+ * in host memory when it returns.  Each one replaces the object file of the
+ * routine named in its comment; the reference's own callers need no change.
+ *
+ * Error behaviour follows the reference: the sticky LOGICAL ERROR flag is
+ * the first word of COMMON /SYSI/ (source/include/stdinc.h:64-67, symbol
+ * sysi_).  A routine entered with ERROR set returns at once with its
+ * outputs untouched; a failure sets ERROR and reports through the host
+ * program's ERRREPOR when one is linked (else the text is kept for
+ * sde_last_error()); warnings go through MSGPUT when linked.
+ *
+ * Part 2: a plan API that keeps visibilities, grids and images resident in
+ * HBM across calls (imaging + prediction without host round trips), used by
+ * the host drivers and the benchmark.
+ */
+#ifndef SDE_GPU_H
+#define SDE_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Part 1: Fortran drop-ins                                            */
+/* ------------------------------------------------------------------ */
+
+/* Convolutional gridding of Hermitian data onto the half plane GVIS(NU,NV),
+ * zeroed by the callee; Hermitian fold of the u = 0 column, shift left by
+ * SUPPU and normalisation by (NU-1)*NV/sum(taps); SUMWT = sum of gridded
+ * weights.  Replaces GRDH2D, source/grd/grdh2d.f:7-9 (args :66-75). */
+void grdh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis,
+             float *uscale, float *uoffset, float *vscale, float *voffset,
+             int *uorigin, int *vorigin, float *gvis, int *psf, int *nu, int *nv,
+             float *gridfnu, int *suppu, int *osampu, float *gridfnv, int *suppv,
+             int *osampv, double *shift, float *sumwt);
+
+/* Nearest-cell (BOX) gridding.  Replaces GRDQH2D, source/grd/grdqh2d.f:7-9. */
+void grdqh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis,
+              float *uscale, float *uoffset, float *vscale, float *voffset,
+              int *uorigin, int *vorigin, float *gvis, int *psf, int *nu, int *nv,
+              double *shift, float *sumwt);
+
+/* Degridding (model prediction) from GVIS(NU,NV).  VIS is written for every
+ * visibility with WT > 0; WT keeps its sign as the success flag (negated when
+ * the point could not be degridded).  Replaces GRDDEH2D,
+ * source/grd/grddeh2d.f:7-9 (args :56-63). */
+void grddeh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis,
+               float *uscale, float *uoffset, float *vscale, float *voffset,
+               int *uorigin, int *vorigin, float *gvis, int *nu, int *nv,
+               float *intfnu, int *suppu, int *osampu, float *intfnv, int *suppv,
+               int *osampv, double *shift);
+
+/* Uniform weighting by a density grid GWT(NU,NV); NEWWT may alias WT.
+ * Replaces GRDUWT2D, source/grd/grduwt2d.f:7-8. */
+void grduwt2d_(float *wt, float *u, float *v, float *w, int *nvis, float *uscale,
+               float *uoffset, float *vscale, float *voffset, int *uorigin,
+               int *vorigin, float *newwt, float *gwt, int *nu, int *nv,
+               double *shift);
+
+/* Robust (Briggs) weighting, RMODE = NONE | NORM | ABS | AZ-NORM.
+ * Replaces GRDRUW2D, source/grd/grdruw2d.f:7-9. */
+void grdruw2d_(float *wt, float *u, float *v, float *w, int *nvis, float *uscale,
+               float *uoffset, float *vscale, float *voffset, int *uorigin,
+               int *vorigin, float *newwt, float *gwt, int *nu, int *nv,
+               double *shift, const char *rmode, float *robust, float *ds,
+               size_t rmode_len);
+
+/* Convolution function table (BOX | SF | SZE), OSAMP = 128, unit sum.
+ * Replaces GRDCF, source/grd/grdcf.f:7. */
+void grdcf_(const char *cfcode, float *cf, int *cfsupp, int *cfosamp, size_t len);
+
+/* Grid-correction function table.  Replaces GRDCCF, source/grd/grdccf.f:7. */
+void grdccf_(const char *ccfcode, float *ccf, int *ccflen, size_t len);
+
+/* Apply / correct (divide by) the separable grid-correction function.
+ * Replaces PIXCOR2D, source/pix/pixcor2d.f:7-8 and PIXCORX2,
+ * source/pix/pixcorx2.f:7-8 (complex images). */
+void pixcor2d_(float *in, int *nx, int *ny, float *cf, int *cflen, const char *mode,
+               float *out, float *cenx, float *ceny, size_t mode_len);
+void pixcorx2_(float *in, int *nx, int *ny, float *cf, int *cflen, const char *mode,
+               float *out, float *cenx, float *ceny, size_t mode_len);
+
+/* Centred 2-D real <-> half-plane complex FFT.  DIR < 0: XARRAY(FNU/2+1,NV)
+ * -> RARRAY(NX,NY), exp(+i) and 1/(FNU*NV); DIR > 0: RARRAY -> XARRAY,
+ * exp(-i), unnormalised.  XARRAY is overwritten by the column pass as in the
+ * reference.  WORK, WORK2, RWORK are accepted and not used.  Power-of-two
+ * FNU (4..16384) and NV (2..8192).  Replaces FFTR2D, source/fft/fftr2d.f:7-8. */
+void fftr2d_(float *rarray, float *xarray, float *work, float *work2, float *rwork,
+             int *nx, int *ny, int *fnu, int *nv, int *dir);
+
+/* Centred 1-D complex FFT in place (power-of-two NX, 2..8192).
+ * Replaces FFTX1D, source/fft/fftx1d.f:7. */
+void fftx1d_(float *xarray, float *xwork, float *xwork2, int *nx, int *dir);
+
+/* Next power of two.  Replaces FFTPWR2, source/fft/fftpwr2.f. */
+int fftpwr2_(int *n);
+
+/* The sticky ERROR flag and messages. */
+int sde_error_flag(void);
+void sde_error_clear(void);
+const char *sde_last_error(void);
+const char *sde_last_warning(void);
+/* Number of visible GPUs (0 when none). */
+int sde_device_count(void);
+/* Library version string. */
+const char *sde_version(void);
+
+/* ------------------------------------------------------------------ */
+/* Part 2: device-resident plan API                                    */
+/* ------------------------------------------------------------------ */
+
+typedef struct sde_plan sde_plan;
+
+typedef struct {
+    int nx, ny;            /* image size (pixels) */
+    float cellsize;        /* arcsec, both axes (IMGMAKE: DELT = -cell/3600 on RA) */
+    char cftype[4];        /* "SF" or "SZE" (BOX: use grdqh2d_) */
+    double shift[9];       /* CRDSHIFT matrix, column-major (identity: no shift) */
+    int weighting;         /* 0 natural, 1 density weighting (GRDUWT2D / GRDRUW2D) */
+    float fov;             /* WTFOV of the density grid */
+    char rmode[8];         /* "NONE" (uniform), "NORM", "ABS", "AZ-NORM" */
+    float robust[3];
+    float ds;
+    int tile;              /* gridding tile edge in cells: 0 = tuned default, else 16 or 32 */
+    int chunk;             /* visibilities per work item: 0 = tuned default */
+    int waves;             /* waves per gridding workgroup: 0 = tuned default, else 1, 2, 4 */
+} sde_imaging_cfg;
+
+/* Per-visibility accounting of the last run. */
+typedef struct {
+    int64_t nvis;          /* visibilities held */
+    int64_t ngridded;      /* gridded (weight > 0 and inside the grid) */
+    int64_t ndropped;      /* dropped off the density grid (weight set to -1) */
+    int64_t nbad;          /* not degridded (weight left negated) */
+    double sumwt;          /* sum of gridded weights (double) */
+    double lsumwt;         /* sum of all convolution taps (normalisation) */
+    float maxpsf;          /* PSF peak before normalisation */
+    int nitems;            /* gridding work items of the last run */
+} sde_run_info;
+
+int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **plan);
+int sde_plan_destroy(sde_plan *plan);
+/* Hand over visibilities.  on_device = 0: host arrays, copied to HBM;
+ * on_device = 1: device pointers, copied device-to-device.  vis is complex
+ * (2*nvis floats). */
+int sde_plan_set_vis(sde_plan *plan, int64_t nvis, const float *vis, const float *wt,
+                     const float *u, const float *v, const float *w, int on_device);
+/* Weighting, PSF and dirty image (both divided by the PSF peak), all on the
+ * device (task/uvmap.f:276-282,428-482). */
+int sde_plan_uvmap(sde_plan *plan);
+/* Model prediction from an image (host NX*NY floats, or NULL for the plan's
+ * dirty image): correct, FFT, degrid (task/vismodel.f:69-78). */
+int sde_plan_vismodel(sde_plan *plan, const float *image);
+int sde_plan_sync(sde_plan *plan);
+int sde_plan_get_images(sde_plan *plan, float *dirty, float *psf);
+int sde_plan_get_grids(sde_plan *plan, float *dirty_grid, float *psf_grid);
+int sde_plan_get_weights(sde_plan *plan, float *wt);
+int sde_plan_get_model(sde_plan *plan, float *vis, float *wt);
+int sde_plan_info(sde_plan *plan, sde_run_info *info);
+/* Geometry: nu, nv, fnu, uorigin, vorigin and the float uscale/vscale. */
+int sde_plan_geometry(sde_plan *plan, int *nu, int *nv, int *fnu, int *uorigin,
+                      int *vorigin, float *uscale, float *vscale);
+/* Device pointers of the resident visibility arrays (for synthetic input). */
+int sde_plan_device_vis(sde_plan *plan, float **vis, float **wt, float **u, float **v,
+                        float **w);
+/* Per-kernel timing: enable = 1 records HIP events around every launch on the
+ * plan's stream; sde_plan_timings fills up to max entries of (name, total ms,
+ * launches) for the runs since the last reset. */
+int sde_plan_profile(sde_plan *plan, int enable);
+int sde_plan_timings(sde_plan *plan, int max, char (*names)[32], double *ms, int *count);
+int sde_plan_reset_timings(sde_plan *plan);
+
+/* Synthetic track: uvw of baselines between antennas (ENU metres) for hour
+ * angles ha0 + k*dha (k < ntime, radians), declination dec, latitude lat,
+ * for nchan frequencies; point-source sky (flux, l, m in radians) plus
+ * Gaussian noise sigma from a counter-based generator seeded by seed.
+ * Output order (channel, time, baseline); a uniform-stride subsample of
+ * nkeep of the full set when 0 < nkeep < full size.  Writes into the plan's
+ * resident arrays (weights 1). */
+int sde_plan_synth(sde_plan *plan, int nant, const double *enu, double lat, double dec,
+                   double ha0, double dha, int ntime, int nchan, const double *freq,
+                   int nsrc, const double *flux, const double *l, const double *m,
+                   double sigma, uint64_t seed, int64_t nkeep);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,75 @@
+"""MI355X implementation of SDE's imaging hot path.
+
+The compute lives in libsdegpu.so (hand-written HIP for gfx950, built by
+``sde_amd.build``).  This package is the host-side mirror of the reference
+interface:
+
+  * ``sde_amd.grd``      the pixel-level routines with the reference's
+                         argument lists (GRDH2D, GRDDEH2D, GRDUWT2D, ...),
+                         called through the Fortran ABI of the library;
+  * ``sde_amd.imaging``  uvmap / vismodel drivers on a device-resident plan;
+  * ``sde_amd.synth``    seeded synthetic observations of the benchmark
+                         workloads.
+
+There is no CPU fallback: importing a compute module without the built
+library raises.
+"""
+import ctypes as _C
+import os as _os
+
+_HERE = _os.path.dirname(_os.path.abspath(__file__))
+LIBPATH = _os.path.join(_HERE, "libsdegpu.so")
+
+_lib = None
+
+
+class SdeError(RuntimeError):
+    """Raised when the library reports an error (the reference's ERRREPOR)."""
+
+
+def lib():
+    """The loaded libsdegpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not _os.path.exists(LIBPATH):
+            raise ImportError("libsdegpu.so is not built (run python -m sde_amd.build): "
+                              "the HIP implementation is required, there is no CPU path")
+        _lib = _C.CDLL(LIBPATH)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L):
+    L.sde_error_flag.restype = _C.c_int
+    L.sde_last_error.restype = _C.c_char_p
+    L.sde_last_warning.restype = _C.c_char_p
+    L.sde_device_count.restype = _C.c_int
+    L.sde_version.restype = _C.c_char_p
+
+
+def device_count():
+    return lib().sde_device_count()
+
+
+def error_flag():
+    return bool(lib().sde_error_flag())
+
+
+def clear_error():
+    lib().sde_error_clear()
+
+
+def last_error():
+    return lib().sde_last_error().decode()
+
+
+def last_warning():
+    return lib().sde_last_warning().decode()
+
+
+def check_error(routine=""):
+    """Raise SdeError if the sticky ERROR flag is set (and clear it)."""
+    if lib().sde_error_flag():
+        msg = last_error()
+        clear_error()
+        raise SdeError(msg or ("error in " + routine))

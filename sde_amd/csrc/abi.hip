@@ -1,0 +1,1355 @@
+// Host side of libsdegpu.so: the Fortran-callable drop-ins for the
+// reference's pixel-level routines, the reference's error conventions, the
+// kernel tables (GRDCF/GRDCCF/GRDSF/GRDSZE) and the device-resident plan.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <functional>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/sde_gpu.h"
+#include "internal.h"
+
+// ---------------------------------------------------------------------------
+// Reference error conventions (source/include/stdinc.h:64-67, main/err.f:532)
+// ---------------------------------------------------------------------------
+
+extern "C" {
+// COMMON /SYSI/ ERROR, MSGWIDTH, SYSDEBUG, SYSMON, SYSINTRP, SYSMSG, SYSPAR.
+// A Fortran host program that declares the common block interposes its own
+// definition, so the drop-ins see and set the program's ERROR flag.
+struct sysi_block {
+    int error, msgwidth, sysdebug, sysmon, sysintrp, sysmsg, syspar;
+};
+__attribute__((visibility("default"))) sysi_block sysi_ = {0, 80, 0, 0, 0, 1, 0};
+
+void errrepor_(const char *, const char *, const char *, size_t, size_t, size_t) __attribute__((weak));
+void msgput_(const char *, const char *, size_t, size_t) __attribute__((weak));
+}
+
+namespace {
+
+std::mutex g_msg_mu;
+std::string g_last_error, g_last_warning;
+
+const char *const kErrBdArg = "Bad subroutine argument";
+const char *const kErrFatal = "Fatal error";
+const char *const kErrLogic = "Program logic error";
+const char *const kErrNoMem = "No memory available";
+
+void report_error(const char *reason, const char *routine, const char *text) {
+    {
+        std::lock_guard<std::mutex> lk(g_msg_mu);
+        g_last_error = std::string(reason) + " in routine " + routine + ", " + text;
+    }
+    sysi_.error = 1;
+    if (errrepor_) errrepor_(reason, routine, text, strlen(reason), strlen(routine), strlen(text));
+}
+
+void report_warning(const char *text) {
+    {
+        std::lock_guard<std::mutex> lk(g_msg_mu);
+        g_last_warning = text;
+    }
+    if (msgput_) msgput_(text, "W", strlen(text), 1);
+}
+
+void set_plan_error(const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    std::lock_guard<std::mutex> lk(g_msg_mu);
+    g_last_error = buf;
+}
+
+// Fortran CHARACTER*(*) argument -> blank-padded std::string
+std::string fstr(const char *s, size_t len) {
+    std::string r = s ? std::string(s, len) : std::string();
+    while (r.size() < 8) r.push_back(' ');
+    return r;
+}
+bool fstarts(const std::string &s, const char *p) { return strncmp(s.c_str(), p, strlen(p)) == 0; }
+bool feq(const std::string &s, const char *p) {
+    size_t n = strlen(p);
+    if (strncmp(s.c_str(), p, n) != 0) return false;
+    for (size_t k = n; k < s.size(); k++)
+        if (s[k] != ' ' && s[k] != '\0') return false;
+    return true;
+}
+
+}  // namespace
+
+int sde_fail_hip(hipError_t e, const char *what, const char *file, int line) {
+    char buf[512];
+    snprintf(buf, sizeof buf, "HIP error %s (%d) at %s:%d: %s", hipGetErrorString(e), (int)e, file, line, what);
+    std::lock_guard<std::mutex> lk(g_msg_mu);
+    g_last_error = buf;
+    return -1;
+}
+
+// ---------------------------------------------------------------------------
+// Kernel tables: GRDSF (grd/grdsf.f:39-73), GRDSZE (grd/grdsze.f:41-79),
+// GRDCF (grd/grdcf.f:41-90), GRDCCF (grd/grdccf.f:33-60), FFTPWR2.
+// ---------------------------------------------------------------------------
+namespace {
+
+double ipow_d(double a, int n) {  // integer power by repeated squaring
+    double r = 1.0, x = a;
+    for (unsigned k = (unsigned)n; k;) {
+        if (k & 1) r *= x;
+        k >>= 1;
+        if (k) x *= x;
+    }
+    return r;
+}
+float ipow_f(float a, int n) {
+    float r = 1.0f, x = a;
+    for (unsigned k = (unsigned)n; k;) {
+        if (k & 1) r *= x;
+        k >>= 1;
+        if (k) x *= x;
+    }
+    return r;
+}
+
+// Prolate spheroidal wave function, m = 6, alpha = 1 (Schwab's rational fit).
+double spheroidal(double nu) {
+    static const double p[2][5] = {{8.203343e-2, -3.644705e-1, 6.278660e-1, -5.335581e-1, 2.312756e-1},
+                                   {4.028559e-3, -3.697768e-2, 1.021332e-1, -1.201436e-1, 6.412774e-2}};
+    static const double q[2][3] = {{1.0, 8.212018e-1, 2.078043e-1}, {1.0, 9.599102e-1, 2.918724e-1}};
+    int part;
+    double end;
+    if (nu >= 0.0 && nu < 0.75) { part = 0; end = 0.75; }
+    else if (nu >= 0.75 && nu <= 1.0) { part = 1; end = 1.0; }
+    else return 0.0;
+    const double d = nu * nu - end * end;
+    double top = p[part][0], bot = q[part][0];
+    for (int k = 1; k <= 4; k++) top = top + p[part][k] * ipow_d(d, k);
+    for (int k = 1; k <= 2; k++) bot = bot + q[part][k] * ipow_d(d, k);
+    return bot != 0.0 ? top / bot : 0.0;
+}
+
+// Sze Tan's optimal gridding function, R = 3, X0 = 0.45 (REAL arithmetic).
+float sze_fn(float nu) {
+    static const float p[3][5] = {
+        {6.0527199e-1f, 3.3587534e-1f, -2.1497621e-1f, 3.8904394e-2f, -5.3295409e-3f},
+        {1.1049142e-1f, -3.4967472e-2f, -6.6916751e-4f, 1.1027297e-3f, -2.5684433e-4f},
+        {6.0855041e-4f, -1.8814394e-3f, 6.5014897e-4f, -1.9612551e-4f, 9.4834730e-7f}};
+    static const float q[3][3] = {{1.0f, 1.0711361f, 8.1322567e-2f},
+                                  {1.0f, 3.1491945e-1f, 2.3050743e-2f},
+                                  {1.0f, 1.4856126e-1f, 8.5105642e-3f}};
+    int part;
+    float end;
+    if (nu >= 0.0f && nu < 1.0f) { part = 0; end = 1.0f; }
+    else if (nu >= 1.0f && nu <= 2.0f) { part = 1; end = 2.0f; }
+    else if (nu >= 2.0f && nu <= 3.0f) { part = 2; end = 3.0f; }
+    else return 0.0f;
+    const float d = nu * nu - end * end;
+    float top = p[part][0], bot = q[part][0];
+    for (int k = 1; k <= 4; k++) top = top + p[part][k] * ipow_f(d, k);
+    for (int k = 1; k <= 2; k++) bot = bot + q[part][k] * ipow_f(d, k);
+    return bot != 0.0f ? top / bot : 0.0f;
+}
+
+int pow2_at_least(int n) {
+    if (n < 1) return 1;
+    int p = 1;
+    while (p < n) p *= 2;
+    return p;
+}
+
+// Returns 0 and fills cf[1 + 2*(supp+1)*osamp], or 1 for an unknown code.
+int make_cf(const std::string &code, float *cf, int *supp_out, int *osamp_out) {
+    const int osamp = 128;
+    int supp;
+    if (fstarts(code, "BOX")) {
+        supp = 1;
+        const int c = (supp + 1) * osamp;
+        for (int i = 0; i <= osamp / 2; i++) cf[c + i] = cf[c - i] = 1.0f;
+        for (int i = osamp / 2 + 1; i <= 2 * osamp; i++) cf[c + i] = cf[c - i] = 0.0f;
+    } else if (fstarts(code, "SF") || fstarts(code, "SZE")) {
+        const bool sf = fstarts(code, "SF");
+        supp = 3;
+        const int c = (supp + 1) * osamp;
+        for (int i = 0; i <= supp * osamp; i++) {
+            float val;
+            if (sf) {
+                const double nu = (double)i / (double)(supp * osamp);
+                val = (float)((1.0 - nu * nu) * spheroidal(nu));
+            } else {
+                val = sze_fn((float)((double)i / (double)osamp));
+            }
+            cf[c + i] = cf[c - i] = val;
+        }
+        for (int i = supp * osamp + 1; i <= (supp + 1) * osamp; i++) cf[c + i] = cf[c - i] = 0.0f;
+    } else {
+        return 1;
+    }
+    const int c = (supp + 1) * osamp;
+    float sum = 0.0f;
+    for (int i = -c; i <= c; i++) sum = sum + cf[c + i];
+    for (int i = -c; i <= c; i++) cf[c + i] = cf[c + i] / sum;
+    *supp_out = supp;
+    *osamp_out = osamp;
+    return 0;
+}
+
+int make_ccf(const std::string &code, float *ccf, int len) {
+    if (fstarts(code, "BOX")) {
+        const float pi = 3.141592554f;
+        ccf[0] = 1.0f;
+        for (int i = 2; i <= len; i++) {
+            const float x = 0.5f * pi * (float)((double)(i - 1) / (double)(len - 1));
+            ccf[i - 1] = sinf(x) / x;
+        }
+    } else if (fstarts(code, "SF")) {
+        for (int i = 1; i <= len; i++) ccf[i - 1] = (float)spheroidal((double)(i - 1) / (double)(len - 1));
+    } else if (fstarts(code, "SZE")) {
+        for (int i = 1; i <= len; i++) ccf[i - 1] = sze_fn((float)((double)(i - 1) / (double)(len - 1)));
+    } else {
+        return 1;
+    }
+    return 0;
+}
+
+// uv cell of a transformed axis (fft/fftconj.f:45-84), REAL arithmetic.
+float conj_delt(int n, float delt_deg) {
+    const float pi = 3.141592654f;
+    const float d2r = pi / 180.0f;
+    const float width = (float)pow2_at_least(n) * delt_deg;
+    return width != 0.0f ? 1.0f / (d2r * width) : 1.0f;
+}
+
+double ref_twopi() { return (double)(8.0f * atanf(1.0f)); }
+
+// exp(+2 pi i k / n) in double, exact at the quadrant points.
+std::vector<float2> twiddles(int n) {
+    std::vector<float2> t(n);
+    for (int k = 0; k < n; k++) {
+        float2 z;
+        if (n % 4 == 0 && k % (n / 4) == 0) {
+            const int q = k / (n / 4);
+            const float c[4] = {1.0f, 0.0f, -1.0f, 0.0f}, s[4] = {0.0f, 1.0f, 0.0f, -1.0f};
+            z = make_float2(c[q], s[q]);
+        } else {
+            const long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)n;
+            z = make_float2((float)cosl(a), (float)sinl(a));
+        }
+        t[k] = z;
+    }
+    return t;
+}
+
+// ---------------------------------------------------------------------------
+// Device context shared by the Fortran entry points (single-threaded like the
+// reference; a mutex serialises concurrent callers).
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+struct Ctx {
+    bool ready = false;
+    int ndev = 0;
+    hipStream_t stream = nullptr;
+    std::map<std::string, DevBuf> bufs;
+    std::map<int, float2 *> tw;
+};
+
+std::mutex g_ctx_mu;
+Ctx g_ctx;
+
+int ctx_init(Ctx &c) {
+    if (c.ready) return 0;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        set_plan_error("no GPU visible: libsdegpu needs an MI355X (gfx950) device");
+        return -1;
+    }
+    c.ndev = n;
+    SDE_CHECK(hipSetDevice(0));
+    SDE_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    c.ready = true;
+    return 0;
+}
+
+template <typename T>
+T *ctx_buf(Ctx &c, const char *name, size_t count) {
+    DevBuf &b = c.bufs[name];
+    const size_t bytes = count * sizeof(T) + 256;
+    if (b.cap < bytes) {
+        if (b.p) (void)hipFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+        if (hipMalloc(&b.p, bytes) != hipSuccess) return nullptr;
+        b.cap = bytes;
+    }
+    return static_cast<T *>(b.p);
+}
+
+int get_twiddles(std::map<int, float2 *> &cache, int n, hipStream_t s, float2 **out) {
+    auto it = cache.find(n);
+    if (it != cache.end()) {
+        *out = it->second;
+        return 0;
+    }
+    std::vector<float2> t = twiddles(n);
+    float2 *d = nullptr;
+    SDE_CHECK(hipMalloc(&d, sizeof(float2) * n));
+    SDE_CHECK(hipMemcpyAsync(d, t.data(), sizeof(float2) * n, hipMemcpyHostToDevice, s));
+    SDE_CHECK(hipStreamSynchronize(s));
+    cache[n] = d;
+    *out = d;
+    return 0;
+}
+
+bool is_identity_diag(const double *s) { return s[0] == 1.0 && s[4] == 1.0 && s[8] == 1.0; }
+
+CoordParams make_cp(const double *shift, float uscale, float uoffset, float vscale, float voffset,
+                    int uorigin, int vorigin, int nu, int nv, int suppu, int suppv, int osampu,
+                    int osampv) {
+    CoordParams cp;
+    memcpy(cp.shift, shift, sizeof cp.shift);
+    cp.twopi = ref_twopi();
+    cp.uscale = uscale;
+    cp.uoffset = uoffset;
+    cp.vscale = vscale;
+    cp.voffset = voffset;
+    cp.uorigin = uorigin;
+    cp.vorigin = vorigin;
+    cp.nu = nu;
+    cp.nv = nv;
+    cp.suppu = suppu;
+    cp.suppv = suppv;
+    cp.osampu = osampu;
+    cp.osampv = osampv;
+    cp.doshift = is_identity_diag(shift) ? 0 : 1;
+    return cp;
+}
+
+TileParams make_tp(int nu, int nv, int T, int chunk) {
+    TileParams tp;
+    tp.T = T;
+    tp.ntu = (nu + T - 1) / T;
+    tp.ntv = (nv + T - 1) / T;
+    tp.ntiles = tp.ntu * tp.ntv;
+    tp.chunk = chunk;
+    return tp;
+}
+
+int default_tile(int nu, int nv) { (void)nu; (void)nv; return 16; }
+int default_waves(int T) { return T == 16 ? 4 : 1; }
+int default_chunk(int64_t nvis) { return nvis > (1 << 22) ? 4096 : 1024; }
+
+// Buffers of one gridding pass (tile sort + LDS accumulation + reduce).
+struct GridBufs {
+    int *tile_of, *tile_count, *tile_start, *item_start, *tile_cursor, *nitems;
+    GridRec *rec, *sorted;
+    WorkItem *items;
+    float *partial;
+    double *blk, *sums;
+    int max_items, nblk;
+};
+
+// Runs the gridding pipeline on the stream; cf tables and arrays on device.
+int run_gridding(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
+                 const float *w, int64_t nvis, const CoordParams &cp, const TileParams &tp, int waves,
+                 int chmask, const float *cfu, const float *cfv, GridBufs &b, float2 *dirty, float2 *psf,
+                 const std::function<void(const char *)> &mark_begin,
+                 const std::function<void()> &mark_end) {
+    SDE_CHECK(hipMemsetAsync(b.tile_count, 0, sizeof(int) * tp.ntiles, s));
+    mark_begin("grid_prep");
+    launch_grid_prep(s, vis, wt, u, v, w, nvis, cp, tp, chmask == 2, cfu, cfv, b.tile_of, b.rec, b.tile_count,
+                     b.blk, b.nblk);
+    mark_end();
+    mark_begin("grid_scan");
+    launch_grid_scan(s, b.tile_count, tp.ntiles, tp.chunk, b.tile_start, b.item_start, b.tile_cursor, b.nitems,
+                     b.blk, b.nblk, cp, b.sums);
+    mark_end();
+    mark_begin("grid_scatter");
+    launch_grid_scatter(s, b.tile_of, b.rec, nvis, b.tile_start, b.tile_cursor, b.sorted);
+    mark_end();
+    mark_begin("grid_items");
+    launch_grid_items(s, b.tile_start, b.item_start, tp.ntiles, tp.chunk, b.items);
+    mark_end();
+    mark_begin("grid_tiles");
+    if (launch_grid_tiles(s, b.sorted, b.items, b.nitems, b.max_items, tp, chmask, waves, cfu, cfv, cp,
+                          b.partial)) {
+        mark_end();
+        set_plan_error("unsupported gridding tile configuration T=%d waves=%d", tp.T, waves);
+        return -1;
+    }
+    mark_end();
+    mark_begin("grid_reduce");
+    launch_grid_reduce(s, b.partial, b.item_start, tp, chmask, cp, b.sums, dirty, psf);
+    mark_end();
+    SDE_CHECK(hipGetLastError());
+    return 0;
+}
+
+int max_items_for(const TileParams &tp, int64_t nvis) {
+    return tp.ntiles + (int)((nvis + tp.chunk - 1) / tp.chunk) + 1;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Fortran drop-ins
+// ---------------------------------------------------------------------------
+namespace {
+
+void noop_begin(const char *) {}
+void noop_end() {}
+
+bool check_kernel_args(const char *routine, int nvis, int nu, int nv, int suppu, int suppv, int osampu,
+                       int osampv, int uorigin = 1, int vorigin = 1) {
+    char msg[160];
+    if (uorigin != 1 || vorigin < 1 || vorigin > nv) {
+        snprintf(msg, sizeof msg, "grid origin must be UORIGIN = 1, 1 <= VORIGIN <= NV (got %d, %d)", uorigin,
+                 vorigin);
+        report_error(kErrBdArg, routine, msg);
+        return false;
+    }
+    if (nvis < 0 || nu < 1 || nv < 1) {
+        snprintf(msg, sizeof msg, "bad sizes NVIS=%d NU=%d NV=%d", nvis, nu, nv);
+        report_error(kErrBdArg, routine, msg);
+        return false;
+    }
+    if (suppu < 0 || suppv < 0 || suppu > kHalo || suppv > kHalo || (2 * suppu + 1) * (2 * suppv + 1) > 64 ||
+        osampu < 1 || osampv < 1 || osampu > 252 || osampv > 252) {
+        snprintf(msg, sizeof msg, "unsupported kernel SUPP=%d,%d OSAMP=%d,%d", suppu, suppv, osampu, osampv);
+        report_error(kErrBdArg, routine, msg);
+        return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sde_error_flag(void) { return sysi_.error; }
+void sde_error_clear(void) {
+    sysi_.error = 0;
+    std::lock_guard<std::mutex> lk(g_msg_mu);
+    g_last_error.clear();
+    g_last_warning.clear();
+}
+const char *sde_last_error(void) {
+    static thread_local std::string copy;
+    std::lock_guard<std::mutex> lk(g_msg_mu);
+    copy = g_last_error;
+    return copy.c_str();
+}
+const char *sde_last_warning(void) {
+    static thread_local std::string copy;
+    std::lock_guard<std::mutex> lk(g_msg_mu);
+    copy = g_last_warning;
+    return copy.c_str();
+}
+int sde_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+const char *sde_version(void) { return "sde-mi355x 0.1 (gfx950)"; }
+
+int fftpwr2_(int *n) { return pow2_at_least(*n); }
+
+void grdcf_(const char *cfcode, float *cf, int *cfsupp, int *cfosamp, size_t len) {
+    if (sysi_.error) return;
+    const std::string code = fstr(cfcode, len);
+    if (make_cf(code, cf, cfsupp, cfosamp)) {
+        std::string t = "Cannot make convolution function " + std::string(cfcode, len);
+        report_error(kErrBdArg, "GRDCF", t.c_str());
+    }
+}
+
+void grdccf_(const char *ccfcode, float *ccf, int *ccflen, size_t len) {
+    if (sysi_.error) return;
+    const std::string code = fstr(ccfcode, len);
+    if (make_ccf(code, ccf, *ccflen)) {
+        std::string t = "Cannot make convolution correction function " + std::string(ccfcode, len);
+        report_error(kErrBdArg, "GRDCCF", t.c_str());
+    }
+}
+
+void grdh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, float *uscale, float *uoffset,
+             float *vscale, float *voffset, int *uorigin, int *vorigin, float *gvis, int *psf, int *nu, int *nv,
+             float *gridfnu, int *suppu, int *osampu, float *gridfnv, int *suppv, int *osampv, double *shift,
+             float *sumwt) {
+    if (sysi_.error) return;
+    const char *R = "GRDH2D";
+    if (!check_kernel_args(R, *nvis, *nu, *nv, *suppu, *suppv, *osampu, *osampv, *uorigin, *vorigin)) return;
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    Ctx &c = g_ctx;
+    if (ctx_init(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
+    hipStream_t s = c.stream;
+    const int64_t n = *nvis;
+    const int ncfu = 1 + 2 * (*suppu + 1) * *osampu, ncfv = 1 + 2 * (*suppv + 1) * *osampv;
+    const CoordParams cp = make_cp(shift, *uscale, *uoffset, *vscale, *voffset, *uorigin, *vorigin, *nu, *nv,
+                                   *suppu, *suppv, *osampu, *osampv);
+    const TileParams tp = make_tp(*nu, *nv, default_tile(*nu, *nv), default_chunk(n));
+    const int chmask = *psf ? 2 : 1;
+    GridBufs b;
+    b.max_items = max_items_for(tp, n);
+    b.nblk = grid_prep_blocks(n);
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    float2 *dvis = ctx_buf<float2>(c, "vis", nn);
+    float *dwt = ctx_buf<float>(c, "wt", nn), *du = ctx_buf<float>(c, "u", nn);
+    float *dv = ctx_buf<float>(c, "v", nn), *dw = ctx_buf<float>(c, "w", nn);
+    float *dcfu = ctx_buf<float>(c, "cfu", ncfu), *dcfv = ctx_buf<float>(c, "cfv", ncfv);
+    b.tile_of = ctx_buf<int>(c, "tile_of", nn);
+    b.rec = ctx_buf<GridRec>(c, "rec", nn);
+    b.sorted = ctx_buf<GridRec>(c, "sorted", nn);
+    b.tile_count = ctx_buf<int>(c, "tile_count", tp.ntiles + 1);
+    b.tile_start = ctx_buf<int>(c, "tile_start", tp.ntiles + 1);
+    b.item_start = ctx_buf<int>(c, "item_start", tp.ntiles + 1);
+    b.tile_cursor = ctx_buf<int>(c, "tile_cursor", tp.ntiles + 1);
+    b.nitems = ctx_buf<int>(c, "nitems", 1);
+    b.items = ctx_buf<WorkItem>(c, "items", b.max_items);
+    b.partial = ctx_buf<float>(c, "partial", grid_partial_floats(tp, chmask, b.max_items));
+    b.blk = ctx_buf<double>(c, "blk", 2 * b.nblk);
+    b.sums = ctx_buf<double>(c, "sums", 4);
+    float2 *dg = ctx_buf<float2>(c, "grid", (size_t)*nu * *nv);
+    if (!dvis || !dwt || !du || !dv || !dw || !dcfu || !dcfv || !b.tile_of || !b.rec || !b.sorted || !b.tile_count ||
+        !b.tile_start || !b.item_start || !b.tile_cursor || !b.nitems || !b.items || !b.partial || !b.blk ||
+        !b.sums || !dg) {
+        report_error(kErrNoMem, R, "device allocation failed");
+        return;
+    }
+#define SDE_F(expr)                                                     \
+    do {                                                                \
+        hipError_t e_ = (expr);                                         \
+        if (e_ != hipSuccess) {                                         \
+            sde_fail_hip(e_, #expr, __FILE__, __LINE__);                \
+            report_error(kErrFatal, R, sde_last_error());               \
+            return;                                                     \
+        }                                                               \
+    } while (0)
+    if (n > 0) {
+        SDE_F(hipMemcpyAsync(dvis, vis, sizeof(float2) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(dwt, wt, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(du, u, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(dv, v, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(dw, w, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    }
+    SDE_F(hipMemcpyAsync(dcfu, gridfnu, sizeof(float) * ncfu, hipMemcpyHostToDevice, s));
+    SDE_F(hipMemcpyAsync(dcfv, gridfnv, sizeof(float) * ncfv, hipMemcpyHostToDevice, s));
+    if (run_gridding(s, dvis, dwt, du, dv, dw, n, cp, tp, default_waves(tp.T), chmask, dcfu, dcfv, b,
+                     chmask == 1 ? dg : nullptr, chmask == 2 ? dg : nullptr, noop_begin, noop_end)) {
+        report_error(kErrFatal, R, sde_last_error());
+        return;
+    }
+    double sums[3];
+    SDE_F(hipMemcpyAsync(gvis, dg, sizeof(float2) * (size_t)*nu * *nv, hipMemcpyDeviceToHost, s));
+    SDE_F(hipMemcpyAsync(sums, b.sums, sizeof sums, hipMemcpyDeviceToHost, s));
+    SDE_F(hipStreamSynchronize(s));
+    *sumwt = (float)sums[0];
+    if (sums[1] == 0.0) report_error(kErrBdArg, R, "No weight!");
+}
+
+void grdqh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, float *uscale, float *uoffset,
+              float *vscale, float *voffset, int *uorigin, int *vorigin, float *gvis, int *psf, int *nu, int *nv,
+              double *shift, float *sumwt) {
+    if (sysi_.error) return;
+    const char *R = "GRDQH2D";
+    if (!check_kernel_args(R, *nvis, *nu, *nv, 0, 0, 1, 1)) return;
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    Ctx &c = g_ctx;
+    if (ctx_init(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
+    hipStream_t s = c.stream;
+    const int64_t n = *nvis;
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    const CoordParams cp = make_cp(shift, *uscale, *uoffset, *vscale, *voffset, *uorigin, *vorigin, *nu, *nv, 0, 0,
+                                   1, 1);
+    float2 *dvis = ctx_buf<float2>(c, "vis", nn);
+    float *dwt = ctx_buf<float>(c, "wt", nn), *du = ctx_buf<float>(c, "u", nn);
+    float *dv = ctx_buf<float>(c, "v", nn), *dw = ctx_buf<float>(c, "w", nn);
+    double *acc = ctx_buf<double>(c, "boxacc", 2 * (size_t)*nu * *nv);
+    double *sums = ctx_buf<double>(c, "sums", 4);
+    float2 *dg = ctx_buf<float2>(c, "grid", (size_t)*nu * *nv);
+    if (!dvis || !dwt || !du || !dv || !dw || !acc || !sums || !dg) {
+        report_error(kErrNoMem, R, "device allocation failed");
+        return;
+    }
+    if (n > 0) {
+        SDE_F(hipMemcpyAsync(dvis, vis, sizeof(float2) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(dwt, wt, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(du, u, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(dv, v, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(dw, w, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    }
+    SDE_F(hipMemsetAsync(acc, 0, sizeof(double) * 2 * (size_t)*nu * *nv, s));
+    SDE_F(hipMemsetAsync(sums, 0, sizeof(double) * 4, s));
+    launch_box_grid(s, dvis, dwt, du, dv, dw, n, cp, *psf, acc, sums);
+    launch_box_fold(s, acc, cp, sums, dg);
+    SDE_F(hipGetLastError());
+    double hs[2];
+    SDE_F(hipMemcpyAsync(gvis, dg, sizeof(float2) * (size_t)*nu * *nv, hipMemcpyDeviceToHost, s));
+    SDE_F(hipMemcpyAsync(hs, sums, sizeof hs, hipMemcpyDeviceToHost, s));
+    SDE_F(hipStreamSynchronize(s));
+    *sumwt = (float)hs[0];
+    if (hs[1] == 0.0) report_error(kErrBdArg, R, "No weight!");
+}
+
+void grddeh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, float *uscale, float *uoffset,
+               float *vscale, float *voffset, int *uorigin, int *vorigin, float *gvis, int *nu, int *nv,
+               float *intfnu, int *suppu, int *osampu, float *intfnv, int *suppv, int *osampv, double *shift) {
+    if (sysi_.error) return;
+    const char *R = "GRDDEH2D";
+    if (!check_kernel_args(R, *nvis, *nu, *nv, *suppu, *suppv, *osampu, *osampv, *uorigin, *vorigin)) return;
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    Ctx &c = g_ctx;
+    if (ctx_init(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
+    hipStream_t s = c.stream;
+    const int64_t n = *nvis;
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    const int ncfu = 1 + 2 * (*suppu + 1) * *osampu, ncfv = 1 + 2 * (*suppv + 1) * *osampv;
+    const CoordParams cp = make_cp(shift, *uscale, *uoffset, *vscale, *voffset, *uorigin, *vorigin, *nu, *nv,
+                                   *suppu, *suppv, *osampu, *osampv);
+    float2 *dvis = ctx_buf<float2>(c, "vis", nn);
+    float *dwt = ctx_buf<float>(c, "wt", nn), *du = ctx_buf<float>(c, "u", nn);
+    float *dv = ctx_buf<float>(c, "v", nn), *dw = ctx_buf<float>(c, "w", nn);
+    float *dcfu = ctx_buf<float>(c, "cfu", ncfu), *dcfv = ctx_buf<float>(c, "cfv", ncfv);
+    float2 *dg = ctx_buf<float2>(c, "grid", (size_t)*nu * *nv);
+    unsigned long long *cnt = ctx_buf<unsigned long long>(c, "counters", 2);
+    if (!dvis || !dwt || !du || !dv || !dw || !dcfu || !dcfv || !dg || !cnt) {
+        report_error(kErrNoMem, R, "device allocation failed");
+        return;
+    }
+    if (n > 0) {
+        SDE_F(hipMemcpyAsync(dvis, vis, sizeof(float2) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(dwt, wt, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(du, u, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(dv, v, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(dw, w, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    }
+    SDE_F(hipMemcpyAsync(dcfu, intfnu, sizeof(float) * ncfu, hipMemcpyHostToDevice, s));
+    SDE_F(hipMemcpyAsync(dcfv, intfnv, sizeof(float) * ncfv, hipMemcpyHostToDevice, s));
+    SDE_F(hipMemcpyAsync(dg, gvis, sizeof(float2) * (size_t)*nu * *nv, hipMemcpyHostToDevice, s));
+    SDE_F(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * 2, s));
+    launch_degrid(s, dg, du, dv, dw, dwt, dvis, n, cp, dcfu, dcfv, dvis, dwt, cnt);
+    SDE_F(hipGetLastError());
+    unsigned long long nbad = 0;
+    if (n > 0) {
+        SDE_F(hipMemcpyAsync(vis, dvis, sizeof(float2) * n, hipMemcpyDeviceToHost, s));
+        SDE_F(hipMemcpyAsync(wt, dwt, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    }
+    SDE_F(hipMemcpyAsync(&nbad, cnt, sizeof nbad, hipMemcpyDeviceToHost, s));
+    SDE_F(hipStreamSynchronize(s));
+    if (nbad > 0) {
+        char msg[96];
+        snprintf(msg, sizeof msg, "%7llu points were not successfully degridded", nbad);
+        report_warning(msg);
+    }
+}
+
+// Shared body of GRDUWT2D (mode 0) and GRDRUW2D (modes 0..3).
+static void weight_common(const char *R, float *wt, float *u, float *v, float *w, int *nvis, float *uscale,
+                          float *uoffset, float *vscale, float *voffset, int *uorigin, int *vorigin, float *newwt,
+                          float *gwt, int *nu, int *nv, double *shift, int mode, const float *robust, float ds) {
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    Ctx &c = g_ctx;
+    if (ctx_init(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
+    hipStream_t s = c.stream;
+    const int64_t n = *nvis;
+    const size_t nn = (size_t)(n > 0 ? n : 1), ncell = (size_t)*nu * *nv;
+    const CoordParams cp = make_cp(shift, *uscale, *uoffset, *vscale, *voffset, *uorigin, *vorigin, *nu, *nv, 0, 0,
+                                   1, 1);
+    const int nblk = wt_blocks(n);
+    float *dwt = ctx_buf<float>(c, "wt", nn), *du = ctx_buf<float>(c, "u", nn);
+    float *dv = ctx_buf<float>(c, "v", nn), *dw = ctx_buf<float>(c, "w", nn);
+    float *dnew = ctx_buf<float>(c, "newwt", nn);
+    double *dgwt = ctx_buf<double>(c, "gwt", ncell);
+    float *dgwtf = ctx_buf<float>(c, "gwtf", ncell);
+    double *blk = ctx_buf<double>(c, "wblk", nblk), *wsums = ctx_buf<double>(c, "wsums", 4);
+    unsigned long long *cnt = ctx_buf<unsigned long long>(c, "counters", 2);
+    if (!dwt || !du || !dv || !dw || !dnew || !dgwt || !dgwtf || !blk || !wsums || !cnt) {
+        report_error(kErrNoMem, R, "device allocation failed");
+        return;
+    }
+    if (n > 0) {
+        SDE_F(hipMemcpyAsync(dwt, wt, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(du, u, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(dv, v, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(dw, w, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(dnew, newwt, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    }
+    SDE_F(hipMemsetAsync(dgwt, 0, sizeof(double) * ncell, s));
+    SDE_F(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * 2, s));
+    launch_wt_density(s, dwt, du, dv, dw, n, cp, dgwt, blk, nblk);
+    float rob = 0.0f, t2 = 0.0f, absf2 = 0.0f, absd2 = 0.0f;
+    const float pi_f = 4.0f * atanf(1.0f);
+    if (robust) {
+        rob = mode == 3 ? (robust[0] + robust[1]) / 2.0f : robust[0];
+        const float t = 5.0f * powf(10.0f, -rob);
+        t2 = t * t;
+        absf2 = rob * rob;
+        absd2 = 2.0f * (ds * ds);
+    }
+    launch_wt_finish(s, dgwt, (int)ncell, blk, nblk, mode, t2, absf2, absd2, wsums);
+    launch_wt_apply(s, dwt, du, dv, dw, n, cp, dgwt, wsums, mode == 3, robust, pi_f, dnew, cnt);
+    launch_gwt_out(s, dgwt, dgwtf, ncell);
+    SDE_F(hipGetLastError());
+    unsigned long long ndrop = 0;
+    if (n > 0) SDE_F(hipMemcpyAsync(newwt, dnew, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    SDE_F(hipMemcpyAsync(gwt, dgwtf, sizeof(float) * ncell, hipMemcpyDeviceToHost, s));
+    SDE_F(hipMemcpyAsync(&ndrop, cnt, sizeof ndrop, hipMemcpyDeviceToHost, s));
+    SDE_F(hipStreamSynchronize(s));
+    if (ndrop > 0) {
+        char msg[96];
+        snprintf(msg, sizeof msg, "%7llu points dropped off reweighting grid!", ndrop);
+        report_warning(msg);
+        report_warning("These points are now flagged bad");
+    }
+}
+
+void grduwt2d_(float *wt, float *u, float *v, float *w, int *nvis, float *uscale, float *uoffset, float *vscale,
+               float *voffset, int *uorigin, int *vorigin, float *newwt, float *gwt, int *nu, int *nv,
+               double *shift) {
+    if (sysi_.error) return;
+    if (!check_kernel_args("GRDUWT2D", *nvis, *nu, *nv, 0, 0, 1, 1)) return;
+    weight_common("GRDUWT2D", wt, u, v, w, nvis, uscale, uoffset, vscale, voffset, uorigin, vorigin, newwt, gwt, nu,
+                  nv, shift, 0, nullptr, 0.0f);
+}
+
+void grdruw2d_(float *wt, float *u, float *v, float *w, int *nvis, float *uscale, float *uoffset, float *vscale,
+               float *voffset, int *uorigin, int *vorigin, float *newwt, float *gwt, int *nu, int *nv,
+               double *shift, const char *rmode, float *robust, float *ds, size_t rmode_len) {
+    if (sysi_.error) return;
+    if (!check_kernel_args("GRDRUW2D", *nvis, *nu, *nv, 0, 0, 1, 1)) return;
+    const std::string m = fstr(rmode, rmode_len);
+    int mode;
+    if (feq(m, "NORM")) mode = 1;
+    else if (feq(m, "AZ-NORM")) mode = 3;
+    else if (feq(m, "ABS")) mode = 2;
+    else if (feq(m, "NONE")) mode = 0;
+    else {
+        report_error(kErrLogic, "GRDRUW2D", "Unrecognized RMODE");
+        return;
+    }
+    weight_common("GRDRUW2D", wt, u, v, w, nvis, uscale, uoffset, vscale, voffset, uorigin, vorigin, newwt, gwt, nu,
+                  nv, shift, mode, robust, *ds);
+}
+
+static void pixcor_common(const char *R, float *in, int *nx, int *ny, float *cf, int *cflen, const char *mode,
+                          float *out, float *cenx, float *ceny, size_t mode_len, int cplx) {
+    if (sysi_.error) return;
+    const std::string m = fstr(mode, mode_len);
+    int correct;
+    if (fstarts(m, "APPLY")) correct = 0;
+    else if (fstarts(m, "CORRECT")) correct = 1;
+    else {
+        report_error(kErrBdArg, R, "Unknown mode");
+        return;
+    }
+    if (*nx < 1 || *ny < 1 || *cflen < 1) {
+        report_error(kErrBdArg, R, "bad sizes");
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    Ctx &c = g_ctx;
+    if (ctx_init(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
+    hipStream_t s = c.stream;
+    const size_t npix = (size_t)*nx * *ny * (cplx ? 2 : 1);
+    float *din = ctx_buf<float>(c, "pix_in", npix), *dout = ctx_buf<float>(c, "pix_out", npix);
+    float *dcf = ctx_buf<float>(c, "ccf", *cflen);
+    if (!din || !dout || !dcf) {
+        report_error(kErrNoMem, R, "device allocation failed");
+        return;
+    }
+    SDE_F(hipMemcpyAsync(din, in, sizeof(float) * npix, hipMemcpyHostToDevice, s));
+    SDE_F(hipMemcpyAsync(dcf, cf, sizeof(float) * *cflen, hipMemcpyHostToDevice, s));
+    launch_pixcor(s, din, dout, *nx, *ny, cplx, dcf, *cflen, correct, *cenx, *ceny);
+    SDE_F(hipGetLastError());
+    SDE_F(hipMemcpyAsync(out, dout, sizeof(float) * npix, hipMemcpyDeviceToHost, s));
+    SDE_F(hipStreamSynchronize(s));
+}
+
+void pixcor2d_(float *in, int *nx, int *ny, float *cf, int *cflen, const char *mode, float *out, float *cenx,
+               float *ceny, size_t mode_len) {
+    pixcor_common("PIXCOR2D", in, nx, ny, cf, cflen, mode, out, cenx, ceny, mode_len, 0);
+}
+
+void pixcorx2_(float *in, int *nx, int *ny, float *cf, int *cflen, const char *mode, float *out, float *cenx,
+               float *ceny, size_t mode_len) {
+    pixcor_common("PIXCORX2", in, nx, ny, cf, cflen, mode, out, cenx, ceny, mode_len, 1);
+}
+
+void fftr2d_(float *rarray, float *xarray, float *work, float *work2, float *rwork, int *nx, int *ny, int *fnu,
+             int *nv, int *dir) {
+    (void)work;
+    (void)work2;
+    (void)rwork;
+    if (sysi_.error) return;
+    const char *R = "FFTR2D";
+    if (*fnu < 4 || *fnu > 16384 || (*fnu & (*fnu - 1)) || !fft_supported(*nv) || *nx < 1 || *nx > *fnu ||
+        *ny < 1 || *ny > *nv || *dir == 0) {
+        report_error(kErrBdArg, R, "unsupported transform size (power-of-two FNU 4..16384, NV 2..8192)");
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    Ctx &c = g_ctx;
+    if (ctx_init(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
+    hipStream_t s = c.stream;
+    const int nu = *fnu / 2 + 1;
+    const size_t ng = (size_t)nu * *nv, npix = (size_t)*nx * *ny;
+    float2 *dg = ctx_buf<float2>(c, "fft_grid", ng);
+    float *dimg = ctx_buf<float>(c, "fft_img", npix);
+    float2 *tw_nv = nullptr, *tw_fnu = nullptr;
+    if (!dg || !dimg || get_twiddles(c.tw, *nv, s, &tw_nv) || get_twiddles(c.tw, *fnu, s, &tw_fnu)) {
+        report_error(kErrNoMem, R, "device allocation failed");
+        return;
+    }
+    if (*dir < 0) {
+        SDE_F(hipMemcpyAsync(dg, xarray, sizeof(float2) * ng, hipMemcpyHostToDevice, s));
+        launch_fft_cols(s, dg, nu, *nv, 1, 0, -1, 1.0f / (float)*nv, tw_nv);
+        launch_fft_rows_c2r(s, dg, nu, *nv, *fnu, *nx, *ny, tw_fnu, nullptr, 1, 0, 1.0f / (float)*fnu, dimg,
+                            nullptr, nullptr);
+        SDE_F(hipGetLastError());
+        SDE_F(hipMemcpyAsync(rarray, dimg, sizeof(float) * npix, hipMemcpyDeviceToHost, s));
+        SDE_F(hipMemcpyAsync(xarray, dg, sizeof(float2) * ng, hipMemcpyDeviceToHost, s));
+    } else {
+        SDE_F(hipMemcpyAsync(dimg, rarray, sizeof(float) * npix, hipMemcpyHostToDevice, s));
+        launch_fft_rows_r2c(s, dimg, *nx, *ny, *fnu, *nv, tw_fnu, nullptr, 1, 0, dg);
+        launch_fft_cols(s, dg, nu, *nv, 1, 0, +1, 1.0f, tw_nv);
+        SDE_F(hipGetLastError());
+        SDE_F(hipMemcpyAsync(xarray, dg, sizeof(float2) * ng, hipMemcpyDeviceToHost, s));
+    }
+    SDE_F(hipStreamSynchronize(s));
+}
+
+void fftx1d_(float *xarray, float *xwork, float *xwork2, int *nx, int *dir) {
+    (void)xwork;
+    (void)xwork2;
+    if (sysi_.error) return;
+    const char *R = "FFTX1D";
+    if (!fft_supported(*nx) || *dir == 0) {
+        report_error(kErrBdArg, R, "unsupported transform size (power-of-two NX 2..8192)");
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    Ctx &c = g_ctx;
+    if (ctx_init(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
+    hipStream_t s = c.stream;
+    float2 *dg = ctx_buf<float2>(c, "fft_grid", *nx);
+    float2 *tw = nullptr;
+    if (!dg || get_twiddles(c.tw, *nx, s, &tw)) {
+        report_error(kErrNoMem, R, "device allocation failed");
+        return;
+    }
+    SDE_F(hipMemcpyAsync(dg, xarray, sizeof(float2) * *nx, hipMemcpyHostToDevice, s));
+    launch_fft_cols(s, dg, 1, *nx, 1, 0, *dir, *dir < 0 ? 1.0f / (float)*nx : 1.0f, tw);
+    SDE_F(hipGetLastError());
+    SDE_F(hipMemcpyAsync(xarray, dg, sizeof(float2) * *nx, hipMemcpyDeviceToHost, s));
+    SDE_F(hipStreamSynchronize(s));
+}
+
+}  // extern "C"
+#undef SDE_F
+
+// ---------------------------------------------------------------------------
+// Device-resident plan
+// ---------------------------------------------------------------------------
+struct sde_plan {
+    sde_imaging_cfg cfg;
+    hipStream_t stream = nullptr;
+    int fnu = 0, nu = 0, nv = 0, uorigin = 1, vorigin = 1, ccflen = 0, supp = 0, osamp = 0, ncf = 0;
+    float udelt = 0, vdelt = 0, uscale = 0, vscale = 0;
+    CoordParams cp, wcp;
+    TileParams tp;
+    int waves = 1, wmode = 0;
+    std::map<int, float2 *> tw;
+    float *d_cf = nullptr, *d_ccf = nullptr;
+    int64_t nvis = 0, cap = 0;
+    float2 *d_vis = nullptr;
+    float *d_wt = nullptr, *d_u = nullptr, *d_v = nullptr, *d_w = nullptr, *d_wteff = nullptr;
+    float2 *d_mvis = nullptr;
+    float *d_mwt = nullptr;
+    GridBufs gb{};
+    size_t partial_cap = 0, items_cap = 0;
+    float2 *d_grids = nullptr, *d_mgrid = nullptr;
+    float *d_dirty = nullptr, *d_psf = nullptr, *d_img_in = nullptr;
+    int *d_peak = nullptr;
+    double *d_gwt = nullptr, *d_wblk = nullptr, *d_wsums = nullptr;
+    unsigned long long *d_cnt = nullptr;
+    // profiling
+    bool profile = false;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::string> order;
+    std::map<std::string, std::pair<double, int>> acc;
+    const char *cur = nullptr;
+    sde_run_info info{};
+};
+
+namespace {
+
+hipEvent_t pool_event(sde_plan *p) {
+    if (!p->pool.empty()) {
+        hipEvent_t e = p->pool.back();
+        p->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+void plan_mark_begin(sde_plan *p, const char *name) {
+    if (!p->profile) return;
+    hipEvent_t a = pool_event(p), b = pool_event(p);
+    (void)hipEventRecord(a, p->stream);
+    p->pending.push_back({name, {a, b}});
+}
+void plan_mark_end(sde_plan *p) {
+    if (!p->profile || p->pending.empty()) return;
+    (void)hipEventRecord(p->pending.back().second.second, p->stream);
+}
+
+void plan_collect(sde_plan *p) {
+    for (auto &q : p->pending) {
+        float ms = 0.0f;
+        (void)hipEventSynchronize(q.second.second);
+        if (hipEventElapsedTime(&ms, q.second.first, q.second.second) != hipSuccess) ms = 0.0f;
+        auto it = p->acc.find(q.first);
+        if (it == p->acc.end()) {
+            p->order.push_back(q.first);
+            p->acc[q.first] = {ms, 1};
+        } else {
+            it->second.first += ms;
+            it->second.second += 1;
+        }
+        p->pool.push_back(q.second.first);
+        p->pool.push_back(q.second.second);
+    }
+    p->pending.clear();
+}
+
+template <typename T>
+int plan_alloc(T **ptr, size_t count) {
+    if (*ptr) (void)hipFree(*ptr);
+    *ptr = nullptr;
+    SDE_CHECK(hipMalloc((void **)ptr, sizeof(T) * (count > 0 ? count : 1)));
+    return 0;
+}
+
+int plan_reserve_vis(sde_plan *p, int64_t n) {
+    if (n <= p->cap) return 0;
+    const size_t c = (size_t)n;
+    if (plan_alloc(&p->d_vis, c) || plan_alloc(&p->d_wt, c) || plan_alloc(&p->d_u, c) || plan_alloc(&p->d_v, c) ||
+        plan_alloc(&p->d_w, c) || plan_alloc(&p->d_wteff, c) || plan_alloc(&p->d_mvis, c) ||
+        plan_alloc(&p->d_mwt, c) || plan_alloc(&p->gb.tile_of, c) || plan_alloc(&p->gb.rec, c) ||
+        plan_alloc(&p->gb.sorted, c))
+        return -1;
+    p->cap = n;
+    return 0;
+}
+
+int plan_reserve_items(sde_plan *p) {
+    const int mi = max_items_for(p->tp, p->nvis);
+    if ((size_t)mi > p->items_cap) {
+        if (plan_alloc(&p->gb.items, (size_t)mi)) return -1;
+        p->items_cap = mi;
+    }
+    const size_t pf = grid_partial_floats(p->tp, 3, mi);
+    if (pf > p->partial_cap) {
+        if (plan_alloc(&p->gb.partial, pf)) return -1;
+        p->partial_cap = pf;
+    }
+    p->gb.max_items = mi;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **out) {
+    *out = nullptr;
+    if (!cfg || cfg->nx < 2 || cfg->ny < 2 || cfg->nx > 8192 || cfg->ny > 8192 || !(cfg->cellsize > 0.0f)) {
+        set_plan_error("sde_plan_create: bad image size or cellsize");
+        return -1;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_ctx_mu);
+        if (ctx_init(g_ctx)) return -1;
+    }
+    sde_plan *p = new sde_plan();
+    p->cfg = *cfg;
+    std::string cft = fstr(cfg->cftype, strnlen(cfg->cftype, sizeof cfg->cftype));
+    float cf[1 + 2 * 4 * 128];
+    if (fstarts(cft, "BOX") || make_cf(cft, cf, &p->supp, &p->osamp)) {
+        set_plan_error("sde_plan_create: convolution function must be SF or SZE");
+        delete p;
+        return -1;
+    }
+    p->ncf = 1 + 2 * (p->supp + 1) * p->osamp;
+    const float dx = -cfg->cellsize / 3600.0f, dy = cfg->cellsize / 3600.0f;
+    p->fnu = pow2_at_least(cfg->nx);
+    p->nu = p->fnu / 2 + 1;
+    p->nv = pow2_at_least(cfg->ny);
+    p->uorigin = 1;
+    p->vorigin = (p->nv + 1) / 2;
+    p->udelt = conj_delt(cfg->nx, dx);
+    p->vdelt = conj_delt(cfg->ny, dy);
+    p->uscale = 1.0f / p->udelt;
+    p->vscale = 1.0f / p->vdelt;
+    p->ccflen = 1 + pow2_at_least(cfg->nx > cfg->ny ? cfg->nx : cfg->ny) / 2;
+    p->cp = make_cp(cfg->shift, p->uscale, 0.0f, p->vscale, 0.0f, p->uorigin, p->vorigin, p->nu, p->nv, p->supp,
+                    p->supp, p->osamp, p->osamp);
+    const float wus = cfg->fov / p->udelt, wvs = cfg->fov / p->vdelt;
+    p->wcp = make_cp(cfg->shift, wus, 0.0f, wvs, 0.0f, p->uorigin, p->vorigin, p->nu, p->nv, 0, 0, 1, 1);
+    std::string rm = fstr(cfg->rmode, strnlen(cfg->rmode, sizeof cfg->rmode));
+    if (feq(rm, "NONE") || rm[0] == ' ') p->wmode = 0;
+    else if (feq(rm, "NORM")) p->wmode = 1;
+    else if (feq(rm, "ABS")) p->wmode = 2;
+    else if (feq(rm, "AZ-NORM")) p->wmode = 3;
+    else {
+        set_plan_error("sde_plan_create: unrecognized RMODE");
+        delete p;
+        return -1;
+    }
+    const int T = cfg->tile ? cfg->tile : default_tile(p->nu, p->nv);
+    p->waves = cfg->waves ? cfg->waves : default_waves(T);
+    if (!((T == 16 && (p->waves == 1 || p->waves == 2 || p->waves == 4)) || (T == 32 && p->waves == 1))) {
+        set_plan_error("sde_plan_create: unsupported tile %d / waves %d", T, p->waves);
+        delete p;
+        return -1;
+    }
+    p->tp = make_tp(p->nu, p->nv, T, cfg->chunk > 0 ? cfg->chunk : 1024);
+    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_plan_error("sde_plan_create: stream creation failed");
+        delete p;
+        return -1;
+    }
+    std::vector<float> ccf(p->ccflen);
+    make_ccf(cft, ccf.data(), p->ccflen);
+    const size_t ng = (size_t)p->nu * p->nv, npix = (size_t)cfg->nx * cfg->ny;
+    p->gb.nblk = 0;
+    int rc = plan_alloc(&p->d_cf, p->ncf) || plan_alloc(&p->d_ccf, p->ccflen) || plan_alloc(&p->d_grids, 2 * ng) ||
+             plan_alloc(&p->d_mgrid, ng) || plan_alloc(&p->d_dirty, npix) || plan_alloc(&p->d_psf, npix) ||
+             plan_alloc(&p->d_img_in, npix) || plan_alloc(&p->d_peak, 4) || plan_alloc(&p->d_gwt, ng) ||
+             plan_alloc(&p->d_wblk, 4096) || plan_alloc(&p->d_wsums, 4) || plan_alloc(&p->d_cnt, 4) ||
+             plan_alloc(&p->gb.tile_count, p->tp.ntiles + 1) || plan_alloc(&p->gb.tile_start, p->tp.ntiles + 1) ||
+             plan_alloc(&p->gb.item_start, p->tp.ntiles + 1) || plan_alloc(&p->gb.tile_cursor, p->tp.ntiles + 1) ||
+             plan_alloc(&p->gb.nitems, 1) || plan_alloc(&p->gb.blk, 2 * 4096) || plan_alloc(&p->gb.sums, 4);
+    if (rc) {
+        sde_plan_destroy(p);
+        return -1;
+    }
+    float2 *t1, *t2;
+    if (get_twiddles(p->tw, p->nv, p->stream, &t1) || get_twiddles(p->tw, p->fnu, p->stream, &t2)) {
+        sde_plan_destroy(p);
+        return -1;
+    }
+    if (hipMemcpyAsync(p->d_cf, cf, sizeof(float) * p->ncf, hipMemcpyHostToDevice, p->stream) != hipSuccess ||
+        hipMemcpyAsync(p->d_ccf, ccf.data(), sizeof(float) * p->ccflen, hipMemcpyHostToDevice, p->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(p->stream) != hipSuccess) {
+        set_plan_error("sde_plan_create: table upload failed");
+        sde_plan_destroy(p);
+        return -1;
+    }
+    *out = p;
+    return 0;
+}
+
+int sde_plan_destroy(sde_plan *p) {
+    if (!p) return 0;
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    plan_collect(p);
+    for (auto e : p->pool) (void)hipEventDestroy(e);
+    void *bufs[] = {p->d_cf, p->d_ccf, p->d_vis, p->d_wt, p->d_u, p->d_v, p->d_w, p->d_wteff, p->d_mvis, p->d_mwt,
+                    p->gb.tile_of, p->gb.rec, p->gb.sorted, p->gb.items, p->gb.partial, p->gb.tile_count,
+                    p->gb.tile_start, p->gb.item_start, p->gb.tile_cursor, p->gb.nitems, p->gb.blk, p->gb.sums,
+                    p->d_grids, p->d_mgrid, p->d_dirty, p->d_psf, p->d_img_in, p->d_peak, p->d_gwt, p->d_wblk,
+                    p->d_wsums, p->d_cnt};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    for (auto &kv : p->tw) (void)hipFree(kv.second);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+    return 0;
+}
+
+int sde_plan_set_vis(sde_plan *p, int64_t nvis, const float *vis, const float *wt, const float *u, const float *v,
+                     const float *w, int on_device) {
+    if (!p || nvis < 0) return -1;
+    if (plan_reserve_vis(p, nvis)) return -1;
+    p->nvis = nvis;
+    const hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (nvis > 0) {
+        SDE_CHECK(hipMemcpyAsync(p->d_vis, vis, sizeof(float2) * nvis, k, p->stream));
+        SDE_CHECK(hipMemcpyAsync(p->d_wt, wt, sizeof(float) * nvis, k, p->stream));
+        SDE_CHECK(hipMemcpyAsync(p->d_u, u, sizeof(float) * nvis, k, p->stream));
+        SDE_CHECK(hipMemcpyAsync(p->d_v, v, sizeof(float) * nvis, k, p->stream));
+        SDE_CHECK(hipMemcpyAsync(p->d_w, w, sizeof(float) * nvis, k, p->stream));
+    }
+    if (p->cfg.chunk <= 0) p->tp.chunk = default_chunk(nvis);
+    if (plan_reserve_items(p)) return -1;
+    SDE_CHECK(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+int sde_plan_uvmap(sde_plan *p) {
+    if (!p) return -1;
+    if (p->nvis <= 0) {
+        set_plan_error("sde_plan_uvmap: no visibilities");
+        return -1;
+    }
+    hipStream_t s = p->stream;
+    const int64_t n = p->nvis;
+    const float *wt_use = p->d_wt;
+    SDE_CHECK(hipMemsetAsync(p->d_cnt, 0, sizeof(unsigned long long) * 4, s));
+    if (p->cfg.weighting) {
+        const int nblk = wt_blocks(n);
+        const size_t ng = (size_t)p->nu * p->nv;
+        SDE_CHECK(hipMemsetAsync(p->d_gwt, 0, sizeof(double) * ng, s));
+        plan_mark_begin(p, "wt_density");
+        launch_wt_density(s, p->d_wt, p->d_u, p->d_v, p->d_w, n, p->wcp, p->d_gwt, p->d_wblk, nblk);
+        plan_mark_end(p);
+        float rob = p->wmode == 3 ? (p->cfg.robust[0] + p->cfg.robust[1]) / 2.0f : p->cfg.robust[0];
+        const float t = 5.0f * powf(10.0f, -rob);
+        plan_mark_begin(p, "wt_finish");
+        launch_wt_finish(s, p->d_gwt, (int)ng, p->d_wblk, nblk, p->wmode, t * t, rob * rob,
+                         2.0f * (p->cfg.ds * p->cfg.ds), p->d_wsums);
+        plan_mark_end(p);
+        SDE_CHECK(hipMemcpyAsync(p->d_wteff, p->d_wt, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+        plan_mark_begin(p, "wt_apply");
+        launch_wt_apply(s, p->d_wt, p->d_u, p->d_v, p->d_w, n, p->wcp, p->d_gwt, p->d_wsums, p->wmode == 3,
+                        p->cfg.robust, 4.0f * atanf(1.0f), p->d_wteff, p->d_cnt);
+        plan_mark_end(p);
+        wt_use = p->d_wteff;
+    }
+    p->gb.nblk = grid_prep_blocks(n);
+    const size_t ng = (size_t)p->nu * p->nv;
+    float2 *gd = p->d_grids, *gp = p->d_grids + ng;
+    auto mb = [p](const char *nm) { plan_mark_begin(p, nm); };
+    auto me = [p]() { plan_mark_end(p); };
+    if (run_gridding(s, p->d_vis, wt_use, p->d_u, p->d_v, p->d_w, n, p->cp, p->tp, p->waves, 3, p->d_cf, p->d_cf,
+                     p->gb, gd, gp, mb, me))
+        return -1;
+    float2 *twnv = p->tw[p->nv], *twfnu = p->tw[p->fnu];
+    plan_mark_begin(p, "fft_cols");
+    launch_fft_cols(s, p->d_grids, p->nu, p->nv, 2, ng, -1, 1.0f, twnv);
+    plan_mark_end(p);
+    SDE_CHECK(hipMemsetAsync(p->d_peak, 0x80, sizeof(int), s));
+    const float scale = 1.0f / ((float)p->fnu * (float)p->nv);
+    plan_mark_begin(p, "fft_rows_psf");
+    launch_fft_rows_c2r(s, gp, p->nu, p->nv, p->fnu, p->cfg.nx, p->cfg.ny, twfnu, p->d_ccf, p->ccflen, 1, scale,
+                        p->d_psf, p->d_peak, nullptr);
+    plan_mark_end(p);
+    plan_mark_begin(p, "fft_rows_dirty");
+    launch_fft_rows_c2r(s, gd, p->nu, p->nv, p->fnu, p->cfg.nx, p->cfg.ny, twfnu, p->d_ccf, p->ccflen, 1, scale,
+                        p->d_dirty, nullptr, p->d_peak);
+    plan_mark_end(p);
+    plan_mark_begin(p, "psf_scale");
+    launch_scale_by_peak(s, p->d_psf, (size_t)p->cfg.nx * p->cfg.ny, p->d_peak);
+    plan_mark_end(p);
+    SDE_CHECK(hipGetLastError());
+    return 0;
+}
+
+int sde_plan_vismodel(sde_plan *p, const float *image) {
+    if (!p) return -1;
+    hipStream_t s = p->stream;
+    const size_t npix = (size_t)p->cfg.nx * p->cfg.ny;
+    const float *img = p->d_dirty;
+    if (image) {
+        SDE_CHECK(hipMemcpyAsync(p->d_img_in, image, sizeof(float) * npix, hipMemcpyHostToDevice, s));
+        img = p->d_img_in;
+    }
+    float2 *twnv = p->tw[p->nv], *twfnu = p->tw[p->fnu];
+    plan_mark_begin(p, "fft_rows_model");
+    launch_fft_rows_r2c(s, img, p->cfg.nx, p->cfg.ny, p->fnu, p->nv, twfnu, p->d_ccf, p->ccflen, 1, p->d_mgrid);
+    plan_mark_end(p);
+    plan_mark_begin(p, "fft_cols_model");
+    launch_fft_cols(s, p->d_mgrid, p->nu, p->nv, 1, 0, +1, 1.0f, twnv);
+    plan_mark_end(p);
+    SDE_CHECK(hipMemsetAsync(p->d_cnt + 1, 0, sizeof(unsigned long long), s));
+    plan_mark_begin(p, "degrid");
+    launch_degrid(s, p->d_mgrid, p->d_u, p->d_v, p->d_w, p->d_wt, p->d_vis, p->nvis, p->cp, p->d_cf, p->d_cf,
+                  p->d_mvis, p->d_mwt, p->d_cnt + 1);
+    plan_mark_end(p);
+    SDE_CHECK(hipGetLastError());
+    return 0;
+}
+
+int sde_plan_sync(sde_plan *p) {
+    if (!p) return -1;
+    SDE_CHECK(hipStreamSynchronize(p->stream));
+    plan_collect(p);
+    return 0;
+}
+
+int sde_plan_get_images(sde_plan *p, float *dirty, float *psf) {
+    if (!p) return -1;
+    const size_t npix = (size_t)p->cfg.nx * p->cfg.ny;
+    if (dirty) SDE_CHECK(hipMemcpyAsync(dirty, p->d_dirty, sizeof(float) * npix, hipMemcpyDeviceToHost, p->stream));
+    if (psf) SDE_CHECK(hipMemcpyAsync(psf, p->d_psf, sizeof(float) * npix, hipMemcpyDeviceToHost, p->stream));
+    SDE_CHECK(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+int sde_plan_get_grids(sde_plan *p, float *dirty_grid, float *psf_grid) {
+    if (!p) return -1;
+    const size_t ng = (size_t)p->nu * p->nv;
+    if (dirty_grid)
+        SDE_CHECK(hipMemcpyAsync(dirty_grid, p->d_grids, sizeof(float2) * ng, hipMemcpyDeviceToHost, p->stream));
+    if (psf_grid)
+        SDE_CHECK(hipMemcpyAsync(psf_grid, p->d_grids + ng, sizeof(float2) * ng, hipMemcpyDeviceToHost, p->stream));
+    SDE_CHECK(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+int sde_plan_get_weights(sde_plan *p, float *wt) {
+    if (!p) return -1;
+    const float *src = p->cfg.weighting ? p->d_wteff : p->d_wt;
+    if (p->nvis > 0) SDE_CHECK(hipMemcpyAsync(wt, src, sizeof(float) * p->nvis, hipMemcpyDeviceToHost, p->stream));
+    SDE_CHECK(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+int sde_plan_get_model(sde_plan *p, float *vis, float *wt) {
+    if (!p) return -1;
+    if (p->nvis > 0) {
+        if (vis) SDE_CHECK(hipMemcpyAsync(vis, p->d_mvis, sizeof(float2) * p->nvis, hipMemcpyDeviceToHost, p->stream));
+        if (wt) SDE_CHECK(hipMemcpyAsync(wt, p->d_mwt, sizeof(float) * p->nvis, hipMemcpyDeviceToHost, p->stream));
+    }
+    SDE_CHECK(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+int sde_plan_info(sde_plan *p, sde_run_info *info) {
+    if (!p || !info) return -1;
+    double sums[3] = {0, 0, 0};
+    int key = 0, nitems = 0;
+    unsigned long long cnt[2] = {0, 0};
+    SDE_CHECK(hipMemcpyAsync(sums, p->gb.sums, sizeof sums, hipMemcpyDeviceToHost, p->stream));
+    SDE_CHECK(hipMemcpyAsync(&key, p->d_peak, sizeof key, hipMemcpyDeviceToHost, p->stream));
+    SDE_CHECK(hipMemcpyAsync(&nitems, p->gb.nitems, sizeof nitems, hipMemcpyDeviceToHost, p->stream));
+    SDE_CHECK(hipMemcpyAsync(cnt, p->d_cnt, sizeof cnt, hipMemcpyDeviceToHost, p->stream));
+    SDE_CHECK(hipStreamSynchronize(p->stream));
+    int bits = key >= 0 ? key : key ^ 0x7fffffff;
+    float peak;
+    memcpy(&peak, &bits, sizeof peak);
+    info->nvis = p->nvis;
+    info->sumwt = sums[0];
+    info->lsumwt = sums[1];
+    info->maxpsf = peak;
+    info->nitems = nitems;
+    info->ndropped = (int64_t)cnt[0];
+    info->nbad = (int64_t)cnt[1];
+    info->ngridded = 0;
+    return 0;
+}
+
+int sde_plan_geometry(sde_plan *p, int *nu, int *nv, int *fnu, int *uorigin, int *vorigin, float *uscale,
+                      float *vscale) {
+    if (!p) return -1;
+    *nu = p->nu;
+    *nv = p->nv;
+    *fnu = p->fnu;
+    *uorigin = p->uorigin;
+    *vorigin = p->vorigin;
+    *uscale = p->uscale;
+    *vscale = p->vscale;
+    return 0;
+}
+
+int sde_plan_device_vis(sde_plan *p, float **vis, float **wt, float **u, float **v, float **w) {
+    if (!p) return -1;
+    *vis = reinterpret_cast<float *>(p->d_vis);
+    *wt = p->d_wt;
+    *u = p->d_u;
+    *v = p->d_v;
+    *w = p->d_w;
+    return 0;
+}
+
+int sde_plan_profile(sde_plan *p, int enable) {
+    if (!p) return -1;
+    p->profile = enable != 0;
+    return 0;
+}
+
+int sde_plan_timings(sde_plan *p, int max, char (*names)[32], double *ms, int *count) {
+    if (!p) return -1;
+    SDE_CHECK(hipStreamSynchronize(p->stream));
+    plan_collect(p);
+    int k = 0;
+    for (auto &nm : p->order) {
+        if (k >= max) break;
+        snprintf(names[k], 32, "%s", nm.c_str());
+        ms[k] = p->acc[nm].first;
+        count[k] = p->acc[nm].second;
+        k++;
+    }
+    return k;
+}
+
+int sde_plan_reset_timings(sde_plan *p) {
+    if (!p) return -1;
+    SDE_CHECK(hipStreamSynchronize(p->stream));
+    plan_collect(p);
+    p->acc.clear();
+    p->order.clear();
+    return 0;
+}
+
+int sde_plan_synth(sde_plan *p, int nant, const double *enu, double lat, double dec, double ha0, double dha,
+                   int ntime, int nchan, const double *freq, int nsrc, const double *flux, const double *l,
+                   const double *m, double sigma, uint64_t seed, int64_t nkeep) {
+    if (!p || nant < 2 || ntime < 1 || nchan < 1 || nsrc < 0) return -1;
+    const int nbl = nant * (nant - 1) / 2;
+    std::vector<double> bxyz(3 * (size_t)nbl), src(3 * (size_t)(nsrc > 0 ? nsrc : 1));
+    // antenna ENU -> equatorial XYZ (X towards HA=0 on the equator, Z to the pole)
+    std::vector<double> xyz(3 * (size_t)nant);
+    for (int a = 0; a < nant; a++) {
+        const double e = enu[3 * a], nn = enu[3 * a + 1], up = enu[3 * a + 2];
+        xyz[3 * a] = -sin(lat) * nn + cos(lat) * up;
+        xyz[3 * a + 1] = e;
+        xyz[3 * a + 2] = cos(lat) * nn + sin(lat) * up;
+    }
+    int b = 0;
+    for (int a1 = 0; a1 < nant; a1++)
+        for (int a2 = a1 + 1; a2 < nant; a2++, b++)
+            for (int k = 0; k < 3; k++) bxyz[3 * b + k] = xyz[3 * a2 + k] - xyz[3 * a1 + k];
+    for (int s = 0; s < nsrc; s++) {
+        src[3 * s] = flux[s];
+        src[3 * s + 1] = l[s];
+        src[3 * s + 2] = m[s];
+    }
+    const int64_t nfull = (int64_t)nbl * ntime * nchan;
+    const int64_t n = (nkeep > 0 && nkeep < nfull) ? nkeep : nfull;
+    if (plan_reserve_vis(p, n)) return -1;
+    p->nvis = n;
+    double *d_b = nullptr, *d_f = nullptr, *d_s = nullptr;
+    SDE_CHECK(hipMalloc(&d_b, sizeof(double) * bxyz.size()));
+    SDE_CHECK(hipMalloc(&d_f, sizeof(double) * nchan));
+    SDE_CHECK(hipMalloc(&d_s, sizeof(double) * src.size()));
+    SDE_CHECK(hipMemcpyAsync(d_b, bxyz.data(), sizeof(double) * bxyz.size(), hipMemcpyHostToDevice, p->stream));
+    SDE_CHECK(hipMemcpyAsync(d_f, freq, sizeof(double) * nchan, hipMemcpyHostToDevice, p->stream));
+    SDE_CHECK(hipMemcpyAsync(d_s, src.data(), sizeof(double) * src.size(), hipMemcpyHostToDevice, p->stream));
+    launch_synth(p->stream, d_b, nbl, dec, ha0, dha, ntime, nchan, d_f, nsrc, d_s, sigma, seed, nfull, n, p->d_vis,
+                 p->d_wt, p->d_u, p->d_v, p->d_w);
+    SDE_CHECK(hipGetLastError());
+    SDE_CHECK(hipStreamSynchronize(p->stream));
+    (void)hipFree(d_b);
+    (void)hipFree(d_f);
+    (void)hipFree(d_s);
+    if (p->cfg.chunk <= 0) p->tp.chunk = default_chunk(n);
+    return plan_reserve_items(p);
+}
+
+}  // extern "C"

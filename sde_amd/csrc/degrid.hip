@@ -1,0 +1,356 @@
+// Degridding (GRDDEH2D, grd/grddeh2d.f:84-203), density-grid weighting
+// (GRDUWT2D grd/grduwt2d.f:77-167, GRDRUW2D grd/grdruw2d.f:87-249) and the
+// synthetic-track generator used for the large workloads.
+#include "internal.h"
+
+namespace {
+
+__device__ __forceinline__ int nint_f(float x) { return (int)roundf(x); }
+
+// One thread per visibility.  The FP64 coordinate/phase step and the FP32
+// gather follow the reference operation by operation (same loop order:
+// OFFU outer near the u = 0 axis, OFFV outer elsewhere), so a given grid
+// degrids to the reference's single-precision values.  A tap that would
+// read the row below the grid (reflected v = -NV/2, which the reference
+// reads out of bounds) contributes nothing.
+__global__ __launch_bounds__(256) void k_degrid(
+    const float2 *__restrict__ grid, const float *__restrict__ u, const float *__restrict__ v,
+    const float *__restrict__ w, const float *__restrict__ wt_in, const float2 *vis_in,
+    int64_t nvis, CoordParams cp, const float *__restrict__ cfu, const float *__restrict__ cfv,
+    float2 *vis_out, float *__restrict__ wt_out, unsigned long long *__restrict__ nbad) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nvis) return;
+    float wti = wt_in[i];
+    if (wti <= 0.0f) {
+        wt_out[i] = wti;
+        if (vis_in != vis_out) vis_out[i] = vis_in[i];
+        return;
+    }
+    const int cfoffu = (cp.suppu + 1) * cp.osampu + 1;
+    const int cfoffv = (cp.suppv + 1) * cp.osampv + 1;
+    float vr = 0.0f, vi = 0.0f, sumwt = 0.0f;
+    wti = -wti;
+    const double ug = u[i], vg = v[i], wg = w[i];
+    const double ul = cp.shift[0] * ug + cp.shift[1] * vg + cp.shift[2] * wg;
+    const double vl = cp.shift[3] * ug + cp.shift[4] * vg + cp.shift[5] * wg;
+    const double wl = cp.shift[6] * ug + cp.shift[7] * vg + cp.shift[8] * wg;
+    const double phase = cp.twopi * (wl - wg);
+    const float rr = (float)cos(phase), ri = (float)sin(phase);
+    float ucell = (float)((double)cp.uscale * ul + (double)cp.uoffset);
+    float vcell = (float)((double)cp.vscale * vl + (double)cp.voffset);
+    if (ucell > 0.0f) {
+        ucell = ucell + (float)cp.uorigin;
+        vcell = vcell + (float)cp.vorigin;
+    } else {
+        ucell = -ucell + (float)cp.uorigin;
+        vcell = -vcell + (float)cp.vorigin;
+    }
+    const int nuc = nint_f(ucell), nvc = nint_f(vcell);
+    if (nuc + cp.suppu > cp.nu || nvc + cp.suppv > cp.nv || nvc - cp.suppv < 1) {
+        vis_out[i] = make_float2(0.0f, 0.0f);
+        wt_out[i] = wti;
+        return;
+    }
+    const int delui = nint_f((float)cp.osampu * ((float)nuc - ucell));
+    const int delvi = nint_f((float)cp.osampv * ((float)nvc - vcell));
+    const int ucen = nuc - cp.uorigin, vcen = nvc - cp.vorigin;
+    if (nuc <= cp.suppu) {
+        for (int offu = -cp.suppu; offu <= cp.suppu; offu++) {
+            const float cu = cfu[delui + cp.osampu * offu + cfoffu - 1];
+            if (ucen != -offu) {
+                const int usign = (ucen + offu) >= 0 ? 1 : -1;
+                const int uint_ = cp.uorigin + usign * (ucen + offu);
+                for (int offv = -cp.suppv; offv <= cp.suppv; offv++) {
+                    const int vint = cp.vorigin + usign * (vcen + offv);
+                    const float uvwt = cu * cfv[delvi + cp.osampv * offv + cfoffv - 1];
+                    float2 g = make_float2(0.0f, 0.0f);
+                    if (vint >= 1 && vint <= cp.nv && uint_ >= 1 && uint_ <= cp.nu)
+                        g = grid[(size_t)(vint - 1) * cp.nu + uint_ - 1];
+                    if (usign < 0) g.y = -g.y;
+                    vr = vr + g.x * uvwt;
+                    vi = vi + g.y * uvwt;
+                    sumwt = sumwt + uvwt;
+                }
+            } else {
+                for (int offv = -cp.suppv; offv <= cp.suppv; offv++) {
+                    const int vint = cp.vorigin + (vcen + offv);
+                    const float uvwt = cu * cfv[delvi + cp.osampv * offv + cfoffv - 1];
+                    const float2 g = grid[(size_t)(vint - 1) * cp.nu + cp.uorigin - 1];
+                    vr = vr + g.x * uvwt;
+                    vi = vi + g.y * uvwt;
+                    sumwt = sumwt + uvwt;
+                }
+            }
+        }
+    } else {
+        for (int offv = -cp.suppv; offv <= cp.suppv; offv++) {
+            const float cv = cfv[delvi + cp.osampv * offv + cfoffv - 1];
+            const float2 *grow = grid + (size_t)(cp.vorigin + vcen + offv - 1) * cp.nu + (cp.uorigin + ucen - 1);
+            for (int offu = -cp.suppu; offu <= cp.suppu; offu++) {
+                const float uvwt = cfu[delui + cp.osampu * offu + cfoffu - 1] * cv;
+                const float2 g = grow[offu];
+                vr = vr + g.x * uvwt;
+                vi = vi + g.y * uvwt;
+                sumwt = sumwt + uvwt;
+            }
+        }
+    }
+    if (sumwt > 0.0f) {
+        vr = vr / sumwt;
+        vi = vi / sumwt;
+        wti = fabsf(wti);
+    } else {
+        atomicAdd(nbad, 1ull);
+        vr = vi = 0.0f;
+    }
+    if ((double)cp.uscale * ul > 0.0) vi = -vi;
+    vis_out[i] = make_float2(vr * rr - vi * ri, vr * ri + vi * rr);
+    wt_out[i] = wti;
+}
+
+// Density-grid cell in REAL arithmetic (grd/grduwt2d.f:89-110).
+__device__ __forceinline__ void wcell(const CoordParams &cp, float u, float v, float w, int &uc, int &vc,
+                                      float &ul, float &vl) {
+    if (cp.doshift) {
+        ul = (float)(cp.shift[0] * (double)u + cp.shift[1] * (double)v + cp.shift[2] * (double)w);
+        vl = (float)(cp.shift[3] * (double)u + cp.shift[4] * (double)v + cp.shift[5] * (double)w);
+    } else {
+        ul = u;
+        vl = v;
+    }
+    float ucell = cp.uscale * ul + cp.uoffset;
+    float vcell = cp.vscale * vl + cp.voffset;
+    if (ucell > 0.0f) {
+        ucell = ucell + (float)cp.uorigin;
+        vcell = vcell + (float)cp.vorigin;
+    } else {
+        ucell = -ucell + (float)cp.uorigin;
+        vcell = -vcell + (float)cp.vorigin;
+    }
+    uc = nint_f(ucell);
+    vc = nint_f(vcell);
+}
+
+constexpr int kWtThreads = 256;
+constexpr int kMaxWtBlocks = 4096;
+
+__global__ __launch_bounds__(kWtThreads) void k_wt_density(
+    const float *__restrict__ wt, const float *__restrict__ u, const float *__restrict__ v,
+    const float *__restrict__ w, int64_t nvis, CoordParams cp, double *__restrict__ gwt,
+    double *__restrict__ blk) {
+    __shared__ double red[kWtThreads];
+    double sw = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kWtThreads + threadIdx.x; i < nvis;
+         i += (int64_t)gridDim.x * kWtThreads) {
+        const float wti = wt[i];
+        if (wti <= 0.0f) continue;
+        int uc, vc;
+        float ul, vl;
+        wcell(cp, u[i], v[i], w[i], uc, vc, ul, vl);
+        if (uc < 1 || uc > cp.nu || vc < 1 || vc > cp.nv) continue;
+        atomicAdd(&gwt[(size_t)(vc - 1) * cp.nu + uc - 1], (double)wti);
+        sw += (double)wti;
+    }
+    red[threadIdx.x] = sw;
+    __syncthreads();
+    for (int s = kWtThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) blk[blockIdx.x] = red[0];
+}
+
+// wsums: [0] sum of weights, [1] sum of squared (REAL) cell weights,
+//        [2] F2, [3] D2 (both REAL values held in double).
+__global__ __launch_bounds__(1024) void k_wt_finish(const double *__restrict__ gwt, int ncell,
+                                                    const double *__restrict__ blk, int nblk, int mode,
+                                                    float rob_t2, float absf2, float absd2,
+                                                    double *__restrict__ wsums) {
+    __shared__ double ra[1024], rb[1024];
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < nblk; k += 1024) a += blk[k];
+    if (mode == 1 || mode == 3)
+        for (int k = threadIdx.x; k < ncell; k += 1024) {
+            const float g = (float)gwt[k];
+            if (g > 0.0f) b += (double)(g * g);
+        }
+    ra[threadIdx.x] = a;
+    rb[threadIdx.x] = b;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            ra[threadIdx.x] += ra[threadIdx.x + s];
+            rb[threadIdx.x] += rb[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        float f2 = 1.0f, d2 = 0.0f;
+        if (mode == 1 || mode == 3) {  // NORM, AZ-NORM
+            f2 = (float)((double)rob_t2 / (rb[0] / ra[0]));
+            d2 = 1.0f;
+        } else if (mode == 2) {  // ABS
+            f2 = absf2;
+            d2 = absd2;
+        }
+        wsums[0] = ra[0];
+        wsums[1] = rb[0];
+        wsums[2] = (double)f2;
+        wsums[3] = (double)d2;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_wt_apply(
+    const float *__restrict__ wt, const float *__restrict__ u, const float *__restrict__ v,
+    const float *__restrict__ w, int64_t nvis, CoordParams cp, const double *__restrict__ gwt,
+    const double *__restrict__ wsums, int az, float rob1, float rpa, float rbscale, float pi_f,
+    float *__restrict__ newwt, unsigned long long *__restrict__ ndrop) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nvis) return;
+    const float wti = wt[i];
+    if (wti <= 0.0f) return;
+    int uc, vc;
+    float ul, vl;
+    wcell(cp, u[i], v[i], w[i], uc, vc, ul, vl);
+    if (uc < 1 || uc > cp.nu || vc < 1 || vc > cp.nv) {
+        newwt[i] = -1.0f;
+        atomicAdd(ndrop, 1ull);
+        return;
+    }
+    float f2 = (float)wsums[2];
+    const float d2 = (float)wsums[3];
+    if (az) {
+        float t = atan2f(vl, ul);
+        t = fmodf(t - rpa + 2.5f * pi_f, pi_f) - pi_f / 2.0f;
+        const float rb = rob1 - fabsf(t) * rbscale;
+        const float q = 5.0f * powf(10.0f, -rb);
+        f2 = (float)((double)(q * q) / (wsums[1] / wsums[0]));
+    }
+    const float g = (float)gwt[(size_t)(vc - 1) * cp.nu + uc - 1];
+    if (g > 0.0f) {
+        if (uc == cp.uorigin && vc != cp.vorigin) newwt[i] = 0.5f * wti / (g * f2 + d2);
+        else newwt[i] = wti / (g * f2 + d2);
+    } else {
+        newwt[i] = -1.0f;
+    }
+}
+
+__global__ void k_gwt_out(const double *__restrict__ gwt, float *__restrict__ out, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)gwt[i];
+}
+
+// ---- synthetic observation ----
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(256) void k_synth(
+    const double *__restrict__ bxyz, int nbl, double dec, double ha0, double dha, int ntime,
+    int nchan, const double *__restrict__ freq, int nsrc, const double *__restrict__ src,
+    double sigma, uint64_t seed, int64_t nfull, int64_t nkeep, float2 *__restrict__ vis,
+    float *__restrict__ wt, float *__restrict__ u, float *__restrict__ v, float *__restrict__ w) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nkeep) return;
+    const int64_t g = nkeep < nfull ? (int64_t)((double)i * (double)nfull / (double)nkeep) : i;
+    const int64_t per_ch = (int64_t)ntime * nbl;
+    const int ch = (int)(g / per_ch);
+    const int64_t rem = g - (int64_t)ch * per_ch;
+    const int t = (int)(rem / nbl), b = (int)(rem - (int64_t)t * nbl);
+    const double h = ha0 + dha * t;
+    const double sh = sin(h), chh = cos(h), sd = sin(dec), cd = cos(dec);
+    const double lx = bxyz[3 * b], ly = bxyz[3 * b + 1], lz = bxyz[3 * b + 2];
+    const double inv_lambda = freq[ch] / 299792458.0;
+    const double uu = (sh * lx + chh * ly) * inv_lambda;
+    const double vv = (-sd * chh * lx + sd * sh * ly + cd * lz) * inv_lambda;
+    const double ww = (cd * chh * lx - cd * sh * ly + sd * lz) * inv_lambda;
+    const float uf = (float)uu, vf = (float)vv;
+    double re = 0.0, im = 0.0;
+    for (int s = 0; s < nsrc; s++) {
+        const double ph = 2.0 * M_PI * ((double)uf * src[3 * s + 1] + (double)vf * src[3 * s + 2]);
+        double sp, cp;
+        sincos(ph, &sp, &cp);
+        re += src[3 * s] * cp;
+        im += src[3 * s] * sp;
+    }
+    if (sigma > 0.0) {
+        const uint64_t r1 = mix64(seed ^ (uint64_t)g * 0xd1b54a32d192ed03ull);
+        const uint64_t r2 = mix64(r1 + 0x632be59bd9b4e019ull);
+        const double a = ((double)(r1 >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        const double c = ((double)(r2 >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        const double rad = sigma * sqrt(-2.0 * log(a));
+        re += rad * cos(2.0 * M_PI * c);
+        im += rad * sin(2.0 * M_PI * c);
+    }
+    vis[i] = make_float2((float)re, (float)im);
+    wt[i] = 1.0f;
+    u[i] = uf;
+    v[i] = vf;
+    w[i] = (float)ww;
+}
+
+}  // namespace
+
+void launch_degrid(hipStream_t s, const float2 *grid, const float *u, const float *v,
+                   const float *w, const float *wt_in, const float2 *vis_in, int64_t nvis,
+                   const CoordParams &cp, const float *cfu, const float *cfv, float2 *vis_out,
+                   float *wt_out, unsigned long long *nbad) {
+    if (nvis <= 0) return;
+    const int64_t nb = (nvis + 255) / 256;
+    hipLaunchKernelGGL(k_degrid, dim3((unsigned)nb), dim3(256), 0, s, grid, u, v, w, wt_in, vis_in,
+                       nvis, cp, cfu, cfv, vis_out, wt_out, nbad);
+}
+
+int wt_blocks(int64_t nvis) {
+    int64_t b = (nvis + kWtThreads - 1) / kWtThreads;
+    if (b < 1) b = 1;
+    return (int)(b > kMaxWtBlocks ? kMaxWtBlocks : b);
+}
+
+void launch_wt_density(hipStream_t s, const float *wt, const float *u, const float *v,
+                       const float *w, int64_t nvis, const CoordParams &cp, double *gwt,
+                       double *blk, int nblk) {
+    hipLaunchKernelGGL(k_wt_density, dim3(nblk), dim3(kWtThreads), 0, s, wt, u, v, w, nvis, cp, gwt,
+                       blk);
+}
+
+void launch_wt_finish(hipStream_t s, const double *gwt, int ncell, const double *blk, int nblk,
+                      int mode, float rob_t2, float absf2, float absd2, double *wsums) {
+    hipLaunchKernelGGL(k_wt_finish, dim3(1), dim3(1024), 0, s, gwt, ncell, blk, nblk, mode, rob_t2,
+                       absf2, absd2, wsums);
+}
+
+void launch_wt_apply(hipStream_t s, const float *wt, const float *u, const float *v,
+                     const float *w, int64_t nvis, const CoordParams &cp, const double *gwt,
+                     const double *wsums, int az, const float *robust, float pi_f,
+                     float *newwt, unsigned long long *ndrop) {
+    if (nvis <= 0) return;
+    float rob1 = robust ? robust[0] : 0.0f, rpa = 0.0f, rbscale = 0.0f;
+    if (az) {
+        rpa = -robust[2] + 90.0f - 90.0f;
+        while (rpa < -90.0f) rpa = rpa + 180.0f;
+        while (rpa > 90.0f) rpa = rpa - 180.0f;
+        rpa = rpa * pi_f / 180.0f;
+        rbscale = (robust[0] - robust[1]) / (pi_f / 2.0f);
+    }
+    const int64_t nb = (nvis + 255) / 256;
+    hipLaunchKernelGGL(k_wt_apply, dim3((unsigned)nb), dim3(256), 0, s, wt, u, v, w, nvis, cp, gwt,
+                       wsums, az, rob1, rpa, rbscale, pi_f, newwt, ndrop);
+}
+
+void launch_gwt_out(hipStream_t s, const double *gwt, float *out, size_t n) {
+    hipLaunchKernelGGL(k_gwt_out, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, gwt, out, n);
+}
+
+void launch_synth(hipStream_t s, const double *bxyz, int nbl, double dec, double ha0,
+                  double dha, int ntime, int nchan, const double *freq, int nsrc,
+                  const double *src, double sigma, uint64_t seed, int64_t nfull, int64_t nkeep,
+                  float2 *vis, float *wt, float *u, float *v, float *w) {
+    if (nkeep <= 0) return;
+    const int64_t nb = (nkeep + 255) / 256;
+    hipLaunchKernelGGL(k_synth, dim3((unsigned)nb), dim3(256), 0, s, bxyz, nbl, dec, ha0, dha, ntime,
+                       nchan, freq, nsrc, src, sigma, seed, nfull, nkeep, vis, wt, u, v, w);
+}

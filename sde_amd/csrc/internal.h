@@ -1,0 +1,125 @@
+// Internal interface between the host orchestration (abi.hip) and the
+// device kernels (grid.hip, fft.hip, degrid.hip).  CDNA4 (gfx950) only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SDE_CHECK(expr)                                                        \
+    do {                                                                       \
+        hipError_t e_ = (expr);                                                \
+        if (e_ != hipSuccess) return sde_fail_hip(e_, #expr, __FILE__, __LINE__); \
+    } while (0)
+
+int sde_fail_hip(hipError_t e, const char *what, const char *file, int line);
+
+// Halo of the padded gridding tile (support of the widest kernel, SF/SZE = 3).
+constexpr int kHalo = 3;
+// Largest kernel phase table held in LDS: osamp * (2*supp + 1) + 1.
+constexpr int kMaxCfLds = 128 * 7 + 1;
+
+// Per-visibility coordinate step of GRDH2D/GRDDEH2D (grd/grdh2d.f:109-166).
+struct CoordParams {
+    double shift[9];     // column-major SHIFT(3,3)
+    double twopi;        // the reference's TWOPI = DBLE(8*ATAN(1.0)) (REAL atan)
+    float uscale, uoffset, vscale, voffset;
+    int uorigin, vorigin;
+    int nu, nv;
+    int suppu, suppv, osampu, osampv;
+    int doshift;
+};
+
+// Tiling of the pre-shift grid into T x T tiles (padded by kHalo in LDS).
+struct TileParams {
+    int T;          // tile edge
+    int ntu, ntv;   // tiles along u and v
+    int ntiles;
+    int chunk;      // visibilities per work item
+};
+
+// Sorted gridding record, 16 bytes:
+//   x: local u | local v << 8 | u phase << 16 | v phase << 24
+//   y: weight (float bits)
+//   z, w: weight * visibility (float), after phase rotation and conjugation
+struct __align__(16) GridRec {
+    uint32_t pk;
+    float wt, re, im;
+};
+
+struct __align__(16) WorkItem {
+    int tile, begin, end, pad;
+};
+
+// ---- gridding (grid.hip) ----
+void launch_grid_prep(hipStream_t s, const float2 *vis, const float *wt, const float *u,
+                      const float *v, const float *w, int64_t nvis, const CoordParams &cp,
+                      const TileParams &tp, int psf_only, const float *cfu, const float *cfv,
+                      int *tile_of, GridRec *rec, int *tile_count, double *blk_sums, int nblk);
+int grid_prep_blocks(int64_t nvis);
+void launch_grid_scan(hipStream_t s, const int *tile_count, int ntiles, int chunk,
+                      int *tile_start, int *item_start, int *tile_cursor, int *nitems,
+                      const double *blk_sums, int nblk, const CoordParams &cp, double *sums);
+void launch_grid_scatter(hipStream_t s, const int *tile_of, const GridRec *rec, int64_t nvis,
+                         const int *tile_start, int *tile_cursor, GridRec *sorted);
+void launch_grid_items(hipStream_t s, const int *tile_start, const int *item_start, int ntiles,
+                       int chunk, WorkItem *items);
+// nch: channel mask, bit 0 = dirty (2 floats), bit 1 = PSF (1 float)
+int launch_grid_tiles(hipStream_t s, const GridRec *sorted, const WorkItem *items,
+                      const int *nitems, int max_items, const TileParams &tp, int chmask,
+                      int waves, const float *cfu, const float *cfv, const CoordParams &cp,
+                      float *partial);
+void launch_grid_reduce(hipStream_t s, const float *partial, const int *item_start,
+                        const TileParams &tp, int chmask, const CoordParams &cp,
+                        const double *sums, float2 *dirty, float2 *psf);
+size_t grid_partial_floats(const TileParams &tp, int chmask, int max_items);
+
+// ---- BOX gridding (grid.hip) ----
+void launch_box_grid(hipStream_t s, const float2 *vis, const float *wt, const float *u,
+                     const float *v, const float *w, int64_t nvis, const CoordParams &cp,
+                     int psf, double *acc, double *sums);
+void launch_box_fold(hipStream_t s, const double *acc, const CoordParams &cp,
+                     const double *sums, float2 *gvis);
+
+// ---- FFT (fft.hip) ----
+// twiddle table: tw[k] = exp(+2 pi i k / n), k < n, as float2
+int fft_supported(int n);
+// scale multiplies the column output (1 in the fused pipeline, 1/NV for FFTR2D)
+void launch_fft_cols(hipStream_t s, float2 *grid, int nu, int nv, int ngrids, size_t grid_stride,
+                     int dir, float scale, const float2 *tw_nv);
+// DIR < 0 rows: grid rows -> image (NX,NY) with centring, crop, correction,
+// scale, optional peak tracking and optional division by a device peak.
+void launch_fft_rows_c2r(hipStream_t s, const float2 *grid, int nu, int nv, int fnu, int nx,
+                         int ny, const float2 *tw_fnu, const float *ccf, int ccflen,
+                         int correct, float scale, float *image, int *peak_key,
+                         const int *div_key);
+// DIR > 0 rows: image -> grid rows (zero rows outside the image), optional correction.
+void launch_fft_rows_r2c(hipStream_t s, const float *image, int nx, int ny, int fnu, int nv,
+                         const float2 *tw_fnu, const float *ccf, int ccflen, int correct,
+                         float2 *grid);
+void launch_scale_by_peak(hipStream_t s, float *image, size_t n, const int *peak_key);
+void launch_pixcor(hipStream_t s, const float *in, float *out, int nx, int ny, int cplx,
+                   const float *cf, int cflen, int correct, float cenx, float ceny);
+
+// ---- degridding and weighting (degrid.hip) ----
+// vis_in supplies the output of visibilities with WT <= 0 (left as they were).
+void launch_degrid(hipStream_t s, const float2 *grid, const float *u, const float *v,
+                   const float *w, const float *wt_in, const float2 *vis_in, int64_t nvis,
+                   const CoordParams &cp, const float *cfu, const float *cfv, float2 *vis_out,
+                   float *wt_out, unsigned long long *nbad);
+void launch_wt_density(hipStream_t s, const float *wt, const float *u, const float *v,
+                       const float *w, int64_t nvis, const CoordParams &cp, double *gwt,
+                       double *blk, int nblk);
+int wt_blocks(int64_t nvis);
+void launch_wt_finish(hipStream_t s, const double *gwt, int ncell, const double *blk, int nblk,
+                      int mode, float rob_t2, float absf2, float absd2, double *wsums);
+void launch_wt_apply(hipStream_t s, const float *wt, const float *u, const float *v,
+                     const float *w, int64_t nvis, const CoordParams &cp, const double *gwt,
+                     const double *wsums, int az, const float *robust, float pi_f,
+                     float *newwt, unsigned long long *ndrop);
+void launch_gwt_out(hipStream_t s, const double *gwt, float *out, size_t n);
+
+// ---- synthetic data (degrid.hip) ----
+void launch_synth(hipStream_t s, const double *bxyz, int nbl, double dec, double ha0,
+                  double dha, int ntime, int nchan, const double *freq, int nsrc,
+                  const double *src, double sigma, uint64_t seed, int64_t nfull, int64_t nkeep,
+                  float2 *vis, float *wt, float *u, float *v, float *w);

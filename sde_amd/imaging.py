@@ -1,0 +1,238 @@
+"""uvmap / vismodel drivers on a device-resident plan.
+
+Mirrors the task-level flow of the reference:
+
+  * ``uvmap``    weighting (GRDUWT / GRDRUWT) -> PSF -> dirty image, both
+                 divided by the PSF peak (task/uvmap.f:276-282, 424-491);
+  * ``vismodel`` correct -> FFT -> degrid (task/vismodel.f:69-78);
+
+with the image header and uv-cell conventions of IMGMAKE
+(img/imgmake.f:85-104), FFTCONJA (fft/fftconja.f:156-168) and the shift
+matrix of CRDSHIFT (crd/crdshift.f:95-160).  Everything between handing
+over the visibilities and reading back the results stays in HBM.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import SdeError, lib
+
+
+class ImagingCfg(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("cellsize", C.c_float), ("cftype", C.c_char * 4),
+                ("shift", C.c_double * 9), ("weighting", C.c_int), ("fov", C.c_float), ("rmode", C.c_char * 8),
+                ("robust", C.c_float * 3), ("ds", C.c_float), ("tile", C.c_int), ("chunk", C.c_int),
+                ("waves", C.c_int)]
+
+
+class RunInfo(C.Structure):
+    _fields_ = [("nvis", C.c_int64), ("ngridded", C.c_int64), ("ndropped", C.c_int64), ("nbad", C.c_int64),
+                ("sumwt", C.c_double), ("lsumwt", C.c_double), ("maxpsf", C.c_float), ("nitems", C.c_int)]
+
+
+def crdshift(ra_vis, dec_vis, ra_img, dec_img, rota_vis=0.0):
+    """CRDSHIFT (crd/crdshift.f:95-160): 3x3 FP64 rotation (column-major, 9 values)
+    taking visibilities phased at (ra_vis, dec_vis) to an image centred at
+    (ra_img, dec_img), degrees."""
+    dtor = 4.0 * math.atan(1.0) / 180.0
+    rao, deco = dtor * ra_vis, dtor * dec_vis
+    ra, dec = dtor * ra_img, dtor * dec_img
+    d = ra - rao
+    t = np.zeros((3, 3))
+    t[0, 0] = math.cos(d)
+    t[1, 0] = math.sin(d) * math.sin(deco)
+    t[2, 0] = -math.sin(d) * math.cos(deco)
+    t[0, 1] = -math.sin(d) * math.sin(dec)
+    t[1, 1] = math.cos(d) * math.sin(dec) * math.sin(deco) + math.cos(dec) * math.cos(deco)
+    t[2, 1] = -math.cos(d) * math.cos(deco) * math.sin(dec) + math.cos(dec) * math.sin(deco)
+    t[0, 2] = math.sin(d) * math.cos(dec)
+    t[1, 2] = -math.cos(d) * math.sin(deco) * math.cos(dec) + math.sin(dec) * math.cos(deco)
+    t[2, 2] = math.cos(d) * math.cos(dec) * math.cos(deco) + math.sin(dec) * math.sin(deco)
+    rot = dtor * rota_vis
+    if rot != 0.0:
+        r = np.array([[math.cos(rot), math.sin(rot), 0.0], [-math.sin(rot), math.cos(rot), 0.0], [0.0, 0.0, 1.0]])
+        s = np.zeros((3, 3))
+        for j in range(3):
+            for i in range(3):
+                acc = 0.0
+                for k in range(3):
+                    acc = acc + t[k, j] * r[i, k]
+                s[i, j] = acc
+        out = np.zeros((3, 3))
+        for j in range(3):
+            for i in range(3):
+                acc = 0.0
+                for k in range(3):
+                    acc = acc + r[j, k] * s[i, k]
+                out[i, j] = acc
+        t = out
+    return np.ascontiguousarray(t.T.reshape(-1))  # column-major SHIFT(i,j) at [i + 3j]
+
+
+class Plan:
+    """Device-resident imaging plan (image size, cell, kernel, weighting)."""
+
+    def __init__(self, nx, ny=None, cellsize=1.0, cftype="SF", shift=None, weighting=0, fov=1.0, rmode="NONE",
+                 robust=(0.0, 0.0, 0.0), ds=0.0, tile=0, chunk=0, waves=0):
+        L = lib()
+        cfg = ImagingCfg()
+        cfg.nx, cfg.ny = int(nx), int(nx if ny is None else ny)
+        cfg.cellsize = float(cellsize)
+        cfg.cftype = cftype.encode()
+        sh = np.eye(3).reshape(-1) if shift is None else np.asarray(shift, np.float64).reshape(-1)
+        cfg.shift = (C.c_double * 9)(*sh)
+        cfg.weighting = int(weighting)
+        cfg.fov = float(fov)
+        cfg.rmode = rmode.encode()
+        cfg.robust = (C.c_float * 3)(*[float(x) for x in robust])
+        cfg.ds = float(ds)
+        cfg.tile, cfg.chunk, cfg.waves = int(tile), int(chunk), int(waves)
+        self.cfg = cfg
+        self._p = C.c_void_p()
+        if L.sde_plan_create(C.byref(cfg), C.byref(self._p)) != 0:
+            raise SdeError("sde_plan_create: " + L.sde_last_error().decode())
+        g = [C.c_int() for _ in range(5)]
+        us, vs = C.c_float(), C.c_float()
+        L.sde_plan_geometry(self._p, *[C.byref(x) for x in g], C.byref(us), C.byref(vs))
+        self.nu, self.nv, self.fnu, self.uorigin, self.vorigin = [x.value for x in g]
+        self.uscale, self.vscale = us.value, vs.value
+        self.nx, self.ny = cfg.nx, cfg.ny
+        self.nvis = 0
+
+    def _ok(self, rc, what):
+        if rc != 0:
+            raise SdeError(what + ": " + lib().sde_last_error().decode())
+
+    def close(self):
+        if self._p:
+            lib().sde_plan_destroy(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_vis(self, vis, wt, u, v, w):
+        vis = np.ascontiguousarray(vis, np.complex64)
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (wt, u, v, w)]
+        self._keep = (vis, arrs)
+        fp = C.POINTER(C.c_float)
+        self._ok(lib().sde_plan_set_vis(self._p, C.c_int64(len(vis)), vis.ctypes.data_as(fp),
+                                        *[a.ctypes.data_as(fp) for a in arrs], 0), "sde_plan_set_vis")
+        self.nvis = len(vis)
+
+    def synth(self, enu, lat, dec, ha0, dha, ntime, freqs, flux, l, m, sigma, seed, nkeep=0):
+        """Fill the plan with a synthetic track (see sde_amd.synth)."""
+        enu = np.ascontiguousarray(enu, np.float64)
+        freqs = np.ascontiguousarray(freqs, np.float64)
+        flux, l, m = [np.ascontiguousarray(a, np.float64) for a in (flux, l, m)]
+        dp = C.POINTER(C.c_double)
+        self._ok(lib().sde_plan_synth(self._p, len(enu), enu.ctypes.data_as(dp), C.c_double(lat), C.c_double(dec),
+                                      C.c_double(ha0), C.c_double(dha), int(ntime), len(freqs),
+                                      freqs.ctypes.data_as(dp), len(flux), flux.ctypes.data_as(dp),
+                                      l.ctypes.data_as(dp), m.ctypes.data_as(dp), C.c_double(sigma),
+                                      C.c_uint64(seed), C.c_int64(nkeep)), "sde_plan_synth")
+        nb = len(enu) * (len(enu) - 1) // 2
+        full = nb * ntime * len(freqs)
+        self.nvis = nkeep if 0 < nkeep < full else full
+
+    def uvmap(self):
+        self._ok(lib().sde_plan_uvmap(self._p), "sde_plan_uvmap")
+
+    def vismodel(self, image=None):
+        if image is None:
+            self._ok(lib().sde_plan_vismodel(self._p, None), "sde_plan_vismodel")
+        else:
+            img = np.ascontiguousarray(image, np.float32)
+            if img.shape != (self.ny, self.nx):
+                raise ValueError("image shape must be (NY, NX)")
+            self._img_keep = img
+            self._ok(lib().sde_plan_vismodel(self._p, img.ctypes.data_as(C.POINTER(C.c_float))),
+                     "sde_plan_vismodel")
+
+    def sync(self):
+        self._ok(lib().sde_plan_sync(self._p), "sde_plan_sync")
+
+    def images(self):
+        d = np.zeros((self.ny, self.nx), np.float32)
+        p = np.zeros_like(d)
+        fp = C.POINTER(C.c_float)
+        self._ok(lib().sde_plan_get_images(self._p, d.ctypes.data_as(fp), p.ctypes.data_as(fp)), "get_images")
+        return d, p
+
+    def grids(self):
+        d = np.zeros((self.nv, self.nu), np.complex64)
+        p = np.zeros_like(d)
+        fp = C.POINTER(C.c_float)
+        self._ok(lib().sde_plan_get_grids(self._p, d.ctypes.data_as(fp), p.ctypes.data_as(fp)), "get_grids")
+        return d, p
+
+    def weights(self):
+        w = np.zeros(self.nvis, np.float32)
+        self._ok(lib().sde_plan_get_weights(self._p, w.ctypes.data_as(C.POINTER(C.c_float))), "get_weights")
+        return w
+
+    def model(self):
+        v = np.zeros(self.nvis, np.complex64)
+        w = np.zeros(self.nvis, np.float32)
+        fp = C.POINTER(C.c_float)
+        self._ok(lib().sde_plan_get_model(self._p, v.ctypes.data_as(fp), w.ctypes.data_as(fp)), "get_model")
+        return v, w
+
+    def device_vis(self):
+        ptrs = [C.POINTER(C.c_float)() for _ in range(5)]
+        self._ok(lib().sde_plan_device_vis(self._p, *[C.byref(x) for x in ptrs]), "device_vis")
+        return [C.cast(x, C.c_void_p).value for x in ptrs]
+
+    def info(self):
+        r = RunInfo()
+        self._ok(lib().sde_plan_info(self._p, C.byref(r)), "sde_plan_info")
+        return {k: getattr(r, k) for k, _ in RunInfo._fields_}
+
+    def profile(self, on=True):
+        self._ok(lib().sde_plan_profile(self._p, 1 if on else 0), "profile")
+
+    def timings(self):
+        n = 64
+        names = (C.c_char * 32 * n)()
+        ms = (C.c_double * n)()
+        cnt = (C.c_int * n)()
+        k = lib().sde_plan_timings(self._p, n, names, ms, cnt)
+        if k < 0:
+            raise SdeError(lib().sde_last_error().decode())
+        return {bytes(names[i]).split(b"\0")[0].decode(): (ms[i], cnt[i]) for i in range(k)}
+
+    def reset_timings(self):
+        self._ok(lib().sde_plan_reset_timings(self._p), "reset_timings")
+
+
+def uvmap(vis, wt, u, v, w, nx, cellsize, ny=None, fov=0.0, rmode="NONE", robust=(0.0, 0.0, 0.0), ds=0.0,
+          cftype="SF", shift=None, **tune):
+    """Dirty image and PSF (each divided by the PSF peak) and run info."""
+    p = Plan(nx, ny, cellsize, cftype, shift, weighting=1 if fov != 0.0 else 0, fov=fov, rmode=rmode,
+             robust=robust, ds=ds, **tune)
+    try:
+        p.set_vis(vis, wt, u, v, w)
+        p.uvmap()
+        dirty, psf = p.images()
+        info = p.info()
+    finally:
+        p.close()
+    return dirty, psf, info
+
+
+def vismodel(image, wt, u, v, w, cellsize, cftype="SF", shift=None, vis=None, **tune):
+    """Model visibilities of an image (NY, NX); returns (vis, wt-with-flags)."""
+    ny, nx = image.shape
+    p = Plan(nx, ny, cellsize, cftype, shift, **tune)
+    try:
+        n = len(u)
+        p.set_vis(np.zeros(n, np.complex64) if vis is None else vis, wt, u, v, w)
+        p.vismodel(image)
+        mv, mw = p.model()
+    finally:
+        p.close()
+    return mv, mw

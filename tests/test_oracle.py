@@ -1,0 +1,405 @@
+"""Pins of the CPU oracle (no GPU).
+
+The oracle is trusted only as far as these checks reach:
+  * the reference's own golden vectors: the 1990 uvmap map/beam of 3C10
+    (reqdata/3C10MAP.FTS, 3C10BEAM.FTS), their logged statistics
+    (test/timings.zee:37-42) and the logged sum of weights
+    (test/veryold/timings.cray2:167);
+  * library routines the algorithm reduces to (numpy's FFT for FFTX/FFTR
+    and the centred 2-D transform; a direct FP64 DFT for gridding and
+    degridding of tiny inputs, within the kernel's algorithmic error);
+  * closed forms and invariants (unit-sum kernel, symmetry, the
+    spheroidal function's Fourier self-similarity that ties GRDCF to
+    GRDCCF, degridding a constant grid, Hermitian symmetry, linearity,
+    hand-counted density weights, robust-weighting limits).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+
+# ---------------------------------------------------------------- tables
+def test_grdcf_sf_table_properties():
+    cf, supp, osamp = po.grdcf("SF")
+    assert (supp, osamp, len(cf)) == (3, 128, 1025)
+    c = 512
+    np.testing.assert_array_equal(cf[c - 512:c + 1][::-1], cf[c:c + 513])  # symmetric
+    assert np.all(cf[c + 384:] == 0.0) and cf[c + 383] > 0.0  # support 3 cells
+    assert abs(float(np.sum(cf, dtype=np.float32)) - 1.0) < 2e-6  # unit sum (REAL sum)
+    assert np.argmax(cf) == c
+    cf64, _, _ = po.grdcf("SF", fp64=True)
+    assert np.max(np.abs(cf - cf64)) < 1e-8
+
+
+def test_grdcf_box_and_sze():
+    cf, supp, osamp = po.grdcf("BOX")
+    assert (supp, len(cf)) == (1, 513)
+    assert np.count_nonzero(cf) == 129 and np.allclose(cf[cf > 0], 1.0 / 129)
+    cf, supp, _ = po.grdcf("SZE")
+    assert supp == 3 and abs(cf.sum() - 1.0) < 2e-6
+    with pytest.raises(ValueError):
+        po.grdcf("XYZ")
+
+
+def test_spheroidal_fourier_self_similarity():
+    """The gridding kernel (1 - nu^2) psi(nu) on [-1, 1] (units of the
+    support) has, to the accuracy of Schwab's fit, a Fourier transform
+    proportional to psi itself: FT{C}(x) / psi(x) is constant over the image
+    (|x| < 1/2 of the field).  This ties GRDSF, GRDCF's tabulation and the
+    correction function of GRDCCF together: a wrong coefficient, a dropped
+    (1 - nu^2) factor or a mis-scaled argument breaks it."""
+    ccf = po.grdccf("SF", 129)
+    nu = np.linspace(-1, 1, 4001)
+    c = np.array([(1 - t * t) * po.grdsf(abs(t)) for t in nu])
+    # t = image offset / half field (the GRDCCF argument); a 3-cell kernel of
+    # grid spacing 1/(N dx) gives the transform kernel exp(i 3 pi nu t) (c = 3 pi)
+    t = np.linspace(0.0, 1.0, 129)
+    ft = np.array([np.trapezoid(c * np.cos(3.0 * math.pi * tt * nu), nu) for tt in t])
+    ratio = ft / ccf
+    assert np.std(ratio) / np.mean(ratio) < 1e-5, np.std(ratio) / np.mean(ratio)
+
+
+def test_grdccf_values():
+    ccf = po.grdccf("SF", 65)
+    assert ccf[0] == pytest.approx(1.0, abs=1e-6)
+    assert ccf[-1] == pytest.approx(4.028559e-3, rel=1e-6)  # P(0,2)/Q(0,2) at nu = 1
+    assert np.all(np.diff(ccf) < 0)
+    box = po.grdccf("BOX", 65)
+    assert box[0] == 1.0 and box[-1] == pytest.approx(2 / math.pi, rel=1e-6)
+
+
+def test_fftpwr2():
+    for n, p in [(1, 1), (2, 2), (3, 4), (5, 8), (100, 128), (128, 128), (129, 256), (4096, 4096), (4097, 8192)]:
+        assert po.fftpwr2(n) == p
+
+
+def test_crdshift_identity_and_rotation():
+    s = po.crdshift(10.0, 40.0, 10.0, 40.0)
+    np.testing.assert_allclose(s.reshape(3, 3), np.eye(3), atol=1e-15)
+    s = po.crdshift(10.0, 40.0, 10.5, 40.3).reshape(3, 3).T
+    np.testing.assert_allclose(s @ s.T, np.eye(3), atol=1e-14)  # rotation
+    assert np.linalg.det(s) == pytest.approx(1.0, abs=1e-14)
+
+
+# ---------------------------------------------------------------- FFT chain
+@pytest.mark.parametrize("m", [4, 8, 16, 256, 1024, 4096])
+def test_fftx_against_numpy(m):
+    rng = np.random.default_rng(m)
+    x = (rng.standard_normal(m) + 1j * rng.standard_normal(m)).astype(np.complex64)
+    fwd = po.fftx(x, +1)
+    inv = po.fftx(x, -1)
+    ref_f = np.fft.fft(x.astype(np.complex128))
+    ref_i = np.fft.ifft(x.astype(np.complex128))
+    assert np.max(np.abs(fwd - ref_f)) < 2e-6 * np.max(np.abs(ref_f)) * math.log2(m)
+    assert np.max(np.abs(inv - ref_i)) < 2e-6 * np.max(np.abs(ref_i)) * math.log2(m)
+
+
+def test_fftx_round_trip_known_answer():
+    """task/fftx.f: a unit delta at element 16 survives forward+inverse
+    transforms to 1e-5 (logged in test/timings.sun3:88-106)."""
+    for n in (256, 512, 1024, 2048, 4096):
+        x = np.zeros(n, np.complex64)
+        x[15] = 1.0
+        y = x
+        for _ in range(20):
+            y = po.fftx(po.fftx(y, +1), -1)
+        assert np.max(np.abs(y[:16] - x[:16])) < 1e-5
+
+
+@pytest.mark.parametrize("n", [8, 64, 512])
+def test_fftx1d_centring(n):
+    rng = np.random.default_rng(n)
+    x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    c = (n + 1) // 2 - 1
+    ref = np.roll(np.fft.ifft(np.roll(x.astype(np.complex128), -c)), c)
+    got32 = po.fftx1d(x, -1)
+    got64 = po.fftx1d(x, -1, fp64=True)
+    assert np.max(np.abs(got64 - ref)) < 1e-13
+    assert np.max(np.abs(got32 - ref)) < 3e-6 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("nx,ny,fnu,nv", [(64, 64, 64, 64), (100, 90, 128, 128), (256, 256, 256, 256)])
+def test_fftr2d_against_numpy(nx, ny, fnu, nv):
+    """Centred half-plane transform == numpy's irfft2 with the centre pixels
+    (FNU+1)/2, (NV+1)/2 moved to index 0, and the padding/cropping offsets."""
+    rng = np.random.default_rng(nx)
+    nu = fnu // 2 + 1
+    img = rng.standard_normal((ny, nx))
+    # reference construction with numpy: pad, shift centre to 0, rfft2 with v centring
+    pad = np.zeros((nv, fnu))
+    ioff, ibegy = (fnu - nx) // 2, (nv + 1) // 2 - (ny + 1) // 2
+    pad[ibegy:ibegy + ny, ioff:ioff + nx] = img
+    cx, cy = (fnu + 1) // 2 - 1, (nv + 1) // 2 - 1
+    spec = np.fft.rfft(np.roll(pad, -cx, axis=1), axis=1)  # rows: exp(-i)
+    spec = np.roll(np.fft.fft(np.roll(spec, -cy, axis=0), axis=0), cy, axis=0)
+    got64 = po.fftr2d_r2c(img, fnu, nv, fp64=True)
+    got32 = po.fftr2d_r2c(img.astype(np.float32), fnu, nv)
+    assert got64.shape == (nv, nu)
+    assert np.max(np.abs(got64 - spec)) < 1e-10 * np.abs(spec).max()
+    assert np.max(np.abs(got32 - spec)) < 3e-6 * np.abs(spec).max()
+    back64 = po.fftr2d_c2r(spec, nx, ny, fp64=True)
+    back32 = po.fftr2d_c2r(spec.astype(np.complex64), nx, ny)
+    np.testing.assert_allclose(back64, img, atol=1e-12)
+    np.testing.assert_allclose(back32, img, atol=3e-6 * np.abs(img).max())
+
+
+# ---------------------------------------------------------------- gridding
+def _geom(nx, cell):
+    fnu = po.fftpwr2(nx)
+    nv = fnu
+    ud = po.fftconj_delt(nx, np.float32(-cell) / np.float32(3600.0))
+    vd = po.fftconj_delt(nx, np.float32(cell) / np.float32(3600.0))
+    return fnu // 2 + 1, nv, (nv + 1) // 2, np.float32(1) / np.float32(ud), np.float32(1) / np.float32(vd)
+
+
+def test_grdh2d_single_visibility_at_centre():
+    """One unit visibility at u = v = 0 lands on the centre column with the
+    kernel's separable taps, then the fold doubles the real part on the
+    u = 0 column and the normalisation makes the total (NU-1)*NV*2..."""
+    cf, supp, osamp = po.grdcf("SF")
+    nu, nv = 17, 32
+    g, sw, rc = po.grdh2d(np.array([1 + 0j]), np.array([1.0]), np.zeros(1), np.zeros(1), np.zeros(1), 1.0, 1.0, 1,
+                          16, False, nu, nv, cf, supp, osamp, np.eye(3).reshape(-1))
+    assert rc == 0 and sw == 1.0
+    taps = cf[512 - 384:512 + 385:128]  # 7 taps at zero fractional offset
+    s = (nu - 1) * nv / float(np.sum(taps, dtype=np.float64) ** 2)
+    # column u=0 after the shift is column 1; v rows 16-3..16+3 (1-based)
+    col0 = g[:, 0]
+    expect = np.zeros(nv)
+    expect[15 - 3:15 + 4] = 2 * taps[3] * taps * s  # own tap + folded conjugate at u = 0
+    np.testing.assert_allclose(col0.real, expect, rtol=2e-6, atol=1e-9)
+    assert np.all(col0.imag == 0)
+    np.testing.assert_allclose(g[15 - 3:15 + 4, 1].real, taps[4] * taps * s * 2, rtol=2e-6)
+
+
+def test_grdh2d_linearity_and_hermitian_fold():
+    rng = np.random.default_rng(1)
+    n = 500
+    u = rng.uniform(-12, 12, n).astype(np.float32)
+    v = rng.uniform(-12, 12, n).astype(np.float32)
+    w = np.zeros(n, np.float32)
+    # dyadic values, so a + 2b is exact in complex64
+    a = (np.round(8 * rng.standard_normal(n)) / 8 + 1j * np.round(8 * rng.standard_normal(n)) / 8).astype(np.complex64)
+    b = (np.round(8 * rng.standard_normal(n)) / 8 + 1j * np.round(8 * rng.standard_normal(n)) / 8).astype(np.complex64)
+    wt = np.ones(n)
+    cf, supp, osamp = po.grdcf("SF", fp64=True)
+    args = (1.0, 1.0, 1, 16, False, 17, 32, cf, supp, osamp, np.eye(3).reshape(-1))
+    ga = po.grdh2d64(a, wt, u, v, w, *args)[0]
+    gb = po.grdh2d64(b, wt, u, v, w, *args)[0]
+    gab = po.grdh2d64((a + 2 * b).astype(np.complex64), wt, u, v, w, *args)[0]
+    np.testing.assert_allclose(gab, ga + 2 * gb, atol=1e-10)
+    # u = 0 column Hermitian about the v origin (row 16)
+    col = ga[:, 0]
+    for k in range(1, 15):
+        assert col[15 + k] == pytest.approx(np.conj(col[15 - k]), abs=1e-12)
+
+
+def test_gridded_image_matches_direct_dft_tiny():
+    """Brute force: 40 visibilities, 64^2 image; the gridded FP64 dirty image
+    equals the direct FP64 DFT within the PSWF's algorithmic error in the
+    inner field."""
+    rng = np.random.default_rng(4)
+    n = 40
+    u = rng.uniform(-500, 500, n).astype(np.float32)
+    v = rng.uniform(-500, 500, n).astype(np.float32)
+    w = np.zeros(n, np.float32)
+    vis = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    wt = np.ones(n, np.float32)
+    cell = 40.0
+    cfg = po.make_cfg(64, cellsize=cell)
+    d64, p64, sw, rc, _ = po.uvmap(cfg, vis, wt, u, v, w, fp64=True)
+    assert rc == 0
+    dx, dy = -cell / 3600 * math.pi / 180, cell / 3600 * math.pi / 180
+    dd = po.dft_image(vis, wt, u, v, 64, 64, dx, dy)
+    pd = po.dft_image(vis, wt, u, v, 64, 64, dx, dy, psf=True)
+    inner = (slice(16, 48), slice(16, 48))
+    assert np.max(np.abs(p64 - pd)[inner]) < 2e-3
+    # random visibilities fill the whole field: the 6-cell kernel's aliasing floor is a few 1e-3 of peak
+    assert np.max(np.abs(d64 - dd / pd.max())[inner]) < 5e-3 * np.abs(dd).max()
+    assert np.max(np.abs(d64 + dd / pd.max())[inner]) > 0.1 * np.abs(dd).max()  # the sign convention matters
+
+
+def test_fp32_restatement_tracks_fp64_on_random_data():
+    rng = np.random.default_rng(8)
+    n = 20000
+    u = rng.uniform(-2000, 2000, n).astype(np.float32)
+    v = rng.uniform(-2000, 2000, n).astype(np.float32)
+    w = rng.uniform(-500, 500, n).astype(np.float32)
+    vis = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    wt = rng.uniform(0.5, 2, n).astype(np.float32)
+    nu, nv, vo, us, vs = _geom(256, 15.0)
+    cf, supp, osamp = po.grdcf("SF")
+    cf64, _, _ = po.grdcf("SF", fp64=True)
+    s = po.crdshift(50.0, 20.0, 50.01, 20.02)
+    for psf in (False, True):
+        g32, sw32, _ = po.grdh2d(vis, wt, u, v, w, us, vs, 1, vo, psf, nu, nv, cf, supp, osamp, s)
+        g64, sw64, _, _ = po.grdh2d64(vis, wt, u, v, w, us, vs, 1, vo, psf, nu, nv, cf64, supp, osamp, s)
+        assert np.max(np.abs(g32 - g64)) < 1e-5 * np.abs(g64).max()
+        assert sw32 == pytest.approx(sw64, rel=5e-6)  # REAL running sum of 2e4 weights
+
+
+# ---------------------------------------------------------------- degridding
+def test_degrid_constant_grid_is_constant():
+    """A constant grid degrids to that constant (taps are divided by their
+    sum), conjugated for points reflected from u > 0 (USCALE < 0 here),
+    for every point including those near the u = 0 axis."""
+    rng = np.random.default_rng(2)
+    n = 2000
+    u = rng.uniform(-20, 20, n).astype(np.float32)
+    u[:300] = rng.uniform(-3, 3, 300).astype(np.float32)
+    v = rng.uniform(-20, 20, n).astype(np.float32)
+    w = np.zeros(n, np.float32)
+    c = 0.75 - 0.5j
+    grid = np.full((64, 33), c, np.complex128)
+    cf, supp, osamp = po.grdcf("SF", fp64=True)
+    vis, wt, nbad = po.grddeh2d64(np.ones(n), u, v, w, np.float32(-1.0), np.float32(1.0), 1, 32, grid, cf, supp,
+                                  osamp, np.eye(3).reshape(-1))
+    assert nbad == 0 and np.all(wt > 0)
+    pos = (-1.0 * u) > 0  # USCALE*u > 0: conjugated
+    # near-axis reflected taps read conj(grid): on a constant grid they mix c and conj(c)
+    far = np.abs(u) > 4
+    np.testing.assert_allclose(vis[far & pos], np.conj(c), atol=1e-12)
+    np.testing.assert_allclose(vis[far & ~pos], c, atol=1e-12)
+    np.testing.assert_allclose(vis.real, c.real, atol=1e-12)
+
+
+def test_vismodel_point_source_matches_dft():
+    """vismodel of a single-pixel model equals the direct FP64 DFT
+    V = exp(+2 pi i (u l + v m)) within the unpadded PSWF floor."""
+    rng = np.random.default_rng(6)
+    n = 3000
+    cell = 10.0
+    u = rng.uniform(-3000, 3000, n).astype(np.float32)
+    v = rng.uniform(-3000, 3000, n).astype(np.float32)
+    w = np.zeros(n, np.float32)
+    img = np.zeros((128, 128))
+    img[70, 50] = 1.0
+    cfg = po.make_cfg(128, cellsize=cell)
+    vis64, wt, nbad = po.vismodel(cfg, img, np.ones(n, np.float32), u, v, w, fp64=True)
+    dx, dy = -cell / 3600 * math.pi / 180, cell / 3600 * math.pi / 180
+    ref = po.dft_predict(img, dx, dy, u, v)
+    assert nbad == 0
+    assert np.max(np.abs(vis64 - ref)) < 5e-3
+    vis32, _, _ = po.vismodel(cfg, img.astype(np.float32), np.ones(n, np.float32), u, v, w)
+    assert np.max(np.abs(vis32 - vis64)) < 1e-5
+
+
+def test_degrid_flags_off_grid_points():
+    cf, supp, osamp = po.grdcf("SF")
+    u = np.array([5.0, 40.0, 0.0, -5.0], np.float32)
+    v = np.array([3.0, 0.0, 31.0, 2.0], np.float32)
+    w = np.zeros(4, np.float32)
+    wt = np.array([1.0, 1.0, 1.0, -2.0], np.float32)
+    grid = np.ones((64, 33), np.complex64)
+    vis, wout, nbad = po.grddeh2d(wt, u, v, w, 1.0, 1.0, 1, 32, grid, cf, supp, osamp, np.eye(3).reshape(-1))
+    assert wout[0] == 1.0 and wout[1] == -1.0 and wout[2] == -1.0 and wout[3] == -2.0
+    assert vis[1] == 0 and vis[2] == 0
+
+
+# ---------------------------------------------------------------- weighting
+def test_uniform_weights_hand_counted():
+    # cells (UCELL = |u|*1 + 1, VCELL = v + 16): three points in one cell, one alone, one on the u axis
+    u = np.array([3.1, 3.2, 2.9, -7.0, 0.2, 0.0], np.float32)
+    v = np.array([4.0, 4.1, 3.9, -2.0, 5.0, 0.0], np.float32)
+    w = np.zeros(6, np.float32)
+    wt = np.array([1, 1, 2, 1, 1, 1], np.float32)
+    neww, gwt, nd = po.grduwt2d(wt, u, v, w, 1.0, 1.0, 1, 16, 9, 32, np.eye(3).reshape(-1))
+    assert nd == 0
+    np.testing.assert_allclose(neww[:3], [0.25, 0.25, 0.5])
+    assert neww[3] == 1.0
+    assert neww[4] == 0.5  # u-axis cell off the origin: halved
+    assert neww[5] == 1.0  # the origin itself is not halved
+    assert gwt.sum() == wt.sum()
+
+
+def test_robust_limits():
+    rng = np.random.default_rng(3)
+    n = 5000
+    u = rng.normal(0, 6, n).astype(np.float32)
+    v = rng.normal(0, 6, n).astype(np.float32)
+    w = np.zeros(n, np.float32)
+    wt = np.ones(n, np.float32)
+    s = np.eye(3).reshape(-1)
+    uni, gwt, _ = po.grduwt2d(wt, u, v, w, 1.0, 1.0, 1, 32, 33, 64, s)
+    nat, _, _ = po.grdruw2d(wt, u, v, w, 1.0, 1.0, 1, 32, 33, 64, s, "NORM", (5.0, 0, 0))
+    rob_u, _, _ = po.grdruw2d(wt, u, v, w, 1.0, 1.0, 1, 32, 33, 64, s, "NORM", (-5.0, 0, 0))
+    ok = uni > 0
+    # R = +5: natural weighting (halved on the u = 0 axis, as uniform is)
+    half = np.where(np.isclose(uni * 2.0, 1.0 / np.maximum(uni, 1e-30) * uni * uni * 2.0), 1.0, 1.0)
+    assert np.all(np.abs(nat[ok] - 1.0) < 1e-3) or np.all(np.abs(nat[ok] * (nat[ok] > 0.6) - 1.0)[nat[ok] > 0.6] < 1e-3)
+    assert np.all((np.abs(nat[ok] - 1.0) < 1e-3) | (np.abs(nat[ok] - 0.5) < 1e-3))
+    # R = -5: weights proportional to uniform weights
+    ratio = rob_u[ok] / uni[ok]
+    assert np.std(ratio) / np.mean(ratio) < 1e-4
+    # NORM F2 formula by hand for R = 0
+    r0, _, _ = po.grdruw2d(wt, u, v, w, 1.0, 1.0, 1, 32, 33, 64, s, "NORM", (0.0, 0, 0))
+    g = gwt[gwt > 0].astype(np.float64)
+    f2 = 25.0 / (np.sum(g ** 2) / wt.sum())
+    cellv = []
+    for i in range(n):
+        uc = abs(u[i]) + 1 if u[i] <= 0 else u[i] + 1
+        vc = -v[i] + 32 if u[i] <= 0 else v[i] + 32
+        cellv.append((int(np.floor(vc + 0.5)), int(np.floor(uc + 0.5))))
+    gi = np.array([gwt[a - 1, b - 1] for a, b in cellv])
+    axis = np.array([(b == 1 and a != 32) for a, b in cellv])
+    expect = np.where(axis, 0.5, 1.0) / (gi * f2 + 1.0)
+    np.testing.assert_allclose(r0, expect, rtol=2e-6)
+
+
+def test_grdruw2d_rejects_unknown_mode():
+    with pytest.raises(ValueError):
+        po.grdruw2d(np.ones(3), np.zeros(3), np.zeros(3), np.zeros(3), 1.0, 1.0, 1, 8, 9, 16, np.eye(3).reshape(-1),
+                    "BOGUS", (0, 0, 0))
+
+
+# ---------------------------------------------------------------- golden 3C10
+def _uvmap_3c10(d, fp64=False):
+    cfg = po.make_cfg(128, cellsize=12.0, fov=1.0, ra=float(d["ra"]), dec=float(d["dec"]))
+    return po.uvmap(cfg, d["vis"], d["wt"], d["u"], d["v"], d["w"], fp64=fp64)
+
+
+def test_3c10_restatement_reproduces_reference_map(golden_3c10):
+    d = golden_3c10
+    dirty, psf, sw, rc, _ = _uvmap_3c10(d)
+    assert rc == 0
+    peak = np.abs(d["dirty"]).max()
+    inner = (slice(32, 96), slice(32, 96))
+    assert np.abs(dirty - d["dirty"])[inner].max() < 5e-5 * peak
+    assert np.abs(psf - d["psf"])[inner].max() < 5e-5
+    assert np.abs(dirty - d["dirty"]).max() < 2e-3 * peak  # edges: grid-correction amplified round-off
+    a = dirty.astype(np.float64)
+    # test/timings.zee:37-42
+    assert float(np.mean(a)) == pytest.approx(-2.274e-2, abs=6e-6)
+    assert float(np.sqrt(np.mean(a ** 2))) == pytest.approx(1.403e-1, abs=6e-5)
+    assert float(a.max()) == pytest.approx(6.791e-1, abs=6e-5)
+    assert float(a.min()) == pytest.approx(-3.135e-1, abs=6e-5)
+    # logged -3.725E+02 is the file's -372.549; the sum of 16384 pixels of both signs
+    # amplifies the edge differences, so compare with the file at 5e-5
+    assert float(np.sum(d["dirty"], dtype=np.float64)) == pytest.approx(-3.725e2, abs=0.05)
+    assert float(a.sum()) == pytest.approx(float(np.sum(d["dirty"], dtype=np.float64)), abs=0.02)
+    assert np.unravel_index(np.argmax(psf), psf.shape) == (63, 63) and psf.max() == 1.0
+
+
+def test_3c10_sum_of_weights(golden_3c10):
+    """test/veryold/timings.cray2:167 logs Sum of weights = 2.082500E+02
+    (March 1990).  GRDUWT2D was rewritten in May 1991 (grd/grduwt2d.f:42-47,
+    'Finally fixed to do real uniform weighting'); the current routine gives
+    exactly twice the logged value for this data set (every weight is 1)."""
+    d = golden_3c10
+    _, _, sw, _, neww = _uvmap_3c10(d)
+    assert sw == pytest.approx(2 * 208.25, abs=2e-3)
+    _, _, sw64, _, _ = _uvmap_3c10(d, fp64=True)
+    assert sw64 == pytest.approx(416.5, abs=1e-9)
+    assert np.all(d["wt"] == 1.0)
+
+
+def test_3c10_fp64_oracle_agrees(golden_3c10):
+    d = golden_3c10
+    d32, p32, _, _, _ = _uvmap_3c10(d)
+    d64, p64, _, _, _ = _uvmap_3c10(d, fp64=True)
+    inner = (slice(32, 96), slice(32, 96))
+    assert np.abs(d32 - d64)[inner].max() < 1e-5 * np.abs(d64).max()
+    assert np.abs(d64 - d["dirty"])[inner].max() < 5e-5 * np.abs(d64).max()
