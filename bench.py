@@ -1,0 +1,197 @@
+"""Benchmark of the imaging hot path on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): plain
+single field, large -- 27-antenna Y array at latitude 34 deg, dec +40,
+8 h at 2 s, 64 channels 1.0-2.0 GHz (351 x 14400 x 64 = 3.23e8 samples,
+a uniform-stride subsample of exactly 1.0e8 kept), 20 point sources plus
+noise, robust weighting NORM R = 0 through the density grid, 4096^2 image,
+SF kernel (7 taps, x128), cell = 1/3 synthesized beam at 2 GHz.
+
+One step = one job through the hot path with the visibilities resident in
+HBM: density weighting, PSF + dirty gridding (fused), two centred 2-D FFTs
+with grid correction and PSF-peak normalisation, then prediction of model
+visibilities from the dirty image (correction, FFT, degridding of all
+1.0e8 points).  value = (gridded + degridded visibilities) / step time.
+
+Prints one JSON line.  --gpus N > 1 exits: this implementation is measured
+on one GPU.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "gridded+degridded visibilities/sec at ≤1e-5 peak err vs FP64; p50/p99 job latency"
+PEAK_HBM_GBS = 8000.0
+
+
+def alg_bytes(kernel, nvis, nu, nv, nx, ny):
+    """Algorithmic HBM bytes of one launch of a kernel of the step."""
+    g = nu * nv * 8
+    n2 = nx * ny * 4
+    table = {
+        "grid_tiles": 24 * nvis + 2 * g,  # gridding, dirty + PSF fused: u,v,w,wt,vis in; two grids out
+        "degrid": 24 * nvis + g,          # u,v,w,wt in, vis out; grid read once
+        "grid_prep": 24 * nvis,
+        "fft_cols": 2 * 2 * g,            # two grids read and written
+        "fft_rows_psf": g + n2,
+        "fft_rows_dirty": g + n2,
+        "wt_density": 20 * nvis,
+        "wt_apply": 20 * nvis,
+    }
+    return table.get(kernel)
+
+
+def read_traffic(kernel, workload_tag):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC summary
+    (profiles/pmc_<kernel>_<workload>.json), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_%s_%s.json" % (kernel, workload_tag))
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(vis, wt, u, v, w, nx, cell, nfull):
+    """The FP32 C restatement of the reference path (oracle/ref32.c), one core,
+    on two bounded samples of the same workload; the per-visibility and
+    fixed costs fitted from the two are extrapolated to the full job."""
+    from oracle import pyoracle as po
+    cfg = po.make_cfg(nx, cellsize=cell, fov=1.0, rmode="NORM", robust=(0.0, 0.0, 0.0))
+    t = []
+    sizes = (500_000, 2_000_000)
+    for n in sizes:
+        t0 = time.perf_counter()
+        dirty, psf, sw, rc, _ = po.uvmap(cfg, vis[:n], wt[:n], u[:n], v[:n], w[:n])
+        po.vismodel(cfg, dirty, wt[:n], u[:n], v[:n], w[:n])
+        t.append(time.perf_counter() - t0)
+    per_vis = (t[1] - t[0]) / (sizes[1] - sizes[0])
+    fixed = max(t[0] - per_vis * sizes[0], 0.0)
+    t_full = fixed + per_vis * nfull
+    return {"value": 2.0 * nfull / t_full, "unit": "visibilities/s", "cores": 1, "kind": "port",
+            "sample": "oracle/ref32.c uvmap+vismodel (robust NORM, 4096^2) on the first %d and %d visibilities "
+                      "of the workload: %.2f s and %.2f s; fitted %.3f us/visibility + %.2f s fixed, "
+                      "extrapolated to %d visibilities" % (sizes[0], sizes[1], t[0], t[1], per_vis * 1e6, fixed,
+                                                           nfull)}
+
+
+def accuracy(plan, vis, wt, u, v, w, nx, cell):
+    """Dirty image and PSF errors against the FP64 oracle on the full job."""
+    from oracle import pyoracle as po
+    dirty, psf = plan.images()
+    cfg = po.make_cfg(nx, cellsize=cell, fov=1.0, rmode="NORM", robust=(0.0, 0.0, 0.0))
+    t0 = time.perf_counter()
+    d64, p64, sw64, rc, _ = po.uvmap(cfg, vis, wt, u, v, w, fp64=True)
+    out = {"oracle_s": round(time.perf_counter() - t0, 1)}
+    q = nx // 4
+    for name, a, b in (("dirty", dirty, d64), ("psf", psf, p64)):
+        e = np.abs(a.astype(np.float64) - b)
+        peak = float(np.abs(b).max())
+        out[name] = {"max_err_rel_peak": float(e.max() / peak), "rms_err_rel_peak": float(np.sqrt((e ** 2).mean()) / peak),
+                     "inner_max_err_rel_peak": float(e[nx // 2 - q:nx // 2 + q, nx // 2 - q:nx // 2 + q].max() / peak)}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nvis", type=int, default=100_000_000)
+    ap.add_argument("--no-check", action="store_true", help="skip the FP64 accuracy check")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--waves", type=int, default=0)
+    ap.add_argument("--chunk", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the imaging hot path runs on a single MI355X "
+              "(independent jobs shard one per GPU with no communication); --gpus %d is not run" % args.gpus)
+        sys.exit(2)
+
+    import sde_amd
+    from sde_amd import imaging, synth
+    if sde_amd.device_count() < 1:
+        raise SystemExit("no GPU visible")
+
+    p = synth.large_field_params(nkeep=args.nvis)
+    nx, cell = p["nx"], p["cellsize"]
+    plan = imaging.Plan(nx, nx, cell, "SF", weighting=1, fov=1.0, rmode="NORM", robust=(0.0, 0.0, 0.0),
+                        tile=args.tile, waves=args.waves, chunk=args.chunk)
+    t0 = time.perf_counter()
+    plan.synth(p["enu"], p["lat"], p["dec"], p["ha0"], p["dha"], p["ntime"], p["freqs"], p["flux"], p["l"], p["m"],
+               p["sigma"], p["seed"], p["nkeep"])
+    t_gen = time.perf_counter() - t0
+    nvis = plan.nvis
+    print("# generated %d visibilities in %.1f s" % (nvis, t_gen), file=sys.stderr, flush=True)
+
+    plan.profile(True)
+    for _ in range(args.warmup):
+        plan.uvmap()
+        plan.vismodel(None)
+        plan.sync()
+    plan.reset_timings()
+    lat = []
+    plan.sync()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ts = time.perf_counter()
+        plan.uvmap()
+        plan.vismodel(None)
+        plan.sync()
+        lat.append(time.perf_counter() - ts)
+        print("# step %d %.1f ms" % (k, lat[-1] * 1e3), file=sys.stderr, flush=True)
+    total = time.perf_counter() - t0
+    timings = plan.timings()
+    info = plan.info()
+    ms = total / args.steps * 1e3
+    value = 2.0 * nvis * args.steps / total
+
+    # roofline of the dominant kernel (HIP events on the plan's stream)
+    dom = max(timings.items(), key=lambda kv: kv[1][0])
+    kname, (kms, kcount) = dom
+    avg_s = kms / kcount * 1e-3
+    ab = alg_bytes(kname, nvis, plan.nu, plan.nv, nx, nx)
+    achieved = ab / avg_s / 1e9 if ab else None
+    roof = {"kernel": kname, "bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
+            "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4) if achieved else None,
+            "traffic": read_traffic(kname, "large_field"), "avg_ms": round(avg_s * 1e3, 3),
+            "alg_bytes": ab}
+    kernels = {k: {"ms_per_launch": round(v[0] / v[1], 3), "launches": v[1]} for k, v in timings.items()}
+
+    out = {"metric": METRIC, "value": round(value, 1), "unit": "visibilities/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms, 2), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32 (FP32 complex grids with compensated accumulation; FP64 coordinates)",
+           "data": "synthetic (seeded track and point-source sky generated on the device)",
+           "config": {"workload": "plain single field large: 4096^2, 1.0e8 vis (uniform-stride subsample of "
+                                  "351 bl x 14400 x 64 ch), robust NORM R=0, SF 7x7 x128",
+                      "nvis": nvis, "nx": nx, "cellsize_arcsec": round(cell, 4), "grid": [plan.nv, plan.nu],
+                      "tile": args.tile, "waves": args.waves, "chunk": args.chunk},
+           "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 2),
+                          "p99": round(float(np.percentile(lat, 99)) * 1e3, 2)},
+           "roofline": roof, "kernels": kernels,
+           "run_info": {"sumwt": info["sumwt"], "maxpsf": info["maxpsf"], "nitems": info["nitems"],
+                        "ndropped": info["ndropped"], "nbad": info["nbad"]}}
+
+    if not args.no_check or not args.no_cpu:
+        vis, wt, u, v, w = plan.get_vis()
+        if not args.no_check:
+            out["accuracy_vs_fp64"] = accuracy(plan, vis, wt, u, v, w, nx, cell)
+        if not args.no_cpu:
+            cb = cpu_baseline(vis, wt, u, v, w, nx, cell, nvis)
+            out["cpu_baseline"] = cb
+            out["speedup_vs_cpu"] = round(value / cb["value"], 1)
+    plan.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -173,6 +173,8 @@ int sde_plan_info(sde_plan *plan, sde_run_info *info);
 /* Geometry: nu, nv, fnu, uorigin, vorigin and the float uscale/vscale. */
 int sde_plan_geometry(sde_plan *plan, int *nu, int *nv, int *fnu, int *uorigin,
                       int *vorigin, float *uscale, float *vscale);
+/* Copy the resident visibility arrays back to host memory (any may be NULL). */
+int sde_plan_get_vis(sde_plan *plan, float *vis, float *wt, float *u, float *v, float *w);
 /* Device pointers of the resident visibility arrays (for synthetic input). */
 int sde_plan_device_vis(sde_plan *plan, float **vis, float **wt, float **u, float **v,
                         float **w);
@@ -182,6 +184,9 @@ int sde_plan_device_vis(sde_plan *plan, float **vis, float **wt, float **u, floa
 int sde_plan_profile(sde_plan *plan, int enable);
 int sde_plan_timings(sde_plan *plan, int max, char (*names)[32], double *ms, int *count);
 int sde_plan_reset_timings(sde_plan *plan);
+/* Bandwidth probe: reps launches of a plain device copy of nbytes on the
+ * plan's stream, timed as "copy_probe" (calibrates HBM counters). */
+int sde_plan_copy_probe(sde_plan *plan, int64_t nbytes, int reps);
 
 /* Synthetic track: uvw of baselines between antennas (ENU metres) for hour
  * angles ha0 + k*dha (k < ntime, radians), declination dec, latitude lat,

@@ -1254,6 +1254,20 @@ int sde_plan_info(sde_plan *p, sde_run_info *info) {
     return 0;
 }
 
+int sde_plan_get_vis(sde_plan *p, float *vis, float *wt, float *u, float *v, float *w) {
+    if (!p) return -1;
+    const size_t n = (size_t)p->nvis;
+    if (n > 0) {
+        if (vis) SDE_CHECK(hipMemcpyAsync(vis, p->d_vis, sizeof(float2) * n, hipMemcpyDeviceToHost, p->stream));
+        if (wt) SDE_CHECK(hipMemcpyAsync(wt, p->d_wt, sizeof(float) * n, hipMemcpyDeviceToHost, p->stream));
+        if (u) SDE_CHECK(hipMemcpyAsync(u, p->d_u, sizeof(float) * n, hipMemcpyDeviceToHost, p->stream));
+        if (v) SDE_CHECK(hipMemcpyAsync(v, p->d_v, sizeof(float) * n, hipMemcpyDeviceToHost, p->stream));
+        if (w) SDE_CHECK(hipMemcpyAsync(w, p->d_w, sizeof(float) * n, hipMemcpyDeviceToHost, p->stream));
+    }
+    SDE_CHECK(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
 int sde_plan_geometry(sde_plan *p, int *nu, int *nv, int *fnu, int *uorigin, int *vorigin, float *uscale,
                       float *vscale) {
     if (!p) return -1;
@@ -1296,6 +1310,25 @@ int sde_plan_timings(sde_plan *p, int max, char (*names)[32], double *ms, int *c
         k++;
     }
     return k;
+}
+
+int sde_plan_copy_probe(sde_plan *p, int64_t nbytes, int reps) {
+    if (!p || nbytes <= 0 || reps < 1) return -1;
+    void *a = nullptr, *b = nullptr;
+    SDE_CHECK(hipMalloc(&a, (size_t)nbytes));
+    SDE_CHECK(hipMalloc(&b, (size_t)nbytes));
+    SDE_CHECK(hipMemsetAsync(a, 0, (size_t)nbytes, p->stream));
+    for (int r = 0; r < reps; r++) {
+        plan_mark_begin(p, "copy_probe");
+        launch_copy(p->stream, static_cast<const float4 *>(a), static_cast<float4 *>(b), (size_t)nbytes / 16);
+        plan_mark_end(p);
+    }
+    SDE_CHECK(hipGetLastError());
+    SDE_CHECK(hipStreamSynchronize(p->stream));
+    plan_collect(p);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return 0;
 }
 
 int sde_plan_reset_timings(sde_plan *p) {

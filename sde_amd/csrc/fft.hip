@@ -274,6 +274,11 @@ __global__ void k_pixcor(const float *__restrict__ in, float *__restrict__ out, 
     }
 }
 
+__global__ void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 int next_pow2_host(int n) {
     int p = 1;
     while (p < n) p *= 2;
@@ -340,6 +345,12 @@ void launch_scale_by_peak(hipStream_t s, float *image, size_t n, const int *peak
     size_t nb = (n + 255) / 256;
     if (nb > 65536) nb = 65536;
     hipLaunchKernelGGL(k_scale_by_peak, dim3((unsigned)nb), dim3(256), 0, s, image, n, peak_key);
+}
+
+void launch_copy(hipStream_t s, const float4 *src, float4 *dst, size_t n) {
+    size_t nb = (n + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)nb), dim3(256), 0, s, src, dst, n);
 }
 
 void launch_pixcor(hipStream_t s, const float *in, float *out, int nx, int ny, int cplx,

@@ -118,6 +118,9 @@ void launch_wt_apply(hipStream_t s, const float *wt, const float *u, const float
                      float *newwt, unsigned long long *ndrop);
 void launch_gwt_out(hipStream_t s, const double *gwt, float *out, size_t n);
 
+// ---- plain copy of n float4 (bandwidth probe, fft.hip) ----
+void launch_copy(hipStream_t s, const float4 *src, float4 *dst, size_t n);
+
 // ---- synthetic data (degrid.hip) ----
 void launch_synth(hipStream_t s, const double *bxyz, int nbl, double dec, double ha0,
                   double dha, int ntime, int nchan, const double *freq, int nsrc,

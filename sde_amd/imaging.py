@@ -182,6 +182,18 @@ class Plan:
         self._ok(lib().sde_plan_get_model(self._p, v.ctypes.data_as(fp), w.ctypes.data_as(fp)), "get_model")
         return v, w
 
+    def get_vis(self):
+        n = self.nvis
+        vis = np.zeros(n, np.complex64)
+        arrs = [np.zeros(n, np.float32) for _ in range(4)]
+        fp = C.POINTER(C.c_float)
+        self._ok(lib().sde_plan_get_vis(self._p, vis.ctypes.data_as(fp), *[a.ctypes.data_as(fp) for a in arrs]),
+                 "get_vis")
+        return (vis,) + tuple(arrs)
+
+    def copy_probe(self, nbytes, reps=5):
+        self._ok(lib().sde_plan_copy_probe(self._p, C.c_int64(int(nbytes)), int(reps)), "copy_probe")
+
     def device_vis(self):
         ptrs = [C.POINTER(C.c_float)() for _ in range(5)]
         self._ok(lib().sde_plan_device_vis(self._p, *[C.byref(x) for x in ptrs]), "device_vis")
