@@ -345,58 +345,46 @@ TileParams make_tp(int nu, int nv, int T, int chunk) {
     return tp;
 }
 
-int default_tile(int nu, int nv) { (void)nu; (void)nv; return 16; }
-int default_waves(int T) { return T == 16 ? 4 : 1; }
-int default_chunk(int64_t nvis) { return nvis > (1 << 22) ? 4096 : 1024; }
+int default_tile(int nu, int nv) { return (int64_t)nu * nv >= (1 << 20) ? 32 : 16; }
+int default_chunk(int64_t nvis) { return nvis > (1 << 22) ? 8192 : 2048; }
 
-// Buffers of one gridding pass (tile sort + LDS accumulation + reduce).
-struct GridBufs {
-    int *tile_of, *tile_count, *tile_start, *item_start, *tile_cursor, *nitems;
-    GridRec *rec, *sorted;
-    WorkItem *items;
-    float *partial;
-    double *blk, *sums;
-    int max_items, nblk;
-};
+int max_items_for(const TileParams &tp, int64_t nvis) {
+    return tp.ntiles + (int)((nvis + tp.chunk - 1) / tp.chunk) + 1;
+}
 
-// Runs the gridding pipeline on the stream; cf tables and arrays on device.
-int run_gridding(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
-                 const float *w, int64_t nvis, const CoordParams &cp, const TileParams &tp, int waves,
-                 int chmask, const float *cfu, const float *cfv, GridBufs &b, float2 *dirty, float2 *psf,
-                 const std::function<void(const char *)> &mark_begin,
-                 const std::function<void()> &mark_end) {
-    SDE_CHECK(hipMemsetAsync(b.tile_count, 0, sizeof(int) * tp.ntiles, s));
-    mark_begin("grid_prep");
-    launch_grid_prep(s, vis, wt, u, v, w, nvis, cp, tp, chmask == 2, cfu, cfv, b.tile_of, b.rec, b.tile_count,
-                     b.blk, b.nblk);
-    mark_end();
-    mark_begin("grid_scan");
-    launch_grid_scan(s, b.tile_count, tp.ntiles, tp.chunk, b.tile_start, b.item_start, b.tile_cursor, b.nitems,
-                     b.blk, b.nblk, cp, b.sums);
-    mark_end();
-    mark_begin("grid_scatter");
-    launch_grid_scatter(s, b.tile_of, b.rec, nvis, b.tile_start, b.tile_cursor, b.sorted);
-    mark_end();
-    mark_begin("grid_items");
-    launch_grid_items(s, b.tile_start, b.item_start, tp.ntiles, tp.chunk, b.items);
-    mark_end();
-    mark_begin("grid_tiles");
-    if (launch_grid_tiles(s, b.sorted, b.items, b.nitems, b.max_items, tp, chmask, waves, cfu, cfv, cp,
-                          b.partial)) {
-        mark_end();
-        set_plan_error("unsupported gridding tile configuration T=%d waves=%d", tp.T, waves);
-        return -1;
-    }
-    mark_end();
-    mark_begin("grid_reduce");
-    launch_grid_reduce(s, b.partial, b.item_start, tp, chmask, cp, b.sums, dirty, psf);
-    mark_end();
+using MarkB = std::function<void(const char *)>;
+using MarkE = std::function<void()>;
+
+// Cell sort of the visibilities (counting sort by tile-major cell).
+int run_sort(hipStream_t s, const float *wt, const float *u, const float *v, const float *w, int64_t nvis,
+             const CoordParams &cp, const TileParams &tp, SortBufs &sb, const MarkB &mb, const MarkE &me) {
+    mb("cell_sort");
+    launch_sort(s, wt, u, v, w, nvis, cp, tp, sb);
+    me();
     SDE_CHECK(hipGetLastError());
     return 0;
 }
 
-int max_items_for(const TileParams &tp, int64_t nvis) {
-    return tp.ntiles + (int)((nvis + tp.chunk - 1) / tp.chunk) + 1;
+// Gridding on a cell sort: records, LDS tiles, reduce (fold, shift, normalise).
+int run_grid(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v, const float *w,
+             int64_t nvis, const CoordParams &cp, const TileParams &tp, int chmask, const float *cfu,
+             const float *cfv, const SortBufs &sb, GridRec *rec, float *partial, double *blk, int nblk,
+             double *sums, float2 *dirty, float2 *psf, const MarkB &mb, const MarkE &me) {
+    mb("grid_fill");
+    launch_grid_fill(s, vis, wt, u, v, w, nvis, cp, tp, chmask == 2, cfu, cfv, sb, rec, blk, nblk, sums);
+    me();
+    mb("grid_tiles");
+    if (launch_grid_tiles(s, rec, sb.items, sb.nitems, sb.max_items, tp, chmask, cfu, cfv, cp, partial)) {
+        me();
+        set_plan_error("unsupported gridding tile T=%d", tp.T);
+        return -1;
+    }
+    me();
+    mb("grid_reduce");
+    launch_grid_reduce(s, partial, sb.item_start, tp, chmask, cp, sums, dirty, psf);
+    me();
+    SDE_CHECK(hipGetLastError());
+    return 0;
 }
 
 }  // namespace
@@ -499,30 +487,31 @@ void grdh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, flo
                                    *suppu, *suppv, *osampu, *osampv);
     const TileParams tp = make_tp(*nu, *nv, default_tile(*nu, *nv), default_chunk(n));
     const int chmask = *psf ? 2 : 1;
-    GridBufs b;
-    b.max_items = max_items_for(tp, n);
-    b.nblk = grid_prep_blocks(n);
+    const int64_t nkeys = sort_nkeys(tp);
+    const int nblk = grid_prep_blocks(n);
     const size_t nn = (size_t)(n > 0 ? n : 1);
+    SortBufs sb;
+    sb.max_items = max_items_for(tp, n);
     float2 *dvis = ctx_buf<float2>(c, "vis", nn);
     float *dwt = ctx_buf<float>(c, "wt", nn), *du = ctx_buf<float>(c, "u", nn);
     float *dv = ctx_buf<float>(c, "v", nn), *dw = ctx_buf<float>(c, "w", nn);
     float *dcfu = ctx_buf<float>(c, "cfu", ncfu), *dcfv = ctx_buf<float>(c, "cfv", ncfv);
-    b.tile_of = ctx_buf<int>(c, "tile_of", nn);
-    b.rec = ctx_buf<GridRec>(c, "rec", nn);
-    b.sorted = ctx_buf<GridRec>(c, "sorted", nn);
-    b.tile_count = ctx_buf<int>(c, "tile_count", tp.ntiles + 1);
-    b.tile_start = ctx_buf<int>(c, "tile_start", tp.ntiles + 1);
-    b.item_start = ctx_buf<int>(c, "item_start", tp.ntiles + 1);
-    b.tile_cursor = ctx_buf<int>(c, "tile_cursor", tp.ntiles + 1);
-    b.nitems = ctx_buf<int>(c, "nitems", 1);
-    b.items = ctx_buf<WorkItem>(c, "items", b.max_items);
-    b.partial = ctx_buf<float>(c, "partial", grid_partial_floats(tp, chmask, b.max_items));
-    b.blk = ctx_buf<double>(c, "blk", 2 * b.nblk);
-    b.sums = ctx_buf<double>(c, "sums", 4);
+    sb.cnt = ctx_buf<int>(c, "cnt", nkeys);
+    sb.start = ctx_buf<int>(c, "start", nkeys);
+    sb.bsum = ctx_buf<int>(c, "bsum", sort_nblocks(tp));
+    sb.key_of = ctx_buf<int>(c, "key_of", nn);
+    sb.rank_of = ctx_buf<int>(c, "rank_of", nn);
+    sb.tile_start = ctx_buf<int>(c, "tile_start", tp.ntiles + 1);
+    sb.item_start = ctx_buf<int>(c, "item_start", tp.ntiles + 1);
+    sb.nitems = ctx_buf<int>(c, "nitems", 1);
+    sb.items = ctx_buf<WorkItem>(c, "items", sb.max_items);
+    GridRec *rec = ctx_buf<GridRec>(c, "rec", nn);
+    float *partial = ctx_buf<float>(c, "partial", grid_partial_floats(tp, chmask, sb.max_items));
+    double *blk = ctx_buf<double>(c, "blk", 2 * nblk), *dsums = ctx_buf<double>(c, "sums", 4);
     float2 *dg = ctx_buf<float2>(c, "grid", (size_t)*nu * *nv);
-    if (!dvis || !dwt || !du || !dv || !dw || !dcfu || !dcfv || !b.tile_of || !b.rec || !b.sorted || !b.tile_count ||
-        !b.tile_start || !b.item_start || !b.tile_cursor || !b.nitems || !b.items || !b.partial || !b.blk ||
-        !b.sums || !dg) {
+    if (!dvis || !dwt || !du || !dv || !dw || !dcfu || !dcfv || !sb.cnt || !sb.start || !sb.bsum || !sb.key_of ||
+        !sb.rank_of || !sb.tile_start || !sb.item_start || !sb.nitems || !sb.items || !rec || !partial || !blk ||
+        !dsums || !dg) {
         report_error(kErrNoMem, R, "device allocation failed");
         return;
     }
@@ -544,14 +533,17 @@ void grdh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, flo
     }
     SDE_F(hipMemcpyAsync(dcfu, gridfnu, sizeof(float) * ncfu, hipMemcpyHostToDevice, s));
     SDE_F(hipMemcpyAsync(dcfv, gridfnv, sizeof(float) * ncfv, hipMemcpyHostToDevice, s));
-    if (run_gridding(s, dvis, dwt, du, dv, dw, n, cp, tp, default_waves(tp.T), chmask, dcfu, dcfv, b,
-                     chmask == 1 ? dg : nullptr, chmask == 2 ? dg : nullptr, noop_begin, noop_end)) {
+    auto nb = [](const char *) {};
+    auto ne = []() {};
+    if (run_sort(s, dwt, du, dv, dw, n, cp, tp, sb, nb, ne) ||
+        run_grid(s, dvis, dwt, du, dv, dw, n, cp, tp, chmask, dcfu, dcfv, sb, rec, partial, blk, nblk, dsums,
+                 chmask == 1 ? dg : nullptr, chmask == 2 ? dg : nullptr, nb, ne)) {
         report_error(kErrFatal, R, sde_last_error());
         return;
     }
     double sums[3];
     SDE_F(hipMemcpyAsync(gvis, dg, sizeof(float2) * (size_t)*nu * *nv, hipMemcpyDeviceToHost, s));
-    SDE_F(hipMemcpyAsync(sums, b.sums, sizeof sums, hipMemcpyDeviceToHost, s));
+    SDE_F(hipMemcpyAsync(sums, dsums, sizeof sums, hipMemcpyDeviceToHost, s));
     SDE_F(hipStreamSynchronize(s));
     *sumwt = (float)sums[0];
     if (sums[1] == 0.0) report_error(kErrBdArg, R, "No weight!");
@@ -867,7 +859,7 @@ struct sde_plan {
     float udelt = 0, vdelt = 0, uscale = 0, vscale = 0;
     CoordParams cp, wcp;
     TileParams tp;
-    int waves = 1, wmode = 0;
+    int wmode = 0;
     std::map<int, float2 *> tw;
     float *d_cf = nullptr, *d_ccf = nullptr;
     int64_t nvis = 0, cap = 0;
@@ -875,8 +867,14 @@ struct sde_plan {
     float *d_wt = nullptr, *d_u = nullptr, *d_v = nullptr, *d_w = nullptr, *d_wteff = nullptr;
     float2 *d_mvis = nullptr;
     float *d_mwt = nullptr;
-    GridBufs gb{};
+    SortBufs sb{};
+    bool sorted = false;  // the cell sort matches the resident visibilities
+    GridRec *d_rec = nullptr;
+    DegridRec *d_drec = nullptr;
+    int *d_list = nullptr, *d_nlist = nullptr;
+    float *d_partial = nullptr;
     size_t partial_cap = 0, items_cap = 0;
+    double *d_blk = nullptr, *d_sums = nullptr;
     float2 *d_grids = nullptr, *d_mgrid = nullptr;
     float *d_dirty = nullptr, *d_psf = nullptr, *d_img_in = nullptr;
     int *d_peak = nullptr;
@@ -888,8 +886,6 @@ struct sde_plan {
     std::vector<hipEvent_t> pool;
     std::vector<std::string> order;
     std::map<std::string, std::pair<double, int>> acc;
-    const char *cur = nullptr;
-    sde_run_info info{};
 };
 
 namespace {
@@ -944,12 +940,13 @@ int plan_alloc(T **ptr, size_t count) {
 }
 
 int plan_reserve_vis(sde_plan *p, int64_t n) {
+    p->sorted = false;
     if (n <= p->cap) return 0;
     const size_t c = (size_t)n;
     if (plan_alloc(&p->d_vis, c) || plan_alloc(&p->d_wt, c) || plan_alloc(&p->d_u, c) || plan_alloc(&p->d_v, c) ||
         plan_alloc(&p->d_w, c) || plan_alloc(&p->d_wteff, c) || plan_alloc(&p->d_mvis, c) ||
-        plan_alloc(&p->d_mwt, c) || plan_alloc(&p->gb.tile_of, c) || plan_alloc(&p->gb.rec, c) ||
-        plan_alloc(&p->gb.sorted, c))
+        plan_alloc(&p->d_mwt, c) || plan_alloc(&p->sb.key_of, c) || plan_alloc(&p->sb.rank_of, c) ||
+        plan_alloc(&p->d_rec, c) || plan_alloc(&p->d_drec, c) || plan_alloc(&p->d_list, c))
         return -1;
     p->cap = n;
     return 0;
@@ -958,15 +955,23 @@ int plan_reserve_vis(sde_plan *p, int64_t n) {
 int plan_reserve_items(sde_plan *p) {
     const int mi = max_items_for(p->tp, p->nvis);
     if ((size_t)mi > p->items_cap) {
-        if (plan_alloc(&p->gb.items, (size_t)mi)) return -1;
+        if (plan_alloc(&p->sb.items, (size_t)mi)) return -1;
         p->items_cap = mi;
     }
     const size_t pf = grid_partial_floats(p->tp, 3, mi);
     if (pf > p->partial_cap) {
-        if (plan_alloc(&p->gb.partial, pf)) return -1;
+        if (plan_alloc(&p->d_partial, pf)) return -1;
         p->partial_cap = pf;
     }
-    p->gb.max_items = mi;
+    p->sb.max_items = mi;
+    return 0;
+}
+
+int plan_sort(sde_plan *p) {
+    auto mb = [p](const char *nm) { plan_mark_begin(p, nm); };
+    auto me = [p]() { plan_mark_end(p); };
+    if (run_sort(p->stream, p->d_wt, p->d_u, p->d_v, p->d_w, p->nvis, p->cp, p->tp, p->sb, mb, me)) return -1;
+    p->sorted = true;
     return 0;
 }
 
@@ -1020,13 +1025,12 @@ int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **out) {
         return -1;
     }
     const int T = cfg->tile ? cfg->tile : default_tile(p->nu, p->nv);
-    p->waves = cfg->waves ? cfg->waves : default_waves(T);
-    if (!((T == 16 && (p->waves == 1 || p->waves == 2 || p->waves == 4)) || (T == 32 && p->waves == 1))) {
-        set_plan_error("sde_plan_create: unsupported tile %d / waves %d", T, p->waves);
+    if (T != 16 && T != 32) {
+        set_plan_error("sde_plan_create: unsupported tile %d (16 or 32)", T);
         delete p;
         return -1;
     }
-    p->tp = make_tp(p->nu, p->nv, T, cfg->chunk > 0 ? cfg->chunk : 1024);
+    p->tp = make_tp(p->nu, p->nv, T, cfg->chunk > 0 ? cfg->chunk : 2048);
     if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
         set_plan_error("sde_plan_create: stream creation failed");
         delete p;
@@ -1035,14 +1039,16 @@ int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **out) {
     std::vector<float> ccf(p->ccflen);
     make_ccf(cft, ccf.data(), p->ccflen);
     const size_t ng = (size_t)p->nu * p->nv, npix = (size_t)cfg->nx * cfg->ny;
-    p->gb.nblk = 0;
+    const int64_t nkeys = sort_nkeys(p->tp);
     int rc = plan_alloc(&p->d_cf, p->ncf) || plan_alloc(&p->d_ccf, p->ccflen) || plan_alloc(&p->d_grids, 2 * ng) ||
              plan_alloc(&p->d_mgrid, ng) || plan_alloc(&p->d_dirty, npix) || plan_alloc(&p->d_psf, npix) ||
              plan_alloc(&p->d_img_in, npix) || plan_alloc(&p->d_peak, 4) || plan_alloc(&p->d_gwt, ng) ||
              plan_alloc(&p->d_wblk, 4096) || plan_alloc(&p->d_wsums, 4) || plan_alloc(&p->d_cnt, 4) ||
-             plan_alloc(&p->gb.tile_count, p->tp.ntiles + 1) || plan_alloc(&p->gb.tile_start, p->tp.ntiles + 1) ||
-             plan_alloc(&p->gb.item_start, p->tp.ntiles + 1) || plan_alloc(&p->gb.tile_cursor, p->tp.ntiles + 1) ||
-             plan_alloc(&p->gb.nitems, 1) || plan_alloc(&p->gb.blk, 2 * 4096) || plan_alloc(&p->gb.sums, 4);
+             plan_alloc(&p->sb.cnt, (size_t)nkeys) || plan_alloc(&p->sb.start, (size_t)nkeys) ||
+             plan_alloc(&p->sb.bsum, (size_t)sort_nblocks(p->tp)) ||
+             plan_alloc(&p->sb.tile_start, p->tp.ntiles + 1) || plan_alloc(&p->sb.item_start, p->tp.ntiles + 1) ||
+             plan_alloc(&p->sb.nitems, 1) || plan_alloc(&p->d_nlist, 1) || plan_alloc(&p->d_blk, 2 * 4096) ||
+             plan_alloc(&p->d_sums, 4);
     if (rc) {
         sde_plan_destroy(p);
         return -1;
@@ -1070,10 +1076,10 @@ int sde_plan_destroy(sde_plan *p) {
     plan_collect(p);
     for (auto e : p->pool) (void)hipEventDestroy(e);
     void *bufs[] = {p->d_cf, p->d_ccf, p->d_vis, p->d_wt, p->d_u, p->d_v, p->d_w, p->d_wteff, p->d_mvis, p->d_mwt,
-                    p->gb.tile_of, p->gb.rec, p->gb.sorted, p->gb.items, p->gb.partial, p->gb.tile_count,
-                    p->gb.tile_start, p->gb.item_start, p->gb.tile_cursor, p->gb.nitems, p->gb.blk, p->gb.sums,
-                    p->d_grids, p->d_mgrid, p->d_dirty, p->d_psf, p->d_img_in, p->d_peak, p->d_gwt, p->d_wblk,
-                    p->d_wsums, p->d_cnt};
+                    p->sb.cnt, p->sb.start, p->sb.bsum, p->sb.key_of, p->sb.rank_of, p->sb.tile_start,
+                    p->sb.item_start, p->sb.nitems, p->sb.items, p->d_rec, p->d_drec, p->d_list, p->d_nlist,
+                    p->d_partial, p->d_blk, p->d_sums, p->d_grids, p->d_mgrid, p->d_dirty, p->d_psf, p->d_img_in,
+                    p->d_peak, p->d_gwt, p->d_wblk, p->d_wsums, p->d_cnt};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &kv : p->tw) (void)hipFree(kv.second);
@@ -1085,6 +1091,10 @@ int sde_plan_destroy(sde_plan *p) {
 int sde_plan_set_vis(sde_plan *p, int64_t nvis, const float *vis, const float *wt, const float *u, const float *v,
                      const float *w, int on_device) {
     if (!p || nvis < 0) return -1;
+    if (nvis > 0x7fffffffLL) {
+        set_plan_error("sde_plan_set_vis: at most 2^31-1 visibilities per plan");
+        return -1;
+    }
     if (plan_reserve_vis(p, nvis)) return -1;
     p->nvis = nvis;
     const hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
@@ -1131,13 +1141,14 @@ int sde_plan_uvmap(sde_plan *p) {
         plan_mark_end(p);
         wt_use = p->d_wteff;
     }
-    p->gb.nblk = grid_prep_blocks(n);
+    // every imaging job sorts its visibilities afresh (the sort is part of the job)
+    if (plan_sort(p)) return -1;
     const size_t ng = (size_t)p->nu * p->nv;
     float2 *gd = p->d_grids, *gp = p->d_grids + ng;
     auto mb = [p](const char *nm) { plan_mark_begin(p, nm); };
     auto me = [p]() { plan_mark_end(p); };
-    if (run_gridding(s, p->d_vis, wt_use, p->d_u, p->d_v, p->d_w, n, p->cp, p->tp, p->waves, 3, p->d_cf, p->d_cf,
-                     p->gb, gd, gp, mb, me))
+    if (run_grid(s, p->d_vis, wt_use, p->d_u, p->d_v, p->d_w, n, p->cp, p->tp, 3, p->d_cf, p->d_cf, p->sb,
+                 p->d_rec, p->d_partial, p->d_blk, grid_prep_blocks(n), p->d_sums, gd, gp, mb, me))
         return -1;
     float2 *twnv = p->tw[p->nv], *twfnu = p->tw[p->fnu];
     plan_mark_begin(p, "fft_cols");
@@ -1176,10 +1187,25 @@ int sde_plan_vismodel(sde_plan *p, const float *image) {
     plan_mark_begin(p, "fft_cols_model");
     launch_fft_cols(s, p->d_mgrid, p->nu, p->nv, 1, 0, +1, 1.0f, twnv);
     plan_mark_end(p);
+    if (p->nvis <= 0) return 0;
+    if (!p->sorted && plan_sort(p)) return -1;
     SDE_CHECK(hipMemsetAsync(p->d_cnt + 1, 0, sizeof(unsigned long long), s));
+    SDE_CHECK(hipMemsetAsync(p->d_nlist, 0, sizeof(int), s));
+    plan_mark_begin(p, "degrid_fill");
+    launch_degrid_fill(s, p->d_u, p->d_v, p->d_w, p->d_wt, p->d_vis, p->nvis, p->cp, p->tp, p->sb, p->d_drec,
+                       p->d_list, p->d_nlist, p->d_mvis, p->d_mwt);
+    plan_mark_end(p);
     plan_mark_begin(p, "degrid");
+    if (launch_degrid_tiles(s, p->d_drec, p->sb.items, p->sb.nitems, p->sb.max_items, p->tp, p->d_mgrid, p->d_cf,
+                            p->d_cf, p->cp, p->d_mvis, p->d_mwt, p->d_cnt + 1)) {
+        plan_mark_end(p);
+        set_plan_error("unsupported degridding tile T=%d", p->tp.T);
+        return -1;
+    }
+    plan_mark_end(p);
+    plan_mark_begin(p, "degrid_axis");
     launch_degrid(s, p->d_mgrid, p->d_u, p->d_v, p->d_w, p->d_wt, p->d_vis, p->nvis, p->cp, p->d_cf, p->d_cf,
-                  p->d_mvis, p->d_mwt, p->d_cnt + 1);
+                  p->d_mvis, p->d_mwt, p->d_cnt + 1, p->d_list, p->d_nlist);
     plan_mark_end(p);
     SDE_CHECK(hipGetLastError());
     return 0;
@@ -1235,9 +1261,9 @@ int sde_plan_info(sde_plan *p, sde_run_info *info) {
     double sums[3] = {0, 0, 0};
     int key = 0, nitems = 0;
     unsigned long long cnt[2] = {0, 0};
-    SDE_CHECK(hipMemcpyAsync(sums, p->gb.sums, sizeof sums, hipMemcpyDeviceToHost, p->stream));
+    SDE_CHECK(hipMemcpyAsync(sums, p->d_sums, sizeof sums, hipMemcpyDeviceToHost, p->stream));
     SDE_CHECK(hipMemcpyAsync(&key, p->d_peak, sizeof key, hipMemcpyDeviceToHost, p->stream));
-    SDE_CHECK(hipMemcpyAsync(&nitems, p->gb.nitems, sizeof nitems, hipMemcpyDeviceToHost, p->stream));
+    SDE_CHECK(hipMemcpyAsync(&nitems, p->sb.nitems, sizeof nitems, hipMemcpyDeviceToHost, p->stream));
     SDE_CHECK(hipMemcpyAsync(cnt, p->d_cnt, sizeof cnt, hipMemcpyDeviceToHost, p->stream));
     SDE_CHECK(hipStreamSynchronize(p->stream));
     int bits = key >= 0 ? key : key ^ 0x7fffffff;
@@ -1374,6 +1400,10 @@ int sde_plan_synth(sde_plan *p, int nant, const double *enu, double lat, double 
     SDE_CHECK(hipMemcpyAsync(d_b, bxyz.data(), sizeof(double) * bxyz.size(), hipMemcpyHostToDevice, p->stream));
     SDE_CHECK(hipMemcpyAsync(d_f, freq, sizeof(double) * nchan, hipMemcpyHostToDevice, p->stream));
     SDE_CHECK(hipMemcpyAsync(d_s, src.data(), sizeof(double) * src.size(), hipMemcpyHostToDevice, p->stream));
+    if (n > 0x7fffffffLL) {
+        set_plan_error("sde_plan_synth: at most 2^31-1 visibilities per plan");
+        return -1;
+    }
     launch_synth(p->stream, d_b, nbl, dec, ha0, dha, ntime, nchan, d_f, nsrc, d_s, sigma, seed, nfull, n, p->d_vis,
                  p->d_wt, p->d_u, p->d_v, p->d_w);
     SDE_CHECK(hipGetLastError());

@@ -17,9 +17,15 @@ __global__ __launch_bounds__(256) void k_degrid(
     const float2 *__restrict__ grid, const float *__restrict__ u, const float *__restrict__ v,
     const float *__restrict__ w, const float *__restrict__ wt_in, const float2 *vis_in,
     int64_t nvis, CoordParams cp, const float *__restrict__ cfu, const float *__restrict__ cfv,
-    float2 *vis_out, float *__restrict__ wt_out, unsigned long long *__restrict__ nbad) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nvis) return;
+    float2 *vis_out, float *__restrict__ wt_out, unsigned long long *__restrict__ nbad,
+    const int *__restrict__ list, const int *__restrict__ nlist) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (list) {
+        if (i >= *nlist) return;
+        i = list[i];
+    } else if (i >= nvis) {
+        return;
+    }
     float wti = wt_in[i];
     if (wti <= 0.0f) {
         wt_out[i] = wti;
@@ -106,6 +112,153 @@ __global__ __launch_bounds__(256) void k_degrid(
     if ((double)cp.uscale * ul > 0.0) vi = -vi;
     vis_out[i] = make_float2(vr * rr - vi * ri, vr * ri + vi * rr);
     wt_out[i] = wti;
+}
+
+constexpr int kScanShiftD = 13;  // must match the block-local scan of the cell sort
+constexpr int kHaloD = 4;       // LDS margin: support (<= 3) + 1 cell of REAL/DOUBLE rounding slack
+
+// Records of the tile path (one thread per visibility, original order).
+// A visibility takes the tile path when it is in the gridding sort, is not
+// within SUPPU cells of the u = 0 axis (those need the reflected reads) and
+// its footprint lies inside its tile's LDS window.
+__global__ __launch_bounds__(256) void k_degrid_fill(
+    const float *__restrict__ u, const float *__restrict__ v, const float *__restrict__ w,
+    const float *__restrict__ wt_in, const float2 *vis_in, int64_t nvis, CoordParams cp, TileParams tp,
+    const int *__restrict__ key_of, const int *__restrict__ rank_of, const int *__restrict__ start,
+    const int *__restrict__ bsum, DegridRec *__restrict__ drec, int *__restrict__ list,
+    int *__restrict__ nlist, float2 *vis_out, float *__restrict__ wt_out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool to_list = false;
+    if (i < nvis) {
+        const float wti = wt_in[i];
+        const int key = key_of[i];
+        int pos = -1;
+        if (key >= 0) pos = start[key] + bsum[key >> kScanShiftD] + rank_of[i];
+        DegridRec r;
+        r.idx = 0xffffffffu;
+        r.pk = 0;
+        r.rr = 0.0f;
+        r.ri = 0.0f;
+        if (wti <= 0.0f) {
+            wt_out[i] = wti;
+            if (vis_in != vis_out) vis_out[i] = vis_in[i];
+        } else {
+            const double ug = u[i], vg = v[i], wg = w[i];
+            const double ul = cp.shift[0] * ug + cp.shift[1] * vg + cp.shift[2] * wg;
+            const double vl = cp.shift[3] * ug + cp.shift[4] * vg + cp.shift[5] * wg;
+            const double wl = cp.shift[6] * ug + cp.shift[7] * vg + cp.shift[8] * wg;
+            float ucell = (float)((double)cp.uscale * ul + (double)cp.uoffset);
+            float vcell = (float)((double)cp.vscale * vl + (double)cp.voffset);
+            if (ucell > 0.0f) {
+                ucell = ucell + (float)cp.uorigin;
+                vcell = vcell + (float)cp.vorigin;
+            } else {
+                ucell = -ucell + (float)cp.uorigin;
+                vcell = -vcell + (float)cp.vorigin;
+            }
+            const int nuc = nint_f(ucell), nvc = nint_f(vcell);
+            if (nuc + cp.suppu > cp.nu || nvc + cp.suppv > cp.nv || nvc - cp.suppv < 1) {
+                vis_out[i] = make_float2(0.0f, 0.0f);
+                wt_out[i] = -wti;
+            } else if (key < 0 || nuc <= cp.suppu) {
+                to_list = true;
+            } else {
+                const int tt = tp.T * tp.T;
+                const int tile = key / tt;
+                const int tu = tile % tp.ntu, tv = tile / tp.ntu;
+                const int lu = (nuc - 1) - (tu * tp.T - cp.suppu - kHaloD);
+                const int lv = (nvc - 1) - (tv * tp.T - kHaloD);
+                const int pl = tp.T + 2 * kHaloD;
+                if (lu - cp.suppu < 0 || lu + cp.suppu >= pl || lv - cp.suppv < 0 || lv + cp.suppv >= pl) {
+                    to_list = true;
+                } else {
+                    const double phase = cp.twopi * (wl - wg);
+                    const int delui = nint_f((float)cp.osampu * ((float)nuc - ucell));
+                    const int delvi = nint_f((float)cp.osampv * ((float)nvc - vcell));
+                    const int poffu = (cp.osampu + 1) / 2, poffv = (cp.osampv + 1) / 2;
+                    const bool conj = (double)cp.uscale * ul > 0.0;
+                    r.idx = (uint32_t)i | (conj ? 0x80000000u : 0u);
+                    r.pk = (uint32_t)lu | ((uint32_t)lv << 8) | ((uint32_t)(delui + poffu) << 16) |
+                           ((uint32_t)(delvi + poffv) << 24);
+                    r.rr = (float)cos(phase);
+                    r.ri = (float)sin(phase);
+                    wt_out[i] = wti;  // restored on success (the tile kernel negates it otherwise)
+                }
+            }
+        }
+        if (pos >= 0) drec[pos] = r;
+    }
+    // one atomic per wave for the list
+    const unsigned long long m = __ballot(to_list);
+    if (m) {
+        const int lane = threadIdx.x & 63;
+        const int leader = __ffsll((long long)m) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(nlist, __popcll(m));
+        base = __shfl(base, leader);
+        if (to_list) list[base + __popcll(m & ((1ull << lane) - 1))] = (int)i;
+    }
+}
+
+template <int T>
+__global__ __launch_bounds__(256) void k_degrid_tiles(const DegridRec *__restrict__ drec,
+                                                      const WorkItem *__restrict__ items,
+                                                      const int *__restrict__ nitems, TileParams tp,
+                                                      const float2 *__restrict__ grid,
+                                                      const float *__restrict__ cfu_g,
+                                                      const float *__restrict__ cfv_g, CoordParams cp,
+                                                      float2 *__restrict__ vis_out, float *__restrict__ wt_out,
+                                                      unsigned long long *__restrict__ nbad) {
+    constexpr int PL = T + 2 * kHaloD;
+    __shared__ float2 g[PL * PL];
+    extern __shared__ float cfl[];
+    const int item = blockIdx.x;
+    if (item >= *nitems) return;
+    const WorkItem wi = items[item];
+    const int tu = wi.tile % tp.ntu, tv = wi.tile / tp.ntu;
+    const int col0 = tu * T - cp.suppu - kHaloD, row0 = tv * T - kHaloD;
+    for (int k = threadIdx.x; k < PL * PL; k += blockDim.x) {
+        const int r = k / PL, c = k - r * PL;
+        const int gc = col0 + c, gr = row0 + r;
+        g[k] = (gc >= 0 && gc < cp.nu && gr >= 0 && gr < cp.nv) ? grid[(size_t)gr * cp.nu + gc]
+                                                               : make_float2(0.0f, 0.0f);
+    }
+    const int poffu = (cp.osampu + 1) / 2, poffv = (cp.osampv + 1) / 2;
+    const int ncfu = cp.osampu * (2 * cp.suppu + 1) + 2;
+    const int ncfv = cp.osampv * (2 * cp.suppv + 1) + 2;
+    float *cflu = cfl, *cflv = cfl + ncfu;
+    for (int k = threadIdx.x; k < ncfu; k += blockDim.x) cflu[k] = cfu_g[cp.osampu - poffu + k];
+    for (int k = threadIdx.x; k < ncfv; k += blockDim.x) cflv[k] = cfv_g[cp.osampv - poffv + k];
+    __syncthreads();
+    for (int r = wi.begin + threadIdx.x; r < wi.end; r += blockDim.x) {
+        const DegridRec d = drec[r];
+        if (d.idx == 0xffffffffu) continue;
+        const int lu = d.pk & 0xff, lv = (d.pk >> 8) & 0xff;
+        const int pu = (d.pk >> 16) & 0xff, pv = d.pk >> 24;
+        float vr = 0.0f, vi = 0.0f, sumwt = 0.0f;
+        for (int offv = -cp.suppv; offv <= cp.suppv; offv++) {
+            const float cv = cflv[pv + cp.osampv * (offv + cp.suppv)];
+            const float2 *grow = g + (lv + offv) * PL + lu;
+            for (int offu = -cp.suppu; offu <= cp.suppu; offu++) {
+                const float uvwt = cflu[pu + cp.osampu * (offu + cp.suppu)] * cv;
+                const float2 gg = grow[offu];
+                vr = vr + gg.x * uvwt;
+                vi = vi + gg.y * uvwt;
+                sumwt = sumwt + uvwt;
+            }
+        }
+        const uint32_t idx = d.idx & 0x7fffffffu;
+        if (sumwt > 0.0f) {
+            vr = vr / sumwt;
+            vi = vi / sumwt;
+        } else {
+            atomicAdd(nbad, 1ull);
+            vr = vi = 0.0f;
+            wt_out[idx] = -wt_out[idx];
+        }
+        if (d.idx & 0x80000000u) vi = -vi;
+        vis_out[idx] = make_float2(vr * d.rr - vi * d.ri, vr * d.ri + vi * d.rr);
+    }
 }
 
 // Density-grid cell in REAL arithmetic (grd/grduwt2d.f:89-110).
@@ -297,11 +450,40 @@ __global__ __launch_bounds__(256) void k_synth(
 void launch_degrid(hipStream_t s, const float2 *grid, const float *u, const float *v,
                    const float *w, const float *wt_in, const float2 *vis_in, int64_t nvis,
                    const CoordParams &cp, const float *cfu, const float *cfv, float2 *vis_out,
-                   float *wt_out, unsigned long long *nbad) {
+                   float *wt_out, unsigned long long *nbad, const int *list, const int *nlist) {
     if (nvis <= 0) return;
     const int64_t nb = (nvis + 255) / 256;
     hipLaunchKernelGGL(k_degrid, dim3((unsigned)nb), dim3(256), 0, s, grid, u, v, w, wt_in, vis_in,
-                       nvis, cp, cfu, cfv, vis_out, wt_out, nbad);
+                       nvis, cp, cfu, cfv, vis_out, wt_out, nbad, list, nlist);
+}
+
+void launch_degrid_fill(hipStream_t s, const float *u, const float *v, const float *w, const float *wt_in,
+                        const float2 *vis_in, int64_t nvis, const CoordParams &cp, const TileParams &tp,
+                        const SortBufs &b, DegridRec *drec, int *list, int *nlist, float2 *vis_out,
+                        float *wt_out) {
+    if (nvis <= 0) return;
+    const int64_t nb = (nvis + 255) / 256;
+    hipLaunchKernelGGL(k_degrid_fill, dim3((unsigned)nb), dim3(256), 0, s, u, v, w, wt_in, vis_in, nvis, cp, tp,
+                       b.key_of, b.rank_of, b.start, b.bsum, drec, list, nlist, vis_out, wt_out);
+}
+
+int launch_degrid_tiles(hipStream_t s, const DegridRec *drec, const WorkItem *items, const int *nitems,
+                        int max_items, const TileParams &tp, const float2 *grid, const float *cfu,
+                        const float *cfv, const CoordParams &cp, float2 *vis_out, float *wt_out,
+                        unsigned long long *nbad) {
+    if (max_items <= 0) return 0;
+    const int ncfu = cp.osampu * (2 * cp.suppu + 1) + 2;
+    const int ncfv = cp.osampv * (2 * cp.suppv + 1) + 2;
+    const size_t lds = sizeof(float) * (ncfu + ncfv);
+    if (tp.T == 16)
+        hipLaunchKernelGGL(k_degrid_tiles<16>, dim3(max_items), dim3(256), lds, s, drec, items, nitems, tp, grid, cfu,
+                           cfv, cp, vis_out, wt_out, nbad);
+    else if (tp.T == 32)
+        hipLaunchKernelGGL(k_degrid_tiles<32>, dim3(max_items), dim3(256), lds, s, drec, items, nitems, tp, grid, cfu,
+                           cfv, cp, vis_out, wt_out, nbad);
+    else
+        return -1;
+    return 0;
 }
 
 int wt_blocks(int64_t nvis) {

@@ -2,42 +2,48 @@
 // GRDQH2D (grd/grdqh2d.f:70-147).
 //
 // Pipeline (one stream, no host synchronisation):
-//   1. k_grid_prep    one thread per visibility: the FP64 coordinate and
-//                     phase step exactly as the reference (cell, kernel
-//                     phase, drop test, conjugate/reflect), a 16-byte record,
-//                     the owning uv tile, FP64 partial sums of SUMWT/LSUMWT.
-//   2. k_grid_scan    one workgroup: tile offsets, work-item offsets (a
-//                     tile's list is cut into chunks), FP64 totals, the
-//                     normalisation (NU-1)*NV/LSUMWT.
-//   3. k_grid_scatter counting-sort scatter of the records by tile.
-//   4. k_grid_items   (tile, begin, end) work items.
-//   5. k_grid_tiles   one workgroup per work item; each wave owns a private
-//                     padded tile (T+6)^2 in LDS and walks its share of the
-//                     records; lane = kernel tap, so the 49 taps of one
-//                     visibility land on 49 distinct cells and no two lanes
-//                     collide; every tap is a Neumaier-compensated FP32 add
-//                     in LDS.  The waves' tiles are summed in FP64 and the
-//                     padded tile is flushed once.  No global atomics.
-//   6. k_grid_reduce  one thread per output cell: FP64 sum of the padded
+//   1. k_sort_count   one thread per visibility: the FP64 cell of GRDH2D
+//                     (the reference's coordinate arithmetic, its drop
+//                     test), key = tile-major cell index, rank = position
+//                     among the points of that cell (one atomic per point
+//                     on ~NU*NV counters: no hot spot).
+//   2. k_scan_*       exclusive scan of the cell counts (block-local scan +
+//                     scanned block totals), tile offsets and work items (a
+//                     tile's records cut into chunks).
+//   3. k_grid_fill    one thread per visibility: phase rotation, conjugate/
+//                     reflect, kernel phases, a 16-byte record written at its
+//                     sorted position; FP64 partial sums of SUMWT and LSUMWT.
+//   4. k_grid_tiles   one workgroup per work item over a padded tile
+//                     (T+6)^2 held in LDS; each wave walks a contiguous run
+//                     of cell-sorted records with lane = kernel tap, so the
+//                     taps of one point land on distinct cells.  Consecutive
+//                     points of one cell accumulate in registers (Kahan);
+//                     when the cell changes the compensated pair is flushed
+//                     into LDS with LDS atomics plus the exact TwoSum error
+//                     into a compensation plane.  The tile is flushed once.
+//                     No global atomics.
+//   5. k_grid_reduce  one thread per output cell: FP64 sum of the padded
 //                     tiles covering it, the Hermitian fold of the u = 0
 //                     column, the shift left by SUPPU and the normalisation.
 #include "internal.h"
 
 namespace {
 
-constexpr int kPrepThreads = 256;
-constexpr int kMaxPrepBlocks = 4096;
+constexpr int kThreads = 256;
+constexpr int kMaxBlocks = 4096;
+constexpr int kScanBlock = 8192;   // keys per block-local scan (1024 threads x 8)
+constexpr int kScanShift = 13;
 
 __device__ __forceinline__ int nint_d(double x) { return (int)round(x); }
 __device__ __forceinline__ int nint_f(float x) { return (int)roundf(x); }
 
 template <typename TT>
 __device__ __forceinline__ void block_sum2(TT &a, TT &b) {
-    __shared__ TT sa[kPrepThreads], sb[kPrepThreads];
+    __shared__ TT sa[kThreads], sb[kThreads];
     sa[threadIdx.x] = a;
     sb[threadIdx.x] = b;
     __syncthreads();
-    for (int s = kPrepThreads / 2; s > 0; s >>= 1) {
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
             sa[threadIdx.x] += sa[threadIdx.x + s];
             sb[threadIdx.x] += sb[threadIdx.x + s];
@@ -48,166 +54,168 @@ __device__ __forceinline__ void block_sum2(TT &a, TT &b) {
     b = sb[0];
 }
 
-__global__ __launch_bounds__(kPrepThreads) void k_grid_prep(
-    const float2 *__restrict__ vis, const float *__restrict__ wt, const float *__restrict__ u,
-    const float *__restrict__ v, const float *__restrict__ w, int64_t nvis, CoordParams cp,
-    TileParams tp, int psf_only, const float *__restrict__ cfu, const float *__restrict__ cfv,
-    int *__restrict__ tile_of, GridRec *__restrict__ rec, int *__restrict__ tile_count,
-    double *__restrict__ blk_sums) {
-    double sw = 0.0, ls = 0.0;
-    const int cfoffu = (cp.suppu + 1) * cp.osampu + 1;
-    const int cfoffv = (cp.suppv + 1) * cp.osampv + 1;
-    const int poffu = (cp.osampu + 1) / 2, poffv = (cp.osampv + 1) / 2;
-    for (int64_t i = (int64_t)blockIdx.x * kPrepThreads + threadIdx.x; i < nvis;
-         i += (int64_t)gridDim.x * kPrepThreads) {
-        const float wti = wt[i];
-        int tile = -1;
-        GridRec r;
-        r.pk = 0; r.wt = 0.0f; r.re = 0.0f; r.im = 0.0f;
-        if (wti > 0.0f) {
-            const double ug = u[i], vg = v[i], wg = w[i];
-            const float2 vv = vis[i];
-            float fr, fi;
-            double ucell, vcell;
-            if (cp.doshift) {
-                const double ul = cp.shift[0] * ug + cp.shift[1] * vg + cp.shift[2] * wg;
-                const double vl = cp.shift[3] * ug + cp.shift[4] * vg + cp.shift[5] * wg;
-                const double wl = cp.shift[6] * ug + cp.shift[7] * vg + cp.shift[8] * wg;
-                const double phase = cp.twopi * (wl - wg);
-                const float cr = (float)cos(phase), ci = (float)(-sin(phase));
-                fr = vv.x * cr - vv.y * ci;
-                fi = vv.x * ci + vv.y * cr;
-                ucell = (double)cp.uscale * ul + (double)cp.uoffset;
-                vcell = (double)cp.vscale * vl + (double)cp.voffset;
-            } else {
-                fr = vv.x;
-                fi = vv.y;
-                ucell = (double)cp.uscale * ug + (double)cp.uoffset;
-                vcell = (double)cp.vscale * vg + (double)cp.voffset;
-            }
-            if (ucell > 0.0) {
-                fi = -fi;
-                ucell = ucell + (double)cp.uorigin;
-                vcell = vcell + (double)cp.vorigin;
-            } else {
-                ucell = -ucell + (double)cp.uorigin;
-                vcell = -vcell + (double)cp.vorigin;
-            }
-            if (psf_only) { fr = 1.0f; fi = 0.0f; }
-            const int nuc = nint_d(ucell), nvc = nint_d(vcell);
-            if (!(nuc + 2 * cp.suppu > cp.nu || nvc + cp.suppv > cp.nv || nvc - cp.suppv < 1)) {
-                const int delui = nint_d((double)cp.osampu * ((double)nuc - ucell));
-                const int delvi = nint_d((double)cp.osampv * ((double)nvc - vcell));
-                double su = 0.0, sv = 0.0;
-                for (int o = -cp.suppu; o <= cp.suppu; o++) su += (double)cfu[delui + cfoffu - 1 + cp.osampu * o];
-                for (int o = -cp.suppv; o <= cp.suppv; o++) sv += (double)cfv[delvi + cfoffv - 1 + cp.osampv * o];
-                const int cu = nuc + cp.suppu - 1, cv = nvc - 1;  // 0-based pre-shift cell
-                if (cu - cp.suppu < 0 || cv - cp.suppv < 0) {
-                    // footprint below the grid (only with UORIGIN/VORIGIN < 1): dropped
-                    tile_of[i] = -1;
-                    rec[i] = r;
-                    continue;
-                }
-                const int tu = cu / tp.T, tv = cv / tp.T;
-                tile = tv * tp.ntu + tu;
-                sw += (double)wti;
-                ls += (double)wti * su * sv;
-                r.pk = (uint32_t)(cu - tu * tp.T) | ((uint32_t)(cv - tv * tp.T) << 8) |
-                       ((uint32_t)(delui + poffu) << 16) | ((uint32_t)(delvi + poffv) << 24);
-                r.wt = wti;
-                r.re = (float)((double)wti * (double)fr);
-                r.im = (float)((double)wti * (double)fi);
-                atomicAdd(&tile_count[tile], 1);
+// The GRDH2D coordinate step (grd/grdh2d.f:115-166).  Returns false when the
+// point is dropped by the grid-edge test.
+struct GridCell {
+    double ucell, vcell;
+    float fr, fi;
+    int nuc, nvc;
+};
+
+__device__ __forceinline__ bool grid_cell(const CoordParams &cp, double ug, double vg, double wg, float2 vv,
+                                          bool want_vis, GridCell &c) {
+    if (cp.doshift) {
+        const double ul = cp.shift[0] * ug + cp.shift[1] * vg + cp.shift[2] * wg;
+        const double vl = cp.shift[3] * ug + cp.shift[4] * vg + cp.shift[5] * wg;
+        if (want_vis) {
+            const double wl = cp.shift[6] * ug + cp.shift[7] * vg + cp.shift[8] * wg;
+            const double phase = cp.twopi * (wl - wg);
+            const float cr = (float)cos(phase), ci = (float)(-sin(phase));
+            c.fr = vv.x * cr - vv.y * ci;
+            c.fi = vv.x * ci + vv.y * cr;
+        }
+        c.ucell = (double)cp.uscale * ul + (double)cp.uoffset;
+        c.vcell = (double)cp.vscale * vl + (double)cp.voffset;
+    } else {
+        c.fr = vv.x;
+        c.fi = vv.y;
+        c.ucell = (double)cp.uscale * ug + (double)cp.uoffset;
+        c.vcell = (double)cp.vscale * vg + (double)cp.voffset;
+    }
+    if (c.ucell > 0.0) {
+        c.fi = -c.fi;
+        c.ucell = c.ucell + (double)cp.uorigin;
+        c.vcell = c.vcell + (double)cp.vorigin;
+    } else {
+        c.ucell = -c.ucell + (double)cp.uorigin;
+        c.vcell = -c.vcell + (double)cp.vorigin;
+    }
+    c.nuc = nint_d(c.ucell);
+    c.nvc = nint_d(c.vcell);
+    if (c.nuc + 2 * cp.suppu > cp.nu || c.nvc + cp.suppv > cp.nv || c.nvc - cp.suppv < 1) return false;
+    // footprint below the grid (only possible with UORIGIN < 1): dropped
+    if (c.nuc < 1) return false;
+    return true;
+}
+
+__device__ __forceinline__ int cell_key(const TileParams &tp, int cu, int cv) {
+    const int tu = cu / tp.T, tv = cv / tp.T;
+    return ((tv * tp.ntu + tu) * tp.T + (cv - tv * tp.T)) * tp.T + (cu - tu * tp.T);
+}
+
+__global__ __launch_bounds__(kThreads) void k_sort_count(const float *__restrict__ wt, const float *__restrict__ u,
+                                                         const float *__restrict__ v, const float *__restrict__ w,
+                                                         int64_t nvis, CoordParams cp, TileParams tp,
+                                                         int *__restrict__ cnt, int *__restrict__ key_of,
+                                                         int *__restrict__ rank_of) {
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < nvis; i += (int64_t)gridDim.x * kThreads) {
+        int key = -1, rank = 0;
+        if (wt[i] > 0.0f) {
+            GridCell c;
+            if (grid_cell(cp, u[i], v[i], w[i], make_float2(0.0f, 0.0f), false, c)) {
+                key = cell_key(tp, c.nuc + cp.suppu - 1, c.nvc - 1);
+                rank = atomicAdd(&cnt[key], 1);
             }
         }
-        tile_of[i] = tile;
-        rec[i] = r;
-    }
-    block_sum2(sw, ls);
-    if (threadIdx.x == 0) {
-        blk_sums[2 * blockIdx.x] = sw;
-        blk_sums[2 * blockIdx.x + 1] = ls;
+        key_of[i] = key;
+        rank_of[i] = rank;
     }
 }
 
-constexpr int kScanThreads = 1024;
-
-__global__ __launch_bounds__(kScanThreads) void k_grid_scan(
-    const int *__restrict__ tile_count, int ntiles, int chunk, int *__restrict__ tile_start,
-    int *__restrict__ item_start, int *__restrict__ tile_cursor, int *__restrict__ nitems,
-    const double *__restrict__ blk_sums, int nblk, double scale_num, double *__restrict__ sums) {
-    __shared__ int sc[kScanThreads], si[kScanThreads];
-    __shared__ double ds[kScanThreads], dl[kScanThreads];
-    const int t = threadIdx.x;
-    const int per = (ntiles + kScanThreads - 1) / kScanThreads;
-    const int lo = min(t * per, ntiles), hi = min(lo + per, ntiles);
-    int c = 0, it = 0;
-    for (int k = lo; k < hi; k++) {
-        const int n = tile_count[k];
-        c += n;
-        it += (n + chunk - 1) / chunk;
+// Block-local exclusive scan of kScanBlock counts; block totals to bsum.
+__global__ __launch_bounds__(1024) void k_scan_local(const int *__restrict__ cnt, int64_t nkeys,
+                                                     int *__restrict__ start, int *__restrict__ bsum) {
+    __shared__ int s[1024];
+    const int64_t base = (int64_t)blockIdx.x * kScanBlock + threadIdx.x * 8;
+    int v[8], tot = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        v[k] = base + k < nkeys ? cnt[base + k] : 0;
+        tot += v[k];
     }
-    sc[t] = c;
-    si[t] = it;
-    // FP64 totals, fixed order
-    double a = 0.0, b = 0.0;
-    for (int k = t; k < nblk; k += kScanThreads) {
-        a += blk_sums[2 * k];
-        b += blk_sums[2 * k + 1];
-    }
-    ds[t] = a;
-    dl[t] = b;
+    s[threadIdx.x] = tot;
     __syncthreads();
-    // inclusive Hillis-Steele scan of the per-thread counts
-    for (int off = 1; off < kScanThreads; off <<= 1) {
-        int x = t >= off ? sc[t - off] : 0;
-        int y = t >= off ? si[t - off] : 0;
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int x = (int)threadIdx.x >= off ? s[threadIdx.x - off] : 0;
         __syncthreads();
-        sc[t] += x;
-        si[t] += y;
+        s[threadIdx.x] += x;
         __syncthreads();
     }
-    int run_c = t > 0 ? sc[t - 1] : 0, run_i = t > 0 ? si[t - 1] : 0;
+    int run = (int)threadIdx.x > 0 ? s[threadIdx.x - 1] : 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        if (base + k < nkeys) start[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 1023) bsum[blockIdx.x] = s[1023];
+}
+
+// Scan of the block totals (nb <= 1024 * 64), then per-tile record ranges,
+// chunk counts and their scan (work items), and the FP64 totals.
+__global__ __launch_bounds__(1024) void k_scan_tiles(int *__restrict__ bsum, int nb, const int *__restrict__ start,
+                                                     TileParams tp, int *__restrict__ tile_start,
+                                                     int *__restrict__ item_start, int *__restrict__ nitems) {
+    __shared__ int s[1024];
+    const int t = threadIdx.x;
+    // exclusive scan of bsum in place (each thread a contiguous segment)
+    const int per = (nb + 1023) / 1024;
+    const int lo = min(t * per, nb), hi = min(lo + per, nb);
+    int tot = 0;
+    for (int k = lo; k < hi; k++) tot += bsum[k];
+    s[t] = tot;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int x = t >= off ? s[t - off] : 0;
+        __syncthreads();
+        s[t] += x;
+        __syncthreads();
+    }
+    int run = t > 0 ? s[t - 1] : 0;
     for (int k = lo; k < hi; k++) {
-        const int n = tile_count[k];
-        tile_start[k] = run_c;
-        item_start[k] = run_i;
-        tile_cursor[k] = 0;
-        run_c += n;
-        run_i += (n + chunk - 1) / chunk;
+        const int b = bsum[k];
+        bsum[k] = run;
+        run += b;
     }
-    if (t == kScanThreads - 1) {
-        tile_start[ntiles] = sc[t];
-        item_start[ntiles] = si[t];
-        *nitems = si[t];
+    const int total = s[1023];
+    __syncthreads();
+    // tile record ranges and chunk counts
+    const int tt = tp.T * tp.T;
+    const int pt = (tp.ntiles + 1023) / 1024;
+    const int tlo = min(t * pt, tp.ntiles), thi = min(tlo + pt, tp.ntiles);
+    int its = 0;
+    for (int k = tlo; k < thi; k++) {
+        const int64_t key = (int64_t)k * tt;
+        const int b0 = start[key] + bsum[key >> kScanShift];
+        const int64_t key1 = key + tt;
+        const int b1 = k + 1 < tp.ntiles ? start[key1] + bsum[key1 >> kScanShift] : total;
+        tile_start[k] = b0;
+        its += (b1 - b0 + tp.chunk - 1) / tp.chunk;
     }
-    for (int s = kScanThreads / 2; s > 0; s >>= 1) {
-        if (t < s) {
-            ds[t] += ds[t + s];
-            dl[t] += dl[t + s];
-        }
+    __syncthreads();
+    s[t] = its;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int x = t >= off ? s[t - off] : 0;
+        __syncthreads();
+        s[t] += x;
         __syncthreads();
     }
-    if (t == 0) {
-        sums[0] = ds[0];
-        sums[1] = dl[0];
-        sums[2] = dl[0] != 0.0 ? scale_num / dl[0] : 0.0;
+    __syncthreads();  // every tile_start written
+    run = t > 0 ? s[t - 1] : 0;
+    for (int k = tlo; k < thi; k++) {
+        item_start[k] = run;
+        const int b1 = k + 1 < tp.ntiles ? tile_start[k + 1] : total;
+        run += (b1 - tile_start[k] + tp.chunk - 1) / tp.chunk;
+    }
+    if (t == 1023) {
+        tile_start[tp.ntiles] = total;
+        item_start[tp.ntiles] = s[1023];
+        *nitems = s[1023];
     }
 }
 
-__global__ void k_grid_scatter(const int *__restrict__ tile_of, const GridRec *__restrict__ rec,
-                               int64_t nvis, const int *__restrict__ tile_start,
-                               int *__restrict__ tile_cursor, GridRec *__restrict__ sorted) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nvis) return;
-    const int t = tile_of[i];
-    if (t < 0) return;
-    const int pos = tile_start[t] + atomicAdd(&tile_cursor[t], 1);
-    sorted[pos] = rec[i];
-}
-
-__global__ void k_grid_items(const int *__restrict__ tile_start, const int *__restrict__ item_start,
-                             int ntiles, int chunk, WorkItem *__restrict__ items) {
+__global__ void k_grid_items(const int *__restrict__ tile_start, const int *__restrict__ item_start, int ntiles,
+                             int chunk, WorkItem *__restrict__ items) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntiles) return;
     const int b = tile_start[t], e = tile_start[t + 1];
@@ -222,28 +230,113 @@ __global__ void k_grid_items(const int *__restrict__ tile_start, const int *__re
     }
 }
 
-// Neumaier-compensated add of x into the (sum, compensation) pair at a.
-__device__ __forceinline__ void comp_add(float2 *buf, int a, float x) {
-    float2 p = buf[a];
-    const float t = p.x + x;
-    const float e = fabsf(p.x) >= fabsf(x) ? (p.x - t) + x : (x - t) + p.x;
-    p.x = t;
-    p.y += e;
-    buf[a] = p;
+__global__ __launch_bounds__(kThreads) void k_grid_fill(
+    const float2 *__restrict__ vis, const float *__restrict__ wt, const float *__restrict__ u,
+    const float *__restrict__ v, const float *__restrict__ w, int64_t nvis, CoordParams cp, TileParams tp,
+    int psf_only, const float *__restrict__ cfu, const float *__restrict__ cfv, const int *__restrict__ key_of,
+    const int *__restrict__ rank_of, const int *__restrict__ start, const int *__restrict__ bsum,
+    GridRec *__restrict__ rec, double *__restrict__ blk_sums) {
+    double sw = 0.0, ls = 0.0;
+    const int cfoffu = (cp.suppu + 1) * cp.osampu + 1;
+    const int cfoffv = (cp.suppv + 1) * cp.osampv + 1;
+    const int poffu = (cp.osampu + 1) / 2, poffv = (cp.osampv + 1) / 2;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < nvis; i += (int64_t)gridDim.x * kThreads) {
+        const int key = key_of[i];
+        if (key < 0) continue;
+        const int pos = start[key] + bsum[key >> kScanShift] + rank_of[i];
+        const float wti = wt[i];
+        GridRec r;
+        r.pk = (uint32_t)(key & (tp.T * tp.T - 1));
+        r.pk = (r.pk % tp.T) | ((r.pk / tp.T) << 8);
+        r.wt = 0.0f;
+        r.re = 0.0f;
+        r.im = 0.0f;
+        if (wti > 0.0f) {
+            GridCell c;
+            grid_cell(cp, u[i], v[i], w[i], vis[i], true, c);
+            if (psf_only) { c.fr = 1.0f; c.fi = 0.0f; }
+            const int delui = nint_d((double)cp.osampu * ((double)c.nuc - c.ucell));
+            const int delvi = nint_d((double)cp.osampv * ((double)c.nvc - c.vcell));
+            double su = 0.0, sv = 0.0;
+            for (int o = -cp.suppu; o <= cp.suppu; o++) su += (double)cfu[delui + cfoffu - 1 + cp.osampu * o];
+            for (int o = -cp.suppv; o <= cp.suppv; o++) sv += (double)cfv[delvi + cfoffv - 1 + cp.osampv * o];
+            sw += (double)wti;
+            ls += (double)wti * su * sv;
+            r.pk |= ((uint32_t)(delui + poffu) << 16) | ((uint32_t)(delvi + poffv) << 24);
+            r.wt = wti;
+            r.re = (float)((double)wti * (double)c.fr);
+            r.im = (float)((double)wti * (double)c.fi);
+        }
+        rec[pos] = r;
+    }
+    block_sum2(sw, ls);
+    if (threadIdx.x == 0) {
+        blk_sums[2 * blockIdx.x] = sw;
+        blk_sums[2 * blockIdx.x + 1] = ls;
+    }
 }
 
-template <int T, int W, int CHM>
-__global__ __launch_bounds__(64 * W) void k_grid_tiles(
-    const GridRec *__restrict__ sorted, const WorkItem *__restrict__ items,
-    const int *__restrict__ nitems, const float *__restrict__ cfu_g,
-    const float *__restrict__ cfv_g, CoordParams cp, float *__restrict__ partial) {
+__global__ __launch_bounds__(1024) void k_grid_totals(const double *__restrict__ blk_sums, int nblk,
+                                                      double scale_num, double *__restrict__ sums) {
+    __shared__ double ds[1024], dl[1024];
+    const int t = threadIdx.x;
+    double a = 0.0, b = 0.0;
+    for (int k = t; k < nblk; k += 1024) {
+        a += blk_sums[2 * k];
+        b += blk_sums[2 * k + 1];
+    }
+    ds[t] = a;
+    dl[t] = b;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (t < s) {
+            ds[t] += ds[t + s];
+            dl[t] += dl[t + s];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        sums[0] = ds[0];
+        sums[1] = dl[0];
+        sums[2] = dl[0] != 0.0 ? scale_num / dl[0] : 0.0;
+    }
+}
+
+__device__ __forceinline__ void kahan(float &s, float &c, float x) {
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// Merge the register pair (s, c) into the LDS planes (S, C) at a: S gets s
+// through an LDS atomic, the exact rounding error of that add (TwoSum of the
+// old S and s) and c go into C.
+__device__ __forceinline__ void flush_pair(float *S, float *Cp, int a, float s, float c) {
+    const float old = atomicAdd(&S[a], s);
+    const float t = old + s;
+    const float bp = t - old;
+    const float ap = t - bp;
+    const float e = (old - ap) + (s - bp);
+    // true value = S - C for both the register pair and the planes
+    atomicAdd(&Cp[a], c - e);
+}
+
+template <int T, int CHM>
+__global__ __launch_bounds__(256) void k_grid_tiles(const GridRec *__restrict__ sorted,
+                                                    const WorkItem *__restrict__ items,
+                                                    const int *__restrict__ nitems, const float *__restrict__ cfu_g,
+                                                    const float *__restrict__ cfv_g, CoordParams cp,
+                                                    float *__restrict__ partial) {
     constexpr int P = T + 2 * kHalo;
     constexpr int NC = P * P;
     constexpr int NCH = ((CHM & 1) ? 2 : 0) + ((CHM & 2) ? 1 : 0);
     constexpr int PSF = (CHM & 1) ? 2 : 0;
-    extern __shared__ float4 smem4[];
-    float2 *acc = reinterpret_cast<float2 *>(smem4);              // [W][NCH][NC]
-    float *cfl = reinterpret_cast<float *>(acc + (size_t)W * NCH * NC);
+    constexpr int W = 4;
+    extern __shared__ float smem[];
+    float *S = smem;              // [NCH][NC] sums
+    float *Cm = smem + NCH * NC;  // [NCH][NC] compensations
+    float *cfl = Cm + NCH * NC;
     const int item = blockIdx.x;
     if (item >= *nitems) return;
     const WorkItem wi = items[item];
@@ -251,12 +344,11 @@ __global__ __launch_bounds__(64 * W) void k_grid_tiles(
     const int poffu = (cp.osampu + 1) / 2, poffv = (cp.osampv + 1) / 2;
     const int ncfu = cp.osampu * (2 * cp.suppu + 1) + 2;
     const int ncfv = cp.osampv * (2 * cp.suppv + 1) + 2;
-    const int baseu = (cp.suppu + 1) * cp.osampu - poffu - cp.osampu * cp.suppu;
-    const int basev = (cp.suppv + 1) * cp.osampv - poffv - cp.osampv * cp.suppv;
+    const int baseu = cp.osampu - poffu, basev = cp.osampv - poffv;
     float *cflu = cfl, *cflv = cfl + ncfu;
     for (int k = threadIdx.x; k < ncfu; k += 64 * W) cflu[k] = cfu_g[baseu + k];
     for (int k = threadIdx.x; k < ncfv; k += 64 * W) cflv[k] = cfv_g[basev + k];
-    for (int k = threadIdx.x; k < W * NCH * NC; k += 64 * W) acc[k] = make_float2(0.0f, 0.0f);
+    for (int k = threadIdx.x; k < 2 * NCH * NC; k += 64 * W) smem[k] = 0.0f;
     __syncthreads();
 
     const int lane = threadIdx.x & 63;
@@ -266,32 +358,50 @@ __global__ __launch_bounds__(64 * W) void k_grid_tiles(
     const bool active = ov < 2 * cp.suppv + 1;
     const int lane_off = (ov - cp.suppv + kHalo) * P + (ou - cp.suppu + kHalo);
     const int cfu_off = cp.osampu * ou, cfv_off = cp.osampv * ov;
-    float2 *my = acc + (size_t)wave * NCH * NC;
 
-    for (int r = wi.begin + wave; r < wi.end; r += W) {
+    const int len = wi.end - wi.begin;
+    const int r0 = wi.begin + (int)(((int64_t)len * wave) / W);
+    const int r1 = wi.begin + (int)(((int64_t)len * (wave + 1)) / W);
+    float s0 = 0.0f, c0 = 0.0f, s1 = 0.0f, c1 = 0.0f, s2 = 0.0f, c2 = 0.0f;
+    int cur = -1;
+    for (int r = r0; r < r1; r++) {
         const GridRec g = sorted[r];
-        const int lu = g.pk & 0xff, lv = (g.pk >> 8) & 0xff;
-        const int pu = (g.pk >> 16) & 0xff, pv = g.pk >> 24;
-        if (active) {
-            const float cw = cflu[pu + cfu_off] * cflv[pv + cfv_off];
-            const int a = lv * P + lu + lane_off;
-            if (CHM & 1) {
-                comp_add(my, a, cw * g.re);
-                comp_add(my + NC, a, cw * g.im);
+        const int cell = (int)(g.pk & 0xffff);
+        if (cell != cur) {
+            if (cur >= 0 && active) {
+                const int a = (cur >> 8) * P + (cur & 0xff) + lane_off;
+                if (CHM & 1) {
+                    flush_pair(S, Cm, a, s0, c0);
+                    flush_pair(S + NC, Cm + NC, a, s1, c1);
+                }
+                if (CHM & 2) flush_pair(S + PSF * NC, Cm + PSF * NC, a, s2, c2);
             }
-            if (CHM & 2) comp_add(my + PSF * NC, a, cw * g.wt);
+            cur = cell;
+            s0 = c0 = s1 = c1 = s2 = c2 = 0.0f;
         }
+        if (active) {
+            const int pu = (g.pk >> 16) & 0xff, pv = g.pk >> 24;
+            const float cw = cflu[pu + cfu_off] * cflv[pv + cfv_off];
+            if (CHM & 1) {
+                kahan(s0, c0, cw * g.re);
+                kahan(s1, c1, cw * g.im);
+            }
+            if (CHM & 2) kahan(s2, c2, cw * g.wt);
+        }
+    }
+    if (cur >= 0 && active) {
+        const int a = (cur >> 8) * P + (cur & 0xff) + lane_off;
+        if (CHM & 1) {
+            flush_pair(S, Cm, a, s0, c0);
+            flush_pair(S + NC, Cm + NC, a, s1, c1);
+        }
+        if (CHM & 2) flush_pair(S + PSF * NC, Cm + PSF * NC, a, s2, c2);
     }
     __syncthreads();
     float *out = partial + (size_t)item * NCH * NC;
     for (int k = threadIdx.x; k < NCH * NC; k += 64 * W) {
-        double s = 0.0;
-#pragma unroll
-        for (int q = 0; q < W; q++) {
-            const float2 p = acc[(size_t)q * NCH * NC + k];
-            s += (double)p.x + (double)p.y;
-        }
-        out[k] = (float)s;
+        // Kahan keeps the running error with the opposite sign: value = S - C
+        out[k] = (float)((double)S[k] - (double)Cm[k]);
     }
 }
 
@@ -367,14 +477,14 @@ __global__ __launch_bounds__(256) void k_grid_reduce(
 }
 
 // ---- BOX (nearest cell), GRDQH2D ----
-__global__ __launch_bounds__(kPrepThreads) void k_box_grid(
+__global__ __launch_bounds__(kThreads) void k_box_grid(
     const float2 *__restrict__ vis, const float *__restrict__ wt, const float *__restrict__ u,
     const float *__restrict__ v, const float *__restrict__ w, int64_t nvis, CoordParams cp,
     int psf, double *__restrict__ acc, double *__restrict__ blk) {
     double sw = 0.0, ls = 0.0;
     const float twopi = 8.0f * atanf(1.0f);
-    for (int64_t i = (int64_t)blockIdx.x * kPrepThreads + threadIdx.x; i < nvis;
-         i += (int64_t)gridDim.x * kPrepThreads) {
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < nvis;
+         i += (int64_t)gridDim.x * kThreads) {
         const float wti = wt[i];
         if (wti <= 0.0f) continue;
         const float ul = (float)(cp.shift[0] * u[i] + cp.shift[1] * v[i] + cp.shift[2] * w[i]);
@@ -432,103 +542,92 @@ __global__ void k_box_fold(const double *__restrict__ acc, CoordParams cp,
     g[(size_t)iv * cp.nu + iu] = out;
 }
 
+int blocks_for(int64_t n) {
+    int64_t b = (n + kThreads - 1) / kThreads;
+    if (b < 1) b = 1;
+    return (int)(b > kMaxBlocks ? kMaxBlocks : b);
+}
+
 }  // namespace
 
-int grid_prep_blocks(int64_t nvis) {
-    int64_t b = (nvis + kPrepThreads - 1) / kPrepThreads;
-    if (b < 1) b = 1;
-    return (int)(b > kMaxPrepBlocks ? kMaxPrepBlocks : b);
+int grid_prep_blocks(int64_t nvis) { return blocks_for(nvis); }
+
+int64_t sort_nkeys(const TileParams &tp) { return (int64_t)tp.ntiles * tp.T * tp.T; }
+int sort_nblocks(const TileParams &tp) { return (int)((sort_nkeys(tp) + kScanBlock - 1) / kScanBlock); }
+
+void launch_sort(hipStream_t s, const float *wt, const float *u, const float *v, const float *w, int64_t nvis,
+                 const CoordParams &cp, const TileParams &tp, SortBufs &b) {
+    const int64_t nkeys = sort_nkeys(tp);
+    const int nb = sort_nblocks(tp);
+    (void)hipMemsetAsync(b.cnt, 0, sizeof(int) * nkeys, s);
+    hipLaunchKernelGGL(k_sort_count, dim3(blocks_for(nvis)), dim3(kThreads), 0, s, wt, u, v, w, nvis, cp, tp, b.cnt,
+                       b.key_of, b.rank_of);
+    hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(1024), 0, s, b.cnt, nkeys, b.start, b.bsum);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, b.bsum, nb, b.start, tp, b.tile_start, b.item_start,
+                       b.nitems);
+    hipLaunchKernelGGL(k_grid_items, dim3((tp.ntiles + 255) / 256), dim3(256), 0, s, b.tile_start, b.item_start,
+                       tp.ntiles, tp.chunk, b.items);
 }
 
-void launch_grid_prep(hipStream_t s, const float2 *vis, const float *wt, const float *u,
-                      const float *v, const float *w, int64_t nvis, const CoordParams &cp,
-                      const TileParams &tp, int psf_only, const float *cfu, const float *cfv,
-                      int *tile_of, GridRec *rec, int *tile_count, double *blk_sums, int nblk) {
-    hipLaunchKernelGGL(k_grid_prep, dim3(nblk), dim3(kPrepThreads), 0, s, vis, wt, u, v, w, nvis,
-                       cp, tp, psf_only, cfu, cfv, tile_of, rec, tile_count, blk_sums);
-}
-
-void launch_grid_scan(hipStream_t s, const int *tile_count, int ntiles, int chunk,
-                      int *tile_start, int *item_start, int *tile_cursor, int *nitems,
-                      const double *blk_sums, int nblk, const CoordParams &cp, double *sums) {
+void launch_grid_fill(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
+                      const float *w, int64_t nvis, const CoordParams &cp, const TileParams &tp, int psf_only,
+                      const float *cfu, const float *cfv, const SortBufs &b, GridRec *rec, double *blk_sums,
+                      int nblk, double *sums) {
+    hipLaunchKernelGGL(k_grid_fill, dim3(nblk), dim3(kThreads), 0, s, vis, wt, u, v, w, nvis, cp, tp, psf_only, cfu,
+                       cfv, b.key_of, b.rank_of, b.start, b.bsum, rec, blk_sums);
     const double num = (double)((float)(cp.nu - 1) * (float)cp.nv);
-    hipLaunchKernelGGL(k_grid_scan, dim3(1), dim3(kScanThreads), 0, s, tile_count, ntiles, chunk,
-                       tile_start, item_start, tile_cursor, nitems, blk_sums, nblk, num, sums);
+    hipLaunchKernelGGL(k_grid_totals, dim3(1), dim3(1024), 0, s, blk_sums, nblk, num, sums);
 }
-
-void launch_grid_scatter(hipStream_t s, const int *tile_of, const GridRec *rec, int64_t nvis,
-                         const int *tile_start, int *tile_cursor, GridRec *sorted) {
-    if (nvis <= 0) return;
-    const int64_t nb = (nvis + 255) / 256;
-    hipLaunchKernelGGL(k_grid_scatter, dim3((unsigned)nb), dim3(256), 0, s, tile_of, rec, nvis,
-                       tile_start, tile_cursor, sorted);
-}
-
-void launch_grid_items(hipStream_t s, const int *tile_start, const int *item_start, int ntiles,
-                       int chunk, WorkItem *items) {
-    hipLaunchKernelGGL(k_grid_items, dim3((ntiles + 255) / 256), dim3(256), 0, s, tile_start,
-                       item_start, ntiles, chunk, items);
-}
-
-static int nch_of(int chmask) { return ((chmask & 1) ? 2 : 0) + ((chmask & 2) ? 1 : 0); }
 
 size_t grid_partial_floats(const TileParams &tp, int chmask, int max_items) {
     const size_t P = (size_t)tp.T + 2 * kHalo;
-    return (size_t)max_items * nch_of(chmask) * P * P;
+    const int nch = ((chmask & 1) ? 2 : 0) + ((chmask & 2) ? 1 : 0);
+    return (size_t)max_items * nch * P * P;
 }
 
-template <int T, int W, int CHM>
-static void launch_tiles_t(hipStream_t s, const GridRec *sorted, const WorkItem *items,
-                           const int *nitems, int max_items, const float *cfu, const float *cfv,
-                           const CoordParams &cp, float *partial) {
+template <int T, int CHM>
+static void launch_tiles_t(hipStream_t s, const GridRec *sorted, const WorkItem *items, const int *nitems,
+                           int max_items, const float *cfu, const float *cfv, const CoordParams &cp,
+                           float *partial) {
     constexpr int P = T + 2 * kHalo;
     constexpr int NCH = ((CHM & 1) ? 2 : 0) + ((CHM & 2) ? 1 : 0);
     const int ncfu = cp.osampu * (2 * cp.suppu + 1) + 2;
     const int ncfv = cp.osampv * (2 * cp.suppv + 1) + 2;
-    const size_t lds = sizeof(float2) * (size_t)W * NCH * P * P + sizeof(float) * (ncfu + ncfv);
-    hipLaunchKernelGGL((k_grid_tiles<T, W, CHM>), dim3(max_items), dim3(64 * W), lds, s, sorted,
-                       items, nitems, cfu, cfv, cp, partial);
+    const size_t lds = sizeof(float) * (2 * (size_t)NCH * P * P + ncfu + ncfv);
+    hipLaunchKernelGGL((k_grid_tiles<T, CHM>), dim3(max_items), dim3(256), lds, s, sorted, items, nitems, cfu, cfv,
+                       cp, partial);
 }
 
-// Workgroup LDS must stay within 64 KiB: T=32 allows W=1 (three channels)
-// or W=2 (one or two channels); T=16 allows W up to 4.
-int launch_grid_tiles(hipStream_t s, const GridRec *sorted, const WorkItem *items,
-                      const int *nitems, int max_items, const TileParams &tp, int chmask,
-                      int waves, const float *cfu, const float *cfv, const CoordParams &cp,
-                      float *partial) {
+int launch_grid_tiles(hipStream_t s, const GridRec *sorted, const WorkItem *items, const int *nitems,
+                      int max_items, const TileParams &tp, int chmask, const float *cfu, const float *cfv,
+                      const CoordParams &cp, float *partial) {
     if (max_items <= 0) return 0;
-#define SDE_TILE_CASE(TT, WW)                                                                \
-    if (tp.T == TT && waves == WW) {                                                         \
-        if (chmask == 1) launch_tiles_t<TT, WW, 1>(s, sorted, items, nitems, max_items, cfu, cfv, cp, partial); \
-        else if (chmask == 2) launch_tiles_t<TT, WW, 2>(s, sorted, items, nitems, max_items, cfu, cfv, cp, partial); \
-        else launch_tiles_t<TT, WW, 3>(s, sorted, items, nitems, max_items, cfu, cfv, cp, partial); \
-        return 0;                                                                            \
+#define SDE_TILE_CASE(TT)                                                                                       \
+    if (tp.T == TT) {                                                                                           \
+        if (chmask == 1) launch_tiles_t<TT, 1>(s, sorted, items, nitems, max_items, cfu, cfv, cp, partial);     \
+        else if (chmask == 2) launch_tiles_t<TT, 2>(s, sorted, items, nitems, max_items, cfu, cfv, cp, partial); \
+        else launch_tiles_t<TT, 3>(s, sorted, items, nitems, max_items, cfu, cfv, cp, partial);                 \
+        return 0;                                                                                               \
     }
-    SDE_TILE_CASE(16, 1)
-    SDE_TILE_CASE(16, 2)
-    SDE_TILE_CASE(16, 4)
-    SDE_TILE_CASE(32, 1)
+    SDE_TILE_CASE(16)
+    SDE_TILE_CASE(32)
 #undef SDE_TILE_CASE
     return -1;
 }
 
-void launch_grid_reduce(hipStream_t s, const float *partial, const int *item_start,
-                        const TileParams &tp, int chmask, const CoordParams &cp,
-                        const double *sums, float2 *dirty, float2 *psf) {
+void launch_grid_reduce(hipStream_t s, const float *partial, const int *item_start, const TileParams &tp,
+                        int chmask, const CoordParams &cp, const double *sums, float2 *dirty, float2 *psf) {
     dim3 grid((cp.nu + 255) / 256, cp.nv);
-    hipLaunchKernelGGL(k_grid_reduce, grid, dim3(256), 0, s, partial, item_start, tp, chmask, cp,
-                       sums, dirty, psf);
+    hipLaunchKernelGGL(k_grid_reduce, grid, dim3(256), 0, s, partial, item_start, tp, chmask, cp, sums, dirty, psf);
 }
 
-void launch_box_grid(hipStream_t s, const float2 *vis, const float *wt, const float *u,
-                     const float *v, const float *w, int64_t nvis, const CoordParams &cp,
-                     int psf, double *acc, double *sums) {
-    hipLaunchKernelGGL(k_box_grid, dim3(grid_prep_blocks(nvis)), dim3(kPrepThreads), 0, s, vis,
-                       wt, u, v, w, nvis, cp, psf, acc, sums);
+void launch_box_grid(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
+                     const float *w, int64_t nvis, const CoordParams &cp, int psf, double *acc, double *sums) {
+    hipLaunchKernelGGL(k_box_grid, dim3(blocks_for(nvis)), dim3(kThreads), 0, s, vis, wt, u, v, w, nvis, cp, psf,
+                       acc, sums);
 }
 
-void launch_box_fold(hipStream_t s, const double *acc, const CoordParams &cp,
-                     const double *sums, float2 *gvis) {
+void launch_box_fold(hipStream_t s, const double *acc, const CoordParams &cp, const double *sums, float2 *gvis) {
     dim3 grid((cp.nu + 255) / 256, cp.nv);
     hipLaunchKernelGGL(k_box_fold, grid, dim3(256), 0, s, acc, cp, sums, gvis);
 }

@@ -50,27 +50,33 @@ struct __align__(16) WorkItem {
     int tile, begin, end, pad;
 };
 
+// ---- cell sort shared by gridding and degridding (grid.hip) ----
+// Counting sort of the visibilities by tile-major cell key.  key_of[i] < 0:
+// not gridded (flagged or off the grid); else the sorted position is
+// start[key] + bsum[key >> 13] + rank_of[i].
+struct SortBufs {
+    int *cnt, *start, *bsum, *key_of, *rank_of, *tile_start, *item_start, *nitems;
+    WorkItem *items;
+    int max_items;
+};
+int64_t sort_nkeys(const TileParams &tp);
+int sort_nblocks(const TileParams &tp);
+void launch_sort(hipStream_t s, const float *wt, const float *u, const float *v, const float *w, int64_t nvis,
+                 const CoordParams &cp, const TileParams &tp, SortBufs &b);
+
 // ---- gridding (grid.hip) ----
-void launch_grid_prep(hipStream_t s, const float2 *vis, const float *wt, const float *u,
-                      const float *v, const float *w, int64_t nvis, const CoordParams &cp,
-                      const TileParams &tp, int psf_only, const float *cfu, const float *cfv,
-                      int *tile_of, GridRec *rec, int *tile_count, double *blk_sums, int nblk);
 int grid_prep_blocks(int64_t nvis);
-void launch_grid_scan(hipStream_t s, const int *tile_count, int ntiles, int chunk,
-                      int *tile_start, int *item_start, int *tile_cursor, int *nitems,
-                      const double *blk_sums, int nblk, const CoordParams &cp, double *sums);
-void launch_grid_scatter(hipStream_t s, const int *tile_of, const GridRec *rec, int64_t nvis,
-                         const int *tile_start, int *tile_cursor, GridRec *sorted);
-void launch_grid_items(hipStream_t s, const int *tile_start, const int *item_start, int ntiles,
-                       int chunk, WorkItem *items);
-// nch: channel mask, bit 0 = dirty (2 floats), bit 1 = PSF (1 float)
-int launch_grid_tiles(hipStream_t s, const GridRec *sorted, const WorkItem *items,
-                      const int *nitems, int max_items, const TileParams &tp, int chmask,
-                      int waves, const float *cfu, const float *cfv, const CoordParams &cp,
-                      float *partial);
-void launch_grid_reduce(hipStream_t s, const float *partial, const int *item_start,
-                        const TileParams &tp, int chmask, const CoordParams &cp,
-                        const double *sums, float2 *dirty, float2 *psf);
+// Records at their sorted positions; SUMWT, LSUMWT and the normalisation to sums[0..2].
+void launch_grid_fill(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
+                      const float *w, int64_t nvis, const CoordParams &cp, const TileParams &tp, int psf_only,
+                      const float *cfu, const float *cfv, const SortBufs &b, GridRec *rec, double *blk_sums,
+                      int nblk, double *sums);
+// chmask: bit 0 = dirty (2 floats), bit 1 = PSF (1 float)
+int launch_grid_tiles(hipStream_t s, const GridRec *sorted, const WorkItem *items, const int *nitems,
+                      int max_items, const TileParams &tp, int chmask, const float *cfu, const float *cfv,
+                      const CoordParams &cp, float *partial);
+void launch_grid_reduce(hipStream_t s, const float *partial, const int *item_start, const TileParams &tp,
+                        int chmask, const CoordParams &cp, const double *sums, float2 *dirty, float2 *psf);
 size_t grid_partial_floats(const TileParams &tp, int chmask, int max_items);
 
 // ---- BOX gridding (grid.hip) ----
@@ -101,11 +107,32 @@ void launch_pixcor(hipStream_t s, const float *in, float *out, int nx, int ny, i
                    const float *cf, int cflen, int correct, float cenx, float ceny);
 
 // ---- degridding and weighting (degrid.hip) ----
-// vis_in supplies the output of visibilities with WT <= 0 (left as they were).
+// Degridding record at the sorted position of a visibility:
+//   idx: original index | conjugate flag << 31 (0xffffffff: slot unused)
+//   pk:  LDS column | LDS row << 8 | u phase << 16 | v phase << 24
+//   rr, ri: the phase rotation
+struct __align__(16) DegridRec {
+    uint32_t idx, pk;
+    float rr, ri;
+};
+
+// One thread per visibility (optionally over an index list): the reference
+// loop.  vis_in supplies the output of visibilities with WT <= 0.
 void launch_degrid(hipStream_t s, const float2 *grid, const float *u, const float *v,
                    const float *w, const float *wt_in, const float2 *vis_in, int64_t nvis,
                    const CoordParams &cp, const float *cfu, const float *cfv, float2 *vis_out,
-                   float *wt_out, unsigned long long *nbad);
+                   float *wt_out, unsigned long long *nbad, const int *list = nullptr,
+                   const int *nlist = nullptr);
+// Tile path: records at the cell-sorted positions (points near the u = 0
+// axis, or not in the gridding sort, go to the list for launch_degrid).
+void launch_degrid_fill(hipStream_t s, const float *u, const float *v, const float *w, const float *wt_in,
+                        const float2 *vis_in, int64_t nvis, const CoordParams &cp, const TileParams &tp,
+                        const SortBufs &b, DegridRec *drec, int *list, int *nlist, float2 *vis_out,
+                        float *wt_out);
+int launch_degrid_tiles(hipStream_t s, const DegridRec *drec, const WorkItem *items, const int *nitems,
+                        int max_items, const TileParams &tp, const float2 *grid, const float *cfu,
+                        const float *cfv, const CoordParams &cp, float2 *vis_out, float *wt_out,
+                        unsigned long long *nbad);
 void launch_wt_density(hipStream_t s, const float *wt, const float *u, const float *v,
                        const float *w, int64_t nvis, const CoordParams &cp, double *gwt,
                        double *blk, int nblk);
