@@ -111,6 +111,8 @@ def main():
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run a 1 GiB device copy first (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the imaging hot path runs on a single MI355X "
@@ -134,6 +136,13 @@ def main():
     print("# generated %d visibilities in %.1f s" % (nvis, t_gen), file=sys.stderr, flush=True)
 
     plan.profile(True)
+    if args.copy_probe:
+        plan.copy_probe(1 << 30, reps=3)
+        cp = plan.timings().get("copy_probe")
+        if cp:
+            print("# copy probe: 2 GiB moved per launch, %.3f ms -> %.0f GB/s" % (cp[0] / cp[1], 2 * 2 ** 30 / (cp[0] / cp[1] * 1e-3) / 1e9),
+                  file=sys.stderr, flush=True)
+        plan.reset_timings()
     for _ in range(args.warmup):
         plan.uvmap()
         plan.vismodel(None)

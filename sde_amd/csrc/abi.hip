@@ -501,6 +501,7 @@ void grdh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, flo
     sb.bsum = ctx_buf<int>(c, "bsum", sort_nblocks(tp));
     sb.key_of = ctx_buf<int>(c, "key_of", nn);
     sb.rank_of = ctx_buf<int>(c, "rank_of", nn);
+    sb.pos_of = ctx_buf<int>(c, "pos_of", nn);
     sb.tile_start = ctx_buf<int>(c, "tile_start", tp.ntiles + 1);
     sb.item_start = ctx_buf<int>(c, "item_start", tp.ntiles + 1);
     sb.nitems = ctx_buf<int>(c, "nitems", 1);
@@ -510,7 +511,7 @@ void grdh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, flo
     double *blk = ctx_buf<double>(c, "blk", 2 * nblk), *dsums = ctx_buf<double>(c, "sums", 4);
     float2 *dg = ctx_buf<float2>(c, "grid", (size_t)*nu * *nv);
     if (!dvis || !dwt || !du || !dv || !dw || !dcfu || !dcfv || !sb.cnt || !sb.start || !sb.bsum || !sb.key_of ||
-        !sb.rank_of || !sb.tile_start || !sb.item_start || !sb.nitems || !sb.items || !rec || !partial || !blk ||
+        !sb.rank_of || !sb.pos_of || !sb.tile_start || !sb.item_start || !sb.nitems || !sb.items || !rec || !partial || !blk ||
         !dsums || !dg) {
         report_error(kErrNoMem, R, "device allocation failed");
         return;
@@ -688,7 +689,12 @@ static void weight_common(const char *R, float *wt, float *u, float *v, float *w
         absf2 = rob * rob;
         absd2 = 2.0f * (ds * ds);
     }
-    launch_wt_finish(s, dgwt, (int)ncell, blk, nblk, mode, t2, absf2, absd2, wsums);
+    double *wsq = ctx_buf<double>(c, "wsq", kWtSqBlocks);
+    if (!wsq) {
+        report_error(kErrNoMem, R, "device allocation failed");
+        return;
+    }
+    launch_wt_finish(s, dgwt, (int)ncell, blk, nblk, mode, t2, absf2, absd2, wsq, wsums);
     launch_wt_apply(s, dwt, du, dv, dw, n, cp, dgwt, wsums, mode == 3, robust, pi_f, dnew, cnt);
     launch_gwt_out(s, dgwt, dgwtf, ncell);
     SDE_F(hipGetLastError());
@@ -871,6 +877,9 @@ struct sde_plan {
     bool sorted = false;  // the cell sort matches the resident visibilities
     GridRec *d_rec = nullptr;
     DegridRec *d_drec = nullptr;
+    float2 *d_drot = nullptr;
+    double *d_wsq = nullptr;
+    bool filled = false;  // pos_of written by a gridding fill on the current sort
     int *d_list = nullptr, *d_nlist = nullptr;
     float *d_partial = nullptr;
     size_t partial_cap = 0, items_cap = 0;
@@ -946,7 +955,8 @@ int plan_reserve_vis(sde_plan *p, int64_t n) {
     if (plan_alloc(&p->d_vis, c) || plan_alloc(&p->d_wt, c) || plan_alloc(&p->d_u, c) || plan_alloc(&p->d_v, c) ||
         plan_alloc(&p->d_w, c) || plan_alloc(&p->d_wteff, c) || plan_alloc(&p->d_mvis, c) ||
         plan_alloc(&p->d_mwt, c) || plan_alloc(&p->sb.key_of, c) || plan_alloc(&p->sb.rank_of, c) ||
-        plan_alloc(&p->d_rec, c) || plan_alloc(&p->d_drec, c) || plan_alloc(&p->d_list, c))
+        plan_alloc(&p->d_rec, c) || plan_alloc(&p->d_drec, c) || plan_alloc(&p->d_list, c) ||
+        plan_alloc(&p->sb.pos_of, c) || plan_alloc(&p->d_drot, p->cp.doshift ? c : 1))
         return -1;
     p->cap = n;
     return 0;
@@ -972,6 +982,7 @@ int plan_sort(sde_plan *p) {
     auto me = [p]() { plan_mark_end(p); };
     if (run_sort(p->stream, p->d_wt, p->d_u, p->d_v, p->d_w, p->nvis, p->cp, p->tp, p->sb, mb, me)) return -1;
     p->sorted = true;
+    p->filled = false;
     return 0;
 }
 
@@ -1044,6 +1055,7 @@ int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **out) {
              plan_alloc(&p->d_mgrid, ng) || plan_alloc(&p->d_dirty, npix) || plan_alloc(&p->d_psf, npix) ||
              plan_alloc(&p->d_img_in, npix) || plan_alloc(&p->d_peak, 4) || plan_alloc(&p->d_gwt, ng) ||
              plan_alloc(&p->d_wblk, 4096) || plan_alloc(&p->d_wsums, 4) || plan_alloc(&p->d_cnt, 4) ||
+             plan_alloc(&p->d_wsq, kWtSqBlocks) ||
              plan_alloc(&p->sb.cnt, (size_t)nkeys) || plan_alloc(&p->sb.start, (size_t)nkeys) ||
              plan_alloc(&p->sb.bsum, (size_t)sort_nblocks(p->tp)) ||
              plan_alloc(&p->sb.tile_start, p->tp.ntiles + 1) || plan_alloc(&p->sb.item_start, p->tp.ntiles + 1) ||
@@ -1079,7 +1091,7 @@ int sde_plan_destroy(sde_plan *p) {
                     p->sb.cnt, p->sb.start, p->sb.bsum, p->sb.key_of, p->sb.rank_of, p->sb.tile_start,
                     p->sb.item_start, p->sb.nitems, p->sb.items, p->d_rec, p->d_drec, p->d_list, p->d_nlist,
                     p->d_partial, p->d_blk, p->d_sums, p->d_grids, p->d_mgrid, p->d_dirty, p->d_psf, p->d_img_in,
-                    p->d_peak, p->d_gwt, p->d_wblk, p->d_wsums, p->d_cnt};
+                    p->d_peak, p->d_gwt, p->d_wblk, p->d_wsums, p->d_cnt, p->sb.pos_of, p->d_drot, p->d_wsq};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &kv : p->tw) (void)hipFree(kv.second);
@@ -1132,7 +1144,7 @@ int sde_plan_uvmap(sde_plan *p) {
         const float t = 5.0f * powf(10.0f, -rob);
         plan_mark_begin(p, "wt_finish");
         launch_wt_finish(s, p->d_gwt, (int)ng, p->d_wblk, nblk, p->wmode, t * t, rob * rob,
-                         2.0f * (p->cfg.ds * p->cfg.ds), p->d_wsums);
+                         2.0f * (p->cfg.ds * p->cfg.ds), p->d_wsq, p->d_wsums);
         plan_mark_end(p);
         SDE_CHECK(hipMemcpyAsync(p->d_wteff, p->d_wt, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
         plan_mark_begin(p, "wt_apply");
@@ -1150,6 +1162,7 @@ int sde_plan_uvmap(sde_plan *p) {
     if (run_grid(s, p->d_vis, wt_use, p->d_u, p->d_v, p->d_w, n, p->cp, p->tp, 3, p->d_cf, p->d_cf, p->sb,
                  p->d_rec, p->d_partial, p->d_blk, grid_prep_blocks(n), p->d_sums, gd, gp, mb, me))
         return -1;
+    p->filled = true;
     float2 *twnv = p->tw[p->nv], *twfnu = p->tw[p->fnu];
     plan_mark_begin(p, "fft_cols");
     launch_fft_cols(s, p->d_grids, p->nu, p->nv, 2, ng, -1, 1.0f, twnv);
@@ -1192,12 +1205,12 @@ int sde_plan_vismodel(sde_plan *p, const float *image) {
     SDE_CHECK(hipMemsetAsync(p->d_cnt + 1, 0, sizeof(unsigned long long), s));
     SDE_CHECK(hipMemsetAsync(p->d_nlist, 0, sizeof(int), s));
     plan_mark_begin(p, "degrid_fill");
-    launch_degrid_fill(s, p->d_u, p->d_v, p->d_w, p->d_wt, p->d_vis, p->nvis, p->cp, p->tp, p->sb, p->d_drec,
-                       p->d_list, p->d_nlist, p->d_mvis, p->d_mwt);
+    launch_degrid_fill(s, p->d_u, p->d_v, p->d_w, p->d_wt, p->d_vis, p->nvis, p->cp, p->tp, p->sb, p->filled,
+                       p->d_drec, p->d_drot, p->d_list, p->d_nlist, p->d_mvis, p->d_mwt);
     plan_mark_end(p);
     plan_mark_begin(p, "degrid");
-    if (launch_degrid_tiles(s, p->d_drec, p->sb.items, p->sb.nitems, p->sb.max_items, p->tp, p->d_mgrid, p->d_cf,
-                            p->d_cf, p->cp, p->d_mvis, p->d_mwt, p->d_cnt + 1)) {
+    if (launch_degrid_tiles(s, p->d_drec, p->d_drot, p->sb.items, p->sb.nitems, p->sb.max_items, p->tp,
+                            p->d_mgrid, p->d_cf, p->d_cf, p->cp, p->d_mvis, p->d_mwt, p->d_cnt + 1)) {
         plan_mark_end(p);
         set_plan_error("unsupported degridding tile T=%d", p->tp.T);
         return -1;

@@ -125,20 +125,20 @@ __global__ __launch_bounds__(256) void k_degrid_fill(
     const float *__restrict__ u, const float *__restrict__ v, const float *__restrict__ w,
     const float *__restrict__ wt_in, const float2 *vis_in, int64_t nvis, CoordParams cp, TileParams tp,
     const int *__restrict__ key_of, const int *__restrict__ rank_of, const int *__restrict__ start,
-    const int *__restrict__ bsum, DegridRec *__restrict__ drec, int *__restrict__ list,
-    int *__restrict__ nlist, float2 *vis_out, float *__restrict__ wt_out) {
+    const int *__restrict__ bsum, const int *__restrict__ pos_of, int use_pos, DegridRec *__restrict__ drec,
+    float2 *__restrict__ drot, int *__restrict__ list, int *__restrict__ nlist, float2 *vis_out,
+    float *__restrict__ wt_out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool to_list = false;
     if (i < nvis) {
         const float wti = wt_in[i];
         const int key = key_of[i];
         int pos = -1;
-        if (key >= 0) pos = start[key] + bsum[key >> kScanShiftD] + rank_of[i];
+        if (key >= 0) pos = use_pos ? pos_of[i] : start[key] + bsum[key >> kScanShiftD] + rank_of[i];
         DegridRec r;
         r.idx = 0xffffffffu;
         r.pk = 0;
-        r.rr = 0.0f;
-        r.ri = 0.0f;
+        float2 rot = make_float2(1.0f, 0.0f);
         if (wti <= 0.0f) {
             wt_out[i] = wti;
             if (vis_in != vis_out) vis_out[i] = vis_in[i];
@@ -180,13 +180,15 @@ __global__ __launch_bounds__(256) void k_degrid_fill(
                     r.idx = (uint32_t)i | (conj ? 0x80000000u : 0u);
                     r.pk = (uint32_t)lu | ((uint32_t)lv << 8) | ((uint32_t)(delui + poffu) << 16) |
                            ((uint32_t)(delvi + poffv) << 24);
-                    r.rr = (float)cos(phase);
-                    r.ri = (float)sin(phase);
+                    rot = make_float2((float)cos(phase), (float)sin(phase));
                     wt_out[i] = wti;  // restored on success (the tile kernel negates it otherwise)
                 }
             }
         }
-        if (pos >= 0) drec[pos] = r;
+        if (pos >= 0) {
+            drec[pos] = r;
+            if (cp.doshift) drot[pos] = rot;
+        }
     }
     // one atomic per wave for the list
     const unsigned long long m = __ballot(to_list);
@@ -202,6 +204,7 @@ __global__ __launch_bounds__(256) void k_degrid_fill(
 
 template <int T>
 __global__ __launch_bounds__(256) void k_degrid_tiles(const DegridRec *__restrict__ drec,
+                                                      const float2 *__restrict__ drot,
                                                       const WorkItem *__restrict__ items,
                                                       const int *__restrict__ nitems, TileParams tp,
                                                       const float2 *__restrict__ grid,
@@ -257,7 +260,9 @@ __global__ __launch_bounds__(256) void k_degrid_tiles(const DegridRec *__restric
             wt_out[idx] = -wt_out[idx];
         }
         if (d.idx & 0x80000000u) vi = -vi;
-        vis_out[idx] = make_float2(vr * d.rr - vi * d.ri, vr * d.ri + vi * d.rr);
+        // GRDDEH2D always rotates: by (1, 0) when there is no shift, exactly
+        const float2 rot = cp.doshift ? drot[r] : make_float2(1.0f, 0.0f);
+        vis_out[idx] = make_float2(vr * rot.x - vi * rot.y, vr * rot.y + vi * rot.x);
     }
 }
 
@@ -313,20 +318,36 @@ __global__ __launch_bounds__(kWtThreads) void k_wt_density(
     if (threadIdx.x == 0) blk[blockIdx.x] = red[0];
 }
 
+// Sum over cells of the squared REAL cell weights (GRDRUW2D's SUMLOCWT),
+// one partial per block.
+__global__ __launch_bounds__(256) void k_wt_sumsq(const double *__restrict__ gwt, int ncell,
+                                                  double *__restrict__ sq) {
+    __shared__ double red[256];
+    double b = 0.0;
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < ncell; k += gridDim.x * 256) {
+        const float g = (float)gwt[k];
+        if (g > 0.0f) b += (double)(g * g);
+    }
+    red[threadIdx.x] = b;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sq[blockIdx.x] = red[0];
+}
+
 // wsums: [0] sum of weights, [1] sum of squared (REAL) cell weights,
 //        [2] F2, [3] D2 (both REAL values held in double).
-__global__ __launch_bounds__(1024) void k_wt_finish(const double *__restrict__ gwt, int ncell,
-                                                    const double *__restrict__ blk, int nblk, int mode,
+__global__ __launch_bounds__(1024) void k_wt_finish(const double *__restrict__ blk, int nblk,
+                                                    const double *__restrict__ sq, int nsq, int mode,
                                                     float rob_t2, float absf2, float absd2,
                                                     double *__restrict__ wsums) {
     __shared__ double ra[1024], rb[1024];
     double a = 0.0, b = 0.0;
     for (int k = threadIdx.x; k < nblk; k += 1024) a += blk[k];
     if (mode == 1 || mode == 3)
-        for (int k = threadIdx.x; k < ncell; k += 1024) {
-            const float g = (float)gwt[k];
-            if (g > 0.0f) b += (double)(g * g);
-        }
+        for (int k = threadIdx.x; k < nsq; k += 1024) b += sq[k];
     ra[threadIdx.x] = a;
     rb[threadIdx.x] = b;
     __syncthreads();
@@ -459,28 +480,29 @@ void launch_degrid(hipStream_t s, const float2 *grid, const float *u, const floa
 
 void launch_degrid_fill(hipStream_t s, const float *u, const float *v, const float *w, const float *wt_in,
                         const float2 *vis_in, int64_t nvis, const CoordParams &cp, const TileParams &tp,
-                        const SortBufs &b, DegridRec *drec, int *list, int *nlist, float2 *vis_out,
-                        float *wt_out) {
+                        const SortBufs &b, int use_pos, DegridRec *drec, float2 *drot, int *list, int *nlist,
+                        float2 *vis_out, float *wt_out) {
     if (nvis <= 0) return;
     const int64_t nb = (nvis + 255) / 256;
     hipLaunchKernelGGL(k_degrid_fill, dim3((unsigned)nb), dim3(256), 0, s, u, v, w, wt_in, vis_in, nvis, cp, tp,
-                       b.key_of, b.rank_of, b.start, b.bsum, drec, list, nlist, vis_out, wt_out);
+                       b.key_of, b.rank_of, b.start, b.bsum, b.pos_of, use_pos, drec, drot, list, nlist, vis_out,
+                       wt_out);
 }
 
-int launch_degrid_tiles(hipStream_t s, const DegridRec *drec, const WorkItem *items, const int *nitems,
-                        int max_items, const TileParams &tp, const float2 *grid, const float *cfu,
-                        const float *cfv, const CoordParams &cp, float2 *vis_out, float *wt_out,
-                        unsigned long long *nbad) {
+int launch_degrid_tiles(hipStream_t s, const DegridRec *drec, const float2 *drot, const WorkItem *items,
+                        const int *nitems, int max_items, const TileParams &tp, const float2 *grid,
+                        const float *cfu, const float *cfv, const CoordParams &cp, float2 *vis_out,
+                        float *wt_out, unsigned long long *nbad) {
     if (max_items <= 0) return 0;
     const int ncfu = cp.osampu * (2 * cp.suppu + 1) + 2;
     const int ncfv = cp.osampv * (2 * cp.suppv + 1) + 2;
     const size_t lds = sizeof(float) * (ncfu + ncfv);
     if (tp.T == 16)
-        hipLaunchKernelGGL(k_degrid_tiles<16>, dim3(max_items), dim3(256), lds, s, drec, items, nitems, tp, grid, cfu,
-                           cfv, cp, vis_out, wt_out, nbad);
+        hipLaunchKernelGGL(k_degrid_tiles<16>, dim3(max_items), dim3(256), lds, s, drec, drot, items, nitems, tp, grid,
+                           cfu, cfv, cp, vis_out, wt_out, nbad);
     else if (tp.T == 32)
-        hipLaunchKernelGGL(k_degrid_tiles<32>, dim3(max_items), dim3(256), lds, s, drec, items, nitems, tp, grid, cfu,
-                           cfv, cp, vis_out, wt_out, nbad);
+        hipLaunchKernelGGL(k_degrid_tiles<32>, dim3(max_items), dim3(256), lds, s, drec, drot, items, nitems, tp, grid,
+                           cfu, cfv, cp, vis_out, wt_out, nbad);
     else
         return -1;
     return 0;
@@ -500,9 +522,16 @@ void launch_wt_density(hipStream_t s, const float *wt, const float *u, const flo
 }
 
 void launch_wt_finish(hipStream_t s, const double *gwt, int ncell, const double *blk, int nblk,
-                      int mode, float rob_t2, float absf2, float absd2, double *wsums) {
-    hipLaunchKernelGGL(k_wt_finish, dim3(1), dim3(1024), 0, s, gwt, ncell, blk, nblk, mode, rob_t2,
-                       absf2, absd2, wsums);
+                      int mode, float rob_t2, float absf2, float absd2, double *sq, double *wsums) {
+    int nsq = 0;
+    if (mode == 1 || mode == 3) {
+        nsq = (ncell + 255) / 256;
+        if (nsq > kWtSqBlocks) nsq = kWtSqBlocks;
+        if (nsq < 1) nsq = 1;
+        hipLaunchKernelGGL(k_wt_sumsq, dim3(nsq), dim3(256), 0, s, gwt, ncell, sq);
+    }
+    hipLaunchKernelGGL(k_wt_finish, dim3(1), dim3(1024), 0, s, blk, nblk, sq, nsq, mode, rob_t2, absf2,
+                       absd2, wsums);
 }
 
 void launch_wt_apply(hipStream_t s, const float *wt, const float *u, const float *v,

@@ -235,7 +235,7 @@ __global__ __launch_bounds__(kThreads) void k_grid_fill(
     const float *__restrict__ v, const float *__restrict__ w, int64_t nvis, CoordParams cp, TileParams tp,
     int psf_only, const float *__restrict__ cfu, const float *__restrict__ cfv, const int *__restrict__ key_of,
     const int *__restrict__ rank_of, const int *__restrict__ start, const int *__restrict__ bsum,
-    GridRec *__restrict__ rec, double *__restrict__ blk_sums) {
+    GridRec *__restrict__ rec, int *__restrict__ pos_of, double *__restrict__ blk_sums) {
     double sw = 0.0, ls = 0.0;
     const int cfoffu = (cp.suppu + 1) * cp.osampu + 1;
     const int cfoffv = (cp.suppv + 1) * cp.osampv + 1;
@@ -244,6 +244,7 @@ __global__ __launch_bounds__(kThreads) void k_grid_fill(
         const int key = key_of[i];
         if (key < 0) continue;
         const int pos = start[key] + bsum[key >> kScanShift] + rank_of[i];
+        pos_of[i] = pos;
         const float wti = wt[i];
         GridRec r;
         r.pk = (uint32_t)(key & (tp.T * tp.T - 1));
@@ -364,30 +365,45 @@ __global__ __launch_bounds__(256) void k_grid_tiles(const GridRec *__restrict__ 
     const int r1 = wi.begin + (int)(((int64_t)len * (wave + 1)) / W);
     float s0 = 0.0f, c0 = 0.0f, s1 = 0.0f, c1 = 0.0f, s2 = 0.0f, c2 = 0.0f;
     int cur = -1;
-    for (int r = r0; r < r1; r++) {
-        const GridRec g = sorted[r];
-        const int cell = (int)(g.pk & 0xffff);
-        if (cell != cur) {
-            if (cur >= 0 && active) {
-                const int a = (cur >> 8) * P + (cur & 0xff) + lane_off;
-                if (CHM & 1) {
-                    flush_pair(S, Cm, a, s0, c0);
-                    flush_pair(S + NC, Cm + NC, a, s1, c1);
+    // records arrive 64 at a time, one per lane (a coalesced 1 KiB load per
+    // wave, the next batch in flight while this one is consumed), and are
+    // broadcast to the wave with readlane
+    const GridRec empty = {0u, 0.0f, 0.0f, 0.0f};
+    GridRec mine = r0 + lane < r1 ? sorted[r0 + lane] : empty;
+    for (int b = r0; b < r1; b += 64) {
+        const GridRec nxt = b + 64 + lane < r1 ? sorted[b + 64 + lane] : empty;
+        const int nb = min(64, r1 - b);
+        for (int j = 0; j < nb; j++) {
+            const uint32_t pk = (uint32_t)__builtin_amdgcn_readlane((int)mine.pk, j);
+            const int cell = (int)(pk & 0xffff);
+            if (cell != cur) {
+                if (cur >= 0 && active) {
+                    const int a = (cur >> 8) * P + (cur & 0xff) + lane_off;
+                    if (CHM & 1) {
+                        flush_pair(S, Cm, a, s0, c0);
+                        flush_pair(S + NC, Cm + NC, a, s1, c1);
+                    }
+                    if (CHM & 2) flush_pair(S + PSF * NC, Cm + PSF * NC, a, s2, c2);
                 }
-                if (CHM & 2) flush_pair(S + PSF * NC, Cm + PSF * NC, a, s2, c2);
+                cur = cell;
+                s0 = c0 = s1 = c1 = s2 = c2 = 0.0f;
             }
-            cur = cell;
-            s0 = c0 = s1 = c1 = s2 = c2 = 0.0f;
-        }
-        if (active) {
-            const int pu = (g.pk >> 16) & 0xff, pv = g.pk >> 24;
-            const float cw = cflu[pu + cfu_off] * cflv[pv + cfv_off];
-            if (CHM & 1) {
-                kahan(s0, c0, cw * g.re);
-                kahan(s1, c1, cw * g.im);
+            if (active) {
+                const int pu = (pk >> 16) & 0xff, pv = pk >> 24;
+                const float cw = cflu[pu + cfu_off] * cflv[pv + cfv_off];
+                if (CHM & 1) {
+                    const float re = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.re), j));
+                    const float im = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.im), j));
+                    kahan(s0, c0, cw * re);
+                    kahan(s1, c1, cw * im);
+                }
+                if (CHM & 2) {
+                    const float wt = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.wt), j));
+                    kahan(s2, c2, cw * wt);
+                }
             }
-            if (CHM & 2) kahan(s2, c2, cw * g.wt);
         }
+        mine = nxt;
     }
     if (cur >= 0 && active) {
         const int a = (cur >> 8) * P + (cur & 0xff) + lane_off;
@@ -574,7 +590,7 @@ void launch_grid_fill(hipStream_t s, const float2 *vis, const float *wt, const f
                       const float *cfu, const float *cfv, const SortBufs &b, GridRec *rec, double *blk_sums,
                       int nblk, double *sums) {
     hipLaunchKernelGGL(k_grid_fill, dim3(nblk), dim3(kThreads), 0, s, vis, wt, u, v, w, nvis, cp, tp, psf_only, cfu,
-                       cfv, b.key_of, b.rank_of, b.start, b.bsum, rec, blk_sums);
+                       cfv, b.key_of, b.rank_of, b.start, b.bsum, rec, b.pos_of, blk_sums);
     const double num = (double)((float)(cp.nu - 1) * (float)cp.nv);
     hipLaunchKernelGGL(k_grid_totals, dim3(1), dim3(1024), 0, s, blk_sums, nblk, num, sums);
 }

@@ -56,6 +56,7 @@ struct __align__(16) WorkItem {
 // start[key] + bsum[key >> 13] + rank_of[i].
 struct SortBufs {
     int *cnt, *start, *bsum, *key_of, *rank_of, *tile_start, *item_start, *nitems;
+    int *pos_of;  // sorted position, written by the gridding fill (valid where key_of >= 0)
     WorkItem *items;
     int max_items;
 };
@@ -110,10 +111,10 @@ void launch_pixcor(hipStream_t s, const float *in, float *out, int nx, int ny, i
 // Degridding record at the sorted position of a visibility:
 //   idx: original index | conjugate flag << 31 (0xffffffff: slot unused)
 //   pk:  LDS column | LDS row << 8 | u phase << 16 | v phase << 24
-//   rr, ri: the phase rotation
-struct __align__(16) DegridRec {
+// plus, when the shift matrix is not the identity, the phase rotation in a
+// parallel float2 array.
+struct __align__(8) DegridRec {
     uint32_t idx, pk;
-    float rr, ri;
 };
 
 // One thread per visibility (optionally over an index list): the reference
@@ -125,20 +126,24 @@ void launch_degrid(hipStream_t s, const float2 *grid, const float *u, const floa
                    const int *nlist = nullptr);
 // Tile path: records at the cell-sorted positions (points near the u = 0
 // axis, or not in the gridding sort, go to the list for launch_degrid).
+// use_pos: take the sorted positions from pos_of (a gridding fill on the same
+// sort ran before), else from the counting-sort offsets.
 void launch_degrid_fill(hipStream_t s, const float *u, const float *v, const float *w, const float *wt_in,
                         const float2 *vis_in, int64_t nvis, const CoordParams &cp, const TileParams &tp,
-                        const SortBufs &b, DegridRec *drec, int *list, int *nlist, float2 *vis_out,
-                        float *wt_out);
-int launch_degrid_tiles(hipStream_t s, const DegridRec *drec, const WorkItem *items, const int *nitems,
-                        int max_items, const TileParams &tp, const float2 *grid, const float *cfu,
-                        const float *cfv, const CoordParams &cp, float2 *vis_out, float *wt_out,
-                        unsigned long long *nbad);
+                        const SortBufs &b, int use_pos, DegridRec *drec, float2 *drot, int *list, int *nlist,
+                        float2 *vis_out, float *wt_out);
+int launch_degrid_tiles(hipStream_t s, const DegridRec *drec, const float2 *drot, const WorkItem *items,
+                        const int *nitems, int max_items, const TileParams &tp, const float2 *grid,
+                        const float *cfu, const float *cfv, const CoordParams &cp, float2 *vis_out,
+                        float *wt_out, unsigned long long *nbad);
 void launch_wt_density(hipStream_t s, const float *wt, const float *u, const float *v,
                        const float *w, int64_t nvis, const CoordParams &cp, double *gwt,
                        double *blk, int nblk);
 int wt_blocks(int64_t nvis);
+// blk holds nblk partial weight sums; sq (>= kWtSqBlocks doubles) is scratch.
 void launch_wt_finish(hipStream_t s, const double *gwt, int ncell, const double *blk, int nblk,
-                      int mode, float rob_t2, float absf2, float absd2, double *wsums);
+                      int mode, float rob_t2, float absf2, float absd2, double *sq, double *wsums);
+constexpr int kWtSqBlocks = 1024;
 void launch_wt_apply(hipStream_t s, const float *wt, const float *u, const float *v,
                      const float *w, int64_t nvis, const CoordParams &cp, const double *gwt,
                      const double *wsums, int az, const float *robust, float pi_f,
