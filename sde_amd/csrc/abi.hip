@@ -366,12 +366,14 @@ int run_sort(hipStream_t s, const float *wt, const float *u, const float *v, con
 }
 
 // Gridding on a cell sort: records, LDS tiles, reduce (fold, shift, normalise).
+// With deg.wt set, the fill also writes the degridding halves of the records
+// (a prediction on the same visibilities then needs no fill of its own).
 int run_grid(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v, const float *w,
              int64_t nvis, const CoordParams &cp, const TileParams &tp, int chmask, const float *cfu,
-             const float *cfv, const SortBufs &sb, GridRec *rec, float *partial, double *blk, int nblk,
-             double *sums, float2 *dirty, float2 *psf, const MarkB &mb, const MarkE &me) {
+             const float *cfv, const SortBufs &sb, VisRec *rec, float *partial, double *blk, double *sums,
+             float2 *dirty, float2 *psf, const DegridOut &deg, const MarkB &mb, const MarkE &me) {
     mb("grid_fill");
-    launch_grid_fill(s, vis, wt, u, v, w, nvis, cp, tp, chmask == 2, cfu, cfv, sb, rec, blk, nblk, sums);
+    launch_grid_fill(s, vis, wt, u, v, w, nvis, cp, tp, chmask == 2, cfu, cfv, sb, rec, blk, sums, deg);
     me();
     mb("grid_tiles");
     if (launch_grid_tiles(s, rec, sb.items, sb.nitems, sb.max_items, tp, chmask, cfu, cfv, cp, partial)) {
@@ -488,7 +490,7 @@ void grdh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, flo
     const TileParams tp = make_tp(*nu, *nv, default_tile(*nu, *nv), default_chunk(n));
     const int chmask = *psf ? 2 : 1;
     const int64_t nkeys = sort_nkeys(tp);
-    const int nblk = grid_prep_blocks(n);
+    const int nblk = grid_fill_blocks(n);
     const size_t nn = (size_t)(n > 0 ? n : 1);
     SortBufs sb;
     sb.max_items = max_items_for(tp, n);
@@ -501,17 +503,16 @@ void grdh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, flo
     sb.bsum = ctx_buf<int>(c, "bsum", sort_nblocks(tp));
     sb.key_of = ctx_buf<int>(c, "key_of", nn);
     sb.rank_of = ctx_buf<int>(c, "rank_of", nn);
-    sb.pos_of = ctx_buf<int>(c, "pos_of", nn);
     sb.tile_start = ctx_buf<int>(c, "tile_start", tp.ntiles + 1);
     sb.item_start = ctx_buf<int>(c, "item_start", tp.ntiles + 1);
     sb.nitems = ctx_buf<int>(c, "nitems", 1);
     sb.items = ctx_buf<WorkItem>(c, "items", sb.max_items);
-    GridRec *rec = ctx_buf<GridRec>(c, "rec", nn);
+    VisRec *rec = ctx_buf<VisRec>(c, "rec", nn);
     float *partial = ctx_buf<float>(c, "partial", grid_partial_floats(tp, chmask, sb.max_items));
     double *blk = ctx_buf<double>(c, "blk", 2 * nblk), *dsums = ctx_buf<double>(c, "sums", 4);
     float2 *dg = ctx_buf<float2>(c, "grid", (size_t)*nu * *nv);
     if (!dvis || !dwt || !du || !dv || !dw || !dcfu || !dcfv || !sb.cnt || !sb.start || !sb.bsum || !sb.key_of ||
-        !sb.rank_of || !sb.pos_of || !sb.tile_start || !sb.item_start || !sb.nitems || !sb.items || !rec || !partial || !blk ||
+        !sb.rank_of || !sb.tile_start || !sb.item_start || !sb.nitems || !sb.items || !rec || !partial || !blk ||
         !dsums || !dg) {
         report_error(kErrNoMem, R, "device allocation failed");
         return;
@@ -537,8 +538,8 @@ void grdh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, flo
     auto nb = [](const char *) {};
     auto ne = []() {};
     if (run_sort(s, dwt, du, dv, dw, n, cp, tp, sb, nb, ne) ||
-        run_grid(s, dvis, dwt, du, dv, dw, n, cp, tp, chmask, dcfu, dcfv, sb, rec, partial, blk, nblk, dsums,
-                 chmask == 1 ? dg : nullptr, chmask == 2 ? dg : nullptr, nb, ne)) {
+        run_grid(s, dvis, dwt, du, dv, dw, n, cp, tp, chmask, dcfu, dcfv, sb, rec, partial, blk, dsums,
+                 chmask == 1 ? dg : nullptr, chmask == 2 ? dg : nullptr, DegridOut(), nb, ne)) {
         report_error(kErrFatal, R, sde_last_error());
         return;
     }
@@ -875,11 +876,9 @@ struct sde_plan {
     float *d_mwt = nullptr;
     SortBufs sb{};
     bool sorted = false;  // the cell sort matches the resident visibilities
-    GridRec *d_rec = nullptr;
-    DegridRec *d_drec = nullptr;
-    float2 *d_drot = nullptr;
+    VisRec *d_rec = nullptr;
     double *d_wsq = nullptr;
-    bool filled = false;  // pos_of written by a gridding fill on the current sort
+    bool filled = false;  // degridding halves written by a gridding fill on the current sort
     int *d_list = nullptr, *d_nlist = nullptr;
     float *d_partial = nullptr;
     size_t partial_cap = 0, items_cap = 0;
@@ -955,8 +954,8 @@ int plan_reserve_vis(sde_plan *p, int64_t n) {
     if (plan_alloc(&p->d_vis, c) || plan_alloc(&p->d_wt, c) || plan_alloc(&p->d_u, c) || plan_alloc(&p->d_v, c) ||
         plan_alloc(&p->d_w, c) || plan_alloc(&p->d_wteff, c) || plan_alloc(&p->d_mvis, c) ||
         plan_alloc(&p->d_mwt, c) || plan_alloc(&p->sb.key_of, c) || plan_alloc(&p->sb.rank_of, c) ||
-        plan_alloc(&p->d_rec, c) || plan_alloc(&p->d_drec, c) || plan_alloc(&p->d_list, c) ||
-        plan_alloc(&p->sb.pos_of, c) || plan_alloc(&p->d_drot, p->cp.doshift ? c : 1))
+        plan_alloc(&p->d_rec, c) || plan_alloc(&p->d_list, c) ||
+        plan_alloc(&p->d_blk, 2 * (size_t)grid_fill_blocks(n)))
         return -1;
     p->cap = n;
     return 0;
@@ -1059,7 +1058,7 @@ int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **out) {
              plan_alloc(&p->sb.cnt, (size_t)nkeys) || plan_alloc(&p->sb.start, (size_t)nkeys) ||
              plan_alloc(&p->sb.bsum, (size_t)sort_nblocks(p->tp)) ||
              plan_alloc(&p->sb.tile_start, p->tp.ntiles + 1) || plan_alloc(&p->sb.item_start, p->tp.ntiles + 1) ||
-             plan_alloc(&p->sb.nitems, 1) || plan_alloc(&p->d_nlist, 1) || plan_alloc(&p->d_blk, 2 * 4096) ||
+             plan_alloc(&p->sb.nitems, 1) || plan_alloc(&p->d_nlist, 1) || plan_alloc(&p->d_blk, 2) ||
              plan_alloc(&p->d_sums, 4);
     if (rc) {
         sde_plan_destroy(p);
@@ -1089,9 +1088,9 @@ int sde_plan_destroy(sde_plan *p) {
     for (auto e : p->pool) (void)hipEventDestroy(e);
     void *bufs[] = {p->d_cf, p->d_ccf, p->d_vis, p->d_wt, p->d_u, p->d_v, p->d_w, p->d_wteff, p->d_mvis, p->d_mwt,
                     p->sb.cnt, p->sb.start, p->sb.bsum, p->sb.key_of, p->sb.rank_of, p->sb.tile_start,
-                    p->sb.item_start, p->sb.nitems, p->sb.items, p->d_rec, p->d_drec, p->d_list, p->d_nlist,
+                    p->sb.item_start, p->sb.nitems, p->sb.items, p->d_rec, p->d_list, p->d_nlist,
                     p->d_partial, p->d_blk, p->d_sums, p->d_grids, p->d_mgrid, p->d_dirty, p->d_psf, p->d_img_in,
-                    p->d_peak, p->d_gwt, p->d_wblk, p->d_wsums, p->d_cnt, p->sb.pos_of, p->d_drot, p->d_wsq};
+                    p->d_peak, p->d_gwt, p->d_wblk, p->d_wsums, p->d_cnt, p->d_wsq};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &kv : p->tw) (void)hipFree(kv.second);
@@ -1159,8 +1158,18 @@ int sde_plan_uvmap(sde_plan *p) {
     float2 *gd = p->d_grids, *gp = p->d_grids + ng;
     auto mb = [p](const char *nm) { plan_mark_begin(p, nm); };
     auto me = [p]() { plan_mark_end(p); };
+    // the fill also writes the degridding halves and the per-point outputs of
+    // a prediction on these visibilities (model VIS/WT are complete after
+    // sde_plan_vismodel)
+    SDE_CHECK(hipMemsetAsync(p->d_nlist, 0, sizeof(int), s));
+    DegridOut deg;
+    deg.wt = p->d_wt;
+    deg.vis_out = p->d_mvis;
+    deg.wt_out = p->d_mwt;
+    deg.list = p->d_list;
+    deg.nlist = p->d_nlist;
     if (run_grid(s, p->d_vis, wt_use, p->d_u, p->d_v, p->d_w, n, p->cp, p->tp, 3, p->d_cf, p->d_cf, p->sb,
-                 p->d_rec, p->d_partial, p->d_blk, grid_prep_blocks(n), p->d_sums, gd, gp, mb, me))
+                 p->d_rec, p->d_partial, p->d_blk, p->d_sums, gd, gp, deg, mb, me))
         return -1;
     p->filled = true;
     float2 *twnv = p->tw[p->nv], *twfnu = p->tw[p->fnu];
@@ -1203,14 +1212,22 @@ int sde_plan_vismodel(sde_plan *p, const float *image) {
     if (p->nvis <= 0) return 0;
     if (!p->sorted && plan_sort(p)) return -1;
     SDE_CHECK(hipMemsetAsync(p->d_cnt + 1, 0, sizeof(unsigned long long), s));
-    SDE_CHECK(hipMemsetAsync(p->d_nlist, 0, sizeof(int), s));
-    plan_mark_begin(p, "degrid_fill");
-    launch_degrid_fill(s, p->d_u, p->d_v, p->d_w, p->d_wt, p->d_vis, p->nvis, p->cp, p->tp, p->sb, p->filled,
-                       p->d_drec, p->d_drot, p->d_list, p->d_nlist, p->d_mvis, p->d_mwt);
-    plan_mark_end(p);
+    if (!p->filled) {
+        SDE_CHECK(hipMemsetAsync(p->d_nlist, 0, sizeof(int), s));
+        DegridOut deg;
+        deg.wt = p->d_wt;
+        deg.vis_out = p->d_mvis;
+        deg.wt_out = p->d_mwt;
+        deg.list = p->d_list;
+        deg.nlist = p->d_nlist;
+        plan_mark_begin(p, "degrid_fill");
+        launch_degrid_fill(s, p->d_u, p->d_v, p->d_w, p->d_vis, p->nvis, p->cp, p->tp, p->sb, p->d_rec, deg);
+        plan_mark_end(p);
+        p->filled = true;
+    }
     plan_mark_begin(p, "degrid");
-    if (launch_degrid_tiles(s, p->d_drec, p->d_drot, p->sb.items, p->sb.nitems, p->sb.max_items, p->tp,
-                            p->d_mgrid, p->d_cf, p->d_cf, p->cp, p->d_mvis, p->d_mwt, p->d_cnt + 1)) {
+    if (launch_degrid_tiles(s, p->d_rec, p->sb.items, p->sb.nitems, p->sb.max_items, p->tp, p->d_mgrid, p->d_cf,
+                            p->d_cf, p->cp, p->d_mvis, p->d_mwt, p->d_cnt + 1)) {
         plan_mark_end(p);
         set_plan_error("unsupported degridding tile T=%d", p->tp.T);
         return -1;

@@ -114,97 +114,48 @@ __global__ __launch_bounds__(256) void k_degrid(
     wt_out[i] = wti;
 }
 
-constexpr int kScanShiftD = 13;  // must match the block-local scan of the cell sort
-constexpr int kHaloD = 4;       // LDS margin: support (<= 3) + 1 cell of REAL/DOUBLE rounding slack
-
-// Records of the tile path (one thread per visibility, original order).
-// A visibility takes the tile path when it is in the gridding sort, is not
+// Degridding halves of the records of the tile path when no gridding fill
+// on the same sort wrote them (one thread per visibility, original order).
+// A visibility takes the tile path when it is in the cell sort, is not
 // within SUPPU cells of the u = 0 axis (those need the reflected reads) and
 // its footprint lies inside its tile's LDS window.
 __global__ __launch_bounds__(256) void k_degrid_fill(
     const float *__restrict__ u, const float *__restrict__ v, const float *__restrict__ w,
-    const float *__restrict__ wt_in, const float2 *vis_in, int64_t nvis, CoordParams cp, TileParams tp,
+    const float *__restrict__ wt_in, const float2 *__restrict__ vis_in, int64_t nvis, CoordParams cp, TileParams tp,
     const int *__restrict__ key_of, const int *__restrict__ rank_of, const int *__restrict__ start,
-    const int *__restrict__ bsum, const int *__restrict__ pos_of, int use_pos, DegridRec *__restrict__ drec,
-    float2 *__restrict__ drot, int *__restrict__ list, int *__restrict__ nlist, float2 *vis_out,
-    float *__restrict__ wt_out) {
+    const int *__restrict__ bsum, VisRec *__restrict__ rec, int *__restrict__ list, int *__restrict__ nlist,
+    float2 *__restrict__ vis_out, float *__restrict__ wt_out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool to_list = false;
     if (i < nvis) {
         const float wti = wt_in[i];
         const int key = key_of[i];
-        int pos = -1;
-        if (key >= 0) pos = use_pos ? pos_of[i] : start[key] + bsum[key >> kScanShiftD] + rank_of[i];
-        DegridRec r;
-        r.idx = 0xffffffffu;
-        r.pk = 0;
-        float2 rot = make_float2(1.0f, 0.0f);
+        DegridRec r = {kNoDegrid, 0u, 1.0f, 0.0f};
         if (wti <= 0.0f) {
             wt_out[i] = wti;
-            if (vis_in != vis_out) vis_out[i] = vis_in[i];
+            vis_out[i] = vis_in[i];
         } else {
-            const double ug = u[i], vg = v[i], wg = w[i];
-            const double ul = cp.shift[0] * ug + cp.shift[1] * vg + cp.shift[2] * wg;
-            const double vl = cp.shift[3] * ug + cp.shift[4] * vg + cp.shift[5] * wg;
-            const double wl = cp.shift[6] * ug + cp.shift[7] * vg + cp.shift[8] * wg;
-            float ucell = (float)((double)cp.uscale * ul + (double)cp.uoffset);
-            float vcell = (float)((double)cp.vscale * vl + (double)cp.voffset);
-            if (ucell > 0.0f) {
-                ucell = ucell + (float)cp.uorigin;
-                vcell = vcell + (float)cp.vorigin;
-            } else {
-                ucell = -ucell + (float)cp.uorigin;
-                vcell = -vcell + (float)cp.vorigin;
-            }
-            const int nuc = nint_f(ucell), nvc = nint_f(vcell);
-            if (nuc + cp.suppu > cp.nu || nvc + cp.suppv > cp.nv || nvc - cp.suppv < 1) {
+            const int path = degrid_half(cp, tp, u[i], v[i], w[i], key, (uint32_t)i, r);
+            if (path == kDegOff) {
                 vis_out[i] = make_float2(0.0f, 0.0f);
                 wt_out[i] = -wti;
-            } else if (key < 0 || nuc <= cp.suppu) {
+            } else if (path == kDegList) {
                 to_list = true;
             } else {
-                const int tt = tp.T * tp.T;
-                const int tile = key / tt;
-                const int tu = tile % tp.ntu, tv = tile / tp.ntu;
-                const int lu = (nuc - 1) - (tu * tp.T - cp.suppu - kHaloD);
-                const int lv = (nvc - 1) - (tv * tp.T - kHaloD);
-                const int pl = tp.T + 2 * kHaloD;
-                if (lu - cp.suppu < 0 || lu + cp.suppu >= pl || lv - cp.suppv < 0 || lv + cp.suppv >= pl) {
-                    to_list = true;
-                } else {
-                    const double phase = cp.twopi * (wl - wg);
-                    const int delui = nint_f((float)cp.osampu * ((float)nuc - ucell));
-                    const int delvi = nint_f((float)cp.osampv * ((float)nvc - vcell));
-                    const int poffu = (cp.osampu + 1) / 2, poffv = (cp.osampv + 1) / 2;
-                    const bool conj = (double)cp.uscale * ul > 0.0;
-                    r.idx = (uint32_t)i | (conj ? 0x80000000u : 0u);
-                    r.pk = (uint32_t)lu | ((uint32_t)lv << 8) | ((uint32_t)(delui + poffu) << 16) |
-                           ((uint32_t)(delvi + poffv) << 24);
-                    rot = make_float2((float)cos(phase), (float)sin(phase));
-                    wt_out[i] = wti;  // restored on success (the tile kernel negates it otherwise)
-                }
+                wt_out[i] = wti;  // negated by the tile kernel if the point fails
             }
         }
-        if (pos >= 0) {
-            drec[pos] = r;
-            if (cp.doshift) drot[pos] = rot;
-        }
+        if (key >= 0) rec[start[key] + bsum[key >> kScanShift] + rank_of[i]].d = r;
     }
-    // one atomic per wave for the list
-    const unsigned long long m = __ballot(to_list);
-    if (m) {
-        const int lane = threadIdx.x & 63;
-        const int leader = __ffsll((long long)m) - 1;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(nlist, __popcll(m));
-        base = __shfl(base, leader);
-        if (to_list) list[base + __popcll(m & ((1ull << lane) - 1))] = (int)i;
-    }
+    wave_append(to_list, (int)i, list, nlist);
 }
 
-template <int T>
-__global__ __launch_bounds__(256) void k_degrid_tiles(const DegridRec *__restrict__ drec,
-                                                      const float2 *__restrict__ drot,
+// One thread per record of a work item; the item's tile and its margin are
+// staged in LDS, the 2*S+1 u weights of the point are held in registers and
+// the gather runs in the reference's loop order (OFFV outer, OFFU inner) in
+// REAL arithmetic, so a given grid degrids to the reference's values.
+template <int T, int S>
+__global__ __launch_bounds__(256) void k_degrid_tiles(const VisRec *__restrict__ rec,
                                                       const WorkItem *__restrict__ items,
                                                       const int *__restrict__ nitems, TileParams tp,
                                                       const float2 *__restrict__ grid,
@@ -219,7 +170,7 @@ __global__ __launch_bounds__(256) void k_degrid_tiles(const DegridRec *__restric
     if (item >= *nitems) return;
     const WorkItem wi = items[item];
     const int tu = wi.tile % tp.ntu, tv = wi.tile / tp.ntu;
-    const int col0 = tu * T - cp.suppu - kHaloD, row0 = tv * T - kHaloD;
+    const int col0 = tu * T - S - kHaloD, row0 = tv * T - kHaloD;
     for (int k = threadIdx.x; k < PL * PL; k += blockDim.x) {
         const int r = k / PL, c = k - r * PL;
         const int gc = col0 + c, gr = row0 + r;
@@ -227,23 +178,28 @@ __global__ __launch_bounds__(256) void k_degrid_tiles(const DegridRec *__restric
                                                                : make_float2(0.0f, 0.0f);
     }
     const int poffu = (cp.osampu + 1) / 2, poffv = (cp.osampv + 1) / 2;
-    const int ncfu = cp.osampu * (2 * cp.suppu + 1) + 2;
-    const int ncfv = cp.osampv * (2 * cp.suppv + 1) + 2;
+    const int ncfu = cp.osampu * (2 * S + 1) + 2;
+    const int ncfv = cp.osampv * (2 * S + 1) + 2;
     float *cflu = cfl, *cflv = cfl + ncfu;
     for (int k = threadIdx.x; k < ncfu; k += blockDim.x) cflu[k] = cfu_g[cp.osampu - poffu + k];
     for (int k = threadIdx.x; k < ncfv; k += blockDim.x) cflv[k] = cfv_g[cp.osampv - poffv + k];
     __syncthreads();
     for (int r = wi.begin + threadIdx.x; r < wi.end; r += blockDim.x) {
-        const DegridRec d = drec[r];
-        if (d.idx == 0xffffffffu) continue;
+        const DegridRec d = rec[r].d;
+        if (d.idx == kNoDegrid) continue;
         const int lu = d.pk & 0xff, lv = (d.pk >> 8) & 0xff;
         const int pu = (d.pk >> 16) & 0xff, pv = d.pk >> 24;
+        float cu[2 * S + 1];
+#pragma unroll
+        for (int k = 0; k < 2 * S + 1; k++) cu[k] = cflu[pu + cp.osampu * k];
         float vr = 0.0f, vi = 0.0f, sumwt = 0.0f;
-        for (int offv = -cp.suppv; offv <= cp.suppv; offv++) {
-            const float cv = cflv[pv + cp.osampv * (offv + cp.suppv)];
+#pragma unroll
+        for (int offv = -S; offv <= S; offv++) {
+            const float cv = cflv[pv + cp.osampv * (offv + S)];
             const float2 *grow = g + (lv + offv) * PL + lu;
-            for (int offu = -cp.suppu; offu <= cp.suppu; offu++) {
-                const float uvwt = cflu[pu + cp.osampu * (offu + cp.suppu)] * cv;
+#pragma unroll
+            for (int offu = -S; offu <= S; offu++) {
+                const float uvwt = cu[offu + S] * cv;
                 const float2 gg = grow[offu];
                 vr = vr + gg.x * uvwt;
                 vi = vi + gg.y * uvwt;
@@ -257,12 +213,11 @@ __global__ __launch_bounds__(256) void k_degrid_tiles(const DegridRec *__restric
         } else {
             atomicAdd(nbad, 1ull);
             vr = vi = 0.0f;
-            wt_out[idx] = -wt_out[idx];
+            wt_out[idx] = -fabsf(wt_out[idx]);
         }
         if (d.idx & 0x80000000u) vi = -vi;
         // GRDDEH2D always rotates: by (1, 0) when there is no shift, exactly
-        const float2 rot = cp.doshift ? drot[r] : make_float2(1.0f, 0.0f);
-        vis_out[idx] = make_float2(vr * rot.x - vi * rot.y, vr * rot.y + vi * rot.x);
+        vis_out[idx] = make_float2(vr * d.rr - vi * d.ri, vr * d.ri + vi * d.rr);
     }
 }
 
@@ -478,34 +433,47 @@ void launch_degrid(hipStream_t s, const float2 *grid, const float *u, const floa
                        nvis, cp, cfu, cfv, vis_out, wt_out, nbad, list, nlist);
 }
 
-void launch_degrid_fill(hipStream_t s, const float *u, const float *v, const float *w, const float *wt_in,
-                        const float2 *vis_in, int64_t nvis, const CoordParams &cp, const TileParams &tp,
-                        const SortBufs &b, int use_pos, DegridRec *drec, float2 *drot, int *list, int *nlist,
-                        float2 *vis_out, float *wt_out) {
+void launch_degrid_fill(hipStream_t s, const float *u, const float *v, const float *w, const float2 *vis_in,
+                        int64_t nvis, const CoordParams &cp, const TileParams &tp, const SortBufs &b,
+                        VisRec *rec, const DegridOut &deg) {
     if (nvis <= 0) return;
     const int64_t nb = (nvis + 255) / 256;
-    hipLaunchKernelGGL(k_degrid_fill, dim3((unsigned)nb), dim3(256), 0, s, u, v, w, wt_in, vis_in, nvis, cp, tp,
-                       b.key_of, b.rank_of, b.start, b.bsum, b.pos_of, use_pos, drec, drot, list, nlist, vis_out,
-                       wt_out);
+    hipLaunchKernelGGL(k_degrid_fill, dim3((unsigned)nb), dim3(256), 0, s, u, v, w, deg.wt, vis_in, nvis, cp, tp,
+                       b.key_of, b.rank_of, b.start, b.bsum, rec, deg.list, deg.nlist, deg.vis_out, deg.wt_out);
 }
 
-int launch_degrid_tiles(hipStream_t s, const DegridRec *drec, const float2 *drot, const WorkItem *items,
-                        const int *nitems, int max_items, const TileParams &tp, const float2 *grid,
-                        const float *cfu, const float *cfv, const CoordParams &cp, float2 *vis_out,
-                        float *wt_out, unsigned long long *nbad) {
-    if (max_items <= 0) return 0;
-    const int ncfu = cp.osampu * (2 * cp.suppu + 1) + 2;
-    const int ncfv = cp.osampv * (2 * cp.suppv + 1) + 2;
+template <int T, int S>
+static void launch_degrid_tiles_t(hipStream_t s, const VisRec *rec, const WorkItem *items, const int *nitems,
+                                  int max_items, const TileParams &tp, const float2 *grid, const float *cfu,
+                                  const float *cfv, const CoordParams &cp, float2 *vis_out, float *wt_out,
+                                  unsigned long long *nbad) {
+    const int ncfu = cp.osampu * (2 * S + 1) + 2;
+    const int ncfv = cp.osampv * (2 * S + 1) + 2;
     const size_t lds = sizeof(float) * (ncfu + ncfv);
-    if (tp.T == 16)
-        hipLaunchKernelGGL(k_degrid_tiles<16>, dim3(max_items), dim3(256), lds, s, drec, drot, items, nitems, tp, grid,
-                           cfu, cfv, cp, vis_out, wt_out, nbad);
-    else if (tp.T == 32)
-        hipLaunchKernelGGL(k_degrid_tiles<32>, dim3(max_items), dim3(256), lds, s, drec, drot, items, nitems, tp, grid,
-                           cfu, cfv, cp, vis_out, wt_out, nbad);
-    else
-        return -1;
-    return 0;
+    hipLaunchKernelGGL((k_degrid_tiles<T, S>), dim3(max_items), dim3(256), lds, s, rec, items, nitems, tp, grid, cfu,
+                       cfv, cp, vis_out, wt_out, nbad);
+}
+
+int launch_degrid_tiles(hipStream_t s, const VisRec *rec, const WorkItem *items, const int *nitems,
+                        int max_items, const TileParams &tp, const float2 *grid, const float *cfu,
+                        const float *cfv, const CoordParams &cp, float2 *vis_out, float *wt_out,
+                        unsigned long long *nbad) {
+    if (max_items <= 0) return 0;
+    if (cp.suppu != cp.suppv) return -1;
+#define SDE_DEG_CASE(TT, SS)                                                                                    \
+    if (tp.T == TT && cp.suppu == SS) {                                                                         \
+        launch_degrid_tiles_t<TT, SS>(s, rec, items, nitems, max_items, tp, grid, cfu, cfv, cp, vis_out, wt_out, \
+                                      nbad);                                                                    \
+        return 0;                                                                                               \
+    }
+    SDE_DEG_CASE(16, 1)
+    SDE_DEG_CASE(16, 2)
+    SDE_DEG_CASE(16, 3)
+    SDE_DEG_CASE(32, 1)
+    SDE_DEG_CASE(32, 2)
+    SDE_DEG_CASE(32, 3)
+#undef SDE_DEG_CASE
+    return -1;
 }
 
 int wt_blocks(int64_t nvis) {

@@ -31,8 +31,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kMaxBlocks = 4096;
-constexpr int kScanBlock = 8192;   // keys per block-local scan (1024 threads x 8)
-constexpr int kScanShift = 13;
+constexpr int kScanBlock = 1 << kScanShift;   // keys per block-local scan (1024 threads x 8)
 
 __device__ __forceinline__ int nint_d(double x) { return (int)round(x); }
 __device__ __forceinline__ int nint_f(float x) { return (int)roundf(x); }
@@ -230,46 +229,78 @@ __global__ void k_grid_items(const int *__restrict__ tile_start, const int *__re
     }
 }
 
+// One thread per visibility.  The gridding half of the record follows
+// GRDH2D's per-point step (phase rotation, conjugation, kernel phases,
+// WT*VIS products); with DEG the degridding half and the per-point outputs
+// of GRDDEH2D for the same point are produced in the same pass.
+template <bool DEG>
 __global__ __launch_bounds__(kThreads) void k_grid_fill(
     const float2 *__restrict__ vis, const float *__restrict__ wt, const float *__restrict__ u,
     const float *__restrict__ v, const float *__restrict__ w, int64_t nvis, CoordParams cp, TileParams tp,
     int psf_only, const float *__restrict__ cfu, const float *__restrict__ cfv, const int *__restrict__ key_of,
     const int *__restrict__ rank_of, const int *__restrict__ start, const int *__restrict__ bsum,
-    GridRec *__restrict__ rec, int *__restrict__ pos_of, double *__restrict__ blk_sums) {
+    VisRec *__restrict__ rec, double *__restrict__ blk_sums, const float *__restrict__ wt_deg,
+    float2 *__restrict__ vis_out, float *__restrict__ wt_out, int *__restrict__ list, int *__restrict__ nlist) {
     double sw = 0.0, ls = 0.0;
-    const int cfoffu = (cp.suppu + 1) * cp.osampu + 1;
-    const int cfoffv = (cp.suppv + 1) * cp.osampv + 1;
-    const int poffu = (cp.osampu + 1) / 2, poffv = (cp.osampv + 1) / 2;
-    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < nvis; i += (int64_t)gridDim.x * kThreads) {
+    bool to_list = false;
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i < nvis) {
         const int key = key_of[i];
-        if (key < 0) continue;
-        const int pos = start[key] + bsum[key >> kScanShift] + rank_of[i];
-        pos_of[i] = pos;
         const float wti = wt[i];
-        GridRec r;
-        r.pk = (uint32_t)(key & (tp.T * tp.T - 1));
-        r.pk = (r.pk % tp.T) | ((r.pk / tp.T) << 8);
-        r.wt = 0.0f;
-        r.re = 0.0f;
-        r.im = 0.0f;
-        if (wti > 0.0f) {
-            GridCell c;
-            grid_cell(cp, u[i], v[i], w[i], vis[i], true, c);
-            if (psf_only) { c.fr = 1.0f; c.fi = 0.0f; }
-            const int delui = nint_d((double)cp.osampu * ((double)c.nuc - c.ucell));
-            const int delvi = nint_d((double)cp.osampv * ((double)c.nvc - c.vcell));
-            double su = 0.0, sv = 0.0;
-            for (int o = -cp.suppu; o <= cp.suppu; o++) su += (double)cfu[delui + cfoffu - 1 + cp.osampu * o];
-            for (int o = -cp.suppv; o <= cp.suppv; o++) sv += (double)cfv[delvi + cfoffv - 1 + cp.osampv * o];
-            sw += (double)wti;
-            ls += (double)wti * su * sv;
-            r.pk |= ((uint32_t)(delui + poffu) << 16) | ((uint32_t)(delvi + poffv) << 24);
-            r.wt = wti;
-            r.re = (float)((double)wti * (double)c.fr);
-            r.im = (float)((double)wti * (double)c.fi);
+        const double ug = u[i], vg = v[i], wg = w[i];
+        DegridRec d = {kNoDegrid, 0u, 1.0f, 0.0f};
+        if (DEG) {
+            const float wd = wt_deg[i];
+            if (wd <= 0.0f) {
+                wt_out[i] = wd;
+                vis_out[i] = vis[i];
+            } else {
+                const int path = degrid_half(cp, tp, ug, vg, wg, key, (uint32_t)i, d);
+                if (path == kDegOff) {
+                    vis_out[i] = make_float2(0.0f, 0.0f);
+                    wt_out[i] = -wd;
+                } else if (path == kDegList) {
+                    to_list = true;
+                } else {
+                    wt_out[i] = wd;  // negated by the tile kernel if the point fails
+                }
+            }
         }
-        rec[pos] = r;
+        if (key >= 0) {
+            const int pos = start[key] + bsum[key >> kScanShift] + rank_of[i];
+            GridRec r;
+            r.pk = (uint32_t)(key & (tp.T * tp.T - 1));
+            r.pk = (r.pk % tp.T) | ((r.pk / tp.T) << 8);
+            r.wt = 0.0f;
+            r.re = 0.0f;
+            r.im = 0.0f;
+            if (wti > 0.0f) {
+                GridCell c;
+                grid_cell(cp, ug, vg, wg, psf_only ? make_float2(1.0f, 0.0f) : vis[i], !psf_only, c);
+                if (psf_only) { c.fr = 1.0f; c.fi = 0.0f; }
+                const int cfoffu = (cp.suppu + 1) * cp.osampu + 1;
+                const int cfoffv = (cp.suppv + 1) * cp.osampv + 1;
+                const int poffu = (cp.osampu + 1) / 2, poffv = (cp.osampv + 1) / 2;
+                const int delui = nint_d((double)cp.osampu * ((double)c.nuc - c.ucell));
+                const int delvi = nint_d((double)cp.osampv * ((double)c.nvc - c.vcell));
+                double su = 0.0, sv = 0.0;
+                for (int o = -cp.suppu; o <= cp.suppu; o++) su += (double)cfu[delui + cfoffu - 1 + cp.osampu * o];
+                for (int o = -cp.suppv; o <= cp.suppv; o++) sv += (double)cfv[delvi + cfoffv - 1 + cp.osampv * o];
+                sw = (double)wti;
+                ls = (double)wti * su * sv;
+                r.pk |= ((uint32_t)(delui + poffu) << 16) | ((uint32_t)(delvi + poffv) << 24);
+                r.wt = wti;
+                r.re = (float)((double)wti * (double)c.fr);
+                r.im = (float)((double)wti * (double)c.fi);
+            }
+            VisRec vr;
+            vr.g = r;
+            vr.d = d;
+            if (DEG) rec[pos] = vr;
+            else rec[pos].g = r;
+        }
     }
+    if (DEG) wave_append(to_list, (int)i, list, nlist);
     block_sum2(sw, ls);
     if (threadIdx.x == 0) {
         blk_sums[2 * blockIdx.x] = sw;
@@ -303,28 +334,13 @@ __global__ __launch_bounds__(1024) void k_grid_totals(const double *__restrict__
     }
 }
 
-__device__ __forceinline__ void kahan(float &s, float &c, float x) {
-    const float y = x - c;
-    const float t = s + y;
-    c = (t - s) - y;
-    s = t;
-}
-
-// Merge the register pair (s, c) into the LDS planes (S, C) at a: S gets s
-// through an LDS atomic, the exact rounding error of that add (TwoSum of the
-// old S and s) and c go into C.
-__device__ __forceinline__ void flush_pair(float *S, float *Cp, int a, float s, float c) {
-    const float old = atomicAdd(&S[a], s);
-    const float t = old + s;
-    const float bp = t - old;
-    const float ap = t - bp;
-    const float e = (old - ap) + (s - bp);
-    // true value = S - C for both the register pair and the planes
-    atomicAdd(&Cp[a], c - e);
-}
+// Kernel phase tables in LDS carry one pad word per 32 so that the seven
+// taps of one point (phases p + OSAMP*k) fall on distinct banks.
+__device__ __forceinline__ int cf_pad(int j) { return j + (j >> 5); }
+__host__ __device__ inline int cf_lds_words(int n) { return n + (n >> 5) + 1; }
 
 template <int T, int CHM>
-__global__ __launch_bounds__(256) void k_grid_tiles(const GridRec *__restrict__ sorted,
+__global__ __launch_bounds__(256) void k_grid_tiles(const VisRec *__restrict__ sorted,
                                                     const WorkItem *__restrict__ items,
                                                     const int *__restrict__ nitems, const float *__restrict__ cfu_g,
                                                     const float *__restrict__ cfv_g, CoordParams cp,
@@ -334,10 +350,9 @@ __global__ __launch_bounds__(256) void k_grid_tiles(const GridRec *__restrict__ 
     constexpr int NCH = ((CHM & 1) ? 2 : 0) + ((CHM & 2) ? 1 : 0);
     constexpr int PSF = (CHM & 1) ? 2 : 0;
     constexpr int W = 4;
-    extern __shared__ float smem[];
-    float *S = smem;              // [NCH][NC] sums
-    float *Cm = smem + NCH * NC;  // [NCH][NC] compensations
-    float *cfl = Cm + NCH * NC;
+    extern __shared__ double smem_d[];
+    double *S = smem_d;                          // [NCH][NC] FP64 sums
+    float *cfl = (float *)(smem_d + NCH * NC);   // padded phase tables
     const int item = blockIdx.x;
     if (item >= *nitems) return;
     const WorkItem wi = items[item];
@@ -346,10 +361,10 @@ __global__ __launch_bounds__(256) void k_grid_tiles(const GridRec *__restrict__ 
     const int ncfu = cp.osampu * (2 * cp.suppu + 1) + 2;
     const int ncfv = cp.osampv * (2 * cp.suppv + 1) + 2;
     const int baseu = cp.osampu - poffu, basev = cp.osampv - poffv;
-    float *cflu = cfl, *cflv = cfl + ncfu;
-    for (int k = threadIdx.x; k < ncfu; k += 64 * W) cflu[k] = cfu_g[baseu + k];
-    for (int k = threadIdx.x; k < ncfv; k += 64 * W) cflv[k] = cfv_g[basev + k];
-    for (int k = threadIdx.x; k < 2 * NCH * NC; k += 64 * W) smem[k] = 0.0f;
+    float *cflu = cfl, *cflv = cfl + cf_lds_words(ncfu);
+    for (int k = threadIdx.x; k < ncfu; k += 64 * W) cflu[cf_pad(k)] = cfu_g[baseu + k];
+    for (int k = threadIdx.x; k < ncfv; k += 64 * W) cflv[cf_pad(k)] = cfv_g[basev + k];
+    for (int k = threadIdx.x; k < NCH * NC; k += 64 * W) S[k] = 0.0;
     __syncthreads();
 
     const int lane = threadIdx.x & 63;
@@ -363,15 +378,19 @@ __global__ __launch_bounds__(256) void k_grid_tiles(const GridRec *__restrict__ 
     const int len = wi.end - wi.begin;
     const int r0 = wi.begin + (int)(((int64_t)len * wave) / W);
     const int r1 = wi.begin + (int)(((int64_t)len * (wave + 1)) / W);
-    float s0 = 0.0f, c0 = 0.0f, s1 = 0.0f, c1 = 0.0f, s2 = 0.0f, c2 = 0.0f;
+    // Each lane is one kernel tap.  Consecutive points of one cell (the
+    // records are cell-sorted) accumulate in FP64 registers -- the product of
+    // two REAL factors is exact in DOUBLE, so one fused multiply-add per
+    // channel is the whole update -- and are added into the FP64 LDS tile
+    // when the cell changes.
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     int cur = -1;
-    // records arrive 64 at a time, one per lane (a coalesced 1 KiB load per
-    // wave, the next batch in flight while this one is consumed), and are
-    // broadcast to the wave with readlane
+    // records arrive 64 at a time, one per lane (the next batch in flight
+    // while this one is consumed), and are broadcast to the wave with readlane
     const GridRec empty = {0u, 0.0f, 0.0f, 0.0f};
-    GridRec mine = r0 + lane < r1 ? sorted[r0 + lane] : empty;
+    GridRec mine = r0 + lane < r1 ? sorted[r0 + lane].g : empty;
     for (int b = r0; b < r1; b += 64) {
-        const GridRec nxt = b + 64 + lane < r1 ? sorted[b + 64 + lane] : empty;
+        const GridRec nxt = b + 64 + lane < r1 ? sorted[b + 64 + lane].g : empty;
         const int nb = min(64, r1 - b);
         for (int j = 0; j < nb; j++) {
             const uint32_t pk = (uint32_t)__builtin_amdgcn_readlane((int)mine.pk, j);
@@ -380,26 +399,26 @@ __global__ __launch_bounds__(256) void k_grid_tiles(const GridRec *__restrict__ 
                 if (cur >= 0 && active) {
                     const int a = (cur >> 8) * P + (cur & 0xff) + lane_off;
                     if (CHM & 1) {
-                        flush_pair(S, Cm, a, s0, c0);
-                        flush_pair(S + NC, Cm + NC, a, s1, c1);
+                        atomicAdd(&S[a], s0);
+                        atomicAdd(&S[NC + a], s1);
                     }
-                    if (CHM & 2) flush_pair(S + PSF * NC, Cm + PSF * NC, a, s2, c2);
+                    if (CHM & 2) atomicAdd(&S[PSF * NC + a], s2);
                 }
                 cur = cell;
-                s0 = c0 = s1 = c1 = s2 = c2 = 0.0f;
+                s0 = s1 = s2 = 0.0;
             }
             if (active) {
                 const int pu = (pk >> 16) & 0xff, pv = pk >> 24;
-                const float cw = cflu[pu + cfu_off] * cflv[pv + cfv_off];
+                const double cw = (double)(cflu[cf_pad(pu + cfu_off)] * cflv[cf_pad(pv + cfv_off)]);
                 if (CHM & 1) {
                     const float re = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.re), j));
                     const float im = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.im), j));
-                    kahan(s0, c0, cw * re);
-                    kahan(s1, c1, cw * im);
+                    s0 = fma(cw, (double)re, s0);
+                    s1 = fma(cw, (double)im, s1);
                 }
                 if (CHM & 2) {
                     const float wt = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.wt), j));
-                    kahan(s2, c2, cw * wt);
+                    s2 = fma(cw, (double)wt, s2);
                 }
             }
         }
@@ -408,17 +427,14 @@ __global__ __launch_bounds__(256) void k_grid_tiles(const GridRec *__restrict__ 
     if (cur >= 0 && active) {
         const int a = (cur >> 8) * P + (cur & 0xff) + lane_off;
         if (CHM & 1) {
-            flush_pair(S, Cm, a, s0, c0);
-            flush_pair(S + NC, Cm + NC, a, s1, c1);
+            atomicAdd(&S[a], s0);
+            atomicAdd(&S[NC + a], s1);
         }
-        if (CHM & 2) flush_pair(S + PSF * NC, Cm + PSF * NC, a, s2, c2);
+        if (CHM & 2) atomicAdd(&S[PSF * NC + a], s2);
     }
     __syncthreads();
     float *out = partial + (size_t)item * NCH * NC;
-    for (int k = threadIdx.x; k < NCH * NC; k += 64 * W) {
-        // Kahan keeps the running error with the opposite sign: value = S - C
-        out[k] = (float)((double)S[k] - (double)Cm[k]);
-    }
+    for (int k = threadIdx.x; k < NCH * NC; k += 64 * W) out[k] = (float)S[k];
 }
 
 struct Sum3 {
@@ -585,12 +601,21 @@ void launch_sort(hipStream_t s, const float *wt, const float *u, const float *v,
                        tp.ntiles, tp.chunk, b.items);
 }
 
+int grid_fill_blocks(int64_t nvis) { return (int)((nvis > 0 ? nvis + kThreads - 1 : kThreads) / kThreads); }
+
 void launch_grid_fill(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
                       const float *w, int64_t nvis, const CoordParams &cp, const TileParams &tp, int psf_only,
-                      const float *cfu, const float *cfv, const SortBufs &b, GridRec *rec, double *blk_sums,
-                      int nblk, double *sums) {
-    hipLaunchKernelGGL(k_grid_fill, dim3(nblk), dim3(kThreads), 0, s, vis, wt, u, v, w, nvis, cp, tp, psf_only, cfu,
-                       cfv, b.key_of, b.rank_of, b.start, b.bsum, rec, b.pos_of, blk_sums);
+                      const float *cfu, const float *cfv, const SortBufs &b, VisRec *rec, double *blk_sums,
+                      double *sums, const DegridOut &deg) {
+    const int nblk = grid_fill_blocks(nvis);
+    if (deg.wt)
+        hipLaunchKernelGGL(k_grid_fill<true>, dim3(nblk), dim3(kThreads), 0, s, vis, wt, u, v, w, nvis, cp, tp,
+                           psf_only, cfu, cfv, b.key_of, b.rank_of, b.start, b.bsum, rec, blk_sums, deg.wt,
+                           deg.vis_out, deg.wt_out, deg.list, deg.nlist);
+    else
+        hipLaunchKernelGGL(k_grid_fill<false>, dim3(nblk), dim3(kThreads), 0, s, vis, wt, u, v, w, nvis, cp, tp,
+                           psf_only, cfu, cfv, b.key_of, b.rank_of, b.start, b.bsum, rec, blk_sums, nullptr,
+                           nullptr, nullptr, nullptr, nullptr);
     const double num = (double)((float)(cp.nu - 1) * (float)cp.nv);
     hipLaunchKernelGGL(k_grid_totals, dim3(1), dim3(1024), 0, s, blk_sums, nblk, num, sums);
 }
@@ -602,19 +627,19 @@ size_t grid_partial_floats(const TileParams &tp, int chmask, int max_items) {
 }
 
 template <int T, int CHM>
-static void launch_tiles_t(hipStream_t s, const GridRec *sorted, const WorkItem *items, const int *nitems,
+static void launch_tiles_t(hipStream_t s, const VisRec *sorted, const WorkItem *items, const int *nitems,
                            int max_items, const float *cfu, const float *cfv, const CoordParams &cp,
                            float *partial) {
     constexpr int P = T + 2 * kHalo;
     constexpr int NCH = ((CHM & 1) ? 2 : 0) + ((CHM & 2) ? 1 : 0);
     const int ncfu = cp.osampu * (2 * cp.suppu + 1) + 2;
     const int ncfv = cp.osampv * (2 * cp.suppv + 1) + 2;
-    const size_t lds = sizeof(float) * (2 * (size_t)NCH * P * P + ncfu + ncfv);
+    const size_t lds = sizeof(double) * (size_t)NCH * P * P + sizeof(float) * (cf_lds_words(ncfu) + cf_lds_words(ncfv));
     hipLaunchKernelGGL((k_grid_tiles<T, CHM>), dim3(max_items), dim3(256), lds, s, sorted, items, nitems, cfu, cfv,
                        cp, partial);
 }
 
-int launch_grid_tiles(hipStream_t s, const GridRec *sorted, const WorkItem *items, const int *nitems,
+int launch_grid_tiles(hipStream_t s, const VisRec *sorted, const WorkItem *items, const int *nitems,
                       int max_items, const TileParams &tp, int chmask, const float *cfu, const float *cfv,
                       const CoordParams &cp, float *partial) {
     if (max_items <= 0) return 0;

@@ -46,6 +46,24 @@ struct __align__(16) GridRec {
     float wt, re, im;
 };
 
+// Degridding half of a sorted record:
+//   idx: original index | conjugate flag << 31; kNoDegrid: the point is not
+//        degridded from this record (list path, off the grid or flagged)
+//   pk:  LDS column | LDS row << 8 | u phase << 16 | v phase << 24
+//   rr, ri: the phase rotation of the shift (1, 0 without a shift)
+struct __align__(16) DegridRec {
+    uint32_t idx, pk;
+    float rr, ri;
+};
+constexpr uint32_t kNoDegrid = 0xffffffffu;
+
+// One 32-byte record per sorted slot: the gridding and degridding halves of
+// a visibility land in one scattered 32-byte write.
+struct __align__(32) VisRec {
+    GridRec g;
+    DegridRec d;
+};
+
 struct __align__(16) WorkItem {
     int tile, begin, end, pad;
 };
@@ -56,7 +74,6 @@ struct __align__(16) WorkItem {
 // start[key] + bsum[key >> 13] + rank_of[i].
 struct SortBufs {
     int *cnt, *start, *bsum, *key_of, *rank_of, *tile_start, *item_start, *nitems;
-    int *pos_of;  // sorted position, written by the gridding fill (valid where key_of >= 0)
     WorkItem *items;
     int max_items;
 };
@@ -67,13 +84,24 @@ void launch_sort(hipStream_t s, const float *wt, const float *u, const float *v,
 
 // ---- gridding (grid.hip) ----
 int grid_prep_blocks(int64_t nvis);
-// Records at their sorted positions; SUMWT, LSUMWT and the normalisation to sums[0..2].
+// Records at their sorted positions; SUMWT, LSUMWT and the normalisation to
+// sums[0..2].  blk_sums holds 2 doubles per block of grid_fill_blocks().
+// With deg.wt (the WT of a later degridding) the degridding halves are
+// written too: deg.vis_out/wt_out receive VIS/WT of every point the tile
+// path does not finish, and the points it cannot take go to deg.list.
+struct DegridOut {
+    const float *wt = nullptr;
+    float2 *vis_out = nullptr;
+    float *wt_out = nullptr;
+    int *list = nullptr, *nlist = nullptr;
+};
+int grid_fill_blocks(int64_t nvis);
 void launch_grid_fill(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
                       const float *w, int64_t nvis, const CoordParams &cp, const TileParams &tp, int psf_only,
-                      const float *cfu, const float *cfv, const SortBufs &b, GridRec *rec, double *blk_sums,
-                      int nblk, double *sums);
+                      const float *cfu, const float *cfv, const SortBufs &b, VisRec *rec, double *blk_sums,
+                      double *sums, const DegridOut &deg);
 // chmask: bit 0 = dirty (2 floats), bit 1 = PSF (1 float)
-int launch_grid_tiles(hipStream_t s, const GridRec *sorted, const WorkItem *items, const int *nitems,
+int launch_grid_tiles(hipStream_t s, const VisRec *sorted, const WorkItem *items, const int *nitems,
                       int max_items, const TileParams &tp, int chmask, const float *cfu, const float *cfv,
                       const CoordParams &cp, float *partial);
 void launch_grid_reduce(hipStream_t s, const float *partial, const int *item_start, const TileParams &tp,
@@ -108,14 +136,77 @@ void launch_pixcor(hipStream_t s, const float *in, float *out, int nx, int ny, i
                    const float *cf, int cflen, int correct, float cenx, float ceny);
 
 // ---- degridding and weighting (degrid.hip) ----
-// Degridding record at the sorted position of a visibility:
-//   idx: original index | conjugate flag << 31 (0xffffffff: slot unused)
-//   pk:  LDS column | LDS row << 8 | u phase << 16 | v phase << 24
-// plus, when the shift matrix is not the identity, the phase rotation in a
-// parallel float2 array.
-struct __align__(8) DegridRec {
-    uint32_t idx, pk;
-};
+
+// LDS margin of the degridding window: support (<= 3) plus one cell of slack
+// between the REAL cell of GRDDEH2D and the DOUBLE cell of the sort key.
+constexpr int kHaloD = 4;
+// Block-local scan width of the cell sort (keys per block = 1 << kScanShift).
+constexpr int kScanShift = 13;
+
+enum DegridPath { kDegTile = 0, kDegList = 1, kDegOff = 2 };
+
+// The GRDDEH2D coordinate step of one point with WT > 0 (grd/grddeh2d.f:
+// 107-139, REAL cell from the DOUBLE rotated coordinates) and the degridding
+// half of its record.  key is the point's cell-sort key (< 0: not sorted).
+// kDegOff: the footprint leaves the grid (VIS = 0, WT negated); kDegList:
+// the point needs the per-point kernel (near the u = 0 axis, not sorted, or
+// outside its tile's LDS window); kDegTile: r is filled.
+__device__ __forceinline__ int degrid_half(const CoordParams &cp, const TileParams &tp, double ug, double vg,
+                                           double wg, int key, uint32_t i, DegridRec &r) {
+    r.idx = kNoDegrid;
+    r.pk = 0;
+    r.rr = 1.0f;
+    r.ri = 0.0f;
+    double ul = ug, vl = vg, wl = wg;
+    if (cp.doshift) {
+        ul = cp.shift[0] * ug + cp.shift[1] * vg + cp.shift[2] * wg;
+        vl = cp.shift[3] * ug + cp.shift[4] * vg + cp.shift[5] * wg;
+        wl = cp.shift[6] * ug + cp.shift[7] * vg + cp.shift[8] * wg;
+    }
+    float ucell = (float)((double)cp.uscale * ul + (double)cp.uoffset);
+    float vcell = (float)((double)cp.vscale * vl + (double)cp.voffset);
+    if (ucell > 0.0f) {
+        ucell = ucell + (float)cp.uorigin;
+        vcell = vcell + (float)cp.vorigin;
+    } else {
+        ucell = -ucell + (float)cp.uorigin;
+        vcell = -vcell + (float)cp.vorigin;
+    }
+    const int nuc = (int)roundf(ucell), nvc = (int)roundf(vcell);
+    if (nuc + cp.suppu > cp.nu || nvc + cp.suppv > cp.nv || nvc - cp.suppv < 1) return kDegOff;
+    if (key < 0 || nuc <= cp.suppu) return kDegList;
+    const int tile = key / (tp.T * tp.T);
+    const int tu = tile % tp.ntu, tv = tile / tp.ntu;
+    const int lu = (nuc - 1) - (tu * tp.T - cp.suppu - kHaloD);
+    const int lv = (nvc - 1) - (tv * tp.T - kHaloD);
+    const int pl = tp.T + 2 * kHaloD;
+    if (lu - cp.suppu < 0 || lu + cp.suppu >= pl || lv - cp.suppv < 0 || lv + cp.suppv >= pl) return kDegList;
+    const int delui = (int)roundf((float)cp.osampu * ((float)nuc - ucell));
+    const int delvi = (int)roundf((float)cp.osampv * ((float)nvc - vcell));
+    const int poffu = (cp.osampu + 1) / 2, poffv = (cp.osampv + 1) / 2;
+    const bool conj = (double)cp.uscale * ul > 0.0;
+    r.idx = i | (conj ? 0x80000000u : 0u);
+    r.pk = (uint32_t)lu | ((uint32_t)lv << 8) | ((uint32_t)(delui + poffu) << 16) | ((uint32_t)(delvi + poffv) << 24);
+    if (cp.doshift) {
+        const double phase = cp.twopi * (wl - wg);
+        r.rr = (float)cos(phase);
+        r.ri = (float)sin(phase);
+    }
+    return kDegTile;
+}
+
+// Append the lanes with flag set to list (one atomic per wave).
+__device__ __forceinline__ void wave_append(bool flag, int value, int *list, int *nlist) {
+    const unsigned long long m = __ballot(flag);
+    if (m) {
+        const int lane = threadIdx.x & 63;
+        const int leader = __ffsll((long long)m) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(nlist, __popcll(m));
+        base = __shfl(base, leader);
+        if (flag) list[base + __popcll(m & ((1ull << lane) - 1))] = value;
+    }
+}
 
 // One thread per visibility (optionally over an index list): the reference
 // loop.  vis_in supplies the output of visibilities with WT <= 0.
@@ -124,18 +215,16 @@ void launch_degrid(hipStream_t s, const float2 *grid, const float *u, const floa
                    const CoordParams &cp, const float *cfu, const float *cfv, float2 *vis_out,
                    float *wt_out, unsigned long long *nbad, const int *list = nullptr,
                    const int *nlist = nullptr);
-// Tile path: records at the cell-sorted positions (points near the u = 0
-// axis, or not in the gridding sort, go to the list for launch_degrid).
-// use_pos: take the sorted positions from pos_of (a gridding fill on the same
-// sort ran before), else from the counting-sort offsets.
-void launch_degrid_fill(hipStream_t s, const float *u, const float *v, const float *w, const float *wt_in,
-                        const float2 *vis_in, int64_t nvis, const CoordParams &cp, const TileParams &tp,
-                        const SortBufs &b, int use_pos, DegridRec *drec, float2 *drot, int *list, int *nlist,
-                        float2 *vis_out, float *wt_out);
-int launch_degrid_tiles(hipStream_t s, const DegridRec *drec, const float2 *drot, const WorkItem *items,
-                        const int *nitems, int max_items, const TileParams &tp, const float2 *grid,
-                        const float *cfu, const float *cfv, const CoordParams &cp, float2 *vis_out,
-                        float *wt_out, unsigned long long *nbad);
+// Tile path: degridding halves at the cell-sorted positions when no
+// gridding fill wrote them (points near the u = 0 axis, or not in the cell
+// sort, go to the list for launch_degrid).
+void launch_degrid_fill(hipStream_t s, const float *u, const float *v, const float *w, const float2 *vis_in,
+                        int64_t nvis, const CoordParams &cp, const TileParams &tp, const SortBufs &b,
+                        VisRec *rec, const DegridOut &deg);
+int launch_degrid_tiles(hipStream_t s, const VisRec *rec, const WorkItem *items, const int *nitems,
+                        int max_items, const TileParams &tp, const float2 *grid, const float *cfu,
+                        const float *cfv, const CoordParams &cp, float2 *vis_out, float *wt_out,
+                        unsigned long long *nbad);
 void launch_wt_density(hipStream_t s, const float *wt, const float *u, const float *v,
                        const float *w, int64_t nvis, const CoordParams &cp, double *gwt,
                        double *blk, int nblk);
