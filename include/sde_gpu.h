@@ -166,6 +166,9 @@ int sde_plan_uvmap(sde_plan *plan);
 int sde_plan_vismodel(sde_plan *plan, const float *image);
 int sde_plan_sync(sde_plan *plan);
 int sde_plan_get_images(sde_plan *plan, float *dirty, float *psf);
+/* The gridded half planes GVIS(NU,NV) of the last uvmap (dirty, PSF), as
+ * GRDH2D returns them: folded, shifted by SUPPU and normalised, before the
+ * FFT.  Complex, NU*NV per plane (either pointer may be NULL). */
 int sde_plan_get_grids(sde_plan *plan, float *dirty_grid, float *psf_grid);
 int sde_plan_get_weights(sde_plan *plan, float *wt);
 int sde_plan_get_model(sde_plan *plan, float *vis, float *wt);

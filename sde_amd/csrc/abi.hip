@@ -811,7 +811,7 @@ void fftr2d_(float *rarray, float *xarray, float *work, float *work2, float *rwo
     }
     if (*dir < 0) {
         SDE_F(hipMemcpyAsync(dg, xarray, sizeof(float2) * ng, hipMemcpyHostToDevice, s));
-        launch_fft_cols(s, dg, nu, *nv, 1, 0, -1, 1.0f / (float)*nv, tw_nv);
+        launch_fft_cols(s, dg, dg, nu, *nv, 1, 0, -1, 1.0f / (float)*nv, tw_nv);
         launch_fft_rows_c2r(s, dg, nu, *nv, *fnu, *nx, *ny, tw_fnu, nullptr, 1, 0, 1.0f / (float)*fnu, dimg,
                             nullptr, nullptr);
         SDE_F(hipGetLastError());
@@ -820,7 +820,7 @@ void fftr2d_(float *rarray, float *xarray, float *work, float *work2, float *rwo
     } else {
         SDE_F(hipMemcpyAsync(dimg, rarray, sizeof(float) * npix, hipMemcpyHostToDevice, s));
         launch_fft_rows_r2c(s, dimg, *nx, *ny, *fnu, *nv, tw_fnu, nullptr, 1, 0, dg);
-        launch_fft_cols(s, dg, nu, *nv, 1, 0, +1, 1.0f, tw_nv);
+        launch_fft_cols(s, dg, dg, nu, *nv, 1, 0, +1, 1.0f, tw_nv);
         SDE_F(hipGetLastError());
         SDE_F(hipMemcpyAsync(xarray, dg, sizeof(float2) * ng, hipMemcpyDeviceToHost, s));
     }
@@ -847,7 +847,7 @@ void fftx1d_(float *xarray, float *xwork, float *xwork2, int *nx, int *dir) {
         return;
     }
     SDE_F(hipMemcpyAsync(dg, xarray, sizeof(float2) * *nx, hipMemcpyHostToDevice, s));
-    launch_fft_cols(s, dg, 1, *nx, 1, 0, *dir, *dir < 0 ? 1.0f / (float)*nx : 1.0f, tw);
+    launch_fft_cols(s, dg, dg, 1, *nx, 1, 0, *dir, *dir < 0 ? 1.0f / (float)*nx : 1.0f, tw);
     SDE_F(hipGetLastError());
     SDE_F(hipMemcpyAsync(xarray, dg, sizeof(float2) * *nx, hipMemcpyDeviceToHost, s));
     SDE_F(hipStreamSynchronize(s));
@@ -883,7 +883,7 @@ struct sde_plan {
     float *d_partial = nullptr;
     size_t partial_cap = 0, items_cap = 0;
     double *d_blk = nullptr, *d_sums = nullptr;
-    float2 *d_grids = nullptr, *d_mgrid = nullptr;
+    float2 *d_grids = nullptr, *d_gwork = nullptr, *d_mgrid = nullptr;
     float *d_dirty = nullptr, *d_psf = nullptr, *d_img_in = nullptr;
     int *d_peak = nullptr;
     double *d_gwt = nullptr, *d_wblk = nullptr, *d_wsums = nullptr;
@@ -1050,7 +1050,7 @@ int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **out) {
     make_ccf(cft, ccf.data(), p->ccflen);
     const size_t ng = (size_t)p->nu * p->nv, npix = (size_t)cfg->nx * cfg->ny;
     const int64_t nkeys = sort_nkeys(p->tp);
-    int rc = plan_alloc(&p->d_cf, p->ncf) || plan_alloc(&p->d_ccf, p->ccflen) || plan_alloc(&p->d_grids, 2 * ng) ||
+    int rc = plan_alloc(&p->d_cf, p->ncf) || plan_alloc(&p->d_ccf, p->ccflen) || plan_alloc(&p->d_grids, 2 * ng) || plan_alloc(&p->d_gwork, 2 * ng) ||
              plan_alloc(&p->d_mgrid, ng) || plan_alloc(&p->d_dirty, npix) || plan_alloc(&p->d_psf, npix) ||
              plan_alloc(&p->d_img_in, npix) || plan_alloc(&p->d_peak, 4) || plan_alloc(&p->d_gwt, ng) ||
              plan_alloc(&p->d_wblk, 4096) || plan_alloc(&p->d_wsums, 4) || plan_alloc(&p->d_cnt, 4) ||
@@ -1089,7 +1089,7 @@ int sde_plan_destroy(sde_plan *p) {
     void *bufs[] = {p->d_cf, p->d_ccf, p->d_vis, p->d_wt, p->d_u, p->d_v, p->d_w, p->d_wteff, p->d_mvis, p->d_mwt,
                     p->sb.cnt, p->sb.start, p->sb.bsum, p->sb.key_of, p->sb.rank_of, p->sb.tile_start,
                     p->sb.item_start, p->sb.nitems, p->sb.items, p->d_rec, p->d_list, p->d_nlist,
-                    p->d_partial, p->d_blk, p->d_sums, p->d_grids, p->d_mgrid, p->d_dirty, p->d_psf, p->d_img_in,
+                    p->d_partial, p->d_blk, p->d_sums, p->d_grids, p->d_gwork, p->d_mgrid, p->d_dirty, p->d_psf, p->d_img_in,
                     p->d_peak, p->d_gwt, p->d_wblk, p->d_wsums, p->d_cnt, p->d_wsq};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -1174,16 +1174,18 @@ int sde_plan_uvmap(sde_plan *p) {
     p->filled = true;
     float2 *twnv = p->tw[p->nv], *twfnu = p->tw[p->fnu];
     plan_mark_begin(p, "fft_cols");
-    launch_fft_cols(s, p->d_grids, p->nu, p->nv, 2, ng, -1, 1.0f, twnv);
+    // out of place: the gridded planes stay readable (sde_plan_get_grids)
+    launch_fft_cols(s, p->d_grids, p->d_gwork, p->nu, p->nv, 2, ng, -1, 1.0f, twnv);
     plan_mark_end(p);
+    const float2 *cd = p->d_gwork, *cpsf = p->d_gwork + ng;
     SDE_CHECK(hipMemsetAsync(p->d_peak, 0x80, sizeof(int), s));
     const float scale = 1.0f / ((float)p->fnu * (float)p->nv);
     plan_mark_begin(p, "fft_rows_psf");
-    launch_fft_rows_c2r(s, gp, p->nu, p->nv, p->fnu, p->cfg.nx, p->cfg.ny, twfnu, p->d_ccf, p->ccflen, 1, scale,
+    launch_fft_rows_c2r(s, cpsf, p->nu, p->nv, p->fnu, p->cfg.nx, p->cfg.ny, twfnu, p->d_ccf, p->ccflen, 1, scale,
                         p->d_psf, p->d_peak, nullptr);
     plan_mark_end(p);
     plan_mark_begin(p, "fft_rows_dirty");
-    launch_fft_rows_c2r(s, gd, p->nu, p->nv, p->fnu, p->cfg.nx, p->cfg.ny, twfnu, p->d_ccf, p->ccflen, 1, scale,
+    launch_fft_rows_c2r(s, cd, p->nu, p->nv, p->fnu, p->cfg.nx, p->cfg.ny, twfnu, p->d_ccf, p->ccflen, 1, scale,
                         p->d_dirty, nullptr, p->d_peak);
     plan_mark_end(p);
     plan_mark_begin(p, "psf_scale");
@@ -1207,7 +1209,7 @@ int sde_plan_vismodel(sde_plan *p, const float *image) {
     launch_fft_rows_r2c(s, img, p->cfg.nx, p->cfg.ny, p->fnu, p->nv, twfnu, p->d_ccf, p->ccflen, 1, p->d_mgrid);
     plan_mark_end(p);
     plan_mark_begin(p, "fft_cols_model");
-    launch_fft_cols(s, p->d_mgrid, p->nu, p->nv, 1, 0, +1, 1.0f, twnv);
+    launch_fft_cols(s, p->d_mgrid, p->d_mgrid, p->nu, p->nv, 1, 0, +1, 1.0f, twnv);
     plan_mark_end(p);
     if (p->nvis <= 0) return 0;
     if (!p->sorted && plan_sort(p)) return -1;

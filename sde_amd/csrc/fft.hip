@@ -105,10 +105,13 @@ __device__ void lds_fft(float2 *x, int n, int log2n, int tid, int nthr, const fl
 __device__ __forceinline__ int ilog2(int n) { return 31 - __clz(n); }
 
 // C2C along v of CB columns per workgroup; dir < 0: exp(+i), dir > 0: exp(-i).
-__global__ void k_fft_cols(float2 *__restrict__ grids, int nu, int nv, size_t stride, int cb,
+// src and dst may be the same array (in place): a block reads all of its
+// columns into LDS before it writes any.
+__global__ void k_fft_cols(const float2 *srcs, float2 *dsts, int nu, int nv, size_t stride, int cb,
                            int tpc, int dir, float scale, const float2 *__restrict__ tw) {
     extern __shared__ float2 xs[];
-    float2 *grid = grids + stride * blockIdx.y;
+    const float2 *grid = srcs + stride * blockIdx.y;
+    float2 *out = dsts + stride * blockIdx.y;
     const int u0 = blockIdx.x * cb;
     const int icen = (nv + 1) / 2 - 1;  // 0-based centre row
     const int nthr = blockDim.x;
@@ -130,7 +133,7 @@ __global__ void k_fft_cols(float2 *__restrict__ grids, int nu, int nv, size_t st
         if (u < nu) {
             float2 r = xs[cc * nv + p];
             if (scale != 1.0f) r = make_float2(r.x * scale, r.y * scale);
-            grid[(size_t)vrow * nu + u] = r;
+            out[(size_t)vrow * nu + u] = r;
         }
     }
 }
@@ -305,8 +308,8 @@ int row_threads(int h) {
 
 int fft_supported(int n) { return n >= 2 && n <= 8192 && (n & (n - 1)) == 0; }
 
-void launch_fft_cols(hipStream_t s, float2 *grid, int nu, int nv, int ngrids, size_t grid_stride,
-                     int dir, float scale, const float2 *tw_nv) {
+void launch_fft_cols(hipStream_t s, const float2 *src, float2 *dst, int nu, int nv, int ngrids,
+                     size_t grid_stride, int dir, float scale, const float2 *tw_nv) {
     int tpc = nv / 16;
     if (tpc < 64) tpc = 64;
     int cb = 65536 / (nv * (int)sizeof(float2));
@@ -315,7 +318,7 @@ void launch_fft_cols(hipStream_t s, float2 *grid, int nu, int nv, int ngrids, si
     while (cb > 1 && cb * tpc > 1024) cb /= 2;
     const size_t lds = sizeof(float2) * (size_t)cb * nv;
     dim3 g((nu + cb - 1) / cb, ngrids);
-    hipLaunchKernelGGL(k_fft_cols, g, dim3(cb * tpc), lds, s, grid, nu, nv, grid_stride, cb, tpc,
+    hipLaunchKernelGGL(k_fft_cols, g, dim3(cb * tpc), lds, s, src, dst, nu, nv, grid_stride, cb, tpc,
                        dir, scale, tw_nv);
 }
 

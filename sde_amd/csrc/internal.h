@@ -118,9 +118,10 @@ void launch_box_fold(hipStream_t s, const double *acc, const CoordParams &cp,
 // ---- FFT (fft.hip) ----
 // twiddle table: tw[k] = exp(+2 pi i k / n), k < n, as float2
 int fft_supported(int n);
-// scale multiplies the column output (1 in the fused pipeline, 1/NV for FFTR2D)
-void launch_fft_cols(hipStream_t s, float2 *grid, int nu, int nv, int ngrids, size_t grid_stride,
-                     int dir, float scale, const float2 *tw_nv);
+// scale multiplies the column output (1 in the fused pipeline, 1/NV for FFTR2D);
+// dst may equal src (in place)
+void launch_fft_cols(hipStream_t s, const float2 *src, float2 *dst, int nu, int nv, int ngrids,
+                     size_t grid_stride, int dir, float scale, const float2 *tw_nv);
 // DIR < 0 rows: grid rows -> image (NX,NY) with centring, crop, correction,
 // scale, optional peak tracking and optional division by a device peak.
 void launch_fft_rows_c2r(hipStream_t s, const float2 *grid, int nu, int nv, int fnu, int nx,

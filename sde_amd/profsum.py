@@ -1,0 +1,97 @@
+"""Summary of rocprofv3 runs of bench.py: per-kernel duration and HBM traffic.
+
+    python -m sde_amd.profsum OUT "<command line>" > profiles/<name>.md
+
+OUT holds three rocprofv3 output folders of the same command:
+    OUT/trace  --kernel-trace --stats
+    OUT/fetch  --pmc FETCH_SIZE
+    OUT/write  --pmc WRITE_SIZE
+each written with ``-o run`` (rocpd SQLite databases, run_results.db).
+
+FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are calibrated against the
+plain device copy that ``bench.py --copy-probe`` launches (k_copy, 1 GiB
+read + 1 GiB written): factor = 1 GiB / counter reading of that copy.  The
+table gives, per kernel, the dispatch count, the average duration, the
+calibrated read and written megabytes per dispatch and the resulting
+bandwidth.  bench.py reads the "hbm_bytes" column for its roofline.
+"""
+import collections
+import glob
+import os
+import re
+import sqlite3
+import sys
+
+
+def short_name(name):
+    name = re.sub(r"\(anonymous namespace\)::", "", name)
+    name = re.sub(r"^void ", "", name)
+    return name.split("(")[0].strip()
+
+
+def _db(folder):
+    hits = glob.glob(os.path.join(folder, "**", "*_results.db"), recursive=True)
+    if not hits:
+        raise SystemExit("no rocpd database under %s" % folder)
+    return hits[0]
+
+
+def trace(folder):
+    c = sqlite3.connect(_db(folder))
+    rows = c.execute("select name, count(*), avg(end - start) from kernels group by name").fetchall()
+    return {short_name(r[0]): (r[1], r[2] / 1e6) for r in rows}
+
+
+def counters(folder):
+    c = sqlite3.connect(_db(folder))
+    acc = collections.defaultdict(lambda: collections.defaultdict(list))
+    for name, ctr, val in c.execute("select kernel_name, counter_name, value from counters_collection"):
+        acc[short_name(name)][ctr].append(val)
+    return {k: {ctr: sum(v) / len(v) for ctr, v in d.items()} for k, d in acc.items()}
+
+
+def summarize(out, command):
+    t = trace(os.path.join(out, "trace"))
+    f = counters(os.path.join(out, "fetch"))
+    w = counters(os.path.join(out, "write"))
+    cf = f.get("k_copy", {}).get("FETCH_SIZE")
+    cw = w.get("k_copy", {}).get("WRITE_SIZE")
+    kf = 1048576.0 / cf if cf else 1.0
+    kw = 1048576.0 / cw if cw else 1.0
+    lines = ["# Kernel durations and HBM traffic", "", "Command (from the repository root):", "", "```"]
+    lines += [command, "```", ""]
+    lines.append("Copy calibration (k_copy, 1 GiB read + 1 GiB written per dispatch): FETCH_SIZE %.0f KiB, "
+                 "WRITE_SIZE %.0f KiB; factors fetch x%.3f, write x%.3f." % (cf or 0, cw or 0, kf, kw))
+    lines.append("")
+    lines.append("| kernel | dispatches | avg_ms | read_MB | written_MB | hbm_bytes | GB/s |")
+    lines.append("|---|---|---|---|---|---|---|")
+    for k, (n, avg) in sorted(t.items(), key=lambda kv: -kv[1][0] * kv[1][1]):
+        rd = f.get(k, {}).get("FETCH_SIZE", 0.0) * kf * 1024.0
+        wr = w.get(k, {}).get("WRITE_SIZE", 0.0) * kw * 1024.0
+        gbs = (rd + wr) / (avg * 1e-3) / 1e9 if avg > 0 else 0.0
+        lines.append("| %s | %d | %.4f | %.1f | %.1f | %.0f | %.0f |" % (k, n, avg, rd / 1e6, wr / 1e6, rd + wr, gbs))
+    return "\n".join(lines) + "\n"
+
+
+def read_table(path):
+    """Rows of a summary written by summarize(): kernel -> dict of columns."""
+    out = {}
+    if not os.path.exists(path):
+        return out
+    with open(path) as fh:
+        for line in fh:
+            p = [x.strip() for x in line.strip().strip("|").split("|")]
+            if len(p) != 7 or p[0] in ("kernel", "---"):
+                continue
+            try:
+                out[p[0]] = {"dispatches": int(p[1]), "avg_ms": float(p[2]), "read_MB": float(p[3]),
+                             "written_MB": float(p[4]), "hbm_bytes": float(p[5])}
+            except ValueError:
+                continue
+    return out
+
+
+if __name__ == "__main__":
+    if len(sys.argv) < 3:
+        raise SystemExit(__doc__)
+    sys.stdout.write(summarize(sys.argv[1], sys.argv[2]))
