@@ -93,26 +93,37 @@ def read_traffic(region, workload_tag):
 
 
 def cpu_baseline(vis, wt, u, v, w, nx, cell, nfull):
-    """The FP32 C restatement of the reference path (oracle/ref32.c), one core,
-    on two bounded samples of the same workload; the per-visibility and
-    fixed costs fitted from the two are extrapolated to the full job."""
+    """The reference path on one host core, on two bounded samples of the
+    same workload; the per-visibility and fixed costs fitted from the two
+    are extrapolated to the full job.  Runs the reference's own routines
+    compiled from its Fortran sources (oracle/_ref, -O2) in task/uvmap.f's
+    and task/vismodel.f's call sequences when that build is present, else
+    the C restatement (oracle/ref32.c)."""
     from oracle import pyoracle as po
-    cfg = po.make_cfg(nx, cellsize=cell, fov=1.0, rmode="NORM", robust=(0.0, 0.0, 0.0))
+    from oracle import refwrap as rw
+    kind = "reference" if rw.available() else "port"
     t = []
     sizes = (4_000_000, 32_000_000)
     for n in sizes:
         t0 = time.perf_counter()
-        dirty, psf, sw, rc, _ = po.uvmap(cfg, vis[:n], wt[:n], u[:n], v[:n], w[:n])
-        po.vismodel(cfg, dirty, wt[:n], u[:n], v[:n], w[:n])
+        if kind == "reference":
+            dirty, psf, sw, _ = rw.uvmap(nx, cell, vis[:n], wt[:n], u[:n], v[:n], w[:n], fov=1.0, rmode="NORM",
+                                         robust=(0.0, 0.0, 0.0))
+            rw.vismodel(nx, cell, dirty, wt[:n], u[:n], v[:n], w[:n])
+        else:
+            cfg = po.make_cfg(nx, cellsize=cell, fov=1.0, rmode="NORM", robust=(0.0, 0.0, 0.0))
+            dirty, psf, sw, rc, _ = po.uvmap(cfg, vis[:n], wt[:n], u[:n], v[:n], w[:n])
+            po.vismodel(cfg, dirty, wt[:n], u[:n], v[:n], w[:n])
         t.append(time.perf_counter() - t0)
     per_vis = (t[1] - t[0]) / (sizes[1] - sizes[0])
     fixed = max(t[0] - per_vis * sizes[0], 0.0)
     t_full = fixed + per_vis * nfull
-    return {"value": 2.0 * nfull / t_full, "unit": "visibilities/s", "cores": 1, "kind": "port",
-            "sample": "oracle/ref32.c uvmap+vismodel (robust NORM, 4096^2) on the first %d and %d visibilities "
-                      "of the workload: %.2f s and %.2f s; fitted %.3f us/visibility + %.2f s fixed, "
-                      "extrapolated to %d visibilities" % (sizes[0], sizes[1], t[0], t[1], per_vis * 1e6, fixed,
-                                                           nfull)}
+    what = ("the reference's GRDRUW2D, GRDH2D x2, FFTR2D, PIXCOR2D, GRDDEH2D compiled from source (flang -O2, "
+            "oracle/_ref)" if kind == "reference" else "oracle/ref32.c")
+    return {"value": 2.0 * nfull / t_full, "unit": "visibilities/s", "cores": 1, "kind": kind,
+            "sample": "%s: uvmap+vismodel (robust NORM, 4096^2) on the first %d and %d visibilities of the "
+                      "workload: %.2f s and %.2f s; fitted %.3f us/visibility + %.2f s fixed, extrapolated to %d "
+                      "visibilities" % (what, sizes[0], sizes[1], t[0], t[1], per_vis * 1e6, fixed, nfull)}
 
 
 def accuracy(plan, vis, wt, u, v, w, nx, cell):

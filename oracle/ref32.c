@@ -275,7 +275,9 @@ int r32_grdh2d(const float *vis, const float *wt, const float *u, const float *v
             for (int offu = -suppu; offu <= suppu; offu++) {
                 int delu = delui + osampu * offu;
                 int ugrid = ucen + offu;
-                double uvwt = (double)wt[i] * (double)gridfnu[delu - 1] * (double)gridfnv[delv - 1];
+                /* UVWT (DOUBLE) = WT*GRIDFNU*GRIDFNV: a REAL expression, rounded
+                 * in single precision before the assignment (grd/grdh2d.f:154) */
+                double uvwt = (double)(wt[i] * gridfnu[delu - 1] * gridfnv[delv - 1]);
                 GRE(gvis, nu, ugrid, vgrid) = (float)((double)GRE(gvis, nu, ugrid, vgrid) + uvwt * (double)fr);
                 GIM(gvis, nu, ugrid, vgrid) = (float)((double)GIM(gvis, nu, ugrid, vgrid) + uvwt * (double)fi);
                 lsumwt = lsumwt + uvwt;
@@ -299,8 +301,11 @@ int r32_grdh2d(const float *vis, const float *wt, const float *u, const float *v
     lsumwt = (double)((float)(nu - 1) * (float)nv) / lsumwt;
     for (int vg = 1; vg <= nv; vg++) {
         for (int ug = 1; ug <= nu - suppu; ug++) {
-            GRE(gvis, nu, ug, vg) = (float)(lsumwt * (double)GRE(gvis, nu, ug + suppu, vg));
-            GIM(gvis, nu, ug, vg) = (float)(lsumwt * (double)GIM(gvis, nu, ug + suppu, vg));
+            /* DOUBLE * COMPLEX: a complex product with (LSUMWT, 0), whose
+             * zero terms decide the sign of zero results */
+            const double a = GRE(gvis, nu, ug + suppu, vg), b = GIM(gvis, nu, ug + suppu, vg);
+            GRE(gvis, nu, ug, vg) = (float)(lsumwt * a - 0.0 * b);
+            GIM(gvis, nu, ug, vg) = (float)(lsumwt * b + 0.0 * a);
         }
         for (int ug = nu - suppu + 1; ug <= nu; ug++) GRE(gvis, nu, ug, vg) = GIM(gvis, nu, ug, vg) = 0.0f;
     }
@@ -340,8 +345,9 @@ int r32_grdqh2d(const float *vis, const float *wt, const float *u, const float *
         int ugrid = nintf(ucell) + 1, vgrid = nintf(vcell);
         if (ugrid < 1 || ugrid > nu || vgrid < 1 || vgrid > nv) continue;
         float uvwt = wt[i];
-        GRE(gvis, nu, ugrid, vgrid) = GRE(gvis, nu, ugrid, vgrid) + uvwt * fr;
-        GIM(gvis, nu, ugrid, vgrid) = GIM(gvis, nu, ugrid, vgrid) + uvwt * fi;
+        /* REAL * COMPLEX: a complex product with (UVWT, 0) */
+        GRE(gvis, nu, ugrid, vgrid) = GRE(gvis, nu, ugrid, vgrid) + (uvwt * fr - 0.0f * fi);
+        GIM(gvis, nu, ugrid, vgrid) = GIM(gvis, nu, ugrid, vgrid) + (uvwt * fi + 0.0f * fr);
         lsumwt = lsumwt + uvwt;
     }
     int uz = uorigin + 1, vz = vorigin;
@@ -355,12 +361,29 @@ int r32_grdqh2d(const float *vis, const float *wt, const float *u, const float *
     lsumwt = (float)(nu - 1) * (float)nv / lsumwt;
     for (int vg = 1; vg <= nv; vg++) {
         for (int ug = 1; ug <= nu - 1; ug++) {
-            GRE(gvis, nu, ug, vg) = lsumwt * GRE(gvis, nu, ug + 1, vg);
-            GIM(gvis, nu, ug, vg) = lsumwt * GIM(gvis, nu, ug + 1, vg);
+            const float a = GRE(gvis, nu, ug + 1, vg), b = GIM(gvis, nu, ug + 1, vg);
+            GRE(gvis, nu, ug, vg) = lsumwt * a - 0.0f * b;
+            GIM(gvis, nu, ug, vg) = lsumwt * b + 0.0f * a;
         }
         GRE(gvis, nu, nu, vg) = GIM(gvis, nu, nu, vg) = 0.0f;
     }
     return 0;
+}
+
+/* COMPLEX / REAL: Fortran converts the REAL divisor to COMPLEX and the
+ * compiler (LLVM flang) emits a call of the C99 Annex G complex division
+ * __divsc3 of LLVM's compiler-rt (lib/builtins/divsc3.c).  That routine
+ * scales the divisor by 2^-logb(max|c|,|d|), forms the quotient as
+ * (a*c' + b*d') / (c'^2 + d'^2) and scales back, so its rounding differs
+ * from re/s, im/s in the last place.  Restated here for a finite, non-zero
+ * real divisor (d = 0), the only case GRDDEH2D reaches (SUMWT > 0). */
+static void cdiv_by_real(float *re, float *im, float s) {
+    const int e = ilogbf(s);
+    const float c = scalbnf(s, -e);
+    const float denom = c * c + 0.0f * 0.0f;
+    const float a = *re, b = *im;
+    *re = scalbnf((a * c + b * 0.0f) / denom, -e);
+    *im = scalbnf((b * c - a * 0.0f) / denom, -e);
 }
 
 /* grd/grddeh2d.f:84-203 */
@@ -448,8 +471,8 @@ void r32_grddeh2d(float *vis, float *wt, const float *u, const float *v, const f
             }
         }
         if (sumwt > 0.0f) {
-            vr = vr / sumwt;
-            vi = vi / sumwt;
+            /* VIS/SUMWT: COMPLEX / REAL, a complex division by (SUMWT, 0) */
+            cdiv_by_real(&vr, &vi, sumwt);
             wt[i] = fabsf(wt[i]);
         } else {
             (*nbad)++;
@@ -625,13 +648,16 @@ void r32_pixcorx2(const float *in, int nx, int ny, const float *cf, int cflen,
             int icfx = 1 + abs(nintf(sx * ((float)ix - cenx)));
             size_t k = 2 * ((size_t)(iy - 1) * nx + ix - 1);
             float c = cf[icfx - 1] * cf[icfy - 1];
+            float a = in[k], b = in[k + 1];
             if (correct) {
-                /* complex / real: componentwise */
-                out[k] = in[k] / c;
-                out[k + 1] = in[k + 1] / c;
+                /* COMPLEX / REAL: complex division by (c, 0) */
+                cdiv_by_real(&a, &b, c);
+                out[k] = a;
+                out[k + 1] = b;
             } else {
-                out[k] = c * in[k];
-                out[k + 1] = c * in[k + 1];
+                /* REAL * COMPLEX: complex product with (c, 0) */
+                out[k] = c * a - 0.0f * b;
+                out[k + 1] = c * b + 0.0f * a;
             }
         }
     }

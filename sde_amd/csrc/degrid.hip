@@ -102,8 +102,7 @@ __global__ __launch_bounds__(256) void k_degrid(
         }
     }
     if (sumwt > 0.0f) {
-        vr = vr / sumwt;
-        vi = vi / sumwt;
+        cdiv_by_real(vr, vi, sumwt);
         wti = fabsf(wti);
     } else {
         atomicAdd(nbad, 1ull);
@@ -208,8 +207,7 @@ __global__ __launch_bounds__(256) void k_degrid_tiles(const VisRec *__restrict__
         }
         const uint32_t idx = d.idx & 0x7fffffffu;
         if (sumwt > 0.0f) {
-            vr = vr / sumwt;
-            vi = vi / sumwt;
+            cdiv_by_real(vr, vi, sumwt);
         } else {
             atomicAdd(nbad, 1ull);
             vr = vi = 0.0f;

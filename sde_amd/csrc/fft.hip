@@ -270,8 +270,17 @@ __global__ void k_pixcor(const float *__restrict__ in, float *__restrict__ out, 
     const float c = cf[icf(cpar.sx, ix + 1, cpar.cenx)] * cf[icf(cpar.sy, iy + 1, cpar.ceny)];
     const size_t k = (size_t)iy * nx + ix;
     if (cplx) {
+        // PIXCORX2 (pix/pixcorx2.f:57,65): COMPLEX / REAL is a complex division
+        // by (c, 0); REAL * COMPLEX a complex product with (c, 0)
         const float2 a = reinterpret_cast<const float2 *>(in)[k];
-        reinterpret_cast<float2 *>(out)[k] = correct ? make_float2(a.x / c, a.y / c) : make_float2(c * a.x, c * a.y);
+        float re = a.x, im = a.y;
+        if (correct) {
+            cdiv_by_real(re, im, c);
+        } else {
+            re = c * a.x - 0.0f * a.y;
+            im = c * a.y + 0.0f * a.x;
+        }
+        reinterpret_cast<float2 *>(out)[k] = make_float2(re, im);
     } else {
         out[k] = correct ? in[k] / c : c * in[k];
     }

@@ -146,6 +146,21 @@ constexpr int kScanShift = 13;
 
 enum DegridPath { kDegTile = 0, kDegList = 1, kDegOff = 2 };
 
+// VIS = VIS/SUMWT of GRDDEH2D (grd/grddeh2d.f:180): COMPLEX / REAL, which
+// Fortran evaluates as a complex division by (SUMWT, 0) and the reference's
+// compiler lowers to the C99 Annex G routine of LLVM's compiler-rt
+// (lib/builtins/divsc3.c): the divisor scaled by 2^-logb, the quotient
+// (a*c' + b*0) / (c'*c' + 0*0), scaled back.  Written out so the degridded
+// values keep the reference's rounding (not re/s, im/s).  s > 0, finite.
+__device__ __forceinline__ void cdiv_by_real(float &re, float &im, float s) {
+    const int e = ilogbf(s);
+    const float c = scalbnf(s, -e);
+    const float denom = c * c + 0.0f * 0.0f;
+    const float a = re, b = im;
+    re = scalbnf((a * c + b * 0.0f) / denom, -e);
+    im = scalbnf((b * c - a * 0.0f) / denom, -e);
+}
+
 // The GRDDEH2D coordinate step of one point with WT > 0 (grd/grddeh2d.f:
 // 107-139, REAL cell from the DOUBLE rotated coordinates) and the degridding
 // half of its record.  key is the point's cell-sort key (< 0: not sorted).
