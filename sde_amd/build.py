@@ -10,7 +10,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = [os.path.join(HERE, "csrc", f) for f in ("abi.hip", "grid.hip", "fft.hip", "degrid.hip")]
+SRC = [os.path.join(HERE, "csrc", f) for f in ("abi.hip", "grid.hip", "fft.hip", "degrid.hip", "sort.hip")]
 OUT = os.path.join(HERE, "libsdegpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -20,6 +20,8 @@ FLAGS = [
     # no packed FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
+    # every symbol resolved at link time
+    "-Wl,--no-undefined",
 ]
 
 

@@ -355,26 +355,40 @@ int max_items_for(const TileParams &tp, int64_t nvis) {
 using MarkB = std::function<void(const char *)>;
 using MarkE = std::function<void()>;
 
-// Cell sort of the visibilities (counting sort by tile-major cell).
-int run_sort(hipStream_t s, const float *wt, const float *u, const float *v, const float *w, int64_t nvis,
-             const CoordParams &cp, const TileParams &tp, SortBufs &sb, const MarkB &mb, const MarkE &me) {
-    mb("cell_sort");
-    launch_sort(s, wt, u, v, w, nvis, cp, tp, sb);
+// Cell sort of the visibilities: the prep pass (keys, staging records and,
+// when asked, the density exceptions and the per-point degridding
+// outputs), the radix sort, tile ranges and work items.
+int run_sort(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v, const float *w,
+             int64_t nvis, const CoordParams &cp, const TileParams &tp, SortBufs &sb, const DensityOut *dens,
+             const DegridOut *deg, const MarkB &mb, const MarkE &me) {
+    mb("prep");
+    launch_prep(s, vis, wt, u, v, w, nvis, cp, tp, sb, dens, deg);
     me();
+    mb("cell_sort");
+    const int rc = launch_sort(s, sb, nvis, tp);
+    me();
+    if (rc) {
+        set_plan_error("cell sort failed");
+        return -1;
+    }
     SDE_CHECK(hipGetLastError());
     return 0;
 }
 
-// Gridding on a cell sort: records, LDS tiles, reduce (fold, shift, normalise).
-// With deg.wt set, the fill also writes the degridding halves of the records
-// (a prediction on the same visibilities then needs no fill of its own).
-int run_grid(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v, const float *w,
-             int64_t nvis, const CoordParams &cp, const TileParams &tp, int chmask, const float *cfu,
-             const float *cfv, const SortBufs &sb, VisRec *rec, float *partial, double *blk, double *sums,
-             float2 *dirty, float2 *psf, const DegridOut &deg, const MarkB &mb, const MarkE &me) {
-    mb("grid_fill");
-    launch_grid_fill(s, vis, wt, u, v, w, nvis, cp, tp, chmask == 2, cfu, cfv, sb, rec, blk, sums, deg);
+// Gridding on a cell sort: gathered records (with the degridding halves
+// when deg), LDS tiles, reduce (fold, shift, normalise).
+int run_grid(hipStream_t s, int64_t nvis, const CoordParams &cp, const TileParams &tp, int chmask, bool deg,
+             const WeightParams &wp, const float *cfu, const float *cfv, const SortBufs &sb, VisRec *rec,
+             float *partial, double *blk, double *sums, float2 *dirty, float2 *psf, const MarkB &mb,
+             const MarkE &me) {
+    const int flags = kGatherGrid | (chmask == 2 ? kGatherPsf : 0) | (deg ? kGatherDegrid : 0);
+    mb("grid_gather");
+    const int rc = launch_grid_gather(s, sb, nvis, cp, tp, flags, wp, cfu, cfv, rec, blk, sums);
     me();
+    if (rc) {
+        set_plan_error("unsupported record gather (flags %d)", flags);
+        return -1;
+    }
     mb("grid_tiles");
     if (launch_grid_tiles(s, rec, sb.items, sb.nitems, sb.max_items, tp, chmask, cfu, cfv, cp, partial)) {
         me();
@@ -489,31 +503,31 @@ void grdh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, flo
                                    *suppu, *suppv, *osampu, *osampv);
     const TileParams tp = make_tp(*nu, *nv, default_tile(*nu, *nv), default_chunk(n));
     const int chmask = *psf ? 2 : 1;
-    const int64_t nkeys = sort_nkeys(tp);
-    const int nblk = grid_fill_blocks(n);
+    const int nblk = gather_blocks(n);
     const size_t nn = (size_t)(n > 0 ? n : 1);
     SortBufs sb;
     sb.max_items = max_items_for(tp, n);
+    sb.temp_bytes = sort_temp_bytes(n > 0 ? n : 1, sort_nbits(tp));
     float2 *dvis = ctx_buf<float2>(c, "vis", nn);
     float *dwt = ctx_buf<float>(c, "wt", nn), *du = ctx_buf<float>(c, "u", nn);
     float *dv = ctx_buf<float>(c, "v", nn), *dw = ctx_buf<float>(c, "w", nn);
     float *dcfu = ctx_buf<float>(c, "cfu", ncfu), *dcfv = ctx_buf<float>(c, "cfv", ncfv);
-    sb.cnt = ctx_buf<int>(c, "cnt", nkeys);
-    sb.start = ctx_buf<int>(c, "start", nkeys);
-    sb.bsum = ctx_buf<int>(c, "bsum", sort_nblocks(tp));
-    sb.key_of = ctx_buf<int>(c, "key_of", nn);
-    sb.rank_of = ctx_buf<int>(c, "rank_of", nn);
+    sb.keys = ctx_buf<uint32_t>(c, "keys", nn);
+    sb.skeys = ctx_buf<uint32_t>(c, "skeys", nn);
+    sb.sidx = ctx_buf<uint32_t>(c, "sidx", nn);
+    sb.stage = ctx_buf<StageRec>(c, "stage", nn);
+    sb.temp = ctx_buf<char>(c, "sort_temp", sb.temp_bytes > 0 ? sb.temp_bytes : 1);
     sb.tile_start = ctx_buf<int>(c, "tile_start", tp.ntiles + 1);
     sb.item_start = ctx_buf<int>(c, "item_start", tp.ntiles + 1);
     sb.nitems = ctx_buf<int>(c, "nitems", 1);
     sb.items = ctx_buf<WorkItem>(c, "items", sb.max_items);
     VisRec *rec = ctx_buf<VisRec>(c, "rec", nn);
     float *partial = ctx_buf<float>(c, "partial", grid_partial_floats(tp, chmask, sb.max_items));
-    double *blk = ctx_buf<double>(c, "blk", 2 * nblk), *dsums = ctx_buf<double>(c, "sums", 4);
+    double *blk = ctx_buf<double>(c, "blk", 2 * (size_t)nblk), *dsums = ctx_buf<double>(c, "sums", 4);
     float2 *dg = ctx_buf<float2>(c, "grid", (size_t)*nu * *nv);
-    if (!dvis || !dwt || !du || !dv || !dw || !dcfu || !dcfv || !sb.cnt || !sb.start || !sb.bsum || !sb.key_of ||
-        !sb.rank_of || !sb.tile_start || !sb.item_start || !sb.nitems || !sb.items || !rec || !partial || !blk ||
-        !dsums || !dg) {
+    if (!dvis || !dwt || !du || !dv || !dw || !dcfu || !dcfv || !sb.keys || !sb.skeys || !sb.sidx || !sb.stage ||
+        !sb.temp || sb.temp_bytes == 0 || !sb.tile_start || !sb.item_start || !sb.nitems || !sb.items || !rec ||
+        !partial || !blk || !dsums || !dg) {
         report_error(kErrNoMem, R, "device allocation failed");
         return;
     }
@@ -537,9 +551,9 @@ void grdh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, flo
     SDE_F(hipMemcpyAsync(dcfv, gridfnv, sizeof(float) * ncfv, hipMemcpyHostToDevice, s));
     auto nb = [](const char *) {};
     auto ne = []() {};
-    if (run_sort(s, dwt, du, dv, dw, n, cp, tp, sb, nb, ne) ||
-        run_grid(s, dvis, dwt, du, dv, dw, n, cp, tp, chmask, dcfu, dcfv, sb, rec, partial, blk, dsums,
-                 chmask == 1 ? dg : nullptr, chmask == 2 ? dg : nullptr, DegridOut(), nb, ne)) {
+    if (run_sort(s, dvis, dwt, du, dv, dw, n, cp, tp, sb, nullptr, nullptr, nb, ne) ||
+        run_grid(s, n, cp, tp, chmask, false, WeightParams(), dcfu, dcfv, sb, rec, partial, blk, dsums,
+                 chmask == 1 ? dg : nullptr, chmask == 2 ? dg : nullptr, nb, ne)) {
         report_error(kErrFatal, R, sde_last_error());
         return;
     }
@@ -875,10 +889,12 @@ struct sde_plan {
     float2 *d_mvis = nullptr;
     float *d_mwt = nullptr;
     SortBufs sb{};
-    bool sorted = false;  // the cell sort matches the resident visibilities
+    // the records hold the degridding halves (and the per-point degridding
+    // outputs are written) for the resident visibilities
+    bool filled = false;
+    bool weighted = false;  // the density grid and weighting sums of the last uvmap are resident
     VisRec *d_rec = nullptr;
     double *d_wsq = nullptr;
-    bool filled = false;  // degridding halves written by a gridding fill on the current sort
     int *d_list = nullptr, *d_nlist = nullptr;
     float *d_partial = nullptr;
     size_t partial_cap = 0, items_cap = 0;
@@ -948,15 +964,25 @@ int plan_alloc(T **ptr, size_t count) {
 }
 
 int plan_reserve_vis(sde_plan *p, int64_t n) {
-    p->sorted = false;
+    p->filled = false;
+    p->weighted = false;
     if (n <= p->cap) return 0;
     const size_t c = (size_t)n;
     if (plan_alloc(&p->d_vis, c) || plan_alloc(&p->d_wt, c) || plan_alloc(&p->d_u, c) || plan_alloc(&p->d_v, c) ||
         plan_alloc(&p->d_w, c) || plan_alloc(&p->d_wteff, c) || plan_alloc(&p->d_mvis, c) ||
-        plan_alloc(&p->d_mwt, c) || plan_alloc(&p->sb.key_of, c) || plan_alloc(&p->sb.rank_of, c) ||
-        plan_alloc(&p->d_rec, c) || plan_alloc(&p->d_list, c) ||
-        plan_alloc(&p->d_blk, 2 * (size_t)grid_fill_blocks(n)))
+        plan_alloc(&p->d_mwt, c) || plan_alloc(&p->sb.keys, c) || plan_alloc(&p->sb.skeys, c) ||
+        plan_alloc(&p->sb.sidx, c) || plan_alloc(&p->sb.stage, c) || plan_alloc(&p->d_rec, c) ||
+        plan_alloc(&p->d_list, (size_t)list_capacity(n)))
         return -1;
+    const size_t tb = sort_temp_bytes(n, sort_nbits(p->tp));
+    if (tb == 0) {
+        set_plan_error("radix sort size query failed");
+        return -1;
+    }
+    char *temp = static_cast<char *>(p->sb.temp);
+    if (plan_alloc(&temp, tb)) return -1;
+    p->sb.temp = temp;
+    p->sb.temp_bytes = tb;
     p->cap = n;
     return 0;
 }
@@ -976,13 +1002,19 @@ int plan_reserve_items(sde_plan *p) {
     return 0;
 }
 
-int plan_sort(sde_plan *p) {
+// Cell sort of the resident visibilities; the prep pass also writes the
+// per-point degridding outputs into the model arrays.
+int plan_sort(sde_plan *p, const DensityOut *dens) {
     auto mb = [p](const char *nm) { plan_mark_begin(p, nm); };
     auto me = [p]() { plan_mark_end(p); };
-    if (run_sort(p->stream, p->d_wt, p->d_u, p->d_v, p->d_w, p->nvis, p->cp, p->tp, p->sb, mb, me)) return -1;
-    p->sorted = true;
+    DegridOut deg;
+    deg.vis_out = p->d_mvis;
+    deg.wt_out = p->d_mwt;
+    deg.list = p->d_list;
+    deg.nlist = p->d_nlist;
     p->filled = false;
-    return 0;
+    return run_sort(p->stream, p->d_vis, p->d_wt, p->d_u, p->d_v, p->d_w, p->nvis, p->cp, p->tp, p->sb, dens, &deg,
+                    mb, me);
 }
 
 }  // namespace
@@ -1049,16 +1081,13 @@ int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **out) {
     std::vector<float> ccf(p->ccflen);
     make_ccf(cft, ccf.data(), p->ccflen);
     const size_t ng = (size_t)p->nu * p->nv, npix = (size_t)cfg->nx * cfg->ny;
-    const int64_t nkeys = sort_nkeys(p->tp);
     int rc = plan_alloc(&p->d_cf, p->ncf) || plan_alloc(&p->d_ccf, p->ccflen) || plan_alloc(&p->d_grids, 2 * ng) || plan_alloc(&p->d_gwork, 2 * ng) ||
              plan_alloc(&p->d_mgrid, ng) || plan_alloc(&p->d_dirty, npix) || plan_alloc(&p->d_psf, npix) ||
              plan_alloc(&p->d_img_in, npix) || plan_alloc(&p->d_peak, 4) || plan_alloc(&p->d_gwt, ng) ||
              plan_alloc(&p->d_wblk, 4096) || plan_alloc(&p->d_wsums, 4) || plan_alloc(&p->d_cnt, 4) ||
              plan_alloc(&p->d_wsq, kWtSqBlocks) ||
-             plan_alloc(&p->sb.cnt, (size_t)nkeys) || plan_alloc(&p->sb.start, (size_t)nkeys) ||
-             plan_alloc(&p->sb.bsum, (size_t)sort_nblocks(p->tp)) ||
              plan_alloc(&p->sb.tile_start, p->tp.ntiles + 1) || plan_alloc(&p->sb.item_start, p->tp.ntiles + 1) ||
-             plan_alloc(&p->sb.nitems, 1) || plan_alloc(&p->d_nlist, 1) || plan_alloc(&p->d_blk, 2) ||
+             plan_alloc(&p->sb.nitems, 1) || plan_alloc(&p->d_nlist, kPrepBlocks) || plan_alloc(&p->d_blk, 2 * (size_t)kGatherBlocks) ||
              plan_alloc(&p->d_sums, 4);
     if (rc) {
         sde_plan_destroy(p);
@@ -1087,7 +1116,7 @@ int sde_plan_destroy(sde_plan *p) {
     plan_collect(p);
     for (auto e : p->pool) (void)hipEventDestroy(e);
     void *bufs[] = {p->d_cf, p->d_ccf, p->d_vis, p->d_wt, p->d_u, p->d_v, p->d_w, p->d_wteff, p->d_mvis, p->d_mwt,
-                    p->sb.cnt, p->sb.start, p->sb.bsum, p->sb.key_of, p->sb.rank_of, p->sb.tile_start,
+                    p->sb.keys, p->sb.skeys, p->sb.sidx, p->sb.stage, p->sb.temp, p->sb.tile_start,
                     p->sb.item_start, p->sb.nitems, p->sb.items, p->d_rec, p->d_list, p->d_nlist,
                     p->d_partial, p->d_blk, p->d_sums, p->d_grids, p->d_gwork, p->d_mgrid, p->d_dirty, p->d_psf, p->d_img_in,
                     p->d_peak, p->d_gwt, p->d_wblk, p->d_wsums, p->d_cnt, p->d_wsq};
@@ -1130,48 +1159,55 @@ int sde_plan_uvmap(sde_plan *p) {
     }
     hipStream_t s = p->stream;
     const int64_t n = p->nvis;
-    const float *wt_use = p->d_wt;
+    const size_t ng = (size_t)p->nu * p->nv;
     SDE_CHECK(hipMemsetAsync(p->d_cnt, 0, sizeof(unsigned long long) * 4, s));
-    if (p->cfg.weighting) {
-        const int nblk = wt_blocks(n);
-        const size_t ng = (size_t)p->nu * p->nv;
+    // every imaging job sorts its visibilities afresh (the sort is part of the job)
+    WeightParams wp;
+    const bool weighting = p->cfg.weighting != 0;
+    // the density grid comes from the sorted runs when its cell is the
+    // gridding cell (WTFOV = 1: the same scales); else from atomics
+    const bool runs = weighting && p->wcp.uscale == p->cp.uscale && p->wcp.vscale == p->cp.vscale;
+    DensityOut dens;
+    if (weighting) {
         SDE_CHECK(hipMemsetAsync(p->d_gwt, 0, sizeof(double) * ng, s));
-        plan_mark_begin(p, "wt_density");
-        launch_wt_density(s, p->d_wt, p->d_u, p->d_v, p->d_w, n, p->wcp, p->d_gwt, p->d_wblk, nblk);
-        plan_mark_end(p);
+        dens.wcp = p->wcp;
+        dens.gwt = runs ? p->d_gwt : nullptr;
+        dens.blk = p->d_wblk;
+        dens.ndrop = p->d_cnt;
+    }
+    if (plan_sort(p, weighting ? &dens : nullptr)) return -1;
+    if (weighting) {
+        int nwblk;
+        if (runs) {
+            const int np = prep_blocks(n);
+            plan_mark_begin(p, "wt_density");
+            launch_density_runs(s, p->sb, n, p->d_wt, p->tp, p->cp, p->d_gwt, p->d_wblk + np);
+            plan_mark_end(p);
+            nwblk = np + run_blocks(n);
+        } else {
+            nwblk = wt_blocks(n);
+            plan_mark_begin(p, "wt_density");
+            launch_wt_density(s, p->d_wt, p->d_u, p->d_v, p->d_w, n, p->wcp, p->d_gwt, p->d_wblk, nwblk);
+            plan_mark_end(p);
+        }
         float rob = p->wmode == 3 ? (p->cfg.robust[0] + p->cfg.robust[1]) / 2.0f : p->cfg.robust[0];
         const float t = 5.0f * powf(10.0f, -rob);
         plan_mark_begin(p, "wt_finish");
-        launch_wt_finish(s, p->d_gwt, (int)ng, p->d_wblk, nblk, p->wmode, t * t, rob * rob,
+        launch_wt_finish(s, p->d_gwt, (int)ng, p->d_wblk, nwblk, p->wmode, t * t, rob * rob,
                          2.0f * (p->cfg.ds * p->cfg.ds), p->d_wsq, p->d_wsums);
         plan_mark_end(p);
-        SDE_CHECK(hipMemcpyAsync(p->d_wteff, p->d_wt, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
-        plan_mark_begin(p, "wt_apply");
-        launch_wt_apply(s, p->d_wt, p->d_u, p->d_v, p->d_w, n, p->wcp, p->d_gwt, p->d_wsums, p->wmode == 3,
-                        p->cfg.robust, 4.0f * atanf(1.0f), p->d_wteff, p->d_cnt);
-        plan_mark_end(p);
-        wt_use = p->d_wteff;
+        wp = make_weight_params(p->wcp, p->wmode == 3, p->cfg.robust, 4.0f * atanf(1.0f), p->d_gwt, p->d_wsums);
     }
-    // every imaging job sorts its visibilities afresh (the sort is part of the job)
-    if (plan_sort(p)) return -1;
-    const size_t ng = (size_t)p->nu * p->nv;
     float2 *gd = p->d_grids, *gp = p->d_grids + ng;
     auto mb = [p](const char *nm) { plan_mark_begin(p, nm); };
     auto me = [p]() { plan_mark_end(p); };
-    // the fill also writes the degridding halves and the per-point outputs of
-    // a prediction on these visibilities (model VIS/WT are complete after
-    // sde_plan_vismodel)
-    SDE_CHECK(hipMemsetAsync(p->d_nlist, 0, sizeof(int), s));
-    DegridOut deg;
-    deg.wt = p->d_wt;
-    deg.vis_out = p->d_mvis;
-    deg.wt_out = p->d_mwt;
-    deg.list = p->d_list;
-    deg.nlist = p->d_nlist;
-    if (run_grid(s, p->d_vis, wt_use, p->d_u, p->d_v, p->d_w, n, p->cp, p->tp, 3, p->d_cf, p->d_cf, p->sb,
-                 p->d_rec, p->d_partial, p->d_blk, p->d_sums, gd, gp, deg, mb, me))
+    // the gather also writes the degridding halves of a prediction on these
+    // visibilities (model VIS/WT are complete after sde_plan_vismodel)
+    if (run_grid(s, n, p->cp, p->tp, 3, true, wp, p->d_cf, p->d_cf, p->sb, p->d_rec, p->d_partial, p->d_blk,
+                 p->d_sums, gd, gp, mb, me))
         return -1;
     p->filled = true;
+    p->weighted = weighting;
     float2 *twnv = p->tw[p->nv], *twfnu = p->tw[p->fnu];
     plan_mark_begin(p, "fft_cols");
     // out of place: the gridded planes stay readable (sde_plan_get_grids)
@@ -1212,19 +1248,17 @@ int sde_plan_vismodel(sde_plan *p, const float *image) {
     launch_fft_cols(s, p->d_mgrid, p->d_mgrid, p->nu, p->nv, 1, 0, +1, 1.0f, twnv);
     plan_mark_end(p);
     if (p->nvis <= 0) return 0;
-    if (!p->sorted && plan_sort(p)) return -1;
     SDE_CHECK(hipMemsetAsync(p->d_cnt + 1, 0, sizeof(unsigned long long), s));
     if (!p->filled) {
-        SDE_CHECK(hipMemsetAsync(p->d_nlist, 0, sizeof(int), s));
-        DegridOut deg;
-        deg.wt = p->d_wt;
-        deg.vis_out = p->d_mvis;
-        deg.wt_out = p->d_mwt;
-        deg.list = p->d_list;
-        deg.nlist = p->d_nlist;
-        plan_mark_begin(p, "degrid_fill");
-        launch_degrid_fill(s, p->d_u, p->d_v, p->d_w, p->d_vis, p->nvis, p->cp, p->tp, p->sb, p->d_rec, deg);
+        if (plan_sort(p, nullptr)) return -1;
+        plan_mark_begin(p, "degrid_gather");
+        const int rc = launch_grid_gather(s, p->sb, p->nvis, p->cp, p->tp, kGatherDegrid, WeightParams(), p->d_cf,
+                                          p->d_cf, p->d_rec, p->d_blk, p->d_sums);
         plan_mark_end(p);
+        if (rc) {
+            set_plan_error("record gather failed");
+            return -1;
+        }
         p->filled = true;
     }
     plan_mark_begin(p, "degrid");
@@ -1236,8 +1270,8 @@ int sde_plan_vismodel(sde_plan *p, const float *image) {
     }
     plan_mark_end(p);
     plan_mark_begin(p, "degrid_axis");
-    launch_degrid(s, p->d_mgrid, p->d_u, p->d_v, p->d_w, p->d_wt, p->d_vis, p->nvis, p->cp, p->d_cf, p->d_cf,
-                  p->d_mvis, p->d_mwt, p->d_cnt + 1, p->d_list, p->d_nlist);
+    launch_degrid_list(s, p->d_mgrid, p->d_u, p->d_v, p->d_w, p->d_wt, p->d_vis, p->nvis, p->cp, p->d_cf, p->d_cf,
+                       p->d_mvis, p->d_mwt, p->d_cnt + 1, p->d_list, p->d_nlist);
     plan_mark_end(p);
     SDE_CHECK(hipGetLastError());
     return 0;
@@ -1272,7 +1306,15 @@ int sde_plan_get_grids(sde_plan *p, float *dirty_grid, float *psf_grid) {
 
 int sde_plan_get_weights(sde_plan *p, float *wt) {
     if (!p) return -1;
-    const float *src = p->cfg.weighting ? p->d_wteff : p->d_wt;
+    const float *src = p->d_wt;
+    if (p->weighted && p->nvis > 0) {
+        // the weights of the last uvmap, from its resident density grid and sums
+        SDE_CHECK(hipMemcpyAsync(p->d_wteff, p->d_wt, sizeof(float) * p->nvis, hipMemcpyDeviceToDevice, p->stream));
+        launch_wt_apply(p->stream, p->d_wt, p->d_u, p->d_v, p->d_w, p->nvis, p->wcp, p->d_gwt, p->d_wsums,
+                        p->wmode == 3, p->cfg.robust, 4.0f * atanf(1.0f), p->d_wteff, p->d_cnt + 2);
+        SDE_CHECK(hipGetLastError());
+        src = p->d_wteff;
+    }
     if (p->nvis > 0) SDE_CHECK(hipMemcpyAsync(wt, src, sizeof(float) * p->nvis, hipMemcpyDeviceToHost, p->stream));
     SDE_CHECK(hipStreamSynchronize(p->stream));
     return 0;
@@ -1423,6 +1465,10 @@ int sde_plan_synth(sde_plan *p, int nant, const double *enu, double lat, double 
     }
     const int64_t nfull = (int64_t)nbl * ntime * nchan;
     const int64_t n = (nkeep > 0 && nkeep < nfull) ? nkeep : nfull;
+    if (n > 0x7fffffffLL) {
+        set_plan_error("sde_plan_synth: at most 2^31-1 visibilities per plan");
+        return -1;
+    }
     if (plan_reserve_vis(p, n)) return -1;
     p->nvis = n;
     double *d_b = nullptr, *d_f = nullptr, *d_s = nullptr;
@@ -1432,10 +1478,6 @@ int sde_plan_synth(sde_plan *p, int nant, const double *enu, double lat, double 
     SDE_CHECK(hipMemcpyAsync(d_b, bxyz.data(), sizeof(double) * bxyz.size(), hipMemcpyHostToDevice, p->stream));
     SDE_CHECK(hipMemcpyAsync(d_f, freq, sizeof(double) * nchan, hipMemcpyHostToDevice, p->stream));
     SDE_CHECK(hipMemcpyAsync(d_s, src.data(), sizeof(double) * src.size(), hipMemcpyHostToDevice, p->stream));
-    if (n > 0x7fffffffLL) {
-        set_plan_error("sde_plan_synth: at most 2^31-1 visibilities per plan");
-        return -1;
-    }
     launch_synth(p->stream, d_b, nbl, dec, ha0, dha, ntime, nchan, d_f, nsrc, d_s, sigma, seed, nfull, n, p->d_vis,
                  p->d_wt, p->d_u, p->d_v, p->d_w);
     SDE_CHECK(hipGetLastError());

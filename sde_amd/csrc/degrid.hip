@@ -13,19 +13,11 @@ __device__ __forceinline__ int nint_f(float x) { return (int)roundf(x); }
 // degrids to the reference's single-precision values.  A tap that would
 // read the row below the grid (reflected v = -NV/2, which the reference
 // reads out of bounds) contributes nothing.
-__global__ __launch_bounds__(256) void k_degrid(
-    const float2 *__restrict__ grid, const float *__restrict__ u, const float *__restrict__ v,
-    const float *__restrict__ w, const float *__restrict__ wt_in, const float2 *vis_in,
-    int64_t nvis, CoordParams cp, const float *__restrict__ cfu, const float *__restrict__ cfv,
-    float2 *vis_out, float *__restrict__ wt_out, unsigned long long *__restrict__ nbad,
-    const int *__restrict__ list, const int *__restrict__ nlist) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (list) {
-        if (i >= *nlist) return;
-        i = list[i];
-    } else if (i >= nvis) {
-        return;
-    }
+__device__ __forceinline__ void degrid_point(
+    int64_t i, const float2 *__restrict__ grid, const float *__restrict__ u, const float *__restrict__ v,
+    const float *__restrict__ w, const float *__restrict__ wt_in, const float2 *vis_in, const CoordParams &cp,
+    const float *__restrict__ cfu, const float *__restrict__ cfv, float2 *vis_out, float *__restrict__ wt_out,
+    unsigned long long *__restrict__ nbad) {
     float wti = wt_in[i];
     if (wti <= 0.0f) {
         wt_out[i] = wti;
@@ -113,40 +105,27 @@ __global__ __launch_bounds__(256) void k_degrid(
     wt_out[i] = wti;
 }
 
-// Degridding halves of the records of the tile path when no gridding fill
-// on the same sort wrote them (one thread per visibility, original order).
-// A visibility takes the tile path when it is in the cell sort, is not
-// within SUPPU cells of the u = 0 axis (those need the reflected reads) and
-// its footprint lies inside its tile's LDS window.
-__global__ __launch_bounds__(256) void k_degrid_fill(
-    const float *__restrict__ u, const float *__restrict__ v, const float *__restrict__ w,
-    const float *__restrict__ wt_in, const float2 *__restrict__ vis_in, int64_t nvis, CoordParams cp, TileParams tp,
-    const int *__restrict__ key_of, const int *__restrict__ rank_of, const int *__restrict__ start,
-    const int *__restrict__ bsum, VisRec *__restrict__ rec, int *__restrict__ list, int *__restrict__ nlist,
-    float2 *__restrict__ vis_out, float *__restrict__ wt_out) {
+__global__ __launch_bounds__(256) void k_degrid(
+    const float2 *__restrict__ grid, const float *__restrict__ u, const float *__restrict__ v,
+    const float *__restrict__ w, const float *__restrict__ wt_in, const float2 *vis_in, int64_t nvis,
+    CoordParams cp, const float *__restrict__ cfu, const float *__restrict__ cfv, float2 *vis_out,
+    float *__restrict__ wt_out, unsigned long long *__restrict__ nbad) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool to_list = false;
-    if (i < nvis) {
-        const float wti = wt_in[i];
-        const int key = key_of[i];
-        DegridRec r = {kNoDegrid, 0u, 1.0f, 0.0f};
-        if (wti <= 0.0f) {
-            wt_out[i] = wti;
-            vis_out[i] = vis_in[i];
-        } else {
-            const int path = degrid_half(cp, tp, u[i], v[i], w[i], key, (uint32_t)i, r);
-            if (path == kDegOff) {
-                vis_out[i] = make_float2(0.0f, 0.0f);
-                wt_out[i] = -wti;
-            } else if (path == kDegList) {
-                to_list = true;
-            } else {
-                wt_out[i] = wti;  // negated by the tile kernel if the point fails
-            }
-        }
-        if (key >= 0) rec[start[key] + bsum[key >> kScanShift] + rank_of[i]].d = r;
-    }
-    wave_append(to_list, (int)i, list, nlist);
+    if (i < nvis) degrid_point(i, grid, u, v, w, wt_in, vis_in, cp, cfu, cfv, vis_out, wt_out, nbad);
+}
+
+// The points of a segmented list (segment b holds counts[b] indices from
+// slot b*seg_cap), one workgroup per segment.
+__global__ __launch_bounds__(256) void k_degrid_list(
+    const float2 *__restrict__ grid, const float *__restrict__ u, const float *__restrict__ v,
+    const float *__restrict__ w, const float *__restrict__ wt_in, const float2 *vis_in, CoordParams cp,
+    const float *__restrict__ cfu, const float *__restrict__ cfv, float2 *vis_out, float *__restrict__ wt_out,
+    unsigned long long *__restrict__ nbad, const int *__restrict__ list, const int *__restrict__ counts,
+    int seg_cap) {
+    const int cnt = counts[blockIdx.x];
+    const int *seg = list + (size_t)blockIdx.x * seg_cap;
+    for (int j = threadIdx.x; j < cnt; j += blockDim.x)
+        degrid_point(seg[j], grid, u, v, w, wt_in, vis_in, cp, cfu, cfv, vis_out, wt_out, nbad);
 }
 
 // One thread per record of a work item; the item's tile and its margin are
@@ -217,29 +196,6 @@ __global__ __launch_bounds__(256) void k_degrid_tiles(const VisRec *__restrict__
         // GRDDEH2D always rotates: by (1, 0) when there is no shift, exactly
         vis_out[idx] = make_float2(vr * d.rr - vi * d.ri, vr * d.ri + vi * d.rr);
     }
-}
-
-// Density-grid cell in REAL arithmetic (grd/grduwt2d.f:89-110).
-__device__ __forceinline__ void wcell(const CoordParams &cp, float u, float v, float w, int &uc, int &vc,
-                                      float &ul, float &vl) {
-    if (cp.doshift) {
-        ul = (float)(cp.shift[0] * (double)u + cp.shift[1] * (double)v + cp.shift[2] * (double)w);
-        vl = (float)(cp.shift[3] * (double)u + cp.shift[4] * (double)v + cp.shift[5] * (double)w);
-    } else {
-        ul = u;
-        vl = v;
-    }
-    float ucell = cp.uscale * ul + cp.uoffset;
-    float vcell = cp.vscale * vl + cp.voffset;
-    if (ucell > 0.0f) {
-        ucell = ucell + (float)cp.uorigin;
-        vcell = vcell + (float)cp.vorigin;
-    } else {
-        ucell = -ucell + (float)cp.uorigin;
-        vcell = -vcell + (float)cp.vorigin;
-    }
-    uc = nint_f(ucell);
-    vc = nint_f(vcell);
 }
 
 constexpr int kWtThreads = 256;
@@ -336,30 +292,9 @@ __global__ __launch_bounds__(256) void k_wt_apply(
     if (i >= nvis) return;
     const float wti = wt[i];
     if (wti <= 0.0f) return;
-    int uc, vc;
-    float ul, vl;
-    wcell(cp, u[i], v[i], w[i], uc, vc, ul, vl);
-    if (uc < 1 || uc > cp.nu || vc < 1 || vc > cp.nv) {
-        newwt[i] = -1.0f;
-        atomicAdd(ndrop, 1ull);
-        return;
-    }
-    float f2 = (float)wsums[2];
-    const float d2 = (float)wsums[3];
-    if (az) {
-        float t = atan2f(vl, ul);
-        t = fmodf(t - rpa + 2.5f * pi_f, pi_f) - pi_f / 2.0f;
-        const float rb = rob1 - fabsf(t) * rbscale;
-        const float q = 5.0f * powf(10.0f, -rb);
-        f2 = (float)((double)(q * q) / (wsums[1] / wsums[0]));
-    }
-    const float g = (float)gwt[(size_t)(vc - 1) * cp.nu + uc - 1];
-    if (g > 0.0f) {
-        if (uc == cp.uorigin && vc != cp.vorigin) newwt[i] = 0.5f * wti / (g * f2 + d2);
-        else newwt[i] = wti / (g * f2 + d2);
-    } else {
-        newwt[i] = -1.0f;
-    }
+    bool off;
+    newwt[i] = robust_weight(cp, u[i], v[i], w[i], wti, gwt, wsums, az, rob1, rpa, rbscale, pi_f, &off);
+    if (off) atomicAdd(ndrop, 1ull);
 }
 
 __global__ void k_gwt_out(const double *__restrict__ gwt, float *__restrict__ out, size_t n) {
@@ -424,20 +359,20 @@ __global__ __launch_bounds__(256) void k_synth(
 void launch_degrid(hipStream_t s, const float2 *grid, const float *u, const float *v,
                    const float *w, const float *wt_in, const float2 *vis_in, int64_t nvis,
                    const CoordParams &cp, const float *cfu, const float *cfv, float2 *vis_out,
-                   float *wt_out, unsigned long long *nbad, const int *list, const int *nlist) {
+                   float *wt_out, unsigned long long *nbad) {
     if (nvis <= 0) return;
     const int64_t nb = (nvis + 255) / 256;
     hipLaunchKernelGGL(k_degrid, dim3((unsigned)nb), dim3(256), 0, s, grid, u, v, w, wt_in, vis_in,
-                       nvis, cp, cfu, cfv, vis_out, wt_out, nbad, list, nlist);
+                       nvis, cp, cfu, cfv, vis_out, wt_out, nbad);
 }
 
-void launch_degrid_fill(hipStream_t s, const float *u, const float *v, const float *w, const float2 *vis_in,
-                        int64_t nvis, const CoordParams &cp, const TileParams &tp, const SortBufs &b,
-                        VisRec *rec, const DegridOut &deg) {
+void launch_degrid_list(hipStream_t s, const float2 *grid, const float *u, const float *v, const float *w,
+                        const float *wt_in, const float2 *vis_in, int64_t nvis, const CoordParams &cp,
+                        const float *cfu, const float *cfv, float2 *vis_out, float *wt_out,
+                        unsigned long long *nbad, const int *list, const int *counts) {
     if (nvis <= 0) return;
-    const int64_t nb = (nvis + 255) / 256;
-    hipLaunchKernelGGL(k_degrid_fill, dim3((unsigned)nb), dim3(256), 0, s, u, v, w, deg.wt, vis_in, nvis, cp, tp,
-                       b.key_of, b.rank_of, b.start, b.bsum, rec, deg.list, deg.nlist, deg.vis_out, deg.wt_out);
+    hipLaunchKernelGGL(k_degrid_list, dim3(prep_blocks(nvis)), dim3(256), 0, s, grid, u, v, w, wt_in, vis_in, cp,
+                       cfu, cfv, vis_out, wt_out, nbad, list, counts, list_seg_cap(nvis));
 }
 
 template <int T, int S>
@@ -500,22 +435,35 @@ void launch_wt_finish(hipStream_t s, const double *gwt, int ncell, const double 
                        absd2, wsums);
 }
 
+WeightParams make_weight_params(const CoordParams &wcp, int az, const float *robust, float pi_f,
+                                const double *gwt, const double *wsums) {
+    WeightParams wp;
+    wp.on = 1;
+    wp.az = az;
+    wp.rob1 = robust ? robust[0] : 0.0f;
+    if (az) {
+        float rpa = -robust[2] + 90.0f - 90.0f;
+        while (rpa < -90.0f) rpa = rpa + 180.0f;
+        while (rpa > 90.0f) rpa = rpa - 180.0f;
+        wp.rpa = rpa * pi_f / 180.0f;
+        wp.rbscale = (robust[0] - robust[1]) / (pi_f / 2.0f);
+    }
+    wp.pi_f = pi_f;
+    wp.wcp = wcp;
+    wp.gwt = gwt;
+    wp.wsums = wsums;
+    return wp;
+}
+
 void launch_wt_apply(hipStream_t s, const float *wt, const float *u, const float *v,
                      const float *w, int64_t nvis, const CoordParams &cp, const double *gwt,
                      const double *wsums, int az, const float *robust, float pi_f,
                      float *newwt, unsigned long long *ndrop) {
     if (nvis <= 0) return;
-    float rob1 = robust ? robust[0] : 0.0f, rpa = 0.0f, rbscale = 0.0f;
-    if (az) {
-        rpa = -robust[2] + 90.0f - 90.0f;
-        while (rpa < -90.0f) rpa = rpa + 180.0f;
-        while (rpa > 90.0f) rpa = rpa - 180.0f;
-        rpa = rpa * pi_f / 180.0f;
-        rbscale = (robust[0] - robust[1]) / (pi_f / 2.0f);
-    }
+    const WeightParams wp = make_weight_params(cp, az, robust, pi_f, gwt, wsums);
     const int64_t nb = (nvis + 255) / 256;
     hipLaunchKernelGGL(k_wt_apply, dim3((unsigned)nb), dim3(256), 0, s, wt, u, v, w, nvis, cp, gwt,
-                       wsums, az, rob1, rpa, rbscale, pi_f, newwt, ndrop);
+                       wsums, az, wp.rob1, wp.rpa, wp.rbscale, pi_f, newwt, ndrop);
 }
 
 void launch_gwt_out(hipStream_t s, const double *gwt, float *out, size_t n) {

@@ -31,7 +31,6 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kMaxBlocks = 4096;
-constexpr int kScanBlock = 1 << kScanShift;   // keys per block-local scan (1024 threads x 8)
 
 __device__ __forceinline__ int nint_d(double x) { return (int)round(x); }
 __device__ __forceinline__ int nint_f(float x) { return (int)roundf(x); }
@@ -102,66 +101,128 @@ __device__ __forceinline__ int cell_key(const TileParams &tp, int cu, int cv) {
     return ((tv * tp.ntu + tu) * tp.T + (cv - tv * tp.T)) * tp.T + (cu - tu * tp.T);
 }
 
-__global__ __launch_bounds__(kThreads) void k_sort_count(const float *__restrict__ wt, const float *__restrict__ u,
-                                                         const float *__restrict__ v, const float *__restrict__ w,
-                                                         int64_t nvis, CoordParams cp, TileParams tp,
-                                                         int *__restrict__ cnt, int *__restrict__ key_of,
-                                                         int *__restrict__ rank_of) {
-    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < nvis; i += (int64_t)gridDim.x * kThreads) {
-        int key = -1, rank = 0;
-        if (wt[i] > 0.0f) {
-            GridCell c;
-            if (grid_cell(cp, u[i], v[i], w[i], make_float2(0.0f, 0.0f), false, c)) {
-                key = cell_key(tp, c.nuc + cp.suppu - 1, c.nvc - 1);
-                rank = atomicAdd(&cnt[key], 1);
+// Prep pass, one thread per visibility in the caller's order: the cell key
+// of GRDH2D's DOUBLE coordinate step and drop test (grd/grdh2d.f:115-145),
+// the 32-byte staging record, and optionally
+//   DM = 1: the count of points off the density grid;
+//   DM = 2: also the density weight of every point whose REAL density cell
+//           (grd/grdruw2d.f:139-160) is not its DOUBLE gridding cell or
+//           that has no gridding cell (these are flagged in bit 31 of the
+//           key when sorted; the sorted runs add all others);
+//   DEG: GRDDEH2D's per-point outputs (grd/grddeh2d.f:88-118) -- WT, VIS of
+//        flagged and off-grid points -- and the list of points the tile
+//        path cannot take.
+template <int DM, bool DEG>
+__global__ __launch_bounds__(kThreads) void k_prep(
+    const float2 *__restrict__ vis, const float *__restrict__ wt, const float *__restrict__ u,
+    const float *__restrict__ v, const float *__restrict__ w, int64_t n, CoordParams cp, TileParams tp,
+    uint32_t sentinel, uint32_t *__restrict__ keys, StageRec *__restrict__ stage, CoordParams wcp,
+    double *__restrict__ gwt, double *__restrict__ dblk, unsigned long long *__restrict__ ndrop,
+    float2 *__restrict__ vis_out, float *__restrict__ wt_out, int *__restrict__ list, int *__restrict__ nlist,
+    int seg_cap) {
+    double dsum = 0.0, ndr = 0.0;
+    __shared__ int s_nlist;
+    if (DEG) {
+        if (threadIdx.x == 0) s_nlist = 0;
+        __syncthreads();
+    }
+    int *seg = DEG ? list + (size_t)blockIdx.x * seg_cap : nullptr;
+    const int64_t stride = (int64_t)gridDim.x * kThreads;
+    // block-uniform loop bounds: every lane of a wave takes part in the list append
+    for (int64_t base = (int64_t)blockIdx.x * kThreads; base < n; base += stride) {
+        const int64_t i = base + threadIdx.x;
+        bool to_list = false;
+        if (i < n) {
+            const float wti = wt[i];
+            const float uf = u[i], vf = v[i], wf = w[i];
+            const float2 vv = vis[i];
+            StageRec sr;
+            sr.u = uf;
+            sr.v = vf;
+            sr.w = wf;
+            sr.wt = wti;
+            sr.vis = vv;
+            sr.pad0 = sr.pad1 = 0u;
+            stage[i] = sr;
+            uint32_t key = sentinel;
+            if (wti > 0.0f) {
+                GridCell c;
+                const bool in = grid_cell(cp, uf, vf, wf, make_float2(0.0f, 0.0f), false, c);
+                if (in) key = (uint32_t)cell_key(tp, c.nuc + cp.suppu - 1, c.nvc - 1);
+                if (DM > 0) {
+                    int uc, vc;
+                    float ul, vl;
+                    wcell(wcp, uf, vf, wf, uc, vc, ul, vl);
+                    if (uc < 1 || uc > wcp.nu || vc < 1 || vc > wcp.nv) {
+                        ndr += 1.0;
+                    } else if (DM == 2 && !(in && uc == c.nuc && vc == c.nvc)) {
+                        atomicAdd(&gwt[(size_t)(vc - 1) * wcp.nu + uc - 1], (double)wti);
+                        dsum += (double)wti;
+                        if (in) key |= kKeyFlag;
+                    }
+                }
+                if (DEG) {
+                    DegridRec d;
+                    const int path = degrid_half(cp, tp, uf, vf, wf, in ? (int)(key & ~kKeyFlag) : -1, (uint32_t)i, d);
+                    if (path == kDegOff) {
+                        vis_out[i] = make_float2(0.0f, 0.0f);
+                        wt_out[i] = -wti;
+                    } else if (path == kDegList) {
+                        to_list = true;
+                    } else {
+                        wt_out[i] = wti;  // negated by the tile kernel if the point fails
+                    }
+                }
+            } else if (DEG) {
+                wt_out[i] = wti;
+                vis_out[i] = vv;
             }
+            keys[i] = key;
         }
-        key_of[i] = key;
-        rank_of[i] = rank;
+        if (DEG) seg_append(to_list, (int)i, seg, &s_nlist);
+    }
+    if (DEG) {
+        __syncthreads();
+        if (threadIdx.x == 0) nlist[blockIdx.x] = s_nlist;
+    }
+    if (DM > 0) {
+        block_sum2(dsum, ndr);
+        if (threadIdx.x == 0) {
+            if (DM == 2) dblk[blockIdx.x] = dsum;
+            if (ndr > 0.0) atomicAdd(ndrop, (unsigned long long)ndr);
+        }
     }
 }
 
-// Block-local exclusive scan of kScanBlock counts; block totals to bsum.
-__global__ __launch_bounds__(1024) void k_scan_local(const int *__restrict__ cnt, int64_t nkeys,
-                                                     int *__restrict__ start, int *__restrict__ bsum) {
-    __shared__ int s[1024];
-    const int64_t base = (int64_t)blockIdx.x * kScanBlock + threadIdx.x * 8;
-    int v[8], tot = 0;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        v[k] = base + k < nkeys ? cnt[base + k] : 0;
-        tot += v[k];
-    }
-    s[threadIdx.x] = tot;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int x = (int)threadIdx.x >= off ? s[threadIdx.x - off] : 0;
-        __syncthreads();
-        s[threadIdx.x] += x;
-        __syncthreads();
-    }
-    int run = (int)threadIdx.x > 0 ? s[threadIdx.x - 1] : 0;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        if (base + k < nkeys) start[base + k] = run;
-        run += v[k];
-    }
-    if (threadIdx.x == 1023) bsum[blockIdx.x] = s[1023];
+__device__ __forceinline__ int tile_of_key(uint32_t k, uint32_t nkeys, int tt, int ntiles) {
+    k &= ~kKeyFlag;
+    return k >= nkeys ? ntiles : (int)(k / (uint32_t)tt);
 }
 
-// Scan of the block totals (nb <= 1024 * 64), then per-tile record ranges,
-// chunk counts and their scan (work items), and the FP64 totals.
-__global__ __launch_bounds__(1024) void k_scan_tiles(int *__restrict__ bsum, int nb, const int *__restrict__ start,
-                                                     TileParams tp, int *__restrict__ tile_start,
+// First sorted slot of every tile (tile_start[ntiles] = number of sorted
+// points): each slot where the tile changes writes the starts of the tiles
+// it closes and opens, so every entry is written once.
+__global__ void k_tile_bounds(const uint32_t *__restrict__ skeys, int64_t n, uint32_t nkeys, int tt, int ntiles,
+                              int *__restrict__ tile_start) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int t = tile_of_key(skeys[p], nkeys, tt, ntiles);
+    const int tprev = p == 0 ? -1 : tile_of_key(skeys[p - 1], nkeys, tt, ntiles);
+    for (int x = tprev + 1; x <= t; x++) tile_start[x] = (int)p;
+    if (p == n - 1)
+        for (int x = t + 1; x <= ntiles; x++) tile_start[x] = (int)n;
+}
+
+// Work items per tile (chunks of at most `chunk` records) and their scan.
+__global__ __launch_bounds__(1024) void k_items_scan(const int *__restrict__ tile_start, int ntiles, int chunk,
                                                      int *__restrict__ item_start, int *__restrict__ nitems) {
     __shared__ int s[1024];
     const int t = threadIdx.x;
-    // exclusive scan of bsum in place (each thread a contiguous segment)
-    const int per = (nb + 1023) / 1024;
-    const int lo = min(t * per, nb), hi = min(lo + per, nb);
-    int tot = 0;
-    for (int k = lo; k < hi; k++) tot += bsum[k];
-    s[t] = tot;
+    const int pt = (ntiles + 1023) / 1024;
+    const int lo = min(t * pt, ntiles), hi = min(lo + pt, ntiles);
+    int its = 0;
+    for (int k = lo; k < hi; k++) its += (tile_start[k + 1] - tile_start[k] + chunk - 1) / chunk;
+    s[t] = its;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
         const int x = t >= off ? s[t - off] : 0;
@@ -171,46 +232,54 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(int *__restrict__ bsum, int
     }
     int run = t > 0 ? s[t - 1] : 0;
     for (int k = lo; k < hi; k++) {
-        const int b = bsum[k];
-        bsum[k] = run;
-        run += b;
-    }
-    const int total = s[1023];
-    __syncthreads();
-    // tile record ranges and chunk counts
-    const int tt = tp.T * tp.T;
-    const int pt = (tp.ntiles + 1023) / 1024;
-    const int tlo = min(t * pt, tp.ntiles), thi = min(tlo + pt, tp.ntiles);
-    int its = 0;
-    for (int k = tlo; k < thi; k++) {
-        const int64_t key = (int64_t)k * tt;
-        const int b0 = start[key] + bsum[key >> kScanShift];
-        const int64_t key1 = key + tt;
-        const int b1 = k + 1 < tp.ntiles ? start[key1] + bsum[key1 >> kScanShift] : total;
-        tile_start[k] = b0;
-        its += (b1 - b0 + tp.chunk - 1) / tp.chunk;
-    }
-    __syncthreads();
-    s[t] = its;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int x = t >= off ? s[t - off] : 0;
-        __syncthreads();
-        s[t] += x;
-        __syncthreads();
-    }
-    __syncthreads();  // every tile_start written
-    run = t > 0 ? s[t - 1] : 0;
-    for (int k = tlo; k < thi; k++) {
         item_start[k] = run;
-        const int b1 = k + 1 < tp.ntiles ? tile_start[k + 1] : total;
-        run += (b1 - tile_start[k] + tp.chunk - 1) / tp.chunk;
+        run += (tile_start[k + 1] - tile_start[k] + chunk - 1) / chunk;
     }
     if (t == 1023) {
-        tile_start[tp.ntiles] = total;
-        item_start[tp.ntiles] = s[1023];
+        item_start[ntiles] = s[1023];
         *nitems = s[1023];
     }
+}
+
+// Density grid from the sorted keys (points whose density cell is their
+// gridding cell): a segmented sum over each wave's runs of equal keys, one
+// FP64 atomic per run.  Flagged points were added by the prep pass.
+__global__ __launch_bounds__(kThreads) void k_density_runs(
+    const uint32_t *__restrict__ skeys, const uint32_t *__restrict__ sidx, const int *__restrict__ tile_start,
+    const float *__restrict__ wt, TileParams tp, int nu, int suppu, double *__restrict__ gwt,
+    double *__restrict__ blk) {
+    const int64_t nvalid = tile_start[tp.ntiles];
+    const int lane = threadIdx.x & 63;
+    const int tt = tp.T * tp.T;
+    double dsum = 0.0, unused = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * kThreads;
+    for (int64_t base = (int64_t)blockIdx.x * kThreads; base < nvalid; base += stride) {
+        const int64_t p = base + threadIdx.x;
+        const bool valid = p < nvalid;
+        const uint32_t k = valid ? skeys[p] : 0xffffffffu;
+        const uint32_t key = valid ? (k & ~kKeyFlag) : 0xffffffffu;
+        double x = (valid && !(k & kKeyFlag)) ? (double)wt[sidx[p]] : 0.0;
+        dsum += x;
+        const uint32_t prev = (uint32_t)__shfl_up((int)key, 1);
+        const bool head = lane == 0 || prev != key;
+        const unsigned long long heads = __ballot(head);
+        const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+        const int seg = 63 - __clzll((long long)(heads & upto));
+        for (int d = 1; d < 64; d <<= 1) {
+            const double y = __shfl_up(x, d);
+            if (lane - d >= seg) x += y;
+        }
+        const uint32_t next = (uint32_t)__shfl_down((int)key, 1);
+        const bool tail = lane == 63 || next != key;
+        if (valid && tail && x != 0.0) {
+            const int tile = (int)(key / (uint32_t)tt), rem = (int)(key - (uint32_t)tile * tt);
+            const int lv = rem / tp.T, lu = rem - lv * tp.T;
+            const int cu = (tile % tp.ntu) * tp.T + lu, cv = (tile / tp.ntu) * tp.T + lv;
+            atomicAdd(&gwt[(size_t)cv * nu + (cu - suppu)], x);
+        }
+    }
+    block_sum2(dsum, unused);
+    if (threadIdx.x == 0) blk[blockIdx.x] = dsum;
 }
 
 __global__ void k_grid_items(const int *__restrict__ tile_start, const int *__restrict__ item_start, int ntiles,
@@ -229,82 +298,74 @@ __global__ void k_grid_items(const int *__restrict__ tile_start, const int *__re
     }
 }
 
-// One thread per visibility.  The gridding half of the record follows
-// GRDH2D's per-point step (phase rotation, conjugation, kernel phases,
-// WT*VIS products); with DEG the degridding half and the per-point outputs
-// of GRDDEH2D for the same point are produced in the same pass.
-template <bool DEG>
-__global__ __launch_bounds__(kThreads) void k_grid_fill(
-    const float2 *__restrict__ vis, const float *__restrict__ wt, const float *__restrict__ u,
-    const float *__restrict__ v, const float *__restrict__ w, int64_t nvis, CoordParams cp, TileParams tp,
-    int psf_only, const float *__restrict__ cfu, const float *__restrict__ cfv, const int *__restrict__ key_of,
-    const int *__restrict__ rank_of, const int *__restrict__ start, const int *__restrict__ bsum,
-    VisRec *__restrict__ rec, double *__restrict__ blk_sums, const float *__restrict__ wt_deg,
-    float2 *__restrict__ vis_out, float *__restrict__ wt_out, int *__restrict__ list, int *__restrict__ nlist) {
+// Records in sorted order, one thread per sorted slot: the staging record
+// of the slot's visibility is gathered (one 32-byte read) and the record
+// written in place (coalesced).  The gridding half follows GRDH2D's
+// per-point step (grd/grdh2d.f:109-160: phase rotation, conjugation,
+// kernel phases, WT*VIS) with the weight of GRDRUW2D/GRDUWT2D when WGT;
+// the degridding half follows GRDDEH2D's (grd/grddeh2d.f:92-118).
+template <int FL, bool WGT>
+__global__ __launch_bounds__(kThreads) void k_grid_gather(
+    const uint32_t *__restrict__ skeys, const uint32_t *__restrict__ sidx, const StageRec *__restrict__ stage,
+    const int *__restrict__ tile_start, CoordParams cp, TileParams tp, WeightParams wp,
+    const float *__restrict__ cfu, const float *__restrict__ cfv, VisRec *__restrict__ rec,
+    double *__restrict__ blk_sums) {
+    const int64_t nvalid = tile_start[tp.ntiles];
     double sw = 0.0, ls = 0.0;
-    bool to_list = false;
-    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (i < nvis) {
-        const int key = key_of[i];
-        const float wti = wt[i];
-        const double ug = u[i], vg = v[i], wg = w[i];
-        DegridRec d = {kNoDegrid, 0u, 1.0f, 0.0f};
-        if (DEG) {
-            const float wd = wt_deg[i];
-            if (wd <= 0.0f) {
-                wt_out[i] = wd;
-                vis_out[i] = vis[i];
-            } else {
-                const int path = degrid_half(cp, tp, ug, vg, wg, key, (uint32_t)i, d);
-                if (path == kDegOff) {
-                    vis_out[i] = make_float2(0.0f, 0.0f);
-                    wt_out[i] = -wd;
-                } else if (path == kDegList) {
-                    to_list = true;
-                } else {
-                    wt_out[i] = wd;  // negated by the tile kernel if the point fails
-                }
-            }
-        }
-        if (key >= 0) {
-            const int pos = start[key] + bsum[key >> kScanShift] + rank_of[i];
+    const int cfoffu = (cp.suppu + 1) * cp.osampu + 1;
+    const int cfoffv = (cp.suppv + 1) * cp.osampv + 1;
+    const int poffu = (cp.osampu + 1) / 2, poffv = (cp.osampv + 1) / 2;
+    for (int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x; p < nvalid; p += (int64_t)gridDim.x * kThreads) {
+        const uint32_t key = skeys[p] & ~kKeyFlag;
+        const uint32_t i = sidx[p];
+        const StageRec sr = stage[i];
+        VisRec out;
+        if (FL & kGatherGrid) {
             GridRec r;
-            r.pk = (uint32_t)(key & (tp.T * tp.T - 1));
-            r.pk = (r.pk % tp.T) | ((r.pk / tp.T) << 8);
+            const uint32_t local = key & (uint32_t)(tp.T * tp.T - 1);
+            r.pk = (local % tp.T) | ((local / tp.T) << 8);
             r.wt = 0.0f;
             r.re = 0.0f;
             r.im = 0.0f;
+            float wti = sr.wt;
+            if (WGT) {
+                bool off;
+                wti = robust_weight(wp.wcp, sr.u, sr.v, sr.w, wti, wp.gwt, wp.wsums, wp.az, wp.rob1, wp.rpa,
+                                    wp.rbscale, wp.pi_f, &off);
+            }
             if (wti > 0.0f) {
+                const bool psf = (FL & kGatherPsf) != 0;
                 GridCell c;
-                grid_cell(cp, ug, vg, wg, psf_only ? make_float2(1.0f, 0.0f) : vis[i], !psf_only, c);
-                if (psf_only) { c.fr = 1.0f; c.fi = 0.0f; }
-                const int cfoffu = (cp.suppu + 1) * cp.osampu + 1;
-                const int cfoffv = (cp.suppv + 1) * cp.osampv + 1;
-                const int poffu = (cp.osampu + 1) / 2, poffv = (cp.osampv + 1) / 2;
+                grid_cell(cp, sr.u, sr.v, sr.w, psf ? make_float2(1.0f, 0.0f) : sr.vis, !psf, c);
+                if (psf) {
+                    c.fr = 1.0f;
+                    c.fi = 0.0f;
+                }
                 const int delui = nint_d((double)cp.osampu * ((double)c.nuc - c.ucell));
                 const int delvi = nint_d((double)cp.osampv * ((double)c.nvc - c.vcell));
                 double su = 0.0, sv = 0.0;
                 for (int o = -cp.suppu; o <= cp.suppu; o++) su += (double)cfu[delui + cfoffu - 1 + cp.osampu * o];
                 for (int o = -cp.suppv; o <= cp.suppv; o++) sv += (double)cfv[delvi + cfoffv - 1 + cp.osampv * o];
-                sw = (double)wti;
-                ls = (double)wti * su * sv;
+                sw += (double)wti;
+                ls += (double)wti * su * sv;
                 r.pk |= ((uint32_t)(delui + poffu) << 16) | ((uint32_t)(delvi + poffv) << 24);
                 r.wt = wti;
                 r.re = (float)((double)wti * (double)c.fr);
                 r.im = (float)((double)wti * (double)c.fi);
             }
-            VisRec vr;
-            vr.g = r;
-            vr.d = d;
-            if (DEG) rec[pos] = vr;
-            else rec[pos].g = r;
+            out.g = r;
         }
+        if (FL & kGatherDegrid) degrid_half(cp, tp, sr.u, sr.v, sr.w, (int)key, i, out.d);
+        if ((FL & kGatherGrid) && (FL & kGatherDegrid)) rec[p] = out;
+        else if (FL & kGatherGrid) rec[p].g = out.g;
+        else rec[p].d = out.d;
     }
-    if (DEG) wave_append(to_list, (int)i, list, nlist);
-    block_sum2(sw, ls);
-    if (threadIdx.x == 0) {
-        blk_sums[2 * blockIdx.x] = sw;
-        blk_sums[2 * blockIdx.x + 1] = ls;
+    if (FL & kGatherGrid) {
+        block_sum2(sw, ls);
+        if (threadIdx.x == 0) {
+            blk_sums[2 * blockIdx.x] = sw;
+            blk_sums[2 * blockIdx.x + 1] = ls;
+        }
     }
 }
 
@@ -582,42 +643,113 @@ int blocks_for(int64_t n) {
 
 }  // namespace
 
-int grid_prep_blocks(int64_t nvis) { return blocks_for(nvis); }
-
 int64_t sort_nkeys(const TileParams &tp) { return (int64_t)tp.ntiles * tp.T * tp.T; }
-int sort_nblocks(const TileParams &tp) { return (int)((sort_nkeys(tp) + kScanBlock - 1) / kScanBlock); }
+// bits that order every key and the sentinel nkeys
+int sort_nbits(const TileParams &tp) {
+    const int64_t top = sort_nkeys(tp);
+    int b = 1;
+    while ((1ll << b) <= top) b++;
+    return b;
+}
 
-void launch_sort(hipStream_t s, const float *wt, const float *u, const float *v, const float *w, int64_t nvis,
-                 const CoordParams &cp, const TileParams &tp, SortBufs &b) {
-    const int64_t nkeys = sort_nkeys(tp);
-    const int nb = sort_nblocks(tp);
-    (void)hipMemsetAsync(b.cnt, 0, sizeof(int) * nkeys, s);
-    hipLaunchKernelGGL(k_sort_count, dim3(blocks_for(nvis)), dim3(kThreads), 0, s, wt, u, v, w, nvis, cp, tp, b.cnt,
-                       b.key_of, b.rank_of);
-    hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(1024), 0, s, b.cnt, nkeys, b.start, b.bsum);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, b.bsum, nb, b.start, tp, b.tile_start, b.item_start,
+static int capped_blocks(int64_t n, int cap) {
+    int64_t b = (n + kThreads - 1) / kThreads;
+    if (b < 1) b = 1;
+    return (int)(b > cap ? cap : b);
+}
+int prep_blocks(int64_t n) { return capped_blocks(n, kPrepBlocks); }
+int run_blocks(int64_t n) { return capped_blocks(n, kRunBlocks); }
+int gather_blocks(int64_t n) { return capped_blocks(n, kGatherBlocks); }
+// slots per prep block: the block's share of a block-uniform grid-stride loop
+int list_seg_cap(int64_t n) {
+    const int64_t per_pass = (int64_t)prep_blocks(n) * kThreads;
+    return (int)(((n + per_pass - 1) / per_pass) * kThreads);
+}
+int64_t list_capacity(int64_t n) { return (int64_t)list_seg_cap(n > 0 ? n : 1) * prep_blocks(n > 0 ? n : 1); }
+
+void launch_prep(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
+                 const float *w, int64_t n, const CoordParams &cp, const TileParams &tp, SortBufs &b,
+                 const DensityOut *dens, const DegridOut *deg) {
+    if (n <= 0) return;
+    const int nb = prep_blocks(n);
+    const uint32_t sentinel = (uint32_t)sort_nkeys(tp);
+    CoordParams wcp = dens ? dens->wcp : cp;
+    double *gwt = dens ? dens->gwt : nullptr, *dblk = dens ? dens->blk : nullptr;
+    unsigned long long *ndrop = dens ? dens->ndrop : nullptr;
+    float2 *vo = deg ? deg->vis_out : nullptr;
+    float *wo = deg ? deg->wt_out : nullptr;
+    int *list = deg ? deg->list : nullptr, *nlist = deg ? deg->nlist : nullptr;
+#define SDE_PREP(DM, DG)                                                                                         \
+    hipLaunchKernelGGL((k_prep<DM, DG>), dim3(nb), dim3(kThreads), 0, s, vis, wt, u, v, w, n, cp, tp, sentinel, \
+                       b.keys, b.stage, wcp, gwt, dblk, ndrop, vo, wo, list, nlist, list_seg_cap(n))
+    const int dm = dens ? (dens->gwt ? 2 : 1) : 0;
+    if (deg) {
+        if (dm == 2) SDE_PREP(2, true);
+        else if (dm == 1) SDE_PREP(1, true);
+        else SDE_PREP(0, true);
+    } else {
+        if (dm == 2) SDE_PREP(2, false);
+        else if (dm == 1) SDE_PREP(1, false);
+        else SDE_PREP(0, false);
+    }
+#undef SDE_PREP
+}
+
+int launch_sort(hipStream_t s, SortBufs &b, int64_t n, const TileParams &tp) {
+    if (n > 0) {
+        if (launch_radix_sort(s, b.temp, b.temp_bytes, b.keys, b.skeys, b.sidx, n, sort_nbits(tp)) != hipSuccess)
+            return -1;
+        hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
+                           b.skeys, n, (uint32_t)sort_nkeys(tp), tp.T * tp.T, tp.ntiles, b.tile_start);
+    } else if (hipMemsetAsync(b.tile_start, 0, sizeof(int) * (tp.ntiles + 1), s) != hipSuccess) {
+        return -1;
+    }
+    hipLaunchKernelGGL(k_items_scan, dim3(1), dim3(1024), 0, s, b.tile_start, tp.ntiles, tp.chunk, b.item_start,
                        b.nitems);
     hipLaunchKernelGGL(k_grid_items, dim3((tp.ntiles + 255) / 256), dim3(256), 0, s, b.tile_start, b.item_start,
                        tp.ntiles, tp.chunk, b.items);
+    return 0;
 }
 
-int grid_fill_blocks(int64_t nvis) { return (int)((nvis > 0 ? nvis + kThreads - 1 : kThreads) / kThreads); }
+void launch_density_runs(hipStream_t s, const SortBufs &b, int64_t n, const float *wt, const TileParams &tp,
+                         const CoordParams &cp, double *gwt, double *blk) {
+    hipLaunchKernelGGL(k_density_runs, dim3(run_blocks(n)), dim3(kThreads), 0, s, b.skeys, b.sidx, b.tile_start, wt,
+                       tp, cp.nu, cp.suppu, gwt, blk);
+}
 
-void launch_grid_fill(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
-                      const float *w, int64_t nvis, const CoordParams &cp, const TileParams &tp, int psf_only,
-                      const float *cfu, const float *cfv, const SortBufs &b, VisRec *rec, double *blk_sums,
-                      double *sums, const DegridOut &deg) {
-    const int nblk = grid_fill_blocks(nvis);
-    if (deg.wt)
-        hipLaunchKernelGGL(k_grid_fill<true>, dim3(nblk), dim3(kThreads), 0, s, vis, wt, u, v, w, nvis, cp, tp,
-                           psf_only, cfu, cfv, b.key_of, b.rank_of, b.start, b.bsum, rec, blk_sums, deg.wt,
-                           deg.vis_out, deg.wt_out, deg.list, deg.nlist);
-    else
-        hipLaunchKernelGGL(k_grid_fill<false>, dim3(nblk), dim3(kThreads), 0, s, vis, wt, u, v, w, nvis, cp, tp,
-                           psf_only, cfu, cfv, b.key_of, b.rank_of, b.start, b.bsum, rec, blk_sums, nullptr,
-                           nullptr, nullptr, nullptr, nullptr);
-    const double num = (double)((float)(cp.nu - 1) * (float)cp.nv);
-    hipLaunchKernelGGL(k_grid_totals, dim3(1), dim3(1024), 0, s, blk_sums, nblk, num, sums);
+int launch_grid_gather(hipStream_t s, const SortBufs &b, int64_t n, const CoordParams &cp, const TileParams &tp,
+                       int flags, const WeightParams &wp, const float *cfu, const float *cfv, VisRec *rec,
+                       double *blk_sums, double *sums) {
+    const int nb = gather_blocks(n);
+#define SDE_GATHER(FL, W)                                                                                    \
+    hipLaunchKernelGGL((k_grid_gather<FL, W>), dim3(nb), dim3(kThreads), 0, s, b.skeys, b.sidx, b.stage,   \
+                       b.tile_start, cp, tp, wp, cfu, cfv, rec, blk_sums)
+    const bool wgt = wp.on != 0;
+    switch (flags) {
+        case kGatherGrid | kGatherDegrid:
+            if (wgt) SDE_GATHER(kGatherGrid | kGatherDegrid, true);
+            else SDE_GATHER(kGatherGrid | kGatherDegrid, false);
+            break;
+        case kGatherGrid:
+            if (wgt) SDE_GATHER(kGatherGrid, true);
+            else SDE_GATHER(kGatherGrid, false);
+            break;
+        case kGatherGrid | kGatherPsf:
+            if (wgt) SDE_GATHER(kGatherGrid | kGatherPsf, true);
+            else SDE_GATHER(kGatherGrid | kGatherPsf, false);
+            break;
+        case kGatherDegrid:
+            SDE_GATHER(kGatherDegrid, false);
+            break;
+        default:
+            return -1;
+    }
+#undef SDE_GATHER
+    if (flags & kGatherGrid) {
+        const double num = (double)((float)(cp.nu - 1) * (float)cp.nv);
+        hipLaunchKernelGGL(k_grid_totals, dim3(1), dim3(1024), 0, s, blk_sums, nb, num, sums);
+    }
+    return 0;
 }
 
 size_t grid_partial_floats(const TileParams &tp, int chmask, int max_items) {

@@ -68,38 +68,107 @@ struct __align__(16) WorkItem {
     int tile, begin, end, pad;
 };
 
-// ---- cell sort shared by gridding and degridding (grid.hip) ----
-// Counting sort of the visibilities by tile-major cell key.  key_of[i] < 0:
-// not gridded (flagged or off the grid); else the sorted position is
-// start[key] + bsum[key >> 13] + rank_of[i].
+// Per-visibility staging record written in the caller's order by the prep
+// pass and gathered in sorted order (one 32-byte read per point).
+struct __align__(32) StageRec {
+    float u, v, w, wt;
+    float2 vis;
+    uint32_t pad0, pad1;
+};
+
+// ---- cell sort shared by gridding and degridding (grid.hip, sort.hip) ----
+// The prep pass gives every visibility its tile-major cell key
+// ((tv*ntu + tu)*T + lv)*T + lu of the pre-shift grid, or the sentinel
+// nkeys (flagged, or dropped by GRDH2D's edge test); bit 31 flags a point
+// whose density weight was added by the prep pass itself.  A stable radix
+// sort over bits [0, nbits) orders (key, index); points of one cell keep
+// their original order.  tile_start[t] (t <= ntiles) is the first sorted
+// slot of tile t, tile_start[ntiles] the number of sorted points.
+constexpr uint32_t kKeyFlag = 0x80000000u;
 struct SortBufs {
-    int *cnt, *start, *bsum, *key_of, *rank_of, *tile_start, *item_start, *nitems;
-    WorkItem *items;
-    int max_items;
+    uint32_t *keys = nullptr;    // per visibility, caller's order
+    uint32_t *skeys = nullptr;   // sorted keys
+    uint32_t *sidx = nullptr;    // original index of each sorted slot
+    StageRec *stage = nullptr;   // per visibility, caller's order
+    void *temp = nullptr;        // radix-sort scratch
+    size_t temp_bytes = 0;
+    int *tile_start = nullptr, *item_start = nullptr, *nitems = nullptr;
+    WorkItem *items = nullptr;
+    int max_items = 0;
 };
 int64_t sort_nkeys(const TileParams &tp);
-int sort_nblocks(const TileParams &tp);
-void launch_sort(hipStream_t s, const float *wt, const float *u, const float *v, const float *w, int64_t nvis,
-                 const CoordParams &cp, const TileParams &tp, SortBufs &b);
+int sort_nbits(const TileParams &tp);
+size_t sort_temp_bytes(int64_t n, int nbits);  // sort.hip
+hipError_t launch_radix_sort(hipStream_t s, void *temp, size_t temp_bytes, const uint32_t *keys_in,
+                             uint32_t *keys_out, uint32_t *idx_out, int64_t n, int nbits);  // sort.hip
 
-// ---- gridding (grid.hip) ----
-int grid_prep_blocks(int64_t nvis);
-// Records at their sorted positions; SUMWT, LSUMWT and the normalisation to
-// sums[0..2].  blk_sums holds 2 doubles per block of grid_fill_blocks().
-// With deg.wt (the WT of a later degridding) the degridding halves are
-// written too: deg.vis_out/wt_out receive VIS/WT of every point the tile
-// path does not finish, and the points it cannot take go to deg.list.
+// Outputs of GRDDEH2D that the prep pass writes in the caller's order for
+// a later degridding: WT of every point, VIS of flagged (copied) and
+// off-grid (zero) points, and the points the tile path cannot take.  The
+// list is segmented, one segment of list_seg_cap(n) slots per prep block
+// (appends go through an LDS counter, no global atomic), with the count of
+// segment b in nlist[b] (prep_blocks(n) counts).
 struct DegridOut {
-    const float *wt = nullptr;
     float2 *vis_out = nullptr;
     float *wt_out = nullptr;
     int *list = nullptr, *nlist = nullptr;
 };
-int grid_fill_blocks(int64_t nvis);
-void launch_grid_fill(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
-                      const float *w, int64_t nvis, const CoordParams &cp, const TileParams &tp, int psf_only,
-                      const float *cfu, const float *cfv, const SortBufs &b, VisRec *rec, double *blk_sums,
-                      double *sums, const DegridOut &deg);
+int list_seg_cap(int64_t n);
+int64_t list_capacity(int64_t n);  // list_seg_cap(n) * prep_blocks(n)
+
+// Density grid built during the sort (robust/uniform weighting with the
+// density cell equal to the gridding cell, WTFOV = 1): the prep pass adds
+// the points whose REAL density cell differs from their DOUBLE gridding
+// cell (or that are outside the gridding footprint) with atomics; the
+// sorted runs of equal cells add the rest, one atomic per run.
+struct DensityOut {
+    CoordParams wcp;       // density-grid coordinate parameters
+    double *gwt = nullptr;
+    double *blk = nullptr;  // per-block partial weight sums
+    unsigned long long *ndrop = nullptr;  // points off the density grid
+};
+
+constexpr int kPrepBlocks = 2048;
+constexpr int kRunBlocks = 2048;
+constexpr int kGatherBlocks = 4096;
+int prep_blocks(int64_t n);
+int run_blocks(int64_t n);
+int gather_blocks(int64_t n);
+
+// Keys and staging records; with dens the exception weights (and their
+// partial sums in dens->blk[0 .. prep_blocks)); with deg the per-point
+// degridding outputs.
+void launch_prep(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
+                 const float *w, int64_t n, const CoordParams &cp, const TileParams &tp, SortBufs &b,
+                 const DensityOut *dens, const DegridOut *deg);
+// Radix sort, tile ranges and work items.
+int launch_sort(hipStream_t s, SortBufs &b, int64_t n, const TileParams &tp);
+// Density runs over the sorted keys: partial sums to blk[0 .. run_blocks).
+void launch_density_runs(hipStream_t s, const SortBufs &b, int64_t n, const float *wt, const TileParams &tp,
+                         const CoordParams &cp, double *gwt, double *blk);
+
+// Robust-weighting parameters of the gather (grd/grdruw2d.f:184-240).
+struct WeightParams {
+    int on = 0, az = 0;
+    float rob1 = 0.0f, rpa = 0.0f, rbscale = 0.0f, pi_f = 0.0f;
+    CoordParams wcp;
+    const double *gwt = nullptr, *wsums = nullptr;
+};
+WeightParams make_weight_params(const CoordParams &wcp, int az, const float *robust, float pi_f,
+                                const double *gwt, const double *wsums);
+
+// Records in sorted order from the staging records: flags kGatherGrid
+// (gridding half, with the weights of wp when on; SUMWT, LSUMWT and the
+// normalisation to sums[0..2] through blk_sums, 2 doubles per block of
+// gather_blocks()), kGatherPsf (PSF only: VIS = 1), kGatherDegrid
+// (degridding half).  Combinations: Grid, Grid|Psf, Grid|Degrid, Degrid;
+// returns -1 for any other.
+constexpr int kGatherGrid = 1, kGatherPsf = 2, kGatherDegrid = 4;
+int launch_grid_gather(hipStream_t s, const SortBufs &b, int64_t n, const CoordParams &cp, const TileParams &tp,
+                       int flags, const WeightParams &wp, const float *cfu, const float *cfv, VisRec *rec,
+                       double *blk_sums, double *sums);
+
+// ---- gridding (grid.hip) ----
 // chmask: bit 0 = dirty (2 floats), bit 1 = PSF (1 float)
 int launch_grid_tiles(hipStream_t s, const VisRec *sorted, const WorkItem *items, const int *nitems,
                       int max_items, const TileParams &tp, int chmask, const float *cfu, const float *cfv,
@@ -211,16 +280,68 @@ __device__ __forceinline__ int degrid_half(const CoordParams &cp, const TilePara
     return kDegTile;
 }
 
-// Append the lanes with flag set to list (one atomic per wave).
-__device__ __forceinline__ void wave_append(bool flag, int value, int *list, int *nlist) {
+// Density-grid cell in REAL arithmetic (grd/grduwt2d.f:89-110,
+// grd/grdruw2d.f:139-160): 1-based (uc, vc) and the rotated (ul, vl).
+__device__ __forceinline__ void wcell(const CoordParams &cp, float u, float v, float w, int &uc, int &vc,
+                                      float &ul, float &vl) {
+    if (cp.doshift) {
+        ul = (float)(cp.shift[0] * (double)u + cp.shift[1] * (double)v + cp.shift[2] * (double)w);
+        vl = (float)(cp.shift[3] * (double)u + cp.shift[4] * (double)v + cp.shift[5] * (double)w);
+    } else {
+        ul = u;
+        vl = v;
+    }
+    float ucell = cp.uscale * ul + cp.uoffset;
+    float vcell = cp.vscale * vl + cp.voffset;
+    if (ucell > 0.0f) {
+        ucell = ucell + (float)cp.uorigin;
+        vcell = vcell + (float)cp.vorigin;
+    } else {
+        ucell = -ucell + (float)cp.uorigin;
+        vcell = -vcell + (float)cp.vorigin;
+    }
+    uc = (int)roundf(ucell);
+    vc = (int)roundf(vcell);
+}
+
+// New weight of one point with WT > 0 from the density grid (the REAL
+// cell value) and the weighting sums wsums = {sum WT, sum GWT^2, F2, D2}
+// (grd/grdruw2d.f:184-240; GRDUWT2D is F2 = 1, D2 = 0).  Returns -1 for a
+// point off the density grid or on an empty cell; *off reports the former.
+__device__ __forceinline__ float robust_weight(const CoordParams &wcp, float u, float v, float w, float wti,
+                                               const double *gwt, const double *wsums, int az, float rob1,
+                                               float rpa, float rbscale, float pi_f, bool *off) {
+    int uc, vc;
+    float ul, vl;
+    wcell(wcp, u, v, w, uc, vc, ul, vl);
+    *off = uc < 1 || uc > wcp.nu || vc < 1 || vc > wcp.nv;
+    if (*off) return -1.0f;
+    float f2 = (float)wsums[2];
+    const float d2 = (float)wsums[3];
+    if (az) {
+        float t = atan2f(vl, ul);
+        t = fmodf(t - rpa + 2.5f * pi_f, pi_f) - pi_f / 2.0f;
+        const float rb = rob1 - fabsf(t) * rbscale;
+        const float q = 5.0f * powf(10.0f, -rb);
+        f2 = (float)((double)(q * q) / (wsums[1] / wsums[0]));
+    }
+    const float g = (float)gwt[(size_t)(vc - 1) * wcp.nu + uc - 1];
+    if (!(g > 0.0f)) return -1.0f;
+    if (uc == wcp.uorigin && vc != wcp.vorigin) return 0.5f * wti / (g * f2 + d2);
+    return wti / (g * f2 + d2);
+}
+
+// Append the lanes with flag set to a workgroup's list segment: one LDS
+// atomic per wave on the segment's counter (called by every lane).
+__device__ __forceinline__ void seg_append(bool flag, int value, int *seg, int *lds_count) {
     const unsigned long long m = __ballot(flag);
     if (m) {
         const int lane = threadIdx.x & 63;
         const int leader = __ffsll((long long)m) - 1;
         int base = 0;
-        if (lane == leader) base = atomicAdd(nlist, __popcll(m));
+        if (lane == leader) base = atomicAdd(lds_count, __popcll(m));
         base = __shfl(base, leader);
-        if (flag) list[base + __popcll(m & ((1ull << lane) - 1))] = value;
+        if (flag) seg[base + __popcll(m & ((1ull << lane) - 1))] = value;
     }
 }
 
@@ -229,14 +350,14 @@ __device__ __forceinline__ void wave_append(bool flag, int value, int *list, int
 void launch_degrid(hipStream_t s, const float2 *grid, const float *u, const float *v,
                    const float *w, const float *wt_in, const float2 *vis_in, int64_t nvis,
                    const CoordParams &cp, const float *cfu, const float *cfv, float2 *vis_out,
-                   float *wt_out, unsigned long long *nbad, const int *list = nullptr,
-                   const int *nlist = nullptr);
-// Tile path: degridding halves at the cell-sorted positions when no
-// gridding fill wrote them (points near the u = 0 axis, or not in the cell
-// sort, go to the list for launch_degrid).
-void launch_degrid_fill(hipStream_t s, const float *u, const float *v, const float *w, const float2 *vis_in,
-                        int64_t nvis, const CoordParams &cp, const TileParams &tp, const SortBufs &b,
-                        VisRec *rec, const DegridOut &deg);
+                   float *wt_out, unsigned long long *nbad);
+// The same over the segmented list the prep pass of nvis points wrote.
+void launch_degrid_list(hipStream_t s, const float2 *grid, const float *u, const float *v, const float *w,
+                        const float *wt_in, const float2 *vis_in, int64_t nvis, const CoordParams &cp,
+                        const float *cfu, const float *cfv, float2 *vis_out, float *wt_out,
+                        unsigned long long *nbad, const int *list, const int *counts);
+// Tile path: one thread per record of the degridding halves (points near
+// the u = 0 axis, or not in the cell sort, go to the list for launch_degrid).
 int launch_degrid_tiles(hipStream_t s, const VisRec *rec, const WorkItem *items, const int *nitems,
                         int max_items, const TileParams &tp, const float2 *grid, const float *cfu,
                         const float *cfv, const CoordParams &cp, float2 *vis_out, float *wt_out,

@@ -314,3 +314,38 @@ def test_plan_uvmap_and_vismodel_small_field(small):
     np.testing.assert_array_equal(mw, w32)
     e, e32 = assert_vis_rel_ref(mv, v32, v64, "C1 vismodel")
     print("C1 vismodel rel err gpu %.2e ref32 %.2e" % (e, e32))
+
+
+@pytest.mark.parametrize("rmode,robust,ds,fov,shifted", [
+    ("NORM", (0.0, 0.0, 0.0), 0.0, 1.0, False),   # density grid from the sorted runs
+    ("NONE", (0.0, 0.0, 0.0), 0.0, 1.0, True),    # uniform, shifted phase centre: runs plus exceptions
+    ("AZ-NORM", (0.5, -0.5, 30.0), 0.0, 1.0, False),
+    ("ABS", (0.5, 0.0, 0.0), 0.2, 0.5, False),    # WTFOV != 1: density grid by atomics
+])
+def test_plan_weighted_uvmap(small, rmode, robust, ds, fov, shifted):
+    """The plan's density weighting (GRDRUW2D/GRDUWT2D inside uvmap) against
+    the oracle: images by the reference-relative image rule, the weights
+    within 1e-5 relative (device libm in AZ-NORM) of the FP32 restatement."""
+    d = small
+    ra, dec = 0.5, 0.7
+    ra_img, dec_img = (ra + 2e-4, dec + 1e-4) if shifted else (ra, dec)
+    s = imaging.crdshift(ra, dec, ra_img, dec_img)
+    p = imaging.Plan(d["nx"], None, d["cellsize"], "SF", s, weighting=1, fov=fov, rmode=rmode, robust=robust, ds=ds)
+    try:
+        p.set_vis(d["vis"], d["wt"], d["u"], d["v"], d["w"])
+        p.uvmap()
+        dirty, psf = p.images()
+        neww = p.weights()
+        info = p.info()
+    finally:
+        p.close()
+    cfg = po.make_cfg(d["nx"], cellsize=d["cellsize"], fov=fov, rmode=rmode, robust=robust, ds=ds, ra=ra, dec=dec,
+                      ra_img=ra_img, dec_img=dec_img)
+    d32, p32, sw32, rc, w32 = po.uvmap(cfg, d["vis"], d["wt"], d["u"], d["v"], d["w"])
+    d64, p64, sw64, rc64, _ = po.uvmap(cfg, d["vis"], d["wt"], d["u"], d["v"], d["w"], fp64=True)
+    assert rc == 0 and rc64 == 0
+    ed = assert_image(dirty, d32, d64, "%s dirty" % rmode)
+    ep = assert_image(psf, p32, p64, "%s psf" % rmode)
+    print(rmode, "dirty", ed, "psf", ep)
+    np.testing.assert_allclose(neww, w32, rtol=1e-5, atol=0)
+    assert info["ndropped"] == int(np.sum((w32 == -1.0) & (d["wt"] > 0)))
