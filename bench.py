@@ -32,17 +32,18 @@ METRIC = "gridded+degridded visibilities/sec at ≤1e-5 peak err vs FP64; p50/p9
 PEAK_HBM_GBS = 8000.0
 
 
-# Kernels launched inside each timed region of the plan (rocprofv3 names).
+# Kernels launched inside each timed region of the plan (rocprofv3 names;
+# a name ending in "*" matches every kernel with that prefix).
 REGION_KERNELS = {
-    "wt_density": ["k_wt_density"],
+    "prep": ["k_prep<2, true>"],
+    "cell_sort": ["rocprim::*", "k_tile_bounds", "k_items_scan", "k_grid_items"],
+    "wt_density": ["k_density_runs"],
     "wt_finish": ["k_wt_sumsq", "k_wt_finish"],
-    "wt_apply": ["k_wt_apply"],
-    "cell_sort": ["k_sort_count", "k_scan_local", "k_scan_tiles", "k_grid_items"],
-    "grid_fill": ["k_grid_fill<true>", "k_grid_totals"],
-    "grid_tiles": ["k_grid_tiles<32, 3>"],
+    "grid_gather": ["k_grid_gather<5, true>", "k_grid_totals"],
+    "grid_tiles": ["k_grid_tiles<16, 3, 4>"],
     "grid_reduce": ["k_grid_reduce"],
-    "degrid": ["k_degrid_tiles<32, 3>"],
-    "degrid_axis": ["k_degrid"],
+    "degrid": ["k_degrid_tiles<16, 3>"],
+    "degrid_axis": ["k_degrid_list"],
 }
 
 
@@ -54,18 +55,15 @@ def alg_bytes(region, nvis, nu, nv, nx, ny, nitems, tile):
     n2 = nx * ny * 4                     # one FP32 image
     part = nitems * 3 * (tile + 6) ** 2 * 4   # FP32 partial tiles (dirty re/im + PSF)
     win = nitems * (tile + 8) ** 2 * 8        # degridding windows
-    ntu, ntv = -(-nu // tile), -(-nv // tile)
-    nkeys = ntu * ntv * tile * tile
     table = {
-        # wt, u, v, w in; one 8-byte read-modify-write of the density cell
-        "wt_density": 16 * nvis + 16 * nvis,
-        # wt, u, v, w in; the density cell read; the new weight out
-        "wt_apply": 16 * nvis + 8 * nvis + 4 * nvis,
-        # wt, u, v, w in; one 4-byte count update; key and rank out; counts cleared, read, offsets written
-        "cell_sort": 16 * nvis + 8 * nvis + 8 * nvis + 12 * nkeys,
-        # vis, wt, u, v, w, WT of the prediction, key, rank and the cell offset in; the 32-byte record
-        # and the prediction WT out
-        "grid_fill": 40 * nvis + 36 * nvis,
+        # vis, wt, u, v, w in; key, 32-byte staging record and the prediction WT out
+        "prep": 24 * nvis + 40 * nvis,
+        # keys in; sorted keys and indices out; sorted keys read once more for the tile ranges
+        "cell_sort": 4 * nvis + 8 * nvis + 4 * nvis,
+        # sorted keys, indices and weights in; the density grid (FP64) written once
+        "wt_density": 12 * nvis + nu * nv * 8,
+        # sorted keys and indices, the staging record and the density cell in; the 32-byte record out
+        "grid_gather": 8 * nvis + 32 * nvis + 32 * nvis + nu * nv * 8,
         # the 16-byte gridding half in; partial tiles out
         "grid_tiles": 16 * nvis + part,
         "grid_reduce": part + 2 * g,
@@ -83,13 +81,21 @@ def alg_bytes(region, nvis, nu, nv, nx, ny, nitems, tile):
 def read_traffic(region, workload_tag):
     """Measured HBM bytes per launch of a region: the calibrated FETCH_SIZE +
     WRITE_SIZE of its kernels in the committed rocprofv3 summary
-    profiles/kernels_<workload>.md, or None when absent."""
+    profiles/kernels_<workload>.md (bytes per dispatch x dispatches, divided
+    by the jobs of the profiled run -- one prep pass per job), or None."""
     from sde_amd import profsum
     rows = profsum.read_table(os.path.join(ROOT, "profiles", "kernels_%s.md" % workload_tag))
     names = REGION_KERNELS.get(region)
-    if not rows or not names or any(k not in rows for k in names):
+    anchor = rows.get(REGION_KERNELS["prep"][0])
+    if not rows or not names or not anchor:
         return None
-    return round(sum(rows[k]["hbm_bytes"] for k in names))
+    total = 0.0
+    for k in names:
+        hits = [r for n, r in rows.items() if (n.startswith(k[:-1]) if k.endswith("*") else n == k)]
+        if not hits:
+            return None
+        total += sum(r["hbm_bytes"] * r["dispatches"] for r in hits)
+    return round(total / anchor["dispatches"])
 
 
 def cpu_baseline(vis, wt, u, v, w, nx, cell, nfull):
@@ -234,7 +240,7 @@ def main():
     dom = max(timings.items(), key=lambda kv: kv[1][0])
     kname, (kms, kcount) = dom
     avg_s = kms / kcount * 1e-3
-    tile = args.tile or (32 if plan.nu * plan.nv >= (1 << 20) else 16)
+    tile = args.tile or 16
     ab = alg_bytes(kname, nvis, plan.nu, plan.nv, nx, nx, info["nitems"], tile)
     achieved = ab / avg_s / 1e9 if ab else None
     roof = {"kernel": kname, "bound": "hbm", "achieved": round(achieved, 1) if achieved else None,

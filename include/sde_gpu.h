@@ -136,7 +136,7 @@ typedef struct {
     float ds;
     int tile;              /* gridding tile edge in cells: 0 = tuned default, else 16 or 32 */
     int chunk;             /* visibilities per work item: 0 = tuned default */
-    int waves;             /* waves per gridding workgroup: 0 = tuned default, else 1, 2, 4 */
+    int waves;             /* waves per gridding workgroup: 0 = tuned default, else 4 or 8 */
 } sde_imaging_cfg;
 
 /* Per-visibility accounting of the last run. */

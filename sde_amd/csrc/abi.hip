@@ -335,17 +335,21 @@ CoordParams make_cp(const double *shift, float uscale, float uoffset, float vsca
     return cp;
 }
 
-TileParams make_tp(int nu, int nv, int T, int chunk) {
+TileParams make_tp(int nu, int nv, int T, int chunk, int waves = 4) {
     TileParams tp;
     tp.T = T;
     tp.ntu = (nu + T - 1) / T;
     tp.ntv = (nv + T - 1) / T;
     tp.ntiles = tp.ntu * tp.ntv;
     tp.chunk = chunk;
+    tp.waves = waves;
     return tp;
 }
 
-int default_tile(int nu, int nv) { return (int64_t)nu * nv >= (1 << 20) ? 32 : 16; }
+// 16-cell tiles: the FP64 tile (22^2 cells x 3 channels, 11.6 KB) leaves
+// room for many workgroups per CU, and the halo overlap stays modest
+// (measured best at 256^2 and 4096^2).
+int default_tile(int, int) { return 16; }
 int default_chunk(int64_t nvis) { return nvis > (1 << 22) ? 8192 : 2048; }
 
 int max_items_for(const TileParams &tp, int64_t nvis) {
@@ -360,12 +364,12 @@ using MarkE = std::function<void()>;
 // outputs), the radix sort, tile ranges and work items.
 int run_sort(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v, const float *w,
              int64_t nvis, const CoordParams &cp, const TileParams &tp, SortBufs &sb, const DensityOut *dens,
-             const DegridOut *deg, const MarkB &mb, const MarkE &me) {
+             const DegridOut *deg, const MarkB &mb, const MarkE &me, bool bounds = true) {
     mb("prep");
     launch_prep(s, vis, wt, u, v, w, nvis, cp, tp, sb, dens, deg);
     me();
     mb("cell_sort");
-    const int rc = launch_sort(s, sb, nvis, tp);
+    const int rc = launch_sort(s, sb, nvis, tp, bounds);
     me();
     if (rc) {
         set_plan_error("cell sort failed");
@@ -378,7 +382,7 @@ int run_sort(hipStream_t s, const float2 *vis, const float *wt, const float *u, 
 // Gridding on a cell sort: gathered records (with the degridding halves
 // when deg), LDS tiles, reduce (fold, shift, normalise).
 int run_grid(hipStream_t s, int64_t nvis, const CoordParams &cp, const TileParams &tp, int chmask, bool deg,
-             const WeightParams &wp, const float *cfu, const float *cfv, const SortBufs &sb, VisRec *rec,
+             const WeightParams &wp, const float *cfu, const float *cfv, const SortBufs &sb, const Records &rec,
              float *partial, double *blk, double *sums, float2 *dirty, float2 *psf, const MarkB &mb,
              const MarkE &me) {
     const int flags = kGatherGrid | (chmask == 2 ? kGatherPsf : 0) | (deg ? kGatherDegrid : 0);
@@ -390,7 +394,7 @@ int run_grid(hipStream_t s, int64_t nvis, const CoordParams &cp, const TileParam
         return -1;
     }
     mb("grid_tiles");
-    if (launch_grid_tiles(s, rec, sb.items, sb.nitems, sb.max_items, tp, chmask, cfu, cfv, cp, partial)) {
+    if (launch_grid_tiles(s, rec.g, sb.items, sb.nitems, sb.max_items, tp, chmask, cfu, cfv, cp, partial)) {
         me();
         set_plan_error("unsupported gridding tile T=%d", tp.T);
         return -1;
@@ -514,19 +518,21 @@ void grdh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, flo
     float *dcfu = ctx_buf<float>(c, "cfu", ncfu), *dcfv = ctx_buf<float>(c, "cfv", ncfv);
     sb.keys = ctx_buf<uint32_t>(c, "keys", nn);
     sb.skeys = ctx_buf<uint32_t>(c, "skeys", nn);
-    sb.sidx = ctx_buf<uint32_t>(c, "sidx", nn);
+    sb.vals = ctx_buf<uint64_t>(c, "vals", nn);
+    sb.svals = ctx_buf<uint64_t>(c, "svals", nn);
     sb.stage = ctx_buf<StageRec>(c, "stage", nn);
     sb.temp = ctx_buf<char>(c, "sort_temp", sb.temp_bytes > 0 ? sb.temp_bytes : 1);
     sb.tile_start = ctx_buf<int>(c, "tile_start", tp.ntiles + 1);
     sb.item_start = ctx_buf<int>(c, "item_start", tp.ntiles + 1);
     sb.nitems = ctx_buf<int>(c, "nitems", 1);
     sb.items = ctx_buf<WorkItem>(c, "items", sb.max_items);
-    VisRec *rec = ctx_buf<VisRec>(c, "rec", nn);
+    Records rec;
+    rec.g = ctx_buf<GridRec>(c, "recg", nn);
     float *partial = ctx_buf<float>(c, "partial", grid_partial_floats(tp, chmask, sb.max_items));
     double *blk = ctx_buf<double>(c, "blk", 2 * (size_t)nblk), *dsums = ctx_buf<double>(c, "sums", 4);
     float2 *dg = ctx_buf<float2>(c, "grid", (size_t)*nu * *nv);
-    if (!dvis || !dwt || !du || !dv || !dw || !dcfu || !dcfv || !sb.keys || !sb.skeys || !sb.sidx || !sb.stage ||
-        !sb.temp || sb.temp_bytes == 0 || !sb.tile_start || !sb.item_start || !sb.nitems || !sb.items || !rec ||
+    if (!dvis || !dwt || !du || !dv || !dw || !dcfu || !dcfv || !sb.keys || !sb.skeys || !sb.vals || !sb.svals || !sb.stage ||
+        !sb.temp || sb.temp_bytes == 0 || !sb.tile_start || !sb.item_start || !sb.nitems || !sb.items || !rec.g ||
         !partial || !blk || !dsums || !dg) {
         report_error(kErrNoMem, R, "device allocation failed");
         return;
@@ -893,7 +899,7 @@ struct sde_plan {
     // outputs are written) for the resident visibilities
     bool filled = false;
     bool weighted = false;  // the density grid and weighting sums of the last uvmap are resident
-    VisRec *d_rec = nullptr;
+    Records rec;
     double *d_wsq = nullptr;
     int *d_list = nullptr, *d_nlist = nullptr;
     float *d_partial = nullptr;
@@ -971,7 +977,7 @@ int plan_reserve_vis(sde_plan *p, int64_t n) {
     if (plan_alloc(&p->d_vis, c) || plan_alloc(&p->d_wt, c) || plan_alloc(&p->d_u, c) || plan_alloc(&p->d_v, c) ||
         plan_alloc(&p->d_w, c) || plan_alloc(&p->d_wteff, c) || plan_alloc(&p->d_mvis, c) ||
         plan_alloc(&p->d_mwt, c) || plan_alloc(&p->sb.keys, c) || plan_alloc(&p->sb.skeys, c) ||
-        plan_alloc(&p->sb.sidx, c) || plan_alloc(&p->sb.stage, c) || plan_alloc(&p->d_rec, c) ||
+        plan_alloc(&p->sb.vals, c) || plan_alloc(&p->sb.svals, c) || plan_alloc(&p->sb.stage, c) || plan_alloc(&p->rec.g, c) || plan_alloc(&p->rec.d, c) ||
         plan_alloc(&p->d_list, (size_t)list_capacity(n)))
         return -1;
     const size_t tb = sort_temp_bytes(n, sort_nbits(p->tp));
@@ -1004,7 +1010,7 @@ int plan_reserve_items(sde_plan *p) {
 
 // Cell sort of the resident visibilities; the prep pass also writes the
 // per-point degridding outputs into the model arrays.
-int plan_sort(sde_plan *p, const DensityOut *dens) {
+int plan_sort(sde_plan *p, const DensityOut *dens, bool bounds = true) {
     auto mb = [p](const char *nm) { plan_mark_begin(p, nm); };
     auto me = [p]() { plan_mark_end(p); };
     DegridOut deg;
@@ -1014,7 +1020,7 @@ int plan_sort(sde_plan *p, const DensityOut *dens) {
     deg.nlist = p->d_nlist;
     p->filled = false;
     return run_sort(p->stream, p->d_vis, p->d_wt, p->d_u, p->d_v, p->d_w, p->nvis, p->cp, p->tp, p->sb, dens, &deg,
-                    mb, me);
+                    mb, me, bounds);
 }
 
 }  // namespace
@@ -1072,7 +1078,12 @@ int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **out) {
         delete p;
         return -1;
     }
-    p->tp = make_tp(p->nu, p->nv, T, cfg->chunk > 0 ? cfg->chunk : 2048);
+    if (cfg->waves != 0 && cfg->waves != 4 && cfg->waves != 8) {
+        set_plan_error("sde_plan_create: unsupported waves %d (4 or 8)", cfg->waves);
+        delete p;
+        return -1;
+    }
+    p->tp = make_tp(p->nu, p->nv, T, cfg->chunk > 0 ? cfg->chunk : 2048, cfg->waves ? cfg->waves : 4);
     if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
         set_plan_error("sde_plan_create: stream creation failed");
         delete p;
@@ -1116,8 +1127,8 @@ int sde_plan_destroy(sde_plan *p) {
     plan_collect(p);
     for (auto e : p->pool) (void)hipEventDestroy(e);
     void *bufs[] = {p->d_cf, p->d_ccf, p->d_vis, p->d_wt, p->d_u, p->d_v, p->d_w, p->d_wteff, p->d_mvis, p->d_mwt,
-                    p->sb.keys, p->sb.skeys, p->sb.sidx, p->sb.stage, p->sb.temp, p->sb.tile_start,
-                    p->sb.item_start, p->sb.nitems, p->sb.items, p->d_rec, p->d_list, p->d_nlist,
+                    p->sb.keys, p->sb.skeys, p->sb.vals, p->sb.svals, p->sb.stage, p->sb.temp, p->sb.tile_start,
+                    p->sb.item_start, p->sb.nitems, p->sb.items, p->rec.g, p->rec.d, p->d_list, p->d_nlist,
                     p->d_partial, p->d_blk, p->d_sums, p->d_grids, p->d_gwork, p->d_mgrid, p->d_dirty, p->d_psf, p->d_img_in,
                     p->d_peak, p->d_gwt, p->d_wblk, p->d_wsums, p->d_cnt, p->d_wsq};
     for (void *b : bufs)
@@ -1175,13 +1186,15 @@ int sde_plan_uvmap(sde_plan *p) {
         dens.blk = p->d_wblk;
         dens.ndrop = p->d_cnt;
     }
-    if (plan_sort(p, weighting ? &dens : nullptr)) return -1;
+    // with the density from the sorted runs, that pass also writes the tile ranges
+    if (plan_sort(p, weighting ? &dens : nullptr, !runs)) return -1;
     if (weighting) {
         int nwblk;
         if (runs) {
             const int np = prep_blocks(n);
             plan_mark_begin(p, "wt_density");
-            launch_density_runs(s, p->sb, n, p->d_wt, p->tp, p->cp, p->d_gwt, p->d_wblk + np);
+            launch_density_runs(s, p->sb, n, p->tp, p->cp, p->d_gwt, p->d_wblk + np);
+            launch_items(s, p->sb, p->tp);
             plan_mark_end(p);
             nwblk = np + run_blocks(n);
         } else {
@@ -1203,7 +1216,7 @@ int sde_plan_uvmap(sde_plan *p) {
     auto me = [p]() { plan_mark_end(p); };
     // the gather also writes the degridding halves of a prediction on these
     // visibilities (model VIS/WT are complete after sde_plan_vismodel)
-    if (run_grid(s, n, p->cp, p->tp, 3, true, wp, p->d_cf, p->d_cf, p->sb, p->d_rec, p->d_partial, p->d_blk,
+    if (run_grid(s, n, p->cp, p->tp, 3, true, wp, p->d_cf, p->d_cf, p->sb, p->rec, p->d_partial, p->d_blk,
                  p->d_sums, gd, gp, mb, me))
         return -1;
     p->filled = true;
@@ -1253,7 +1266,7 @@ int sde_plan_vismodel(sde_plan *p, const float *image) {
         if (plan_sort(p, nullptr)) return -1;
         plan_mark_begin(p, "degrid_gather");
         const int rc = launch_grid_gather(s, p->sb, p->nvis, p->cp, p->tp, kGatherDegrid, WeightParams(), p->d_cf,
-                                          p->d_cf, p->d_rec, p->d_blk, p->d_sums);
+                                          p->d_cf, p->rec, p->d_blk, p->d_sums);
         plan_mark_end(p);
         if (rc) {
             set_plan_error("record gather failed");
@@ -1262,7 +1275,7 @@ int sde_plan_vismodel(sde_plan *p, const float *image) {
         p->filled = true;
     }
     plan_mark_begin(p, "degrid");
-    if (launch_degrid_tiles(s, p->d_rec, p->sb.items, p->sb.nitems, p->sb.max_items, p->tp, p->d_mgrid, p->d_cf,
+    if (launch_degrid_tiles(s, p->rec.d, p->sb.items, p->sb.nitems, p->sb.max_items, p->tp, p->d_mgrid, p->d_cf,
                             p->d_cf, p->cp, p->d_mvis, p->d_mwt, p->d_cnt + 1)) {
         plan_mark_end(p);
         set_plan_error("unsupported degridding tile T=%d", p->tp.T);

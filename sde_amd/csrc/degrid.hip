@@ -133,7 +133,7 @@ __global__ __launch_bounds__(256) void k_degrid_list(
 // the gather runs in the reference's loop order (OFFV outer, OFFU inner) in
 // REAL arithmetic, so a given grid degrids to the reference's values.
 template <int T, int S>
-__global__ __launch_bounds__(256) void k_degrid_tiles(const VisRec *__restrict__ rec,
+__global__ __launch_bounds__(256) void k_degrid_tiles(const DegridRec *__restrict__ rec,
                                                       const WorkItem *__restrict__ items,
                                                       const int *__restrict__ nitems, TileParams tp,
                                                       const float2 *__restrict__ grid,
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(256) void k_degrid_tiles(const VisRec *__restrict__
     for (int k = threadIdx.x; k < ncfv; k += blockDim.x) cflv[k] = cfv_g[cp.osampv - poffv + k];
     __syncthreads();
     for (int r = wi.begin + threadIdx.x; r < wi.end; r += blockDim.x) {
-        const DegridRec d = rec[r].d;
+        const DegridRec d = rec[r];
         if (d.idx == kNoDegrid) continue;
         const int lu = d.pk & 0xff, lv = (d.pk >> 8) & 0xff;
         const int pu = (d.pk >> 16) & 0xff, pv = d.pk >> 24;
@@ -376,7 +376,7 @@ void launch_degrid_list(hipStream_t s, const float2 *grid, const float *u, const
 }
 
 template <int T, int S>
-static void launch_degrid_tiles_t(hipStream_t s, const VisRec *rec, const WorkItem *items, const int *nitems,
+static void launch_degrid_tiles_t(hipStream_t s, const DegridRec *rec, const WorkItem *items, const int *nitems,
                                   int max_items, const TileParams &tp, const float2 *grid, const float *cfu,
                                   const float *cfv, const CoordParams &cp, float2 *vis_out, float *wt_out,
                                   unsigned long long *nbad) {
@@ -387,7 +387,7 @@ static void launch_degrid_tiles_t(hipStream_t s, const VisRec *rec, const WorkIt
                        cfv, cp, vis_out, wt_out, nbad);
 }
 
-int launch_degrid_tiles(hipStream_t s, const VisRec *rec, const WorkItem *items, const int *nitems,
+int launch_degrid_tiles(hipStream_t s, const DegridRec *rec, const WorkItem *items, const int *nitems,
                         int max_items, const TileParams &tp, const float2 *grid, const float *cfu,
                         const float *cfv, const CoordParams &cp, float2 *vis_out, float *wt_out,
                         unsigned long long *nbad) {

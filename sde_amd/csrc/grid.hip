@@ -116,7 +116,8 @@ template <int DM, bool DEG>
 __global__ __launch_bounds__(kThreads) void k_prep(
     const float2 *__restrict__ vis, const float *__restrict__ wt, const float *__restrict__ u,
     const float *__restrict__ v, const float *__restrict__ w, int64_t n, CoordParams cp, TileParams tp,
-    uint32_t sentinel, uint32_t *__restrict__ keys, StageRec *__restrict__ stage, CoordParams wcp,
+    uint32_t sentinel, uint32_t *__restrict__ keys, uint64_t *__restrict__ vals, StageRec *__restrict__ stage,
+    CoordParams wcp,
     double *__restrict__ gwt, double *__restrict__ dblk, unsigned long long *__restrict__ ndrop,
     float2 *__restrict__ vis_out, float *__restrict__ wt_out, int *__restrict__ list, int *__restrict__ nlist,
     int seg_cap) {
@@ -178,6 +179,7 @@ __global__ __launch_bounds__(kThreads) void k_prep(
                 vis_out[i] = vv;
             }
             keys[i] = key;
+            vals[i] = (uint64_t)(uint32_t)i | ((uint64_t)__float_as_uint(wti) << 32);
         }
         if (DEG) seg_append(to_list, (int)i, seg, &s_nlist);
     }
@@ -241,26 +243,35 @@ __global__ __launch_bounds__(1024) void k_items_scan(const int *__restrict__ til
     }
 }
 
-// Density grid from the sorted keys (points whose density cell is their
-// gridding cell): a segmented sum over each wave's runs of equal keys, one
-// FP64 atomic per run.  Flagged points were added by the prep pass.
+// Density grid from the sorted keys and weights (points whose density cell
+// is their gridding cell): a segmented sum over each wave's runs of equal
+// keys, one FP64 atomic per run.  Flagged points were added by the prep
+// pass.  The same pass writes the tile ranges (as k_tile_bounds).
 __global__ __launch_bounds__(kThreads) void k_density_runs(
-    const uint32_t *__restrict__ skeys, const uint32_t *__restrict__ sidx, const int *__restrict__ tile_start,
-    const float *__restrict__ wt, TileParams tp, int nu, int suppu, double *__restrict__ gwt,
-    double *__restrict__ blk) {
-    const int64_t nvalid = tile_start[tp.ntiles];
+    const uint32_t *__restrict__ skeys, const uint64_t *__restrict__ svals, int64_t n, uint32_t nkeys,
+    TileParams tp, int nu, int suppu, double *__restrict__ gwt, double *__restrict__ blk,
+    int *__restrict__ tile_start) {
     const int lane = threadIdx.x & 63;
     const int tt = tp.T * tp.T;
     double dsum = 0.0, unused = 0.0;
     const int64_t stride = (int64_t)gridDim.x * kThreads;
-    for (int64_t base = (int64_t)blockIdx.x * kThreads; base < nvalid; base += stride) {
+    for (int64_t base = (int64_t)blockIdx.x * kThreads; base < n; base += stride) {
         const int64_t p = base + threadIdx.x;
-        const bool valid = p < nvalid;
-        const uint32_t k = valid ? skeys[p] : 0xffffffffu;
-        const uint32_t key = valid ? (k & ~kKeyFlag) : 0xffffffffu;
-        double x = (valid && !(k & kKeyFlag)) ? (double)wt[sidx[p]] : 0.0;
+        const bool inr = p < n;
+        const uint32_t k = inr ? skeys[p] : 0xffffffffu;
+        const uint32_t key = k & ~kKeyFlag;
+        const bool valid = inr && key < nkeys;
+        double x = (valid && !(k & kKeyFlag)) ? (double)sval_wt(svals[p]) : 0.0;
         dsum += x;
-        const uint32_t prev = (uint32_t)__shfl_up((int)key, 1);
+        uint32_t prev = (uint32_t)__shfl_up((int)key, 1);
+        if (lane == 0) prev = (inr && p > 0) ? (skeys[p - 1] & ~kKeyFlag) : 0xffffffffu;
+        if (inr) {
+            const int t = tile_of_key(key, nkeys, tt, tp.ntiles);
+            const int tprev = p == 0 ? -1 : tile_of_key(prev, nkeys, tt, tp.ntiles);
+            for (int q = tprev + 1; q <= t; q++) tile_start[q] = (int)p;
+            if (p == n - 1)
+                for (int q = t + 1; q <= tp.ntiles; q++) tile_start[q] = (int)n;
+        }
         const bool head = lane == 0 || prev != key;
         const unsigned long long heads = __ballot(head);
         const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
@@ -282,6 +293,7 @@ __global__ __launch_bounds__(kThreads) void k_density_runs(
     if (threadIdx.x == 0) blk[blockIdx.x] = dsum;
 }
 
+// Work items of every tile: chunks of at most `chunk` records.
 __global__ void k_grid_items(const int *__restrict__ tile_start, const int *__restrict__ item_start, int ntiles,
                              int chunk, WorkItem *__restrict__ items) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -306,10 +318,10 @@ __global__ void k_grid_items(const int *__restrict__ tile_start, const int *__re
 // the degridding half follows GRDDEH2D's (grd/grddeh2d.f:92-118).
 template <int FL, bool WGT>
 __global__ __launch_bounds__(kThreads) void k_grid_gather(
-    const uint32_t *__restrict__ skeys, const uint32_t *__restrict__ sidx, const StageRec *__restrict__ stage,
+    const uint32_t *__restrict__ skeys, const uint64_t *__restrict__ svals, const StageRec *__restrict__ stage,
     const int *__restrict__ tile_start, CoordParams cp, TileParams tp, WeightParams wp,
-    const float *__restrict__ cfu, const float *__restrict__ cfv, VisRec *__restrict__ rec,
-    double *__restrict__ blk_sums) {
+    const float *__restrict__ cfu, const float *__restrict__ cfv, GridRec *__restrict__ rec_g,
+    DegridRec *__restrict__ rec_d, double *__restrict__ blk_sums) {
     const int64_t nvalid = tile_start[tp.ntiles];
     double sw = 0.0, ls = 0.0;
     const int cfoffu = (cp.suppu + 1) * cp.osampu + 1;
@@ -317,9 +329,10 @@ __global__ __launch_bounds__(kThreads) void k_grid_gather(
     const int poffu = (cp.osampu + 1) / 2, poffv = (cp.osampv + 1) / 2;
     for (int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x; p < nvalid; p += (int64_t)gridDim.x * kThreads) {
         const uint32_t key = skeys[p] & ~kKeyFlag;
-        const uint32_t i = sidx[p];
+        const uint32_t i = sval_index(svals[p]);
         const StageRec sr = stage[i];
-        VisRec out;
+        GridRec og;
+        DegridRec od;
         if (FL & kGatherGrid) {
             GridRec r;
             const uint32_t local = key & (uint32_t)(tp.T * tp.T - 1);
@@ -353,12 +366,11 @@ __global__ __launch_bounds__(kThreads) void k_grid_gather(
                 r.re = (float)((double)wti * (double)c.fr);
                 r.im = (float)((double)wti * (double)c.fi);
             }
-            out.g = r;
+            og = r;
         }
-        if (FL & kGatherDegrid) degrid_half(cp, tp, sr.u, sr.v, sr.w, (int)key, i, out.d);
-        if ((FL & kGatherGrid) && (FL & kGatherDegrid)) rec[p] = out;
-        else if (FL & kGatherGrid) rec[p].g = out.g;
-        else rec[p].d = out.d;
+        if (FL & kGatherDegrid) degrid_half(cp, tp, sr.u, sr.v, sr.w, (int)key, i, od);
+        if (FL & kGatherGrid) rec_g[p] = og;
+        if (FL & kGatherDegrid) rec_d[p] = od;
     }
     if (FL & kGatherGrid) {
         block_sum2(sw, ls);
@@ -395,46 +407,56 @@ __global__ __launch_bounds__(1024) void k_grid_totals(const double *__restrict__
     }
 }
 
-// Kernel phase tables in LDS carry one pad word per 32 so that the seven
-// taps of one point (phases p + OSAMP*k) fall on distinct banks.
-__device__ __forceinline__ int cf_pad(int j) { return j + (j >> 5); }
-__host__ __device__ inline int cf_lds_words(int n) { return n + (n >> 5) + 1; }
+// Kernel phase tables in LDS: one row per tap, row o holding the 1 + OSAMP
+// phases p of CF(p + OSAMP*o) (relative to the table's first used entry),
+// rows cf_row_stride() words apart -- 4 modulo 32 -- so the seven taps of a
+// point (same phase, seven rows) fall on distinct banks and each lookup is
+// a per-lane row base plus the point's phase.
+__host__ __device__ inline int cf_row_stride(int osamp) {
+    const int n0 = osamp + 1;
+    return n0 + (((4 - n0) % 32) + 32) % 32;
+}
 
-template <int T, int CHM>
-__global__ __launch_bounds__(256) void k_grid_tiles(const VisRec *__restrict__ sorted,
-                                                    const WorkItem *__restrict__ items,
-                                                    const int *__restrict__ nitems, const float *__restrict__ cfu_g,
-                                                    const float *__restrict__ cfv_g, CoordParams cp,
-                                                    float *__restrict__ partial) {
+template <int T, int CHM, int W>
+__global__ __launch_bounds__(64 * W) void k_grid_tiles(const GridRec *__restrict__ sorted,
+                                                       const WorkItem *__restrict__ items,
+                                                       const int *__restrict__ nitems, const float *__restrict__ cfu_g,
+                                                       const float *__restrict__ cfv_g, CoordParams cp,
+                                                       float *__restrict__ partial) {
     constexpr int P = T + 2 * kHalo;
     constexpr int NC = P * P;
     constexpr int NCH = ((CHM & 1) ? 2 : 0) + ((CHM & 2) ? 1 : 0);
     constexpr int PSF = (CHM & 1) ? 2 : 0;
-    constexpr int W = 4;
     extern __shared__ double smem_d[];
     double *S = smem_d;                          // [NCH][NC] FP64 sums
-    float *cfl = (float *)(smem_d + NCH * NC);   // padded phase tables
+    float *cfl = (float *)(smem_d + NCH * NC);   // phase tables, one row per tap
     const int item = blockIdx.x;
     if (item >= *nitems) return;
     const WorkItem wi = items[item];
 
     const int poffu = (cp.osampu + 1) / 2, poffv = (cp.osampv + 1) / 2;
-    const int ncfu = cp.osampu * (2 * cp.suppu + 1) + 2;
-    const int ncfv = cp.osampv * (2 * cp.suppv + 1) + 2;
+    const int ntapu = 2 * cp.suppu + 1, ntapv = 2 * cp.suppv + 1;
+    const int rsu = cf_row_stride(cp.osampu), rsv = cf_row_stride(cp.osampv);
     const int baseu = cp.osampu - poffu, basev = cp.osampv - poffv;
-    float *cflu = cfl, *cflv = cfl + cf_lds_words(ncfu);
-    for (int k = threadIdx.x; k < ncfu; k += 64 * W) cflu[cf_pad(k)] = cfu_g[baseu + k];
-    for (int k = threadIdx.x; k < ncfv; k += 64 * W) cflv[cf_pad(k)] = cfv_g[basev + k];
+    float *cflu = cfl, *cflv = cfl + ntapu * rsu;
+    for (int k = threadIdx.x; k < ntapu * (cp.osampu + 1); k += 64 * W) {
+        const int o = k / (cp.osampu + 1), ph = k - o * (cp.osampu + 1);
+        cflu[o * rsu + ph] = cfu_g[baseu + ph + cp.osampu * o];
+    }
+    for (int k = threadIdx.x; k < ntapv * (cp.osampv + 1); k += 64 * W) {
+        const int o = k / (cp.osampv + 1), ph = k - o * (cp.osampv + 1);
+        cflv[o * rsv + ph] = cfv_g[basev + ph + cp.osampv * o];
+    }
     for (int k = threadIdx.x; k < NCH * NC; k += 64 * W) S[k] = 0.0;
     __syncthreads();
 
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int ntapu = 2 * cp.suppu + 1;
     const int ou = lane % ntapu, ov = lane / ntapu;
-    const bool active = ov < 2 * cp.suppv + 1;
+    const bool active = ov < ntapv;
     const int lane_off = (ov - cp.suppv + kHalo) * P + (ou - cp.suppu + kHalo);
-    const int cfu_off = cp.osampu * ou, cfv_off = cp.osampv * ov;
+    const float *rowu = cflu + (active ? ou * rsu : 0);
+    const float *rowv = cflv + (active ? ov * rsv : 0);
 
     const int len = wi.end - wi.begin;
     const int r0 = wi.begin + (int)(((int64_t)len * wave) / W);
@@ -449,9 +471,9 @@ __global__ __launch_bounds__(256) void k_grid_tiles(const VisRec *__restrict__ s
     // records arrive 64 at a time, one per lane (the next batch in flight
     // while this one is consumed), and are broadcast to the wave with readlane
     const GridRec empty = {0u, 0.0f, 0.0f, 0.0f};
-    GridRec mine = r0 + lane < r1 ? sorted[r0 + lane].g : empty;
+    GridRec mine = r0 + lane < r1 ? sorted[r0 + lane] : empty;
     for (int b = r0; b < r1; b += 64) {
-        const GridRec nxt = b + 64 + lane < r1 ? sorted[b + 64 + lane].g : empty;
+        const GridRec nxt = b + 64 + lane < r1 ? sorted[b + 64 + lane] : empty;
         const int nb = min(64, r1 - b);
         for (int j = 0; j < nb; j++) {
             const uint32_t pk = (uint32_t)__builtin_amdgcn_readlane((int)mine.pk, j);
@@ -470,7 +492,7 @@ __global__ __launch_bounds__(256) void k_grid_tiles(const VisRec *__restrict__ s
             }
             if (active) {
                 const int pu = (pk >> 16) & 0xff, pv = pk >> 24;
-                const double cw = (double)(cflu[cf_pad(pu + cfu_off)] * cflv[cf_pad(pv + cfv_off)]);
+                const double cw = (double)(rowu[pu] * rowv[pv]);
                 if (CHM & 1) {
                     const float re = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.re), j));
                     const float im = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.im), j));
@@ -681,7 +703,7 @@ void launch_prep(hipStream_t s, const float2 *vis, const float *wt, const float 
     int *list = deg ? deg->list : nullptr, *nlist = deg ? deg->nlist : nullptr;
 #define SDE_PREP(DM, DG)                                                                                         \
     hipLaunchKernelGGL((k_prep<DM, DG>), dim3(nb), dim3(kThreads), 0, s, vis, wt, u, v, w, n, cp, tp, sentinel, \
-                       b.keys, b.stage, wcp, gwt, dblk, ndrop, vo, wo, list, nlist, list_seg_cap(n))
+                       b.keys, b.vals, b.stage, wcp, gwt, dblk, ndrop, vo, wo, list, nlist, list_seg_cap(n))
     const int dm = dens ? (dens->gwt ? 2 : 1) : 0;
     if (deg) {
         if (dm == 2) SDE_PREP(2, true);
@@ -695,35 +717,41 @@ void launch_prep(hipStream_t s, const float2 *vis, const float *wt, const float 
 #undef SDE_PREP
 }
 
-int launch_sort(hipStream_t s, SortBufs &b, int64_t n, const TileParams &tp) {
+int launch_sort(hipStream_t s, SortBufs &b, int64_t n, const TileParams &tp, bool bounds) {
     if (n > 0) {
-        if (launch_radix_sort(s, b.temp, b.temp_bytes, b.keys, b.skeys, b.sidx, n, sort_nbits(tp)) != hipSuccess)
+        if (launch_radix_sort(s, b.temp, b.temp_bytes, b.keys, b.skeys, b.vals, b.svals, n, sort_nbits(tp)) !=
+            hipSuccess)
             return -1;
-        hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
-                           b.skeys, n, (uint32_t)sort_nkeys(tp), tp.T * tp.T, tp.ntiles, b.tile_start);
+        if (bounds)
+            hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
+                               b.skeys, n, (uint32_t)sort_nkeys(tp), tp.T * tp.T, tp.ntiles, b.tile_start);
     } else if (hipMemsetAsync(b.tile_start, 0, sizeof(int) * (tp.ntiles + 1), s) != hipSuccess) {
         return -1;
     }
+    if (bounds || n <= 0) launch_items(s, b, tp);
+    return 0;
+}
+
+void launch_items(hipStream_t s, SortBufs &b, const TileParams &tp) {
     hipLaunchKernelGGL(k_items_scan, dim3(1), dim3(1024), 0, s, b.tile_start, tp.ntiles, tp.chunk, b.item_start,
                        b.nitems);
     hipLaunchKernelGGL(k_grid_items, dim3((tp.ntiles + 255) / 256), dim3(256), 0, s, b.tile_start, b.item_start,
                        tp.ntiles, tp.chunk, b.items);
-    return 0;
 }
 
-void launch_density_runs(hipStream_t s, const SortBufs &b, int64_t n, const float *wt, const TileParams &tp,
+void launch_density_runs(hipStream_t s, const SortBufs &b, int64_t n, const TileParams &tp,
                          const CoordParams &cp, double *gwt, double *blk) {
-    hipLaunchKernelGGL(k_density_runs, dim3(run_blocks(n)), dim3(kThreads), 0, s, b.skeys, b.sidx, b.tile_start, wt,
-                       tp, cp.nu, cp.suppu, gwt, blk);
+    hipLaunchKernelGGL(k_density_runs, dim3(run_blocks(n)), dim3(kThreads), 0, s, b.skeys, b.svals, n,
+                       (uint32_t)sort_nkeys(tp), tp, cp.nu, cp.suppu, gwt, blk, b.tile_start);
 }
 
 int launch_grid_gather(hipStream_t s, const SortBufs &b, int64_t n, const CoordParams &cp, const TileParams &tp,
-                       int flags, const WeightParams &wp, const float *cfu, const float *cfv, VisRec *rec,
+                       int flags, const WeightParams &wp, const float *cfu, const float *cfv, const Records &rec,
                        double *blk_sums, double *sums) {
     const int nb = gather_blocks(n);
 #define SDE_GATHER(FL, W)                                                                                    \
-    hipLaunchKernelGGL((k_grid_gather<FL, W>), dim3(nb), dim3(kThreads), 0, s, b.skeys, b.sidx, b.stage,   \
-                       b.tile_start, cp, tp, wp, cfu, cfv, rec, blk_sums)
+    hipLaunchKernelGGL((k_grid_gather<FL, W>), dim3(nb), dim3(kThreads), 0, s, b.skeys, b.svals, b.stage,  \
+                       b.tile_start, cp, tp, wp, cfu, cfv, rec.g, rec.d, blk_sums)
     const bool wgt = wp.on != 0;
     switch (flags) {
         case kGatherGrid | kGatherDegrid:
@@ -758,32 +786,35 @@ size_t grid_partial_floats(const TileParams &tp, int chmask, int max_items) {
     return (size_t)max_items * nch * P * P;
 }
 
-template <int T, int CHM>
-static void launch_tiles_t(hipStream_t s, const VisRec *sorted, const WorkItem *items, const int *nitems,
+template <int T, int CHM, int W>
+static void launch_tiles_t(hipStream_t s, const GridRec *sorted, const WorkItem *items, const int *nitems,
                            int max_items, const float *cfu, const float *cfv, const CoordParams &cp,
                            float *partial) {
     constexpr int P = T + 2 * kHalo;
     constexpr int NCH = ((CHM & 1) ? 2 : 0) + ((CHM & 2) ? 1 : 0);
-    const int ncfu = cp.osampu * (2 * cp.suppu + 1) + 2;
-    const int ncfv = cp.osampv * (2 * cp.suppv + 1) + 2;
-    const size_t lds = sizeof(double) * (size_t)NCH * P * P + sizeof(float) * (cf_lds_words(ncfu) + cf_lds_words(ncfv));
-    hipLaunchKernelGGL((k_grid_tiles<T, CHM>), dim3(max_items), dim3(256), lds, s, sorted, items, nitems, cfu, cfv,
-                       cp, partial);
+    const size_t tables = (size_t)(2 * cp.suppu + 1) * cf_row_stride(cp.osampu) +
+                          (size_t)(2 * cp.suppv + 1) * cf_row_stride(cp.osampv);
+    const size_t lds = sizeof(double) * (size_t)NCH * P * P + sizeof(float) * tables;
+    hipLaunchKernelGGL((k_grid_tiles<T, CHM, W>), dim3(max_items), dim3(64 * W), lds, s, sorted, items, nitems, cfu,
+                       cfv, cp, partial);
 }
 
-int launch_grid_tiles(hipStream_t s, const VisRec *sorted, const WorkItem *items, const int *nitems,
+int launch_grid_tiles(hipStream_t s, const GridRec *sorted, const WorkItem *items, const int *nitems,
                       int max_items, const TileParams &tp, int chmask, const float *cfu, const float *cfv,
                       const CoordParams &cp, float *partial) {
     if (max_items <= 0) return 0;
-#define SDE_TILE_CASE(TT)                                                                                       \
-    if (tp.T == TT) {                                                                                           \
-        if (chmask == 1) launch_tiles_t<TT, 1>(s, sorted, items, nitems, max_items, cfu, cfv, cp, partial);     \
-        else if (chmask == 2) launch_tiles_t<TT, 2>(s, sorted, items, nitems, max_items, cfu, cfv, cp, partial); \
-        else launch_tiles_t<TT, 3>(s, sorted, items, nitems, max_items, cfu, cfv, cp, partial);                 \
-        return 0;                                                                                               \
+    const int W = tp.waves == 8 ? 8 : 4;
+#define SDE_TILE_CASE(TT, WW)                                                                                      \
+    if (tp.T == TT && W == WW) {                                                                                   \
+        if (chmask == 1) launch_tiles_t<TT, 1, WW>(s, sorted, items, nitems, max_items, cfu, cfv, cp, partial);     \
+        else if (chmask == 2) launch_tiles_t<TT, 2, WW>(s, sorted, items, nitems, max_items, cfu, cfv, cp, partial); \
+        else launch_tiles_t<TT, 3, WW>(s, sorted, items, nitems, max_items, cfu, cfv, cp, partial);                 \
+        return 0;                                                                                                  \
     }
-    SDE_TILE_CASE(16)
-    SDE_TILE_CASE(32)
+    SDE_TILE_CASE(16, 4)
+    SDE_TILE_CASE(16, 8)
+    SDE_TILE_CASE(32, 4)
+    SDE_TILE_CASE(32, 8)
 #undef SDE_TILE_CASE
     return -1;
 }

@@ -35,6 +35,7 @@ struct TileParams {
     int ntu, ntv;   // tiles along u and v
     int ntiles;
     int chunk;      // visibilities per work item
+    int waves;      // waves per gridding workgroup (4 or 8)
 };
 
 // Sorted gridding record, 16 bytes:
@@ -57,11 +58,11 @@ struct __align__(16) DegridRec {
 };
 constexpr uint32_t kNoDegrid = 0xffffffffu;
 
-// One 32-byte record per sorted slot: the gridding and degridding halves of
-// a visibility land in one scattered 32-byte write.
-struct __align__(32) VisRec {
-    GridRec g;
-    DegridRec d;
+// The sorted records: gridding and degridding halves in two arrays, so each
+// tile kernel streams only the half it uses.
+struct Records {
+    GridRec *g = nullptr;
+    DegridRec *d = nullptr;
 };
 
 struct __align__(16) WorkItem {
@@ -85,10 +86,13 @@ struct __align__(32) StageRec {
 // their original order.  tile_start[t] (t <= ntiles) is the first sorted
 // slot of tile t, tile_start[ntiles] the number of sorted points.
 constexpr uint32_t kKeyFlag = 0x80000000u;
+// Sort values: original index | WT bits << 32, so the sorted runs carry
+// their weights (the density pass reads no staging record).
 struct SortBufs {
     uint32_t *keys = nullptr;    // per visibility, caller's order
+    uint64_t *vals = nullptr;    // per visibility, caller's order
     uint32_t *skeys = nullptr;   // sorted keys
-    uint32_t *sidx = nullptr;    // original index of each sorted slot
+    uint64_t *svals = nullptr;   // sorted values
     StageRec *stage = nullptr;   // per visibility, caller's order
     void *temp = nullptr;        // radix-sort scratch
     size_t temp_bytes = 0;
@@ -100,7 +104,10 @@ int64_t sort_nkeys(const TileParams &tp);
 int sort_nbits(const TileParams &tp);
 size_t sort_temp_bytes(int64_t n, int nbits);  // sort.hip
 hipError_t launch_radix_sort(hipStream_t s, void *temp, size_t temp_bytes, const uint32_t *keys_in,
-                             uint32_t *keys_out, uint32_t *idx_out, int64_t n, int nbits);  // sort.hip
+                             uint32_t *keys_out, const uint64_t *vals_in, uint64_t *vals_out, int64_t n,
+                             int nbits);  // sort.hip
+__device__ __forceinline__ uint32_t sval_index(uint64_t v) { return (uint32_t)v; }
+__device__ __forceinline__ float sval_wt(uint64_t v) { return __uint_as_float((uint32_t)(v >> 32)); }
 
 // Outputs of GRDDEH2D that the prep pass writes in the caller's order for
 // a later degridding: WT of every point, VIS of flagged (copied) and
@@ -141,10 +148,13 @@ int gather_blocks(int64_t n);
 void launch_prep(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
                  const float *w, int64_t n, const CoordParams &cp, const TileParams &tp, SortBufs &b,
                  const DensityOut *dens, const DegridOut *deg);
-// Radix sort, tile ranges and work items.
-int launch_sort(hipStream_t s, SortBufs &b, int64_t n, const TileParams &tp);
-// Density runs over the sorted keys: partial sums to blk[0 .. run_blocks).
-void launch_density_runs(hipStream_t s, const SortBufs &b, int64_t n, const float *wt, const TileParams &tp,
+// Radix sort; with bounds also the tile ranges and work items (else the
+// density runs write the ranges and launch_items follows them).
+int launch_sort(hipStream_t s, SortBufs &b, int64_t n, const TileParams &tp, bool bounds = true);
+void launch_items(hipStream_t s, SortBufs &b, const TileParams &tp);
+// Density runs over the sorted keys and weights, with the tile ranges:
+// partial sums to blk[0 .. run_blocks).
+void launch_density_runs(hipStream_t s, const SortBufs &b, int64_t n, const TileParams &tp,
                          const CoordParams &cp, double *gwt, double *blk);
 
 // Robust-weighting parameters of the gather (grd/grdruw2d.f:184-240).
@@ -165,12 +175,12 @@ WeightParams make_weight_params(const CoordParams &wcp, int az, const float *rob
 // returns -1 for any other.
 constexpr int kGatherGrid = 1, kGatherPsf = 2, kGatherDegrid = 4;
 int launch_grid_gather(hipStream_t s, const SortBufs &b, int64_t n, const CoordParams &cp, const TileParams &tp,
-                       int flags, const WeightParams &wp, const float *cfu, const float *cfv, VisRec *rec,
+                       int flags, const WeightParams &wp, const float *cfu, const float *cfv, const Records &rec,
                        double *blk_sums, double *sums);
 
 // ---- gridding (grid.hip) ----
 // chmask: bit 0 = dirty (2 floats), bit 1 = PSF (1 float)
-int launch_grid_tiles(hipStream_t s, const VisRec *sorted, const WorkItem *items, const int *nitems,
+int launch_grid_tiles(hipStream_t s, const GridRec *sorted, const WorkItem *items, const int *nitems,
                       int max_items, const TileParams &tp, int chmask, const float *cfu, const float *cfv,
                       const CoordParams &cp, float *partial);
 void launch_grid_reduce(hipStream_t s, const float *partial, const int *item_start, const TileParams &tp,
@@ -358,7 +368,7 @@ void launch_degrid_list(hipStream_t s, const float2 *grid, const float *u, const
                         unsigned long long *nbad, const int *list, const int *counts);
 // Tile path: one thread per record of the degridding halves (points near
 // the u = 0 axis, or not in the cell sort, go to the list for launch_degrid).
-int launch_degrid_tiles(hipStream_t s, const VisRec *rec, const WorkItem *items, const int *nitems,
+int launch_degrid_tiles(hipStream_t s, const DegridRec *rec, const WorkItem *items, const int *nitems,
                         int max_items, const TileParams &tp, const float2 *grid, const float *cfu,
                         const float *cfv, const CoordParams &cp, float2 *vis_out, float *wt_out,
                         unsigned long long *nbad);
