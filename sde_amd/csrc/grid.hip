@@ -469,12 +469,15 @@ __global__ __launch_bounds__(64 * W) void k_grid_tiles(const GridRec *__restrict
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     int cur = -1;
     // records arrive 64 at a time, one per lane (the next batch in flight
-    // while this one is consumed), and are broadcast to the wave with readlane
+    // while this one is consumed), and are broadcast to the wave with
+    // readlane; the loads are whole 16-byte loads of a clamped slot (lanes
+    // past the end re-read the wave's last record, which is never broadcast
+    // from them); an empty range (wave-uniform) loads nothing
     const GridRec empty = {0u, 0.0f, 0.0f, 0.0f};
-    GridRec mine = r0 + lane < r1 ? sorted[r0 + lane] : empty;
+    GridRec mine = r0 < r1 ? sorted[min(r0 + lane, r1 - 1)] : empty;
     for (int b = r0; b < r1; b += 64) {
-        const GridRec nxt = b + 64 + lane < r1 ? sorted[b + 64 + lane] : empty;
-        const int nb = min(64, r1 - b);
+        const GridRec nxt = sorted[min(b + 64 + lane, r1 - 1)];
+        const int nb = __builtin_amdgcn_readfirstlane(min(64, r1 - b));
         for (int j = 0; j < nb; j++) {
             const uint32_t pk = (uint32_t)__builtin_amdgcn_readlane((int)mine.pk, j);
             const int cell = (int)(pk & 0xffff);
