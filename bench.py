@@ -36,8 +36,9 @@ PEAK_HBM_GBS = 8000.0
 # a name ending in "*" matches every kernel with that prefix).
 REGION_KERNELS = {
     "prep": ["k_prep<2, true>"],
-    "cell_sort": ["rocprim::*", "k_tile_bounds", "k_items_scan", "k_grid_items"],
-    "wt_density": ["k_density_runs"],
+    "cell_sort": ["rocprim::*"],
+    # the density runs also write the tile ranges; the work items follow in the same region
+    "wt_density": ["k_density_runs", "k_items_scan", "k_grid_items"],
     "wt_finish": ["k_wt_sumsq", "k_wt_finish"],
     "grid_gather": ["k_grid_gather<5, true>", "k_grid_totals"],
     "grid_tiles": ["k_grid_tiles<16, 3, 4>"],
@@ -58,9 +59,9 @@ def alg_bytes(region, nvis, nu, nv, nx, ny, nitems, tile):
     table = {
         # vis, wt, u, v, w in; key, 32-byte staging record and the prediction WT out
         "prep": 24 * nvis + 40 * nvis,
-        # keys in; sorted keys and indices out; sorted keys read once more for the tile ranges
-        "cell_sort": 4 * nvis + 8 * nvis + 4 * nvis,
-        # sorted keys, indices and weights in; the density grid (FP64) written once
+        # keys and values in; sorted keys and values out
+        "cell_sort": 12 * nvis + 12 * nvis,
+        # sorted keys and values (index, weight) in; the density grid (FP64) written once
         "wt_density": 12 * nvis + nu * nv * 8,
         # sorted keys and indices, the staging record and the density cell in; the 32-byte record out
         "grid_gather": 8 * nvis + 32 * nvis + 32 * nvis + nu * nv * 8,
