@@ -40,8 +40,7 @@ REGION_KERNELS = {
     # the density runs also write the tile ranges; the work items follow in the same region
     "wt_density": ["k_density_runs", "k_items_scan", "k_grid_items"],
     "wt_finish": ["k_wt_sumsq", "k_wt_finish"],
-    "grid_gather": ["k_grid_gather<5, true>", "k_grid_totals"],
-    "grid_tiles": ["k_grid_tiles<16, 3, 4>"],
+    "grid_tiles": ["k_grid_tiles<16, 3, 4, true, 1>", "k_grid_totals"],
     "grid_reduce": ["k_grid_reduce"],
     "degrid": ["k_degrid_tiles<16, 3>"],
     "degrid_axis": ["k_degrid_list"],
@@ -63,10 +62,9 @@ def alg_bytes(region, nvis, nu, nv, nx, ny, nitems, tile):
         "cell_sort": 12 * nvis + 12 * nvis,
         # sorted keys and values (index, weight) in; the density grid (FP64) written once
         "wt_density": 12 * nvis + nu * nv * 8,
-        # sorted keys and indices, the staging record and the density cell in; the 32-byte record out
-        "grid_gather": 8 * nvis + 32 * nvis + 32 * nvis + nu * nv * 8,
-        # the 16-byte gridding half in; partial tiles out
-        "grid_tiles": 16 * nvis + part,
+        # sorted values (index, weight), the 32-byte staging record and the density grid in; the
+        # 16-byte degridding half and the partial tiles out
+        "grid_tiles": 8 * nvis + 32 * nvis + nu * nv * 8 + 16 * nvis + part,
         "grid_reduce": part + 2 * g,
         # the 16-byte degridding half in, the predicted visibility out, each window read once
         "degrid": 16 * nvis + 8 * nvis + win,
@@ -184,6 +182,8 @@ def main():
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--weighting", choices=("robust", "natural"), default="robust",
+                    help="robust NORM R=0 (the benchmark's configuration) or natural weights")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy first (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -199,7 +199,8 @@ def main():
 
     p = synth.large_field_params(nkeep=args.nvis)
     nx, cell = p["nx"], p["cellsize"]
-    plan = imaging.Plan(nx, nx, cell, "SF", weighting=1, fov=1.0, rmode="NORM", robust=(0.0, 0.0, 0.0),
+    plan = imaging.Plan(nx, nx, cell, "SF", weighting=int(args.weighting == "robust"), fov=1.0, rmode="NORM",
+                        robust=(0.0, 0.0, 0.0),
                         tile=args.tile, waves=args.waves, chunk=args.chunk)
     t0 = time.perf_counter()
     plan.synth(p["enu"], p["lat"], p["dec"], p["ha0"], p["dha"], p["ntime"], p["freqs"], p["flux"], p["l"], p["m"],
@@ -255,7 +256,8 @@ def main():
            "vs_baseline": None, "dtype": "f32",
            "data": "synthetic (seeded track and point-source sky generated on the device)",
            "config": {"workload": "plain single field large: 4096^2, 1.0e8 vis (uniform-stride subsample of "
-                                  "351 bl x 14400 x 64 ch), robust NORM R=0, SF 7x7 x128",
+                                  "351 bl x 14400 x 64 ch), %s, SF 7x7 x128"
+                                  % ("robust NORM R=0" if args.weighting == "robust" else "natural weights"),
                       "nvis": nvis, "nx": nx, "cellsize_arcsec": round(cell, 4), "grid": [plan.nv, plan.nu],
                       "tile": args.tile, "waves": args.waves, "chunk": args.chunk,
                       "arithmetic": "FP32 visibilities, grids and images; FP64 coordinates and tile accumulation"},

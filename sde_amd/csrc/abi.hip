@@ -379,24 +379,18 @@ int run_sort(hipStream_t s, const float2 *vis, const float *wt, const float *u, 
     return 0;
 }
 
-// Gridding on a cell sort: gathered records (with the degridding halves
-// when deg), LDS tiles, reduce (fold, shift, normalise).
-int run_grid(hipStream_t s, int64_t nvis, const CoordParams &cp, const TileParams &tp, int chmask, bool deg,
-             const WeightParams &wp, const float *cfu, const float *cfv, const SortBufs &sb, const Records &rec,
-             float *partial, double *blk, double *sums, float2 *dirty, float2 *psf, const MarkB &mb,
+// Gridding on a cell sort: one tile kernel builds the records from the
+// staging records (and writes the degridding halves to rec_d when deg),
+// then the reduce (fold, shift, normalise).  item_sums holds 2 doubles per
+// work item.
+int run_grid(hipStream_t s, const CoordParams &cp, const TileParams &tp, int chmask, bool deg,
+             const WeightParams &wp, const float *cfu, const float *cfv, const SortBufs &sb, DegridRec *rec_d,
+             float *partial, double *item_sums, double *sums, float2 *dirty, float2 *psf, const MarkB &mb,
              const MarkE &me) {
-    const int flags = kGatherGrid | (chmask == 2 ? kGatherPsf : 0) | (deg ? kGatherDegrid : 0);
-    mb("grid_gather");
-    const int rc = launch_grid_gather(s, sb, nvis, cp, tp, flags, wp, cfu, cfv, rec, blk, sums);
-    me();
-    if (rc) {
-        set_plan_error("unsupported record gather (flags %d)", flags);
-        return -1;
-    }
     mb("grid_tiles");
-    if (launch_grid_tiles(s, rec.g, sb.items, sb.nitems, sb.max_items, tp, chmask, cfu, cfv, cp, partial)) {
+    if (launch_grid_tiles(s, sb, tp, chmask, deg, wp, cfu, cfv, cp, rec_d, item_sums, sums, partial)) {
         me();
-        set_plan_error("unsupported gridding tile T=%d", tp.T);
+        set_plan_error("unsupported gridding run (T=%d, channels %d, degridding %d)", tp.T, chmask, (int)deg);
         return -1;
     }
     me();
@@ -507,7 +501,6 @@ void grdh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, flo
                                    *suppu, *suppv, *osampu, *osampv);
     const TileParams tp = make_tp(*nu, *nv, default_tile(*nu, *nv), default_chunk(n));
     const int chmask = *psf ? 2 : 1;
-    const int nblk = gather_blocks(n);
     const size_t nn = (size_t)(n > 0 ? n : 1);
     SortBufs sb;
     sb.max_items = max_items_for(tp, n);
@@ -526,13 +519,11 @@ void grdh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, flo
     sb.item_start = ctx_buf<int>(c, "item_start", tp.ntiles + 1);
     sb.nitems = ctx_buf<int>(c, "nitems", 1);
     sb.items = ctx_buf<WorkItem>(c, "items", sb.max_items);
-    Records rec;
-    rec.g = ctx_buf<GridRec>(c, "recg", nn);
     float *partial = ctx_buf<float>(c, "partial", grid_partial_floats(tp, chmask, sb.max_items));
-    double *blk = ctx_buf<double>(c, "blk", 2 * (size_t)nblk), *dsums = ctx_buf<double>(c, "sums", 4);
+    double *blk = ctx_buf<double>(c, "blk", 2 * (size_t)sb.max_items), *dsums = ctx_buf<double>(c, "sums", 4);
     float2 *dg = ctx_buf<float2>(c, "grid", (size_t)*nu * *nv);
     if (!dvis || !dwt || !du || !dv || !dw || !dcfu || !dcfv || !sb.keys || !sb.skeys || !sb.vals || !sb.svals || !sb.stage ||
-        !sb.temp || sb.temp_bytes == 0 || !sb.tile_start || !sb.item_start || !sb.nitems || !sb.items || !rec.g ||
+        !sb.temp || sb.temp_bytes == 0 || !sb.tile_start || !sb.item_start || !sb.nitems || !sb.items ||
         !partial || !blk || !dsums || !dg) {
         report_error(kErrNoMem, R, "device allocation failed");
         return;
@@ -558,7 +549,7 @@ void grdh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, flo
     auto nb = [](const char *) {};
     auto ne = []() {};
     if (run_sort(s, dvis, dwt, du, dv, dw, n, cp, tp, sb, nullptr, nullptr, nb, ne) ||
-        run_grid(s, n, cp, tp, chmask, false, WeightParams(), dcfu, dcfv, sb, rec, partial, blk, dsums,
+        run_grid(s, cp, tp, chmask, false, WeightParams(), dcfu, dcfv, sb, nullptr, partial, blk, dsums,
                  chmask == 1 ? dg : nullptr, chmask == 2 ? dg : nullptr, nb, ne)) {
         report_error(kErrFatal, R, sde_last_error());
         return;
@@ -895,16 +886,16 @@ struct sde_plan {
     float2 *d_mvis = nullptr;
     float *d_mwt = nullptr;
     SortBufs sb{};
-    // the records hold the degridding halves (and the per-point degridding
-    // outputs are written) for the resident visibilities
+    // the degridding halves (d_recd) and the per-point degridding outputs
+    // are written for the resident visibilities
     bool filled = false;
     bool weighted = false;  // the density grid and weighting sums of the last uvmap are resident
-    Records rec;
+    DegridRec *d_recd = nullptr;
     double *d_wsq = nullptr;
     int *d_list = nullptr, *d_nlist = nullptr;
     float *d_partial = nullptr;
     size_t partial_cap = 0, items_cap = 0;
-    double *d_blk = nullptr, *d_sums = nullptr;
+    double *d_isums = nullptr, *d_sums = nullptr;  // per-work-item SUMWT/LSUMWT; totals
     float2 *d_grids = nullptr, *d_gwork = nullptr, *d_mgrid = nullptr;
     float *d_dirty = nullptr, *d_psf = nullptr, *d_img_in = nullptr;
     int *d_peak = nullptr;
@@ -977,7 +968,7 @@ int plan_reserve_vis(sde_plan *p, int64_t n) {
     if (plan_alloc(&p->d_vis, c) || plan_alloc(&p->d_wt, c) || plan_alloc(&p->d_u, c) || plan_alloc(&p->d_v, c) ||
         plan_alloc(&p->d_w, c) || plan_alloc(&p->d_wteff, c) || plan_alloc(&p->d_mvis, c) ||
         plan_alloc(&p->d_mwt, c) || plan_alloc(&p->sb.keys, c) || plan_alloc(&p->sb.skeys, c) ||
-        plan_alloc(&p->sb.vals, c) || plan_alloc(&p->sb.svals, c) || plan_alloc(&p->sb.stage, c) || plan_alloc(&p->rec.g, c) || plan_alloc(&p->rec.d, c) ||
+        plan_alloc(&p->sb.vals, c) || plan_alloc(&p->sb.svals, c) || plan_alloc(&p->sb.stage, c) || plan_alloc(&p->d_recd, c) ||
         plan_alloc(&p->d_list, (size_t)list_capacity(n)))
         return -1;
     const size_t tb = sort_temp_bytes(n, sort_nbits(p->tp));
@@ -996,7 +987,7 @@ int plan_reserve_vis(sde_plan *p, int64_t n) {
 int plan_reserve_items(sde_plan *p) {
     const int mi = max_items_for(p->tp, p->nvis);
     if ((size_t)mi > p->items_cap) {
-        if (plan_alloc(&p->sb.items, (size_t)mi)) return -1;
+        if (plan_alloc(&p->sb.items, (size_t)mi) || plan_alloc(&p->d_isums, 2 * (size_t)mi)) return -1;
         p->items_cap = mi;
     }
     const size_t pf = grid_partial_floats(p->tp, 3, mi);
@@ -1098,7 +1089,7 @@ int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **out) {
              plan_alloc(&p->d_wblk, 4096) || plan_alloc(&p->d_wsums, 4) || plan_alloc(&p->d_cnt, 4) ||
              plan_alloc(&p->d_wsq, kWtSqBlocks) ||
              plan_alloc(&p->sb.tile_start, p->tp.ntiles + 1) || plan_alloc(&p->sb.item_start, p->tp.ntiles + 1) ||
-             plan_alloc(&p->sb.nitems, 1) || plan_alloc(&p->d_nlist, kPrepBlocks) || plan_alloc(&p->d_blk, 2 * (size_t)kGatherBlocks) ||
+             plan_alloc(&p->sb.nitems, 1) || plan_alloc(&p->d_nlist, kPrepBlocks) ||
              plan_alloc(&p->d_sums, 4);
     if (rc) {
         sde_plan_destroy(p);
@@ -1128,8 +1119,8 @@ int sde_plan_destroy(sde_plan *p) {
     for (auto e : p->pool) (void)hipEventDestroy(e);
     void *bufs[] = {p->d_cf, p->d_ccf, p->d_vis, p->d_wt, p->d_u, p->d_v, p->d_w, p->d_wteff, p->d_mvis, p->d_mwt,
                     p->sb.keys, p->sb.skeys, p->sb.vals, p->sb.svals, p->sb.stage, p->sb.temp, p->sb.tile_start,
-                    p->sb.item_start, p->sb.nitems, p->sb.items, p->rec.g, p->rec.d, p->d_list, p->d_nlist,
-                    p->d_partial, p->d_blk, p->d_sums, p->d_grids, p->d_gwork, p->d_mgrid, p->d_dirty, p->d_psf, p->d_img_in,
+                    p->sb.item_start, p->sb.nitems, p->sb.items, p->d_recd, p->d_list, p->d_nlist,
+                    p->d_partial, p->d_isums, p->d_sums, p->d_grids, p->d_gwork, p->d_mgrid, p->d_dirty, p->d_psf, p->d_img_in,
                     p->d_peak, p->d_gwt, p->d_wblk, p->d_wsums, p->d_cnt, p->d_wsq};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -1214,9 +1205,9 @@ int sde_plan_uvmap(sde_plan *p) {
     float2 *gd = p->d_grids, *gp = p->d_grids + ng;
     auto mb = [p](const char *nm) { plan_mark_begin(p, nm); };
     auto me = [p]() { plan_mark_end(p); };
-    // the gather also writes the degridding halves of a prediction on these
+    // the gridding run also writes the degridding halves of a prediction on these
     // visibilities (model VIS/WT are complete after sde_plan_vismodel)
-    if (run_grid(s, n, p->cp, p->tp, 3, true, wp, p->d_cf, p->d_cf, p->sb, p->rec, p->d_partial, p->d_blk,
+    if (run_grid(s, p->cp, p->tp, 3, true, wp, p->d_cf, p->d_cf, p->sb, p->d_recd, p->d_partial, p->d_isums,
                  p->d_sums, gd, gp, mb, me))
         return -1;
     p->filled = true;
@@ -1265,17 +1256,12 @@ int sde_plan_vismodel(sde_plan *p, const float *image) {
     if (!p->filled) {
         if (plan_sort(p, nullptr)) return -1;
         plan_mark_begin(p, "degrid_gather");
-        const int rc = launch_grid_gather(s, p->sb, p->nvis, p->cp, p->tp, kGatherDegrid, WeightParams(), p->d_cf,
-                                          p->d_cf, p->rec, p->d_blk, p->d_sums);
+        launch_degrid_gather(s, p->sb, p->nvis, p->cp, p->tp, p->d_recd);
         plan_mark_end(p);
-        if (rc) {
-            set_plan_error("record gather failed");
-            return -1;
-        }
         p->filled = true;
     }
     plan_mark_begin(p, "degrid");
-    if (launch_degrid_tiles(s, p->rec.d, p->sb.items, p->sb.nitems, p->sb.max_items, p->tp, p->d_mgrid, p->d_cf,
+    if (launch_degrid_tiles(s, p->d_recd, p->sb.items, p->sb.nitems, p->sb.max_items, p->tp, p->d_mgrid, p->d_cf,
                             p->d_cf, p->cp, p->d_mvis, p->d_mwt, p->d_cnt + 1)) {
         plan_mark_end(p);
         set_plan_error("unsupported degridding tile T=%d", p->tp.T);

@@ -52,8 +52,9 @@ __device__ __forceinline__ void block_sum2(TT &a, TT &b) {
     b = sb[0];
 }
 
-// The GRDH2D coordinate step (grd/grdh2d.f:115-166).  Returns false when the
-// point is dropped by the grid-edge test.
+// The GRDH2D coordinate step (grd/grdh2d.f:115-166), with the phase
+// rotation rot = shift_phase(cp, ug, vg, wg) applied to the visibility when
+// want_vis.  Returns false when the point is dropped by the grid-edge test.
 struct GridCell {
     double ucell, vcell;
     float fr, fi;
@@ -61,14 +62,13 @@ struct GridCell {
 };
 
 __device__ __forceinline__ bool grid_cell(const CoordParams &cp, double ug, double vg, double wg, float2 vv,
-                                          bool want_vis, GridCell &c) {
+                                          float2 rot, bool want_vis, GridCell &c) {
     if (cp.doshift) {
         const double ul = cp.shift[0] * ug + cp.shift[1] * vg + cp.shift[2] * wg;
         const double vl = cp.shift[3] * ug + cp.shift[4] * vg + cp.shift[5] * wg;
         if (want_vis) {
-            const double wl = cp.shift[6] * ug + cp.shift[7] * vg + cp.shift[8] * wg;
-            const double phase = cp.twopi * (wl - wg);
-            const float cr = (float)cos(phase), ci = (float)(-sin(phase));
+            // VIS * CMPLX(COS(PHASE), -SIN(PHASE))
+            const float cr = rot.x, ci = -rot.y;
             c.fr = vv.x * cr - vv.y * ci;
             c.fi = vv.x * ci + vv.y * cr;
         }
@@ -143,12 +143,12 @@ __global__ __launch_bounds__(kThreads) void k_prep(
             sr.w = wf;
             sr.wt = wti;
             sr.vis = vv;
-            sr.pad0 = sr.pad1 = 0u;
+            sr.rot = shift_phase(cp, uf, vf, wf);
             stage[i] = sr;
             uint32_t key = sentinel;
             if (wti > 0.0f) {
                 GridCell c;
-                const bool in = grid_cell(cp, uf, vf, wf, make_float2(0.0f, 0.0f), false, c);
+                const bool in = grid_cell(cp, uf, vf, wf, make_float2(0.0f, 0.0f), sr.rot, false, c);
                 if (in) key = (uint32_t)cell_key(tp, c.nuc + cp.suppu - 1, c.nvc - 1);
                 if (DM > 0) {
                     int uc, vc;
@@ -164,7 +164,8 @@ __global__ __launch_bounds__(kThreads) void k_prep(
                 }
                 if (DEG) {
                     DegridRec d;
-                    const int path = degrid_half(cp, tp, uf, vf, wf, in ? (int)(key & ~kKeyFlag) : -1, (uint32_t)i, d);
+                    const int path = degrid_half(cp, tp, uf, vf, wf, in ? (int)(key & ~kKeyFlag) : -1, (uint32_t)i,
+                                                 sr.rot, d);
                     if (path == kDegOff) {
                         vis_out[i] = make_float2(0.0f, 0.0f);
                         wt_out[i] = -wti;
@@ -310,74 +311,21 @@ __global__ void k_grid_items(const int *__restrict__ tile_start, const int *__re
     }
 }
 
-// Records in sorted order, one thread per sorted slot: the staging record
-// of the slot's visibility is gathered (one 32-byte read) and the record
-// written in place (coalesced).  The gridding half follows GRDH2D's
-// per-point step (grd/grdh2d.f:109-160: phase rotation, conjugation,
-// kernel phases, WT*VIS) with the weight of GRDRUW2D/GRDUWT2D when WGT;
-// the degridding half follows GRDDEH2D's (grd/grddeh2d.f:92-118).
-template <int FL, bool WGT>
-__global__ __launch_bounds__(kThreads) void k_grid_gather(
+// Degridding halves in sorted order, one thread per sorted slot: the
+// staging record of the slot's visibility is gathered (one 32-byte read)
+// and GRDDEH2D's per-point step (grd/grddeh2d.f:92-118) written in place
+// (coalesced).  Used when the records were not built by a gridding run.
+__global__ __launch_bounds__(kThreads) void k_degrid_gather(
     const uint32_t *__restrict__ skeys, const uint64_t *__restrict__ svals, const StageRec *__restrict__ stage,
-    const int *__restrict__ tile_start, CoordParams cp, TileParams tp, WeightParams wp,
-    const float *__restrict__ cfu, const float *__restrict__ cfv, GridRec *__restrict__ rec_g,
-    DegridRec *__restrict__ rec_d, double *__restrict__ blk_sums) {
+    const int *__restrict__ tile_start, CoordParams cp, TileParams tp, DegridRec *__restrict__ rec_d) {
     const int64_t nvalid = tile_start[tp.ntiles];
-    double sw = 0.0, ls = 0.0;
-    const int cfoffu = (cp.suppu + 1) * cp.osampu + 1;
-    const int cfoffv = (cp.suppv + 1) * cp.osampv + 1;
-    const int poffu = (cp.osampu + 1) / 2, poffv = (cp.osampv + 1) / 2;
     for (int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x; p < nvalid; p += (int64_t)gridDim.x * kThreads) {
         const uint32_t key = skeys[p] & ~kKeyFlag;
         const uint32_t i = sval_index(svals[p]);
         const StageRec sr = stage[i];
-        GridRec og;
         DegridRec od;
-        if (FL & kGatherGrid) {
-            GridRec r;
-            const uint32_t local = key & (uint32_t)(tp.T * tp.T - 1);
-            r.pk = (local % tp.T) | ((local / tp.T) << 8);
-            r.wt = 0.0f;
-            r.re = 0.0f;
-            r.im = 0.0f;
-            float wti = sr.wt;
-            if (WGT) {
-                bool off;
-                wti = robust_weight(wp.wcp, sr.u, sr.v, sr.w, wti, wp.gwt, wp.wsums, wp.az, wp.rob1, wp.rpa,
-                                    wp.rbscale, wp.pi_f, &off);
-            }
-            if (wti > 0.0f) {
-                const bool psf = (FL & kGatherPsf) != 0;
-                GridCell c;
-                grid_cell(cp, sr.u, sr.v, sr.w, psf ? make_float2(1.0f, 0.0f) : sr.vis, !psf, c);
-                if (psf) {
-                    c.fr = 1.0f;
-                    c.fi = 0.0f;
-                }
-                const int delui = nint_d((double)cp.osampu * ((double)c.nuc - c.ucell));
-                const int delvi = nint_d((double)cp.osampv * ((double)c.nvc - c.vcell));
-                double su = 0.0, sv = 0.0;
-                for (int o = -cp.suppu; o <= cp.suppu; o++) su += (double)cfu[delui + cfoffu - 1 + cp.osampu * o];
-                for (int o = -cp.suppv; o <= cp.suppv; o++) sv += (double)cfv[delvi + cfoffv - 1 + cp.osampv * o];
-                sw += (double)wti;
-                ls += (double)wti * su * sv;
-                r.pk |= ((uint32_t)(delui + poffu) << 16) | ((uint32_t)(delvi + poffv) << 24);
-                r.wt = wti;
-                r.re = (float)((double)wti * (double)c.fr);
-                r.im = (float)((double)wti * (double)c.fi);
-            }
-            og = r;
-        }
-        if (FL & kGatherDegrid) degrid_half(cp, tp, sr.u, sr.v, sr.w, (int)key, i, od);
-        if (FL & kGatherGrid) rec_g[p] = og;
-        if (FL & kGatherDegrid) rec_d[p] = od;
-    }
-    if (FL & kGatherGrid) {
-        block_sum2(sw, ls);
-        if (threadIdx.x == 0) {
-            blk_sums[2 * blockIdx.x] = sw;
-            blk_sums[2 * blockIdx.x + 1] = ls;
-        }
+        degrid_half(cp, tp, sr.u, sr.v, sr.w, (int)key, i, sr.rot, od);
+        rec_d[p] = od;
     }
 }
 
@@ -417,21 +365,51 @@ __host__ __device__ inline int cf_row_stride(int osamp) {
     return n0 + (((4 - n0) % 32) + 32) % 32;
 }
 
-template <int T, int CHM, int W>
-__global__ __launch_bounds__(64 * W) void k_grid_tiles(const GridRec *__restrict__ sorted,
-                                                       const WorkItem *__restrict__ items,
-                                                       const int *__restrict__ nitems, const float *__restrict__ cfu_g,
-                                                       const float *__restrict__ cfv_g, CoordParams cp,
-                                                       float *__restrict__ partial) {
+// One work item of the gridding: a contiguous run of cell-sorted slots of
+// one tile, accumulated on the tile padded by kHalo, held in LDS in FP64.
+//
+// The gridding records are built by the wave that grids them, from the
+// staging records gathered in sorted order (no record array goes through
+// HBM): GRDH2D's per-point step (grd/grdh2d.f:109-160: phase rotation,
+// conjugation, kernel phases, WT*VIS), with the weight of GRDRUW2D /
+// GRDUWT2D when WGT (grd/grdruw2d.f:184-240); SUMWT and LSUMWT per work
+// item to item_sums.  With DEG the wave also writes the degridding halves
+// of its slots (GRDDEH2D's per-point step, grd/grddeh2d.f:92-118).
+//
+// Each wave takes a contiguous share of the item, 64 slots per batch: the
+// sort values two batches ahead and the staging records one batch ahead
+// are in flight while a batch is built (one slot per lane) and gridded.
+// Gridding: each lane is one kernel tap; the batch's records are broadcast
+// with readlane, consecutive points of one cell (the slots are cell-sorted)
+// accumulate in FP64 registers -- the product of two REAL factors is exact
+// in DOUBLE, so one fused multiply-add per channel is the whole update --
+// and are added into the LDS tile when the cell changes.
+//
+// Occupancy: 5 waves per SIMD (at most 96 VGPRs, no spills), so the
+// gathers of one wave are covered by the tap loops of the others.
+template <int T, int CHM, int W, bool DEG, int WGT>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) void k_grid_tiles(
+    const uint64_t *__restrict__ svals, const StageRec *__restrict__ stage, const WorkItem *__restrict__ items,
+    const int *__restrict__ nitems, const float *__restrict__ cfu_g, const float *__restrict__ cfv_g, CoordParams cp,
+    TileParams tp, WeightParams wp, DegridRec *__restrict__ rec_d, double *__restrict__ item_sums,
+    float *__restrict__ partial) {
     constexpr int P = T + 2 * kHalo;
     constexpr int NC = P * P;
     constexpr int NCH = ((CHM & 1) ? 2 : 0) + ((CHM & 2) ? 1 : 0);
     constexpr int PSF = (CHM & 1) ? 2 : 0;
+    constexpr bool PSF_ONLY = CHM == 2;
     extern __shared__ double smem_d[];
+    __shared__ double s_red[2 * W];
     double *S = smem_d;                          // [NCH][NC] FP64 sums
     float *cfl = (float *)(smem_d + NCH * NC);   // phase tables, one row per tap
     const int item = blockIdx.x;
-    if (item >= *nitems) return;
+    if (item >= *nitems) {
+        if (threadIdx.x == 0) {
+            item_sums[2 * item] = 0.0;
+            item_sums[2 * item + 1] = 0.0;
+        }
+        return;
+    }
     const WorkItem wi = items[item];
 
     const int poffu = (cp.osampu + 1) / 2, poffv = (cp.osampv + 1) / 2;
@@ -457,58 +435,103 @@ __global__ __launch_bounds__(64 * W) void k_grid_tiles(const GridRec *__restrict
     const int lane_off = (ov - cp.suppv + kHalo) * P + (ou - cp.suppu + kHalo);
     const float *rowu = cflu + (active ? ou * rsu : 0);
     const float *rowv = cflv + (active ? ov * rsv : 0);
+    // the sort key of a point is the cell (NUC + SUPPU - 1, NVC - 1) of the
+    // pre-shift grid, tile-major; this item's tile starts at (tu*T, tv*T)
+    const int tile = wi.tile;
+    const int lu0 = (tile % tp.ntu) * T - cp.suppu + 1, lv0 = (tile / tp.ntu) * T + 1;
 
     const int len = wi.end - wi.begin;
     const int r0 = wi.begin + (int)(((int64_t)len * wave) / W);
     const int r1 = wi.begin + (int)(((int64_t)len * (wave + 1)) / W);
-    // Each lane is one kernel tap.  Consecutive points of one cell (the
-    // records are cell-sorted) accumulate in FP64 registers -- the product of
-    // two REAL factors is exact in DOUBLE, so one fused multiply-add per
-    // channel is the whole update -- and are added into the FP64 LDS tile
-    // when the cell changes.
+    double sw = 0.0, ls = 0.0;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     int cur = -1;
-    // records arrive 64 at a time, one per lane (the next batch in flight
-    // while this one is consumed), and are broadcast to the wave with
-    // readlane; the loads are whole 16-byte loads of a clamped slot (lanes
-    // past the end re-read the wave's last record, which is never broadcast
-    // from them); an empty range (wave-uniform) loads nothing
-    const GridRec empty = {0u, 0.0f, 0.0f, 0.0f};
-    GridRec mine = r0 < r1 ? sorted[min(r0 + lane, r1 - 1)] : empty;
-    for (int b = r0; b < r1; b += 64) {
-        const GridRec nxt = sorted[min(b + 64 + lane, r1 - 1)];
-        const int nb = __builtin_amdgcn_readfirstlane(min(64, r1 - b));
-        for (int j = 0; j < nb; j++) {
-            const uint32_t pk = (uint32_t)__builtin_amdgcn_readlane((int)mine.pk, j);
-            const int cell = (int)(pk & 0xffff);
-            if (cell != cur) {
-                if (cur >= 0 && active) {
-                    const int a = (cur >> 8) * P + (cur & 0xff) + lane_off;
-                    if (CHM & 1) {
-                        atomicAdd(&S[a], s0);
-                        atomicAdd(&S[NC + a], s1);
+    if (r0 < r1) {  // wave-uniform; lanes past the end load the wave's last slot
+        uint64_t v0 = svals[min(r0 + lane, r1 - 1)];
+        StageRec st0 = stage[sval_index(v0)];
+        uint64_t v1 = svals[min(r0 + 64 + lane, r1 - 1)];
+        for (int b = r0; b < r1; b += 64) {
+            const StageRec st1 = stage[sval_index(v1)];
+            const uint64_t v2 = svals[min(b + 128 + lane, r1 - 1)];
+
+            // build this lane's record of the batch
+            const int p = b + lane;
+            const bool valid = p < r1;
+            float wti = st0.wt;
+            if (WGT) {
+                bool off;
+                wti = robust_weight(wp.wcp, st0.u, st0.v, st0.w, wti, wp.gwt, wp.wsums, WGT == 2, wp.rob1, wp.rpa,
+                                    wp.rbscale, wp.pi_f, &off);
+            }
+            GridCell c;
+            grid_cell(cp, st0.u, st0.v, st0.w, PSF_ONLY ? make_float2(1.0f, 0.0f) : st0.vis, st0.rot, !PSF_ONLY, c);
+            const int lu = c.nuc - lu0, lv = c.nvc - lv0;
+            GridRec mine;
+            mine.pk = (uint32_t)lu | ((uint32_t)lv << 8);
+            mine.wt = 0.0f;
+            mine.re = 0.0f;
+            mine.im = 0.0f;
+            if (wti > 0.0f) {
+                if (PSF_ONLY) {
+                    c.fr = 1.0f;
+                    c.fi = 0.0f;
+                }
+                const int pu = nint_d((double)cp.osampu * ((double)c.nuc - c.ucell)) + poffu;
+                const int pv = nint_d((double)cp.osampv * ((double)c.nvc - c.vcell)) + poffv;
+                double su = 0.0, sv = 0.0;
+                for (int o = 0; o < ntapu; o++) su += (double)cflu[o * rsu + pu];
+                for (int o = 0; o < ntapv; o++) sv += (double)cflv[o * rsv + pv];
+                if (valid) {
+                    sw += (double)wti;
+                    ls += (double)wti * su * sv;
+                }
+                mine.pk |= ((uint32_t)pu << 16) | ((uint32_t)pv << 24);
+                mine.wt = wti;
+                mine.re = (float)((double)wti * (double)c.fr);
+                mine.im = (float)((double)wti * (double)c.fi);
+            }
+            if (DEG) {
+                DegridRec od;
+                degrid_half(cp, tp, st0.u, st0.v, st0.w, (tile * T + lv) * T + lu, sval_index(v0), st0.rot, od);
+                if (valid) rec_d[p] = od;
+            }
+
+            // grid the batch
+            const int nb = __builtin_amdgcn_readfirstlane(min(64, r1 - b));
+            for (int j = 0; j < nb; j++) {
+                const uint32_t pk = (uint32_t)__builtin_amdgcn_readlane((int)mine.pk, j);
+                const int cell = (int)(pk & 0xffff);
+                if (cell != cur) {
+                    if (cur >= 0 && active) {
+                        const int a = (cur >> 8) * P + (cur & 0xff) + lane_off;
+                        if (CHM & 1) {
+                            atomicAdd(&S[a], s0);
+                            atomicAdd(&S[NC + a], s1);
+                        }
+                        if (CHM & 2) atomicAdd(&S[PSF * NC + a], s2);
                     }
-                    if (CHM & 2) atomicAdd(&S[PSF * NC + a], s2);
+                    cur = cell;
+                    s0 = s1 = s2 = 0.0;
                 }
-                cur = cell;
-                s0 = s1 = s2 = 0.0;
+                if (active) {
+                    const int pu = (pk >> 16) & 0xff, pv = pk >> 24;
+                    const double cw = (double)(rowu[pu] * rowv[pv]);
+                    if (CHM & 1) {
+                        const float re = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.re), j));
+                        const float im = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.im), j));
+                        s0 = fma(cw, (double)re, s0);
+                        s1 = fma(cw, (double)im, s1);
+                    }
+                    if (CHM & 2) {
+                        const float wt = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.wt), j));
+                        s2 = fma(cw, (double)wt, s2);
+                    }
+                }
             }
-            if (active) {
-                const int pu = (pk >> 16) & 0xff, pv = pk >> 24;
-                const double cw = (double)(rowu[pu] * rowv[pv]);
-                if (CHM & 1) {
-                    const float re = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.re), j));
-                    const float im = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.im), j));
-                    s0 = fma(cw, (double)re, s0);
-                    s1 = fma(cw, (double)im, s1);
-                }
-                if (CHM & 2) {
-                    const float wt = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.wt), j));
-                    s2 = fma(cw, (double)wt, s2);
-                }
-            }
+            v0 = v1;
+            v1 = v2;
+            st0 = st1;
         }
-        mine = nxt;
     }
     if (cur >= 0 && active) {
         const int a = (cur >> 8) * P + (cur & 0xff) + lane_off;
@@ -518,7 +541,25 @@ __global__ __launch_bounds__(64 * W) void k_grid_tiles(const GridRec *__restrict
         }
         if (CHM & 2) atomicAdd(&S[PSF * NC + a], s2);
     }
+    // SUMWT and LSUMWT of the item: lanes, then waves, in a fixed order
+    for (int d = 32; d > 0; d >>= 1) {
+        sw += __shfl_down(sw, d);
+        ls += __shfl_down(ls, d);
+    }
+    if (lane == 0) {
+        s_red[2 * wave] = sw;
+        s_red[2 * wave + 1] = ls;
+    }
     __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int k = 0; k < W; k++) {
+            a += s_red[2 * k];
+            b += s_red[2 * k + 1];
+        }
+        item_sums[2 * item] = a;
+        item_sums[2 * item + 1] = b;
+    }
     float *out = partial + (size_t)item * NCH * NC;
     for (int k = threadIdx.x; k < NCH * NC; k += 64 * W) out[k] = (float)S[k];
 }
@@ -748,39 +789,10 @@ void launch_density_runs(hipStream_t s, const SortBufs &b, int64_t n, const Tile
                        (uint32_t)sort_nkeys(tp), tp, cp.nu, cp.suppu, gwt, blk, b.tile_start);
 }
 
-int launch_grid_gather(hipStream_t s, const SortBufs &b, int64_t n, const CoordParams &cp, const TileParams &tp,
-                       int flags, const WeightParams &wp, const float *cfu, const float *cfv, const Records &rec,
-                       double *blk_sums, double *sums) {
-    const int nb = gather_blocks(n);
-#define SDE_GATHER(FL, W)                                                                                    \
-    hipLaunchKernelGGL((k_grid_gather<FL, W>), dim3(nb), dim3(kThreads), 0, s, b.skeys, b.svals, b.stage,  \
-                       b.tile_start, cp, tp, wp, cfu, cfv, rec.g, rec.d, blk_sums)
-    const bool wgt = wp.on != 0;
-    switch (flags) {
-        case kGatherGrid | kGatherDegrid:
-            if (wgt) SDE_GATHER(kGatherGrid | kGatherDegrid, true);
-            else SDE_GATHER(kGatherGrid | kGatherDegrid, false);
-            break;
-        case kGatherGrid:
-            if (wgt) SDE_GATHER(kGatherGrid, true);
-            else SDE_GATHER(kGatherGrid, false);
-            break;
-        case kGatherGrid | kGatherPsf:
-            if (wgt) SDE_GATHER(kGatherGrid | kGatherPsf, true);
-            else SDE_GATHER(kGatherGrid | kGatherPsf, false);
-            break;
-        case kGatherDegrid:
-            SDE_GATHER(kGatherDegrid, false);
-            break;
-        default:
-            return -1;
-    }
-#undef SDE_GATHER
-    if (flags & kGatherGrid) {
-        const double num = (double)((float)(cp.nu - 1) * (float)cp.nv);
-        hipLaunchKernelGGL(k_grid_totals, dim3(1), dim3(1024), 0, s, blk_sums, nb, num, sums);
-    }
-    return 0;
+void launch_degrid_gather(hipStream_t s, const SortBufs &b, int64_t n, const CoordParams &cp, const TileParams &tp,
+                          DegridRec *rec_d) {
+    hipLaunchKernelGGL(k_degrid_gather, dim3(gather_blocks(n)), dim3(kThreads), 0, s, b.skeys, b.svals, b.stage,
+                       b.tile_start, cp, tp, rec_d);
 }
 
 size_t grid_partial_floats(const TileParams &tp, int chmask, int max_items) {
@@ -789,37 +801,54 @@ size_t grid_partial_floats(const TileParams &tp, int chmask, int max_items) {
     return (size_t)max_items * nch * P * P;
 }
 
-template <int T, int CHM, int W>
-static void launch_tiles_t(hipStream_t s, const GridRec *sorted, const WorkItem *items, const int *nitems,
-                           int max_items, const float *cfu, const float *cfv, const CoordParams &cp,
-                           float *partial) {
+template <int T, int CHM, int W, bool DEG, int WGT>
+static void launch_tiles_t(hipStream_t s, const SortBufs &b, const TileParams &tp, const WeightParams &wp,
+                           const float *cfu, const float *cfv, const CoordParams &cp, DegridRec *rec_d,
+                           double *item_sums, float *partial) {
     constexpr int P = T + 2 * kHalo;
     constexpr int NCH = ((CHM & 1) ? 2 : 0) + ((CHM & 2) ? 1 : 0);
     const size_t tables = (size_t)(2 * cp.suppu + 1) * cf_row_stride(cp.osampu) +
                           (size_t)(2 * cp.suppv + 1) * cf_row_stride(cp.osampv);
     const size_t lds = sizeof(double) * (size_t)NCH * P * P + sizeof(float) * tables;
-    hipLaunchKernelGGL((k_grid_tiles<T, CHM, W>), dim3(max_items), dim3(64 * W), lds, s, sorted, items, nitems, cfu,
-                       cfv, cp, partial);
+    hipLaunchKernelGGL((k_grid_tiles<T, CHM, W, DEG, WGT>), dim3(b.max_items), dim3(64 * W), lds, s, b.svals,
+                       b.stage, b.items, b.nitems, cfu, cfv, cp, tp, wp, rec_d, item_sums, partial);
 }
 
-int launch_grid_tiles(hipStream_t s, const GridRec *sorted, const WorkItem *items, const int *nitems,
-                      int max_items, const TileParams &tp, int chmask, const float *cfu, const float *cfv,
-                      const CoordParams &cp, float *partial) {
-    if (max_items <= 0) return 0;
+template <int T, int W>
+static int launch_tiles_w(hipStream_t s, const SortBufs &b, const TileParams &tp, int chmask, bool deg,
+                          const WeightParams &wp, const float *cfu, const float *cfv, const CoordParams &cp,
+                          DegridRec *rec_d, double *item_sums, float *partial) {
+    // weights: 0 as given, 1 robust/uniform, 2 robust with the azimuthal robustness
+    const int wgt = wp.on ? (wp.az ? 2 : 1) : 0;
+    if (chmask == 1 && !deg && wgt == 0)
+        launch_tiles_t<T, 1, W, false, 0>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial);
+    else if (chmask == 2 && !deg && wgt == 0)
+        launch_tiles_t<T, 2, W, false, 0>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial);
+    else if (chmask == 3 && deg && wgt == 0)
+        launch_tiles_t<T, 3, W, true, 0>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial);
+    else if (chmask == 3 && deg && wgt == 1)
+        launch_tiles_t<T, 3, W, true, 1>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial);
+    else if (chmask == 3 && deg && wgt == 2)
+        launch_tiles_t<T, 3, W, true, 2>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial);
+    else
+        return -1;
+    return 0;
+}
+
+int launch_grid_tiles(hipStream_t s, const SortBufs &b, const TileParams &tp, int chmask, bool deg,
+                      const WeightParams &wp, const float *cfu, const float *cfv, const CoordParams &cp,
+                      DegridRec *rec_d, double *item_sums, double *sums, float *partial) {
+    if (b.max_items <= 0) return 0;
     const int W = tp.waves == 8 ? 8 : 4;
-#define SDE_TILE_CASE(TT, WW)                                                                                      \
-    if (tp.T == TT && W == WW) {                                                                                   \
-        if (chmask == 1) launch_tiles_t<TT, 1, WW>(s, sorted, items, nitems, max_items, cfu, cfv, cp, partial);     \
-        else if (chmask == 2) launch_tiles_t<TT, 2, WW>(s, sorted, items, nitems, max_items, cfu, cfv, cp, partial); \
-        else launch_tiles_t<TT, 3, WW>(s, sorted, items, nitems, max_items, cfu, cfv, cp, partial);                 \
-        return 0;                                                                                                  \
-    }
-    SDE_TILE_CASE(16, 4)
-    SDE_TILE_CASE(16, 8)
-    SDE_TILE_CASE(32, 4)
-    SDE_TILE_CASE(32, 8)
-#undef SDE_TILE_CASE
-    return -1;
+    int rc = -1;
+    if (tp.T == 16 && W == 4) rc = launch_tiles_w<16, 4>(s, b, tp, chmask, deg, wp, cfu, cfv, cp, rec_d, item_sums, partial);
+    else if (tp.T == 16 && W == 8) rc = launch_tiles_w<16, 8>(s, b, tp, chmask, deg, wp, cfu, cfv, cp, rec_d, item_sums, partial);
+    else if (tp.T == 32 && W == 4) rc = launch_tiles_w<32, 4>(s, b, tp, chmask, deg, wp, cfu, cfv, cp, rec_d, item_sums, partial);
+    else if (tp.T == 32 && W == 8) rc = launch_tiles_w<32, 8>(s, b, tp, chmask, deg, wp, cfu, cfv, cp, rec_d, item_sums, partial);
+    if (rc) return rc;
+    const double num = (double)((float)(cp.nu - 1) * (float)cp.nv);
+    hipLaunchKernelGGL(k_grid_totals, dim3(1), dim3(1024), 0, s, item_sums, b.max_items, num, sums);
+    return 0;
 }
 
 void launch_grid_reduce(hipStream_t s, const float *partial, const int *item_start, const TileParams &tp,

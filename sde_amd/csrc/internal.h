@@ -38,16 +38,18 @@ struct TileParams {
     int waves;      // waves per gridding workgroup (4 or 8)
 };
 
-// Sorted gridding record, 16 bytes:
-//   x: local u | local v << 8 | u phase << 16 | v phase << 24
-//   y: weight (float bits)
-//   z, w: weight * visibility (float), after phase rotation and conjugation
+// Gridding record of one sorted point, built and held in registers by the
+// gridding wave (16 bytes, one per lane):
+//   pk: local u | local v << 8 | u phase << 16 | v phase << 24
+//   wt: weight
+//   re, im: weight * visibility (float), after phase rotation and conjugation
 struct __align__(16) GridRec {
     uint32_t pk;
     float wt, re, im;
 };
 
-// Degridding half of a sorted record:
+// Degridding half of a sorted record, written in sorted order by the
+// gridding run (or a gather) and streamed by the degridding tile kernel:
 //   idx: original index | conjugate flag << 31; kNoDegrid: the point is not
 //        degridded from this record (list path, off the grid or flagged)
 //   pk:  LDS column | LDS row << 8 | u phase << 16 | v phase << 24
@@ -58,23 +60,18 @@ struct __align__(16) DegridRec {
 };
 constexpr uint32_t kNoDegrid = 0xffffffffu;
 
-// The sorted records: gridding and degridding halves in two arrays, so each
-// tile kernel streams only the half it uses.
-struct Records {
-    GridRec *g = nullptr;
-    DegridRec *d = nullptr;
-};
-
 struct __align__(16) WorkItem {
     int tile, begin, end, pad;
 };
 
 // Per-visibility staging record written in the caller's order by the prep
-// pass and gathered in sorted order (one 32-byte read per point).
+// pass and gathered in sorted order (one 32-byte read per point).  rot is
+// the phase rotation of the shift, ((float)cos, (float)sin) of shift_phase
+// -- (1, 0) without a shift -- so no later pass evaluates it again.
 struct __align__(32) StageRec {
     float u, v, w, wt;
     float2 vis;
-    uint32_t pad0, pad1;
+    float2 rot;
 };
 
 // ---- cell sort shared by gridding and degridding (grid.hip, sort.hip) ----
@@ -157,7 +154,7 @@ void launch_items(hipStream_t s, SortBufs &b, const TileParams &tp);
 void launch_density_runs(hipStream_t s, const SortBufs &b, int64_t n, const TileParams &tp,
                          const CoordParams &cp, double *gwt, double *blk);
 
-// Robust-weighting parameters of the gather (grd/grdruw2d.f:184-240).
+// Robust-weighting parameters of the gridding run (grd/grdruw2d.f:184-240).
 struct WeightParams {
     int on = 0, az = 0;
     float rob1 = 0.0f, rpa = 0.0f, rbscale = 0.0f, pi_f = 0.0f;
@@ -167,22 +164,21 @@ struct WeightParams {
 WeightParams make_weight_params(const CoordParams &wcp, int az, const float *robust, float pi_f,
                                 const double *gwt, const double *wsums);
 
-// Records in sorted order from the staging records: flags kGatherGrid
-// (gridding half, with the weights of wp when on; SUMWT, LSUMWT and the
-// normalisation to sums[0..2] through blk_sums, 2 doubles per block of
-// gather_blocks()), kGatherPsf (PSF only: VIS = 1), kGatherDegrid
-// (degridding half).  Combinations: Grid, Grid|Psf, Grid|Degrid, Degrid;
-// returns -1 for any other.
-constexpr int kGatherGrid = 1, kGatherPsf = 2, kGatherDegrid = 4;
-int launch_grid_gather(hipStream_t s, const SortBufs &b, int64_t n, const CoordParams &cp, const TileParams &tp,
-                       int flags, const WeightParams &wp, const float *cfu, const float *cfv, const Records &rec,
-                       double *blk_sums, double *sums);
+// Degridding halves in sorted order from the staging records (when no
+// gridding run wrote them).
+void launch_degrid_gather(hipStream_t s, const SortBufs &b, int64_t n, const CoordParams &cp, const TileParams &tp,
+                          DegridRec *rec_d);
 
 // ---- gridding (grid.hip) ----
-// chmask: bit 0 = dirty (2 floats), bit 1 = PSF (1 float)
-int launch_grid_tiles(hipStream_t s, const GridRec *sorted, const WorkItem *items, const int *nitems,
-                      int max_items, const TileParams &tp, int chmask, const float *cfu, const float *cfv,
-                      const CoordParams &cp, float *partial);
+// chmask: bit 0 = dirty (2 floats), bit 1 = PSF (1 float).  One workgroup
+// per work item builds its records from the staging records (with the
+// weights of wp when on) and grids them into partial; with deg it writes
+// the degridding halves to rec_d.  SUMWT, LSUMWT and the normalisation go
+// to sums[0..2] through item_sums (2 doubles per work item, max_items).
+// Supported: chmask 1 or 2 without deg and weights; chmask 3 with deg.
+int launch_grid_tiles(hipStream_t s, const SortBufs &b, const TileParams &tp, int chmask, bool deg,
+                      const WeightParams &wp, const float *cfu, const float *cfv, const CoordParams &cp,
+                      DegridRec *rec_d, double *item_sums, double *sums, float *partial);
 void launch_grid_reduce(hipStream_t s, const float *partial, const int *item_start, const TileParams &tp,
                         int chmask, const CoordParams &cp, const double *sums, float2 *dirty, float2 *psf);
 size_t grid_partial_floats(const TileParams &tp, int chmask, int max_items);
@@ -240,23 +236,33 @@ __device__ __forceinline__ void cdiv_by_real(float &re, float &im, float s) {
     im = scalbnf((b * c - a * 0.0f) / denom, -e);
 }
 
+// The phase of the shift of one point, TWOPI*(WL - W) with WL the rotated w
+// in DOUBLE (grd/grdh2d.f:124-125, grd/grddeh2d.f:101-102), as the pair
+// ((float)cos, (float)sin) both routines round it to.
+__device__ __forceinline__ float2 shift_phase(const CoordParams &cp, double ug, double vg, double wg) {
+    if (!cp.doshift) return make_float2(1.0f, 0.0f);
+    const double wl = cp.shift[6] * ug + cp.shift[7] * vg + cp.shift[8] * wg;
+    const double phase = cp.twopi * (wl - wg);
+    return make_float2((float)cos(phase), (float)sin(phase));
+}
+
 // The GRDDEH2D coordinate step of one point with WT > 0 (grd/grddeh2d.f:
 // 107-139, REAL cell from the DOUBLE rotated coordinates) and the degridding
 // half of its record.  key is the point's cell-sort key (< 0: not sorted).
 // kDegOff: the footprint leaves the grid (VIS = 0, WT negated); kDegList:
 // the point needs the per-point kernel (near the u = 0 axis, not sorted, or
-// outside its tile's LDS window); kDegTile: r is filled.
+// outside its tile's LDS window); kDegTile: r is filled, with the rotation
+// rot = shift_phase(cp, ug, vg, wg).
 __device__ __forceinline__ int degrid_half(const CoordParams &cp, const TileParams &tp, double ug, double vg,
-                                           double wg, int key, uint32_t i, DegridRec &r) {
+                                           double wg, int key, uint32_t i, float2 rot, DegridRec &r) {
     r.idx = kNoDegrid;
     r.pk = 0;
     r.rr = 1.0f;
     r.ri = 0.0f;
-    double ul = ug, vl = vg, wl = wg;
+    double ul = ug, vl = vg;
     if (cp.doshift) {
         ul = cp.shift[0] * ug + cp.shift[1] * vg + cp.shift[2] * wg;
         vl = cp.shift[3] * ug + cp.shift[4] * vg + cp.shift[5] * wg;
-        wl = cp.shift[6] * ug + cp.shift[7] * vg + cp.shift[8] * wg;
     }
     float ucell = (float)((double)cp.uscale * ul + (double)cp.uoffset);
     float vcell = (float)((double)cp.vscale * vl + (double)cp.voffset);
@@ -282,11 +288,8 @@ __device__ __forceinline__ int degrid_half(const CoordParams &cp, const TilePara
     const bool conj = (double)cp.uscale * ul > 0.0;
     r.idx = i | (conj ? 0x80000000u : 0u);
     r.pk = (uint32_t)lu | ((uint32_t)lv << 8) | ((uint32_t)(delui + poffu) << 16) | ((uint32_t)(delvi + poffv) << 24);
-    if (cp.doshift) {
-        const double phase = cp.twopi * (wl - wg);
-        r.rr = (float)cos(phase);
-        r.ri = (float)sin(phase);
-    }
+    r.rr = rot.x;
+    r.ri = rot.y;
     return kDegTile;
 }
 
