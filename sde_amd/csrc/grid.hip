@@ -486,9 +486,11 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
                     ls += (double)wti * su * sv;
                 }
                 mine.pk |= ((uint32_t)pu << 16) | ((uint32_t)pv << 24);
-                mine.wt = wti;
-                mine.re = (float)((double)wti * (double)c.fr);
-                mine.im = (float)((double)wti * (double)c.fi);
+                if (valid) {
+                    mine.wt = wti;
+                    mine.re = (float)((double)wti * (double)c.fr);
+                    mine.im = (float)((double)wti * (double)c.fi);
+                }
             }
             if (DEG) {
                 DegridRec od;
@@ -496,10 +498,12 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
                 if (valid) rec_d[p] = od;
             }
 
-            // grid the batch
+            // grid the batch; the table entries of the next record are
+            // read while this one accumulates (two records per turn, so the
+            // entries in flight never need a register copy).  Lanes past the
+            // taps compute on row 0 and never flush.
             const int nb = __builtin_amdgcn_readfirstlane(min(64, r1 - b));
-            for (int j = 0; j < nb; j++) {
-                const uint32_t pk = (uint32_t)__builtin_amdgcn_readlane((int)mine.pk, j);
+            auto step = [&](uint32_t pk, float ta, float tb, int j) {
                 const int cell = (int)(pk & 0xffff);
                 if (cell != cur) {
                     if (cur >= 0 && active) {
@@ -513,20 +517,31 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
                     cur = cell;
                     s0 = s1 = s2 = 0.0;
                 }
-                if (active) {
-                    const int pu = (pk >> 16) & 0xff, pv = pk >> 24;
-                    const double cw = (double)(rowu[pu] * rowv[pv]);
-                    if (CHM & 1) {
-                        const float re = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.re), j));
-                        const float im = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.im), j));
-                        s0 = fma(cw, (double)re, s0);
-                        s1 = fma(cw, (double)im, s1);
-                    }
-                    if (CHM & 2) {
-                        const float wt = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.wt), j));
-                        s2 = fma(cw, (double)wt, s2);
-                    }
+                const double cw = (double)(ta * tb);
+                if (CHM & 1) {
+                    const float re = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.re), j));
+                    const float im = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.im), j));
+                    s0 = fma(cw, (double)re, s0);
+                    s1 = fma(cw, (double)im, s1);
                 }
+                if (CHM & 2) {
+                    const float wt = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.wt), j));
+                    s2 = fma(cw, (double)wt, s2);
+                }
+            };
+            // an odd batch takes one more lane: a copy of the last record
+            // with zero weight (same cell, adds nothing)
+            const int nb2 = (nb + 1) & ~1;
+            uint32_t pk0 = (uint32_t)__builtin_amdgcn_readlane((int)mine.pk, 0);
+            float ta0 = rowu[(pk0 >> 16) & 0xff], tb0 = rowv[pk0 >> 24];
+            for (int j = 0; j < nb2; j += 2) {
+                const uint32_t pk1 = (uint32_t)__builtin_amdgcn_readlane((int)mine.pk, j + 1);
+                const float ta1 = rowu[(pk1 >> 16) & 0xff], tb1 = rowv[pk1 >> 24];
+                step(pk0, ta0, tb0, j);
+                pk0 = (uint32_t)__builtin_amdgcn_readlane((int)mine.pk, min(j + 2, 63));
+                ta0 = rowu[(pk0 >> 16) & 0xff];
+                tb0 = rowv[pk0 >> 24];
+                step(pk1, ta1, tb1, j + 1);
             }
             v0 = v1;
             v1 = v2;
