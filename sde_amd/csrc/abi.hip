@@ -1173,6 +1173,7 @@ int sde_plan_uvmap(sde_plan *p) {
     if (weighting) {
         SDE_CHECK(hipMemsetAsync(p->d_gwt, 0, sizeof(double) * ng, s));
         dens.wcp = p->wcp;
+        dens.az = make_az_params(p->wmode == 3, p->cfg.robust, 4.0f * atanf(1.0f));
         dens.gwt = runs ? p->d_gwt : nullptr;
         dens.blk = p->d_wblk;
         dens.ndrop = p->d_cnt;

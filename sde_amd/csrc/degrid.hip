@@ -286,14 +286,14 @@ __global__ __launch_bounds__(1024) void k_wt_finish(const double *__restrict__ b
 __global__ __launch_bounds__(256) void k_wt_apply(
     const float *__restrict__ wt, const float *__restrict__ u, const float *__restrict__ v,
     const float *__restrict__ w, int64_t nvis, CoordParams cp, const double *__restrict__ gwt,
-    const double *__restrict__ wsums, int az, float rob1, float rpa, float rbscale, float pi_f,
-    float *__restrict__ newwt, unsigned long long *__restrict__ ndrop) {
+    const double *__restrict__ wsums, AzParams az, float *__restrict__ newwt,
+    unsigned long long *__restrict__ ndrop) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nvis) return;
     const float wti = wt[i];
     if (wti <= 0.0f) return;
     bool off;
-    newwt[i] = robust_weight(cp, u[i], v[i], w[i], wti, gwt, wsums, az, rob1, rpa, rbscale, pi_f, &off);
+    newwt[i] = robust_weight(cp, u[i], v[i], w[i], wti, gwt, wsums, az, &off);
     if (off) atomicAdd(ndrop, 1ull);
 }
 
@@ -435,20 +435,26 @@ void launch_wt_finish(hipStream_t s, const double *gwt, int ncell, const double 
                        absd2, wsums);
 }
 
-WeightParams make_weight_params(const CoordParams &wcp, int az, const float *robust, float pi_f,
-                                const double *gwt, const double *wsums) {
-    WeightParams wp;
-    wp.on = 1;
-    wp.az = az;
-    wp.rob1 = robust ? robust[0] : 0.0f;
+AzParams make_az_params(int az, const float *robust, float pi_f) {
+    AzParams a;
+    a.on = az;
+    a.rob1 = robust ? robust[0] : 0.0f;
     if (az) {
         float rpa = -robust[2] + 90.0f - 90.0f;
         while (rpa < -90.0f) rpa = rpa + 180.0f;
         while (rpa > 90.0f) rpa = rpa - 180.0f;
-        wp.rpa = rpa * pi_f / 180.0f;
-        wp.rbscale = (robust[0] - robust[1]) / (pi_f / 2.0f);
+        a.rpa = rpa * pi_f / 180.0f;
+        a.rbscale = (robust[0] - robust[1]) / (pi_f / 2.0f);
     }
-    wp.pi_f = pi_f;
+    a.pi_f = pi_f;
+    return a;
+}
+
+WeightParams make_weight_params(const CoordParams &wcp, int az, const float *robust, float pi_f,
+                                const double *gwt, const double *wsums) {
+    WeightParams wp;
+    wp.on = 1;
+    wp.az = make_az_params(az, robust, pi_f);
     wp.wcp = wcp;
     wp.gwt = gwt;
     wp.wsums = wsums;
@@ -460,10 +466,9 @@ void launch_wt_apply(hipStream_t s, const float *wt, const float *u, const float
                      const double *wsums, int az, const float *robust, float pi_f,
                      float *newwt, unsigned long long *ndrop) {
     if (nvis <= 0) return;
-    const WeightParams wp = make_weight_params(cp, az, robust, pi_f, gwt, wsums);
     const int64_t nb = (nvis + 255) / 256;
     hipLaunchKernelGGL(k_wt_apply, dim3((unsigned)nb), dim3(256), 0, s, wt, u, v, w, nvis, cp, gwt,
-                       wsums, az, wp.rob1, wp.rpa, wp.rbscale, pi_f, newwt, ndrop);
+                       wsums, make_az_params(az, robust, pi_f), newwt, ndrop);
 }
 
 void launch_gwt_out(hipStream_t s, const double *gwt, float *out, size_t n) {

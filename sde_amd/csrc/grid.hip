@@ -102,8 +102,11 @@ __device__ __forceinline__ int cell_key(const TileParams &tp, int cu, int cv) {
 }
 
 // Prep pass, one thread per visibility in the caller's order: the cell key
-// of GRDH2D's DOUBLE coordinate step and drop test (grd/grdh2d.f:115-145),
-// the 32-byte staging record, and optionally
+// of GRDH2D's DOUBLE coordinate step and drop test (grd/grdh2d.f:115-145)
+// and the 32-byte staging record -- the gridding record's cell, kernel
+// phases and rotated visibility (grd/grdh2d.f:124-160), the density cell of
+// GRDRUW2D / GRDUWT2D (grd/grdruw2d.f:197-229) when DM > 0, the degridding
+// record of GRDDEH2D (grd/grddeh2d.f:92-118) when DEG -- and optionally
 //   DM = 1: the count of points off the density grid;
 //   DM = 2: also the density weight of every point whose REAL density cell
 //           (grd/grdruw2d.f:139-160) is not its DOUBLE gridding cell or
@@ -117,7 +120,7 @@ __global__ __launch_bounds__(kThreads) void k_prep(
     const float2 *__restrict__ vis, const float *__restrict__ wt, const float *__restrict__ u,
     const float *__restrict__ v, const float *__restrict__ w, int64_t n, CoordParams cp, TileParams tp,
     uint32_t sentinel, uint32_t *__restrict__ keys, uint64_t *__restrict__ vals, StageRec *__restrict__ stage,
-    CoordParams wcp,
+    CoordParams wcp, AzParams az,
     double *__restrict__ gwt, double *__restrict__ dblk, unsigned long long *__restrict__ ndrop,
     float2 *__restrict__ vis_out, float *__restrict__ wt_out, int *__restrict__ list, int *__restrict__ nlist,
     int seg_cap) {
@@ -128,6 +131,7 @@ __global__ __launch_bounds__(kThreads) void k_prep(
         __syncthreads();
     }
     int *seg = DEG ? list + (size_t)blockIdx.x * seg_cap : nullptr;
+    const int poffu = (cp.osampu + 1) / 2, poffv = (cp.osampv + 1) / 2;
     const int64_t stride = (int64_t)gridDim.x * kThreads;
     // block-uniform loop bounds: every lane of a wave takes part in the list append
     for (int64_t base = (int64_t)blockIdx.x * kThreads; base < n; base += stride) {
@@ -138,28 +142,41 @@ __global__ __launch_bounds__(kThreads) void k_prep(
             const float uf = u[i], vf = v[i], wf = w[i];
             const float2 vv = vis[i];
             StageRec sr;
-            sr.u = uf;
-            sr.v = vf;
-            sr.w = wf;
-            sr.wt = wti;
-            sr.vis = vv;
-            sr.rot = shift_phase(cp, uf, vf, wf);
-            stage[i] = sr;
+            sr.gpk = 0u;
+            sr.fr = 0.0f;
+            sr.fi = 0.0f;
+            sr.wcell = kNoCell;
+            sr.dpk = 0u;
+            sr.q2 = 0.0f;
+            sr.rot = make_float2(1.0f, 0.0f);
             uint32_t key = sentinel;
             if (wti > 0.0f) {
+                sr.rot = shift_phase(cp, uf, vf, wf);
                 GridCell c;
-                const bool in = grid_cell(cp, uf, vf, wf, make_float2(0.0f, 0.0f), sr.rot, false, c);
-                if (in) key = (uint32_t)cell_key(tp, c.nuc + cp.suppu - 1, c.nvc - 1);
+                const bool in = grid_cell(cp, uf, vf, wf, vv, sr.rot, true, c);
+                if (in) {
+                    key = (uint32_t)cell_key(tp, c.nuc + cp.suppu - 1, c.nvc - 1);
+                    const uint32_t local = key & (uint32_t)(tp.T * tp.T - 1);
+                    const int pu = nint_d((double)cp.osampu * ((double)c.nuc - c.ucell)) + poffu;
+                    const int pv = nint_d((double)cp.osampv * ((double)c.nvc - c.vcell)) + poffv;
+                    sr.gpk = (local % (uint32_t)tp.T) | ((local / (uint32_t)tp.T) << 8) | ((uint32_t)pu << 16) |
+                             ((uint32_t)pv << 24);
+                    sr.fr = c.fr;
+                    sr.fi = c.fi;
+                }
                 if (DM > 0) {
-                    int uc, vc;
-                    float ul, vl;
-                    wcell(wcp, uf, vf, wf, uc, vc, ul, vl);
-                    if (uc < 1 || uc > wcp.nu || vc < 1 || vc > wcp.nv) {
+                    const RobustCell rc = robust_cell(wcp, uf, vf, wf, az);
+                    sr.wcell = rc.cell;
+                    sr.q2 = rc.q2;
+                    if (rc.half) sr.gpk |= kStageHalf;
+                    if (rc.cell == kNoCell) {
                         ndr += 1.0;
-                    } else if (DM == 2 && !(in && uc == c.nuc && vc == c.nvc)) {
-                        atomicAdd(&gwt[(size_t)(vc - 1) * wcp.nu + uc - 1], (double)wti);
-                        dsum += (double)wti;
-                        if (in) key |= kKeyFlag;
+                    } else if (DM == 2) {
+                        if (!(in && rc.uc == c.nuc && rc.vc == c.nvc)) {
+                            atomicAdd(&gwt[rc.cell], (double)wti);
+                            dsum += (double)wti;
+                            if (in) key |= kKeyFlag;
+                        }
                     }
                 }
                 if (DEG) {
@@ -173,12 +190,15 @@ __global__ __launch_bounds__(kThreads) void k_prep(
                         to_list = true;
                     } else {
                         wt_out[i] = wti;  // negated by the tile kernel if the point fails
+                        sr.dpk = d.pk;
+                        sr.gpk |= kStageDegTile | ((d.idx & 0x80000000u) ? kStageConj : 0u);
                     }
                 }
             } else if (DEG) {
                 wt_out[i] = wti;
                 vis_out[i] = vv;
             }
+            stage[i] = sr;
             keys[i] = key;
             vals[i] = (uint64_t)(uint32_t)i | ((uint64_t)__float_as_uint(wti) << 32);
         }
@@ -311,21 +331,27 @@ __global__ void k_grid_items(const int *__restrict__ tile_start, const int *__re
     }
 }
 
-// Degridding halves in sorted order, one thread per sorted slot: the
-// staging record of the slot's visibility is gathered (one 32-byte read)
-// and GRDDEH2D's per-point step (grd/grddeh2d.f:92-118) written in place
-// (coalesced).  Used when the records were not built by a gridding run.
-__global__ __launch_bounds__(kThreads) void k_degrid_gather(
-    const uint32_t *__restrict__ skeys, const uint64_t *__restrict__ svals, const StageRec *__restrict__ stage,
-    const int *__restrict__ tile_start, CoordParams cp, TileParams tp, DegridRec *__restrict__ rec_d) {
+// Degridding halves in sorted order, one thread per sorted slot, copied
+// from the staging record of the slot's visibility (one 32-byte gather per
+// point, coalesced writes).  Used when the records were not built by a
+// gridding run.
+__device__ __forceinline__ DegridRec degrid_rec(const StageRec &sr, uint32_t i) {
+    DegridRec d;
+    d.idx = (sr.gpk & kStageDegTile) ? (i | ((sr.gpk & kStageConj) ? 0x80000000u : 0u)) : kNoDegrid;
+    d.pk = sr.dpk;
+    d.rr = sr.rot.x;
+    d.ri = sr.rot.y;
+    return d;
+}
+
+__global__ __launch_bounds__(kThreads) void k_degrid_gather(const uint64_t *__restrict__ svals,
+                                                             const StageRec *__restrict__ stage,
+                                                             const int *__restrict__ tile_start, TileParams tp,
+                                                             DegridRec *__restrict__ rec_d) {
     const int64_t nvalid = tile_start[tp.ntiles];
     for (int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x; p < nvalid; p += (int64_t)gridDim.x * kThreads) {
-        const uint32_t key = skeys[p] & ~kKeyFlag;
         const uint32_t i = sval_index(svals[p]);
-        const StageRec sr = stage[i];
-        DegridRec od;
-        degrid_half(cp, tp, sr.u, sr.v, sr.w, (int)key, i, sr.rot, od);
-        rec_d[p] = od;
+        rec_d[p] = degrid_rec(stage[i], i);
     }
 }
 
@@ -401,7 +427,9 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
     extern __shared__ double smem_d[];
     __shared__ double s_red[2 * W];
     double *S = smem_d;                          // [NCH][NC] FP64 sums
-    float *cfl = (float *)(smem_d + NCH * NC);   // phase tables, one row per tap
+    double *tsu = smem_d + NCH * NC;             // sum of the taps of each phase
+    double *tsv = tsu + cp.osampu + 1;
+    float *cfl = (float *)(tsv + cp.osampv + 1); // phase tables, one row per tap
     const int item = blockIdx.x;
     if (item >= *nitems) {
         if (threadIdx.x == 0) {
@@ -427,6 +455,18 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
     }
     for (int k = threadIdx.x; k < NCH * NC; k += 64 * W) S[k] = 0.0;
     __syncthreads();
+    // the normalisation's sum over taps of one phase, in the reference's order
+    for (int k = threadIdx.x; k <= cp.osampu; k += 64 * W) {
+        double t = 0.0;
+        for (int o = 0; o < ntapu; o++) t += (double)cflu[o * rsu + k];
+        tsu[k] = t;
+    }
+    for (int k = threadIdx.x; k <= cp.osampv; k += 64 * W) {
+        double t = 0.0;
+        for (int o = 0; o < ntapv; o++) t += (double)cflv[o * rsv + k];
+        tsv[k] = t;
+    }
+    __syncthreads();
 
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -435,10 +475,6 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
     const int lane_off = (ov - cp.suppv + kHalo) * P + (ou - cp.suppu + kHalo);
     const float *rowu = cflu + (active ? ou * rsu : 0);
     const float *rowv = cflv + (active ? ov * rsv : 0);
-    // the sort key of a point is the cell (NUC + SUPPU - 1, NVC - 1) of the
-    // pre-shift grid, tile-major; this item's tile starts at (tu*T, tv*T)
-    const int tile = wi.tile;
-    const int lu0 = (tile % tp.ntu) * T - cp.suppu + 1, lv0 = (tile / tp.ntu) * T + 1;
 
     const int len = wi.end - wi.begin;
     const int r0 = wi.begin + (int)(((int64_t)len * wave) / W);
@@ -454,49 +490,28 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
             const StageRec st1 = stage[sval_index(v1)];
             const uint64_t v2 = svals[min(b + 128 + lane, r1 - 1)];
 
-            // build this lane's record of the batch
+            // this lane's record of the batch: the staged gridding record
+            // with its final weight
             const int p = b + lane;
             const bool valid = p < r1;
-            float wti = st0.wt;
-            if (WGT) {
-                bool off;
-                wti = robust_weight(wp.wcp, st0.u, st0.v, st0.w, wti, wp.gwt, wp.wsums, WGT == 2, wp.rob1, wp.rpa,
-                                    wp.rbscale, wp.pi_f, &off);
-            }
-            GridCell c;
-            grid_cell(cp, st0.u, st0.v, st0.w, PSF_ONLY ? make_float2(1.0f, 0.0f) : st0.vis, st0.rot, !PSF_ONLY, c);
-            const int lu = c.nuc - lu0, lv = c.nvc - lv0;
+            float wti = sval_wt(v0);
+            if (WGT)
+                wti = robust_apply(wti, st0.wcell, (st0.gpk & kStageHalf) != 0, st0.q2, WGT == 2, wp.gwt,
+                                   wp.wsums);
             GridRec mine;
-            mine.pk = (uint32_t)lu | ((uint32_t)lv << 8);
+            mine.pk = st0.gpk & ~kStageFlags;
             mine.wt = 0.0f;
             mine.re = 0.0f;
             mine.im = 0.0f;
-            if (wti > 0.0f) {
-                if (PSF_ONLY) {
-                    c.fr = 1.0f;
-                    c.fi = 0.0f;
-                }
-                const int pu = nint_d((double)cp.osampu * ((double)c.nuc - c.ucell)) + poffu;
-                const int pv = nint_d((double)cp.osampv * ((double)c.nvc - c.vcell)) + poffv;
-                double su = 0.0, sv = 0.0;
-                for (int o = 0; o < ntapu; o++) su += (double)cflu[o * rsu + pu];
-                for (int o = 0; o < ntapv; o++) sv += (double)cflv[o * rsv + pv];
-                if (valid) {
-                    sw += (double)wti;
-                    ls += (double)wti * su * sv;
-                }
-                mine.pk |= ((uint32_t)pu << 16) | ((uint32_t)pv << 24);
-                if (valid) {
-                    mine.wt = wti;
-                    mine.re = (float)((double)wti * (double)c.fr);
-                    mine.im = (float)((double)wti * (double)c.fi);
-                }
+            if (wti > 0.0f && valid) {
+                sw += (double)wti;
+                ls += (double)wti * tsu[(mine.pk >> 16) & 0xff] * tsv[mine.pk >> 24];
+                mine.wt = wti;
+                // the PSF grids VIS = 1
+                mine.re = PSF_ONLY ? wti : (float)((double)wti * (double)st0.fr);
+                mine.im = PSF_ONLY ? 0.0f : (float)((double)wti * (double)st0.fi);
             }
-            if (DEG) {
-                DegridRec od;
-                degrid_half(cp, tp, st0.u, st0.v, st0.w, (tile * T + lv) * T + lu, sval_index(v0), st0.rot, od);
-                if (valid) rec_d[p] = od;
-            }
+            if (DEG && valid) rec_d[p] = degrid_rec(st0, sval_index(v0));
 
             // grid the batch; the table entries of the next record are
             // read while this one accumulates (two records per turn, so the
@@ -755,6 +770,7 @@ void launch_prep(hipStream_t s, const float2 *vis, const float *wt, const float 
     const int nb = prep_blocks(n);
     const uint32_t sentinel = (uint32_t)sort_nkeys(tp);
     CoordParams wcp = dens ? dens->wcp : cp;
+    const AzParams az = dens ? dens->az : AzParams();
     double *gwt = dens ? dens->gwt : nullptr, *dblk = dens ? dens->blk : nullptr;
     unsigned long long *ndrop = dens ? dens->ndrop : nullptr;
     float2 *vo = deg ? deg->vis_out : nullptr;
@@ -762,7 +778,7 @@ void launch_prep(hipStream_t s, const float2 *vis, const float *wt, const float 
     int *list = deg ? deg->list : nullptr, *nlist = deg ? deg->nlist : nullptr;
 #define SDE_PREP(DM, DG)                                                                                         \
     hipLaunchKernelGGL((k_prep<DM, DG>), dim3(nb), dim3(kThreads), 0, s, vis, wt, u, v, w, n, cp, tp, sentinel, \
-                       b.keys, b.vals, b.stage, wcp, gwt, dblk, ndrop, vo, wo, list, nlist, list_seg_cap(n))
+                       b.keys, b.vals, b.stage, wcp, az, gwt, dblk, ndrop, vo, wo, list, nlist, list_seg_cap(n))
     const int dm = dens ? (dens->gwt ? 2 : 1) : 0;
     if (deg) {
         if (dm == 2) SDE_PREP(2, true);
@@ -806,8 +822,8 @@ void launch_density_runs(hipStream_t s, const SortBufs &b, int64_t n, const Tile
 
 void launch_degrid_gather(hipStream_t s, const SortBufs &b, int64_t n, const CoordParams &cp, const TileParams &tp,
                           DegridRec *rec_d) {
-    hipLaunchKernelGGL(k_degrid_gather, dim3(gather_blocks(n)), dim3(kThreads), 0, s, b.skeys, b.svals, b.stage,
-                       b.tile_start, cp, tp, rec_d);
+    hipLaunchKernelGGL(k_degrid_gather, dim3(gather_blocks(n)), dim3(kThreads), 0, s, b.svals, b.stage,
+                       b.tile_start, tp, rec_d);
 }
 
 size_t grid_partial_floats(const TileParams &tp, int chmask, int max_items) {
@@ -824,7 +840,7 @@ static void launch_tiles_t(hipStream_t s, const SortBufs &b, const TileParams &t
     constexpr int NCH = ((CHM & 1) ? 2 : 0) + ((CHM & 2) ? 1 : 0);
     const size_t tables = (size_t)(2 * cp.suppu + 1) * cf_row_stride(cp.osampu) +
                           (size_t)(2 * cp.suppv + 1) * cf_row_stride(cp.osampv);
-    const size_t lds = sizeof(double) * (size_t)NCH * P * P + sizeof(float) * tables;
+    const size_t lds = sizeof(double) * ((size_t)NCH * P * P + cp.osampu + 1 + cp.osampv + 1) + sizeof(float) * tables;
     hipLaunchKernelGGL((k_grid_tiles<T, CHM, W, DEG, WGT>), dim3(b.max_items), dim3(64 * W), lds, s, b.svals,
                        b.stage, b.items, b.nitems, cfu, cfv, cp, tp, wp, rec_d, item_sums, partial);
 }
@@ -834,7 +850,7 @@ static int launch_tiles_w(hipStream_t s, const SortBufs &b, const TileParams &tp
                           const WeightParams &wp, const float *cfu, const float *cfv, const CoordParams &cp,
                           DegridRec *rec_d, double *item_sums, float *partial) {
     // weights: 0 as given, 1 robust/uniform, 2 robust with the azimuthal robustness
-    const int wgt = wp.on ? (wp.az ? 2 : 1) : 0;
+    const int wgt = wp.on ? (wp.az.on ? 2 : 1) : 0;
     if (chmask == 1 && !deg && wgt == 0)
         launch_tiles_t<T, 1, W, false, 0>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial);
     else if (chmask == 2 && !deg && wgt == 0)

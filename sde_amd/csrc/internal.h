@@ -64,15 +64,33 @@ struct __align__(16) WorkItem {
     int tile, begin, end, pad;
 };
 
-// Per-visibility staging record written in the caller's order by the prep
-// pass and gathered in sorted order (one 32-byte read per point).  rot is
-// the phase rotation of the shift, ((float)cos, (float)sin) of shift_phase
-// -- (1, 0) without a shift -- so no later pass evaluates it again.
+// Per-visibility staging record, written in the caller's order by the prep
+// pass and gathered in sorted order (one 32-byte read per point): every
+// part of the gridding and degridding records that does not depend on the
+// density weights, so the sorted passes evaluate no coordinate step.
+//   gpk:   the gridding record's pk (local u | local v << 8 | u phase << 16 |
+//          v phase << 24), with kStageFlags in bits 5-7 (local u < 32)
+//   fr, fi: GRDH2D's FVIS, the visibility after the shift rotation and the
+//          conjugation (grd/grdh2d.f:124-140)
+//   wcell: the point's density cell (0-based, row-major), kNoCell when off
+//          the density grid or not weighting
+//   dpk:   the degridding record's pk (valid with kStageDegTile)
+//   q2:    the azimuthal robustness factor (5*10**(-ROB))**2 of AZ-NORM
+//   rot:   the phase rotation of the shift, ((float)cos, (float)sin) of
+//          shift_phase -- (1, 0) without a shift
 struct __align__(32) StageRec {
-    float u, v, w, wt;
-    float2 vis;
+    uint32_t gpk;
+    float fr, fi;
+    uint32_t wcell;
+    uint32_t dpk;
+    float q2;
     float2 rot;
 };
+constexpr uint32_t kStageConj = 1u << 5;     // degridding: conjugate (u > 0 half)
+constexpr uint32_t kStageDegTile = 1u << 6;  // degridding: on the tile path
+constexpr uint32_t kStageHalf = 1u << 7;     // density weight halved (u-origin column)
+constexpr uint32_t kStageFlags = kStageConj | kStageDegTile | kStageHalf;
+constexpr uint32_t kNoCell = 0xffffffffu;
 
 // ---- cell sort shared by gridding and degridding (grid.hip, sort.hip) ----
 // The prep pass gives every visibility its tile-major cell key
@@ -125,8 +143,17 @@ int64_t list_capacity(int64_t n);  // list_seg_cap(n) * prep_blocks(n)
 // the points whose REAL density cell differs from their DOUBLE gridding
 // cell (or that are outside the gridding footprint) with atomics; the
 // sorted runs of equal cells add the rest, one atomic per run.
+// Azimuthally varying robustness of GRDRUW2D's AZ-NORM mode
+// (grd/grdruw2d.f:225-229).
+struct AzParams {
+    int on = 0;
+    float rob1 = 0.0f, rpa = 0.0f, rbscale = 0.0f, pi_f = 0.0f;
+};
+AzParams make_az_params(int az, const float *robust, float pi_f);
+
 struct DensityOut {
     CoordParams wcp;       // density-grid coordinate parameters
+    AzParams az;
     double *gwt = nullptr;
     double *blk = nullptr;  // per-block partial weight sums
     unsigned long long *ndrop = nullptr;  // points off the density grid
@@ -156,8 +183,8 @@ void launch_density_runs(hipStream_t s, const SortBufs &b, int64_t n, const Tile
 
 // Robust-weighting parameters of the gridding run (grd/grdruw2d.f:184-240).
 struct WeightParams {
-    int on = 0, az = 0;
-    float rob1 = 0.0f, rpa = 0.0f, rbscale = 0.0f, pi_f = 0.0f;
+    int on = 0;
+    AzParams az;
     CoordParams wcp;
     const double *gwt = nullptr, *wsums = nullptr;
 };
@@ -317,31 +344,60 @@ __device__ __forceinline__ void wcell(const CoordParams &cp, float u, float v, f
     vc = (int)roundf(vcell);
 }
 
-// New weight of one point with WT > 0 from the density grid (the REAL
-// cell value) and the weighting sums wsums = {sum WT, sum GWT^2, F2, D2}
-// (grd/grdruw2d.f:184-240; GRDUWT2D is F2 = 1, D2 = 0).  Returns -1 for a
-// point off the density grid or on an empty cell; *off reports the former.
-__device__ __forceinline__ float robust_weight(const CoordParams &wcp, float u, float v, float w, float wti,
-                                               const double *gwt, const double *wsums, int az, float rob1,
-                                               float rpa, float rbscale, float pi_f, bool *off) {
-    int uc, vc;
+// The weight-independent part of GRDRUW2D's step for one point with WT > 0
+// (grd/grdruw2d.f:197-229): the density cell, whether its weight is halved
+// (the u-origin column away from the v origin) and, for AZ-NORM, the
+// robustness factor (5*10**(-ROB))**2 of the point's angle.
+struct RobustCell {
+    uint32_t cell;  // 0-based row-major density cell; kNoCell: off the density grid
+    int uc, vc;     // 1-based density cell
+    bool half;
+    float q2;
+};
+__device__ __forceinline__ RobustCell robust_cell(const CoordParams &wcp, float u, float v, float w,
+                                                  const AzParams &az) {
+    RobustCell rc;
     float ul, vl;
-    wcell(wcp, u, v, w, uc, vc, ul, vl);
-    *off = uc < 1 || uc > wcp.nu || vc < 1 || vc > wcp.nv;
-    if (*off) return -1.0f;
-    float f2 = (float)wsums[2];
-    const float d2 = (float)wsums[3];
-    if (az) {
-        float t = atan2f(vl, ul);
-        t = fmodf(t - rpa + 2.5f * pi_f, pi_f) - pi_f / 2.0f;
-        const float rb = rob1 - fabsf(t) * rbscale;
-        const float q = 5.0f * powf(10.0f, -rb);
-        f2 = (float)((double)(q * q) / (wsums[1] / wsums[0]));
+    wcell(wcp, u, v, w, rc.uc, rc.vc, ul, vl);
+    rc.half = rc.uc == wcp.uorigin && rc.vc != wcp.vorigin;
+    rc.q2 = 0.0f;
+    if (rc.uc < 1 || rc.uc > wcp.nu || rc.vc < 1 || rc.vc > wcp.nv) {
+        rc.cell = kNoCell;
+        return rc;
     }
-    const float g = (float)gwt[(size_t)(vc - 1) * wcp.nu + uc - 1];
+    rc.cell = (uint32_t)((rc.vc - 1) * wcp.nu + rc.uc - 1);
+    if (az.on) {
+        float t = atan2f(vl, ul);
+        t = fmodf(t - az.rpa + 2.5f * az.pi_f, az.pi_f) - az.pi_f / 2.0f;
+        const float rb = az.rob1 - fabsf(t) * az.rbscale;
+        const float q = 5.0f * powf(10.0f, -rb);
+        rc.q2 = q * q;
+    }
+    return rc;
+}
+
+// New weight of one point with WT > 0 from its density cell and the
+// weighting sums wsums = {sum WT, sum GWT^2, F2, D2} (grd/grdruw2d.f:
+// 225-240; GRDUWT2D is F2 = 1, D2 = 0): -1 off the density grid or on an
+// empty cell.
+__device__ __forceinline__ float robust_apply(float wti, uint32_t cell, bool half, float q2, bool az,
+                                              const double *gwt, const double *wsums) {
+    if (cell == kNoCell) return -1.0f;
+    const float f2 = az ? (float)((double)q2 / (wsums[1] / wsums[0])) : (float)wsums[2];
+    const float d2 = (float)wsums[3];
+    const float g = (float)gwt[cell];
     if (!(g > 0.0f)) return -1.0f;
-    if (uc == wcp.uorigin && vc != wcp.vorigin) return 0.5f * wti / (g * f2 + d2);
+    if (half) return 0.5f * wti / (g * f2 + d2);
     return wti / (g * f2 + d2);
+}
+
+// Both steps; *off reports a point off the density grid.
+__device__ __forceinline__ float robust_weight(const CoordParams &wcp, float u, float v, float w, float wti,
+                                               const double *gwt, const double *wsums, const AzParams &az,
+                                               bool *off) {
+    const RobustCell rc = robust_cell(wcp, u, v, w, az);
+    *off = rc.cell == kNoCell;
+    return robust_apply(wti, rc.cell, rc.half, rc.q2, az.on != 0, gwt, wsums);
 }
 
 // Append the lanes with flag set to a workgroup's list segment: one LDS
