@@ -2,29 +2,26 @@
 // GRDQH2D (grd/grdqh2d.f:70-147).
 //
 // Pipeline (one stream, no host synchronisation):
-//   1. k_sort_count   one thread per visibility: the FP64 cell of GRDH2D
-//                     (the reference's coordinate arithmetic, its drop
-//                     test), key = tile-major cell index, rank = position
-//                     among the points of that cell (one atomic per point
-//                     on ~NU*NV counters: no hot spot).
-//   2. k_scan_*       exclusive scan of the cell counts (block-local scan +
-//                     scanned block totals), tile offsets and work items (a
-//                     tile's records cut into chunks).
-//   3. k_grid_fill    one thread per visibility: phase rotation, conjugate/
-//                     reflect, kernel phases, a 16-byte record written at its
-//                     sorted position; FP64 partial sums of SUMWT and LSUMWT.
-//   4. k_grid_tiles   one workgroup per work item over a padded tile
-//                     (T+6)^2 held in LDS; each wave walks a contiguous run
-//                     of cell-sorted records with lane = kernel tap, so the
-//                     taps of one point land on distinct cells.  Consecutive
-//                     points of one cell accumulate in registers (Kahan);
-//                     when the cell changes the compensated pair is flushed
-//                     into LDS with LDS atomics plus the exact TwoSum error
-//                     into a compensation plane.  The tile is flushed once.
-//                     No global atomics.
-//   5. k_grid_reduce  one thread per output cell: FP64 sum of the padded
-//                     tiles covering it, the Hermitian fold of the u = 0
-//                     column, the shift left by SUPPU and the normalisation.
+//   1. k_prep          one thread per visibility in the caller's order: the
+//                      FP64 cell of GRDH2D and its drop test give the sort
+//                      key (tile-major cell); everything of the gridding and
+//                      degridding records that does not depend on the
+//                      density weights goes into a 32-byte staging record.
+//   2. radix sort      (key, index|weight) pairs, stable (sort.hip); the
+//                      density runs or k_tile_bounds give the tile ranges,
+//                      k_items_scan/k_grid_items the work items (a tile's
+//                      slots cut into chunks).
+//   3. k_grid_tiles    one workgroup per work item over a padded tile
+//                      (T+6)^2 held in LDS in FP64.  Each wave gathers the
+//                      staging records of a contiguous run of cell-sorted
+//                      slots, builds the records in registers (final weight,
+//                      WT*FVIS; the degridding halves go out coalesced) and
+//                      grids them with lane = kernel tap: consecutive points
+//                      of one cell accumulate in FP64 registers and are added
+//                      into LDS when the cell changes.  No global atomics.
+//   4. k_grid_reduce   one thread per output cell: FP64 sum of the padded
+//                      tiles covering it, the Hermitian fold of the u = 0
+//                      column, the shift left by SUPPU and the normalisation.
 #include "internal.h"
 
 namespace {
