@@ -153,9 +153,12 @@ typedef struct {
 
 int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **plan);
 int sde_plan_destroy(sde_plan *plan);
-/* Hand over visibilities.  on_device = 0: host arrays, copied to HBM;
- * on_device = 1: device pointers, copied device-to-device.  vis is complex
- * (2*nvis floats). */
+/* Hand over visibilities.  on_device = 0: host arrays, copied to HBM before
+ * the call returns; on_device = 1: device pointers, copied device-to-device;
+ * on_device = 2: host arrays, copied on the plan's stream without waiting
+ * (the caller keeps them unchanged until the plan's work has completed;
+ * pinned memory from sde_host_alloc makes the copy asynchronous).  vis is
+ * complex (2*nvis floats). */
 int sde_plan_set_vis(sde_plan *plan, int64_t nvis, const float *vis, const float *wt,
                      const float *u, const float *v, const float *w, int on_device);
 /* Weighting, PSF and dirty image (both divided by the PSF peak), all on the
@@ -164,8 +167,38 @@ int sde_plan_uvmap(sde_plan *plan);
 /* Model prediction from an image (host NX*NY floats, or NULL for the plan's
  * dirty image): correct, FFT, degrid (task/vismodel.f:69-78). */
 int sde_plan_vismodel(sde_plan *plan, const float *image);
+/* The same from an NX*NY image already in device memory (another plan's
+ * image buffer or the caller's); nothing is copied. */
+int sde_plan_vismodel_device(sde_plan *plan, const float *device_image);
+/* Load an NX*NY model image into the plan's image buffer (host or device
+ * source; synchronous) and return that buffer's device address, so several
+ * plans can predict from one uploaded image. */
+int sde_plan_set_image(sde_plan *plan, const float *image, int on_device);
+int sde_plan_device_image(sde_plan *plan, float **image);
 int sde_plan_sync(sde_plan *plan);
+/* Change the image cell of a plan (arcsec; the uv cell and coordinate
+ * parameters follow as in sde_plan_create). */
+int sde_plan_set_cell(sde_plan *plan, float cellsize);
+/* Allocate room for nvis visibilities now, so that handing over at most
+ * nvis later allocates (and frees) nothing. */
+int sde_plan_reserve(sde_plan *plan, int64_t nvis);
+/* Non-blocking completion test of the work issued on the plan's stream:
+ * 1 done, 0 still running, -1 error.  Plans run on their own streams, so the
+ * work of several plans issued back to back overlaps on the device. */
+int sde_plan_query(sde_plan *plan);
+/* Device bytes the plan holds, and the bytes a plan of cfg holding nvis
+ * visibilities will hold (admission of jobs by free HBM). */
+int64_t sde_plan_device_bytes(sde_plan *plan);
+int64_t sde_plan_estimate_bytes(const sde_imaging_cfg *cfg, int64_t nvis);
+/* Free and total device memory (hipMemGetInfo). */
+int sde_device_memory(size_t *free_bytes, size_t *total_bytes);
 int sde_plan_get_images(sde_plan *plan, float *dirty, float *psf);
+/* The same copies issued on the plan's stream without waiting (complete when
+ * sde_plan_query reports the plan done; pinned buffers for an asynchronous copy). */
+int sde_plan_get_images_async(sde_plan *plan, float *dirty, float *psf);
+/* Pinned (page-locked) host memory for asynchronous transfers. */
+void *sde_host_alloc(size_t bytes);
+int sde_host_free(void *ptr);
 /* The gridded half planes GVIS(NU,NV) of the last uvmap (dirty, PSF), as
  * GRDH2D returns them: folded, shifted by SUPPU and normalised, before the
  * FFT.  Complex, NU*NV per plane (either pointer may be NULL). */

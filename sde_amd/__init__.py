@@ -45,6 +45,26 @@ def _declare(L):
     L.sde_last_warning.restype = _C.c_char_p
     L.sde_device_count.restype = _C.c_int
     L.sde_version.restype = _C.c_char_p
+    L.sde_plan_device_bytes.restype = _C.c_int64
+    L.sde_plan_estimate_bytes.restype = _C.c_int64
+    L.sde_host_alloc.restype = _C.c_void_p
+    L.sde_host_alloc.argtypes = [_C.c_size_t]
+    L.sde_host_free.argtypes = [_C.c_void_p]
+
+
+def pinned_empty(shape, dtype):
+    """A numpy array in pinned (page-locked) host memory, for asynchronous
+    transfers; the memory is freed when the last view of it goes."""
+    import weakref
+    import numpy as np
+    dtype = np.dtype(dtype)
+    n = int(np.prod(shape)) * dtype.itemsize
+    ptr = lib().sde_host_alloc(n)
+    if not ptr:
+        raise MemoryError("sde_host_alloc: %d bytes: %s" % (n, last_error()))
+    buf = (_C.c_char * max(n, 1)).from_address(ptr)
+    weakref.finalize(buf, lib().sde_host_free, ptr)
+    return np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
 
 
 def device_count():

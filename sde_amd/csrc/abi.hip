@@ -895,6 +895,7 @@ struct sde_plan {
     int *d_list = nullptr, *d_nlist = nullptr;
     float *d_partial = nullptr;
     size_t partial_cap = 0, items_cap = 0;
+    std::map<void *, size_t> alloc_bytes;  // device buffers held (twiddle tables apart)
     double *d_isums = nullptr, *d_sums = nullptr;  // per-work-item SUMWT/LSUMWT; totals
     float2 *d_grids = nullptr, *d_gwork = nullptr, *d_mgrid = nullptr;
     float *d_dirty = nullptr, *d_psf = nullptr, *d_img_in = nullptr;
@@ -952,11 +953,17 @@ void plan_collect(sde_plan *p) {
     p->pending.clear();
 }
 
+// Device buffers of a plan; the bytes held are tracked for admission by free HBM.
 template <typename T>
-int plan_alloc(T **ptr, size_t count) {
-    if (*ptr) (void)hipFree(*ptr);
+int plan_alloc(sde_plan *p, T **ptr, size_t count) {
+    if (*ptr) {
+        p->alloc_bytes.erase((void *)*ptr);
+        (void)hipFree(*ptr);
+    }
     *ptr = nullptr;
-    SDE_CHECK(hipMalloc((void **)ptr, sizeof(T) * (count > 0 ? count : 1)));
+    const size_t bytes = sizeof(T) * (count > 0 ? count : 1);
+    SDE_CHECK(hipMalloc((void **)ptr, bytes));
+    p->alloc_bytes[(void *)*ptr] = bytes;
     return 0;
 }
 
@@ -965,11 +972,11 @@ int plan_reserve_vis(sde_plan *p, int64_t n) {
     p->weighted = false;
     if (n <= p->cap) return 0;
     const size_t c = (size_t)n;
-    if (plan_alloc(&p->d_vis, c) || plan_alloc(&p->d_wt, c) || plan_alloc(&p->d_u, c) || plan_alloc(&p->d_v, c) ||
-        plan_alloc(&p->d_w, c) || plan_alloc(&p->d_wteff, c) || plan_alloc(&p->d_mvis, c) ||
-        plan_alloc(&p->d_mwt, c) || plan_alloc(&p->sb.keys, c) || plan_alloc(&p->sb.skeys, c) ||
-        plan_alloc(&p->sb.vals, c) || plan_alloc(&p->sb.svals, c) || plan_alloc(&p->sb.stage, c) || plan_alloc(&p->d_recd, c) ||
-        plan_alloc(&p->d_list, (size_t)list_capacity(n)))
+    if (plan_alloc(p, &p->d_vis, c) || plan_alloc(p, &p->d_wt, c) || plan_alloc(p, &p->d_u, c) || plan_alloc(p, &p->d_v, c) ||
+        plan_alloc(p, &p->d_w, c) || plan_alloc(p, &p->d_wteff, c) || plan_alloc(p, &p->d_mvis, c) ||
+        plan_alloc(p, &p->d_mwt, c) || plan_alloc(p, &p->sb.keys, c) || plan_alloc(p, &p->sb.skeys, c) ||
+        plan_alloc(p, &p->sb.vals, c) || plan_alloc(p, &p->sb.svals, c) || plan_alloc(p, &p->sb.stage, c) || plan_alloc(p, &p->d_recd, c) ||
+        plan_alloc(p, &p->d_list, (size_t)list_capacity(n)))
         return -1;
     const size_t tb = sort_temp_bytes(n, sort_nbits(p->tp));
     if (tb == 0) {
@@ -977,7 +984,7 @@ int plan_reserve_vis(sde_plan *p, int64_t n) {
         return -1;
     }
     char *temp = static_cast<char *>(p->sb.temp);
-    if (plan_alloc(&temp, tb)) return -1;
+    if (plan_alloc(p, &temp, tb)) return -1;
     p->sb.temp = temp;
     p->sb.temp_bytes = tb;
     p->cap = n;
@@ -987,12 +994,12 @@ int plan_reserve_vis(sde_plan *p, int64_t n) {
 int plan_reserve_items(sde_plan *p) {
     const int mi = max_items_for(p->tp, p->nvis);
     if ((size_t)mi > p->items_cap) {
-        if (plan_alloc(&p->sb.items, (size_t)mi) || plan_alloc(&p->d_isums, 2 * (size_t)mi)) return -1;
+        if (plan_alloc(p, &p->sb.items, (size_t)mi) || plan_alloc(p, &p->d_isums, 2 * (size_t)mi)) return -1;
         p->items_cap = mi;
     }
     const size_t pf = grid_partial_floats(p->tp, 3, mi);
     if (pf > p->partial_cap) {
-        if (plan_alloc(&p->d_partial, pf)) return -1;
+        if (plan_alloc(p, &p->d_partial, pf)) return -1;
         p->partial_cap = pf;
     }
     p->sb.max_items = mi;
@@ -1012,6 +1019,54 @@ int plan_sort(sde_plan *p, const DensityOut *dens, bool bounds = true) {
     p->filled = false;
     return run_sort(p->stream, p->d_vis, p->d_wt, p->d_u, p->d_v, p->d_w, p->nvis, p->cp, p->tp, p->sb, dens, &deg,
                     mb, me, bounds);
+}
+
+// uv cell and coordinate parameters of an image cell (dx, dy in degrees,
+// IMGMAKE's DELT; FFTCONJA's uv increments, fft/fftconja.f:156-168).
+void plan_set_geometry(sde_plan *p, float dx, float dy) {
+    const sde_imaging_cfg *cfg = &p->cfg;
+    p->udelt = conj_delt(cfg->nx, dx);
+    p->vdelt = conj_delt(cfg->ny, dy);
+    p->uscale = 1.0f / p->udelt;
+    p->vscale = 1.0f / p->vdelt;
+    p->cp = make_cp(cfg->shift, p->uscale, 0.0f, p->vscale, 0.0f, p->uorigin, p->vorigin, p->nu, p->nv, p->supp,
+                    p->supp, p->osamp, p->osamp);
+    const float wus = cfg->fov / p->udelt, wvs = cfg->fov / p->vdelt;
+    p->wcp = make_cp(cfg->shift, wus, 0.0f, wvs, 0.0f, p->uorigin, p->vorigin, p->nu, p->nv, 0, 0, 1, 1);
+}
+
+int plan_vismodel(sde_plan *p, const float *img) {
+    hipStream_t s = p->stream;
+    float2 *twnv = p->tw[p->nv], *twfnu = p->tw[p->fnu];
+    plan_mark_begin(p, "fft_rows_model");
+    launch_fft_rows_r2c(s, img, p->cfg.nx, p->cfg.ny, p->fnu, p->nv, twfnu, p->d_ccf, p->ccflen, 1, p->d_mgrid);
+    plan_mark_end(p);
+    plan_mark_begin(p, "fft_cols_model");
+    launch_fft_cols(s, p->d_mgrid, p->d_mgrid, p->nu, p->nv, 1, 0, +1, 1.0f, twnv);
+    plan_mark_end(p);
+    if (p->nvis <= 0) return 0;
+    SDE_CHECK(hipMemsetAsync(p->d_cnt + 1, 0, sizeof(unsigned long long), s));
+    if (!p->filled) {
+        if (plan_sort(p, nullptr)) return -1;
+        plan_mark_begin(p, "degrid_gather");
+        launch_degrid_gather(s, p->sb, p->nvis, p->cp, p->tp, p->d_recd);
+        plan_mark_end(p);
+        p->filled = true;
+    }
+    plan_mark_begin(p, "degrid");
+    if (launch_degrid_tiles(s, p->d_recd, p->sb.items, p->sb.nitems, p->sb.max_items, p->tp, p->d_mgrid, p->d_cf,
+                            p->d_cf, p->cp, p->d_mvis, p->d_mwt, p->d_cnt + 1)) {
+        plan_mark_end(p);
+        set_plan_error("unsupported degridding tile T=%d", p->tp.T);
+        return -1;
+    }
+    plan_mark_end(p);
+    plan_mark_begin(p, "degrid_axis");
+    launch_degrid_list(s, p->d_mgrid, p->d_u, p->d_v, p->d_w, p->d_wt, p->d_vis, p->nvis, p->cp, p->d_cf, p->d_cf,
+                       p->d_mvis, p->d_mwt, p->d_cnt + 1, p->d_list, p->d_nlist);
+    plan_mark_end(p);
+    SDE_CHECK(hipGetLastError());
+    return 0;
 }
 
 }  // namespace
@@ -1044,15 +1099,8 @@ int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **out) {
     p->nv = pow2_at_least(cfg->ny);
     p->uorigin = 1;
     p->vorigin = (p->nv + 1) / 2;
-    p->udelt = conj_delt(cfg->nx, dx);
-    p->vdelt = conj_delt(cfg->ny, dy);
-    p->uscale = 1.0f / p->udelt;
-    p->vscale = 1.0f / p->vdelt;
     p->ccflen = 1 + pow2_at_least(cfg->nx > cfg->ny ? cfg->nx : cfg->ny) / 2;
-    p->cp = make_cp(cfg->shift, p->uscale, 0.0f, p->vscale, 0.0f, p->uorigin, p->vorigin, p->nu, p->nv, p->supp,
-                    p->supp, p->osamp, p->osamp);
-    const float wus = cfg->fov / p->udelt, wvs = cfg->fov / p->vdelt;
-    p->wcp = make_cp(cfg->shift, wus, 0.0f, wvs, 0.0f, p->uorigin, p->vorigin, p->nu, p->nv, 0, 0, 1, 1);
+    plan_set_geometry(p, dx, dy);
     std::string rm = fstr(cfg->rmode, strnlen(cfg->rmode, sizeof cfg->rmode));
     if (feq(rm, "NONE") || rm[0] == ' ') p->wmode = 0;
     else if (feq(rm, "NORM")) p->wmode = 1;
@@ -1083,14 +1131,14 @@ int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **out) {
     std::vector<float> ccf(p->ccflen);
     make_ccf(cft, ccf.data(), p->ccflen);
     const size_t ng = (size_t)p->nu * p->nv, npix = (size_t)cfg->nx * cfg->ny;
-    int rc = plan_alloc(&p->d_cf, p->ncf) || plan_alloc(&p->d_ccf, p->ccflen) || plan_alloc(&p->d_grids, 2 * ng) || plan_alloc(&p->d_gwork, 2 * ng) ||
-             plan_alloc(&p->d_mgrid, ng) || plan_alloc(&p->d_dirty, npix) || plan_alloc(&p->d_psf, npix) ||
-             plan_alloc(&p->d_img_in, npix) || plan_alloc(&p->d_peak, 4) || plan_alloc(&p->d_gwt, ng) ||
-             plan_alloc(&p->d_wblk, 4096) || plan_alloc(&p->d_wsums, 4) || plan_alloc(&p->d_cnt, 4) ||
-             plan_alloc(&p->d_wsq, kWtSqBlocks) ||
-             plan_alloc(&p->sb.tile_start, p->tp.ntiles + 1) || plan_alloc(&p->sb.item_start, p->tp.ntiles + 1) ||
-             plan_alloc(&p->sb.nitems, 1) || plan_alloc(&p->d_nlist, kPrepBlocks) ||
-             plan_alloc(&p->d_sums, 4);
+    int rc = plan_alloc(p, &p->d_cf, p->ncf) || plan_alloc(p, &p->d_ccf, p->ccflen) || plan_alloc(p, &p->d_grids, 2 * ng) || plan_alloc(p, &p->d_gwork, 2 * ng) ||
+             plan_alloc(p, &p->d_mgrid, ng) || plan_alloc(p, &p->d_dirty, npix) || plan_alloc(p, &p->d_psf, npix) ||
+             plan_alloc(p, &p->d_img_in, npix) || plan_alloc(p, &p->d_peak, 4) || plan_alloc(p, &p->d_gwt, ng) ||
+             plan_alloc(p, &p->d_wblk, 4096) || plan_alloc(p, &p->d_wsums, 4) || plan_alloc(p, &p->d_cnt, 4) ||
+             plan_alloc(p, &p->d_wsq, kWtSqBlocks) ||
+             plan_alloc(p, &p->sb.tile_start, p->tp.ntiles + 1) || plan_alloc(p, &p->sb.item_start, p->tp.ntiles + 1) ||
+             plan_alloc(p, &p->sb.nitems, 1) || plan_alloc(p, &p->d_nlist, kPrepBlocks) ||
+             plan_alloc(p, &p->d_sums, 4);
     if (rc) {
         sde_plan_destroy(p);
         return -1;
@@ -1139,7 +1187,7 @@ int sde_plan_set_vis(sde_plan *p, int64_t nvis, const float *vis, const float *w
     }
     if (plan_reserve_vis(p, nvis)) return -1;
     p->nvis = nvis;
-    const hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipMemcpyKind k = on_device == 1 ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     if (nvis > 0) {
         SDE_CHECK(hipMemcpyAsync(p->d_vis, vis, sizeof(float2) * nvis, k, p->stream));
         SDE_CHECK(hipMemcpyAsync(p->d_wt, wt, sizeof(float) * nvis, k, p->stream));
@@ -1149,7 +1197,7 @@ int sde_plan_set_vis(sde_plan *p, int64_t nvis, const float *vis, const float *w
     }
     if (p->cfg.chunk <= 0) p->tp.chunk = default_chunk(nvis);
     if (plan_reserve_items(p)) return -1;
-    SDE_CHECK(hipStreamSynchronize(p->stream));
+    if (on_device != 2) SDE_CHECK(hipStreamSynchronize(p->stream));
     return 0;
 }
 
@@ -1238,42 +1286,62 @@ int sde_plan_uvmap(sde_plan *p) {
 
 int sde_plan_vismodel(sde_plan *p, const float *image) {
     if (!p) return -1;
-    hipStream_t s = p->stream;
-    const size_t npix = (size_t)p->cfg.nx * p->cfg.ny;
     const float *img = p->d_dirty;
     if (image) {
-        SDE_CHECK(hipMemcpyAsync(p->d_img_in, image, sizeof(float) * npix, hipMemcpyHostToDevice, s));
+        const size_t npix = (size_t)p->cfg.nx * p->cfg.ny;
+        SDE_CHECK(hipMemcpyAsync(p->d_img_in, image, sizeof(float) * npix, hipMemcpyHostToDevice, p->stream));
         img = p->d_img_in;
     }
-    float2 *twnv = p->tw[p->nv], *twfnu = p->tw[p->fnu];
-    plan_mark_begin(p, "fft_rows_model");
-    launch_fft_rows_r2c(s, img, p->cfg.nx, p->cfg.ny, p->fnu, p->nv, twfnu, p->d_ccf, p->ccflen, 1, p->d_mgrid);
-    plan_mark_end(p);
-    plan_mark_begin(p, "fft_cols_model");
-    launch_fft_cols(s, p->d_mgrid, p->d_mgrid, p->nu, p->nv, 1, 0, +1, 1.0f, twnv);
-    plan_mark_end(p);
-    if (p->nvis <= 0) return 0;
-    SDE_CHECK(hipMemsetAsync(p->d_cnt + 1, 0, sizeof(unsigned long long), s));
-    if (!p->filled) {
-        if (plan_sort(p, nullptr)) return -1;
-        plan_mark_begin(p, "degrid_gather");
-        launch_degrid_gather(s, p->sb, p->nvis, p->cp, p->tp, p->d_recd);
-        plan_mark_end(p);
-        p->filled = true;
-    }
-    plan_mark_begin(p, "degrid");
-    if (launch_degrid_tiles(s, p->d_recd, p->sb.items, p->sb.nitems, p->sb.max_items, p->tp, p->d_mgrid, p->d_cf,
-                            p->d_cf, p->cp, p->d_mvis, p->d_mwt, p->d_cnt + 1)) {
-        plan_mark_end(p);
-        set_plan_error("unsupported degridding tile T=%d", p->tp.T);
+    return plan_vismodel(p, img);
+}
+
+int sde_plan_vismodel_device(sde_plan *p, const float *device_image) {
+    if (!p || !device_image) return -1;
+    return plan_vismodel(p, device_image);
+}
+
+int sde_plan_set_image(sde_plan *p, const float *image, int on_device) {
+    if (!p || !image) return -1;
+    const size_t npix = (size_t)p->cfg.nx * p->cfg.ny;
+    SDE_CHECK(hipMemcpyAsync(p->d_img_in, image, sizeof(float) * npix,
+                             on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, p->stream));
+    SDE_CHECK(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+int sde_plan_device_image(sde_plan *p, float **image) {
+    if (!p || !image) return -1;
+    *image = p->d_img_in;
+    return 0;
+}
+
+int sde_plan_set_cell(sde_plan *p, float cellsize) {
+    if (!p || !(cellsize > 0.0f)) {
+        set_plan_error("sde_plan_set_cell: bad cellsize");
         return -1;
     }
-    plan_mark_end(p);
-    plan_mark_begin(p, "degrid_axis");
-    launch_degrid_list(s, p->d_mgrid, p->d_u, p->d_v, p->d_w, p->d_wt, p->d_vis, p->nvis, p->cp, p->d_cf, p->d_cf,
-                       p->d_mvis, p->d_mwt, p->d_cnt + 1, p->d_list, p->d_nlist);
-    plan_mark_end(p);
-    SDE_CHECK(hipGetLastError());
+    p->cfg.cellsize = cellsize;
+    plan_set_geometry(p, -cellsize / 3600.0f, cellsize / 3600.0f);
+    p->filled = false;
+    p->weighted = false;
+    return 0;
+}
+
+int sde_plan_reserve(sde_plan *p, int64_t nvis) {
+    if (!p || nvis < 0) return -1;
+    if (plan_reserve_vis(p, nvis)) return -1;
+    // work items for nvis at the smallest default chunk, so any set_vis of at
+    // most nvis visibilities allocates nothing
+    const int chunk = p->tp.chunk;
+    const int64_t n0 = p->nvis;
+    p->tp.chunk = p->cfg.chunk > 0 ? p->cfg.chunk : default_chunk(0);
+    p->nvis = nvis;
+    const int rc = plan_reserve_items(p);
+    p->nvis = n0;
+    p->tp.chunk = chunk;
+    if (rc) return -1;
+    p->sb.max_items = max_items_for(p->tp, p->nvis);
+    SDE_CHECK(hipStreamSynchronize(p->stream));
     return 0;
 }
 
@@ -1284,12 +1352,81 @@ int sde_plan_sync(sde_plan *p) {
     return 0;
 }
 
+int sde_plan_query(sde_plan *p) {
+    if (!p) return -1;
+    const hipError_t e = hipStreamQuery(p->stream);
+    if (e == hipErrorNotReady) return 0;
+    if (e != hipSuccess) return sde_fail_hip(e, "hipStreamQuery", __FILE__, __LINE__);
+    plan_collect(p);
+    return 1;
+}
+
+int64_t sde_plan_device_bytes(sde_plan *p) {
+    if (!p) return -1;
+    size_t b = 0;
+    for (const auto &kv : p->alloc_bytes) b += kv.second;
+    for (const auto &kv : p->tw) b += sizeof(float2) * (size_t)kv.first;
+    return (int64_t)b;
+}
+
+int64_t sde_plan_estimate_bytes(const sde_imaging_cfg *cfg, int64_t nvis) {
+    if (!cfg || cfg->nx < 2 || cfg->ny < 2 || cfg->nx > 8192 || cfg->ny > 8192 || nvis < 0) return -1;
+    // the buffers sde_plan_create, sde_plan_set_vis and the work items allocate
+    const int fnu = pow2_at_least(cfg->nx), nu = fnu / 2 + 1, nv = pow2_at_least(cfg->ny);
+    const int ccflen = 1 + pow2_at_least(cfg->nx > cfg->ny ? cfg->nx : cfg->ny) / 2;
+    const size_t ng = (size_t)nu * nv, npix = (size_t)cfg->nx * cfg->ny;
+    const TileParams tp = make_tp(nu, nv, cfg->tile ? cfg->tile : default_tile(nu, nv),
+                                  cfg->chunk > 0 ? cfg->chunk : default_chunk(nvis), cfg->waves ? cfg->waves : 4);
+    size_t b = sizeof(float) * (1 + 2 * 4 * 128 + (size_t)ccflen) + sizeof(float2) * 5 * ng +
+               sizeof(float) * 3 * npix + sizeof(double) * ng + sizeof(double) * (4096 + kWtSqBlocks + 8) +
+               sizeof(int) * (2 * ((size_t)tp.ntiles + 1) + 1 + kPrepBlocks + 4) + sizeof(float2) * (size_t)(nv + fnu);
+    if (nvis > 0) {
+        const size_t n = (size_t)nvis;
+        b += n * (sizeof(float2) * 2 + sizeof(float) * 6 + sizeof(uint32_t) * 2 + sizeof(uint64_t) * 2 +
+                  sizeof(StageRec) + sizeof(DegridRec));
+        b += sizeof(int) * (size_t)list_capacity(nvis) + sort_temp_bytes(nvis, sort_nbits(tp));
+    }
+    const size_t mi = (size_t)max_items_for(tp, nvis);
+    b += mi * (sizeof(WorkItem) + 2 * sizeof(double)) + sizeof(float) * grid_partial_floats(tp, 3, (int)mi);
+    return (int64_t)b;
+}
+
+int sde_device_memory(size_t *free_bytes, size_t *total_bytes) {
+    size_t f = 0, t = 0;
+    SDE_CHECK(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return 0;
+}
+
 int sde_plan_get_images(sde_plan *p, float *dirty, float *psf) {
     if (!p) return -1;
     const size_t npix = (size_t)p->cfg.nx * p->cfg.ny;
     if (dirty) SDE_CHECK(hipMemcpyAsync(dirty, p->d_dirty, sizeof(float) * npix, hipMemcpyDeviceToHost, p->stream));
     if (psf) SDE_CHECK(hipMemcpyAsync(psf, p->d_psf, sizeof(float) * npix, hipMemcpyDeviceToHost, p->stream));
     SDE_CHECK(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+int sde_plan_get_images_async(sde_plan *p, float *dirty, float *psf) {
+    if (!p) return -1;
+    const size_t npix = (size_t)p->cfg.nx * p->cfg.ny;
+    if (dirty) SDE_CHECK(hipMemcpyAsync(dirty, p->d_dirty, sizeof(float) * npix, hipMemcpyDeviceToHost, p->stream));
+    if (psf) SDE_CHECK(hipMemcpyAsync(psf, p->d_psf, sizeof(float) * npix, hipMemcpyDeviceToHost, p->stream));
+    return 0;
+}
+
+void *sde_host_alloc(size_t bytes) {
+    void *ptr = nullptr;
+    if (hipHostMalloc(&ptr, bytes > 0 ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+        set_plan_error("sde_host_alloc: pinned allocation of %zu bytes failed", bytes);
+        return nullptr;
+    }
+    return ptr;
+}
+
+int sde_host_free(void *ptr) {
+    if (ptr) SDE_CHECK(hipHostFree(ptr));
     return 0;
 }
 

@@ -76,29 +76,32 @@ class Plan:
     def __init__(self, nx, ny=None, cellsize=1.0, cftype="SF", shift=None, weighting=0, fov=1.0, rmode="NONE",
                  robust=(0.0, 0.0, 0.0), ds=0.0, tile=0, chunk=0, waves=0):
         L = lib()
-        cfg = ImagingCfg()
-        cfg.nx, cfg.ny = int(nx), int(nx if ny is None else ny)
-        cfg.cellsize = float(cellsize)
-        cfg.cftype = cftype.encode()
-        sh = np.eye(3).reshape(-1) if shift is None else np.asarray(shift, np.float64).reshape(-1)
-        cfg.shift = (C.c_double * 9)(*sh)
-        cfg.weighting = int(weighting)
-        cfg.fov = float(fov)
-        cfg.rmode = rmode.encode()
-        cfg.robust = (C.c_float * 3)(*[float(x) for x in robust])
-        cfg.ds = float(ds)
-        cfg.tile, cfg.chunk, cfg.waves = int(tile), int(chunk), int(waves)
+        cfg = _cfg(nx, ny, cellsize, cftype, shift, weighting, fov, rmode, robust, ds, tile, chunk, waves)
         self.cfg = cfg
         self._p = C.c_void_p()
         if L.sde_plan_create(C.byref(cfg), C.byref(self._p)) != 0:
             raise SdeError("sde_plan_create: " + L.sde_last_error().decode())
-        g = [C.c_int() for _ in range(5)]
-        us, vs = C.c_float(), C.c_float()
-        L.sde_plan_geometry(self._p, *[C.byref(x) for x in g], C.byref(us), C.byref(vs))
-        self.nu, self.nv, self.fnu, self.uorigin, self.vorigin = [x.value for x in g]
-        self.uscale, self.vscale = us.value, vs.value
+        self._geometry()
         self.nx, self.ny = cfg.nx, cfg.ny
         self.nvis = 0
+
+    def _geometry(self):
+        g = [C.c_int() for _ in range(5)]
+        us, vs = C.c_float(), C.c_float()
+        lib().sde_plan_geometry(self._p, *[C.byref(x) for x in g], C.byref(us), C.byref(vs))
+        self.nu, self.nv, self.fnu, self.uorigin, self.vorigin = [x.value for x in g]
+        self.uscale, self.vscale = us.value, vs.value
+
+    def set_cell(self, cellsize):
+        """Change the image cell (arcsec) of the plan."""
+        if float(cellsize) != self.cfg.cellsize:
+            self._ok(lib().sde_plan_set_cell(self._p, C.c_float(float(cellsize))), "sde_plan_set_cell")
+            self.cfg.cellsize = float(cellsize)
+            self._geometry()
+
+    def reserve(self, nvis):
+        """Room for nvis visibilities, so later hand-overs of at most nvis allocate nothing."""
+        self._ok(lib().sde_plan_reserve(self._p, C.c_int64(int(nvis))), "sde_plan_reserve")
 
     def _ok(self, rc, what):
         if rc != 0:
@@ -123,6 +126,32 @@ class Plan:
         self._ok(lib().sde_plan_set_vis(self._p, C.c_int64(len(vis)), vis.ctypes.data_as(fp),
                                         *[a.ctypes.data_as(fp) for a in arrs], 0), "sde_plan_set_vis")
         self.nvis = len(vis)
+
+    def set_vis_async(self, vis, wt, u, v, w):
+        """Hand over host arrays on the plan's stream without waiting: the arrays
+        (pinned, from sde_amd.pinned_empty, for an asynchronous copy) must stay
+        unchanged until the plan's work has completed."""
+        if vis.dtype != np.complex64 or any(a.dtype != np.float32 for a in (wt, u, v, w)):
+            raise TypeError("set_vis_async: complex64 vis and float32 wt, u, v, w")
+        if not all(a.flags.c_contiguous for a in (vis, wt, u, v, w)):
+            raise ValueError("set_vis_async: contiguous arrays")
+        self._keep = (vis, wt, u, v, w)
+        fp = C.POINTER(C.c_float)
+        self._ok(lib().sde_plan_set_vis(self._p, C.c_int64(len(vis)), vis.ctypes.data_as(fp),
+                                        *[a.ctypes.data_as(fp) for a in (wt, u, v, w)], 2), "sde_plan_set_vis")
+        self.nvis = len(vis)
+
+    def images_async(self, dirty, psf):
+        """Issue the copies of the dirty image and PSF into (NY, NX) float32 host
+        arrays (pinned for an asynchronous copy) on the plan's stream; they are
+        complete when query() reports the plan done."""
+        for a in (dirty, psf):
+            if a.dtype != np.float32 or a.shape != (self.ny, self.nx) or not a.flags.c_contiguous:
+                raise ValueError("images_async: contiguous float32 (NY, NX) arrays")
+        self._img_out = (dirty, psf)
+        fp = C.POINTER(C.c_float)
+        self._ok(lib().sde_plan_get_images_async(self._p, dirty.ctypes.data_as(fp), psf.ctypes.data_as(fp)),
+                 "get_images_async")
 
     def synth(self, enu, lat, dec, ha0, dha, ntime, freqs, flux, l, m, sigma, seed, nkeep=0):
         """Fill the plan with a synthetic track (see sde_amd.synth)."""
@@ -153,8 +182,37 @@ class Plan:
             self._ok(lib().sde_plan_vismodel(self._p, img.ctypes.data_as(C.POINTER(C.c_float))),
                      "sde_plan_vismodel")
 
+    def set_image(self, image):
+        """Load a model image (NY, NX) into the plan's image buffer (synchronous)."""
+        img = np.ascontiguousarray(image, np.float32)
+        if img.shape != (self.ny, self.nx):
+            raise ValueError("image shape must be (NY, NX)")
+        self._ok(lib().sde_plan_set_image(self._p, img.ctypes.data_as(C.POINTER(C.c_float)), 0), "set_image")
+
+    def device_image(self):
+        """Device address of the plan's image buffer."""
+        ptr = C.POINTER(C.c_float)()
+        self._ok(lib().sde_plan_device_image(self._p, C.byref(ptr)), "device_image")
+        return C.cast(ptr, C.c_void_p).value
+
+    def vismodel_device(self, device_image):
+        """Predict from an (NY, NX) image already in device memory (an address
+        from device_image() of this or another plan of the same image size)."""
+        self._ok(lib().sde_plan_vismodel_device(self._p, C.c_void_p(device_image)), "sde_plan_vismodel_device")
+
     def sync(self):
         self._ok(lib().sde_plan_sync(self._p), "sde_plan_sync")
+
+    def query(self):
+        """True when the work issued on the plan's stream has completed (non-blocking)."""
+        rc = lib().sde_plan_query(self._p)
+        if rc < 0:
+            raise SdeError("sde_plan_query: " + lib().sde_last_error().decode())
+        return rc == 1
+
+    def device_bytes(self):
+        """Device memory the plan holds."""
+        return int(lib().sde_plan_device_bytes(self._p))
 
     def images(self):
         d = np.zeros((self.ny, self.nx), np.float32)
@@ -219,6 +277,39 @@ class Plan:
 
     def reset_timings(self):
         self._ok(lib().sde_plan_reset_timings(self._p), "reset_timings")
+
+
+def _cfg(nx, ny=None, cellsize=1.0, cftype="SF", shift=None, weighting=0, fov=1.0, rmode="NONE",
+         robust=(0.0, 0.0, 0.0), ds=0.0, tile=0, chunk=0, waves=0):
+    cfg = ImagingCfg()
+    cfg.nx, cfg.ny = int(nx), int(nx if ny is None else ny)
+    cfg.cellsize = float(cellsize)
+    cfg.cftype = cftype.encode()
+    sh = np.eye(3).reshape(-1) if shift is None else np.asarray(shift, np.float64).reshape(-1)
+    cfg.shift = (C.c_double * 9)(*sh)
+    cfg.weighting = int(weighting)
+    cfg.fov = float(fov)
+    cfg.rmode = rmode.encode()
+    cfg.robust = (C.c_float * 3)(*[float(x) for x in robust])
+    cfg.ds = float(ds)
+    cfg.tile, cfg.chunk, cfg.waves = int(tile), int(chunk), int(waves)
+    return cfg
+
+
+def estimate_bytes(nvis, nx, ny=None, **plan_args):
+    """Device bytes a plan of these arguments holding nvis visibilities will hold."""
+    b = lib().sde_plan_estimate_bytes(C.byref(_cfg(nx, ny, **plan_args)), C.c_int64(int(nvis)))
+    if b < 0:
+        raise SdeError("sde_plan_estimate_bytes: bad arguments")
+    return int(b)
+
+
+def device_memory():
+    """(free, total) device bytes."""
+    f, t = C.c_size_t(), C.c_size_t()
+    if lib().sde_device_memory(C.byref(f), C.byref(t)) != 0:
+        raise SdeError("sde_device_memory: " + lib().sde_last_error().decode())
+    return f.value, t.value
 
 
 def uvmap(vis, wt, u, v, w, nx, cellsize, ny=None, fov=0.0, rmode="NONE", robust=(0.0, 0.0, 0.0), ds=0.0,
