@@ -339,3 +339,51 @@ def vismodel(image, wt, u, v, w, cellsize, cftype="SF", shift=None, vis=None, **
     finally:
         p.close()
     return mv, mw
+
+
+def imgdoubl(image):
+    """IMGDOUBL + PIXRPAD (img/imgdoubl.f:61-86, pix/pixrpad.f:46-69): pad an
+    (NY, NX) image to FFTPWR2(2N) on each axis, the image placed with left
+    margin LSHIFT = NINT(PN/2 - RPIX) so that its reference pixel RPIX =
+    (N+1)/2 (IMGMAKE, integer division) lands on PN/2.  Returns the padded
+    image and (LSHIFT_x, LSHIFT_y)."""
+    from . import grd
+    img = np.asarray(image, np.float32)
+    ny, nx = img.shape
+    out_shape, shifts = [], []
+    for n in (nx, ny):
+        if n > 1:
+            pn = max(1, grd.fftpwr2(2 * n))
+            rpix = np.float32((n + 1) // 2)
+            ls = int(np.floor(np.float32(pn // 2) - rpix + np.float32(0.5)))  # NINT of a non-negative REAL
+        else:
+            pn, ls = 1, 0
+        out_shape.append(pn)
+        shifts.append(ls)
+    pad = np.zeros((out_shape[1], out_shape[0]), np.float32)
+    pad[shifts[1]:shifts[1] + ny, shifts[0]:shifts[0] + nx] = img
+    return pad, (shifts[0], shifts[1])
+
+
+def imgtovis(image, wt, u, v, w, cellsize, cftype="SF", shift=None, vis=None, **tune):
+    """IMGTOVIS (img/imgtovis.f:55-84): the padded prediction -- the image
+    doubled (IMGDOUBL), its grid derived from the padded image (FFTCONJA,
+    uv cells half as large), then correct, FFT and degrid as vismodel.
+    Returns (vis, wt-with-flags)."""
+    pad, _ = imgdoubl(image)
+    return vismodel(pad, wt, u, v, w, cellsize, cftype=cftype, shift=shift, vis=vis, **tune)
+
+
+def uvmap_files(uv_path, dirty_path, beam_path, nx, cellsize, ny=None, fov=0.0, rmode="NONE",
+                robust=(0.0, 0.0, 0.0), ds=0.0, cftype="SF", **tune):
+    """uvmap from a random-groups visibility file to dirty and beam image
+    files (sde_amd.fits: FTSGRPRE/VISSTD/VISCONVS ingest, IMGMAKE header).
+    Returns the run info."""
+    from . import fits
+    d = fits.read_uvfits(uv_path)
+    dirty, psf, info = uvmap(d["vis"], d["wt"], d["u"], d["v"], d["w"], nx, cellsize, ny=ny, fov=fov, rmode=rmode,
+                             robust=robust, ds=ds, cftype=cftype, **tune)
+    fits.write_image(dirty_path, dirty, cellsize, ra=d["ra"], dec=d["dec"], freq=d["freq"])
+    fits.write_image(beam_path, psf, cellsize, ra=d["ra"], dec=d["dec"], freq=d["freq"])
+    return info
+

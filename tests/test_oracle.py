@@ -403,3 +403,22 @@ def test_3c10_fp64_oracle_agrees(golden_3c10):
     inner = (slice(32, 96), slice(32, 96))
     assert np.abs(d32 - d64)[inner].max() < 1e-5 * np.abs(d64).max()
     assert np.abs(d64 - d["dirty"])[inner].max() < 5e-5 * np.abs(d64).max()
+
+
+def test_imgdoubl_padding_and_reference_pixel():
+    """IMGDOUBL/PIXRPAD (img/imgdoubl.f:61-86, pix/pixrpad.f:46-69): PN =
+    FFTPWR2(2N), LSHIFT = NINT(PN/2 - RPIX) with RPIX = (N+1)/2, so the
+    reference pixel lands on PN/2; the image is copied with that left margin
+    into zeros."""
+    from sde_amd import imaging
+    for (nx, ny) in ((256, 256), (100, 60), (101, 33), (128, 1)):
+        img = np.arange(nx * ny, dtype=np.float32).reshape(ny, nx) + 1.0
+        pad, (lx, ly) = imaging.imgdoubl(img)
+        pnx = po.fftpwr2(2 * nx)
+        pny = po.fftpwr2(2 * ny) if ny > 1 else 1
+        assert pad.shape == (pny, pnx)
+        assert lx == pnx // 2 - (nx + 1) // 2
+        assert ly == (pny // 2 - (ny + 1) // 2 if ny > 1 else 0)
+        assert (nx + 1) // 2 + lx == pnx // 2
+        np.testing.assert_array_equal(pad[ly:ly + ny, lx:lx + nx], img)
+        assert float(pad.sum()) == float(img.sum())
