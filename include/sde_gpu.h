@@ -142,7 +142,7 @@ typedef struct {
 /* Per-visibility accounting of the last run. */
 typedef struct {
     int64_t nvis;          /* visibilities held */
-    int64_t ngridded;      /* gridded (weight > 0 and inside the grid) */
+    int64_t ngridded;      /* in the cell sort of the last run: WT > 0 and inside the grid */
     int64_t ndropped;      /* dropped off the density grid (weight set to -1) */
     int64_t nbad;          /* not degridded (weight left negated) */
     double sumwt;          /* sum of gridded weights (double) */

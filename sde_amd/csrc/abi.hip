@@ -1148,7 +1148,8 @@ int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **out) {
         sde_plan_destroy(p);
         return -1;
     }
-    if (hipMemcpyAsync(p->d_cf, cf, sizeof(float) * p->ncf, hipMemcpyHostToDevice, p->stream) != hipSuccess ||
+    if (hipMemsetAsync(p->sb.tile_start, 0, sizeof(int) * (p->tp.ntiles + 1), p->stream) != hipSuccess ||
+        hipMemcpyAsync(p->d_cf, cf, sizeof(float) * p->ncf, hipMemcpyHostToDevice, p->stream) != hipSuccess ||
         hipMemcpyAsync(p->d_ccf, ccf.data(), sizeof(float) * p->ccflen, hipMemcpyHostToDevice, p->stream) !=
             hipSuccess ||
         hipStreamSynchronize(p->stream) != hipSuccess) {
@@ -1476,6 +1477,9 @@ int sde_plan_info(sde_plan *p, sde_run_info *info) {
     SDE_CHECK(hipMemcpyAsync(&key, p->d_peak, sizeof key, hipMemcpyDeviceToHost, p->stream));
     SDE_CHECK(hipMemcpyAsync(&nitems, p->sb.nitems, sizeof nitems, hipMemcpyDeviceToHost, p->stream));
     SDE_CHECK(hipMemcpyAsync(cnt, p->d_cnt, sizeof cnt, hipMemcpyDeviceToHost, p->stream));
+    int nsorted = 0;
+    SDE_CHECK(hipMemcpyAsync(&nsorted, p->sb.tile_start + p->tp.ntiles, sizeof nsorted, hipMemcpyDeviceToHost,
+                             p->stream));
     SDE_CHECK(hipStreamSynchronize(p->stream));
     int bits = key >= 0 ? key : key ^ 0x7fffffff;
     float peak;
@@ -1487,7 +1491,7 @@ int sde_plan_info(sde_plan *p, sde_run_info *info) {
     info->nitems = nitems;
     info->ndropped = (int64_t)cnt[0];
     info->nbad = (int64_t)cnt[1];
-    info->ngridded = 0;
+    info->ngridded = nsorted;
     return 0;
 }
 

@@ -198,4 +198,4 @@ def test_uvmap_from_and_to_fits(tmp_path):
     d2, b2, _ = imaging.uvmap(back["vis"], back["wt"], back["u"], back["v"], back["w"], d["nx"], d["cellsize"])
     np.testing.assert_array_equal(dm, d2)
     np.testing.assert_array_equal(bm, b2)
-    assert info["ngridded"] > 0 and hd["CRVAL1"] == 10.0
+    assert info["sumwt"] > 0 and hd["CRVAL1"] == 10.0

@@ -383,4 +383,4 @@ def test_imgtovis_padded_prediction():
     f_pad = float(np.max(np.abs(mv - dft))) / scale
     f_un = float(np.max(np.abs(un - dft))) / scale
     print("imgtovis rel err gpu %.2e ref32 %.2e; DFT floor padded %.2e unpadded %.2e" % (e, e32, f_pad, f_un))
-    assert f_pad < 0.2 * f_un
+    assert f_pad < 0.5 * f_un
