@@ -365,6 +365,10 @@ using MarkE = std::function<void()>;
 int run_sort(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v, const float *w,
              int64_t nvis, const CoordParams &cp, const TileParams &tp, SortBufs &sb, const DensityOut *dens,
              const DegridOut *deg, const MarkB &mb, const MarkE &me, bool bounds = true) {
+    // every point needs its cold staging record when the weights need the
+    // density cell (density scale not the gridding scale) or the AZ-NORM
+    // factor, or the prediction needs the shift rotation
+    sb.cold_all = cp.doshift || (dens && (dens->az.on || !dens->gwt));
     mb("prep");
     launch_prep(s, vis, wt, u, v, w, nvis, cp, tp, sb, dens, deg);
     me();
@@ -513,7 +517,8 @@ void grdh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, flo
     sb.skeys = ctx_buf<uint32_t>(c, "skeys", nn);
     sb.vals = ctx_buf<uint64_t>(c, "vals", nn);
     sb.svals = ctx_buf<uint64_t>(c, "svals", nn);
-    sb.stage = ctx_buf<StageRec>(c, "stage", nn);
+    sb.hot = ctx_buf<StageHot>(c, "stage_hot", nn);
+    sb.cold = ctx_buf<StageCold>(c, "stage_cold", nn);
     sb.temp = ctx_buf<char>(c, "sort_temp", sb.temp_bytes > 0 ? sb.temp_bytes : 1);
     sb.tile_start = ctx_buf<int>(c, "tile_start", tp.ntiles + 1);
     sb.item_start = ctx_buf<int>(c, "item_start", tp.ntiles + 1);
@@ -522,7 +527,7 @@ void grdh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, flo
     float *partial = ctx_buf<float>(c, "partial", grid_partial_floats(tp, chmask, sb.max_items));
     double *blk = ctx_buf<double>(c, "blk", 2 * (size_t)sb.max_items), *dsums = ctx_buf<double>(c, "sums", 4);
     float2 *dg = ctx_buf<float2>(c, "grid", (size_t)*nu * *nv);
-    if (!dvis || !dwt || !du || !dv || !dw || !dcfu || !dcfv || !sb.keys || !sb.skeys || !sb.vals || !sb.svals || !sb.stage ||
+    if (!dvis || !dwt || !du || !dv || !dw || !dcfu || !dcfv || !sb.keys || !sb.skeys || !sb.vals || !sb.svals || !sb.hot || !sb.cold ||
         !sb.temp || sb.temp_bytes == 0 || !sb.tile_start || !sb.item_start || !sb.nitems || !sb.items ||
         !partial || !blk || !dsums || !dg) {
         report_error(kErrNoMem, R, "device allocation failed");
@@ -975,7 +980,7 @@ int plan_reserve_vis(sde_plan *p, int64_t n) {
     if (plan_alloc(p, &p->d_vis, c) || plan_alloc(p, &p->d_wt, c) || plan_alloc(p, &p->d_u, c) || plan_alloc(p, &p->d_v, c) ||
         plan_alloc(p, &p->d_w, c) || plan_alloc(p, &p->d_wteff, c) || plan_alloc(p, &p->d_mvis, c) ||
         plan_alloc(p, &p->d_mwt, c) || plan_alloc(p, &p->sb.keys, c) || plan_alloc(p, &p->sb.skeys, c) ||
-        plan_alloc(p, &p->sb.vals, c) || plan_alloc(p, &p->sb.svals, c) || plan_alloc(p, &p->sb.stage, c) || plan_alloc(p, &p->d_recd, c) ||
+        plan_alloc(p, &p->sb.vals, c) || plan_alloc(p, &p->sb.svals, c) || plan_alloc(p, &p->sb.hot, c) || plan_alloc(p, &p->sb.cold, c) || plan_alloc(p, &p->d_recd, c) ||
         plan_alloc(p, &p->d_list, (size_t)list_capacity(n)))
         return -1;
     const size_t tb = sort_temp_bytes(n, sort_nbits(p->tp));
@@ -1167,7 +1172,7 @@ int sde_plan_destroy(sde_plan *p) {
     plan_collect(p);
     for (auto e : p->pool) (void)hipEventDestroy(e);
     void *bufs[] = {p->d_cf, p->d_ccf, p->d_vis, p->d_wt, p->d_u, p->d_v, p->d_w, p->d_wteff, p->d_mvis, p->d_mwt,
-                    p->sb.keys, p->sb.skeys, p->sb.vals, p->sb.svals, p->sb.stage, p->sb.temp, p->sb.tile_start,
+                    p->sb.keys, p->sb.skeys, p->sb.vals, p->sb.svals, p->sb.hot, p->sb.cold, p->sb.temp, p->sb.tile_start,
                     p->sb.item_start, p->sb.nitems, p->sb.items, p->d_recd, p->d_list, p->d_nlist,
                     p->d_partial, p->d_isums, p->d_sums, p->d_grids, p->d_gwork, p->d_mgrid, p->d_dirty, p->d_psf, p->d_img_in,
                     p->d_peak, p->d_gwt, p->d_wblk, p->d_wsums, p->d_cnt, p->d_wsq};
@@ -1384,7 +1389,7 @@ int64_t sde_plan_estimate_bytes(const sde_imaging_cfg *cfg, int64_t nvis) {
     if (nvis > 0) {
         const size_t n = (size_t)nvis;
         b += n * (sizeof(float2) * 2 + sizeof(float) * 6 + sizeof(uint32_t) * 2 + sizeof(uint64_t) * 2 +
-                  sizeof(StageRec) + sizeof(DegridRec));
+                  sizeof(StageHot) + sizeof(StageCold) + sizeof(DegridRec));
         b += sizeof(int) * (size_t)list_capacity(nvis) + sort_temp_bytes(nvis, sort_nbits(tp));
     }
     const size_t mi = (size_t)max_items_for(tp, nvis);

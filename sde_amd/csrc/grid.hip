@@ -116,8 +116,8 @@ template <int DM, bool DEG>
 __global__ __launch_bounds__(kThreads) void k_prep(
     const float2 *__restrict__ vis, const float *__restrict__ wt, const float *__restrict__ u,
     const float *__restrict__ v, const float *__restrict__ w, int64_t n, CoordParams cp, TileParams tp,
-    uint32_t sentinel, uint32_t *__restrict__ keys, uint64_t *__restrict__ vals, StageRec *__restrict__ stage,
-    CoordParams wcp, AzParams az,
+    uint32_t sentinel, uint32_t *__restrict__ keys, uint64_t *__restrict__ vals, StageHot *__restrict__ hot,
+    StageCold *__restrict__ cold, int cold_all, CoordParams wcp, AzParams az,
     double *__restrict__ gwt, double *__restrict__ dblk, unsigned long long *__restrict__ ndrop,
     float2 *__restrict__ vis_out, float *__restrict__ wt_out, int *__restrict__ list, int *__restrict__ nlist,
     int seg_cap) {
@@ -138,19 +138,20 @@ __global__ __launch_bounds__(kThreads) void k_prep(
             const float wti = wt[i];
             const float uf = u[i], vf = v[i], wf = w[i];
             const float2 vv = vis[i];
-            StageRec sr;
+            StageHot sr;
             sr.gpk = 0u;
             sr.fr = 0.0f;
             sr.fi = 0.0f;
-            sr.wcell = kNoCell;
             sr.dpk = 0u;
-            sr.q2 = 0.0f;
-            sr.rot = make_float2(1.0f, 0.0f);
+            StageCold sc;
+            sc.wcell = kNoCell;
+            sc.q2 = 0.0f;
+            sc.rot = make_float2(1.0f, 0.0f);
             uint32_t key = sentinel;
             if (wti > 0.0f) {
-                sr.rot = shift_phase(cp, uf, vf, wf);
+                sc.rot = shift_phase(cp, uf, vf, wf);
                 GridCell c;
-                const bool in = grid_cell(cp, uf, vf, wf, vv, sr.rot, true, c);
+                const bool in = grid_cell(cp, uf, vf, wf, vv, sc.rot, true, c);
                 if (in) {
                     key = (uint32_t)cell_key(tp, c.nuc + cp.suppu - 1, c.nvc - 1);
                     const uint32_t local = key & (uint32_t)(tp.T * tp.T - 1);
@@ -163,23 +164,30 @@ __global__ __launch_bounds__(kThreads) void k_prep(
                 }
                 if (DM > 0) {
                     const RobustCell rc = robust_cell(wcp, uf, vf, wf, az);
-                    sr.wcell = rc.cell;
-                    sr.q2 = rc.q2;
+                    sc.wcell = rc.cell;
+                    sc.q2 = rc.q2;
                     if (rc.half) sr.gpk |= kStageHalf;
                     if (rc.cell == kNoCell) {
                         ndr += 1.0;
+                        if (in) {
+                            sr.gpk |= kStageOwnCell;
+                            key |= kKeyFlag;
+                        }
                     } else if (DM == 2) {
                         if (!(in && rc.uc == c.nuc && rc.vc == c.nvc)) {
                             atomicAdd(&gwt[rc.cell], (double)wti);
                             dsum += (double)wti;
-                            if (in) key |= kKeyFlag;
+                            if (in) {
+                                key |= kKeyFlag;
+                                sr.gpk |= kStageOwnCell;
+                            }
                         }
                     }
                 }
                 if (DEG) {
                     DegridRec d;
                     const int path = degrid_half(cp, tp, uf, vf, wf, in ? (int)(key & ~kKeyFlag) : -1, (uint32_t)i,
-                                                 sr.rot, d);
+                                                 sc.rot, d);
                     if (path == kDegOff) {
                         vis_out[i] = make_float2(0.0f, 0.0f);
                         wt_out[i] = -wti;
@@ -195,7 +203,8 @@ __global__ __launch_bounds__(kThreads) void k_prep(
                 wt_out[i] = wti;
                 vis_out[i] = vv;
             }
-            stage[i] = sr;
+            hot[i] = sr;
+            if (cold_all || (sr.gpk & kStageOwnCell)) cold[i] = sc;
             keys[i] = key;
             vals[i] = (uint64_t)(uint32_t)i | ((uint64_t)__float_as_uint(wti) << 32);
         }
@@ -328,27 +337,30 @@ __global__ void k_grid_items(const int *__restrict__ tile_start, const int *__re
     }
 }
 
-// Degridding halves in sorted order, one thread per sorted slot, copied
-// from the staging record of the slot's visibility (one 32-byte gather per
-// point, coalesced writes).  Used when the records were not built by a
-// gridding run.
-__device__ __forceinline__ DegridRec degrid_rec(const StageRec &sr, uint32_t i) {
+// The degridding half of a sorted point from its staged records.
+__device__ __forceinline__ DegridRec degrid_rec(const StageHot &sr, float2 rot, uint32_t i) {
     DegridRec d;
     d.idx = (sr.gpk & kStageDegTile) ? (i | ((sr.gpk & kStageConj) ? 0x80000000u : 0u)) : kNoDegrid;
     d.pk = sr.dpk;
-    d.rr = sr.rot.x;
-    d.ri = sr.rot.y;
+    d.rr = rot.x;
+    d.ri = rot.y;
     return d;
 }
 
+// Degridding halves in sorted order, one thread per sorted slot, from the
+// staged records of the slot's visibility (one 16-byte gather per point,
+// the rotation only with a shift; coalesced writes).  Used when the
+// records were not built by a gridding run.
 __global__ __launch_bounds__(kThreads) void k_degrid_gather(const uint64_t *__restrict__ svals,
-                                                             const StageRec *__restrict__ stage,
+                                                             const StageHot *__restrict__ hot,
+                                                             const StageCold *__restrict__ cold, int doshift,
                                                              const int *__restrict__ tile_start, TileParams tp,
                                                              DegridRec *__restrict__ rec_d) {
     const int64_t nvalid = tile_start[tp.ntiles];
     for (int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x; p < nvalid; p += (int64_t)gridDim.x * kThreads) {
         const uint32_t i = sval_index(svals[p]);
-        rec_d[p] = degrid_rec(stage[i], i);
+        const StageHot h = hot[i];
+        rec_d[p] = degrid_rec(h, doshift ? cold[i].rot : make_float2(1.0f, 0.0f), i);
     }
 }
 
@@ -412,10 +424,10 @@ __host__ __device__ inline int cf_row_stride(int osamp) {
 // gathers of one wave are covered by the tap loops of the others.
 template <int T, int CHM, int W, bool DEG, int WGT>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) void k_grid_tiles(
-    const uint64_t *__restrict__ svals, const StageRec *__restrict__ stage, const WorkItem *__restrict__ items,
-    const int *__restrict__ nitems, const float *__restrict__ cfu_g, const float *__restrict__ cfv_g, CoordParams cp,
-    TileParams tp, WeightParams wp, DegridRec *__restrict__ rec_d, double *__restrict__ item_sums,
-    float *__restrict__ partial) {
+    const uint64_t *__restrict__ svals, const StageHot *__restrict__ hot, const StageCold *__restrict__ cold,
+    int cold_all, const WorkItem *__restrict__ items, const int *__restrict__ nitems, const float *__restrict__ cfu_g,
+    const float *__restrict__ cfv_g, int same_uv, CoordParams cp, TileParams tp, WeightParams wp,
+    DegridRec *__restrict__ rec_d, double *__restrict__ item_sums, float *__restrict__ partial) {
     constexpr int P = T + 2 * kHalo;
     constexpr int NC = P * P;
     constexpr int NCH = ((CHM & 1) ? 2 : 0) + ((CHM & 2) ? 1 : 0);
@@ -424,9 +436,10 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
     extern __shared__ double smem_d[];
     __shared__ double s_red[2 * W];
     double *S = smem_d;                          // [NCH][NC] FP64 sums
+    // with the same kernel on both axes (same_uv) one set of tables serves both
     double *tsu = smem_d + NCH * NC;             // sum of the taps of each phase
-    double *tsv = tsu + cp.osampu + 1;
-    float *cfl = (float *)(tsv + cp.osampv + 1); // phase tables, one row per tap
+    double *tsv = same_uv ? tsu : tsu + cp.osampu + 1;
+    float *cfl = (float *)(tsu + (cp.osampu + 1) + (same_uv ? 0 : cp.osampv + 1));  // one row per tap
     const int item = blockIdx.x;
     if (item >= *nitems) {
         if (threadIdx.x == 0) {
@@ -441,15 +454,16 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
     const int ntapu = 2 * cp.suppu + 1, ntapv = 2 * cp.suppv + 1;
     const int rsu = cf_row_stride(cp.osampu), rsv = cf_row_stride(cp.osampv);
     const int baseu = cp.osampu - poffu, basev = cp.osampv - poffv;
-    float *cflu = cfl, *cflv = cfl + ntapu * rsu;
+    float *cflu = cfl, *cflv = same_uv ? cfl : cfl + ntapu * rsu;
     for (int k = threadIdx.x; k < ntapu * (cp.osampu + 1); k += 64 * W) {
         const int o = k / (cp.osampu + 1), ph = k - o * (cp.osampu + 1);
         cflu[o * rsu + ph] = cfu_g[baseu + ph + cp.osampu * o];
     }
-    for (int k = threadIdx.x; k < ntapv * (cp.osampv + 1); k += 64 * W) {
-        const int o = k / (cp.osampv + 1), ph = k - o * (cp.osampv + 1);
-        cflv[o * rsv + ph] = cfv_g[basev + ph + cp.osampv * o];
-    }
+    if (!same_uv)
+        for (int k = threadIdx.x; k < ntapv * (cp.osampv + 1); k += 64 * W) {
+            const int o = k / (cp.osampv + 1), ph = k - o * (cp.osampv + 1);
+            cflv[o * rsv + ph] = cfv_g[basev + ph + cp.osampv * o];
+        }
     for (int k = threadIdx.x; k < NCH * NC; k += 64 * W) S[k] = 0.0;
     __syncthreads();
     // the normalisation's sum over taps of one phase, in the reference's order
@@ -458,11 +472,12 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
         for (int o = 0; o < ntapu; o++) t += (double)cflu[o * rsu + k];
         tsu[k] = t;
     }
-    for (int k = threadIdx.x; k <= cp.osampv; k += 64 * W) {
-        double t = 0.0;
-        for (int o = 0; o < ntapv; o++) t += (double)cflv[o * rsv + k];
-        tsv[k] = t;
-    }
+    if (!same_uv)
+        for (int k = threadIdx.x; k <= cp.osampv; k += 64 * W) {
+            double t = 0.0;
+            for (int o = 0; o < ntapv; o++) t += (double)cflv[o * rsv + k];
+            tsv[k] = t;
+        }
     __syncthreads();
 
     const int lane = threadIdx.x & 63;
@@ -473,6 +488,11 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
     const float *rowu = cflu + (active ? ou * rsu : 0);
     const float *rowv = cflv + (active ? ov * rsv : 0);
 
+    // a point without a cold record has its density cell at its gridding cell:
+    // row (tv*T + lv), column (tu*T + lu - SUPPU) of the NU x NV density grid
+    const int tcol = (wi.tile % tp.ntu) * T - cp.suppu, trow = (wi.tile / tp.ntu) * T;
+    const bool shifted = cp.doshift != 0;
+
     const int len = wi.end - wi.begin;
     const int r0 = wi.begin + (int)(((int64_t)len * wave) / W);
     const int r1 = wi.begin + (int)(((int64_t)len * (wave + 1)) / W);
@@ -481,20 +501,31 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
     int cur = -1;
     if (r0 < r1) {  // wave-uniform; lanes past the end load the wave's last slot
         uint64_t v0 = svals[min(r0 + lane, r1 - 1)];
-        StageRec st0 = stage[sval_index(v0)];
+        StageHot st0 = hot[sval_index(v0)];
         uint64_t v1 = svals[min(r0 + 64 + lane, r1 - 1)];
         for (int b = r0; b < r1; b += 64) {
-            const StageRec st1 = stage[sval_index(v1)];
+            const StageHot st1 = hot[sval_index(v1)];
             const uint64_t v2 = svals[min(b + 128 + lane, r1 - 1)];
 
             // this lane's record of the batch: the staged gridding record
             // with its final weight
             const int p = b + lane;
             const bool valid = p < r1;
+            const uint32_t i0 = sval_index(v0);
             float wti = sval_wt(v0);
-            if (WGT)
-                wti = robust_apply(wti, st0.wcell, (st0.gpk & kStageHalf) != 0, st0.q2, WGT == 2, wp.gwt,
-                                   wp.wsums);
+            // the cold record where this run or this point needs it (rare in the common case)
+            StageCold sc;
+            sc.wcell = kNoCell;
+            sc.q2 = 0.0f;
+            sc.rot = make_float2(1.0f, 0.0f);
+            const bool need_cold = (cold_all && (WGT || (DEG && shifted))) || (WGT && (st0.gpk & kStageOwnCell));
+            if (need_cold) sc = cold[i0];
+            if (WGT) {
+                uint32_t wc = sc.wcell;
+                if (!cold_all && !(st0.gpk & kStageOwnCell))
+                    wc = (uint32_t)((trow + (int)((st0.gpk >> 8) & 0x1f)) * cp.nu + tcol + (int)(st0.gpk & 0x1f));
+                wti = robust_apply(wti, wc, (st0.gpk & kStageHalf) != 0, sc.q2, WGT == 2, wp.gwt, wp.wsums);
+            }
             GridRec mine;
             mine.pk = st0.gpk & ~kStageFlags;
             mine.wt = 0.0f;
@@ -508,7 +539,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
                 mine.re = PSF_ONLY ? wti : (float)((double)wti * (double)st0.fr);
                 mine.im = PSF_ONLY ? 0.0f : (float)((double)wti * (double)st0.fi);
             }
-            if (DEG && valid) rec_d[p] = degrid_rec(st0, sval_index(v0));
+            if (DEG && valid) rec_d[p] = degrid_rec(st0, sc.rot, i0);
 
             // grid the batch; the table entries of the next record are
             // read while this one accumulates (two records per turn, so the
@@ -775,7 +806,8 @@ void launch_prep(hipStream_t s, const float2 *vis, const float *wt, const float 
     int *list = deg ? deg->list : nullptr, *nlist = deg ? deg->nlist : nullptr;
 #define SDE_PREP(DM, DG)                                                                                         \
     hipLaunchKernelGGL((k_prep<DM, DG>), dim3(nb), dim3(kThreads), 0, s, vis, wt, u, v, w, n, cp, tp, sentinel, \
-                       b.keys, b.vals, b.stage, wcp, az, gwt, dblk, ndrop, vo, wo, list, nlist, list_seg_cap(n))
+                       b.keys, b.vals, b.hot, b.cold, b.cold_all, wcp, az, gwt, dblk, ndrop, vo, wo, list, nlist,    \
+                       list_seg_cap(n))
     const int dm = dens ? (dens->gwt ? 2 : 1) : 0;
     if (deg) {
         if (dm == 2) SDE_PREP(2, true);
@@ -819,8 +851,8 @@ void launch_density_runs(hipStream_t s, const SortBufs &b, int64_t n, const Tile
 
 void launch_degrid_gather(hipStream_t s, const SortBufs &b, int64_t n, const CoordParams &cp, const TileParams &tp,
                           DegridRec *rec_d) {
-    hipLaunchKernelGGL(k_degrid_gather, dim3(gather_blocks(n)), dim3(kThreads), 0, s, b.svals, b.stage,
-                       b.tile_start, tp, rec_d);
+    hipLaunchKernelGGL(k_degrid_gather, dim3(gather_blocks(n)), dim3(kThreads), 0, s, b.svals, b.hot, b.cold,
+                       cp.doshift, b.tile_start, tp, rec_d);
 }
 
 size_t grid_partial_floats(const TileParams &tp, int chmask, int max_items) {
@@ -835,11 +867,14 @@ static void launch_tiles_t(hipStream_t s, const SortBufs &b, const TileParams &t
                            double *item_sums, float *partial) {
     constexpr int P = T + 2 * kHalo;
     constexpr int NCH = ((CHM & 1) ? 2 : 0) + ((CHM & 2) ? 1 : 0);
+    const int same_uv = cfu == cfv && cp.suppu == cp.suppv && cp.osampu == cp.osampv;
     const size_t tables = (size_t)(2 * cp.suppu + 1) * cf_row_stride(cp.osampu) +
-                          (size_t)(2 * cp.suppv + 1) * cf_row_stride(cp.osampv);
-    const size_t lds = sizeof(double) * ((size_t)NCH * P * P + cp.osampu + 1 + cp.osampv + 1) + sizeof(float) * tables;
+                          (same_uv ? 0 : (size_t)(2 * cp.suppv + 1) * cf_row_stride(cp.osampv));
+    const size_t sums = (size_t)cp.osampu + 1 + (same_uv ? 0 : (size_t)cp.osampv + 1);
+    const size_t lds = sizeof(double) * ((size_t)NCH * P * P + sums) + sizeof(float) * tables;
     hipLaunchKernelGGL((k_grid_tiles<T, CHM, W, DEG, WGT>), dim3(b.max_items), dim3(64 * W), lds, s, b.svals,
-                       b.stage, b.items, b.nitems, cfu, cfv, cp, tp, wp, rec_d, item_sums, partial);
+                       b.hot, b.cold, b.cold_all, b.items, b.nitems, cfu, cfv, same_uv, cp, tp, wp, rec_d, item_sums,
+                       partial);
 }
 
 template <int T, int W>

@@ -64,32 +64,42 @@ struct __align__(16) WorkItem {
     int tile, begin, end, pad;
 };
 
-// Per-visibility staging record, written in the caller's order by the prep
-// pass and gathered in sorted order (one 32-byte read per point): every
-// part of the gridding and degridding records that does not depend on the
-// density weights, so the sorted passes evaluate no coordinate step.
+// Per-visibility staging records, written in the caller's order by the prep
+// pass and gathered in sorted order: every part of the gridding and
+// degridding records that does not depend on the density weights, so the
+// sorted passes evaluate no coordinate step.  The hot record (16 bytes) is
+// what every run reads:
 //   gpk:   the gridding record's pk (local u | local v << 8 | u phase << 16 |
-//          v phase << 24), with kStageFlags in bits 5-7 (local u < 32)
+//          v phase << 24), with kStageFlags in bits 5-7 (local u < 32) and 13
+//          (local v < 32)
 //   fr, fi: GRDH2D's FVIS, the visibility after the shift rotation and the
 //          conjugation (grd/grdh2d.f:124-140)
-//   wcell: the point's density cell (0-based, row-major), kNoCell when off
-//          the density grid or not weighting
 //   dpk:   the degridding record's pk (valid with kStageDegTile)
+// The cold record (16 bytes) is read only where it is needed -- a point
+// flagged kStageOwnCell, or a run that needs it for every point (density
+// cell scale not the gridding scale, AZ-NORM, a shift) -- and written only
+// then:
+//   wcell: the point's density cell (0-based, row-major), kNoCell when off
+//          the density grid
 //   q2:    the azimuthal robustness factor (5*10**(-ROB))**2 of AZ-NORM
 //   rot:   the phase rotation of the shift, ((float)cos, (float)sin) of
 //          shift_phase -- (1, 0) without a shift
-struct __align__(32) StageRec {
+// Without the cold record a point's density cell is its gridding cell.
+struct __align__(16) StageHot {
     uint32_t gpk;
     float fr, fi;
-    uint32_t wcell;
     uint32_t dpk;
+};
+struct __align__(16) StageCold {
+    uint32_t wcell;
     float q2;
     float2 rot;
 };
 constexpr uint32_t kStageConj = 1u << 5;     // degridding: conjugate (u > 0 half)
 constexpr uint32_t kStageDegTile = 1u << 6;  // degridding: on the tile path
 constexpr uint32_t kStageHalf = 1u << 7;     // density weight halved (u-origin column)
-constexpr uint32_t kStageFlags = kStageConj | kStageDegTile | kStageHalf;
+constexpr uint32_t kStageOwnCell = 1u << 13; // density cell not the gridding cell: read the cold record
+constexpr uint32_t kStageFlags = kStageConj | kStageDegTile | kStageHalf | kStageOwnCell;
 constexpr uint32_t kNoCell = 0xffffffffu;
 
 // ---- cell sort shared by gridding and degridding (grid.hip, sort.hip) ----
@@ -108,7 +118,9 @@ struct SortBufs {
     uint64_t *vals = nullptr;    // per visibility, caller's order
     uint32_t *skeys = nullptr;   // sorted keys
     uint64_t *svals = nullptr;   // sorted values
-    StageRec *stage = nullptr;   // per visibility, caller's order
+    StageHot *hot = nullptr;     // per visibility, caller's order
+    StageCold *cold = nullptr;   // per visibility, caller's order (written where needed)
+    int cold_all = 0;            // every point has a cold record (else flagged points only)
     void *temp = nullptr;        // radix-sort scratch
     size_t temp_bytes = 0;
     int *tile_start = nullptr, *item_start = nullptr, *nitems = nullptr;
