@@ -878,6 +878,8 @@ void fftx1d_(float *xarray, float *xwork, float *xwork2, int *nx, int *dir) {
 struct sde_plan {
     sde_imaging_cfg cfg;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;  // independent work beside the main sequence (joined into it)
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int fnu = 0, nu = 0, nv = 0, uorigin = 1, vorigin = 1, ccflen = 0, supp = 0, osamp = 0, ncf = 0;
     float udelt = 0, vdelt = 0, uscale = 0, vscale = 0;
     CoordParams cp, wcp;
@@ -928,15 +930,15 @@ hipEvent_t pool_event(sde_plan *p) {
     return e;
 }
 
-void plan_mark_begin(sde_plan *p, const char *name) {
+void plan_mark_begin(sde_plan *p, const char *name, hipStream_t s = nullptr) {
     if (!p->profile) return;
     hipEvent_t a = pool_event(p), b = pool_event(p);
-    (void)hipEventRecord(a, p->stream);
+    (void)hipEventRecord(a, s ? s : p->stream);
     p->pending.push_back({name, {a, b}});
 }
-void plan_mark_end(sde_plan *p) {
+void plan_mark_end(sde_plan *p, hipStream_t s = nullptr) {
     if (!p->profile || p->pending.empty()) return;
-    (void)hipEventRecord(p->pending.back().second.second, p->stream);
+    (void)hipEventRecord(p->pending.back().second.second, s ? s : p->stream);
 }
 
 void plan_collect(sde_plan *p) {
@@ -1058,18 +1060,23 @@ int plan_vismodel(sde_plan *p, const float *img) {
         plan_mark_end(p);
         p->filled = true;
     }
+    // the near-axis list (disjoint points) runs on the side stream beside the tile path
+    SDE_CHECK(hipEventRecord(p->ev_fork, s));
+    SDE_CHECK(hipStreamWaitEvent(p->side, p->ev_fork, 0));
+    plan_mark_begin(p, "degrid_axis", p->side);
+    launch_degrid_list(p->side, p->d_mgrid, p->d_u, p->d_v, p->d_w, p->d_wt, p->d_vis, p->nvis, p->cp, p->d_cf,
+                       p->d_cf, p->d_mvis, p->d_mwt, p->d_cnt + 1, p->d_list, p->d_nlist);
+    plan_mark_end(p, p->side);
+    SDE_CHECK(hipEventRecord(p->ev_join, p->side));
     plan_mark_begin(p, "degrid");
-    if (launch_degrid_tiles(s, p->d_recd, p->sb.items, p->sb.nitems, p->sb.max_items, p->tp, p->d_mgrid, p->d_cf,
-                            p->d_cf, p->cp, p->d_mvis, p->d_mwt, p->d_cnt + 1)) {
-        plan_mark_end(p);
+    const int rc = launch_degrid_tiles(s, p->d_recd, p->sb.items, p->sb.nitems, p->sb.max_items, p->tp, p->d_mgrid,
+                                       p->d_cf, p->d_cf, p->cp, p->d_mvis, p->d_mwt, p->d_cnt + 1);
+    plan_mark_end(p);
+    SDE_CHECK(hipStreamWaitEvent(s, p->ev_join, 0));
+    if (rc) {
         set_plan_error("unsupported degridding tile T=%d", p->tp.T);
         return -1;
     }
-    plan_mark_end(p);
-    plan_mark_begin(p, "degrid_axis");
-    launch_degrid_list(s, p->d_mgrid, p->d_u, p->d_v, p->d_w, p->d_wt, p->d_vis, p->nvis, p->cp, p->d_cf, p->d_cf,
-                       p->d_mvis, p->d_mwt, p->d_cnt + 1, p->d_list, p->d_nlist);
-    plan_mark_end(p);
     SDE_CHECK(hipGetLastError());
     return 0;
 }
@@ -1128,7 +1135,10 @@ int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **out) {
         return -1;
     }
     p->tp = make_tp(p->nu, p->nv, T, cfg->chunk > 0 ? cfg->chunk : 2048, cfg->waves ? cfg->waves : 4);
-    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming) != hipSuccess) {
         set_plan_error("sde_plan_create: stream creation failed");
         delete p;
         return -1;
@@ -1179,6 +1189,10 @@ int sde_plan_destroy(sde_plan *p) {
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &kv : p->tw) (void)hipFree(kv.second);
+    if (p->side) (void)hipStreamSynchronize(p->side);
+    if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+    if (p->ev_join) (void)hipEventDestroy(p->ev_join);
+    if (p->side) (void)hipStreamDestroy(p->side);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
     return 0;
