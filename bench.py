@@ -268,6 +268,46 @@ def bench_served(args):
             "identical_results": bool(same), "plans": pool.created}
 
 
+def bench_major_cycle(args):
+    """Major cycles on the headline field: after one full uvmap, each cycle
+    predicts a model image, subtracts it and images the residual on the
+    standing cell sort and density weights (the repeated VISTOIMG of the
+    CLEAN and mosaic tasks)."""
+    from sde_amd import imaging, synth
+    p = synth.large_field_params(nkeep=args.nvis)
+    nx, cell = p["nx"], p["cellsize"]
+    plan = imaging.Plan(nx, nx, cell, "SF", weighting=1, fov=1.0, rmode="NORM", robust=(0.0, 0.0, 0.0))
+    plan.synth(p["enu"], p["lat"], p["dec"], p["ha0"], p["dha"], p["ntime"], p["freqs"], p["flux"], p["l"], p["m"],
+               p["sigma"], p["seed"], p["nkeep"])
+    nvis = plan.nvis
+    plan.uvmap()
+    dirty, _ = plan.images()
+    model = np.where(dirty > 0.5 * dirty.max(), 0.1 * dirty, 0).astype(np.float32)
+    plan.set_image(model)
+    src = plan.device_image()
+    for _ in range(args.warmup):
+        plan.vismodel_device(src)
+        plan.subtract_model()
+        plan.uvmap_residual()
+    plan.sync()
+    plan.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.vismodel_device(src)
+        plan.subtract_model()
+        plan.uvmap_residual()
+    plan.sync()
+    t = (time.perf_counter() - t0) / args.steps
+    kernels = {k: round(v[0] / v[1], 3) for k, v in plan.timings().items()}
+    plan.close()
+    return {"metric": "major-cycle visibilities/sec (predict + subtract + residual image, standing sort)",
+            "value": round(2.0 * nvis / t, 1), "unit": "visibilities/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(1e3 * t, 3), "higher_is_better": True, "dtype": "f32",
+            "data": "synthetic (seeded, on the device)",
+            "config": {"workload": "major cycle on the plain single field large: 4096^2, 1.0e8 vis, robust NORM R=0",
+                       "nvis": nvis}, "kernels_ms": kernels}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -283,7 +323,8 @@ def main():
                     help="robust NORM R=0 (the benchmark's configuration) or natural weights")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy first (HBM counter calibration under rocprofv3)")
-    ap.add_argument("--workload", choices=("large_field", "mosaic", "predict", "served"), default="large_field",
+    ap.add_argument("--workload", choices=("large_field", "mosaic", "predict", "served", "major_cycle"),
+                    default="large_field",
                     help="large_field: the headline job (default); the others are the batched and served forms")
     ap.add_argument("--jobs", type=int, default=100, help="served: number of requests")
     ap.add_argument("--rate", type=float, default=400.0, help="served: request rate (1/s)")
@@ -299,7 +340,8 @@ def main():
     if sde_amd.device_count() < 1:
         raise SystemExit("no GPU visible")
     if args.workload != "large_field":
-        fn = {"mosaic": bench_mosaic, "predict": bench_predict, "served": bench_served}[args.workload]
+        fn = {"mosaic": bench_mosaic, "predict": bench_predict, "served": bench_served,
+              "major_cycle": bench_major_cycle}[args.workload]
         print(json.dumps(fn(args)))
         return
 

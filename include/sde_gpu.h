@@ -164,6 +164,14 @@ int sde_plan_set_vis(sde_plan *plan, int64_t nvis, const float *vis, const float
 /* Weighting, PSF and dirty image (both divided by the PSF peak), all on the
  * device (task/uvmap.f:276-282,428-482). */
 int sde_plan_uvmap(sde_plan *plan);
+/* Major cycle (the CLEAN and mosaic tasks' repeated VISTOIMG/IMGTOVIS,
+ * e.g. task/mosclean.f): sde_plan_subtract_model replaces the resident
+ * visibilities by VIS - MODEL wherever the last sde_plan_vismodel predicted
+ * them (model WT > 0); sde_plan_uvmap_residual images them on the standing
+ * cell sort and density weights of the last sde_plan_uvmap (uvw and weights
+ * unchanged), restaging only the visibility values. */
+int sde_plan_subtract_model(sde_plan *plan);
+int sde_plan_uvmap_residual(sde_plan *plan);
 /* Model prediction from an image (host NX*NY floats, or NULL for the plan's
  * dirty image): correct, FFT, degrid (task/vismodel.f:69-78). */
 int sde_plan_vismodel(sde_plan *plan, const float *image);

@@ -897,6 +897,10 @@ struct sde_plan {
     // are written for the resident visibilities
     bool filled = false;
     bool weighted = false;  // the density grid and weighting sums of the last uvmap are resident
+    // the cell sort, work items, staged coordinate parts and density weights of
+    // the last uvmap stand for the resident uvw and weights (residual imaging)
+    bool sorted_ok = false;
+    bool predicted = false;  // model VIS/WT of the resident visibilities are complete
     DegridRec *d_recd = nullptr;
     double *d_wsq = nullptr;
     int *d_list = nullptr, *d_nlist = nullptr;
@@ -977,6 +981,8 @@ int plan_alloc(sde_plan *p, T **ptr, size_t count) {
 int plan_reserve_vis(sde_plan *p, int64_t n) {
     p->filled = false;
     p->weighted = false;
+    p->sorted_ok = false;
+    p->predicted = false;
     if (n <= p->cap) return 0;
     const size_t c = (size_t)n;
     if (plan_alloc(p, &p->d_vis, c) || plan_alloc(p, &p->d_wt, c) || plan_alloc(p, &p->d_u, c) || plan_alloc(p, &p->d_v, c) ||
@@ -1016,6 +1022,7 @@ int plan_reserve_items(sde_plan *p) {
 // Cell sort of the resident visibilities; the prep pass also writes the
 // per-point degridding outputs into the model arrays.
 int plan_sort(sde_plan *p, const DensityOut *dens, bool bounds = true) {
+    p->sorted_ok = false;
     auto mb = [p](const char *nm) { plan_mark_begin(p, nm); };
     auto me = [p]() { plan_mark_end(p); };
     DegridOut deg;
@@ -1077,6 +1084,116 @@ int plan_vismodel(sde_plan *p, const float *img) {
         set_plan_error("unsupported degridding tile T=%d", p->tp.T);
         return -1;
     }
+    SDE_CHECK(hipGetLastError());
+    p->predicted = true;
+    return 0;
+}
+
+int plan_image(sde_plan *p, const WeightParams &wp);
+
+// uvmap; with reuse (residual imaging) the cell sort, work items, staged
+// coordinate parts and density weights of the last uvmap stand and only the
+// visibility values are restaged.
+int plan_uvmap(sde_plan *p, bool reuse) {
+    if (!p) return -1;
+    if (p->nvis <= 0) {
+        set_plan_error("sde_plan_uvmap: no visibilities");
+        return -1;
+    }
+    hipStream_t s = p->stream;
+    const int64_t n = p->nvis;
+    const size_t ng = (size_t)p->nu * p->nv;
+    WeightParams wp;
+    const bool weighting = p->cfg.weighting != 0;
+    if (reuse) {
+        if (!p->sorted_ok) {
+            set_plan_error("sde_plan_uvmap_residual: no standing cell sort (run sde_plan_uvmap on these visibilities)");
+            return -1;
+        }
+        plan_mark_begin(p, "restage");
+        launch_restage(s, p->d_vis, p->d_wt, p->d_u, p->d_v, p->d_w, n, p->cp, p->sb.hot);
+        plan_mark_end(p);
+        if (weighting)
+            wp = make_weight_params(p->wcp, p->wmode == 3, p->cfg.robust, 4.0f * atanf(1.0f), p->d_gwt, p->d_wsums);
+        return plan_image(p, wp);
+    }
+    SDE_CHECK(hipMemsetAsync(p->d_cnt, 0, sizeof(unsigned long long) * 4, s));
+    // every imaging job sorts its visibilities afresh (the sort is part of the job)
+    // the density grid comes from the sorted runs when its cell is the
+    // gridding cell (WTFOV = 1: the same scales); else from atomics
+    const bool runs = weighting && p->wcp.uscale == p->cp.uscale && p->wcp.vscale == p->cp.vscale;
+    DensityOut dens;
+    if (weighting) {
+        SDE_CHECK(hipMemsetAsync(p->d_gwt, 0, sizeof(double) * ng, s));
+        dens.wcp = p->wcp;
+        dens.az = make_az_params(p->wmode == 3, p->cfg.robust, 4.0f * atanf(1.0f));
+        dens.gwt = runs ? p->d_gwt : nullptr;
+        dens.blk = p->d_wblk;
+        dens.ndrop = p->d_cnt;
+    }
+    // with the density from the sorted runs, that pass also writes the tile ranges
+    if (plan_sort(p, weighting ? &dens : nullptr, !runs)) return -1;
+    if (weighting) {
+        int nwblk;
+        if (runs) {
+            const int np = prep_blocks(n);
+            plan_mark_begin(p, "wt_density");
+            launch_density_runs(s, p->sb, n, p->tp, p->cp, p->d_gwt, p->d_wblk + np);
+            launch_items(s, p->sb, p->tp);
+            plan_mark_end(p);
+            nwblk = np + run_blocks(n);
+        } else {
+            nwblk = wt_blocks(n);
+            plan_mark_begin(p, "wt_density");
+            launch_wt_density(s, p->d_wt, p->d_u, p->d_v, p->d_w, n, p->wcp, p->d_gwt, p->d_wblk, nwblk);
+            plan_mark_end(p);
+        }
+        float rob = p->wmode == 3 ? (p->cfg.robust[0] + p->cfg.robust[1]) / 2.0f : p->cfg.robust[0];
+        const float t = 5.0f * powf(10.0f, -rob);
+        plan_mark_begin(p, "wt_finish");
+        launch_wt_finish(s, p->d_gwt, (int)ng, p->d_wblk, nwblk, p->wmode, t * t, rob * rob,
+                         2.0f * (p->cfg.ds * p->cfg.ds), p->d_wsq, p->d_wsums);
+        plan_mark_end(p);
+        wp = make_weight_params(p->wcp, p->wmode == 3, p->cfg.robust, 4.0f * atanf(1.0f), p->d_gwt, p->d_wsums);
+    }
+    p->weighted = weighting;
+    p->sorted_ok = true;
+    return plan_image(p, wp);
+}
+
+// Gridding of PSF and dirty planes on the standing sort, FFTs, correction
+// and PSF-peak normalisation (task/uvmap.f:424-491).
+int plan_image(sde_plan *p, const WeightParams &wp) {
+    hipStream_t s = p->stream;
+    const size_t ng = (size_t)p->nu * p->nv;
+    float2 *gd = p->d_grids, *gp = p->d_grids + ng;
+    auto mb = [p](const char *nm) { plan_mark_begin(p, nm); };
+    auto me = [p]() { plan_mark_end(p); };
+    // the gridding run also writes the degridding halves of a prediction on these
+    // visibilities (model VIS/WT are complete after sde_plan_vismodel)
+    if (run_grid(s, p->cp, p->tp, 3, true, wp, p->d_cf, p->d_cf, p->sb, p->d_recd, p->d_partial, p->d_isums,
+                 p->d_sums, gd, gp, mb, me))
+        return -1;
+    p->filled = true;
+    float2 *twnv = p->tw[p->nv], *twfnu = p->tw[p->fnu];
+    plan_mark_begin(p, "fft_cols");
+    // out of place: the gridded planes stay readable (sde_plan_get_grids)
+    launch_fft_cols(s, p->d_grids, p->d_gwork, p->nu, p->nv, 2, ng, -1, 1.0f, twnv);
+    plan_mark_end(p);
+    const float2 *cd = p->d_gwork, *cpsf = p->d_gwork + ng;
+    SDE_CHECK(hipMemsetAsync(p->d_peak, 0x80, sizeof(int), s));
+    const float scale = 1.0f / ((float)p->fnu * (float)p->nv);
+    plan_mark_begin(p, "fft_rows_psf");
+    launch_fft_rows_c2r(s, cpsf, p->nu, p->nv, p->fnu, p->cfg.nx, p->cfg.ny, twfnu, p->d_ccf, p->ccflen, 1, scale,
+                        p->d_psf, p->d_peak, nullptr);
+    plan_mark_end(p);
+    plan_mark_begin(p, "fft_rows_dirty");
+    launch_fft_rows_c2r(s, cd, p->nu, p->nv, p->fnu, p->cfg.nx, p->cfg.ny, twfnu, p->d_ccf, p->ccflen, 1, scale,
+                        p->d_dirty, nullptr, p->d_peak);
+    plan_mark_end(p);
+    plan_mark_begin(p, "psf_scale");
+    launch_scale_by_peak(s, p->d_psf, (size_t)p->cfg.nx * p->cfg.ny, p->d_peak);
+    plan_mark_end(p);
     SDE_CHECK(hipGetLastError());
     return 0;
 }
@@ -1221,84 +1338,18 @@ int sde_plan_set_vis(sde_plan *p, int64_t nvis, const float *vis, const float *w
     return 0;
 }
 
-int sde_plan_uvmap(sde_plan *p) {
+int sde_plan_uvmap(sde_plan *p) { return plan_uvmap(p, false); }
+
+int sde_plan_uvmap_residual(sde_plan *p) { return plan_uvmap(p, true); }
+
+int sde_plan_subtract_model(sde_plan *p) {
     if (!p) return -1;
-    if (p->nvis <= 0) {
-        set_plan_error("sde_plan_uvmap: no visibilities");
+    if (!p->predicted) {
+        set_plan_error("sde_plan_subtract_model: no prediction on these visibilities (run sde_plan_vismodel)");
         return -1;
     }
-    hipStream_t s = p->stream;
-    const int64_t n = p->nvis;
-    const size_t ng = (size_t)p->nu * p->nv;
-    SDE_CHECK(hipMemsetAsync(p->d_cnt, 0, sizeof(unsigned long long) * 4, s));
-    // every imaging job sorts its visibilities afresh (the sort is part of the job)
-    WeightParams wp;
-    const bool weighting = p->cfg.weighting != 0;
-    // the density grid comes from the sorted runs when its cell is the
-    // gridding cell (WTFOV = 1: the same scales); else from atomics
-    const bool runs = weighting && p->wcp.uscale == p->cp.uscale && p->wcp.vscale == p->cp.vscale;
-    DensityOut dens;
-    if (weighting) {
-        SDE_CHECK(hipMemsetAsync(p->d_gwt, 0, sizeof(double) * ng, s));
-        dens.wcp = p->wcp;
-        dens.az = make_az_params(p->wmode == 3, p->cfg.robust, 4.0f * atanf(1.0f));
-        dens.gwt = runs ? p->d_gwt : nullptr;
-        dens.blk = p->d_wblk;
-        dens.ndrop = p->d_cnt;
-    }
-    // with the density from the sorted runs, that pass also writes the tile ranges
-    if (plan_sort(p, weighting ? &dens : nullptr, !runs)) return -1;
-    if (weighting) {
-        int nwblk;
-        if (runs) {
-            const int np = prep_blocks(n);
-            plan_mark_begin(p, "wt_density");
-            launch_density_runs(s, p->sb, n, p->tp, p->cp, p->d_gwt, p->d_wblk + np);
-            launch_items(s, p->sb, p->tp);
-            plan_mark_end(p);
-            nwblk = np + run_blocks(n);
-        } else {
-            nwblk = wt_blocks(n);
-            plan_mark_begin(p, "wt_density");
-            launch_wt_density(s, p->d_wt, p->d_u, p->d_v, p->d_w, n, p->wcp, p->d_gwt, p->d_wblk, nwblk);
-            plan_mark_end(p);
-        }
-        float rob = p->wmode == 3 ? (p->cfg.robust[0] + p->cfg.robust[1]) / 2.0f : p->cfg.robust[0];
-        const float t = 5.0f * powf(10.0f, -rob);
-        plan_mark_begin(p, "wt_finish");
-        launch_wt_finish(s, p->d_gwt, (int)ng, p->d_wblk, nwblk, p->wmode, t * t, rob * rob,
-                         2.0f * (p->cfg.ds * p->cfg.ds), p->d_wsq, p->d_wsums);
-        plan_mark_end(p);
-        wp = make_weight_params(p->wcp, p->wmode == 3, p->cfg.robust, 4.0f * atanf(1.0f), p->d_gwt, p->d_wsums);
-    }
-    float2 *gd = p->d_grids, *gp = p->d_grids + ng;
-    auto mb = [p](const char *nm) { plan_mark_begin(p, nm); };
-    auto me = [p]() { plan_mark_end(p); };
-    // the gridding run also writes the degridding halves of a prediction on these
-    // visibilities (model VIS/WT are complete after sde_plan_vismodel)
-    if (run_grid(s, p->cp, p->tp, 3, true, wp, p->d_cf, p->d_cf, p->sb, p->d_recd, p->d_partial, p->d_isums,
-                 p->d_sums, gd, gp, mb, me))
-        return -1;
-    p->filled = true;
-    p->weighted = weighting;
-    float2 *twnv = p->tw[p->nv], *twfnu = p->tw[p->fnu];
-    plan_mark_begin(p, "fft_cols");
-    // out of place: the gridded planes stay readable (sde_plan_get_grids)
-    launch_fft_cols(s, p->d_grids, p->d_gwork, p->nu, p->nv, 2, ng, -1, 1.0f, twnv);
-    plan_mark_end(p);
-    const float2 *cd = p->d_gwork, *cpsf = p->d_gwork + ng;
-    SDE_CHECK(hipMemsetAsync(p->d_peak, 0x80, sizeof(int), s));
-    const float scale = 1.0f / ((float)p->fnu * (float)p->nv);
-    plan_mark_begin(p, "fft_rows_psf");
-    launch_fft_rows_c2r(s, cpsf, p->nu, p->nv, p->fnu, p->cfg.nx, p->cfg.ny, twfnu, p->d_ccf, p->ccflen, 1, scale,
-                        p->d_psf, p->d_peak, nullptr);
-    plan_mark_end(p);
-    plan_mark_begin(p, "fft_rows_dirty");
-    launch_fft_rows_c2r(s, cd, p->nu, p->nv, p->fnu, p->cfg.nx, p->cfg.ny, twfnu, p->d_ccf, p->ccflen, 1, scale,
-                        p->d_dirty, nullptr, p->d_peak);
-    plan_mark_end(p);
-    plan_mark_begin(p, "psf_scale");
-    launch_scale_by_peak(s, p->d_psf, (size_t)p->cfg.nx * p->cfg.ny, p->d_peak);
+    plan_mark_begin(p, "subtract");
+    launch_subtract(p->stream, p->d_vis, p->d_mvis, p->d_mwt, p->nvis);
     plan_mark_end(p);
     SDE_CHECK(hipGetLastError());
     return 0;
@@ -1344,6 +1395,8 @@ int sde_plan_set_cell(sde_plan *p, float cellsize) {
     plan_set_geometry(p, -cellsize / 3600.0f, cellsize / 3600.0f);
     p->filled = false;
     p->weighted = false;
+    p->sorted_ok = false;
+    p->predicted = false;
     return 0;
 }
 

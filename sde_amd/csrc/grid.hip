@@ -223,6 +223,37 @@ __global__ __launch_bounds__(kThreads) void k_prep(
     }
 }
 
+// New visibility values on a standing cell sort (a residual after a
+// model was subtracted; uvw and weights unchanged): the rotated and
+// conjugated FVIS of every gridded point, as k_prep forms it, patched into
+// its hot staging record.
+__global__ __launch_bounds__(kThreads) void k_restage(const float2 *__restrict__ vis, const float *__restrict__ wt,
+                                                      const float *__restrict__ u, const float *__restrict__ v,
+                                                      const float *__restrict__ w, int64_t n, CoordParams cp,
+                                                      StageHot *__restrict__ hot) {
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        if (!(wt[i] > 0.0f)) continue;
+        const float uf = u[i], vf = v[i], wf = w[i];
+        GridCell c;
+        if (!grid_cell(cp, uf, vf, wf, vis[i], shift_phase(cp, uf, vf, wf), true, c)) continue;
+        StageHot h = hot[i];
+        h.fr = c.fr;
+        h.fi = c.fi;
+        hot[i] = h;
+    }
+}
+
+// Residual visibilities: VIS - MODEL where the model was predicted (model WT > 0).
+__global__ __launch_bounds__(kThreads) void k_subtract(float2 *__restrict__ vis, const float2 *__restrict__ mvis,
+                                                       const float *__restrict__ mwt, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        if (mwt[i] > 0.0f) {
+            const float2 a = vis[i], b = mvis[i];
+            vis[i] = make_float2(a.x - b.x, a.y - b.y);
+        }
+    }
+}
+
 __device__ __forceinline__ int tile_of_key(uint32_t k, uint32_t nkeys, int tt, int ntiles) {
     k &= ~kKeyFlag;
     return k >= nkeys ? ntiles : (int)(k / (uint32_t)tt);
@@ -847,6 +878,17 @@ void launch_density_runs(hipStream_t s, const SortBufs &b, int64_t n, const Tile
                          const CoordParams &cp, double *gwt, double *blk) {
     hipLaunchKernelGGL(k_density_runs, dim3(run_blocks(n)), dim3(kThreads), 0, s, b.skeys, b.svals, n,
                        (uint32_t)sort_nkeys(tp), tp, cp.nu, cp.suppu, gwt, blk, b.tile_start);
+}
+
+void launch_restage(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v, const float *w,
+                    int64_t n, const CoordParams &cp, StageHot *hot) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_restage, dim3(prep_blocks(n)), dim3(kThreads), 0, s, vis, wt, u, v, w, n, cp, hot);
+}
+
+void launch_subtract(hipStream_t s, float2 *vis, const float2 *mvis, const float *mwt, int64_t n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_subtract, dim3(prep_blocks(n)), dim3(kThreads), 0, s, vis, mvis, mwt, n);
 }
 
 void launch_degrid_gather(hipStream_t s, const SortBufs &b, int64_t n, const CoordParams &cp, const TileParams &tp,

@@ -203,6 +203,13 @@ struct WeightParams {
 WeightParams make_weight_params(const CoordParams &wcp, int az, const float *robust, float pi_f,
                                 const double *gwt, const double *wsums);
 
+// New visibility values on a standing cell sort: the hot staging records'
+// rotated visibilities (uvw, weights and the sort unchanged).
+void launch_restage(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v, const float *w,
+                    int64_t n, const CoordParams &cp, StageHot *hot);
+// vis -= mvis where mwt > 0 (residual visibilities).
+void launch_subtract(hipStream_t s, float2 *vis, const float2 *mvis, const float *mwt, int64_t n);
+
 // Degridding halves in sorted order from the staging records (when no
 // gridding run wrote them).
 void launch_degrid_gather(hipStream_t s, const SortBufs &b, int64_t n, const CoordParams &cp, const TileParams &tp,

@@ -171,6 +171,15 @@ class Plan:
     def uvmap(self):
         self._ok(lib().sde_plan_uvmap(self._p), "sde_plan_uvmap")
 
+    def subtract_model(self):
+        """Resident visibilities <- VIS - MODEL where the last vismodel predicted them."""
+        self._ok(lib().sde_plan_subtract_model(self._p), "sde_plan_subtract_model")
+
+    def uvmap_residual(self):
+        """uvmap of the resident (residual) visibilities on the standing cell sort
+        and density weights of the last uvmap."""
+        self._ok(lib().sde_plan_uvmap_residual(self._p), "sde_plan_uvmap_residual")
+
     def vismodel(self, image=None):
         if image is None:
             self._ok(lib().sde_plan_vismodel(self._p, None), "sde_plan_vismodel")
@@ -386,4 +395,15 @@ def uvmap_files(uv_path, dirty_path, beam_path, nx, cellsize, ny=None, fov=0.0, 
     fits.write_image(dirty_path, dirty, cellsize, ra=d["ra"], dec=d["dec"], freq=d["freq"])
     fits.write_image(beam_path, psf, cellsize, ra=d["ra"], dec=d["dec"], freq=d["freq"])
     return info
+
+
+def major_cycle(plan, model_image):
+    """One major cycle on a plan that has run uvmap on its visibilities:
+    predict the model image, subtract, and image the residual on the
+    standing sort (the repeated VISTOIMG of the CLEAN and mosaic tasks, e.g.
+    task/mosclean.f).  Returns the residual dirty image and PSF."""
+    plan.vismodel(model_image)
+    plan.subtract_model()
+    plan.uvmap_residual()
+    return plan.images()
 
