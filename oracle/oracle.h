@@ -75,6 +75,30 @@ void o64_grddeh2d(double *vis, float *wt, const float *u, const float *v, const 
                   const double *intfn, int supp, int osamp, const double *shift,
                   int *nbad);
 
+/* ---- SlowZ 3-D (grd/grdh23d.f, grd/grddeh23.f): NZ planes GVIS(NU,NV,NZ) ---- */
+int r32_grdh23d(const float *vis, const float *wt, const float *u, const float *v,
+                const float *w, int nvis, float uscale, float uoffset, float vscale,
+                float voffset, int uorigin, int vorigin, float *gvis, int psf, int nu,
+                int nv, const float *gridfnu, int suppu, int osampu,
+                const float *gridfnv, int suppv, int osampv, int nz, float cellz,
+                float refz, const double *shift, float *sumwt);
+int o64_grdh23d(const float *vis, const double *wt, const float *u, const float *v,
+                const float *w, int nvis, float uscale, float uoffset, float vscale,
+                float voffset, int uorigin, int vorigin, double *gvis, int psf, int nu,
+                int nv, const double *gridfn, int supp, int osamp, int nz, float cellz,
+                float refz, const double *shift, double *sumwt);
+void r32_grddeh23(float *vis, float *wt, const float *u, const float *v, const float *w,
+                  int nvis, float uscale, float uoffset, float vscale, float voffset,
+                  int uorigin, int vorigin, const float *gvis, int nu, int nv,
+                  const float *intfnu, int suppu, int osampu, const float *intfnv,
+                  int suppv, int osampv, int nz, float cellz, float refz, const double *shift,
+                  int *nbad);
+void o64_grddeh23(double *vis, float *wt, const float *u, const float *v, const float *w,
+                  int nvis, float uscale, float uoffset, float vscale, float voffset,
+                  int uorigin, int vorigin, const double *gvis, int nu, int nv,
+                  const double *intfn, int supp, int osamp, int nz, float cellz, float refz,
+                  const double *shift, int *nbad);
+
 /* ---- density-grid weighting (grd/grduwt2d.f, grd/grdruw2d.f) ---- */
 void r32_grduwt2d(const float *wt, const float *u, const float *v, const float *w,
                   int nvis, float uscale, float uoffset, float vscale, float voffset,

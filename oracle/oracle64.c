@@ -586,3 +586,190 @@ void o64_dft_image(const float *vis, const double *wt, const float *u, const flo
         }
     }
 }
+
+/* ---- SlowZ 3-D (grd/grdh23d.f, grd/grddeh23.f): REAL cell decisions as the
+ * reference (its ULOCAL, UCELL are REAL), values in double ---- */
+#define GRE3(g, nu, nv, iu, iv, iz) (g)[2 * (((size_t)(iz) - 1) * (size_t)(nv) * (nu) + (size_t)((iv) - 1) * (nu) + (iu) - 1)]
+#define GIM3(g, nu, nv, iu, iv, iz) (g)[2 * (((size_t)(iz) - 1) * (size_t)(nv) * (nu) + (size_t)((iv) - 1) * (nu) + (iu) - 1) + 1]
+
+static void local_uvw3(const double *shift, float u, float v, float w, float *ul, float *vl, float *wl) {
+    const double ug = u, vg = v, wg = w;
+    *ul = (float)(shift[0] * ug + shift[1] * vg + shift[2] * wg);
+    *vl = (float)(shift[3] * ug + shift[4] * vg + shift[5] * wg);
+    *wl = (float)(shift[6] * ug + shift[7] * vg + shift[8] * wg);
+}
+
+/* grd/grdh23d.f:76-201 */
+int o64_grdh23d(const float *vis, const double *wt, const float *u, const float *v,
+                const float *w, int nvis, float uscale, float uoffset, float vscale,
+                float voffset, int uorigin, int vorigin, double *gvis, int psf, int nu,
+                int nv, const double *gridfn, int supp, int osamp, int nz, float cellz,
+                float refz, const double *shift, double *sumwt) {
+    const double twopi = (double)(8.0f * atanf(1.0f));
+    const double dtor = twopi / 360.0;
+    const int cfoffset = (supp + 1) * osamp + 1;
+    double lsumwt = 0.0;
+    memset(gvis, 0, sizeof(double) * 2 * (size_t)nu * nv * nz);
+    *sumwt = 0.0;
+    for (int i = 0; i < nvis; i++) {
+        float ul, vl, wl, ucell, vcell;
+        if (wt[i] <= 0.0) continue;
+        *sumwt += wt[i];
+        local_uvw3(shift, u[i], v[i], w[i], &ul, &vl, &wl);
+        const double phase = twopi * ((double)wl - (double)w[i]);
+        const double c = cos(phase), s = -sin(phase);
+        double fr = vis[2 * i] * c - vis[2 * i + 1] * s, fi = vis[2 * i] * s + vis[2 * i + 1] * c;
+        double wz = wl;
+        ucell = uscale * ul + uoffset;
+        vcell = vscale * vl + voffset;
+        if (ucell > 0.0f) {
+            fi = -fi;
+            ucell = ucell + (float)uorigin;
+            vcell = vcell + (float)vorigin;
+        } else {
+            ucell = -ucell + (float)uorigin;
+            vcell = -vcell + (float)vorigin;
+            wz = -wz;
+        }
+        if (psf) { fr = 1.0; fi = 0.0; }
+        if (nintf(ucell) + 2 * supp > nu) continue;
+        if (nintf(vcell) + supp > nv) continue;
+        if (nintf(vcell) - supp < 1) continue;
+        const int delui = nintf((float)osamp * ((float)nintf(ucell) - ucell));
+        const int delvi = nintf((float)osamp * ((float)nintf(vcell) - vcell));
+        const int uzero = nintf(ucell) + supp, vzero = nintf(vcell);
+        for (int iz = 1; iz <= nz; iz++) {
+            const double ph = -(twopi * wz * dtor * (double)cellz * ((double)iz - (double)refz));
+            const double cr = cos(ph), ci = -sin(ph);
+            const double rr = fr * cr - fi * ci, ri = fr * ci + fi * cr;
+            for (int offv = -supp; offv <= supp; offv++) {
+                const double gv = gridfn[delvi + osamp * offv + cfoffset - 1];
+                for (int offu = -supp; offu <= supp; offu++) {
+                    const double uvwt = wt[i] * gridfn[delui + osamp * offu + cfoffset - 1] * gv;
+                    GRE3(gvis, nu, nv, uzero + offu, vzero + offv, iz) += uvwt * rr;
+                    GIM3(gvis, nu, nv, uzero + offu, vzero + offv, iz) += uvwt * ri;
+                    lsumwt += uvwt;
+                }
+            }
+        }
+    }
+    if (lsumwt == 0.0) return 1;
+    const int uz = uorigin + supp, vz = vorigin;
+    for (int iz = 1; iz <= nz; iz++) {
+        for (int offv = -nv / 2 + 1; offv <= nv / 2 - 1; offv++)
+            for (int offu = 1; offu <= supp; offu++) {
+                GRE3(gvis, nu, nv, uz + offu, vz + offv, iz) += GRE3(gvis, nu, nv, uz - offu, vz - offv, iz);
+                GIM3(gvis, nu, nv, uz + offu, vz + offv, iz) -= GIM3(gvis, nu, nv, uz - offu, vz - offv, iz);
+            }
+        for (int offv = 0; offv <= nv / 2 - 1; offv++) {
+            GRE3(gvis, nu, nv, uz, vz + offv, iz) += GRE3(gvis, nu, nv, uz, vz - offv, iz);
+            GIM3(gvis, nu, nv, uz, vz + offv, iz) -= GIM3(gvis, nu, nv, uz, vz - offv, iz);
+            GRE3(gvis, nu, nv, uz, vz - offv, iz) = GRE3(gvis, nu, nv, uz, vz + offv, iz);
+            GIM3(gvis, nu, nv, uz, vz - offv, iz) = -GIM3(gvis, nu, nv, uz, vz + offv, iz);
+        }
+    }
+    const double scale = (double)(nu - 1) * (double)nv * (double)nz / lsumwt;
+    for (int iz = 1; iz <= nz; iz++)
+        for (int vg = 1; vg <= nv; vg++) {
+            for (int ug = 1; ug <= nu - supp; ug++) {
+                GRE3(gvis, nu, nv, ug, vg, iz) = scale * GRE3(gvis, nu, nv, ug + supp, vg, iz);
+                GIM3(gvis, nu, nv, ug, vg, iz) = scale * GIM3(gvis, nu, nv, ug + supp, vg, iz);
+            }
+            for (int ug = nu - supp + 1; ug <= nu; ug++)
+                GRE3(gvis, nu, nv, ug, vg, iz) = GIM3(gvis, nu, nv, ug, vg, iz) = 0.0;
+        }
+    return 0;
+}
+
+/* grd/grddeh23.f:78-206; the branches and loop orders of the reference
+ * (OFFU outer near the u = 0 axis, OFFV outer elsewhere).  A tap on a row
+ * outside the grid (reflected v, which the reference reads out of bounds)
+ * adds nothing.  nz <= 1024. */
+void o64_grddeh23(double *vis, float *wt, const float *u, const float *v, const float *w,
+                  int nvis, float uscale, float uoffset, float vscale, float voffset,
+                  int uorigin, int vorigin, const double *gvis, int nu, int nv,
+                  const double *intfn, int supp, int osamp, int nz, float cellz, float refz,
+                  const double *shift, int *nbad) {
+    const double twopi = (double)(8.0f * atanf(1.0f));
+    const double dtor = twopi / 360.0;
+    const int cfoffset = (supp + 1) * osamp + 1;
+    double crr[1024], cri[1024];
+    *nbad = 0;
+    for (int i = 0; i < nvis; i++) {
+        float ul, vl, wl, ucell, vcell;
+        double vr = 0.0, vi = 0.0, sumwt = 0.0;
+        if (wt[i] <= 0.0f) continue;
+        vis[2 * i] = vis[2 * i + 1] = 0.0;
+        wt[i] = -wt[i];
+        local_uvw3(shift, u[i], v[i], w[i], &ul, &vl, &wl);
+        const double phase = twopi * ((double)wl - (double)w[i]);
+        const double rr = cos(phase), ri = sin(phase);
+        double wz = wl;
+        ucell = uscale * ul + uoffset;
+        vcell = vscale * vl + voffset;
+        if (ucell > 0.0f) {
+            ucell = ucell + (float)uorigin;
+            vcell = vcell + (float)vorigin;
+        } else {
+            ucell = -ucell + (float)uorigin;
+            vcell = -vcell + (float)vorigin;
+            wz = -wz;
+        }
+        if (nintf(ucell) + supp > nu) continue;
+        if (nintf(vcell) + supp > nv) continue;
+        if (nintf(vcell) - supp < 1) continue;
+        const int delui = nintf((float)osamp * ((float)nintf(ucell) - ucell));
+        const int delvi = nintf((float)osamp * ((float)nintf(vcell) - vcell));
+        const int ucen = nintf(ucell) - uorigin, vcen = nintf(vcell) - vorigin;
+        for (int iz = 1; iz <= nz; iz++) {
+            const double ph = -(twopi * wz * dtor * (double)cellz * ((double)iz - (double)refz));
+            crr[iz - 1] = cos(ph);
+            cri[iz - 1] = sin(ph);
+        }
+        if (nintf(ucell) <= supp) {
+            for (int offu = -supp; offu <= supp; offu++) {
+                const double cu = intfn[delui + osamp * offu + cfoffset - 1];
+                const int usign = ucen == -offu ? 1 : ((ucen + offu) >= 0 ? 1 : -1);
+                const int uint_ = ucen == -offu ? uorigin : uorigin + usign * (ucen + offu);
+                for (int offv = -supp; offv <= supp; offv++) {
+                    const int vint = vorigin + usign * (vcen + offv);
+                    const double uvwt = cu * intfn[delvi + osamp * offv + cfoffset - 1];
+                    sumwt += uvwt;
+                    if (vint < 1 || vint > nv) continue;
+                    for (int iz = 1; iz <= nz; iz++) {
+                        const double gr = GRE3(gvis, nu, nv, uint_, vint, iz);
+                        const double gi = usign < 0 ? -GIM3(gvis, nu, nv, uint_, vint, iz) : GIM3(gvis, nu, nv, uint_, vint, iz);
+                        vr += (gr * crr[iz - 1] - gi * cri[iz - 1]) * uvwt;
+                        vi += (gr * cri[iz - 1] + gi * crr[iz - 1]) * uvwt;
+                    }
+                }
+            }
+        } else {
+            for (int offv = -supp; offv <= supp; offv++) {
+                const double cv = intfn[delvi + osamp * offv + cfoffset - 1];
+                const int vint = vorigin + vcen + offv;
+                for (int offu = -supp; offu <= supp; offu++) {
+                    const int uint_ = uorigin + ucen + offu;
+                    const double uvwt = intfn[delui + osamp * offu + cfoffset - 1] * cv;
+                    sumwt += uvwt;
+                    for (int iz = 1; iz <= nz; iz++) {
+                        const double gr = GRE3(gvis, nu, nv, uint_, vint, iz), gi = GIM3(gvis, nu, nv, uint_, vint, iz);
+                        vr += (gr * crr[iz - 1] - gi * cri[iz - 1]) * uvwt;
+                        vi += (gr * cri[iz - 1] + gi * crr[iz - 1]) * uvwt;
+                    }
+                }
+            }
+        }
+        if (sumwt > 0.0) {
+            vr /= sumwt;
+            vi /= sumwt;
+            wt[i] = fabsf(wt[i]);
+        } else {
+            (*nbad)++;
+            vr = vi = 0.0;
+        }
+        if (uscale * ul > 0.0f) vi = -vi;
+        vis[2 * i] = vr * rr - vi * ri;
+        vis[2 * i + 1] = vr * ri + vi * rr;
+    }
+}

@@ -92,6 +92,18 @@ for name, res, args in [
                             C.c_int, _d, _i]),
     ("o64_grddeh2d", None, [_d, _f, _f, _f, _f, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
                             C.c_int, C.c_int, _d, C.c_int, C.c_int, _d, C.c_int, C.c_int, _d, _i]),
+    ("r32_grdh23d", C.c_int, [_f, _f, _f, _f, _f, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
+                              C.c_int, C.c_int, _f, C.c_int, C.c_int, C.c_int, _f, C.c_int, C.c_int, _f,
+                              C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, _d, _f]),
+    ("o64_grdh23d", C.c_int, [_f, _d, _f, _f, _f, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
+                              C.c_int, C.c_int, _d, C.c_int, C.c_int, C.c_int, _d, C.c_int, C.c_int,
+                              C.c_int, C.c_float, C.c_float, _d, _d]),
+    ("r32_grddeh23", None, [_f, _f, _f, _f, _f, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
+                            C.c_int, C.c_int, _f, C.c_int, C.c_int, _f, C.c_int, C.c_int, _f, C.c_int,
+                            C.c_int, C.c_int, C.c_float, C.c_float, _d, _i]),
+    ("o64_grddeh23", None, [_d, _f, _f, _f, _f, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
+                            C.c_int, C.c_int, _d, C.c_int, C.c_int, _d, C.c_int, C.c_int, C.c_int,
+                            C.c_float, C.c_float, _d, _i]),
     ("r32_grduwt2d", None, [_f, _f, _f, _f, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
                             C.c_int, _f, _f, C.c_int, C.c_int, _d, _i]),
     ("r32_grdruw2d", C.c_int, [_f, _f, _f, _f, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
@@ -235,6 +247,69 @@ def grddeh2d64(wt, u, v, w, uscale, vscale, uorigin, vorigin, grid, cf, supp, os
     cf = np.ascontiguousarray(cf, np.float64)
     lib.o64_grddeh2d(_pd(vis), _pf(wt), _pf(u), _pf(v), _pf(w), len(u), uscale, uoffset, vscale, voffset,
                      uorigin, vorigin, _pd(grid), nu, nv, _pd(cf), supp, osamp,
+                     _pd(np.ascontiguousarray(shift, np.float64)), C.byref(nbad))
+    return vis, wt, nbad.value
+
+
+def grdh23d(vis, wt, u, v, w, uscale, vscale, uorigin, vorigin, psf, nu, nv, cf, supp, osamp, nz, cellz, refz,
+            shift, uoffset=0.0, voffset=0.0):
+    """REAL restatement of GRDH23D -> (GVIS as (NZ, NV, NU) complex64, SUMWT, rc)."""
+    vis, wt, u, v, w = c64(vis), f32(wt), f32(u), f32(v), f32(w)
+    cf = f32(cf)
+    g = np.zeros((nz, nv, nu), np.complex64)
+    sw = np.zeros(1, np.float32)
+    rc = lib.r32_grdh23d(_pf(vis), _pf(wt), _pf(u), _pf(v), _pf(w), len(u), uscale, uoffset, vscale, voffset,
+                         uorigin, vorigin, _pf(g), int(psf), nu, nv, _pf(cf), supp, osamp, _pf(cf), supp, osamp,
+                         nz, cellz, refz, _pd(np.ascontiguousarray(shift, np.float64)), _pf(sw))
+    return g, float(sw[0]), rc
+
+
+def grdh23d64(vis, wt, u, v, w, uscale, vscale, uorigin, vorigin, psf, nu, nv, cf, supp, osamp, nz, cellz, refz,
+              shift, uoffset=0.0, voffset=0.0):
+    """FP64 GRDH23D -> (GVIS as (NZ, NV, NU) complex128, SUMWT, rc)."""
+    vis, u, v, w = c64(vis), f32(u), f32(v), f32(w)
+    wt = np.ascontiguousarray(wt, np.float64)
+    cf = np.ascontiguousarray(cf, np.float64)
+    g = np.zeros((nz, nv, nu), np.complex128)
+    sw = C.c_double()
+    rc = lib.o64_grdh23d(_pf(vis), _pd(wt), _pf(u), _pf(v), _pf(w), len(u), uscale, uoffset, vscale, voffset,
+                         uorigin, vorigin, _pd(g), int(psf), nu, nv, _pd(cf), supp, osamp, nz, cellz, refz,
+                         _pd(np.ascontiguousarray(shift, np.float64)), C.byref(sw))
+    return g, sw.value, rc
+
+
+def grddeh23(wt, u, v, w, uscale, vscale, uorigin, vorigin, grid, cf, supp, osamp, cellz, refz, shift,
+             uoffset=0.0, voffset=0.0):
+    """REAL restatement of GRDDEH23 on GVIS (NZ, NV, NU) -> (VIS, WT with its sign flag, NBAD)."""
+    wt = f32(wt).copy()
+    u, v, w = f32(u), f32(v), f32(w)
+    grid = c64(grid)
+    nz, nv, nu = grid.shape
+    if nz > 128:
+        raise ValueError("nz <= 128")
+    vis = np.zeros(len(u), np.complex64)
+    nbad = C.c_int()
+    cf = f32(cf)
+    lib.r32_grddeh23(_pf(vis), _pf(wt), _pf(u), _pf(v), _pf(w), len(u), uscale, uoffset, vscale, voffset,
+                     uorigin, vorigin, _pf(grid), nu, nv, _pf(cf), supp, osamp, _pf(cf), supp, osamp, nz, cellz,
+                     refz, _pd(np.ascontiguousarray(shift, np.float64)), C.byref(nbad))
+    return vis, wt, nbad.value
+
+
+def grddeh23_64(wt, u, v, w, uscale, vscale, uorigin, vorigin, grid, cf, supp, osamp, cellz, refz, shift,
+                uoffset=0.0, voffset=0.0):
+    """FP64 GRDDEH23 -> (VIS complex128, WT, NBAD)."""
+    wt = f32(wt).copy()
+    u, v, w = f32(u), f32(v), f32(w)
+    grid = np.ascontiguousarray(grid, np.complex128)
+    nz, nv, nu = grid.shape
+    if nz > 1024:
+        raise ValueError("nz <= 1024")
+    vis = np.zeros(len(u), np.complex128)
+    nbad = C.c_int()
+    cf = np.ascontiguousarray(cf, np.float64)
+    lib.o64_grddeh23(_pd(vis), _pf(wt), _pf(u), _pf(v), _pf(w), len(u), uscale, uoffset, vscale, voffset,
+                     uorigin, vorigin, _pd(grid), nu, nv, _pd(cf), supp, osamp, nz, cellz, refz,
                      _pd(np.ascontiguousarray(shift, np.float64)), C.byref(nbad))
     return vis, wt, nbad.value
 

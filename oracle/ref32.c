@@ -1021,3 +1021,221 @@ int r32_vismodel(const or_uvmap_cfg *cfg, const float *image, float *vis, float 
     free(ccf);
     return 0;
 }
+
+/* ---- SlowZ 3-D: GRDH23D / GRDDEH23 (REAL coordinates, one plane per Z) ---- */
+
+/* REAL <- DOUBLE PRECISION SHIFT times REAL u, v, w (grd/grdh23d.f:100-105) */
+static void local_uvw(const double *shift, float u, float v, float w, float *ul, float *vl, float *wl) {
+    const double ug = u, vg = v, wg = w;
+    *ul = (float)(shift[0] * ug + shift[1] * vg + shift[2] * wg);
+    *vl = (float)(shift[3] * ug + shift[4] * vg + shift[5] * wg);
+    *wl = (float)(shift[6] * ug + shift[7] * vg + shift[8] * wg);
+}
+
+/* COMPLEX * COMPLEX */
+static void cmul(float a, float b, float c, float d, float *re, float *im) {
+    *re = a * c - b * d;
+    *im = a * d + b * c;
+}
+
+#define GRE3(g, nu, nv, iu, iv, iz) (g)[2 * (((size_t)(iz) - 1) * (size_t)(nv) * (nu) + (size_t)((iv) - 1) * (nu) + (iu) - 1)]
+#define GIM3(g, nu, nv, iu, iv, iz) (g)[2 * (((size_t)(iz) - 1) * (size_t)(nv) * (nu) + (size_t)((iv) - 1) * (nu) + (iu) - 1) + 1]
+
+/* grd/grdh23d.f:76-201 */
+int r32_grdh23d(const float *vis, const float *wt, const float *u, const float *v,
+                const float *w, int nvis, float uscale, float uoffset, float vscale,
+                float voffset, int uorigin, int vorigin, float *gvis, int psf, int nu,
+                int nv, const float *gridfnu, int suppu, int osampu,
+                const float *gridfnv, int suppv, int osampv, int nz, float cellz,
+                float refz, const double *shift, float *sumwt) {
+    const float twopi = 8.0f * atanf(1.0f);
+    const float dtor = twopi / 360.0f;
+    const int cfoffset = (suppu + 1) * osampu + 1;
+    float lsumwt = 0.0f;
+    memset(gvis, 0, sizeof(float) * 2 * (size_t)nu * nv * nz);
+    *sumwt = 0.0f;
+    for (int i = 0; i < nvis; i++) {
+        float ul, vl, wl, fr, fi, ucell, vcell;
+        if (wt[i] <= 0.0f) continue;
+        *sumwt = *sumwt + wt[i];
+        local_uvw(shift, u[i], v[i], w[i], &ul, &vl, &wl);
+        float phase = twopi * (wl - w[i]);
+        cmul(vis[2 * i], vis[2 * i + 1], cosf(phase), -sinf(phase), &fr, &fi);
+        ucell = uscale * ul + uoffset;
+        vcell = vscale * vl + voffset;
+        if (ucell > 0.0f) {
+            fi = -fi;
+            ucell = ucell + (float)uorigin;
+            vcell = vcell + (float)vorigin;
+        } else {
+            ucell = -ucell + (float)uorigin;
+            vcell = -vcell + (float)vorigin;
+            wl = -wl;
+        }
+        if (psf) { fr = 1.0f; fi = 0.0f; }
+        if (nintf(ucell) + 2 * suppu > nu) continue;
+        if (nintf(vcell) + suppv > nv) continue;
+        if (nintf(vcell) - suppv < 1) continue;
+        const int delui = nintf((float)osampu * ((float)nintf(ucell) - ucell));
+        const int delvi = nintf((float)osampv * ((float)nintf(vcell) - vcell));
+        const int uzero = nintf(ucell) + suppu, vzero = nintf(vcell);
+        for (int iz = 1; iz <= nz; iz++) {
+            const float ph = -(twopi * wl * dtor * cellz * ((float)iz - refz));
+            float rr, ri;
+            cmul(fr, fi, cosf(ph), -sinf(ph), &rr, &ri);
+            for (int offv = -suppv; offv <= suppv; offv++) {
+                const int delv = delvi + osampv * offv + cfoffset;
+                const int vgrid = vzero + offv;
+                for (int offu = -suppu; offu <= suppu; offu++) {
+                    const int delu = delui + osampu * offu + cfoffset;
+                    const int ugrid = uzero + offu;
+                    const float uvwt = wt[i] * gridfnu[delu - 1] * gridfnv[delv - 1];
+                    /* REAL * COMPLEX: a complex product with (UVWT, 0) */
+                    GRE3(gvis, nu, nv, ugrid, vgrid, iz) = GRE3(gvis, nu, nv, ugrid, vgrid, iz) + (uvwt * rr - 0.0f * ri);
+                    GIM3(gvis, nu, nv, ugrid, vgrid, iz) = GIM3(gvis, nu, nv, ugrid, vgrid, iz) + (uvwt * ri + 0.0f * rr);
+                    lsumwt = lsumwt + uvwt;
+                }
+            }
+        }
+    }
+    if (lsumwt == 0.0f) return 1;
+    const int uz = uorigin + suppu, vz = vorigin;
+    for (int iz = 1; iz <= nz; iz++)
+        for (int offv = -nv / 2 + 1; offv <= nv / 2 - 1; offv++)
+            for (int offu = 1; offu <= suppu; offu++) {
+                GRE3(gvis, nu, nv, uz + offu, vz + offv, iz) = GRE3(gvis, nu, nv, uz + offu, vz + offv, iz) + GRE3(gvis, nu, nv, uz - offu, vz - offv, iz);
+                GIM3(gvis, nu, nv, uz + offu, vz + offv, iz) = GIM3(gvis, nu, nv, uz + offu, vz + offv, iz) - GIM3(gvis, nu, nv, uz - offu, vz - offv, iz);
+            }
+    for (int iz = 1; iz <= nz; iz++)
+        for (int offv = 0; offv <= nv / 2 - 1; offv++) {
+            GRE3(gvis, nu, nv, uz, vz + offv, iz) = GRE3(gvis, nu, nv, uz, vz + offv, iz) + GRE3(gvis, nu, nv, uz, vz - offv, iz);
+            GIM3(gvis, nu, nv, uz, vz + offv, iz) = GIM3(gvis, nu, nv, uz, vz + offv, iz) - GIM3(gvis, nu, nv, uz, vz - offv, iz);
+            GRE3(gvis, nu, nv, uz, vz - offv, iz) = GRE3(gvis, nu, nv, uz, vz + offv, iz);
+            GIM3(gvis, nu, nv, uz, vz - offv, iz) = -GIM3(gvis, nu, nv, uz, vz + offv, iz);
+        }
+    lsumwt = (float)(nu - 1) * (float)nv * (float)nz / lsumwt;
+    for (int iz = 1; iz <= nz; iz++)
+        for (int vg = 1; vg <= nv; vg++) {
+            for (int ug = 1; ug <= nu - suppu; ug++) {
+                const float a = GRE3(gvis, nu, nv, ug + suppu, vg, iz), b = GIM3(gvis, nu, nv, ug + suppu, vg, iz);
+                GRE3(gvis, nu, nv, ug, vg, iz) = lsumwt * a - 0.0f * b;
+                GIM3(gvis, nu, nv, ug, vg, iz) = lsumwt * b + 0.0f * a;
+            }
+            for (int ug = nu - suppu + 1; ug <= nu; ug++)
+                GRE3(gvis, nu, nv, ug, vg, iz) = GIM3(gvis, nu, nv, ug, vg, iz) = 0.0f;
+        }
+    return 0;
+}
+
+/* grd/grddeh23.f:78-206 */
+void r32_grddeh23(float *vis, float *wt, const float *u, const float *v, const float *w,
+                  int nvis, float uscale, float uoffset, float vscale, float voffset,
+                  int uorigin, int vorigin, const float *gvis, int nu, int nv,
+                  const float *intfnu, int suppu, int osampu, const float *intfnv,
+                  int suppv, int osampv, int nz, float cellz, float refz, const double *shift,
+                  int *nbad) {
+    const float twopi = 8.0f * atanf(1.0f);
+    const float dtor = twopi / 360.0f;
+    const int cfoffset = (suppu + 1) * osampu + 1;
+    float crr[128], cri[128];
+    *nbad = 0;
+    for (int i = 0; i < nvis; i++) {
+        float ul, vl, wl, ucell, vcell, vr = 0.0f, vi = 0.0f, sumwt = 0.0f;
+        if (wt[i] <= 0.0f) continue;
+        vis[2 * i] = vis[2 * i + 1] = 0.0f;
+        wt[i] = -wt[i];
+        local_uvw(shift, u[i], v[i], w[i], &ul, &vl, &wl);
+        const float phase = twopi * (wl - w[i]);
+        const float rr = cosf(phase), ri = sinf(phase);
+        ucell = uscale * ul + uoffset;
+        vcell = vscale * vl + voffset;
+        if (ucell > 0.0f) {
+            ucell = ucell + (float)uorigin;
+            vcell = vcell + (float)vorigin;
+        } else {
+            ucell = -ucell + (float)uorigin;
+            vcell = -vcell + (float)vorigin;
+            wl = -wl;
+        }
+        if (nintf(ucell) + suppu > nu) continue;
+        if (nintf(vcell) + suppv > nv) continue;
+        if (nintf(vcell) - suppv < 1) continue;
+        const int delui = nintf((float)osampu * ((float)nintf(ucell) - ucell));
+        const int delvi = nintf((float)osampv * ((float)nintf(vcell) - vcell));
+        const int ucen = nintf(ucell) - uorigin, vcen = nintf(vcell) - vorigin;
+        for (int iz = 1; iz <= nz; iz++) {
+            const float ph = -(twopi * wl * dtor * cellz * ((float)iz - refz));
+            crr[iz - 1] = cosf(ph);
+            cri[iz - 1] = sinf(ph);
+        }
+        if (nintf(ucell) <= suppu) {
+            for (int offu = -suppu; offu <= suppu; offu++) {
+                const int delu = delui + osampu * offu + cfoffset;
+                if (ucen != -offu) {
+                    const int usign = (ucen + offu) >= 0 ? 1 : -1;
+                    const int uint_ = uorigin + usign * (ucen + offu);
+                    for (int offv = -suppv; offv <= suppv; offv++) {
+                        const int delv = delvi + osampv * offv + cfoffset;
+                        const int vint = vorigin + usign * (vcen + offv);
+                        const float uvwt = intfnu[delu - 1] * intfnv[delv - 1];
+                        sumwt = sumwt + uvwt;
+                        for (int iz = 1; iz <= nz; iz++) {
+                            float gr = 0.0f, gi = 0.0f, pr, pi;
+                            if (vint >= 1 && vint <= nv) {
+                                gr = GRE3(gvis, nu, nv, uint_, vint, iz);
+                                gi = GIM3(gvis, nu, nv, uint_, vint, iz);
+                            }
+                            if (usign < 0) gi = -gi;
+                            /* (FVIS * CROTZ) * UVWT */
+                            cmul(gr, gi, crr[iz - 1], cri[iz - 1], &pr, &pi);
+                            vr = vr + (pr * uvwt - pi * 0.0f);
+                            vi = vi + (pr * 0.0f + pi * uvwt);
+                        }
+                    }
+                } else {
+                    for (int offv = -suppv; offv <= suppv; offv++) {
+                        const int delv = delvi + osampv * offv + cfoffset;
+                        const int vint = vorigin + (vcen + offv);
+                        const float uvwt = intfnu[delu - 1] * intfnv[delv - 1];
+                        sumwt = sumwt + uvwt;
+                        for (int iz = 1; iz <= nz; iz++) {
+                            /* (GVIS * UVWT) * CROTZ */
+                            const float gr = GRE3(gvis, nu, nv, uorigin, vint, iz), gi = GIM3(gvis, nu, nv, uorigin, vint, iz);
+                            float pr, pi;
+                            cmul(gr * uvwt - gi * 0.0f, gr * 0.0f + gi * uvwt, crr[iz - 1], cri[iz - 1], &pr, &pi);
+                            vr = vr + pr;
+                            vi = vi + pi;
+                        }
+                    }
+                }
+            }
+        } else {
+            for (int offv = -suppv; offv <= suppv; offv++) {
+                const int delv = delvi + osampv * offv + cfoffset;
+                const int vint = vorigin + vcen + offv;
+                for (int offu = -suppu; offu <= suppu; offu++) {
+                    const int delu = delui + osampu * offu + cfoffset;
+                    const int uint_ = uorigin + ucen + offu;
+                    const float uvwt = intfnu[delu - 1] * intfnv[delv - 1];
+                    sumwt = sumwt + uvwt;
+                    for (int iz = 1; iz <= nz; iz++) {
+                        const float gr = GRE3(gvis, nu, nv, uint_, vint, iz), gi = GIM3(gvis, nu, nv, uint_, vint, iz);
+                        float pr, pi;
+                        cmul(gr * uvwt - gi * 0.0f, gr * 0.0f + gi * uvwt, crr[iz - 1], cri[iz - 1], &pr, &pi);
+                        vr = vr + pr;
+                        vi = vi + pi;
+                    }
+                }
+            }
+        }
+        if (sumwt > 0.0f) {
+            cdiv_by_real(&vr, &vi, sumwt);
+            wt[i] = fabsf(wt[i]);
+        } else {
+            (*nbad)++;
+            vr = vi = 0.0f;
+        }
+        if (uscale * ul > 0.0f) vi = -vi;
+        cmul(vr, vi, rr, ri, &vis[2 * i], &vis[2 * i + 1]);
+    }
+}

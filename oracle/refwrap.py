@@ -129,6 +129,34 @@ def grddeh2d(wt, u, v, w, uscale, vscale, uorigin, vorigin, grid, cf, supp, osam
     return vis, wt
 
 
+def grdh23d(vis, wt, u, v, w, uscale, vscale, uorigin, vorigin, psf, nu, nv, cf, supp, osamp, nz, cellz, refz,
+            shift, uoffset=0.0, voffset=0.0, variant=""):
+    """GRDH23D (grd/grdh23d.f:7-10) -> (GVIS(NU,NV,NZ) as (NZ, NV, NU) complex64, SUMWT)."""
+    vis, wt, u, v, w, cf = _c64(vis), _f32(wt), _f32(u), _f32(v), _f32(w), _f32(cf)
+    g = np.zeros((nz, nv, nu), np.complex64)
+    sw = C.c_float(0.0)
+    lib(variant).grdh23d_(_p(vis), _p(wt), _p(u), _p(v), _p(w), _i(len(u)), _r(uscale), _r(uoffset), _r(vscale),
+                          _r(voffset), _i(uorigin), _i(vorigin), _p(g), _i(1 if psf else 0), _i(nu), _i(nv), _p(cf),
+                          _i(supp), _i(osamp), _p(cf), _i(supp), _i(osamp), _i(nz), _r(cellz), _r(refz),
+                          _p(_shift(shift)), C.byref(sw))
+    return g, sw.value
+
+
+def grddeh23(wt, u, v, w, uscale, vscale, uorigin, vorigin, grid, cf, supp, osamp, cellz, refz, shift,
+             uoffset=0.0, voffset=0.0, variant=""):
+    """GRDDEH23 (grd/grddeh23.f:7-10) on GVIS (NZ, NV, NU) -> (VIS complex64, WT with its sign flag).
+    Every point must degrid (the routine reports failures through MSGPUT,
+    which this library leaves unresolved)."""
+    wt = _f32(wt).copy()
+    u, v, w, cf, grid = _f32(u), _f32(v), _f32(w), _f32(cf), _c64(grid)
+    nz, nv, nu = grid.shape
+    vis = np.zeros(len(u), np.complex64)
+    lib(variant).grddeh23_(_p(vis), _p(wt), _p(u), _p(v), _p(w), _i(len(u)), _r(uscale), _r(uoffset), _r(vscale),
+                           _r(voffset), _i(uorigin), _i(vorigin), _p(grid), _i(nu), _i(nv), _p(cf), _i(supp),
+                           _i(osamp), _p(cf), _i(supp), _i(osamp), _i(nz), _r(cellz), _r(refz), _p(_shift(shift)))
+    return vis, wt
+
+
 def grduwt2d(wt, u, v, w, uscale, vscale, uorigin, vorigin, nu, nv, shift, uoffset=0.0, voffset=0.0, variant=""):
     """GRDUWT2D (grd/grduwt2d.f:7-8) -> (NEWWT, GWT as (NV, NU))."""
     wt, u, v, w = _f32(wt), _f32(u), _f32(v), _f32(w)

@@ -422,3 +422,160 @@ def test_imgdoubl_padding_and_reference_pixel():
         assert (nx + 1) // 2 + lx == pnx // 2
         np.testing.assert_array_equal(pad[ly:ly + ny, lx:lx + nx], img)
         assert float(pad.sum()) == float(img.sum())
+
+
+# ---------------------------------------------------------------- w planes (SlowZ)
+def _slowz_points(n, seed):
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(-40, 40, n).astype(np.float32)
+    u[: n // 10] = rng.uniform(-3.5, 3.5, n // 10).astype(np.float32)
+    v = rng.uniform(-50, 50, n).astype(np.float32)
+    w = rng.uniform(-5, 5, n).astype(np.float32)
+    wt = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    # multiples of 1/1024: the REAL cell coordinates of GRDH23D (v + VORIGIN
+    # rounded to REAL) are then those of GRDH2D's DOUBLE coordinate step
+    u, v = np.round(u * 1024) / 1024, np.round(v * 1024) / 1024
+    return u.astype(np.float32), v.astype(np.float32), w, wt
+
+
+def _n_of_plane(k, cellz, refz):
+    """Direction cosine offset of plane k (1-based): DTOR * CELLZ * (k - REFZ)
+    with the reference's REAL TWOPI (grd/grdh23d.f:77,89,154-155)."""
+    twopi = float(np.float32(8.0) * np.arctan(np.float32(1.0)))
+    return twopi / 360.0 * cellz * (k - refz)
+
+
+def test_slowz_gridding_focuses_a_plane():
+    """A source whose visibilities carry the phase exp(+2 pi i w n_k) of
+    plane k is brought to zero phase on that plane: GRDH23D's plane k
+    equals GRDH2D's PSF grid of the same points (same taps, same fold, and
+    the NZ of the normalisation cancels against LSUMWT summed over NZ
+    planes), and the other planes do not."""
+    u, v, w, wt = _slowz_points(3000, 31)
+    nu, nv, vo = 65, 128, 64
+    cf, supp, osamp = po.grdcf("SF", fp64=True)
+    nz, cellz, refz = 5, 3.0, 2.0
+    ident = np.eye(3).reshape(-1)
+    psf2d, sw2, _, rc2 = po.grdh2d64(np.ones(len(u)), wt, u, v, w, 1.0, 1.0, 1, vo, True, nu, nv, cf, supp, osamp,
+                                     ident)
+    assert rc2 == 0
+    for k in (1, 4):
+        vis = np.exp(2j * np.pi * w.astype(np.float64) * _n_of_plane(k, cellz, refz)).astype(np.complex64)
+        g3, sw3, rc = po.grdh23d64(vis, wt, u, v, w, 1.0, 1.0, 1, vo, False, nu, nv, cf, supp, osamp, nz, cellz,
+                                   refz, ident)
+        assert rc == 0 and sw3 == pytest.approx(sw2, rel=1e-12)
+        scale = np.abs(psf2d).max()
+        assert np.abs(g3[k - 1] - psf2d).max() < 2e-6 * scale
+        other = 3 if k == 1 else 1
+        assert np.abs(g3[other - 1] - psf2d).max() > 0.1 * scale
+
+
+def test_slowz_psf_reference_plane_is_the_2d_psf():
+    """With PSF the reference plane (n = 0) is GRDH2D's PSF; the plane at -n
+    is the complex conjugate of the plane at +n (a real source distribution
+    at +n and -n)."""
+    u, v, w, wt = _slowz_points(2000, 32)
+    nu, nv, vo = 65, 128, 64
+    cf, supp, osamp = po.grdcf("SF", fp64=True)
+    ident = np.eye(3).reshape(-1)
+    psf2d = po.grdh2d64(np.ones(len(u)), wt, u, v, w, 1.0, 1.0, 1, vo, True, nu, nv, cf, supp, osamp, ident)[0]
+    g3, _, rc = po.grdh23d64(np.ones(len(u)), wt, u, v, w, 1.0, 1.0, 1, vo, True, nu, nv, cf, supp, osamp, 5, 2.0,
+                             3.0, ident)
+    assert rc == 0
+    scale = np.abs(psf2d).max()
+    assert np.abs(g3[2] - psf2d).max() < 1e-12 * scale
+    assert np.abs(g3[1] - np.conj(g3[3])).max() < 1e-6 * scale
+    assert np.abs(g3[1].imag).max() > 1e-3 * scale
+
+
+def test_slowz_degridding_is_the_2d_degrid_times_the_plane_phase():
+    """A grid with only plane k set degrids to GRDDEH2D of that plane times
+    exp(+2 pi i w n_k), on both halves of the u axis and near it."""
+    u, v, w, wt = _slowz_points(2000, 33)
+    nu, nv, vo = 65, 128, 64
+    cf, supp, osamp = po.grdcf("SF", fp64=True)
+    ident = np.eye(3).reshape(-1)
+    rng = np.random.default_rng(34)
+    g2 = rng.standard_normal((nv, nu)) + 1j * rng.standard_normal((nv, nu))
+    nz, cellz, refz = 4, 2.5, 1.5
+    for k in (1, 3):
+        g3 = np.zeros((nz, nv, nu), np.complex128)
+        g3[k - 1] = g2
+        v3, w3, nbad3 = po.grddeh23_64(wt, u, v, w, 1.0, 1.0, 1, vo, g3, cf, supp, osamp, cellz, refz, ident)
+        v2, w2, nbad2 = po.grddeh2d64(wt, u, v, w, 1.0, 1.0, 1, vo, g2, cf, supp, osamp, ident)
+        assert nbad3 == nbad2 == 0 and np.array_equal(w3, w2)
+        want = v2 * np.exp(2j * np.pi * w.astype(np.float64) * _n_of_plane(k, cellz, refz))
+        assert np.abs(v3 - want).max() < 1e-6 * np.abs(v2).max()
+
+
+def test_slowz_gridding_is_the_adjoint_of_degridding():
+    """<degrid(G), V> = <G, grid(V)> up to the normalisations, the
+    reflection fold aside: points away from the u = 0 axis only."""
+    rng = np.random.default_rng(35)
+    n = 1500
+    u = (np.round(rng.uniform(-40, -8, n) * 1024) / 1024).astype(np.float32)
+    v = (np.round(rng.uniform(-50, 50, n) * 1024) / 1024).astype(np.float32)
+    w = rng.uniform(-5, 5, n).astype(np.float32)
+    wt = np.ones(n, np.float32)
+    nu, nv, vo, nz = 65, 128, 64, 3
+    cf, supp, osamp = po.grdcf("SF", fp64=True)
+    ident = np.eye(3).reshape(-1)
+    vis = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    g, _, rc = po.grdh23d64(vis, wt, u, v, w, 1.0, 1.0, 1, vo, False, nu, nv, cf, supp, osamp, nz, 2.0, 2.0, ident)
+    assert rc == 0
+    G = rng.standard_normal((nz, nv, nu)) + 1j * rng.standard_normal((nz, nv, nu))
+    dv, _, nbad = po.grddeh23_64(wt, u, v, w, 1.0, 1.0, 1, vo, G, cf, supp, osamp, 2.0, 2.0, ident)
+    assert nbad == 0
+    # undo the normalisations: GRDH23D scales by (NU-1)*NV*NZ/LSUMWT (its
+    # shift left by SUPP undoes the SUPP columns it gridded with, so its
+    # output columns are GRDDEH23's); GRDDEH23 divides by the tap sum of
+    # each point
+    tsum = np.array([np.sum(np.outer(*_taps(cf, supp, osamp, ui, vi))) for ui, vi in zip(-u, -v)])
+    lsum = nz * tsum.sum()
+    graw = g * lsum / ((nu - 1) * nv * nz)
+    lhs = np.sum(np.conj(dv * tsum) * vis)
+    rhs = np.sum(np.conj(G) * graw)
+    assert abs(lhs - rhs) < 1e-9 * abs(rhs)
+
+
+def _taps(cf, supp, osamp, ucell, vcell):
+    """The 2*SUPP+1 kernel values along u and v of a point at cell
+    coordinates (ucell, vcell) relative to the origin (grd/grdh23d.f:127-150)."""
+    cfo = (supp + 1) * osamp + 1
+    out = []
+    for c in (np.float32(ucell), np.float32(vcell)):
+        d = _nint(np.float32(osamp) * (np.float32(_nint(c)) - c))
+        out.append(np.array([cf[d + osamp * o + cfo - 1] for o in range(-supp, supp + 1)]))
+    return out
+
+
+def _nint(x):
+    """Fortran NINT: halves round away from zero."""
+    return int(math.floor(x + 0.5)) if x >= 0 else int(math.ceil(x - 0.5))
+
+
+def test_slowz_fp32_restatement_tracks_fp64():
+    u, v, w, wt = _slowz_points(4000, 36)
+    rng = np.random.default_rng(37)
+    vis = (rng.standard_normal(len(u)) + 1j * rng.standard_normal(len(u))).astype(np.complex64)
+    nu, nv, vo = 65, 128, 64
+    cf, supp, osamp = po.grdcf("SF")
+    cf64, _, _ = po.grdcf("SF", fp64=True)
+    s = po.crdshift(0.3, 0.6, 0.3002, 0.6001)
+    for psf in (False, True):
+        g32, sw32, rc32 = po.grdh23d(vis, wt, u, v, w, 1.0, 1.0, 1, vo, psf, nu, nv, cf, supp, osamp, 4, 3.0, 2.0, s)
+        g64, sw64, rc64 = po.grdh23d64(vis, wt, u, v, w, 1.0, 1.0, 1, vo, psf, nu, nv, cf64, supp, osamp, 4, 3.0,
+                                       2.0, s)
+        assert rc32 == rc64 == 0
+        # the REAL running LSUMWT over NZ*NVIS*(2*SUPP+1)**2 = 784000 taps
+        # drifts by a few 1e-4: one factor over the whole cube
+        big = np.abs(g64) > 0.3 * np.abs(g64).max()
+        f = np.median(np.abs(g32[big] / g64[big]))
+        assert abs(f - 1) < 2e-3
+        assert np.abs(g32 / f - g64).max() < 2e-5 * np.abs(g64).max()
+        assert sw32 == pytest.approx(sw64, rel=5e-6)
+    G = (rng.standard_normal((4, nv, nu)) + 1j * rng.standard_normal((4, nv, nu))).astype(np.complex64)
+    a, wa, na = po.grddeh23(wt, u, v, w, 1.0, 1.0, 1, vo, G, cf, supp, osamp, 3.0, 2.0, s)
+    b, wb, nb = po.grddeh23_64(wt, u, v, w, 1.0, 1.0, 1, vo, G, cf64, supp, osamp, 3.0, 2.0, s)
+    assert na == nb == 0 and np.array_equal(wa, wb)
+    assert np.abs(a - b).max() < 2e-5 * np.abs(b).max()

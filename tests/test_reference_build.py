@@ -143,6 +143,36 @@ def test_grddeh2d(shifted):
     assert _bits_equal(a_vis[ok], b_vis[ok])
 
 
+@pytest.mark.parametrize("psf", [0, 1])
+@pytest.mark.parametrize("shifted", [False, True])
+@pytest.mark.parametrize("nz,cellz,refz", [(4, 3.0, 2.0), (7, 1.5, 4.5)])
+def test_grdh23d(psf, shifted, nz, cellz, refz):
+    vis, wt, u, v, w = _vis(3000, 11, near_axis=200)
+    cf, supp, osamp = po.grdcf("SF")
+    sh = _shifted() if shifted else IDENTITY
+    g1, s1 = rw.grdh23d(vis, wt, u, v, w, 1.0, 1.0, 1, VORIGIN, psf, NU, NV, cf, supp, osamp, nz, cellz, refz, sh)
+    g2, s2, rc = po.grdh23d(vis, wt, u, v, w, 1.0, 1.0, 1, VORIGIN, psf, NU, NV, cf, supp, osamp, nz, cellz, refz,
+                            sh)
+    assert rc == 0 and s1 == s2
+    assert _bits_equal(g1, g2)
+
+
+@pytest.mark.parametrize("shifted", [False, True])
+@pytest.mark.parametrize("nz,cellz,refz", [(4, 3.0, 2.0), (7, 1.5, 4.5)])
+def test_grddeh23(shifted, nz, cellz, refz):
+    vis, wt, u, v, w = _vis(3000, 12, near_axis=300)
+    rng = np.random.default_rng(13)
+    grid = (rng.standard_normal((nz, NV, NU)) + 1j * rng.standard_normal((nz, NV, NU))).astype(np.complex64)
+    cf, supp, osamp = po.grdcf("SF")
+    sh = _shifted() if shifted else IDENTITY
+    a_vis, a_wt = rw.grddeh23(wt, u, v, w, 1.0, 1.0, 1, VORIGIN, grid, cf, supp, osamp, cellz, refz, sh)
+    b_vis, b_wt, nbad = po.grddeh23(wt, u, v, w, 1.0, 1.0, 1, VORIGIN, grid, cf, supp, osamp, cellz, refz, sh)
+    assert nbad == 0
+    ok = wt > 0
+    assert _bits_equal(a_wt, b_wt)
+    assert _bits_equal(a_vis[ok], b_vis[ok])
+
+
 def test_grduwt2d():
     vis, wt, u, v, w = _vis(5000, 8, near_axis=300)
     a = rw.grduwt2d(wt, u, v, w, 1.0, 1.0, 1, VORIGIN, NU, NV, _shifted())
