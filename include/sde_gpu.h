@@ -63,6 +63,29 @@ void grddeh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis,
                float *intfnu, int *suppu, int *osampu, float *intfnv, int *suppv,
                int *osampv, double *shift);
 
+/* Gridding onto NZ planes GVIS(NU,NV,NZ) of a slow transform in Z (w
+ * planes): plane IZ holds the visibilities rotated by the phase of its
+ * offset DTOR*CELLZ*(IZ-REFZ) in n; each plane folded, shifted and
+ * normalised as GRDH2D's, by (NU-1)*NV*NZ over the taps of all planes.
+ * The cell step is the reference's REAL one.  NZ <= 128.  Replaces GRDH23D,
+ * source/grd/grdh23d.f:7-10 (args :54-62). */
+void grdh23d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis,
+              float *uscale, float *uoffset, float *vscale, float *voffset,
+              int *uorigin, int *vorigin, float *gvis, int *psf, int *nu, int *nv,
+              float *gridfnu, int *suppu, int *osampu, float *gridfnv, int *suppv,
+              int *osampv, int *nz, float *cellz, float *refz, double *shift,
+              float *sumwt);
+
+/* Degridding from the NZ planes GVIS(NU,NV,NZ): every tap summed over the
+ * planes rotated back by their phases; VIS and the WT sign flag as
+ * GRDDEH2D's.  NZ <= 128.  Replaces GRDDEH23, source/grd/grddeh23.f:7-10
+ * (args :55-63). */
+void grddeh23_(float *vis, float *wt, float *u, float *v, float *w, int *nvis,
+               float *uscale, float *uoffset, float *vscale, float *voffset,
+               int *uorigin, int *vorigin, float *gvis, int *nu, int *nv,
+               float *intfnu, int *suppu, int *osampu, float *intfnv, int *suppv,
+               int *osampv, int *nz, float *cellz, float *refz, double *shift);
+
 /* Uniform weighting by a density grid GWT(NU,NV); NEWWT may alias WT.
  * Replaces GRDUWT2D, source/grd/grduwt2d.f:7-8. */
 void grduwt2d_(float *wt, float *u, float *v, float *w, int *nvis, float *uscale,

@@ -438,6 +438,26 @@ bool check_kernel_args(const char *routine, int nvis, int nu, int nv, int suppu,
     return true;
 }
 
+bool check_planes(const char *routine, int nz) {
+    if (nz < 1 || nz > kMaxPlanes) {
+        char msg[96];
+        snprintf(msg, sizeof msg, "number of planes NZ=%d outside 1..%d", nz, kMaxPlanes);
+        report_error(kErrBdArg, routine, msg);
+        return false;
+    }
+    return true;
+}
+
+PlaneParams make_pp(int nz, float cellz, float refz) {
+    PlaneParams pp;
+    pp.twopi = (float)ref_twopi();
+    pp.dtor = pp.twopi / 360.0f;
+    pp.cellz = cellz;
+    pp.refz = refz;
+    pp.nz = nz;
+    return pp;
+}
+
 }  // namespace
 
 extern "C" {
@@ -648,6 +668,152 @@ void grddeh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, f
     SDE_F(hipMemcpyAsync(dg, gvis, sizeof(float2) * (size_t)*nu * *nv, hipMemcpyHostToDevice, s));
     SDE_F(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * 2, s));
     launch_degrid(s, dg, du, dv, dw, dwt, dvis, n, cp, dcfu, dcfv, dvis, dwt, cnt);
+    SDE_F(hipGetLastError());
+    unsigned long long nbad = 0;
+    if (n > 0) {
+        SDE_F(hipMemcpyAsync(vis, dvis, sizeof(float2) * n, hipMemcpyDeviceToHost, s));
+        SDE_F(hipMemcpyAsync(wt, dwt, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    }
+    SDE_F(hipMemcpyAsync(&nbad, cnt, sizeof nbad, hipMemcpyDeviceToHost, s));
+    SDE_F(hipStreamSynchronize(s));
+    if (nbad > 0) {
+        char msg[96];
+        snprintf(msg, sizeof msg, "%7llu points were not successfully degridded", nbad);
+        report_warning(msg);
+    }
+}
+
+// GRDH23D: one cell sort of the points, then per plane the plane's phase
+// into the hot records, the tile kernel and the reduce (fold, shift,
+// normalise) into that plane of the output.  The normalisation
+// (NU-1)*NV*NZ/LSUMWT with LSUMWT summed over NZ identical planes is
+// (NU-1)*NV over one plane's LSUMWT.
+void grdh23d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, float *uscale, float *uoffset,
+              float *vscale, float *voffset, int *uorigin, int *vorigin, float *gvis, int *psf, int *nu, int *nv,
+              float *gridfnu, int *suppu, int *osampu, float *gridfnv, int *suppv, int *osampv, int *nz,
+              float *cellz, float *refz, double *shift, float *sumwt) {
+    if (sysi_.error) return;
+    const char *R = "GRDH23D";
+    if (!check_kernel_args(R, *nvis, *nu, *nv, *suppu, *suppv, *osampu, *osampv, *uorigin, *vorigin) ||
+        !check_planes(R, *nz))
+        return;
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    Ctx &c = g_ctx;
+    if (ctx_init(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
+    hipStream_t s = c.stream;
+    const int64_t n = *nvis;
+    const int ncfu = 1 + 2 * (*suppu + 1) * *osampu, ncfv = 1 + 2 * (*suppv + 1) * *osampv;
+    const CoordParams cp = make_cp(shift, *uscale, *uoffset, *vscale, *voffset, *uorigin, *vorigin, *nu, *nv,
+                                   *suppu, *suppv, *osampu, *osampv);
+    const PlaneParams pp = make_pp(*nz, *cellz, *refz);
+    const TileParams tp = make_tp(*nu, *nv, default_tile(*nu, *nv), default_chunk(n));
+    const size_t nn = (size_t)(n > 0 ? n : 1), np = (size_t)*nu * *nv;
+    SortBufs sb;
+    sb.max_items = max_items_for(tp, n);
+    sb.temp_bytes = sort_temp_bytes(n > 0 ? n : 1, sort_nbits(tp));
+    float2 *dvis = ctx_buf<float2>(c, "vis", nn);
+    float *dwt = ctx_buf<float>(c, "wt", nn), *du = ctx_buf<float>(c, "u", nn);
+    float *dv = ctx_buf<float>(c, "v", nn), *dw = ctx_buf<float>(c, "w", nn);
+    float *dcfu = ctx_buf<float>(c, "cfu", ncfu), *dcfv = ctx_buf<float>(c, "cfv", ncfv);
+    float4 *zbase = ctx_buf<float4>(c, "zbase", nn);
+    sb.keys = ctx_buf<uint32_t>(c, "keys", nn);
+    sb.skeys = ctx_buf<uint32_t>(c, "skeys", nn);
+    sb.vals = ctx_buf<uint64_t>(c, "vals", nn);
+    sb.svals = ctx_buf<uint64_t>(c, "svals", nn);
+    sb.hot = ctx_buf<StageHot>(c, "stage_hot", nn);
+    sb.cold = nullptr;
+    sb.cold_all = 0;
+    sb.temp = ctx_buf<char>(c, "sort_temp", sb.temp_bytes > 0 ? sb.temp_bytes : 1);
+    sb.tile_start = ctx_buf<int>(c, "tile_start", tp.ntiles + 1);
+    sb.item_start = ctx_buf<int>(c, "item_start", tp.ntiles + 1);
+    sb.nitems = ctx_buf<int>(c, "nitems", 1);
+    sb.items = ctx_buf<WorkItem>(c, "items", sb.max_items);
+    float *partial = ctx_buf<float>(c, "partial", grid_partial_floats(tp, 1, sb.max_items));
+    double *blk = ctx_buf<double>(c, "blk", 2 * (size_t)sb.max_items), *dsums = ctx_buf<double>(c, "sums", 4);
+    float2 *dg = ctx_buf<float2>(c, "grid", np * *nz);
+    double *pblk = ctx_buf<double>(c, "prep_blk", 2 * (size_t)prep_blocks(n > 0 ? n : 1));
+    double *dsw = ctx_buf<double>(c, "sumwt", 4);
+    if (!pblk || !dsw || !dvis || !dwt || !du || !dv || !dw || !dcfu || !dcfv || !zbase || !sb.keys || !sb.skeys || !sb.vals ||
+        !sb.svals || !sb.hot || !sb.temp || sb.temp_bytes == 0 || !sb.tile_start || !sb.item_start || !sb.nitems ||
+        !sb.items || !partial || !blk || !dsums || !dg) {
+        report_error(kErrNoMem, R, "device allocation failed");
+        return;
+    }
+    if (n > 0) {
+        SDE_F(hipMemcpyAsync(dvis, vis, sizeof(float2) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(dwt, wt, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(du, u, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(dv, v, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(dw, w, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    }
+    SDE_F(hipMemcpyAsync(dcfu, gridfnu, sizeof(float) * ncfu, hipMemcpyHostToDevice, s));
+    SDE_F(hipMemcpyAsync(dcfv, gridfnv, sizeof(float) * ncfv, hipMemcpyHostToDevice, s));
+    launch_prep_slowz(s, dvis, dwt, du, dv, dw, n, cp, tp, *psf, pp, sb, zbase, pblk, dsw);
+    if (launch_sort(s, sb, n, tp, true)) {
+        report_error(kErrFatal, R, "cell sort failed");
+        return;
+    }
+    auto nb = [](const char *) {};
+    auto ne = []() {};
+    for (int iz = 1; iz <= *nz; iz++) {
+        launch_plane_phase(s, zbase, sb.hot, n, pp, iz);
+        if (run_grid(s, cp, tp, 1, false, WeightParams(), dcfu, dcfv, sb, nullptr, partial, blk, dsums,
+                     dg + (size_t)(iz - 1) * np, nullptr, nb, ne)) {
+            report_error(kErrFatal, R, sde_last_error());
+            return;
+        }
+    }
+    double sums[3], sw[3];
+    SDE_F(hipMemcpyAsync(gvis, dg, sizeof(float2) * np * *nz, hipMemcpyDeviceToHost, s));
+    SDE_F(hipMemcpyAsync(sums, dsums, sizeof sums, hipMemcpyDeviceToHost, s));
+    SDE_F(hipMemcpyAsync(sw, dsw, sizeof sw, hipMemcpyDeviceToHost, s));
+    SDE_F(hipStreamSynchronize(s));
+    *sumwt = (float)sw[0];
+    if (sums[1] == 0.0) report_error(kErrBdArg, R, "No weight!");
+}
+
+// GRDDEH23: the per-point kernel over the NZ planes.
+void grddeh23_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, float *uscale, float *uoffset,
+               float *vscale, float *voffset, int *uorigin, int *vorigin, float *gvis, int *nu, int *nv,
+               float *intfnu, int *suppu, int *osampu, float *intfnv, int *suppv, int *osampv, int *nz,
+               float *cellz, float *refz, double *shift) {
+    if (sysi_.error) return;
+    const char *R = "GRDDEH23";
+    if (!check_kernel_args(R, *nvis, *nu, *nv, *suppu, *suppv, *osampu, *osampv, *uorigin, *vorigin) ||
+        !check_planes(R, *nz))
+        return;
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    Ctx &c = g_ctx;
+    if (ctx_init(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
+    hipStream_t s = c.stream;
+    const int64_t n = *nvis;
+    const size_t nn = (size_t)(n > 0 ? n : 1), ng = (size_t)*nu * *nv * *nz;
+    const int ncfu = 1 + 2 * (*suppu + 1) * *osampu, ncfv = 1 + 2 * (*suppv + 1) * *osampv;
+    const CoordParams cp = make_cp(shift, *uscale, *uoffset, *vscale, *voffset, *uorigin, *vorigin, *nu, *nv,
+                                   *suppu, *suppv, *osampu, *osampv);
+    const PlaneParams pp = make_pp(*nz, *cellz, *refz);
+    float2 *dvis = ctx_buf<float2>(c, "vis", nn);
+    float *dwt = ctx_buf<float>(c, "wt", nn), *du = ctx_buf<float>(c, "u", nn);
+    float *dv = ctx_buf<float>(c, "v", nn), *dw = ctx_buf<float>(c, "w", nn);
+    float *dcfu = ctx_buf<float>(c, "cfu", ncfu), *dcfv = ctx_buf<float>(c, "cfv", ncfv);
+    float2 *dg = ctx_buf<float2>(c, "grid", ng);
+    unsigned long long *cnt = ctx_buf<unsigned long long>(c, "counters", 2);
+    if (!dvis || !dwt || !du || !dv || !dw || !dcfu || !dcfv || !dg || !cnt) {
+        report_error(kErrNoMem, R, "device allocation failed");
+        return;
+    }
+    if (n > 0) {
+        SDE_F(hipMemcpyAsync(dvis, vis, sizeof(float2) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(dwt, wt, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(du, u, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(dv, v, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(dw, w, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    }
+    SDE_F(hipMemcpyAsync(dcfu, intfnu, sizeof(float) * ncfu, hipMemcpyHostToDevice, s));
+    SDE_F(hipMemcpyAsync(dcfv, intfnv, sizeof(float) * ncfv, hipMemcpyHostToDevice, s));
+    SDE_F(hipMemcpyAsync(dg, gvis, sizeof(float2) * ng, hipMemcpyHostToDevice, s));
+    SDE_F(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * 2, s));
+    launch_degrid_slowz(s, dg, du, dv, dw, dwt, dvis, n, cp, dcfu, dcfv, pp, dvis, dwt, cnt);
     SDE_F(hipGetLastError());
     unsigned long long nbad = 0;
     if (n > 0) {

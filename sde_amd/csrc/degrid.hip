@@ -114,6 +114,130 @@ __global__ __launch_bounds__(256) void k_degrid(
     if (i < nvis) degrid_point(i, grid, u, v, w, wt_in, vis_in, cp, cfu, cfv, vis_out, wt_out, nbad);
 }
 
+// GRDDEH23 (grd/grddeh23.f:86-210), one thread per visibility, 64 per
+// workgroup: the point's NZ plane rotations CROTZ are computed once into
+// its own LDS column (plane-major, so the 64 lanes of a plane read 64
+// consecutive entries), then the taps are gathered in the reference's loop
+// order -- OFFU outer near the u = 0 axis, OFFV outer elsewhere, the planes
+// innermost -- in its REAL arithmetic: (GVIS * CROTZ) * UVWT near the axis
+// on the conjugating side, (GVIS * UVWT) * CROTZ elsewhere.  A tap on a
+// reflected row outside the grid (which the reference reads out of bounds)
+// adds nothing.
+__global__ __launch_bounds__(64) void k_degrid_slowz(
+    const float2 *__restrict__ grid, const float *__restrict__ u, const float *__restrict__ v,
+    const float *__restrict__ w, const float *__restrict__ wt_in, const float2 *vis_in, int64_t nvis,
+    CoordParams cp, const float *__restrict__ cfu, const float *__restrict__ cfv, PlaneParams pp,
+    float2 *vis_out, float *__restrict__ wt_out, unsigned long long *__restrict__ nbad) {
+    extern __shared__ float2 crotz[];  // [NZ][64]
+    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (i >= nvis) return;
+    float wti = wt_in[i];
+    if (wti <= 0.0f) {
+        wt_out[i] = wti;
+        if (vis_in != vis_out) vis_out[i] = vis_in[i];
+        return;
+    }
+    const int cfoffu = (cp.suppu + 1) * cp.osampu + 1;
+    const int cfoffv = (cp.suppv + 1) * cp.osampv + 1;
+    const size_t plane = (size_t)cp.nu * cp.nv;
+    wti = -wti;
+    const float uf = u[i], vf = v[i], wf = w[i];
+    const double ug = uf, vg = vf, wg = wf;
+    const float ul = (float)(cp.shift[0] * ug + cp.shift[1] * vg + cp.shift[2] * wg);
+    const float vl = (float)(cp.shift[3] * ug + cp.shift[4] * vg + cp.shift[5] * wg);
+    const float wl = (float)(cp.shift[6] * ug + cp.shift[7] * vg + cp.shift[8] * wg);
+    const float phase = pp.twopi * (wl - wf);
+    const float rr = cosf(phase), ri = sinf(phase);
+    float ucell = cp.uscale * ul + cp.uoffset, vcell = cp.vscale * vl + cp.voffset;
+    float wz = wl;
+    if (ucell > 0.0f) {
+        ucell = ucell + (float)cp.uorigin;
+        vcell = vcell + (float)cp.vorigin;
+    } else {
+        ucell = -ucell + (float)cp.uorigin;
+        vcell = -vcell + (float)cp.vorigin;
+        wz = -wz;
+    }
+    const int nuc = nint_f(ucell), nvc = nint_f(vcell);
+    if (nuc + cp.suppu > cp.nu || nvc + cp.suppv > cp.nv || nvc - cp.suppv < 1) {
+        vis_out[i] = make_float2(0.0f, 0.0f);
+        wt_out[i] = wti;
+        return;
+    }
+    const int delui = nint_f((float)cp.osampu * ((float)nuc - ucell));
+    const int delvi = nint_f((float)cp.osampv * ((float)nvc - vcell));
+    const int ucen = nuc - cp.uorigin, vcen = nvc - cp.vorigin;
+    float2 *cz = crotz + threadIdx.x;
+    for (int iz = 1; iz <= pp.nz; iz++) {
+        const float ph = -(pp.twopi * wz * pp.dtor * pp.cellz * ((float)iz - pp.refz));
+        cz[(iz - 1) * 64] = make_float2(cosf(ph), sinf(ph));
+    }
+    float vr = 0.0f, vi = 0.0f, sumwt = 0.0f;
+    // (G * UVWT) * CROTZ of every plane of one tap, G read at cell (ui, vj)
+    auto taps_scaled = [&](int ui, int vj, float uvwt) {
+        const float2 *g = grid + (size_t)(vj - 1) * cp.nu + (ui - 1);
+        for (int iz = 0; iz < pp.nz; iz++) {
+            const float2 gv = g[iz * plane];
+            const float2 c = cz[iz * 64];
+            const float ar = gv.x * uvwt, ai = gv.y * uvwt;
+            vr = vr + (ar * c.x - ai * c.y);
+            vi = vi + (ar * c.y + ai * c.x);
+        }
+    };
+    if (nuc <= cp.suppu) {
+        for (int offu = -cp.suppu; offu <= cp.suppu; offu++) {
+            const float cu = cfu[delui + cp.osampu * offu + cfoffu - 1];
+            if (ucen != -offu) {
+                const int usign = (ucen + offu) >= 0 ? 1 : -1;
+                const int uint_ = cp.uorigin + usign * (ucen + offu);
+                for (int offv = -cp.suppv; offv <= cp.suppv; offv++) {
+                    const int vint = cp.vorigin + usign * (vcen + offv);
+                    const float uvwt = cu * cfv[delvi + cp.osampv * offv + cfoffv - 1];
+                    sumwt = sumwt + uvwt;
+                    const bool in = vint >= 1 && vint <= cp.nv && uint_ >= 1 && uint_ <= cp.nu;
+                    const float2 *g = grid + (in ? (size_t)(vint - 1) * cp.nu + (uint_ - 1) : 0);
+                    for (int iz = 0; iz < pp.nz; iz++) {
+                        float2 gv = in ? g[iz * plane] : make_float2(0.0f, 0.0f);
+                        if (usign < 0) gv.y = -gv.y;
+                        const float2 c = cz[iz * 64];
+                        // (FVIS * CROTZ) * UVWT
+                        const float pr = gv.x * c.x - gv.y * c.y, pi = gv.x * c.y + gv.y * c.x;
+                        vr = vr + pr * uvwt;
+                        vi = vi + pi * uvwt;
+                    }
+                }
+            } else {
+                for (int offv = -cp.suppv; offv <= cp.suppv; offv++) {
+                    const int vint = cp.vorigin + (vcen + offv);
+                    const float uvwt = cu * cfv[delvi + cp.osampv * offv + cfoffv - 1];
+                    sumwt = sumwt + uvwt;
+                    taps_scaled(cp.uorigin, vint, uvwt);
+                }
+            }
+        }
+    } else {
+        for (int offv = -cp.suppv; offv <= cp.suppv; offv++) {
+            const float cv = cfv[delvi + cp.osampv * offv + cfoffv - 1];
+            const int vint = cp.vorigin + vcen + offv;
+            for (int offu = -cp.suppu; offu <= cp.suppu; offu++) {
+                const float uvwt = cfu[delui + cp.osampu * offu + cfoffu - 1] * cv;
+                sumwt = sumwt + uvwt;
+                taps_scaled(cp.uorigin + ucen + offu, vint, uvwt);
+            }
+        }
+    }
+    if (sumwt > 0.0f) {
+        cdiv_by_real(vr, vi, sumwt);
+        wti = fabsf(wti);
+    } else {
+        atomicAdd(nbad, 1ull);
+        vr = vi = 0.0f;
+    }
+    if (cp.uscale * ul > 0.0f) vi = -vi;
+    vis_out[i] = make_float2(vr * rr - vi * ri, vr * ri + vi * rr);
+    wt_out[i] = wti;
+}
+
 // The points of a segmented list (segment b holds counts[b] indices from
 // slot b*seg_cap), one workgroup per segment.
 __global__ __launch_bounds__(256) void k_degrid_list(
@@ -364,6 +488,16 @@ void launch_degrid(hipStream_t s, const float2 *grid, const float *u, const floa
     const int64_t nb = (nvis + 255) / 256;
     hipLaunchKernelGGL(k_degrid, dim3((unsigned)nb), dim3(256), 0, s, grid, u, v, w, wt_in, vis_in,
                        nvis, cp, cfu, cfv, vis_out, wt_out, nbad);
+}
+
+void launch_degrid_slowz(hipStream_t s, const float2 *grid, const float *u, const float *v, const float *w,
+                         const float *wt_in, const float2 *vis_in, int64_t nvis, const CoordParams &cp,
+                         const float *cfu, const float *cfv, const PlaneParams &pp, float2 *vis_out,
+                         float *wt_out, unsigned long long *nbad) {
+    if (nvis <= 0) return;
+    const int64_t nb = (nvis + 63) / 64;
+    hipLaunchKernelGGL(k_degrid_slowz, dim3((unsigned)nb), dim3(64), sizeof(float2) * 64 * (size_t)pp.nz, s, grid,
+                       u, v, w, wt_in, vis_in, nvis, cp, cfu, cfv, pp, vis_out, wt_out, nbad);
 }
 
 void launch_degrid_list(hipStream_t s, const float2 *grid, const float *u, const float *v, const float *w,

@@ -790,6 +790,102 @@ __global__ void k_box_fold(const double *__restrict__ acc, CoordParams cp,
     g[(size_t)iv * cp.nu + iu] = out;
 }
 
+// ---- w planes (GRDH23D) ----
+// REAL <- DOUBLE PRECISION SHIFT times REAL coordinates: the products and
+// sums in DOUBLE, rounded once (grd/grdh23d.f:100-105).
+__device__ __forceinline__ float3 local_uvw(const CoordParams &cp, float uf, float vf, float wf) {
+    const double ug = uf, vg = vf, wg = wf;
+    return make_float3((float)(cp.shift[0] * ug + cp.shift[1] * vg + cp.shift[2] * wg),
+                       (float)(cp.shift[3] * ug + cp.shift[4] * vg + cp.shift[5] * wg),
+                       (float)(cp.shift[6] * ug + cp.shift[7] * vg + cp.shift[8] * wg));
+}
+
+// Prep pass of GRDH23D, one thread per visibility in the caller's order
+// (grd/grdh23d.f:98-134): the shift rotation of VIS, the REAL cell step and
+// drop test, the reflection (conjugate VIS for u > 0, negate WLOCAL for the
+// reflected half), the PSF's VIS = 1, the cell key and the hot record's
+// cell and kernel phases.  No density cell, no degridding record.
+__global__ __launch_bounds__(kThreads) void k_prep_slowz(
+    const float2 *__restrict__ vis, const float *__restrict__ wt, const float *__restrict__ u,
+    const float *__restrict__ v, const float *__restrict__ w, int64_t n, CoordParams cp, TileParams tp,
+    uint32_t sentinel, int psf, float twopi, uint32_t *__restrict__ keys, uint64_t *__restrict__ vals,
+    StageHot *__restrict__ hot, float4 *__restrict__ zbase, double *__restrict__ blk) {
+    const int poffu = (cp.osampu + 1) / 2, poffv = (cp.osampv + 1) / 2;
+    double sw = 0.0, zero = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        const float wti = wt[i];
+        if (wti > 0.0f) sw += (double)wti;  // SUMWT counts the dropped points too (grd/grdh23d.f:98-100)
+        StageHot sr;
+        sr.gpk = 0u;
+        sr.fr = 0.0f;
+        sr.fi = 0.0f;
+        sr.dpk = 0u;
+        float4 zb = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        uint32_t key = sentinel;
+        if (wti > 0.0f) {
+            const float wf = w[i];
+            const float3 l = local_uvw(cp, u[i], v[i], wf);
+            // FVIS = VIS * CMPLX(COS(PHASE), -SIN(PHASE))
+            const float phase = twopi * (l.z - wf);
+            const float cr = cosf(phase), ci = -sinf(phase);
+            const float2 vv = vis[i];
+            float fr = vv.x * cr - vv.y * ci, fi = vv.x * ci + vv.y * cr;
+            float ucell = cp.uscale * l.x + cp.uoffset, vcell = cp.vscale * l.y + cp.voffset;
+            float wz = l.z;
+            if (ucell > 0.0f) {
+                fi = -fi;
+                ucell = ucell + (float)cp.uorigin;
+                vcell = vcell + (float)cp.vorigin;
+            } else {
+                ucell = -ucell + (float)cp.uorigin;
+                vcell = -vcell + (float)cp.vorigin;
+                wz = -wz;
+            }
+            if (psf) {
+                fr = 1.0f;
+                fi = 0.0f;
+            }
+            const int nuc = nint_f(ucell), nvc = nint_f(vcell);
+            // footprint below the grid (only with UORIGIN < 1): dropped
+            if (!(nuc + 2 * cp.suppu > cp.nu || nvc + cp.suppv > cp.nv || nvc - cp.suppv < 1 || nuc < 1)) {
+                key = (uint32_t)cell_key(tp, nuc + cp.suppu - 1, nvc - 1);
+                const uint32_t local = key & (uint32_t)(tp.T * tp.T - 1);
+                const int pu = nint_f((float)cp.osampu * ((float)nuc - ucell)) + poffu;
+                const int pv = nint_f((float)cp.osampv * ((float)nvc - vcell)) + poffv;
+                sr.gpk = (local % (uint32_t)tp.T) | ((local / (uint32_t)tp.T) << 8) | ((uint32_t)pu << 16) |
+                         ((uint32_t)pv << 24);
+                sr.fr = fr;
+                sr.fi = fi;
+                zb = make_float4(fr, fi, wz, 0.0f);
+            }
+        }
+        hot[i] = sr;
+        zbase[i] = zb;
+        keys[i] = key;
+        vals[i] = (uint64_t)(uint32_t)i | ((uint64_t)__float_as_uint(wti) << 32);
+    }
+    block_sum2(sw, zero);
+    if (threadIdx.x == 0) {
+        blk[2 * blockIdx.x] = sw;
+        blk[2 * blockIdx.x + 1] = 0.0;
+    }
+}
+
+// ROTFVIS of plane iz for every point (grd/grdh23d.f:154-156), written into
+// the visibility of its hot record; the cell and phases stay.
+__global__ __launch_bounds__(kThreads) void k_plane_phase(const float4 *__restrict__ zbase,
+                                                          StageHot *__restrict__ hot, int64_t n, PlaneParams pp,
+                                                          float dz) {
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        const float4 zb = zbase[i];
+        const float phase = -(pp.twopi * zb.z * pp.dtor * pp.cellz * dz);
+        const float cr = cosf(phase), ci = -sinf(phase);
+        float *rec = reinterpret_cast<float *>(hot + i);
+        rec[1] = zb.x * cr - zb.y * ci;
+        rec[2] = zb.x * ci + zb.y * cr;
+    }
+}
+
 int blocks_for(int64_t n) {
     int64_t b = (n + kThreads - 1) / kThreads;
     if (b < 1) b = 1;
@@ -960,6 +1056,23 @@ void launch_grid_reduce(hipStream_t s, const float *partial, const int *item_sta
                         int chmask, const CoordParams &cp, const double *sums, float2 *dirty, float2 *psf) {
     dim3 grid((cp.nu + 255) / 256, cp.nv);
     hipLaunchKernelGGL(k_grid_reduce, grid, dim3(256), 0, s, partial, item_start, tp, chmask, cp, sums, dirty, psf);
+}
+
+void launch_prep_slowz(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
+                       const float *w, int64_t n, const CoordParams &cp, const TileParams &tp, int psf,
+                       const PlaneParams &pp, SortBufs &b, float4 *zbase, double *blk, double *sumwt) {
+    const int nb = prep_blocks(n > 0 ? n : 1);
+    if (n > 0)
+        hipLaunchKernelGGL(k_prep_slowz, dim3(nb), dim3(kThreads), 0, s, vis, wt, u, v, w, n, cp, tp,
+                           (uint32_t)sort_nkeys(tp), psf, pp.twopi, b.keys, b.vals, b.hot, zbase, blk);
+    hipLaunchKernelGGL(k_grid_totals, dim3(1), dim3(1024), 0, s, blk, n > 0 ? nb : 0, 0.0, sumwt);
+}
+
+void launch_plane_phase(hipStream_t s, const float4 *zbase, StageHot *hot, int64_t n, const PlaneParams &pp,
+                        int iz) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_plane_phase, dim3(prep_blocks(n)), dim3(kThreads), 0, s, zbase, hot, n, pp,
+                       (float)iz - pp.refz);
 }
 
 void launch_box_grid(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,

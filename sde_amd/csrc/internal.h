@@ -464,6 +464,34 @@ void launch_wt_apply(hipStream_t s, const float *wt, const float *u, const float
                      float *newwt, unsigned long long *ndrop);
 void launch_gwt_out(hipStream_t s, const double *gwt, float *out, size_t n);
 
+// ---- w planes: the slow transform in Z (GRDH23D, GRDDEH23) ----
+// Plane iz (1-based) of NZ holds the visibilities rotated by the phase
+// -TWOPI*W*DTOR*CELLZ*(IZ-REFZ) of its offset in n (grd/grdh23d.f:154-156,
+// grd/grddeh23.f:121-125); every quantity here is the reference's REAL.
+struct PlaneParams {
+    float twopi = 0.0f, dtor = 0.0f, cellz = 0.0f, refz = 0.0f;
+    int nz = 1;
+};
+constexpr int kMaxPlanes = 128;  // the degridder keeps 64 x NZ rotations in LDS (64 KB)
+// Keys, values and hot staging records of GRDH23D's per-point step in its
+// REAL coordinates (grd/grdh23d.f:98-130): the hot record carries FVIS of
+// plane 1 until launch_plane_phase rewrites it; zbase[i] = (FVIS, WLOCAL
+// after the reflection, 0).  SUMWT, the weights of every point with WT > 0
+// (dropped or not), goes to sumwt[0] (blk: 2 * prep_blocks(n) doubles).
+void launch_prep_slowz(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
+                       const float *w, int64_t n, const CoordParams &cp, const TileParams &tp, int psf,
+                       const PlaneParams &pp, SortBufs &b, float4 *zbase, double *blk, double *sumwt);
+// The hot records' visibility for plane iz: ROTFVIS = FVIS * CMPLX(COS, -SIN)
+// of the plane's phase (grd/grdh23d.f:154-156).
+void launch_plane_phase(hipStream_t s, const float4 *zbase, StageHot *hot, int64_t n, const PlaneParams &pp,
+                        int iz);
+// GRDDEH23, one thread per visibility, in the reference's loop order and
+// REAL arithmetic (grd/grddeh23.f:86-210); grid holds NZ planes of NU x NV.
+void launch_degrid_slowz(hipStream_t s, const float2 *grid, const float *u, const float *v, const float *w,
+                         const float *wt_in, const float2 *vis_in, int64_t nvis, const CoordParams &cp,
+                         const float *cfu, const float *cfv, const PlaneParams &pp, float2 *vis_out,
+                         float *wt_out, unsigned long long *nbad);
+
 // ---- plain copy of n float4 (bandwidth probe, fft.hip) ----
 void launch_copy(hipStream_t s, const float4 *src, float4 *dst, size_t n);
 

@@ -140,6 +140,50 @@ def grddeh2d(vis, wt, u, v, w, uscale, uoffset, vscale, voffset, uorigin, vorigi
     return vis, wt
 
 
+def grdh23d(vis, wt, u, v, w, uscale, uoffset, vscale, voffset, uorigin, vorigin, psf, nu, nv,
+            gridfnu, suppu, osampu, nz, cellz, refz, gridfnv=None, suppv=None, osampv=None, shift=IDENTITY):
+    """GRDH23D (grd/grdh23d.f:7-203): grid onto NZ planes of a slow
+    transform in Z, plane IZ at the n offset DTOR*CELLZ*(IZ-REFZ).
+
+    Returns (gvis (NZ,NV,NU) complex64, sumwt)."""
+    vis, wt, u, v, w = _c64(vis), _f32(wt), _f32(u), _f32(v), _f32(w)
+    gridfnu = _f32(gridfnu)
+    gridfnv = gridfnu if gridfnv is None else _f32(gridfnv)
+    suppv = suppu if suppv is None else suppv
+    osampv = osampu if osampv is None else osampv
+    g = np.zeros((int(nz), int(nv), int(nu)), np.complex64)
+    sumwt = C.c_float(0.0)
+    lib().grdh23d_(_pf(vis), _pf(wt), _pf(u), _pf(v), _pf(w), _ri(len(u)), _rf(uscale), _rf(uoffset), _rf(vscale),
+                   _rf(voffset), _ri(uorigin), _ri(vorigin), _pf(g), _ri(1 if psf else 0), _ri(nu), _ri(nv),
+                   _pf(gridfnu), _ri(suppu), _ri(osampu), _pf(gridfnv), _ri(suppv), _ri(osampv), _ri(nz),
+                   _rf(cellz), _rf(refz), _pd(_shift(shift)), C.byref(sumwt))
+    check_error("GRDH23D")
+    return g, sumwt.value
+
+
+def grddeh23(vis, wt, u, v, w, uscale, uoffset, vscale, voffset, uorigin, vorigin, gvis, intfnu, suppu,
+             osampu, cellz, refz, intfnv=None, suppv=None, osampv=None, shift=IDENTITY):
+    """GRDDEH23 (grd/grddeh23.f:7-219): degrid from the NZ planes gvis
+    (NZ,NV,NU).  VIS and WT as grddeh2d.
+
+    Returns (vis complex64, wt float32) as new arrays."""
+    vis = _c64(vis).copy()
+    wt = _f32(wt).copy()
+    u, v, w = _f32(u), _f32(v), _f32(w)
+    gvis = _c64(gvis)
+    nz, nv, nu = gvis.shape
+    intfnu = _f32(intfnu)
+    intfnv = intfnu if intfnv is None else _f32(intfnv)
+    suppv = suppu if suppv is None else suppv
+    osampv = osampu if osampv is None else osampv
+    lib().grddeh23_(_pf(vis), _pf(wt), _pf(u), _pf(v), _pf(w), _ri(len(u)), _rf(uscale), _rf(uoffset),
+                    _rf(vscale), _rf(voffset), _ri(uorigin), _ri(vorigin), _pf(gvis), _ri(nu), _ri(nv),
+                    _pf(intfnu), _ri(suppu), _ri(osampu), _pf(intfnv), _ri(suppv), _ri(osampv), _ri(nz),
+                    _rf(cellz), _rf(refz), _pd(_shift(shift)))
+    check_error("GRDDEH23")
+    return vis, wt
+
+
 def grduwt2d(wt, u, v, w, uscale, uoffset, vscale, voffset, uorigin, vorigin, nu, nv, shift=IDENTITY,
              newwt=None):
     """GRDUWT2D (grd/grduwt2d.f:7-170): uniform weights.  Returns (newwt, gwt)."""

@@ -36,7 +36,7 @@ def lib():
 
 def test_header_declares_the_reference_interface():
     names = declared_functions()
-    for f in ("grdh2d_", "grdqh2d_", "grddeh2d_", "grduwt2d_", "grdruw2d_", "grdcf_", "grdccf_", "pixcor2d_",
+    for f in ("grdh2d_", "grdqh2d_", "grddeh2d_", "grdh23d_", "grddeh23_", "grduwt2d_", "grdruw2d_", "grdcf_", "grdccf_", "pixcor2d_",
               "pixcorx2_", "fftr2d_", "fftx1d_", "fftpwr2_", "sde_plan_create", "sde_plan_uvmap",
               "sde_plan_vismodel"):
         assert f in names, f
@@ -102,3 +102,32 @@ def test_unknown_kernel_sets_the_error_flag(lib):
     assert np.array_equal(cf, before)
     lib.sde_error_clear()
     assert lib.sde_error_flag() == 0
+
+
+@pytest.mark.parametrize("nz", [0, 129])
+def test_w_plane_routines_reject_plane_counts_before_any_device_work(lib, nz):
+    """GRDH23D / GRDDEH23 check NZ (1..128) on the host and set ERROR with
+    their outputs untouched; no device is touched on this path."""
+    lib.sde_error_flag.restype = C.c_int
+    lib.sde_last_error.restype = C.c_char_p
+    cf, supp, osamp = po.grdcf("SF")
+    n = 4
+    f = np.ones(n, np.float32)
+    vis = np.ones(n, np.complex64)
+    g = np.full(17 * 32 * max(nz, 1), 5 + 5j, np.complex64)
+    P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    I = lambda x: C.byref(C.c_int(x))  # noqa: E731
+    R = lambda x: C.byref(C.c_float(x))  # noqa: E731
+    shift = np.eye(3).reshape(-1)
+    sw = C.c_float(-1.0)
+    lib.sde_error_clear()
+    lib.grdh23d_(P(vis), P(f), P(f), P(f), P(f), I(n), R(1), R(0), R(1), R(0), I(1), I(16), P(g), I(0), I(17), I(32),
+                 P(cf), I(supp), I(osamp), P(cf), I(supp), I(osamp), I(nz), R(1.0), R(1.0), P(shift), C.byref(sw))
+    assert lib.sde_error_flag() == 1 and b"NZ=%d" % nz in lib.sde_last_error()
+    assert sw.value == -1.0 and np.all(g == 5 + 5j)
+    lib.sde_error_clear()
+    lib.grddeh23_(P(vis), P(f), P(f), P(f), P(f), I(n), R(1), R(0), R(1), R(0), I(1), I(16), P(g), I(17), I(32),
+                  P(cf), I(supp), I(osamp), P(cf), I(supp), I(osamp), I(nz), R(1.0), R(1.0), P(shift))
+    assert lib.sde_error_flag() == 1 and b"GRDDEH23" in lib.sde_last_error()
+    assert np.all(vis == 1) and np.all(f == 1)
+    lib.sde_error_clear()

@@ -1,0 +1,142 @@
+"""Parity of the w-plane (slow transform in Z) drop-ins GRDH23D and
+GRDDEH23 with the oracle, through the C-ABI.
+
+Gridding: every plane within 1e-5 of the largest FP64 grid value (the
+FP64 oracle o64_grdh23d, same REAL cell decisions).  The reference's own
+single-precision output drifts further (its REAL running LSUMWT over
+NZ*NVIS*(2*SUPP+1)**2 taps), so the FP64 oracle is the target.
+Degridding: the per-point kernel runs the reference's loop order in REAL
+arithmetic: WT flags identical to the FP32 restatement, VIS within 1e-5
+of the largest FP64 value.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from oracle import refwrap as rw
+from tests.tolerance import assert_close_rel
+
+pytestmark = pytest.mark.gpu
+
+sde = pytest.importorskip("sde_amd")
+from sde_amd import grd  # noqa: E402
+
+IDENT = np.eye(3).reshape(-1)
+
+
+def _points(n, seed, umax, vmax, near_axis=0, quantize=False):
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(-umax, umax, n)
+    v = rng.uniform(-vmax, vmax, n)
+    if near_axis:
+        u[:near_axis] = rng.uniform(-3.5, 3.5, near_axis)
+    if quantize:  # multiples of 1/1024: REAL and DOUBLE cell steps agree
+        u, v = np.round(u * 1024) / 1024, np.round(v * 1024) / 1024
+    w = rng.uniform(-8, 8, n)
+    vis = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    wt = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    wt[::23] = -1.0
+    return vis, wt, u.astype(np.float32), v.astype(np.float32), w.astype(np.float32)
+
+
+def _shift(shifted):
+    return np.asarray(po.crdshift(0.3, 0.6, 0.3002, 0.6001), np.float64).reshape(-1) if shifted else IDENT
+
+
+@pytest.mark.parametrize("psf", [False, True])
+@pytest.mark.parametrize("shifted", [False, True])
+def test_grdh23d_matches_fp64(psf, shifted):
+    nu, nv, vo, nz, cellz, refz = 65, 128, 64, 5, 3.0, 2.5
+    vis, wt, u, v, w = _points(4000, 41, 40, 55, near_axis=400)
+    cf, supp, osamp = po.grdcf("SF")
+    cf64, _, _ = po.grdcf("SF", fp64=True)
+    sh = _shift(shifted)
+    print("grdh23d psf=%s shifted=%s n=%d nu=%d nv=%d nz=%d" % (psf, shifted, len(u), nu, nv, nz), flush=True)
+    g, sw = grd.grdh23d(vis, wt, u, v, w, 1.0, 0.0, 1.0, 0.0, 1, vo, psf, nu, nv, cf, supp, osamp, nz, cellz, refz,
+                        shift=sh)
+    g64, sw64, rc = po.grdh23d64(vis, wt, u, v, w, 1.0, 1.0, 1, vo, psf, nu, nv, cf64, supp, osamp, nz, cellz, refz,
+                                 sh)
+    assert rc == 0
+    assert sw == pytest.approx(sw64, rel=1e-6)
+    for iz in range(nz):
+        assert_close_rel(g[iz], g64[iz], "plane %d" % (iz + 1), rel=1e-5 * np.abs(g64).max() / np.abs(g64[iz]).max())
+    g32, _, _ = po.grdh23d(vis, wt, u, v, w, 1.0, 1.0, 1, vo, psf, nu, nv, cf, supp, osamp, nz, cellz, refz, sh)
+    scale = np.abs(g64).max()
+    print("rel err gpu %.2e ref32 %.2e" % (np.abs(g - g64).max() / scale, np.abs(g32 - g64).max() / scale))
+
+
+def test_grdh23d_larger_grid_and_drops():
+    nu, nv, vo, nz = 257, 512, 256, 8
+    vis, wt, u, v, w = _points(60000, 42, 270, 270, near_axis=2000)  # some footprints leave the grid
+    cf, supp, osamp = po.grdcf("SF")
+    cf64, _, _ = po.grdcf("SF", fp64=True)
+    sh = _shift(True)
+    print("grdh23d n=%d nu=%d nv=%d nz=%d" % (len(u), nu, nv, nz), flush=True)
+    g, sw = grd.grdh23d(vis, wt, u, v, w, 1.0, 0.0, 1.0, 0.0, 1, vo, False, nu, nv, cf, supp, osamp, nz, 1.0, 4.0,
+                        shift=sh)
+    g64, sw64, rc = po.grdh23d64(vis, wt, u, v, w, 1.0, 1.0, 1, vo, False, nu, nv, cf64, supp, osamp, nz, 1.0, 4.0,
+                                 sh)
+    assert rc == 0 and sw == pytest.approx(sw64, rel=1e-6)
+    assert_close_rel(g, g64, "cube")
+
+
+def test_grdh23d_single_plane_is_grdh2d():
+    """NZ = 1 at REFZ = 1 is GRDH2D (the plane phase is zero and the
+    normalisations agree) where the REAL and DOUBLE cell steps agree."""
+    nu, nv, vo = 129, 256, 128
+    vis, wt, u, v, w = _points(20000, 43, 120, 120, near_axis=1000, quantize=True)
+    cf, supp, osamp = po.grdcf("SF")
+    g3, sw3 = grd.grdh23d(vis, wt, u, v, w, 1.0, 0.0, 1.0, 0.0, 1, vo, False, nu, nv, cf, supp, osamp, 1, 2.0, 1.0)
+    g2, sw2 = grd.grdh2d(vis, wt, u, v, w, 1.0, 0.0, 1.0, 0.0, 1, vo, False, nu, nv, cf, supp, osamp)
+    assert sw3 == sw2
+    assert np.abs(g3[0] - g2).max() <= 1e-6 * np.abs(g2).max()
+
+
+def test_grdh23d_no_weight_and_plane_limits():
+    cf, supp, osamp = po.grdcf("SF")
+    n = 10
+    z = np.zeros(n, np.float32)
+    with pytest.raises(sde.SdeError, match="No weight"):
+        grd.grdh23d(np.ones(n, np.complex64), -np.ones(n, np.float32), z, z, z, 1.0, 0.0, 1.0, 0.0, 1, 16, False, 17,
+                    32, cf, supp, osamp, 3, 1.0, 2.0)
+    with pytest.raises(sde.SdeError, match="NZ"):
+        grd.grdh23d(np.ones(n, np.complex64), np.ones(n, np.float32), z, z, z, 1.0, 0.0, 1.0, 0.0, 1, 16, False, 17,
+                    32, cf, supp, osamp, 0, 1.0, 2.0)
+
+
+@pytest.mark.parametrize("shifted", [False, True])
+@pytest.mark.parametrize("nz,cellz,refz", [(4, 3.0, 2.0), (9, 1.5, 5.0)])
+def test_grddeh23_matches_reference_arithmetic(shifted, nz, cellz, refz):
+    nu, nv, vo = 65, 128, 64
+    vis, wt, u, v, w = _points(5000, 44, 70, 60, near_axis=600)  # |u| up to 70: some points off the grid
+    rng = np.random.default_rng(45)
+    g64 = rng.standard_normal((nz, nv, nu)) + 1j * rng.standard_normal((nz, nv, nu))
+    g32 = g64.astype(np.complex64)
+    cf, supp, osamp = po.grdcf("SF")
+    cf64, _, _ = po.grdcf("SF", fp64=True)
+    sh = _shift(shifted)
+    vin = np.full(len(u), 3 + 4j, np.complex64)
+    print("grddeh23 shifted=%s n=%d nz=%d" % (shifted, len(u), nz), flush=True)
+    vg, wg = grd.grddeh23(vin, wt, u, v, w, 1.0, 0.0, 1.0, 0.0, 1, vo, g32, cf, supp, osamp, cellz, refz, shift=sh)
+    vr, wr, nbad = po.grddeh23(wt, u, v, w, 1.0, 1.0, 1, vo, g32, cf, supp, osamp, cellz, refz, sh)
+    v64, w64, _ = po.grddeh23_64(wt, u, v, w, 1.0, 1.0, 1, vo, g32.astype(np.complex128), cf64, supp, osamp, cellz,
+                                 refz, sh)
+    flagged = wt <= 0
+    assert np.all(vg[flagged] == vin[flagged])
+    np.testing.assert_array_equal(wg, wr)
+    assert np.any(wg < 0) and np.any((wg < 0) & ~flagged)  # points off the grid keep the negated flag
+    good = wg > 0
+    assert_close_rel(vg[good], v64[good], "degrid vs FP64")
+    # the device's REAL cos/sin may differ from the host's by an ulp, so not
+    # every value is the restatement's bit for bit; the error stays that of
+    # the reference's own REAL arithmetic
+    same = np.mean(vg[good] == vr[good])
+    scale = np.abs(v64[good]).max()
+    eg = np.abs(vg[good] - v64[good]).max() / scale
+    er = np.abs(vr[good] - v64[good]).max() / scale
+    print("identical to the REAL restatement: %.3f of points; rel err gpu %.2e ref32 %.2e" % (same, eg, er))
+    assert eg <= 2 * er + 1e-7
+    if rw.available():
+        va, wa = rw.grddeh23(wt, u, v, w, 1.0, 1.0, 1, vo, g32, cf, supp, osamp, cellz, refz, sh)
+        np.testing.assert_array_equal(wa, wr)
+        assert np.array_equal(va[good], vr[good])
