@@ -308,6 +308,106 @@ def bench_major_cycle(args):
                        "nvis": nvis}, "kernels_ms": kernels}
 
 
+def wplane_cpu_baseline(vis, wt, u, v, w, geo, cf, supp, osamp, nz, cellz, refz, nfull):
+    """GRDH23D + GRDDEH23 on one host core, on two bounded samples of the
+    w-plane workload, fitted and extrapolated as cpu_baseline: the
+    reference's own routines compiled from source (oracle/_ref) when built,
+    else the C restatement."""
+    from oracle import pyoracle as po
+    from oracle import refwrap as rw
+    kind = "reference" if rw.available() else "port"
+    nu, nv, vo, us, vs = geo
+    ident = np.eye(3).reshape(-1)
+    sizes = (250_000, 2_000_000)
+    t = []
+    for n in sizes:
+        t0 = time.perf_counter()
+        if kind == "reference":
+            g, _ = rw.grdh23d(vis[:n], wt[:n], u[:n], v[:n], w[:n], us, vs, 1, vo, False, nu, nv, cf, supp, osamp,
+                              nz, cellz, refz, ident)
+            rw.grddeh23(wt[:n], u[:n], v[:n], w[:n], us, vs, 1, vo, g, cf, supp, osamp, cellz, refz, ident)
+        else:
+            g, _, _ = po.grdh23d(vis[:n], wt[:n], u[:n], v[:n], w[:n], us, vs, 1, vo, False, nu, nv, cf, supp,
+                                 osamp, nz, cellz, refz, ident)
+            po.grddeh23(wt[:n], u[:n], v[:n], w[:n], us, vs, 1, vo, g, cf, supp, osamp, cellz, refz, ident)
+        t.append(time.perf_counter() - t0)
+        del g
+    per_vis = (t[1] - t[0]) / (sizes[1] - sizes[0])
+    fixed = max(t[0] - per_vis * sizes[0], 0.0)
+    t_full = fixed + per_vis * nfull
+    what = ("the reference's GRDH23D and GRDDEH23 compiled from source (flang -O2, oracle/_ref)"
+            if kind == "reference" else "oracle/ref32.c")
+    return {"value": 2.0 * nfull / t_full, "unit": "visibilities/s", "cores": 1, "kind": kind,
+            "sample": "%s on the first %d and %d visibilities: %.2f s and %.2f s; fitted %.3f us/visibility + "
+                      "%.2f s fixed, extrapolated to %d visibilities" % (what, sizes[0], sizes[1], t[0], t[1],
+                                                                        per_vis * 1e6, fixed, nfull)}
+
+
+def bench_wplanes(args):
+    """W-plane imaging, the slow transform in Z (GRDH23D / GRDDEH23): the
+    large-field tracks subsampled to --wvis visibilities, gridded onto NZ
+    planes of the 4096^2 field's 2049 x 4096 grid and degridded back from
+    them, on device-resident data (natural weights, identity shift)."""
+    from sde_amd import DeviceArray, grd, imaging, synth
+    p = synth.large_field_params(nkeep=args.wvis)
+    nx, cell = p["nx"], p["cellsize"]
+    plan = imaging.Plan(nx, nx, cell, "SF", weighting=0, fov=1.0)
+    plan.synth(p["enu"], p["lat"], p["dec"], p["ha0"], p["dha"], p["ntime"], p["freqs"], p["flux"], p["l"], p["m"],
+               p["sigma"], p["seed"], p["nkeep"])
+    vis, wt, u, v, w = plan.get_vis()
+    geo = (plan.nu, plan.nv, plan.vorigin, plan.uscale, plan.vscale)
+    plan.close()
+    nu, nv, vo, us, vs = geo
+    nvis = len(u)
+    nz = args.planes
+    refz = 0.5 * (nz + 1)
+    # planes spanning the n offset of the field corner, 1 - sqrt(1 - l^2 - m^2)
+    half = 0.5 * nx * cell / 3600.0 * math.pi / 180.0
+    nmax = 1.0 - math.sqrt(1.0 - 2.0 * half * half)
+    cellz = float(np.float32(nmax / (0.5 * nz) / (math.pi / 180.0)))
+    cf, supp, osamp = grd.grdcf("SF")
+    tv, tw, tu, tvv, tww = (DeviceArray.from_numpy(a) for a in (vis, wt, u, v, w))
+    tcf = DeviceArray.from_numpy(cf)
+    g = DeviceArray((nz, nv, nu), np.complex64)
+    vout, wout = DeviceArray.from_numpy(vis), DeviceArray.from_numpy(wt)
+
+    def grid():
+        return grd.grdh23d_device(tv, tw, tu, tvv, tww, us, 0.0, vs, 0.0, 1, vo, False, g, tcf, supp, osamp, cellz,
+                                  refz)
+
+    def degrid():
+        wout.copy_from(tw)  # the same points every step (failures negate WT)
+        t0 = time.perf_counter()
+        nbad = grd.grddeh23_device(vout, wout, tu, tvv, tww, us, 0.0, vs, 0.0, 1, vo, g, tcf, supp, osamp, cellz,
+                                   refz)
+        return time.perf_counter() - t0, nbad
+
+    for _ in range(args.warmup):
+        grid()
+        degrid()
+    tg, td = [], []
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        sumwt = grid()
+        tg.append(time.perf_counter() - t0)
+        dt, nbad = degrid()
+        td.append(dt)
+        print("# step %d grid %.1f ms degrid %.1f ms" % (k, tg[-1] * 1e3, dt * 1e3), file=sys.stderr, flush=True)
+    t = float(np.mean(tg) + np.mean(td))
+    out = {"metric": "w-plane visibilities/sec (GRDH23D grid + GRDDEH23 degrid, %d planes)" % nz,
+           "value": round(2.0 * nvis / t, 1), "unit": "visibilities/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(1e3 * t, 3), "higher_is_better": True, "dtype": "f32",
+           "data": "synthetic (seeded large-field tracks, subsampled)",
+           "config": {"workload": "w planes: %d vis of the large-field tracks, 2049 x 4096 grid x %d planes, SF 7x7 "
+                                  "x128, natural weights" % (nvis, nz), "nvis": nvis, "nz": nz,
+                      "cellz_deg": cellz, "refz": refz},
+           "grid_ms": round(1e3 * float(np.mean(tg)), 3), "degrid_ms": round(1e3 * float(np.mean(td)), 3),
+           "run_info": {"sumwt": sumwt, "nbad": nbad}}
+    if not args.no_cpu:
+        out["cpu_baseline"] = wplane_cpu_baseline(vis, wt, u, v, w, geo, cf, supp, osamp, nz, cellz, refz, nvis)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -323,12 +423,14 @@ def main():
                     help="robust NORM R=0 (the benchmark's configuration) or natural weights")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy first (HBM counter calibration under rocprofv3)")
-    ap.add_argument("--workload", choices=("large_field", "mosaic", "predict", "served", "major_cycle"),
+    ap.add_argument("--workload", choices=("large_field", "mosaic", "predict", "served", "major_cycle", "wplanes"),
                     default="large_field",
                     help="large_field: the headline job (default); the others are the batched and served forms")
     ap.add_argument("--jobs", type=int, default=100, help="served: number of requests")
     ap.add_argument("--rate", type=float, default=400.0, help="served: request rate (1/s)")
     ap.add_argument("--inflight", type=int, default=16, help="served: most jobs in flight")
+    ap.add_argument("--wvis", type=int, default=20_000_000, help="wplanes: visibilities")
+    ap.add_argument("--planes", type=int, default=8, help="wplanes: number of w planes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the imaging hot path runs on a single MI355X "
@@ -341,7 +443,7 @@ def main():
         raise SystemExit("no GPU visible")
     if args.workload != "large_field":
         fn = {"mosaic": bench_mosaic, "predict": bench_predict, "served": bench_served,
-              "major_cycle": bench_major_cycle}[args.workload]
+              "major_cycle": bench_major_cycle, "wplanes": bench_wplanes}[args.workload]
         print(json.dumps(fn(args)))
         return
 

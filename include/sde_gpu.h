@@ -223,6 +223,31 @@ int64_t sde_plan_device_bytes(sde_plan *plan);
 int64_t sde_plan_estimate_bytes(const sde_imaging_cfg *cfg, int64_t nvis);
 /* Free and total device memory (hipMemGetInfo). */
 int sde_device_memory(size_t *free_bytes, size_t *total_bytes);
+
+/* GRDH23D on device-resident arrays (the Fortran routine's arguments by
+ * value; VIS, WT, U, V, W, GVIS (NU*NV*NZ complex) and the kernel tables in
+ * device memory, SHIFT on the host), run on the library's stream and
+ * synchronised before return; SUMWT to *sumwt.  For callers that keep
+ * their visibilities on the GPU (the w-plane imaging of a major cycle).
+ * Returns 0, or -1 with sde_last_error() set.  No reference counterpart: the
+ * reference's only entry is grdh23d_. */
+int sde_grdh23d_device(const void *vis, const float *wt, const float *u, const float *v,
+                       const float *w, int64_t nvis, float uscale, float uoffset,
+                       float vscale, float voffset, int uorigin, int vorigin, void *gvis,
+                       int psf, int nu, int nv, const float *gridfnu, int suppu, int osampu,
+                       const float *gridfnv, int suppv, int osampv, int nz, float cellz,
+                       float refz, const double *shift, double *sumwt);
+
+/* GRDDEH23 on device-resident arrays, in place on VIS and WT; returns the
+ * number of points without kernel support (the reference's NBAD warning),
+ * or -1 with sde_last_error() set.  No reference counterpart (see
+ * sde_grdh23d_device). */
+int64_t sde_grddeh23_device(void *vis, float *wt, const float *u, const float *v,
+                            const float *w, int64_t nvis, float uscale, float uoffset,
+                            float vscale, float voffset, int uorigin, int vorigin,
+                            const void *gvis, int nu, int nv, const float *intfnu, int suppu,
+                            int osampu, const float *intfnv, int suppv, int osampv, int nz,
+                            float cellz, float refz, const double *shift);
 int sde_plan_get_images(sde_plan *plan, float *dirty, float *psf);
 /* The same copies issued on the plan's stream without waiting (complete when
  * sde_plan_query reports the plan done; pinned buffers for an asynchronous copy). */
@@ -230,6 +255,14 @@ int sde_plan_get_images_async(sde_plan *plan, float *dirty, float *psf);
 /* Pinned (page-locked) host memory for asynchronous transfers. */
 void *sde_host_alloc(size_t bytes);
 int sde_host_free(void *ptr);
+
+/* Device memory of the library's own runtime for the *_device entry points
+ * (callers whose arrays already live on the GPU hand over those addresses
+ * instead); sde_copy is a synchronous copy in any direction.  No reference
+ * counterpart. */
+void *sde_device_alloc(size_t bytes);
+int sde_device_free(void *ptr);
+int sde_copy(void *dst, const void *src, size_t bytes);
 /* The gridded half planes GVIS(NU,NV) of the last uvmap (dirty, PSF), as
  * GRDH2D returns them: folded, shifted by SUPPU and normalised, before the
  * FFT.  Complex, NU*NV per plane (either pointer may be NULL). */

@@ -47,6 +47,12 @@ def _declare(L):
     L.sde_version.restype = _C.c_char_p
     L.sde_plan_device_bytes.restype = _C.c_int64
     L.sde_plan_estimate_bytes.restype = _C.c_int64
+    L.sde_grdh23d_device.restype = _C.c_int
+    L.sde_device_alloc.restype = _C.c_void_p
+    L.sde_device_alloc.argtypes = [_C.c_size_t]
+    L.sde_device_free.argtypes = [_C.c_void_p]
+    L.sde_copy.argtypes = [_C.c_void_p, _C.c_void_p, _C.c_size_t]
+    L.sde_grddeh23_device.restype = _C.c_int64
     L.sde_host_alloc.restype = _C.c_void_p
     L.sde_host_alloc.argtypes = [_C.c_size_t]
     L.sde_host_free.argtypes = [_C.c_void_p]
@@ -65,6 +71,55 @@ def pinned_empty(shape, dtype):
     buf = (_C.c_char * max(n, 1)).from_address(ptr)
     weakref.finalize(buf, lib().sde_host_free, ptr)
     return np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
+
+
+class DeviceArray:
+    """An array in device memory of the library's HIP runtime, for the
+    *_device entry points (torch's bundled HIP runtime is a separate one
+    here: its tensors are not handed to the library)."""
+
+    def __init__(self, shape, dtype):
+        import weakref
+        import numpy as np
+        self.shape = tuple(int(x) for x in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = lib().sde_device_alloc(self.nbytes)
+        if not self.ptr:
+            raise MemoryError("sde_device_alloc: %d bytes: %s" % (self.nbytes, last_error()))
+        self._fin = weakref.finalize(self, lib().sde_device_free, _C.c_void_p(self.ptr))
+
+    @classmethod
+    def from_numpy(cls, a):
+        import numpy as np
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        import numpy as np
+        a = np.ascontiguousarray(a, self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        if lib().sde_copy(_C.c_void_p(self.ptr), a.ctypes.data_as(_C.c_void_p), self.nbytes) != 0:
+            raise SdeError("sde_copy: " + last_error())
+
+    def copy_from(self, other):
+        if other.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        if lib().sde_copy(_C.c_void_p(self.ptr), _C.c_void_p(other.ptr), self.nbytes) != 0:
+            raise SdeError("sde_copy: " + last_error())
+
+    def numpy(self):
+        import numpy as np
+        out = np.empty(self.shape, self.dtype)
+        if lib().sde_copy(out.ctypes.data_as(_C.c_void_p), _C.c_void_p(self.ptr), self.nbytes) != 0:
+            raise SdeError("sde_copy: " + last_error())
+        return out
+
+    def free(self):
+        self._fin()
 
 
 def device_count():

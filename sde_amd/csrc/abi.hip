@@ -458,6 +458,87 @@ PlaneParams make_pp(int nz, float cellz, float refz) {
     return pp;
 }
 
+// GRDH23D on device arrays: one cell sort of the points, then per plane the
+// plane's phase into the hot records, the tile kernel and the reduce (fold,
+// shift, normalise) into that plane of dg (NZ planes of NU x NV).  The
+// normalisation (NU-1)*NV*NZ/LSUMWT with LSUMWT summed over NZ identical
+// planes is (NU-1)*NV over one plane's LSUMWT.  Ends with the stream
+// synchronised; out = (SUMWT, one plane's LSUMWT).  The caller holds the
+// context lock.
+int slowz_grid(Ctx &c, const float2 *dvis, const float *dwt, const float *du, const float *dv, const float *dw,
+               int64_t n, const CoordParams &cp, const PlaneParams &pp, int psf, const float *dcfu,
+               const float *dcfv, float2 *dg, double out[2]) {
+    hipStream_t s = c.stream;
+    const TileParams tp = make_tp(cp.nu, cp.nv, default_tile(cp.nu, cp.nv), default_chunk(n));
+    const size_t nn = (size_t)(n > 0 ? n : 1), np = (size_t)cp.nu * cp.nv;
+    SortBufs sb;
+    sb.max_items = max_items_for(tp, n);
+    sb.temp_bytes = sort_temp_bytes(n > 0 ? n : 1, sort_nbits(tp));
+    float4 *zbase = ctx_buf<float4>(c, "zbase", nn);
+    sb.keys = ctx_buf<uint32_t>(c, "keys", nn);
+    sb.skeys = ctx_buf<uint32_t>(c, "skeys", nn);
+    sb.vals = ctx_buf<uint64_t>(c, "vals", nn);
+    sb.svals = ctx_buf<uint64_t>(c, "svals", nn);
+    sb.hot = ctx_buf<StageHot>(c, "stage_hot", nn);
+    sb.cold = nullptr;
+    sb.cold_all = 0;
+    sb.temp = ctx_buf<char>(c, "sort_temp", sb.temp_bytes > 0 ? sb.temp_bytes : 1);
+    sb.tile_start = ctx_buf<int>(c, "tile_start", tp.ntiles + 1);
+    sb.item_start = ctx_buf<int>(c, "item_start", tp.ntiles + 1);
+    sb.nitems = ctx_buf<int>(c, "nitems", 1);
+    sb.items = ctx_buf<WorkItem>(c, "items", sb.max_items);
+    float *partial = ctx_buf<float>(c, "partial", grid_partial_floats(tp, 1, sb.max_items));
+    double *blk = ctx_buf<double>(c, "blk", 2 * (size_t)sb.max_items), *dsums = ctx_buf<double>(c, "sums", 4);
+    double *pblk = ctx_buf<double>(c, "prep_blk", 2 * (size_t)prep_blocks(n > 0 ? n : 1));
+    double *dsw = ctx_buf<double>(c, "sumwt", 4);
+    if (!zbase || !sb.keys || !sb.skeys || !sb.vals || !sb.svals || !sb.hot || !sb.temp || sb.temp_bytes == 0 ||
+        !sb.tile_start || !sb.item_start || !sb.nitems || !sb.items || !partial || !blk || !dsums || !pblk || !dsw) {
+        set_plan_error("device allocation failed");
+        return -1;
+    }
+    launch_prep_slowz(s, dvis, dwt, du, dv, dw, n, cp, tp, psf, pp, sb, zbase, pblk, dsw);
+    if (launch_sort(s, sb, n, tp, true)) {
+        set_plan_error("cell sort failed");
+        return -1;
+    }
+    auto nb = [](const char *) {};
+    auto ne = []() {};
+    for (int iz = 1; iz <= pp.nz; iz++) {
+        launch_plane_phase(s, zbase, sb.hot, n, pp, iz);
+        if (run_grid(s, cp, tp, 1, false, WeightParams(), dcfu, dcfv, sb, nullptr, partial, blk, dsums,
+                     dg + (size_t)(iz - 1) * np, nullptr, nb, ne))
+            return -1;
+    }
+    double sums[3], sw[3];
+    SDE_CHECK(hipMemcpyAsync(sums, dsums, sizeof sums, hipMemcpyDeviceToHost, s));
+    SDE_CHECK(hipMemcpyAsync(sw, dsw, sizeof sw, hipMemcpyDeviceToHost, s));
+    SDE_CHECK(hipStreamSynchronize(s));
+    out[0] = sw[0];
+    out[1] = sums[1];
+    return 0;
+}
+
+// GRDDEH23 on device arrays (in place on vis/wt), synchronised; returns the
+// number of points without kernel support, or -1.
+long long slowz_degrid(Ctx &c, const float2 *dg, float2 *dvis, float *dwt, const float *du, const float *dv,
+                       const float *dw, int64_t n, const CoordParams &cp, const PlaneParams &pp, const float *dcfu,
+                       const float *dcfv) {
+    hipStream_t s = c.stream;
+    unsigned long long *cnt = ctx_buf<unsigned long long>(c, "counters", 2);
+    float2 *inner = ctx_buf<float2>(c, "grid_inner", (size_t)cp.nu * cp.nv * pp.nz);
+    if (!cnt || !inner) {
+        set_plan_error("device allocation failed");
+        return -1;
+    }
+    SDE_CHECK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * 2, s));
+    launch_degrid_slowz(s, dg, inner, du, dv, dw, dwt, dvis, n, cp, dcfu, dcfv, pp, dvis, dwt, cnt);
+    SDE_CHECK(hipGetLastError());
+    unsigned long long nbad = 0;
+    SDE_CHECK(hipMemcpyAsync(&nbad, cnt, sizeof nbad, hipMemcpyDeviceToHost, s));
+    SDE_CHECK(hipStreamSynchronize(s));
+    return (long long)nbad;
+}
+
 }  // namespace
 
 extern "C" {
@@ -683,11 +764,7 @@ void grddeh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, f
     }
 }
 
-// GRDH23D: one cell sort of the points, then per plane the plane's phase
-// into the hot records, the tile kernel and the reduce (fold, shift,
-// normalise) into that plane of the output.  The normalisation
-// (NU-1)*NV*NZ/LSUMWT with LSUMWT summed over NZ identical planes is
-// (NU-1)*NV over one plane's LSUMWT.
+// GRDH23D: the inputs copied to the device, then slowz_grid.
 void grdh23d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, float *uscale, float *uoffset,
               float *vscale, float *voffset, int *uorigin, int *vorigin, float *gvis, int *psf, int *nu, int *nv,
               float *gridfnu, int *suppu, int *osampu, float *gridfnv, int *suppv, int *osampv, int *nz,
@@ -706,36 +783,13 @@ void grdh23d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, fl
     const CoordParams cp = make_cp(shift, *uscale, *uoffset, *vscale, *voffset, *uorigin, *vorigin, *nu, *nv,
                                    *suppu, *suppv, *osampu, *osampv);
     const PlaneParams pp = make_pp(*nz, *cellz, *refz);
-    const TileParams tp = make_tp(*nu, *nv, default_tile(*nu, *nv), default_chunk(n));
-    const size_t nn = (size_t)(n > 0 ? n : 1), np = (size_t)*nu * *nv;
-    SortBufs sb;
-    sb.max_items = max_items_for(tp, n);
-    sb.temp_bytes = sort_temp_bytes(n > 0 ? n : 1, sort_nbits(tp));
+    const size_t nn = (size_t)(n > 0 ? n : 1), ng = (size_t)*nu * *nv * *nz;
     float2 *dvis = ctx_buf<float2>(c, "vis", nn);
     float *dwt = ctx_buf<float>(c, "wt", nn), *du = ctx_buf<float>(c, "u", nn);
     float *dv = ctx_buf<float>(c, "v", nn), *dw = ctx_buf<float>(c, "w", nn);
     float *dcfu = ctx_buf<float>(c, "cfu", ncfu), *dcfv = ctx_buf<float>(c, "cfv", ncfv);
-    float4 *zbase = ctx_buf<float4>(c, "zbase", nn);
-    sb.keys = ctx_buf<uint32_t>(c, "keys", nn);
-    sb.skeys = ctx_buf<uint32_t>(c, "skeys", nn);
-    sb.vals = ctx_buf<uint64_t>(c, "vals", nn);
-    sb.svals = ctx_buf<uint64_t>(c, "svals", nn);
-    sb.hot = ctx_buf<StageHot>(c, "stage_hot", nn);
-    sb.cold = nullptr;
-    sb.cold_all = 0;
-    sb.temp = ctx_buf<char>(c, "sort_temp", sb.temp_bytes > 0 ? sb.temp_bytes : 1);
-    sb.tile_start = ctx_buf<int>(c, "tile_start", tp.ntiles + 1);
-    sb.item_start = ctx_buf<int>(c, "item_start", tp.ntiles + 1);
-    sb.nitems = ctx_buf<int>(c, "nitems", 1);
-    sb.items = ctx_buf<WorkItem>(c, "items", sb.max_items);
-    float *partial = ctx_buf<float>(c, "partial", grid_partial_floats(tp, 1, sb.max_items));
-    double *blk = ctx_buf<double>(c, "blk", 2 * (size_t)sb.max_items), *dsums = ctx_buf<double>(c, "sums", 4);
-    float2 *dg = ctx_buf<float2>(c, "grid", np * *nz);
-    double *pblk = ctx_buf<double>(c, "prep_blk", 2 * (size_t)prep_blocks(n > 0 ? n : 1));
-    double *dsw = ctx_buf<double>(c, "sumwt", 4);
-    if (!pblk || !dsw || !dvis || !dwt || !du || !dv || !dw || !dcfu || !dcfv || !zbase || !sb.keys || !sb.skeys || !sb.vals ||
-        !sb.svals || !sb.hot || !sb.temp || sb.temp_bytes == 0 || !sb.tile_start || !sb.item_start || !sb.nitems ||
-        !sb.items || !partial || !blk || !dsums || !dg) {
+    float2 *dg = ctx_buf<float2>(c, "grid", ng);
+    if (!dvis || !dwt || !du || !dv || !dw || !dcfu || !dcfv || !dg) {
         report_error(kErrNoMem, R, "device allocation failed");
         return;
     }
@@ -748,31 +802,18 @@ void grdh23d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, fl
     }
     SDE_F(hipMemcpyAsync(dcfu, gridfnu, sizeof(float) * ncfu, hipMemcpyHostToDevice, s));
     SDE_F(hipMemcpyAsync(dcfv, gridfnv, sizeof(float) * ncfv, hipMemcpyHostToDevice, s));
-    launch_prep_slowz(s, dvis, dwt, du, dv, dw, n, cp, tp, *psf, pp, sb, zbase, pblk, dsw);
-    if (launch_sort(s, sb, n, tp, true)) {
-        report_error(kErrFatal, R, "cell sort failed");
+    double out[2];
+    if (slowz_grid(c, dvis, dwt, du, dv, dw, n, cp, pp, *psf, dcfu, dcfv, dg, out)) {
+        report_error(kErrFatal, R, sde_last_error());
         return;
     }
-    auto nb = [](const char *) {};
-    auto ne = []() {};
-    for (int iz = 1; iz <= *nz; iz++) {
-        launch_plane_phase(s, zbase, sb.hot, n, pp, iz);
-        if (run_grid(s, cp, tp, 1, false, WeightParams(), dcfu, dcfv, sb, nullptr, partial, blk, dsums,
-                     dg + (size_t)(iz - 1) * np, nullptr, nb, ne)) {
-            report_error(kErrFatal, R, sde_last_error());
-            return;
-        }
-    }
-    double sums[3], sw[3];
-    SDE_F(hipMemcpyAsync(gvis, dg, sizeof(float2) * np * *nz, hipMemcpyDeviceToHost, s));
-    SDE_F(hipMemcpyAsync(sums, dsums, sizeof sums, hipMemcpyDeviceToHost, s));
-    SDE_F(hipMemcpyAsync(sw, dsw, sizeof sw, hipMemcpyDeviceToHost, s));
+    SDE_F(hipMemcpyAsync(gvis, dg, sizeof(float2) * ng, hipMemcpyDeviceToHost, s));
     SDE_F(hipStreamSynchronize(s));
-    *sumwt = (float)sw[0];
-    if (sums[1] == 0.0) report_error(kErrBdArg, R, "No weight!");
+    *sumwt = (float)out[0];
+    if (out[1] == 0.0) report_error(kErrBdArg, R, "No weight!");
 }
 
-// GRDDEH23: the per-point kernel over the NZ planes.
+// GRDDEH23: the inputs copied to the device, then slowz_degrid.
 void grddeh23_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, float *uscale, float *uoffset,
                float *vscale, float *voffset, int *uorigin, int *vorigin, float *gvis, int *nu, int *nv,
                float *intfnu, int *suppu, int *osampu, float *intfnv, int *suppv, int *osampv, int *nz,
@@ -797,8 +838,7 @@ void grddeh23_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, f
     float *dv = ctx_buf<float>(c, "v", nn), *dw = ctx_buf<float>(c, "w", nn);
     float *dcfu = ctx_buf<float>(c, "cfu", ncfu), *dcfv = ctx_buf<float>(c, "cfv", ncfv);
     float2 *dg = ctx_buf<float2>(c, "grid", ng);
-    unsigned long long *cnt = ctx_buf<unsigned long long>(c, "counters", 2);
-    if (!dvis || !dwt || !du || !dv || !dw || !dcfu || !dcfv || !dg || !cnt) {
+    if (!dvis || !dwt || !du || !dv || !dw || !dcfu || !dcfv || !dg) {
         report_error(kErrNoMem, R, "device allocation failed");
         return;
     }
@@ -812,19 +852,19 @@ void grddeh23_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, f
     SDE_F(hipMemcpyAsync(dcfu, intfnu, sizeof(float) * ncfu, hipMemcpyHostToDevice, s));
     SDE_F(hipMemcpyAsync(dcfv, intfnv, sizeof(float) * ncfv, hipMemcpyHostToDevice, s));
     SDE_F(hipMemcpyAsync(dg, gvis, sizeof(float2) * ng, hipMemcpyHostToDevice, s));
-    SDE_F(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * 2, s));
-    launch_degrid_slowz(s, dg, du, dv, dw, dwt, dvis, n, cp, dcfu, dcfv, pp, dvis, dwt, cnt);
-    SDE_F(hipGetLastError());
-    unsigned long long nbad = 0;
+    const long long nbad = slowz_degrid(c, dg, dvis, dwt, du, dv, dw, n, cp, pp, dcfu, dcfv);
+    if (nbad < 0) {
+        report_error(kErrFatal, R, sde_last_error());
+        return;
+    }
     if (n > 0) {
         SDE_F(hipMemcpyAsync(vis, dvis, sizeof(float2) * n, hipMemcpyDeviceToHost, s));
         SDE_F(hipMemcpyAsync(wt, dwt, sizeof(float) * n, hipMemcpyDeviceToHost, s));
     }
-    SDE_F(hipMemcpyAsync(&nbad, cnt, sizeof nbad, hipMemcpyDeviceToHost, s));
     SDE_F(hipStreamSynchronize(s));
     if (nbad > 0) {
         char msg[96];
-        snprintf(msg, sizeof msg, "%7llu points were not successfully degridded", nbad);
+        snprintf(msg, sizeof msg, "%7lld points were not successfully degridded", nbad);
         report_warning(msg);
     }
 }
@@ -1638,6 +1678,50 @@ int sde_device_memory(size_t *free_bytes, size_t *total_bytes) {
     return 0;
 }
 
+int sde_grdh23d_device(const void *vis, const float *wt, const float *u, const float *v, const float *w,
+                       int64_t nvis, float uscale, float uoffset, float vscale, float voffset, int uorigin,
+                       int vorigin, void *gvis, int psf, int nu, int nv, const float *gridfnu, int suppu, int osampu,
+                       const float *gridfnv, int suppv, int osampv, int nz, float cellz, float refz,
+                       const double *shift, double *sumwt) {
+    const char *R = "GRDH23D";
+    if (nvis > INT32_MAX || !check_kernel_args(R, (int)nvis, nu, nv, suppu, suppv, osampu, osampv, uorigin, vorigin) ||
+        !check_planes(R, nz))
+        return -1;
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    Ctx &c = g_ctx;
+    if (ctx_init(c)) return -1;
+    const CoordParams cp = make_cp(shift, uscale, uoffset, vscale, voffset, uorigin, vorigin, nu, nv, suppu, suppv,
+                                   osampu, osampv);
+    double out[2];
+    if (slowz_grid(c, (const float2 *)vis, wt, u, v, w, nvis, cp, make_pp(nz, cellz, refz), psf, gridfnu, gridfnv,
+                   (float2 *)gvis, out))
+        return -1;
+    if (sumwt) *sumwt = out[0];
+    if (out[1] == 0.0) {
+        set_plan_error("GRDH23D: No weight!");
+        return -1;
+    }
+    return 0;
+}
+
+int64_t sde_grddeh23_device(void *vis, float *wt, const float *u, const float *v, const float *w, int64_t nvis,
+                            float uscale, float uoffset, float vscale, float voffset, int uorigin, int vorigin,
+                            const void *gvis, int nu, int nv, const float *intfnu, int suppu, int osampu,
+                            const float *intfnv, int suppv, int osampv, int nz, float cellz, float refz,
+                            const double *shift) {
+    const char *R = "GRDDEH23";
+    if (nvis > INT32_MAX || !check_kernel_args(R, (int)nvis, nu, nv, suppu, suppv, osampu, osampv, uorigin, vorigin) ||
+        !check_planes(R, nz))
+        return -1;
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    Ctx &c = g_ctx;
+    if (ctx_init(c)) return -1;
+    const CoordParams cp = make_cp(shift, uscale, uoffset, vscale, voffset, uorigin, vorigin, nu, nv, suppu, suppv,
+                                   osampu, osampv);
+    return slowz_degrid(c, (const float2 *)gvis, (float2 *)vis, wt, u, v, w, nvis, cp, make_pp(nz, cellz, refz),
+                        intfnu, intfnv);
+}
+
 int sde_plan_get_images(sde_plan *p, float *dirty, float *psf) {
     if (!p) return -1;
     const size_t npix = (size_t)p->cfg.nx * p->cfg.ny;
@@ -1666,6 +1750,27 @@ void *sde_host_alloc(size_t bytes) {
 
 int sde_host_free(void *ptr) {
     if (ptr) SDE_CHECK(hipHostFree(ptr));
+    return 0;
+}
+
+void *sde_device_alloc(size_t bytes) {
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    if (ctx_init(g_ctx)) return nullptr;
+    void *ptr = nullptr;
+    if (hipMalloc(&ptr, bytes > 0 ? bytes : 1) != hipSuccess) {
+        set_plan_error("sde_device_alloc: %zu bytes failed", bytes);
+        return nullptr;
+    }
+    return ptr;
+}
+
+int sde_device_free(void *ptr) {
+    if (ptr) SDE_CHECK(hipFree(ptr));
+    return 0;
+}
+
+int sde_copy(void *dst, const void *src, size_t bytes) {
+    if (bytes) SDE_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDefault));
     return 0;
 }
 

@@ -1,6 +1,8 @@
 // Degridding (GRDDEH2D, grd/grddeh2d.f:84-203), density-grid weighting
 // (GRDUWT2D grd/grduwt2d.f:77-167, GRDRUW2D grd/grdruw2d.f:87-249) and the
 // synthetic-track generator used for the large workloads.
+#include <algorithm>
+
 #include "internal.h"
 
 namespace {
@@ -114,15 +116,24 @@ __global__ __launch_bounds__(256) void k_degrid(
     if (i < nvis) degrid_point(i, grid, u, v, w, wt_in, vis_in, cp, cfu, cfv, vis_out, wt_out, nbad);
 }
 
+// The NZ planes GVIS(NU,NV,NZ) re-laid cell-major, out[cell*NZ + iz] =
+// in[iz*NU*NV + cell], so the NZ values of one tap are one contiguous run
+// (one cache line at NZ = 8) for the degridder's gathers.
+__global__ __launch_bounds__(256) void k_planes_inner(const float2 *__restrict__ in, float2 *__restrict__ out,
+                                                      size_t ncell, int nz) {
+    for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < ncell; c += (size_t)gridDim.x * 256)
+        for (int iz = 0; iz < nz; iz++) out[c * nz + iz] = in[(size_t)iz * ncell + c];
+}
+
 // GRDDEH23 (grd/grddeh23.f:86-210), one thread per visibility, 64 per
-// workgroup: the point's NZ plane rotations CROTZ are computed once into
-// its own LDS column (plane-major, so the 64 lanes of a plane read 64
-// consecutive entries), then the taps are gathered in the reference's loop
-// order -- OFFU outer near the u = 0 axis, OFFV outer elsewhere, the planes
-// innermost -- in its REAL arithmetic: (GVIS * CROTZ) * UVWT near the axis
-// on the conjugating side, (GVIS * UVWT) * CROTZ elsewhere.  A tap on a
-// reflected row outside the grid (which the reference reads out of bounds)
-// adds nothing.
+// workgroup, on the cell-major planes of k_planes_inner: the point's NZ
+// plane rotations CROTZ are computed once into its own LDS column
+// (plane-major, so the 64 lanes of a plane read 64 consecutive entries),
+// then the taps are gathered in the reference's loop order -- OFFU outer
+// near the u = 0 axis, OFFV outer elsewhere, the planes innermost -- in its
+// REAL arithmetic: (GVIS * CROTZ) * UVWT near the axis on the conjugating
+// side, (GVIS * UVWT) * CROTZ elsewhere.  A tap on a reflected row outside
+// the grid (which the reference reads out of bounds) adds nothing.
 __global__ __launch_bounds__(64) void k_degrid_slowz(
     const float2 *__restrict__ grid, const float *__restrict__ u, const float *__restrict__ v,
     const float *__restrict__ w, const float *__restrict__ wt_in, const float2 *vis_in, int64_t nvis,
@@ -139,7 +150,7 @@ __global__ __launch_bounds__(64) void k_degrid_slowz(
     }
     const int cfoffu = (cp.suppu + 1) * cp.osampu + 1;
     const int cfoffv = (cp.suppv + 1) * cp.osampv + 1;
-    const size_t plane = (size_t)cp.nu * cp.nv;
+    const int nz = pp.nz;
     wti = -wti;
     const float uf = u[i], vf = v[i], wf = w[i];
     const double ug = uf, vg = vf, wg = wf;
@@ -175,9 +186,9 @@ __global__ __launch_bounds__(64) void k_degrid_slowz(
     float vr = 0.0f, vi = 0.0f, sumwt = 0.0f;
     // (G * UVWT) * CROTZ of every plane of one tap, G read at cell (ui, vj)
     auto taps_scaled = [&](int ui, int vj, float uvwt) {
-        const float2 *g = grid + (size_t)(vj - 1) * cp.nu + (ui - 1);
-        for (int iz = 0; iz < pp.nz; iz++) {
-            const float2 gv = g[iz * plane];
+        const float2 *g = grid + ((size_t)(vj - 1) * cp.nu + (ui - 1)) * nz;
+        for (int iz = 0; iz < nz; iz++) {
+            const float2 gv = g[iz];
             const float2 c = cz[iz * 64];
             const float ar = gv.x * uvwt, ai = gv.y * uvwt;
             vr = vr + (ar * c.x - ai * c.y);
@@ -195,9 +206,9 @@ __global__ __launch_bounds__(64) void k_degrid_slowz(
                     const float uvwt = cu * cfv[delvi + cp.osampv * offv + cfoffv - 1];
                     sumwt = sumwt + uvwt;
                     const bool in = vint >= 1 && vint <= cp.nv && uint_ >= 1 && uint_ <= cp.nu;
-                    const float2 *g = grid + (in ? (size_t)(vint - 1) * cp.nu + (uint_ - 1) : 0);
-                    for (int iz = 0; iz < pp.nz; iz++) {
-                        float2 gv = in ? g[iz * plane] : make_float2(0.0f, 0.0f);
+                    const float2 *g = grid + (in ? ((size_t)(vint - 1) * cp.nu + (uint_ - 1)) * nz : 0);
+                    for (int iz = 0; iz < nz; iz++) {
+                        float2 gv = in ? g[iz] : make_float2(0.0f, 0.0f);
                         if (usign < 0) gv.y = -gv.y;
                         const float2 c = cz[iz * 64];
                         // (FVIS * CROTZ) * UVWT
@@ -490,14 +501,17 @@ void launch_degrid(hipStream_t s, const float2 *grid, const float *u, const floa
                        nvis, cp, cfu, cfv, vis_out, wt_out, nbad);
 }
 
-void launch_degrid_slowz(hipStream_t s, const float2 *grid, const float *u, const float *v, const float *w,
-                         const float *wt_in, const float2 *vis_in, int64_t nvis, const CoordParams &cp,
-                         const float *cfu, const float *cfv, const PlaneParams &pp, float2 *vis_out,
-                         float *wt_out, unsigned long long *nbad) {
+void launch_degrid_slowz(hipStream_t s, const float2 *grid, float2 *grid_inner, const float *u, const float *v,
+                         const float *w, const float *wt_in, const float2 *vis_in, int64_t nvis,
+                         const CoordParams &cp, const float *cfu, const float *cfv, const PlaneParams &pp,
+                         float2 *vis_out, float *wt_out, unsigned long long *nbad) {
     if (nvis <= 0) return;
+    const size_t ncell = (size_t)cp.nu * cp.nv;
+    hipLaunchKernelGGL(k_planes_inner, dim3((unsigned)std::min<size_t>((ncell + 255) / 256, 8192)), dim3(256), 0, s,
+                       grid, grid_inner, ncell, pp.nz);
     const int64_t nb = (nvis + 63) / 64;
-    hipLaunchKernelGGL(k_degrid_slowz, dim3((unsigned)nb), dim3(64), sizeof(float2) * 64 * (size_t)pp.nz, s, grid,
-                       u, v, w, wt_in, vis_in, nvis, cp, cfu, cfv, pp, vis_out, wt_out, nbad);
+    hipLaunchKernelGGL(k_degrid_slowz, dim3((unsigned)nb), dim3(64), sizeof(float2) * 64 * (size_t)pp.nz, s,
+                       grid_inner, u, v, w, wt_in, vis_in, nvis, cp, cfu, cfv, pp, vis_out, wt_out, nbad);
 }
 
 void launch_degrid_list(hipStream_t s, const float2 *grid, const float *u, const float *v, const float *w,

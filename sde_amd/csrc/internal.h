@@ -486,11 +486,12 @@ void launch_prep_slowz(hipStream_t s, const float2 *vis, const float *wt, const 
 void launch_plane_phase(hipStream_t s, const float4 *zbase, StageHot *hot, int64_t n, const PlaneParams &pp,
                         int iz);
 // GRDDEH23, one thread per visibility, in the reference's loop order and
-// REAL arithmetic (grd/grddeh23.f:86-210); grid holds NZ planes of NU x NV.
-void launch_degrid_slowz(hipStream_t s, const float2 *grid, const float *u, const float *v, const float *w,
-                         const float *wt_in, const float2 *vis_in, int64_t nvis, const CoordParams &cp,
-                         const float *cfu, const float *cfv, const PlaneParams &pp, float2 *vis_out,
-                         float *wt_out, unsigned long long *nbad);
+// REAL arithmetic (grd/grddeh23.f:86-210); grid holds NZ planes of NU x NV,
+// re-laid cell-major into grid_inner (NU*NV*NZ float2 of scratch) first.
+void launch_degrid_slowz(hipStream_t s, const float2 *grid, float2 *grid_inner, const float *u, const float *v,
+                         const float *w, const float *wt_in, const float2 *vis_in, int64_t nvis,
+                         const CoordParams &cp, const float *cfu, const float *cfv, const PlaneParams &pp,
+                         float2 *vis_out, float *wt_out, unsigned long long *nbad);
 
 // ---- plain copy of n float4 (bandwidth probe, fft.hip) ----
 void launch_copy(hipStream_t s, const float4 *src, float4 *dst, size_t n);

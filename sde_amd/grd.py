@@ -184,6 +184,59 @@ def grddeh23(vis, wt, u, v, w, uscale, uoffset, vscale, voffset, uorigin, vorigi
     return vis, wt
 
 
+def _dev(a, dtype):
+    """Address of a DeviceArray of the given dtype."""
+    from . import DeviceArray
+    if not isinstance(a, DeviceArray) or a.dtype != np.dtype(dtype):
+        raise ValueError("expected a DeviceArray of %s" % np.dtype(dtype))
+    return C.c_void_p(a.ptr)
+
+
+def _raise_device(name):
+    from . import SdeError
+    msg = lib().sde_last_error().decode(errors="replace")
+    lib().sde_error_clear()
+    raise SdeError("%s: %s" % (name, msg))
+
+
+def grdh23d_device(vis, wt, u, v, w, uscale, uoffset, vscale, voffset, uorigin, vorigin, psf, gvis, gridfnu, suppu,
+                   osampu, cellz, refz, shift=IDENTITY):
+    """GRDH23D on DeviceArrays: vis complex64 (NVIS), wt/u/v/w float32,
+    gvis complex64 (NZ, NV, NU) written, gridfnu float32.  Returns SUMWT."""
+    nz, nv, nu = gvis.shape
+    sw = C.c_double(0.0)
+    f = np.float32
+    rc = lib().sde_grdh23d_device(_dev(vis, np.complex64), _dev(wt, f), _dev(u, f), _dev(v, f), _dev(w, f),
+                                  C.c_int64(vis.shape[0]), C.c_float(uscale), C.c_float(uoffset), C.c_float(vscale),
+                                  C.c_float(voffset), C.c_int(uorigin), C.c_int(vorigin),
+                                  _dev(gvis, np.complex64), C.c_int(1 if psf else 0), C.c_int(nu), C.c_int(nv),
+                                  _dev(gridfnu, f), C.c_int(suppu), C.c_int(osampu), _dev(gridfnu, f),
+                                  C.c_int(suppu), C.c_int(osampu), C.c_int(nz), C.c_float(cellz), C.c_float(refz),
+                                  _pd(_shift(shift)), C.byref(sw))
+    if rc != 0:
+        _raise_device("sde_grdh23d_device")
+    return sw.value
+
+
+def grddeh23_device(vis, wt, u, v, w, uscale, uoffset, vscale, voffset, uorigin, vorigin, gvis, intfnu, suppu,
+                    osampu, cellz, refz, shift=IDENTITY):
+    """GRDDEH23 on DeviceArrays, in place on vis (complex64) and wt; gvis
+    complex64 (NZ, NV, NU).  Returns the number of points without kernel
+    support."""
+    nz, nv, nu = gvis.shape
+    f = np.float32
+    nbad = lib().sde_grddeh23_device(_dev(vis, np.complex64), _dev(wt, f), _dev(u, f), _dev(v, f), _dev(w, f),
+                                     C.c_int64(vis.shape[0]), C.c_float(uscale), C.c_float(uoffset),
+                                     C.c_float(vscale), C.c_float(voffset), C.c_int(uorigin), C.c_int(vorigin),
+                                     _dev(gvis, np.complex64), C.c_int(nu), C.c_int(nv), _dev(intfnu, f),
+                                     C.c_int(suppu), C.c_int(osampu), _dev(intfnu, f), C.c_int(suppu),
+                                     C.c_int(osampu), C.c_int(nz), C.c_float(cellz), C.c_float(refz),
+                                     _pd(_shift(shift)))
+    if nbad < 0:
+        _raise_device("sde_grddeh23_device")
+    return int(nbad)
+
+
 def grduwt2d(wt, u, v, w, uscale, uoffset, vscale, voffset, uorigin, vorigin, nu, nv, shift=IDENTITY,
              newwt=None):
     """GRDUWT2D (grd/grduwt2d.f:7-170): uniform weights.  Returns (newwt, gwt)."""

@@ -140,3 +140,30 @@ def test_grddeh23_matches_reference_arithmetic(shifted, nz, cellz, refz):
         va, wa = rw.grddeh23(wt, u, v, w, 1.0, 1.0, 1, vo, g32, cf, supp, osamp, cellz, refz, sh)
         np.testing.assert_array_equal(wa, wr)
         assert np.array_equal(va[good], vr[good])
+
+
+def test_device_entry_points_match_the_drop_ins():
+    """sde_grdh23d_device / sde_grddeh23_device on device arrays give the
+    drop-ins' results bit for bit (the same kernels on the same inputs)."""
+    from sde_amd import DeviceArray
+    nu, nv, vo, nz, cellz, refz = 65, 128, 64, 3, 2.0, 2.0
+    vis, wt, u, v, w = _points(3000, 46, 70, 60, near_axis=300)
+    cf, supp, osamp = po.grdcf("SF")
+    sh = _shift(True)
+    g, sw = grd.grdh23d(vis, wt, u, v, w, 1.0, 0.0, 1.0, 0.0, 1, vo, False, nu, nv, cf, supp, osamp, nz, cellz, refz,
+                        shift=sh)
+    d = [DeviceArray.from_numpy(a) for a in (vis, wt, u, v, w)]
+    dcf = DeviceArray.from_numpy(cf)
+    dg = DeviceArray((nz, nv, nu), np.complex64)
+    swd = grd.grdh23d_device(*d, 1.0, 0.0, 1.0, 0.0, 1, vo, False, dg, dcf, supp, osamp, cellz, refz, shift=sh)
+    assert np.float32(swd) == np.float32(sw)
+    np.testing.assert_array_equal(dg.numpy(), g)
+    vg, wg = grd.grddeh23(vis, wt, u, v, w, 1.0, 0.0, 1.0, 0.0, 1, vo, g, cf, supp, osamp, cellz, refz, shift=sh)
+    nbad = grd.grddeh23_device(d[0], d[1], d[2], d[3], d[4], 1.0, 0.0, 1.0, 0.0, 1, vo, dg, dcf, supp, osamp, cellz,
+                               refz, shift=sh)
+    assert nbad == 0
+    np.testing.assert_array_equal(d[0].numpy(), vg)
+    np.testing.assert_array_equal(d[1].numpy(), wg)
+    with pytest.raises(sde.SdeError, match="NZ"):
+        grd.grdh23d_device(*d, 1.0, 0.0, 1.0, 0.0, 1, vo, False, DeviceArray((129, nv, nu), np.complex64), dcf, supp,
+                           osamp, cellz, refz)
