@@ -238,6 +238,20 @@ int sde_grdh23d_device(const void *vis, const float *wt, const float *u, const f
                        const float *gridfnv, int suppv, int osampv, int nz, float cellz,
                        float refz, const double *shift, double *sumwt);
 
+/* VISTOIMG of a w-plane (SlowZ) cube on device-resident arrays
+ * (vis/vistoimg.f:48-60 with CTYPE(3) = 'N----SIN'): GRDH23D with VISGRID's
+ * scales (vis/visgrid.f:200-227: UOFFSET = VOFFSET = 0, kernel CFTYPE),
+ * then per plane FFTR2D (arr/arrfftr.f:143-151) and PIXCOR2D 'CORRECT'
+ * (img/imggridc.f:103-109).  gvis: NU*NV*NZ complex (NU = FNU/2+1), left
+ * holding the gridded planes; images: NX*NY*NZ floats.  Returns 0, or -1
+ * with sde_last_error() set.  No reference counterpart at this level (the
+ * reference composes it from named objects). */
+int sde_vistoimg_slowz_device(const void *vis, const float *wt, const float *u, const float *v,
+                              const float *w, int64_t nvis, float uscale, float vscale,
+                              int uorigin, int vorigin, int psf, int nx, int ny, int fnu, int nv,
+                              const char *cftype, int nz, float cellz, float refz,
+                              const double *shift, void *gvis, float *images, double *sumwt);
+
 /* GRDDEH23 on device-resident arrays, in place on VIS and WT; returns the
  * number of points without kernel support (the reference's NBAD warning),
  * or -1 with sde_last_error() set.  No reference counterpart (see

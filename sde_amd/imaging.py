@@ -407,3 +407,38 @@ def major_cycle(plan, model_image):
     plan.uvmap_residual()
     return plan.images()
 
+
+
+def vistoimg_slowz(vis, wt, u, v, w, nx, cellsize, nz, cellz, refz, ny=None, psf=False, shift=None, cftype="SF"):
+    """VISTOIMG of a w-plane cube (vis/vistoimg.f:48-60 with a third axis of
+    type N----SIN): the visibilities gridded onto NZ planes by GRDH23D with
+    VISGRID's scales (vis/visgrid.f:200-227), each plane transformed by
+    FFTR2D and corrected by PIXCOR2D (arr/arrfftr.f:143-151,
+    img/imggridc.f:103-109).  Plane IZ (1-based) is the image at the n
+    offset DTOR*CELLZ*(IZ-REFZ).  The visibilities are copied to the device
+    once; everything else runs there.
+
+    Returns (cube (NZ, NY, NX) float32, gvis (NZ, NV, NU) complex64, SUMWT)."""
+    from . import DeviceArray
+    ny = nx if ny is None else ny
+    geo = Plan(nx, ny, cellsize, cftype, weighting=0, fov=1.0)
+    nu, nv, fnu, uorigin, vorigin = geo.nu, geo.nv, geo.fnu, geo.uorigin, geo.vorigin
+    uscale, vscale = geo.uscale, geo.vscale
+    geo.close()
+    d = [DeviceArray.from_numpy(np.ascontiguousarray(a, t)) for a, t in
+         ((vis, np.complex64), (wt, np.float32), (u, np.float32), (v, np.float32), (w, np.float32))]
+    g = DeviceArray((nz, nv, nu), np.complex64)
+    cube = DeviceArray((nz, ny, nx), np.float32)
+    sh = np.ascontiguousarray(np.eye(3).reshape(-1) if shift is None else shift, np.float64).reshape(-1)
+    sw = C.c_double(0.0)
+    rc = lib().sde_vistoimg_slowz_device(*[C.c_void_p(x.ptr) for x in d], C.c_int64(d[0].shape[0]),
+                                         C.c_float(uscale), C.c_float(vscale), C.c_int(uorigin), C.c_int(vorigin),
+                                         C.c_int(1 if psf else 0), C.c_int(nx), C.c_int(ny), C.c_int(fnu),
+                                         C.c_int(nv), C.c_char_p(cftype.encode()), C.c_int(nz), C.c_float(cellz),
+                                         C.c_float(refz), sh.ctypes.data_as(C.POINTER(C.c_double)),
+                                         C.c_void_p(g.ptr), C.c_void_p(cube.ptr), C.byref(sw))
+    if rc != 0:
+        msg = lib().sde_last_error().decode(errors="replace")
+        lib().sde_error_clear()
+        raise SdeError("sde_vistoimg_slowz_device: " + msg)
+    return cube.numpy(), g.numpy(), sw.value

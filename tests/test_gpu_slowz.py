@@ -167,3 +167,46 @@ def test_device_entry_points_match_the_drop_ins():
     with pytest.raises(sde.SdeError, match="NZ"):
         grd.grdh23d_device(*d, 1.0, 0.0, 1.0, 0.0, 1, vo, False, DeviceArray((129, nv, nu), np.complex64), dcf, supp,
                            osamp, cellz, refz)
+
+
+def _geometry(nx, cellsize):
+    fnu = po.fftpwr2(nx)
+    nu, nv = fnu // 2 + 1, po.fftpwr2(nx)
+    udelt = po.fftconj_delt(nx, np.float32(-cellsize) / np.float32(3600.0))
+    vdelt = po.fftconj_delt(nx, np.float32(cellsize) / np.float32(3600.0))
+    return nu, nv, (nv + 1) // 2, np.float32(np.float32(1.0) / np.float32(udelt)), \
+        np.float32(np.float32(1.0) / np.float32(vdelt))
+
+
+@pytest.mark.parametrize("psf", [False, True])
+def test_vistoimg_slowz_cube_matches_the_oracle(psf):
+    """The w-plane cube: per plane, the FP64 oracle's GRDH23D -> FFTR2D ->
+    PIXCOR2D; the image criterion of the 2-D maps (inner half within 1e-5
+    of the peak; the whole field within the reference's own FP32 error +
+    1e-5 of the peak)."""
+    from sde_amd import imaging, synth
+    from tests.tolerance import assert_image
+    d = synth.small_field()
+    nx, cell = d["nx"], d["cellsize"]
+    nz, refz = 4, 2.5
+    half = 0.5 * nx * cell / 3600.0 * np.pi / 180.0
+    cellz = float(np.float32((1.0 - np.sqrt(1.0 - 2.0 * half * half)) / 2.0 / (np.pi / 180.0)))
+    print("vistoimg slowz psf=%s n=%d nx=%d nz=%d" % (psf, len(d["u"]), nx, nz), flush=True)
+    cube, g, sw = imaging.vistoimg_slowz(d["vis"], d["wt"], d["u"], d["v"], d["w"], nx, cell, nz, cellz, refz,
+                                         psf=psf)
+    nu, nv, vo, us, vs = _geometry(nx, cell)
+    assert g.shape == (nz, nv, nu) and cube.shape == (nz, nx, nx)
+    cf, supp, osamp = po.grdcf("SF")
+    cf64, _, _ = po.grdcf("SF", fp64=True)
+    g64, sw64, rc = po.grdh23d64(d["vis"], d["wt"], d["u"], d["v"], d["w"], us, vs, 1, vo, psf, nu, nv, cf64, supp,
+                                 osamp, nz, cellz, refz, IDENT)
+    g32, _, _ = po.grdh23d(d["vis"], d["wt"], d["u"], d["v"], d["w"], us, vs, 1, vo, psf, nu, nv, cf, supp, osamp,
+                           nz, cellz, refz, IDENT)
+    assert rc == 0 and sw == pytest.approx(sw64, rel=1e-6)
+    assert_close_rel(g, g64, "planes")
+    ccflen = 1 + po.fftpwr2(nx) // 2
+    for iz in range(nz):
+        i64 = po.pixcor2d(po.fftr2d_c2r(g64[iz], nx, nx, fp64=True), po.grdccf("SF", ccflen, fp64=True), True,
+                          fp64=True)
+        i32 = po.pixcor2d(po.fftr2d_c2r(g32[iz], nx, nx), po.grdccf("SF", ccflen), True)
+        assert_image(cube[iz], i32, i64, "plane %d" % (iz + 1))
