@@ -356,6 +356,8 @@ def bench_wplanes(args):
                p["sigma"], p["seed"], p["nkeep"])
     vis, wt, u, v, w = plan.get_vis()
     geo = (plan.nu, plan.nv, plan.vorigin, plan.uscale, plan.vscale)
+    if args.copy_probe:
+        plan.copy_probe(1 << 30, reps=3)
     plan.close()
     nu, nv, vo, us, vs = geo
     nvis = len(u)

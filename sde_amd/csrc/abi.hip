@@ -524,14 +524,26 @@ long long slowz_degrid(Ctx &c, const float2 *dg, float2 *dvis, float *dwt, const
                        const float *dw, int64_t n, const CoordParams &cp, const PlaneParams &pp, const float *dcfu,
                        const float *dcfv) {
     hipStream_t s = c.stream;
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    const int nbits = slowz_key_bits(cp);
+    const size_t temp_bytes = sort_temp_bytes((int64_t)nn, nbits);
     unsigned long long *cnt = ctx_buf<unsigned long long>(c, "counters", 2);
     float2 *inner = ctx_buf<float2>(c, "grid_inner", (size_t)cp.nu * cp.nv * pp.nz);
-    if (!cnt || !inner) {
+    uint32_t *keys = ctx_buf<uint32_t>(c, "keys", nn), *skeys = ctx_buf<uint32_t>(c, "skeys", nn);
+    uint64_t *vals = ctx_buf<uint64_t>(c, "vals", nn), *svals = ctx_buf<uint64_t>(c, "svals", nn);
+    char *temp = ctx_buf<char>(c, "sort_temp", temp_bytes > 0 ? temp_bytes : 1);
+    if (!cnt || !inner || !keys || !skeys || !vals || !svals || !temp || temp_bytes == 0) {
         set_plan_error("device allocation failed");
         return -1;
     }
     SDE_CHECK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * 2, s));
-    launch_degrid_slowz(s, dg, inner, du, dv, dw, dwt, dvis, n, cp, dcfu, dcfv, pp, dvis, dwt, cnt);
+    // points in coarse cell order: neighbouring points share the planes' cache lines
+    launch_slowz_keys(s, du, dv, dw, dwt, n, cp, keys, vals);
+    if (n > 0 && launch_radix_sort(s, temp, temp_bytes, keys, skeys, vals, svals, n, nbits) != hipSuccess) {
+        set_plan_error("cell sort failed");
+        return -1;
+    }
+    launch_degrid_slowz(s, dg, inner, du, dv, dw, dwt, dvis, n, cp, dcfu, dcfv, pp, svals, dvis, dwt, cnt);
     SDE_CHECK(hipGetLastError());
     unsigned long long nbad = 0;
     SDE_CHECK(hipMemcpyAsync(&nbad, cnt, sizeof nbad, hipMemcpyDeviceToHost, s));

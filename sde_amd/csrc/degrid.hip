@@ -125,6 +125,36 @@ __global__ __launch_bounds__(256) void k_planes_inner(const float2 *__restrict__
         for (int iz = 0; iz < nz; iz++) out[c * nz + iz] = in[(size_t)iz * ncell + c];
 }
 
+// Coarse cell keys of the w-plane degridding: the point's REAL cell
+// (grd/grddeh23.f:92-114) in blocks of 8 x 8 cells, row-major; points with
+// WT <= 0 or off the grid take the last key.  vals = index.
+__global__ __launch_bounds__(256) void k_slowz_keys(const float *__restrict__ u, const float *__restrict__ v,
+                                                    const float *__restrict__ w, const float *__restrict__ wt,
+                                                    int64_t n, CoordParams cp, int nbu, int nbv,
+                                                    uint32_t *__restrict__ keys, uint64_t *__restrict__ vals) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        uint32_t key = (uint32_t)nbu * (uint32_t)nbv;
+        if (wt[i] > 0.0f) {
+            const double ug = u[i], vg = v[i], wg = w[i];
+            const float ul = (float)(cp.shift[0] * ug + cp.shift[1] * vg + cp.shift[2] * wg);
+            const float vl = (float)(cp.shift[3] * ug + cp.shift[4] * vg + cp.shift[5] * wg);
+            float ucell = cp.uscale * ul + cp.uoffset, vcell = cp.vscale * vl + cp.voffset;
+            if (ucell > 0.0f) {
+                ucell = ucell + (float)cp.uorigin;
+                vcell = vcell + (float)cp.vorigin;
+            } else {
+                ucell = -ucell + (float)cp.uorigin;
+                vcell = -vcell + (float)cp.vorigin;
+            }
+            const int nuc = nint_f(ucell), nvc = nint_f(vcell);
+            if (nuc >= 1 && nuc <= cp.nu && nvc >= 1 && nvc <= cp.nv)
+                key = (uint32_t)(((nvc - 1) >> 3) * nbu + ((nuc - 1) >> 3));
+        }
+        keys[i] = key;
+        vals[i] = (uint64_t)(uint32_t)i;
+    }
+}
+
 // GRDDEH23 (grd/grddeh23.f:86-210), one thread per visibility, 64 per
 // workgroup, on the cell-major planes of k_planes_inner: the point's NZ
 // plane rotations CROTZ are computed once into its own LDS column
@@ -134,14 +164,21 @@ __global__ __launch_bounds__(256) void k_planes_inner(const float2 *__restrict__
 // REAL arithmetic: (GVIS * CROTZ) * UVWT near the axis on the conjugating
 // side, (GVIS * UVWT) * CROTZ elsewhere.  A tap on a reflected row outside
 // the grid (which the reference reads out of bounds) adds nothing.
+//
+// The points are taken in the order of `order` (the low 32 bits of each
+// entry are a point's index; null: the caller's order): a coarse cell sort
+// makes the 64 points of a workgroup neighbours on the grid, so their taps
+// share cache lines instead of each fetching its own from HBM.
 __global__ __launch_bounds__(64) void k_degrid_slowz(
     const float2 *__restrict__ grid, const float *__restrict__ u, const float *__restrict__ v,
     const float *__restrict__ w, const float *__restrict__ wt_in, const float2 *vis_in, int64_t nvis,
     CoordParams cp, const float *__restrict__ cfu, const float *__restrict__ cfv, PlaneParams pp,
-    float2 *vis_out, float *__restrict__ wt_out, unsigned long long *__restrict__ nbad) {
+    const uint64_t *__restrict__ order, float2 *vis_out, float *__restrict__ wt_out,
+    unsigned long long *__restrict__ nbad) {
     extern __shared__ float2 crotz[];  // [NZ][64]
-    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
-    if (i >= nvis) return;
+    const int64_t j = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (j >= nvis) return;
+    const int64_t i = order ? (int64_t)(uint32_t)order[j] : j;
     float wti = wt_in[i];
     if (wti <= 0.0f) {
         wt_out[i] = wti;
@@ -501,17 +538,31 @@ void launch_degrid(hipStream_t s, const float2 *grid, const float *u, const floa
                        nvis, cp, cfu, cfv, vis_out, wt_out, nbad);
 }
 
+int slowz_key_bits(const CoordParams &cp) {
+    const uint32_t nkeys = (uint32_t)((cp.nu + 7) / 8) * (uint32_t)((cp.nv + 7) / 8);
+    int b = 1;
+    while (b < 32 && (1u << b) <= nkeys) b++;
+    return b;
+}
+
+void launch_slowz_keys(hipStream_t s, const float *u, const float *v, const float *w, const float *wt, int64_t n,
+                       const CoordParams &cp, uint32_t *keys, uint64_t *vals) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_slowz_keys, dim3(prep_blocks(n)), dim3(256), 0, s, u, v, w, wt, n, cp, (cp.nu + 7) / 8,
+                       (cp.nv + 7) / 8, keys, vals);
+}
+
 void launch_degrid_slowz(hipStream_t s, const float2 *grid, float2 *grid_inner, const float *u, const float *v,
                          const float *w, const float *wt_in, const float2 *vis_in, int64_t nvis,
                          const CoordParams &cp, const float *cfu, const float *cfv, const PlaneParams &pp,
-                         float2 *vis_out, float *wt_out, unsigned long long *nbad) {
+                         const uint64_t *order, float2 *vis_out, float *wt_out, unsigned long long *nbad) {
     if (nvis <= 0) return;
     const size_t ncell = (size_t)cp.nu * cp.nv;
     hipLaunchKernelGGL(k_planes_inner, dim3((unsigned)std::min<size_t>((ncell + 255) / 256, 8192)), dim3(256), 0, s,
                        grid, grid_inner, ncell, pp.nz);
     const int64_t nb = (nvis + 63) / 64;
     hipLaunchKernelGGL(k_degrid_slowz, dim3((unsigned)nb), dim3(64), sizeof(float2) * 64 * (size_t)pp.nz, s,
-                       grid_inner, u, v, w, wt_in, vis_in, nvis, cp, cfu, cfv, pp, vis_out, wt_out, nbad);
+                       grid_inner, u, v, w, wt_in, vis_in, nvis, cp, cfu, cfv, pp, order, vis_out, wt_out, nbad);
 }
 
 void launch_degrid_list(hipStream_t s, const float2 *grid, const float *u, const float *v, const float *w,

@@ -488,10 +488,17 @@ void launch_plane_phase(hipStream_t s, const float4 *zbase, StageHot *hot, int64
 // GRDDEH23, one thread per visibility, in the reference's loop order and
 // REAL arithmetic (grd/grddeh23.f:86-210); grid holds NZ planes of NU x NV,
 // re-laid cell-major into grid_inner (NU*NV*NZ float2 of scratch) first.
+// The points are taken in the order of `order` (index in the low 32 bits;
+// null: the caller's order).
 void launch_degrid_slowz(hipStream_t s, const float2 *grid, float2 *grid_inner, const float *u, const float *v,
                          const float *w, const float *wt_in, const float2 *vis_in, int64_t nvis,
                          const CoordParams &cp, const float *cfu, const float *cfv, const PlaneParams &pp,
-                         float2 *vis_out, float *wt_out, unsigned long long *nbad);
+                         const uint64_t *order, float2 *vis_out, float *wt_out, unsigned long long *nbad);
+// Coarse (8 x 8 cell) keys of the points for that order, with vals = index;
+// sort over slowz_key_bits(cp) bits.
+void launch_slowz_keys(hipStream_t s, const float *u, const float *v, const float *w, const float *wt, int64_t n,
+                       const CoordParams &cp, uint32_t *keys, uint64_t *vals);
+int slowz_key_bits(const CoordParams &cp);
 
 // ---- plain copy of n float4 (bandwidth probe, fft.hip) ----
 void launch_copy(hipStream_t s, const float4 *src, float4 *dst, size_t n);
