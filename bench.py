@@ -40,7 +40,7 @@ REGION_KERNELS = {
     # the density runs also write the tile ranges; the work items follow in the same region
     "wt_density": ["k_density_runs", "k_items_scan", "k_grid_items"],
     "wt_finish": ["k_wt_sumsq", "k_wt_finish"],
-    "grid_tiles": ["k_grid_tiles<16, 3, 4, true, 1>", "k_grid_totals"],
+    "grid_tiles": ["k_grid_tiles<16, 3, 4, true, 1, 0>", "k_grid_totals"],
     "grid_reduce": ["k_grid_reduce"],
     "degrid": ["k_degrid_tiles<16, 3>"],
     "degrid_axis": ["k_degrid_list"],

@@ -390,9 +390,9 @@ int run_sort(hipStream_t s, const float2 *vis, const float *wt, const float *u, 
 int run_grid(hipStream_t s, const CoordParams &cp, const TileParams &tp, int chmask, bool deg,
              const WeightParams &wp, const float *cfu, const float *cfv, const SortBufs &sb, DegridRec *rec_d,
              float *partial, double *item_sums, double *sums, float2 *dirty, float2 *psf, const MarkB &mb,
-             const MarkE &me) {
+             const MarkE &me, const ZPlanes *zp = nullptr) {
     mb("grid_tiles");
-    if (launch_grid_tiles(s, sb, tp, chmask, deg, wp, cfu, cfv, cp, rec_d, item_sums, sums, partial)) {
+    if (launch_grid_tiles(s, sb, tp, chmask, deg, wp, cfu, cfv, cp, rec_d, item_sums, sums, partial, zp)) {
         me();
         set_plan_error("unsupported gridding run (T=%d, channels %d, degridding %d)", tp.T, chmask, (int)deg);
         return -1;
@@ -459,8 +459,9 @@ PlaneParams make_pp(int nz, float cellz, float refz) {
 }
 
 // GRDH23D on device arrays: one cell sort of the points, then per plane the
-// plane's phase into the hot records, the tile kernel and the reduce (fold,
-// shift, normalise) into that plane of dg (NZ planes of NU x NV).  The
+// tile kernel in its w-plane mode (each record rotated by the plane's phase
+// as it is built) and the reduce (fold, shift, normalise) into that plane
+// of dg (NZ planes of NU x NV).  The
 // normalisation (NU-1)*NV*NZ/LSUMWT with LSUMWT summed over NZ identical
 // planes is (NU-1)*NV over one plane's LSUMWT.  Ends with the stream
 // synchronised; out = (SUMWT, one plane's LSUMWT).  The caller holds the
@@ -474,7 +475,7 @@ int slowz_grid(Ctx &c, const float2 *dvis, const float *dwt, const float *du, co
     SortBufs sb;
     sb.max_items = max_items_for(tp, n);
     sb.temp_bytes = sort_temp_bytes(n > 0 ? n : 1, sort_nbits(tp));
-    float4 *zbase = ctx_buf<float4>(c, "zbase", nn);
+    StageHot *zsorted = ctx_buf<StageHot>(c, "stage_sorted", nn);
     sb.keys = ctx_buf<uint32_t>(c, "keys", nn);
     sb.skeys = ctx_buf<uint32_t>(c, "skeys", nn);
     sb.vals = ctx_buf<uint64_t>(c, "vals", nn);
@@ -491,22 +492,30 @@ int slowz_grid(Ctx &c, const float2 *dvis, const float *dwt, const float *du, co
     double *blk = ctx_buf<double>(c, "blk", 2 * (size_t)sb.max_items), *dsums = ctx_buf<double>(c, "sums", 4);
     double *pblk = ctx_buf<double>(c, "prep_blk", 2 * (size_t)prep_blocks(n > 0 ? n : 1));
     double *dsw = ctx_buf<double>(c, "sumwt", 4);
-    if (!zbase || !sb.keys || !sb.skeys || !sb.vals || !sb.svals || !sb.hot || !sb.temp || sb.temp_bytes == 0 ||
+    if (!zsorted || !sb.keys || !sb.skeys || !sb.vals || !sb.svals || !sb.hot || !sb.temp || sb.temp_bytes == 0 ||
         !sb.tile_start || !sb.item_start || !sb.nitems || !sb.items || !partial || !blk || !dsums || !pblk || !dsw) {
         set_plan_error("device allocation failed");
         return -1;
     }
-    launch_prep_slowz(s, dvis, dwt, du, dv, dw, n, cp, tp, psf, pp, sb, zbase, pblk, dsw);
+    launch_prep_slowz(s, dvis, dwt, du, dv, dw, n, cp, tp, psf, pp, sb, pblk, dsw);
     if (launch_sort(s, sb, n, tp, true)) {
         set_plan_error("cell sort failed");
         return -1;
     }
     auto nb = [](const char *) {};
     auto ne = []() {};
+    // plane 1 gathers the staging records and keeps them in sorted order; the
+    // other planes read that copy
+    ZPlanes zp;
+    zp.twopi = pp.twopi;
+    zp.dtor = pp.dtor;
+    zp.cellz = pp.cellz;
+    zp.sorted = zsorted;
     for (int iz = 1; iz <= pp.nz; iz++) {
-        launch_plane_phase(s, zbase, sb.hot, n, pp, iz);
+        zp.mode = iz == 1 ? 1 : 2;
+        zp.dz = (float)iz - pp.refz;
         if (run_grid(s, cp, tp, 1, false, WeightParams(), dcfu, dcfv, sb, nullptr, partial, blk, dsums,
-                     dg + (size_t)(iz - 1) * np, nullptr, nb, ne))
+                     dg + (size_t)(iz - 1) * np, nullptr, nb, ne, &zp))
             return -1;
     }
     double sums[3], sw[3];

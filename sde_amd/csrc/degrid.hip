@@ -169,6 +169,9 @@ __global__ __launch_bounds__(256) void k_slowz_keys(const float *__restrict__ u,
 // entry are a point's index; null: the caller's order): a coarse cell sort
 // makes the 64 points of a workgroup neighbours on the grid, so their taps
 // share cache lines instead of each fetching its own from HBM.
+// NZC > 0 fixes the number of planes at compile time (the plane loops
+// unroll and a tap's loads issue together); 0 takes it from pp.
+template <int NZC>
 __global__ __launch_bounds__(64) void k_degrid_slowz(
     const float2 *__restrict__ grid, const float *__restrict__ u, const float *__restrict__ v,
     const float *__restrict__ w, const float *__restrict__ wt_in, const float2 *vis_in, int64_t nvis,
@@ -187,7 +190,7 @@ __global__ __launch_bounds__(64) void k_degrid_slowz(
     }
     const int cfoffu = (cp.suppu + 1) * cp.osampu + 1;
     const int cfoffv = (cp.suppv + 1) * cp.osampv + 1;
-    const int nz = pp.nz;
+    const int nz = NZC > 0 ? NZC : pp.nz;
     wti = -wti;
     const float uf = u[i], vf = v[i], wf = w[i];
     const double ug = uf, vg = vf, wg = wf;
@@ -216,7 +219,7 @@ __global__ __launch_bounds__(64) void k_degrid_slowz(
     const int delvi = nint_f((float)cp.osampv * ((float)nvc - vcell));
     const int ucen = nuc - cp.uorigin, vcen = nvc - cp.vorigin;
     float2 *cz = crotz + threadIdx.x;
-    for (int iz = 1; iz <= pp.nz; iz++) {
+    for (int iz = 1; iz <= nz; iz++) {
         const float ph = -(pp.twopi * wz * pp.dtor * pp.cellz * ((float)iz - pp.refz));
         cz[(iz - 1) * 64] = make_float2(cosf(ph), sinf(ph));
     }
@@ -561,8 +564,15 @@ void launch_degrid_slowz(hipStream_t s, const float2 *grid, float2 *grid_inner, 
     hipLaunchKernelGGL(k_planes_inner, dim3((unsigned)std::min<size_t>((ncell + 255) / 256, 8192)), dim3(256), 0, s,
                        grid, grid_inner, ncell, pp.nz);
     const int64_t nb = (nvis + 63) / 64;
-    hipLaunchKernelGGL(k_degrid_slowz, dim3((unsigned)nb), dim3(64), sizeof(float2) * 64 * (size_t)pp.nz, s,
-                       grid_inner, u, v, w, wt_in, vis_in, nvis, cp, cfu, cfv, pp, order, vis_out, wt_out, nbad);
+    const size_t lds = sizeof(float2) * 64 * (size_t)pp.nz;
+#define SDE_DSZ(NZC)                                                                                          \
+    hipLaunchKernelGGL(k_degrid_slowz<NZC>, dim3((unsigned)nb), dim3(64), lds, s, grid_inner, u, v, w, wt_in, \
+                       vis_in, nvis, cp, cfu, cfv, pp, order, vis_out, wt_out, nbad)
+    if (pp.nz == 4) SDE_DSZ(4);
+    else if (pp.nz == 8) SDE_DSZ(8);
+    else if (pp.nz == 16) SDE_DSZ(16);
+    else SDE_DSZ(0);
+#undef SDE_DSZ
 }
 
 void launch_degrid_list(hipStream_t s, const float2 *grid, const float *u, const float *v, const float *w,

@@ -453,12 +453,17 @@ __host__ __device__ inline int cf_row_stride(int osamp) {
 //
 // Occupancy: 5 waves per SIMD (at most 96 VGPRs, no spills), so the
 // gathers of one wave are covered by the tap loops of the others.
-template <int T, int CHM, int W, bool DEG, int WGT>
+//
+// ZP (w planes, GRDH23D): the records are rotated by the plane's phase as
+// they are built (grd/grdh23d.f:154-156); ZP = 1 also writes the gathered
+// staging records in sorted order, ZP = 2 reads them from there instead of
+// gathering.
+template <int T, int CHM, int W, bool DEG, int WGT, int ZP>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) void k_grid_tiles(
     const uint64_t *__restrict__ svals, const StageHot *__restrict__ hot, const StageCold *__restrict__ cold,
     int cold_all, const WorkItem *__restrict__ items, const int *__restrict__ nitems, const float *__restrict__ cfu_g,
     const float *__restrict__ cfv_g, int same_uv, CoordParams cp, TileParams tp, WeightParams wp,
-    DegridRec *__restrict__ rec_d, double *__restrict__ item_sums, float *__restrict__ partial) {
+    DegridRec *__restrict__ rec_d, double *__restrict__ item_sums, float *__restrict__ partial, ZPlanes zp) {
     constexpr int P = T + 2 * kHalo;
     constexpr int NC = P * P;
     constexpr int NCH = ((CHM & 1) ? 2 : 0) + ((CHM & 2) ? 1 : 0);
@@ -532,10 +537,10 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
     int cur = -1;
     if (r0 < r1) {  // wave-uniform; lanes past the end load the wave's last slot
         uint64_t v0 = svals[min(r0 + lane, r1 - 1)];
-        StageHot st0 = hot[sval_index(v0)];
+        StageHot st0 = ZP == 2 ? zp.sorted[min(r0 + lane, r1 - 1)] : hot[sval_index(v0)];
         uint64_t v1 = svals[min(r0 + 64 + lane, r1 - 1)];
         for (int b = r0; b < r1; b += 64) {
-            const StageHot st1 = hot[sval_index(v1)];
+            const StageHot st1 = ZP == 2 ? zp.sorted[min(b + 64 + lane, r1 - 1)] : hot[sval_index(v1)];
             const uint64_t v2 = svals[min(b + 128 + lane, r1 - 1)];
 
             // this lane's record of the batch: the staged gridding record
@@ -566,10 +571,20 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
                 sw += (double)wti;
                 ls += (double)wti * tsu[(mine.pk >> 16) & 0xff] * tsv[mine.pk >> 24];
                 mine.wt = wti;
+                float fr = st0.fr, fi = st0.fi;
+                if (ZP) {
+                    // ROTFVIS = FVIS * CMPLX(COS(PHASE), -SIN(PHASE))
+                    const float phase = -(zp.twopi * __uint_as_float(st0.dpk) * zp.dtor * zp.cellz * zp.dz);
+                    const float cr = cosf(phase), ci = -sinf(phase);
+                    const float a = fr, b2 = fi;
+                    fr = a * cr - b2 * ci;
+                    fi = a * ci + b2 * cr;
+                }
                 // the PSF grids VIS = 1
-                mine.re = PSF_ONLY ? wti : (float)((double)wti * (double)st0.fr);
-                mine.im = PSF_ONLY ? 0.0f : (float)((double)wti * (double)st0.fi);
+                mine.re = PSF_ONLY ? wti : (float)((double)wti * (double)fr);
+                mine.im = PSF_ONLY ? 0.0f : (float)((double)wti * (double)fi);
             }
+            if (ZP == 1 && valid) zp.sorted[p] = st0;
             if (DEG && valid) rec_d[p] = degrid_rec(st0, sc.rot, i0);
 
             // grid the batch; the table entries of the next record are
@@ -803,13 +818,14 @@ __device__ __forceinline__ float3 local_uvw(const CoordParams &cp, float uf, flo
 // Prep pass of GRDH23D, one thread per visibility in the caller's order
 // (grd/grdh23d.f:98-134): the shift rotation of VIS, the REAL cell step and
 // drop test, the reflection (conjugate VIS for u > 0, negate WLOCAL for the
-// reflected half), the PSF's VIS = 1, the cell key and the hot record's
-// cell and kernel phases.  No density cell, no degridding record.
+// reflected half), the PSF's VIS = 1, the cell key and the hot record:
+// cell and kernel phases, FVIS, and WLOCAL's bits in dpk (the tile gridder
+// rotates by each plane's phase).  No density cell, no degridding record.
 __global__ __launch_bounds__(kThreads) void k_prep_slowz(
     const float2 *__restrict__ vis, const float *__restrict__ wt, const float *__restrict__ u,
     const float *__restrict__ v, const float *__restrict__ w, int64_t n, CoordParams cp, TileParams tp,
     uint32_t sentinel, int psf, float twopi, uint32_t *__restrict__ keys, uint64_t *__restrict__ vals,
-    StageHot *__restrict__ hot, float4 *__restrict__ zbase, double *__restrict__ blk) {
+    StageHot *__restrict__ hot, double *__restrict__ blk) {
     const int poffu = (cp.osampu + 1) / 2, poffv = (cp.osampv + 1) / 2;
     double sw = 0.0, zero = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
@@ -820,7 +836,6 @@ __global__ __launch_bounds__(kThreads) void k_prep_slowz(
         sr.fr = 0.0f;
         sr.fi = 0.0f;
         sr.dpk = 0u;
-        float4 zb = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         uint32_t key = sentinel;
         if (wti > 0.0f) {
             const float wf = w[i];
@@ -856,11 +871,10 @@ __global__ __launch_bounds__(kThreads) void k_prep_slowz(
                          ((uint32_t)pv << 24);
                 sr.fr = fr;
                 sr.fi = fi;
-                zb = make_float4(fr, fi, wz, 0.0f);
+                sr.dpk = __float_as_uint(wz);
             }
         }
         hot[i] = sr;
-        zbase[i] = zb;
         keys[i] = key;
         vals[i] = (uint64_t)(uint32_t)i | ((uint64_t)__float_as_uint(wti) << 32);
     }
@@ -868,21 +882,6 @@ __global__ __launch_bounds__(kThreads) void k_prep_slowz(
     if (threadIdx.x == 0) {
         blk[2 * blockIdx.x] = sw;
         blk[2 * blockIdx.x + 1] = 0.0;
-    }
-}
-
-// ROTFVIS of plane iz for every point (grd/grdh23d.f:154-156), written into
-// the visibility of its hot record; the cell and phases stay.
-__global__ __launch_bounds__(kThreads) void k_plane_phase(const float4 *__restrict__ zbase,
-                                                          StageHot *__restrict__ hot, int64_t n, PlaneParams pp,
-                                                          float dz) {
-    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-        const float4 zb = zbase[i];
-        const float phase = -(pp.twopi * zb.z * pp.dtor * pp.cellz * dz);
-        const float cr = cosf(phase), ci = -sinf(phase);
-        float *rec = reinterpret_cast<float *>(hot + i);
-        rec[1] = zb.x * cr - zb.y * ci;
-        rec[2] = zb.x * ci + zb.y * cr;
     }
 }
 
@@ -999,10 +998,10 @@ size_t grid_partial_floats(const TileParams &tp, int chmask, int max_items) {
     return (size_t)max_items * nch * P * P;
 }
 
-template <int T, int CHM, int W, bool DEG, int WGT>
+template <int T, int CHM, int W, bool DEG, int WGT, int ZP>
 static void launch_tiles_t(hipStream_t s, const SortBufs &b, const TileParams &tp, const WeightParams &wp,
                            const float *cfu, const float *cfv, const CoordParams &cp, DegridRec *rec_d,
-                           double *item_sums, float *partial) {
+                           double *item_sums, float *partial, const ZPlanes &zp) {
     constexpr int P = T + 2 * kHalo;
     constexpr int NCH = ((CHM & 1) ? 2 : 0) + ((CHM & 2) ? 1 : 0);
     const int same_uv = cfu == cfv && cp.suppu == cp.suppv && cp.osampu == cp.osampv;
@@ -1010,27 +1009,33 @@ static void launch_tiles_t(hipStream_t s, const SortBufs &b, const TileParams &t
                           (same_uv ? 0 : (size_t)(2 * cp.suppv + 1) * cf_row_stride(cp.osampv));
     const size_t sums = (size_t)cp.osampu + 1 + (same_uv ? 0 : (size_t)cp.osampv + 1);
     const size_t lds = sizeof(double) * ((size_t)NCH * P * P + sums) + sizeof(float) * tables;
-    hipLaunchKernelGGL((k_grid_tiles<T, CHM, W, DEG, WGT>), dim3(b.max_items), dim3(64 * W), lds, s, b.svals,
+    hipLaunchKernelGGL((k_grid_tiles<T, CHM, W, DEG, WGT, ZP>), dim3(b.max_items), dim3(64 * W), lds, s, b.svals,
                        b.hot, b.cold, b.cold_all, b.items, b.nitems, cfu, cfv, same_uv, cp, tp, wp, rec_d, item_sums,
-                       partial);
+                       partial, zp);
 }
 
 template <int T, int W>
 static int launch_tiles_w(hipStream_t s, const SortBufs &b, const TileParams &tp, int chmask, bool deg,
                           const WeightParams &wp, const float *cfu, const float *cfv, const CoordParams &cp,
-                          DegridRec *rec_d, double *item_sums, float *partial) {
+                          DegridRec *rec_d, double *item_sums, float *partial, const ZPlanes &zp) {
     // weights: 0 as given, 1 robust/uniform, 2 robust with the azimuthal robustness
     const int wgt = wp.on ? (wp.az.on ? 2 : 1) : 0;
-    if (chmask == 1 && !deg && wgt == 0)
-        launch_tiles_t<T, 1, W, false, 0>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial);
+    if (zp.mode != 0) {
+        if (chmask != 1 || deg || wgt != 0 || !zp.sorted) return -1;
+        if (zp.mode == 1)
+            launch_tiles_t<T, 1, W, false, 0, 1>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial, zp);
+        else
+            launch_tiles_t<T, 1, W, false, 0, 2>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial, zp);
+    } else if (chmask == 1 && !deg && wgt == 0)
+        launch_tiles_t<T, 1, W, false, 0, 0>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial, zp);
     else if (chmask == 2 && !deg && wgt == 0)
-        launch_tiles_t<T, 2, W, false, 0>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial);
+        launch_tiles_t<T, 2, W, false, 0, 0>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial, zp);
     else if (chmask == 3 && deg && wgt == 0)
-        launch_tiles_t<T, 3, W, true, 0>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial);
+        launch_tiles_t<T, 3, W, true, 0, 0>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial, zp);
     else if (chmask == 3 && deg && wgt == 1)
-        launch_tiles_t<T, 3, W, true, 1>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial);
+        launch_tiles_t<T, 3, W, true, 1, 0>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial, zp);
     else if (chmask == 3 && deg && wgt == 2)
-        launch_tiles_t<T, 3, W, true, 2>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial);
+        launch_tiles_t<T, 3, W, true, 2, 0>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial, zp);
     else
         return -1;
     return 0;
@@ -1038,14 +1043,15 @@ static int launch_tiles_w(hipStream_t s, const SortBufs &b, const TileParams &tp
 
 int launch_grid_tiles(hipStream_t s, const SortBufs &b, const TileParams &tp, int chmask, bool deg,
                       const WeightParams &wp, const float *cfu, const float *cfv, const CoordParams &cp,
-                      DegridRec *rec_d, double *item_sums, double *sums, float *partial) {
+                      DegridRec *rec_d, double *item_sums, double *sums, float *partial, const ZPlanes *zpp) {
     if (b.max_items <= 0) return 0;
     const int W = tp.waves == 8 ? 8 : 4;
+    const ZPlanes zp = zpp ? *zpp : ZPlanes();
     int rc = -1;
-    if (tp.T == 16 && W == 4) rc = launch_tiles_w<16, 4>(s, b, tp, chmask, deg, wp, cfu, cfv, cp, rec_d, item_sums, partial);
-    else if (tp.T == 16 && W == 8) rc = launch_tiles_w<16, 8>(s, b, tp, chmask, deg, wp, cfu, cfv, cp, rec_d, item_sums, partial);
-    else if (tp.T == 32 && W == 4) rc = launch_tiles_w<32, 4>(s, b, tp, chmask, deg, wp, cfu, cfv, cp, rec_d, item_sums, partial);
-    else if (tp.T == 32 && W == 8) rc = launch_tiles_w<32, 8>(s, b, tp, chmask, deg, wp, cfu, cfv, cp, rec_d, item_sums, partial);
+    if (tp.T == 16 && W == 4) rc = launch_tiles_w<16, 4>(s, b, tp, chmask, deg, wp, cfu, cfv, cp, rec_d, item_sums, partial, zp);
+    else if (tp.T == 16 && W == 8) rc = launch_tiles_w<16, 8>(s, b, tp, chmask, deg, wp, cfu, cfv, cp, rec_d, item_sums, partial, zp);
+    else if (tp.T == 32 && W == 4) rc = launch_tiles_w<32, 4>(s, b, tp, chmask, deg, wp, cfu, cfv, cp, rec_d, item_sums, partial, zp);
+    else if (tp.T == 32 && W == 8) rc = launch_tiles_w<32, 8>(s, b, tp, chmask, deg, wp, cfu, cfv, cp, rec_d, item_sums, partial, zp);
     if (rc) return rc;
     const double num = (double)((float)(cp.nu - 1) * (float)cp.nv);
     hipLaunchKernelGGL(k_grid_totals, dim3(1), dim3(1024), 0, s, item_sums, b.max_items, num, sums);
@@ -1060,19 +1066,12 @@ void launch_grid_reduce(hipStream_t s, const float *partial, const int *item_sta
 
 void launch_prep_slowz(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
                        const float *w, int64_t n, const CoordParams &cp, const TileParams &tp, int psf,
-                       const PlaneParams &pp, SortBufs &b, float4 *zbase, double *blk, double *sumwt) {
+                       const PlaneParams &pp, SortBufs &b, double *blk, double *sumwt) {
     const int nb = prep_blocks(n > 0 ? n : 1);
     if (n > 0)
         hipLaunchKernelGGL(k_prep_slowz, dim3(nb), dim3(kThreads), 0, s, vis, wt, u, v, w, n, cp, tp,
-                           (uint32_t)sort_nkeys(tp), psf, pp.twopi, b.keys, b.vals, b.hot, zbase, blk);
+                           (uint32_t)sort_nkeys(tp), psf, pp.twopi, b.keys, b.vals, b.hot, blk);
     hipLaunchKernelGGL(k_grid_totals, dim3(1), dim3(1024), 0, s, blk, n > 0 ? nb : 0, 0.0, sumwt);
-}
-
-void launch_plane_phase(hipStream_t s, const float4 *zbase, StageHot *hot, int64_t n, const PlaneParams &pp,
-                        int iz) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_plane_phase, dim3(prep_blocks(n)), dim3(kThreads), 0, s, zbase, hot, n, pp,
-                       (float)iz - pp.refz);
 }
 
 void launch_box_grid(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,

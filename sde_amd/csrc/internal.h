@@ -215,6 +215,17 @@ void launch_subtract(hipStream_t s, float2 *vis, const float2 *mvis, const float
 void launch_degrid_gather(hipStream_t s, const SortBufs &b, int64_t n, const CoordParams &cp, const TileParams &tp,
                           DegridRec *rec_d);
 
+// The w-plane mode of the tile gridder: the hot records carry FVIS and
+// WLOCAL (bits in dpk); the gridding wave rotates each record by its
+// plane's phase -(TWOPI*WLOCAL*DTOR*CELLZ*DZ) while building it.  mode 1
+// gathers the records and writes them to `sorted` in sorted order; mode 2
+// (the following planes) reads them from there, coalesced.
+struct ZPlanes {
+    int mode = 0;
+    float twopi = 0.0f, dtor = 0.0f, cellz = 0.0f, dz = 0.0f;
+    StageHot *sorted = nullptr;
+};
+
 // ---- gridding (grid.hip) ----
 // chmask: bit 0 = dirty (2 floats), bit 1 = PSF (1 float).  One workgroup
 // per work item builds its records from the staging records (with the
@@ -224,7 +235,8 @@ void launch_degrid_gather(hipStream_t s, const SortBufs &b, int64_t n, const Coo
 // Supported: chmask 1 or 2 without deg and weights; chmask 3 with deg.
 int launch_grid_tiles(hipStream_t s, const SortBufs &b, const TileParams &tp, int chmask, bool deg,
                       const WeightParams &wp, const float *cfu, const float *cfv, const CoordParams &cp,
-                      DegridRec *rec_d, double *item_sums, double *sums, float *partial);
+                      DegridRec *rec_d, double *item_sums, double *sums, float *partial,
+                      const ZPlanes *zp = nullptr);
 void launch_grid_reduce(hipStream_t s, const float *partial, const int *item_start, const TileParams &tp,
                         int chmask, const CoordParams &cp, const double *sums, float2 *dirty, float2 *psf);
 size_t grid_partial_floats(const TileParams &tp, int chmask, int max_items);
@@ -474,17 +486,13 @@ struct PlaneParams {
 };
 constexpr int kMaxPlanes = 128;  // the degridder keeps 64 x NZ rotations in LDS (64 KB)
 // Keys, values and hot staging records of GRDH23D's per-point step in its
-// REAL coordinates (grd/grdh23d.f:98-130): the hot record carries FVIS of
-// plane 1 until launch_plane_phase rewrites it; zbase[i] = (FVIS, WLOCAL
-// after the reflection, 0).  SUMWT, the weights of every point with WT > 0
-// (dropped or not), goes to sumwt[0] (blk: 2 * prep_blocks(n) doubles).
+// REAL coordinates (grd/grdh23d.f:98-130): the hot record carries FVIS and
+// WLOCAL after the reflection (bits in dpk), for the tile gridder's
+// w-plane mode.  SUMWT, the weights of every point with WT > 0 (dropped or
+// not), goes to sumwt[0] (blk: 2 * prep_blocks(n) doubles).
 void launch_prep_slowz(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
                        const float *w, int64_t n, const CoordParams &cp, const TileParams &tp, int psf,
-                       const PlaneParams &pp, SortBufs &b, float4 *zbase, double *blk, double *sumwt);
-// The hot records' visibility for plane iz: ROTFVIS = FVIS * CMPLX(COS, -SIN)
-// of the plane's phase (grd/grdh23d.f:154-156).
-void launch_plane_phase(hipStream_t s, const float4 *zbase, StageHot *hot, int64_t n, const PlaneParams &pp,
-                        int iz);
+                       const PlaneParams &pp, SortBufs &b, double *blk, double *sumwt);
 // GRDDEH23, one thread per visibility, in the reference's loop order and
 // REAL arithmetic (grd/grddeh23.f:86-210); grid holds NZ planes of NU x NV,
 // re-laid cell-major into grid_inner (NU*NV*NZ float2 of scratch) first.

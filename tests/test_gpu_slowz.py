@@ -105,7 +105,7 @@ def test_grdh23d_no_weight_and_plane_limits():
 
 
 @pytest.mark.parametrize("shifted", [False, True])
-@pytest.mark.parametrize("nz,cellz,refz", [(4, 3.0, 2.0), (9, 1.5, 5.0)])
+@pytest.mark.parametrize("nz,cellz,refz", [(4, 3.0, 2.0), (8, 2.0, 4.5), (9, 1.5, 5.0), (16, 0.5, 8.0)])
 def test_grddeh23_matches_reference_arithmetic(shifted, nz, cellz, refz):
     nu, nv, vo = 65, 128, 64
     vis, wt, u, v, w = _points(5000, 44, 70, 60, near_axis=600)  # |u| up to 70: some points off the grid
