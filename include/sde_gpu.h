@@ -224,44 +224,9 @@ int64_t sde_plan_estimate_bytes(const sde_imaging_cfg *cfg, int64_t nvis);
 /* Free and total device memory (hipMemGetInfo). */
 int sde_device_memory(size_t *free_bytes, size_t *total_bytes);
 
-/* GRDH23D on device-resident arrays (the Fortran routine's arguments by
- * value; VIS, WT, U, V, W, GVIS (NU*NV*NZ complex) and the kernel tables in
- * device memory, SHIFT on the host), run on the library's stream and
- * synchronised before return; SUMWT to *sumwt.  For callers that keep
- * their visibilities on the GPU (the w-plane imaging of a major cycle).
- * Returns 0, or -1 with sde_last_error() set.  No reference counterpart: the
- * reference's only entry is grdh23d_. */
-int sde_grdh23d_device(const void *vis, const float *wt, const float *u, const float *v,
-                       const float *w, int64_t nvis, float uscale, float uoffset,
-                       float vscale, float voffset, int uorigin, int vorigin, void *gvis,
-                       int psf, int nu, int nv, const float *gridfnu, int suppu, int osampu,
-                       const float *gridfnv, int suppv, int osampv, int nz, float cellz,
-                       float refz, const double *shift, double *sumwt);
-
-/* VISTOIMG of a w-plane (SlowZ) cube on device-resident arrays
- * (vis/vistoimg.f:48-60 with CTYPE(3) = 'N----SIN'): GRDH23D with VISGRID's
- * scales (vis/visgrid.f:200-227: UOFFSET = VOFFSET = 0, kernel CFTYPE),
- * then per plane FFTR2D (arr/arrfftr.f:143-151) and PIXCOR2D 'CORRECT'
- * (img/imggridc.f:103-109).  gvis: NU*NV*NZ complex (NU = FNU/2+1), left
- * holding the gridded planes; images: NX*NY*NZ floats.  Returns 0, or -1
- * with sde_last_error() set.  No reference counterpart at this level (the
- * reference composes it from named objects). */
-int sde_vistoimg_slowz_device(const void *vis, const float *wt, const float *u, const float *v,
-                              const float *w, int64_t nvis, float uscale, float vscale,
-                              int uorigin, int vorigin, int psf, int nx, int ny, int fnu, int nv,
-                              const char *cftype, int nz, float cellz, float refz,
-                              const double *shift, void *gvis, float *images, double *sumwt);
-
-/* GRDDEH23 on device-resident arrays, in place on VIS and WT; returns the
- * number of points without kernel support (the reference's NBAD warning),
- * or -1 with sde_last_error() set.  No reference counterpart (see
- * sde_grdh23d_device). */
-int64_t sde_grddeh23_device(void *vis, float *wt, const float *u, const float *v,
-                            const float *w, int64_t nvis, float uscale, float uoffset,
-                            float vscale, float voffset, int uorigin, int vorigin,
-                            const void *gvis, int nu, int nv, const float *intfnu, int suppu,
-                            int osampu, const float *intfnv, int suppv, int osampv, int nz,
-                            float cellz, float refz, const double *shift);
+/* Copy the dirty image and PSF of the last uvmap (synchronous).  Returns -1
+ * with "GRDH2D: No weight!" when that uvmap gridded nothing (the reference's
+ * ERRREPOR in grd/grdh2d.f:195-197); the images are then undefined. */
 int sde_plan_get_images(sde_plan *plan, float *dirty, float *psf);
 /* The same copies issued on the plan's stream without waiting (complete when
  * sde_plan_query reports the plan done; pinned buffers for an asynchronous copy). */
@@ -313,6 +278,49 @@ int sde_plan_synth(sde_plan *plan, int nant, const double *enu, double lat, doub
                    double ha0, double dha, int ntime, int nchan, const double *freq,
                    int nsrc, const double *flux, const double *l, const double *m,
                    double sigma, uint64_t seed, int64_t nkeep);
+
+/* ------------------------------------------------------------------ */
+/* Part 3: w planes (SlowZ) on device-resident arrays                   */
+/* ------------------------------------------------------------------ */
+
+/* GRDH23D on device-resident arrays (the Fortran routine's arguments by
+ * value; VIS, WT, U, V, W, GVIS (NU*NV*NZ complex) and the kernel tables in
+ * device memory, SHIFT on the host), run on the library's stream and
+ * synchronised before return; SUMWT to *sumwt.  For callers that keep
+ * their visibilities on the GPU (the w-plane imaging of a major cycle).
+ * Returns 0, or -1 with sde_last_error() set.  No reference counterpart: the
+ * reference's only entry is grdh23d_. */
+int sde_grdh23d_device(const void *vis, const float *wt, const float *u, const float *v,
+                       const float *w, int64_t nvis, float uscale, float uoffset,
+                       float vscale, float voffset, int uorigin, int vorigin, void *gvis,
+                       int psf, int nu, int nv, const float *gridfnu, int suppu, int osampu,
+                       const float *gridfnv, int suppv, int osampv, int nz, float cellz,
+                       float refz, const double *shift, double *sumwt);
+
+/* VISTOIMG of a w-plane (SlowZ) cube on device-resident arrays
+ * (vis/vistoimg.f:48-60 with CTYPE(3) = 'N----SIN'): GRDH23D with VISGRID's
+ * scales (vis/visgrid.f:200-227: UOFFSET = VOFFSET = 0, kernel CFTYPE),
+ * then per plane FFTR2D (arr/arrfftr.f:143-151) and PIXCOR2D 'CORRECT'
+ * (img/imggridc.f:103-109).  gvis: NU*NV*NZ complex (NU = FNU/2+1), left
+ * holding the gridded planes; images: NX*NY*NZ floats.  Returns 0, or -1
+ * with sde_last_error() set.  No reference counterpart at this level (the
+ * reference composes it from named objects). */
+int sde_vistoimg_slowz_device(const void *vis, const float *wt, const float *u, const float *v,
+                              const float *w, int64_t nvis, float uscale, float vscale,
+                              int uorigin, int vorigin, int psf, int nx, int ny, int fnu, int nv,
+                              const char *cftype, int nz, float cellz, float refz,
+                              const double *shift, void *gvis, float *images, double *sumwt);
+
+/* GRDDEH23 on device-resident arrays, in place on VIS and WT; returns the
+ * number of points without kernel support (the reference's NBAD warning),
+ * or -1 with sde_last_error() set.  No reference counterpart (see
+ * sde_grdh23d_device). */
+int64_t sde_grddeh23_device(void *vis, float *wt, const float *u, const float *v,
+                            const float *w, int64_t nvis, float uscale, float uoffset,
+                            float vscale, float voffset, int uorigin, int vorigin,
+                            const void *gvis, int nu, int nv, const float *intfnu, int suppu,
+                            int osampu, const float *intfnv, int suppv, int osampv, int nz,
+                            float cellz, float refz, const double *shift);
 
 #ifdef __cplusplus
 }

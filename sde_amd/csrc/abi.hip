@@ -1128,6 +1128,7 @@ struct sde_plan {
     // the last uvmap stand for the resident uvw and weights (residual imaging)
     bool sorted_ok = false;
     bool predicted = false;  // model VIS/WT of the resident visibilities are complete
+    bool imaged = false;     // the images hold a uvmap's output (sums valid)
     DegridRec *d_recd = nullptr;
     double *d_wsq = nullptr;
     int *d_list = nullptr, *d_nlist = nullptr;
@@ -1402,6 +1403,7 @@ int plan_image(sde_plan *p, const WeightParams &wp) {
                  p->d_sums, gd, gp, mb, me))
         return -1;
     p->filled = true;
+    p->imaged = true;
     float2 *twnv = p->tw[p->nv], *twfnu = p->tw[p->fnu];
     plan_mark_begin(p, "fft_cols");
     // out of place: the gridded planes stay readable (sde_plan_get_grids)
@@ -1809,9 +1811,16 @@ int64_t sde_grddeh23_device(void *vis, float *wt, const float *u, const float *v
 int sde_plan_get_images(sde_plan *p, float *dirty, float *psf) {
     if (!p) return -1;
     const size_t npix = (size_t)p->cfg.nx * p->cfg.ny;
+    double sums[2] = {0.0, 0.0};
     if (dirty) SDE_CHECK(hipMemcpyAsync(dirty, p->d_dirty, sizeof(float) * npix, hipMemcpyDeviceToHost, p->stream));
     if (psf) SDE_CHECK(hipMemcpyAsync(psf, p->d_psf, sizeof(float) * npix, hipMemcpyDeviceToHost, p->stream));
+    if (p->imaged) SDE_CHECK(hipMemcpyAsync(sums, p->d_sums, sizeof sums, hipMemcpyDeviceToHost, p->stream));
     SDE_CHECK(hipStreamSynchronize(p->stream));
+    // GRDH2D's error when nothing was gridded (grd/grdh2d.f:195-197): the images are not defined
+    if (p->imaged && sums[1] == 0.0) {
+        set_plan_error("GRDH2D: No weight!");
+        return -1;
+    }
     return 0;
 }
 
