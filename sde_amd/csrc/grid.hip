@@ -302,9 +302,24 @@ __global__ __launch_bounds__(1024) void k_items_scan(const int *__restrict__ til
 }
 
 // Density grid from the sorted keys and weights (points whose density cell
-// is their gridding cell): a segmented sum over each wave's runs of equal
-// keys, one FP64 atomic per run.  Flagged points were added by the prep
-// pass.  The same pass writes the tile ranges (as k_tile_bounds).
+// is their gridding cell): a segmented sum over the runs of equal keys, one
+// FP64 atomic per run (per wave chunk).  Flagged points were added by the
+// prep pass.  The same pass writes the tile ranges (as k_tile_bounds).
+//
+// Each lane takes 4 consecutive slots (one 16-byte key load, two 16-byte
+// value loads): runs inside a lane are summed there; the run open at a
+// lane's end is carried across lanes by one segmented scan per wave (a lane
+// whose 4 keys are equal extends the run of the lane before it).
+__device__ __forceinline__ void density_add(uint32_t key, double x, uint32_t nkeys, const TileParams &tp, int nu,
+                                            int suppu, double *__restrict__ gwt) {
+    if (key >= nkeys || x == 0.0) return;
+    const int tt = tp.T * tp.T;
+    const int tile = (int)(key / (uint32_t)tt), rem = (int)(key - (uint32_t)tile * tt);
+    const int lv = rem / tp.T, lu = rem - lv * tp.T;
+    const int cu = (tile % tp.ntu) * tp.T + lu, cv = (tile / tp.ntu) * tp.T + lv;
+    atomicAdd(&gwt[(size_t)cv * nu + (cu - suppu)], x);
+}
+
 __global__ __launch_bounds__(kThreads) void k_density_runs(
     const uint32_t *__restrict__ skeys, const uint64_t *__restrict__ svals, int64_t n, uint32_t nkeys,
     TileParams tp, int nu, int suppu, double *__restrict__ gwt, double *__restrict__ blk,
@@ -312,40 +327,84 @@ __global__ __launch_bounds__(kThreads) void k_density_runs(
     const int lane = threadIdx.x & 63;
     const int tt = tp.T * tp.T;
     double dsum = 0.0, unused = 0.0;
-    const int64_t stride = (int64_t)gridDim.x * kThreads;
-    for (int64_t base = (int64_t)blockIdx.x * kThreads; base < n; base += stride) {
-        const int64_t p = base + threadIdx.x;
-        const bool inr = p < n;
-        const uint32_t k = inr ? skeys[p] : 0xffffffffu;
-        const uint32_t key = k & ~kKeyFlag;
-        const bool valid = inr && key < nkeys;
-        double x = (valid && !(k & kKeyFlag)) ? (double)sval_wt(svals[p]) : 0.0;
-        dsum += x;
-        uint32_t prev = (uint32_t)__shfl_up((int)key, 1);
-        if (lane == 0) prev = (inr && p > 0) ? (skeys[p - 1] & ~kKeyFlag) : 0xffffffffu;
-        if (inr) {
-            const int t = tile_of_key(key, nkeys, tt, tp.ntiles);
-            const int tprev = p == 0 ? -1 : tile_of_key(prev, nkeys, tt, tp.ntiles);
-            for (int q = tprev + 1; q <= t; q++) tile_start[q] = (int)p;
-            if (p == n - 1)
-                for (int q = t + 1; q <= tp.ntiles; q++) tile_start[q] = (int)n;
+    const int64_t stride = (int64_t)gridDim.x * kThreads * 4;
+    for (int64_t base = (int64_t)blockIdx.x * kThreads * 4; base < n; base += stride) {
+        const int64_t q0 = base + 4 * (int64_t)threadIdx.x;
+        uint32_t kk[4];
+        double xx[4];
+        if (q0 + 3 < n) {
+            const uint4 k4 = *reinterpret_cast<const uint4 *>(skeys + q0);
+            const ulonglong2 va = *reinterpret_cast<const ulonglong2 *>(svals + q0);
+            const ulonglong2 vb = *reinterpret_cast<const ulonglong2 *>(svals + q0 + 2);
+            const uint32_t kr[4] = {k4.x, k4.y, k4.z, k4.w};
+            const uint64_t vr[4] = {va.x, va.y, vb.x, vb.y};
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                kk[e] = kr[e] & ~kKeyFlag;
+                xx[e] = (kk[e] < nkeys && !(kr[e] & kKeyFlag)) ? (double)sval_wt(vr[e]) : 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int64_t p = q0 + e;
+                const uint32_t k = p < n ? skeys[p] : 0xffffffffu;
+                kk[e] = p < n ? (k & ~kKeyFlag) : 0xffffffffu;
+                xx[e] = (p < n && kk[e] < nkeys && !(k & kKeyFlag)) ? (double)sval_wt(svals[p]) : 0.0;
+            }
         }
-        const bool head = lane == 0 || prev != key;
+        // key of the slot before this lane's first
+        uint32_t prev = (uint32_t)__shfl_up((int)kk[3], 1);
+        if (lane == 0) prev = (q0 > 0 && q0 - 1 < n) ? (skeys[q0 - 1] & ~kKeyFlag) : 0xffffffffu;
+        // tile ranges
+        uint32_t kp = q0 == 0 ? 0xffffffffu : prev;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int64_t p = q0 + e;
+            if (p < n) {
+                const int t = tile_of_key(kk[e], nkeys, tt, tp.ntiles);
+                const int tprev = p == 0 ? -1 : tile_of_key(kp, nkeys, tt, tp.ntiles);
+                for (int q = tprev + 1; q <= t; q++) tile_start[q] = (int)p;
+                if (p == n - 1)
+                    for (int q = t + 1; q <= tp.ntiles; q++) tile_start[q] = (int)n;
+            }
+            kp = kk[e];
+        }
+        // runs inside the lane
+        double first = 0.0;
+        int e0 = 0;
+        while (e0 < 4 && kk[e0] == kk[0]) first += xx[e0++];
+        const bool same = e0 == 4;
+        double last = 0.0;
+        if (!same) {
+            int t0 = 3;
+            while (kk[t0 - 1] == kk[3]) t0--;
+            for (int e = t0; e < 4; e++) last += xx[e];
+            int j = e0;
+            while (j < t0) {
+                const uint32_t kj = kk[j];
+                double sj = 0.0;
+                while (j < t0 && kk[j] == kj) sj += xx[j++];
+                density_add(kj, sj, nkeys, tp, nu, suppu, gwt);
+            }
+        }
+        dsum += xx[0] + xx[1] + xx[2] + xx[3];
+        // the run open at each lane's end: a segmented scan over the lanes
+        const bool cont = kk[0] == prev;  // this lane's first run continues the previous lane's last
+        double c = same ? first : last;
+        const bool head = lane == 0 || !same || !cont;
         const unsigned long long heads = __ballot(head);
         const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
         const int seg = 63 - __clzll((long long)(heads & upto));
         for (int d = 1; d < 64; d <<= 1) {
-            const double y = __shfl_up(x, d);
-            if (lane - d >= seg) x += y;
+            const double y = __shfl_up(c, d);
+            if (lane - d >= seg) c += y;
         }
-        const uint32_t next = (uint32_t)__shfl_down((int)key, 1);
-        const bool tail = lane == 63 || next != key;
-        if (valid && tail && x != 0.0) {
-            const int tile = (int)(key / (uint32_t)tt), rem = (int)(key - (uint32_t)tile * tt);
-            const int lv = rem / tp.T, lu = rem - lv * tp.T;
-            const int cu = (tile % tp.ntu) * tp.T + lu, cv = (tile / tp.ntu) * tp.T + lv;
-            atomicAdd(&gwt[(size_t)cv * nu + (cu - suppu)], x);
-        }
+        // carry into this lane: the previous lane's open run, when it continues here
+        double cin = __shfl_up(c, 1);
+        if (lane == 0 || !cont) cin = 0.0;
+        if (!same) density_add(kk[0], cin + first, nkeys, tp, nu, suppu, gwt);
+        const uint32_t next = (uint32_t)__shfl_down((int)kk[0], 1);
+        if (lane == 63 || next != kk[3]) density_add(kk[3], c, nkeys, tp, nu, suppu, gwt);
     }
     block_sum2(dsum, unused);
     if (threadIdx.x == 0) blk[blockIdx.x] = dsum;
