@@ -76,6 +76,16 @@ void grdh23d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis,
               int *osampv, int *nz, float *cellz, float *refz, double *shift,
               float *sumwt);
 
+/* Nearest-cell (BOX) gridding onto NZ w planes GVIS(NU,NV,NZ): plane IZ
+ * receives WT*VIS rotated by TWOPI*W*DTOR*CELLZ*(REFZ-IZ); the u-origin
+ * column folded, shifted left by one and normalised by (NU-1)*NV/SUMWT
+ * (SUMWT over every point with WT > 0).  NZ <= 128.  Replaces GRDQH23D,
+ * source/grd/grdqh23d.f:7-9 (args :53-61). */
+void grdqh23d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis,
+               float *uscale, float *uoffset, float *vscale, float *voffset,
+               int *uorigin, int *vorigin, float *gvis, int *psf, int *nu, int *nv,
+               int *nz, float *cellz, float *refz, double *shift, float *sumwt);
+
 /* Degridding from the NZ planes GVIS(NU,NV,NZ): every tap summed over the
  * planes rotated back by their phases; VIS and the WT sign flag as
  * GRDDEH2D's.  NZ <= 128.  Replaces GRDDEH23, source/grd/grddeh23.f:7-10
@@ -300,8 +310,8 @@ int sde_grdh23d_device(const void *vis, const float *wt, const float *u, const f
 /* VISTOIMG of a w-plane (SlowZ) cube on device-resident arrays
  * (vis/vistoimg.f:48-60 with CTYPE(3) = 'N----SIN'): GRDH23D with VISGRID's
  * scales (vis/visgrid.f:200-227: UOFFSET = VOFFSET = 0, kernel CFTYPE),
- * then per plane FFTR2D (arr/arrfftr.f:143-151) and PIXCOR2D 'CORRECT'
- * (img/imggridc.f:103-109).  gvis: NU*NV*NZ complex (NU = FNU/2+1), left
+ * GRDQH23D for CFTYPE BOX (vis/visgrid.f:212-226), then per plane FFTR2D
+ * (arr/arrfftr.f:143-151) and PIXCOR2D 'CORRECT' (img/imggridc.f:103-109).  gvis: NU*NV*NZ complex (NU = FNU/2+1), left
  * holding the gridded planes; images: NX*NY*NZ floats.  Returns 0, or -1
  * with sde_last_error() set.  No reference counterpart at this level (the
  * reference composes it from named objects). */

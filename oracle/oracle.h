@@ -98,6 +98,14 @@ void o64_grddeh23(double *vis, float *wt, const float *u, const float *v, const 
                   int uorigin, int vorigin, const double *gvis, int nu, int nv,
                   const double *intfn, int supp, int osamp, int nz, float cellz, float refz,
                   const double *shift, int *nbad);
+int r32_grdqh23d(const float *vis, const float *wt, const float *u, const float *v,
+                 const float *w, int nvis, float uscale, float uoffset, float vscale,
+                 float voffset, int uorigin, int vorigin, float *gvis, int psf, int nu,
+                 int nv, int nz, float cellz, float refz, const double *shift, float *sumwt);
+int o64_grdqh23d(const float *vis, const double *wt, const float *u, const float *v,
+                 const float *w, int nvis, float uscale, float uoffset, float vscale,
+                 float voffset, int uorigin, int vorigin, double *gvis, int psf, int nu,
+                 int nv, int nz, float cellz, float refz, const double *shift, double *sumwt);
 
 /* ---- density-grid weighting (grd/grduwt2d.f, grd/grdruw2d.f) ---- */
 void r32_grduwt2d(const float *wt, const float *u, const float *v, const float *w,

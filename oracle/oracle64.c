@@ -773,3 +773,64 @@ void o64_grddeh23(double *vis, float *wt, const float *u, const float *v, const 
         vis[2 * i + 1] = vr * ri + vi * rr;
     }
 }
+
+/* grd/grdqh23d.f:73-158 in DOUBLE (REAL cell decisions) */
+int o64_grdqh23d(const float *vis, const double *wt, const float *u, const float *v,
+                 const float *w, int nvis, float uscale, float uoffset, float vscale,
+                 float voffset, int uorigin, int vorigin, double *gvis, int psf, int nu,
+                 int nv, int nz, float cellz, float refz, const double *shift, double *sumwt) {
+    const double twopi = (double)(8.0f * atanf(1.0f));
+    const double dtor = twopi / 360.0;
+    memset(gvis, 0, sizeof(double) * 2 * (size_t)nu * nv * nz);
+    *sumwt = 0.0;
+    for (int i = 0; i < nvis; i++) {
+        float ul, vl, wl, ucell, vcell;
+        if (wt[i] <= 0.0) continue;
+        local_uvw3(shift, u[i], v[i], w[i], &ul, &vl, &wl);
+        const double phase = twopi * ((double)wl - (double)w[i]);
+        const double c = cos(phase), s = -sin(phase);
+        const double ar = wt[i] * vis[2 * i], ai = wt[i] * vis[2 * i + 1];
+        double fr = ar * c - ai * s, fi = ar * s + ai * c;
+        double wz = wl;
+        *sumwt += wt[i];
+        ucell = uscale * ul + uoffset;
+        vcell = vscale * vl + voffset;
+        if (ucell > 0.0f) {
+            fi = -fi;
+            ucell = ucell + (float)uorigin;
+            vcell = vcell + (float)vorigin;
+        } else {
+            ucell = -ucell + (float)uorigin;
+            vcell = -vcell + (float)vorigin;
+            wz = -wz;
+        }
+        if (psf) { fr = wt[i]; fi = 0.0; }
+        const int ugrid = nintf(ucell) + 1, vgrid = nintf(vcell);
+        if (ugrid < 1 || ugrid > nu || vgrid < 1 || vgrid > nv) continue;
+        for (int iz = 1; iz <= nz; iz++) {
+            const double ph = twopi * wz * dtor * (double)cellz * ((double)refz - (double)iz);
+            const double cr = cos(ph), ci = -sin(ph);
+            GRE3(gvis, nu, nv, ugrid, vgrid, iz) += fr * cr - fi * ci;
+            GIM3(gvis, nu, nv, ugrid, vgrid, iz) += fr * ci + fi * cr;
+        }
+    }
+    const int uz = uorigin + 1, vz = vorigin;
+    for (int iz = 1; iz <= nz; iz++)
+        for (int offv = 0; offv <= nv / 2 - 1; offv++) {
+            GRE3(gvis, nu, nv, uz, vz + offv, iz) += GRE3(gvis, nu, nv, uz, vz - offv, iz);
+            GIM3(gvis, nu, nv, uz, vz + offv, iz) -= GIM3(gvis, nu, nv, uz, vz - offv, iz);
+            GRE3(gvis, nu, nv, uz, vz - offv, iz) = GRE3(gvis, nu, nv, uz, vz + offv, iz);
+            GIM3(gvis, nu, nv, uz, vz - offv, iz) = -GIM3(gvis, nu, nv, uz, vz + offv, iz);
+        }
+    if (*sumwt == 0.0) return 1;
+    const double rnorm = (double)(nu - 1) * (double)nv / *sumwt;
+    for (int iz = 1; iz <= nz; iz++)
+        for (int vg = 1; vg <= nv; vg++) {
+            for (int ug = 1; ug <= nu - 1; ug++) {
+                GRE3(gvis, nu, nv, ug, vg, iz) = rnorm * GRE3(gvis, nu, nv, ug + 1, vg, iz);
+                GIM3(gvis, nu, nv, ug, vg, iz) = rnorm * GIM3(gvis, nu, nv, ug + 1, vg, iz);
+            }
+            GRE3(gvis, nu, nv, nu, vg, iz) = GIM3(gvis, nu, nv, nu, vg, iz) = 0.0;
+        }
+    return 0;
+}

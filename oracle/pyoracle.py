@@ -104,6 +104,12 @@ for name, res, args in [
     ("o64_grddeh23", None, [_d, _f, _f, _f, _f, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
                             C.c_int, C.c_int, _d, C.c_int, C.c_int, _d, C.c_int, C.c_int, C.c_int,
                             C.c_float, C.c_float, _d, _i]),
+    ("r32_grdqh23d", C.c_int, [_f, _f, _f, _f, _f, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
+                               C.c_int, C.c_int, _f, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                               _d, _f]),
+    ("o64_grdqh23d", C.c_int, [_f, _d, _f, _f, _f, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
+                               C.c_int, C.c_int, _d, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                               _d, _d]),
     ("r32_grduwt2d", None, [_f, _f, _f, _f, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
                             C.c_int, _f, _f, C.c_int, C.c_int, _d, _i]),
     ("r32_grdruw2d", C.c_int, [_f, _f, _f, _f, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
@@ -312,6 +318,26 @@ def grddeh23_64(wt, u, v, w, uscale, vscale, uorigin, vorigin, grid, cf, supp, o
                      uorigin, vorigin, _pd(grid), nu, nv, _pd(cf), supp, osamp, nz, cellz, refz,
                      _pd(np.ascontiguousarray(shift, np.float64)), C.byref(nbad))
     return vis, wt, nbad.value
+
+
+def grdqh23d(vis, wt, u, v, w, uscale, vscale, uorigin, vorigin, psf, nu, nv, nz, cellz, refz, shift,
+             uoffset=0.0, voffset=0.0, fp64=False):
+    """GRDQH23D restated (REAL, or DOUBLE with fp64) -> (GVIS (NZ, NV, NU), SUMWT, rc)."""
+    vis, u, v, w = c64(vis), f32(u), f32(v), f32(w)
+    sh = _pd(np.ascontiguousarray(shift, np.float64))
+    if fp64:
+        wt = np.ascontiguousarray(wt, np.float64)
+        g = np.zeros((nz, nv, nu), np.complex128)
+        sw = C.c_double()
+        rc = lib.o64_grdqh23d(_pf(vis), _pd(wt), _pf(u), _pf(v), _pf(w), len(u), uscale, uoffset, vscale, voffset,
+                              uorigin, vorigin, _pd(g), int(psf), nu, nv, nz, cellz, refz, sh, C.byref(sw))
+        return g, sw.value, rc
+    wt = f32(wt)
+    g = np.zeros((nz, nv, nu), np.complex64)
+    sw = np.zeros(1, np.float32)
+    rc = lib.r32_grdqh23d(_pf(vis), _pf(wt), _pf(u), _pf(v), _pf(w), len(u), uscale, uoffset, vscale, voffset,
+                          uorigin, vorigin, _pf(g), int(psf), nu, nv, nz, cellz, refz, sh, _pf(sw))
+    return g, float(sw[0]), rc
 
 
 def grduwt2d(wt, u, v, w, uscale, vscale, uorigin, vorigin, nu, nv, shift, uoffset=0.0, voffset=0.0):

@@ -21,7 +21,7 @@ FFLAGS ?= -O2
 FPP = -cpp -ffixed-form -fno-automatic -DCOMP_GNU -DMACH_LINUX -DOS_LINUX -I$(REF)/include
 
 SRC = $(REF)/grd/grdh2d.f $(REF)/grd/grddeh2d.f $(REF)/grd/grdqh2d.f $(REF)/grd/grdcf.f \
-      $(REF)/grd/grdh23d.f $(REF)/grd/grddeh23.f \
+      $(REF)/grd/grdh23d.f $(REF)/grd/grddeh23.f $(REF)/grd/grdqh23d.f \
       $(REF)/grd/grdsf.f $(REF)/grd/grdsze.f $(REF)/grd/grdccf.f $(REF)/grd/grduwt2d.f \
       $(REF)/grd/grdruw2d.f $(REF)/pix/pixcor2d.f $(REF)/pix/pixcorx2.f $(REF)/fft/fftr2d.f \
       $(REF)/fft/fftr1d2.f $(REF)/fft/fftr1d.f $(REF)/fft/fftr.f $(REF)/fft/fftx1d.f \

@@ -1239,3 +1239,66 @@ void r32_grddeh23(float *vis, float *wt, const float *u, const float *v, const f
         cmul(vr, vi, rr, ri, &vis[2 * i], &vis[2 * i + 1]);
     }
 }
+
+/* grd/grdqh23d.f:73-158: nearest-cell gridding onto NZ w planes */
+int r32_grdqh23d(const float *vis, const float *wt, const float *u, const float *v,
+                 const float *w, int nvis, float uscale, float uoffset, float vscale,
+                 float voffset, int uorigin, int vorigin, float *gvis, int psf, int nu,
+                 int nv, int nz, float cellz, float refz, const double *shift, float *sumwt) {
+    const float twopi = 8.0f * atanf(1.0f);
+    const float dtor = twopi / 360.0f;
+    memset(gvis, 0, sizeof(float) * 2 * (size_t)nu * nv * nz);
+    *sumwt = 0.0f;
+    for (int i = 0; i < nvis; i++) {
+        float ul, vl, wl, fr, fi, ucell, vcell;
+        if (wt[i] <= 0.0f) continue;
+        local_uvw(shift, u[i], v[i], w[i], &ul, &vl, &wl);
+        const float phase = twopi * (wl - w[i]);
+        /* (WT * VIS) * CMPLX(COS, -SIN): REAL * COMPLEX keeps its 0 terms */
+        const float ar = wt[i] * vis[2 * i] - 0.0f * vis[2 * i + 1];
+        const float ai = wt[i] * vis[2 * i + 1] + 0.0f * vis[2 * i];
+        cmul(ar, ai, cosf(phase), -sinf(phase), &fr, &fi);
+        *sumwt = *sumwt + wt[i];
+        ucell = uscale * ul + uoffset;
+        vcell = vscale * vl + voffset;
+        if (ucell > 0.0f) {
+            fi = -fi;
+            ucell = ucell + (float)uorigin;
+            vcell = vcell + (float)vorigin;
+        } else {
+            ucell = -ucell + (float)uorigin;
+            vcell = -vcell + (float)vorigin;
+            wl = -wl;
+        }
+        if (psf) { fr = wt[i]; fi = 0.0f; }
+        const int ugrid = nintf(ucell) + 1, vgrid = nintf(vcell);
+        if (ugrid < 1 || ugrid > nu || vgrid < 1 || vgrid > nv) continue;
+        for (int iz = 1; iz <= nz; iz++) {
+            const float ph = twopi * wl * dtor * cellz * (refz - (float)iz);
+            float rr, ri;
+            cmul(fr, fi, cosf(ph), -sinf(ph), &rr, &ri);
+            GRE3(gvis, nu, nv, ugrid, vgrid, iz) = GRE3(gvis, nu, nv, ugrid, vgrid, iz) + rr;
+            GIM3(gvis, nu, nv, ugrid, vgrid, iz) = GIM3(gvis, nu, nv, ugrid, vgrid, iz) + ri;
+        }
+    }
+    const int uz = uorigin + 1, vz = vorigin;
+    for (int iz = 1; iz <= nz; iz++)
+        for (int offv = 0; offv <= nv / 2 - 1; offv++) {
+            GRE3(gvis, nu, nv, uz, vz + offv, iz) = GRE3(gvis, nu, nv, uz, vz + offv, iz) + GRE3(gvis, nu, nv, uz, vz - offv, iz);
+            GIM3(gvis, nu, nv, uz, vz + offv, iz) = GIM3(gvis, nu, nv, uz, vz + offv, iz) - GIM3(gvis, nu, nv, uz, vz - offv, iz);
+            GRE3(gvis, nu, nv, uz, vz - offv, iz) = GRE3(gvis, nu, nv, uz, vz + offv, iz);
+            GIM3(gvis, nu, nv, uz, vz - offv, iz) = -GIM3(gvis, nu, nv, uz, vz + offv, iz);
+        }
+    if (*sumwt == 0.0f) return 1;
+    const float rnorm = (float)(nu - 1) * (float)nv / *sumwt;
+    for (int iz = 1; iz <= nz; iz++)
+        for (int vg = 1; vg <= nv; vg++) {
+            for (int ug = 1; ug <= nu - 1; ug++) {
+                const float a = GRE3(gvis, nu, nv, ug + 1, vg, iz), b = GIM3(gvis, nu, nv, ug + 1, vg, iz);
+                GRE3(gvis, nu, nv, ug, vg, iz) = rnorm * a - 0.0f * b;
+                GIM3(gvis, nu, nv, ug, vg, iz) = rnorm * b + 0.0f * a;
+            }
+            GRE3(gvis, nu, nv, nu, vg, iz) = GIM3(gvis, nu, nv, nu, vg, iz) = 0.0f;
+        }
+    return 0;
+}

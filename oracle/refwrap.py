@@ -157,6 +157,18 @@ def grddeh23(wt, u, v, w, uscale, vscale, uorigin, vorigin, grid, cf, supp, osam
     return vis, wt
 
 
+def grdqh23d(vis, wt, u, v, w, uscale, vscale, uorigin, vorigin, psf, nu, nv, nz, cellz, refz, shift,
+             uoffset=0.0, voffset=0.0, variant=""):
+    """GRDQH23D (grd/grdqh23d.f:7-9) -> (GVIS(NU,NV,NZ) as (NZ, NV, NU) complex64, SUMWT)."""
+    vis, wt, u, v, w = _c64(vis), _f32(wt), _f32(u), _f32(v), _f32(w)
+    g = np.zeros((nz, nv, nu), np.complex64)
+    sw = C.c_float(0.0)
+    lib(variant).grdqh23d_(_p(vis), _p(wt), _p(u), _p(v), _p(w), _i(len(u)), _r(uscale), _r(uoffset), _r(vscale),
+                           _r(voffset), _i(uorigin), _i(vorigin), _p(g), _i(1 if psf else 0), _i(nu), _i(nv),
+                           _i(nz), _r(cellz), _r(refz), _p(_shift(shift)), C.byref(sw))
+    return g, sw.value
+
+
 def grduwt2d(wt, u, v, w, uscale, vscale, uorigin, vorigin, nu, nv, shift, uoffset=0.0, voffset=0.0, variant=""):
     """GRDUWT2D (grd/grduwt2d.f:7-8) -> (NEWWT, GWT as (NV, NU))."""
     wt, u, v, w = _f32(wt), _f32(u), _f32(v), _f32(w)

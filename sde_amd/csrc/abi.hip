@@ -527,6 +527,29 @@ int slowz_grid(Ctx &c, const float2 *dvis, const float *dwt, const float *du, co
     return 0;
 }
 
+// GRDQH23D on device arrays: nearest-cell gridding onto the NZ planes of dg,
+// synchronised; out = (SUMWT, SUMWT).
+int slowz_box_grid(Ctx &c, const float2 *dvis, const float *dwt, const float *du, const float *dv, const float *dw,
+                   int64_t n, const CoordParams &cp, const PlaneParams &pp, int psf, float2 *dg, double out[2]) {
+    hipStream_t s = c.stream;
+    const size_t nacc = 2 * (size_t)cp.nu * cp.nv * pp.nz;
+    double *acc = ctx_buf<double>(c, "boxacc", nacc), *dsw = ctx_buf<double>(c, "sumwt", 4);
+    if (!acc || !dsw) {
+        set_plan_error("device allocation failed");
+        return -1;
+    }
+    SDE_CHECK(hipMemsetAsync(acc, 0, sizeof(double) * nacc, s));
+    SDE_CHECK(hipMemsetAsync(dsw, 0, sizeof(double) * 4, s));
+    if (n > 0) launch_box_grid_slowz(s, dvis, dwt, du, dv, dw, n, cp, psf, pp, acc, dsw);
+    launch_box_fold_slowz(s, acc, cp, pp.nz, dsw, dg);
+    SDE_CHECK(hipGetLastError());
+    double sw[2];
+    SDE_CHECK(hipMemcpyAsync(sw, dsw, sizeof sw, hipMemcpyDeviceToHost, s));
+    SDE_CHECK(hipStreamSynchronize(s));
+    out[0] = out[1] = sw[0];
+    return 0;
+}
+
 // GRDDEH23 on device arrays (in place on vis/wt), synchronised; returns the
 // number of points without kernel support, or -1.
 long long slowz_degrid(Ctx &c, const float2 *dg, float2 *dvis, float *dwt, const float *du, const float *dv,
@@ -832,6 +855,47 @@ void grdh23d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, fl
     SDE_F(hipStreamSynchronize(s));
     *sumwt = (float)out[0];
     if (out[1] == 0.0) report_error(kErrBdArg, R, "No weight!");
+}
+
+// GRDQH23D: the inputs copied to the device, then slowz_box_grid.
+void grdqh23d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, float *uscale, float *uoffset,
+               float *vscale, float *voffset, int *uorigin, int *vorigin, float *gvis, int *psf, int *nu, int *nv,
+               int *nz, float *cellz, float *refz, double *shift, float *sumwt) {
+    if (sysi_.error) return;
+    const char *R = "GRDQH23D";
+    if (!check_kernel_args(R, *nvis, *nu, *nv, 0, 0, 1, 1) || !check_planes(R, *nz)) return;
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    Ctx &c = g_ctx;
+    if (ctx_init(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
+    hipStream_t s = c.stream;
+    const int64_t n = *nvis;
+    const size_t nn = (size_t)(n > 0 ? n : 1), ng = (size_t)*nu * *nv * *nz;
+    const CoordParams cp = make_cp(shift, *uscale, *uoffset, *vscale, *voffset, *uorigin, *vorigin, *nu, *nv, 0, 0,
+                                   1, 1);
+    float2 *dvis = ctx_buf<float2>(c, "vis", nn);
+    float *dwt = ctx_buf<float>(c, "wt", nn), *du = ctx_buf<float>(c, "u", nn);
+    float *dv = ctx_buf<float>(c, "v", nn), *dw = ctx_buf<float>(c, "w", nn);
+    float2 *dg = ctx_buf<float2>(c, "grid", ng);
+    if (!dvis || !dwt || !du || !dv || !dw || !dg) {
+        report_error(kErrNoMem, R, "device allocation failed");
+        return;
+    }
+    if (n > 0) {
+        SDE_F(hipMemcpyAsync(dvis, vis, sizeof(float2) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(dwt, wt, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(du, u, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(dv, v, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(dw, w, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    }
+    double out[2];
+    if (slowz_box_grid(c, dvis, dwt, du, dv, dw, n, cp, make_pp(*nz, *cellz, *refz), *psf, dg, out)) {
+        report_error(kErrFatal, R, sde_last_error());
+        return;
+    }
+    SDE_F(hipMemcpyAsync(gvis, dg, sizeof(float2) * ng, hipMemcpyDeviceToHost, s));
+    SDE_F(hipStreamSynchronize(s));
+    *sumwt = (float)out[0];
+    if (out[0] == 0.0) report_error(kErrBdArg, R, "No weight!");
 }
 
 // GRDDEH23: the inputs copied to the device, then slowz_degrid.
@@ -1769,7 +1833,11 @@ int sde_vistoimg_slowz_device(const void *vis, const float *wt, const float *u, 
                                    osamp);
     double out[2];
     float2 *g = (float2 *)gvis;
-    if (slowz_grid(c, (const float2 *)vis, wt, u, v, w, nvis, cp, make_pp(nz, cellz, refz), psf, dcf, dcf, g, out))
+    // VISGRID's dispatch (vis/visgrid.f:212-226): BOX is GRDQH23D, the others GRDH23D
+    const bool box = fstarts(code, "BOX");
+    if (box ? slowz_box_grid(c, (const float2 *)vis, wt, u, v, w, nvis, cp, make_pp(nz, cellz, refz), psf, g, out)
+            : slowz_grid(c, (const float2 *)vis, wt, u, v, w, nvis, cp, make_pp(nz, cellz, refz), psf, dcf, dcf, g,
+                         out))
         return -1;
     if (out[1] == 0.0) {
         set_plan_error("VISTOIMG: No weight!");

@@ -944,6 +944,88 @@ __global__ __launch_bounds__(kThreads) void k_prep_slowz(
     }
 }
 
+// ---- BOX (nearest cell) w planes, GRDQH23D ----
+// One thread per visibility (grd/grdqh23d.f:73-135): FVIS = (WT*VIS) *
+// CMPLX(COS, -SIN) of the shift, reflection (conjugate for u > 0, negate
+// WLOCAL otherwise), the PSF's FVIS = WT, the cell NINT(UCELL)+1, NINT(VCELL)
+// and its bounds test; then per plane FVIS * CMPLX(COS, -SIN) of
+// TWOPI*WLOCAL*DTOR*CELLZ*(REFZ-IZ), all in REAL arithmetic, added to the
+// plane's cell in FP64.  SUMWT counts every point with WT > 0.
+__global__ __launch_bounds__(kThreads) void k_box_grid_slowz(
+    const float2 *__restrict__ vis, const float *__restrict__ wt, const float *__restrict__ u,
+    const float *__restrict__ v, const float *__restrict__ w, int64_t nvis, CoordParams cp, int psf, PlaneParams pp,
+    double *__restrict__ acc, double *__restrict__ sums) {
+    double sw = 0.0, zero = 0.0;
+    const size_t plane = 2 * (size_t)cp.nu * cp.nv;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < nvis; i += (int64_t)gridDim.x * kThreads) {
+        const float wti = wt[i];
+        if (wti <= 0.0f) continue;
+        const float wf = w[i];
+        const float3 l = local_uvw(cp, u[i], v[i], wf);
+        const float phase = pp.twopi * (l.z - wf);
+        const float cr = cosf(phase), ci = -sinf(phase);
+        const float2 vv = vis[i];
+        const float ar = wti * vv.x, ai = wti * vv.y;
+        float fr = ar * cr - ai * ci, fi = ar * ci + ai * cr;
+        sw += (double)wti;
+        float ucell = cp.uscale * l.x + cp.uoffset, vcell = cp.vscale * l.y + cp.voffset;
+        float wz = l.z;
+        if (ucell > 0.0f) {
+            fi = -fi;
+            ucell = ucell + (float)cp.uorigin;
+            vcell = vcell + (float)cp.vorigin;
+        } else {
+            ucell = -ucell + (float)cp.uorigin;
+            vcell = -vcell + (float)cp.vorigin;
+            wz = -wz;
+        }
+        if (psf) {
+            fr = wti;
+            fi = 0.0f;
+        }
+        const int ug = nint_f(ucell) + 1, vg = nint_f(vcell);
+        if (ug < 1 || ug > cp.nu || vg < 1 || vg > cp.nv) continue;
+        double *a = acc + 2 * ((size_t)(vg - 1) * cp.nu + ug - 1);
+        for (int iz = 1; iz <= pp.nz; iz++) {
+            const float ph = pp.twopi * wz * pp.dtor * pp.cellz * (pp.refz - (float)iz);
+            const float rc = cosf(ph), rs = -sinf(ph);
+            atomicAdd(a, (double)(fr * rc - fi * rs));
+            atomicAdd(a + 1, (double)(fr * rs + fi * rc));
+            a += plane;
+        }
+    }
+    block_sum2(sw, zero);
+    if (threadIdx.x == 0) atomicAdd(&sums[0], sw);
+}
+
+// Fold of the u-origin column (loop 80 of grd/grdqh23d.f:139-148), shift
+// left by one column and normalisation by (NU-1)*NV/SUMWT, one plane per
+// blockIdx.z.
+__global__ void k_box_fold_slowz(const double *__restrict__ acc, CoordParams cp, const double *__restrict__ sums,
+                                 float2 *__restrict__ g) {
+    const int iu = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iv = blockIdx.y;
+    if (iu >= cp.nu) return;
+    const size_t np = (size_t)cp.nu * cp.nv;
+    const double *pa = acc + 2 * np * blockIdx.z;
+    float2 out = make_float2(0.0f, 0.0f);
+    const double sw = sums[0];
+    if (iu + 1 <= cp.nu - 1 && sw != 0.0) {
+        const int col = iu + 2, row = iv + 1;  // 1-based pre-shift
+        const int uz = cp.uorigin + 1, vz = cp.vorigin;
+        double re = pa[2 * ((size_t)(row - 1) * cp.nu + col - 1)];
+        double im = pa[2 * ((size_t)(row - 1) * cp.nu + col - 1) + 1];
+        const int mr = 2 * vz - row;
+        if (col == uz && row >= vz - (cp.nv / 2 - 1) && row <= vz + cp.nv / 2 - 1 && mr >= 1 && mr <= cp.nv) {
+            re += pa[2 * ((size_t)(mr - 1) * cp.nu + col - 1)];
+            im -= pa[2 * ((size_t)(mr - 1) * cp.nu + col - 1) + 1];
+        }
+        const double scale = (double)((float)(cp.nu - 1) * (float)cp.nv) / sw;
+        out = make_float2((float)(scale * re), (float)(scale * im));
+    }
+    g[np * blockIdx.z + (size_t)iv * cp.nu + iu] = out;
+}
+
 int blocks_for(int64_t n) {
     int64_t b = (n + kThreads - 1) / kThreads;
     if (b < 1) b = 1;
@@ -1131,6 +1213,19 @@ void launch_prep_slowz(hipStream_t s, const float2 *vis, const float *wt, const 
         hipLaunchKernelGGL(k_prep_slowz, dim3(nb), dim3(kThreads), 0, s, vis, wt, u, v, w, n, cp, tp,
                            (uint32_t)sort_nkeys(tp), psf, pp.twopi, b.keys, b.vals, b.hot, blk);
     hipLaunchKernelGGL(k_grid_totals, dim3(1), dim3(1024), 0, s, blk, n > 0 ? nb : 0, 0.0, sumwt);
+}
+
+void launch_box_grid_slowz(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
+                           const float *w, int64_t nvis, const CoordParams &cp, int psf, const PlaneParams &pp,
+                           double *acc, double *sums) {
+    hipLaunchKernelGGL(k_box_grid_slowz, dim3(blocks_for(nvis)), dim3(kThreads), 0, s, vis, wt, u, v, w, nvis, cp,
+                       psf, pp, acc, sums);
+}
+
+void launch_box_fold_slowz(hipStream_t s, const double *acc, const CoordParams &cp, int nz, const double *sums,
+                           float2 *gvis) {
+    dim3 grid((cp.nu + 255) / 256, cp.nv, nz);
+    hipLaunchKernelGGL(k_box_fold_slowz, grid, dim3(256), 0, s, acc, cp, sums, gvis);
 }
 
 void launch_box_grid(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,

@@ -502,6 +502,13 @@ void launch_degrid_slowz(hipStream_t s, const float2 *grid, float2 *grid_inner, 
                          const float *w, const float *wt_in, const float2 *vis_in, int64_t nvis,
                          const CoordParams &cp, const float *cfu, const float *cfv, const PlaneParams &pp,
                          const uint64_t *order, float2 *vis_out, float *wt_out, unsigned long long *nbad);
+// GRDQH23D (BOX w planes): acc (2*NU*NV*NZ doubles) and sums[0] (SUMWT)
+// zeroed by the caller; the fold writes the NZ planes of gvis.
+void launch_box_grid_slowz(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
+                           const float *w, int64_t nvis, const CoordParams &cp, int psf, const PlaneParams &pp,
+                           double *acc, double *sums);
+void launch_box_fold_slowz(hipStream_t s, const double *acc, const CoordParams &cp, int nz, const double *sums,
+                           float2 *gvis);
 // Coarse (8 x 8 cell) keys of the points for that order, with vals = index;
 // sort over slowz_key_bits(cp) bits.
 void launch_slowz_keys(hipStream_t s, const float *u, const float *v, const float *w, const float *wt, int64_t n,

@@ -161,6 +161,20 @@ def grdh23d(vis, wt, u, v, w, uscale, uoffset, vscale, voffset, uorigin, vorigin
     return g, sumwt.value
 
 
+def grdqh23d(vis, wt, u, v, w, uscale, uoffset, vscale, voffset, uorigin, vorigin, psf, nu, nv, nz, cellz, refz,
+             shift=IDENTITY):
+    """GRDQH23D (grd/grdqh23d.f:7-158): nearest-cell (BOX) gridding onto NZ
+    w planes.  Returns (gvis (NZ,NV,NU) complex64, sumwt)."""
+    vis, wt, u, v, w = _c64(vis), _f32(wt), _f32(u), _f32(v), _f32(w)
+    g = np.zeros((int(nz), int(nv), int(nu)), np.complex64)
+    sumwt = C.c_float(0.0)
+    lib().grdqh23d_(_pf(vis), _pf(wt), _pf(u), _pf(v), _pf(w), _ri(len(u)), _rf(uscale), _rf(uoffset),
+                    _rf(vscale), _rf(voffset), _ri(uorigin), _ri(vorigin), _pf(g), _ri(1 if psf else 0), _ri(nu),
+                    _ri(nv), _ri(nz), _rf(cellz), _rf(refz), _pd(_shift(shift)), C.byref(sumwt))
+    check_error("GRDQH23D")
+    return g, sumwt.value
+
+
 def grddeh23(vis, wt, u, v, w, uscale, uoffset, vscale, voffset, uorigin, vorigin, gvis, intfnu, suppu,
              osampu, cellz, refz, intfnv=None, suppv=None, osampv=None, shift=IDENTITY):
     """GRDDEH23 (grd/grddeh23.f:7-219): degrid from the NZ planes gvis

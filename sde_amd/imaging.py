@@ -421,7 +421,7 @@ def vistoimg_slowz(vis, wt, u, v, w, nx, cellsize, nz, cellz, refz, ny=None, psf
     Returns (cube (NZ, NY, NX) float32, gvis (NZ, NV, NU) complex64, SUMWT)."""
     from . import DeviceArray
     ny = nx if ny is None else ny
-    geo = Plan(nx, ny, cellsize, cftype, weighting=0, fov=1.0)
+    geo = Plan(nx, ny, cellsize, "SF", weighting=0, fov=1.0)  # the geometry only
     nu, nv, fnu, uorigin, vorigin = geo.nu, geo.nv, geo.fnu, geo.uorigin, geo.vorigin
     uscale, vscale = geo.uscale, geo.vscale
     geo.close()

@@ -36,7 +36,7 @@ def lib():
 
 def test_header_declares_the_reference_interface():
     names = declared_functions()
-    for f in ("grdh2d_", "grdqh2d_", "grddeh2d_", "grdh23d_", "grddeh23_", "grduwt2d_", "grdruw2d_", "grdcf_", "grdccf_", "pixcor2d_",
+    for f in ("grdh2d_", "grdqh2d_", "grddeh2d_", "grdh23d_", "grdqh23d_", "grddeh23_", "grduwt2d_", "grdruw2d_", "grdcf_", "grdccf_", "pixcor2d_",
               "pixcorx2_", "fftr2d_", "fftx1d_", "fftpwr2_", "sde_plan_create", "sde_plan_uvmap",
               "sde_plan_vismodel"):
         assert f in names, f

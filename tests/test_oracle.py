@@ -579,3 +579,27 @@ def test_slowz_fp32_restatement_tracks_fp64():
     b, wb, nb = po.grddeh23_64(wt, u, v, w, 1.0, 1.0, 1, vo, G, cf64, supp, osamp, 3.0, 2.0, s)
     assert na == nb == 0 and np.array_equal(wa, wb)
     assert np.abs(a - b).max() < 2e-5 * np.abs(b).max()
+
+
+def test_slowz_box_gridding_focuses_a_plane_and_matches_grdqh2d():
+    """GRDQH23D: visibilities carrying the phase exp(+2 pi i w n_k) of plane
+    k grid onto that plane as GRDQH2D's PSF of the same points (its planes
+    rotate by TWOPI*W*DTOR*CELLZ*(REFZ-IZ), grd/grdqh23d.f:129-134), the
+    same normalisation (NU-1)*NV/SUMWT; the FP32 restatement agrees."""
+    u, v, w, wt = _slowz_points(3000, 38)
+    nu, nv, vo, nz, cellz, refz = 65, 128, 64, 4, 3.0, 2.0
+    ident = np.eye(3).reshape(-1)
+    p2, sw2, rc2 = po.grdqh2d(np.ones(len(u)), wt, u, v, w, 1.0, 1.0, 1, vo, True, nu, nv, ident)
+    assert rc2 == 0
+    for k in (1, 3):
+        vis = np.exp(2j * np.pi * w.astype(np.float64) * _n_of_plane(k, cellz, refz)).astype(np.complex64)
+        g64, sw64, rc = po.grdqh23d(vis, wt, u, v, w, 1.0, 1.0, 1, vo, False, nu, nv, nz, cellz, refz, ident,
+                                    fp64=True)
+        assert rc == 0 and sw64 == pytest.approx(sw2, rel=1e-6)
+        assert np.abs(g64[k - 1] - p2).max() < 2e-6 * np.abs(p2).max()
+        g32, sw32, rc32 = po.grdqh23d(vis, wt, u, v, w, 1.0, 1.0, 1, vo, False, nu, nv, nz, cellz, refz, ident)
+        assert rc32 == 0 and np.abs(g32 - g64).max() < 2e-5 * np.abs(g64).max()
+    # the PSF's reference plane (REFZ = 2) is GRDQH2D's PSF
+    g64, _, _ = po.grdqh23d(np.ones(len(u)), wt, u, v, w, 1.0, 1.0, 1, vo, True, nu, nv, nz, cellz, refz, ident,
+                            fp64=True)
+    assert np.abs(g64[1] - p2).max() < 1e-6 * np.abs(p2).max()

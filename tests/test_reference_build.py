@@ -157,6 +157,17 @@ def test_grdh23d(psf, shifted, nz, cellz, refz):
     assert _bits_equal(g1, g2)
 
 
+@pytest.mark.parametrize("psf", [0, 1])
+@pytest.mark.parametrize("shifted", [False, True])
+def test_grdqh23d(psf, shifted):
+    vis, wt, u, v, w = _vis(3000, 14, umax=70.0, vmax=70.0, near_axis=300)  # some cells off the grid
+    sh = _shifted() if shifted else IDENTITY
+    g1, s1 = rw.grdqh23d(vis, wt, u, v, w, 1.0, 1.0, 1, VORIGIN, psf, NU, NV, 5, 2.0, 3.0, sh)
+    g2, s2, rc = po.grdqh23d(vis, wt, u, v, w, 1.0, 1.0, 1, VORIGIN, psf, NU, NV, 5, 2.0, 3.0, sh)
+    assert rc == 0 and s1 == s2
+    assert _bits_equal(g1, g2)
+
+
 @pytest.mark.parametrize("shifted", [False, True])
 @pytest.mark.parametrize("nz,cellz,refz", [(4, 3.0, 2.0), (7, 1.5, 4.5)])
 def test_grddeh23(shifted, nz, cellz, refz):
