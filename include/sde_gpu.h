@@ -321,6 +321,20 @@ int sde_vistoimg_slowz_device(const void *vis, const float *wt, const float *u, 
                               const char *cftype, int nz, float cellz, float refz,
                               const double *shift, void *gvis, float *images, double *sumwt);
 
+/* Prediction from a w-plane image cube on device-resident arrays
+ * (vis/visdegri.f:141-160 with the planes of IMGFFT and IMGGRIDC): per
+ * plane PIXCOR2D 'CORRECT' and FFTR2D (image -> grid), then GRDDEH23 with
+ * the interpolation function CFTYPE and VISDEGRI's scales.  images:
+ * NX*NY*NZ floats; gvis: NU*NV*NZ complex scratch, left holding the planes'
+ * grids.  VIS and WT in place as GRDDEH23's.  Returns the number of points
+ * without kernel support, or -1 with sde_last_error() set.  No reference
+ * counterpart at this level (see sde_vistoimg_slowz_device). */
+int64_t sde_imgtovis_slowz_device(const float *images, int nx, int ny, int fnu, int nv,
+                                  const char *cftype, int nz, float cellz, float refz, void *vis,
+                                  float *wt, const float *u, const float *v, const float *w,
+                                  int64_t nvis, float uscale, float vscale, int uorigin,
+                                  int vorigin, const double *shift, void *gvis);
+
 /* GRDDEH23 on device-resident arrays, in place on VIS and WT; returns the
  * number of points without kernel support (the reference's NBAD warning),
  * or -1 with sde_last_error() set.  No reference counterpart (see

@@ -49,6 +49,7 @@ def _declare(L):
     L.sde_plan_estimate_bytes.restype = _C.c_int64
     L.sde_grdh23d_device.restype = _C.c_int
     L.sde_vistoimg_slowz_device.restype = _C.c_int
+    L.sde_imgtovis_slowz_device.restype = _C.c_int64
     L.sde_device_alloc.restype = _C.c_void_p
     L.sde_device_alloc.argtypes = [_C.c_size_t]
     L.sde_device_free.argtypes = [_C.c_void_p]

@@ -1858,6 +1858,56 @@ int sde_vistoimg_slowz_device(const void *vis, const float *wt, const float *u, 
     return 0;
 }
 
+int64_t sde_imgtovis_slowz_device(const float *images, int nx, int ny, int fnu, int nv, const char *cftype, int nz,
+                                  float cellz, float refz, void *vis, float *wt, const float *u, const float *v,
+                                  const float *w, int64_t nvis, float uscale, float vscale, int uorigin, int vorigin,
+                                  const double *shift, void *gvis) {
+    const char *R = "VISDEGRI";
+    const std::string code = cftype ? cftype : "SF";
+    std::vector<float> cf(1 + 2 * 4 * 128);
+    int supp = 0, osamp = 0;
+    if (make_cf(code, cf.data(), &supp, &osamp)) {
+        set_plan_error("VISDEGRI: unknown interpolation function %s", code.c_str());
+        return -1;
+    }
+    const int nu = fnu / 2 + 1;
+    if (fnu < 2 || fnu != pow2_at_least(fnu) || nv != pow2_at_least(nv) || nx > fnu || ny > nv || nx < 1 ||
+        ny < 1 || !fft_supported(fnu) || !fft_supported(nv)) {
+        set_plan_error("VISDEGRI: unsupported sizes NX=%d NY=%d FNU=%d NV=%d", nx, ny, fnu, nv);
+        return -1;
+    }
+    if (nvis > INT32_MAX || !check_kernel_args(R, (int)nvis, nu, nv, supp, supp, osamp, osamp, uorigin, vorigin) ||
+        !check_planes(R, nz))
+        return -1;
+    const int ccflen = 1 + pow2_at_least(nx > ny ? nx : ny) / 2;
+    std::vector<float> ccf(ccflen);
+    make_ccf(code, ccf.data(), ccflen);
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    Ctx &c = g_ctx;
+    if (ctx_init(c)) return -1;
+    hipStream_t s = c.stream;
+    const int ncf = 1 + 2 * (supp + 1) * osamp;
+    float *dcf = ctx_buf<float>(c, "cfz", ncf), *dccf = ctx_buf<float>(c, "ccfz", ccflen);
+    float2 *twnv = nullptr, *twfnu = nullptr;
+    if (!dcf || !dccf || get_twiddles(c.tw, nv, s, &twnv) || get_twiddles(c.tw, fnu, s, &twfnu)) {
+        set_plan_error("VISDEGRI: device allocation failed");
+        return -1;
+    }
+    SDE_CHECK(hipMemcpyAsync(dcf, cf.data(), sizeof(float) * ncf, hipMemcpyHostToDevice, s));
+    SDE_CHECK(hipMemcpyAsync(dccf, ccf.data(), sizeof(float) * ccflen, hipMemcpyHostToDevice, s));
+    // IMGGRIDC 'CORRECT' and IMGFFT one plane at a time (img/imggridc.f:103-109, arr/arrfftr.f:143-151)
+    float2 *g = (float2 *)gvis;
+    const size_t np = (size_t)nu * nv, npix = (size_t)nx * ny;
+    for (int iz = 0; iz < nz; iz++) {
+        launch_fft_rows_r2c(s, images + iz * npix, nx, ny, fnu, nv, twfnu, dccf, ccflen, 1, g + iz * np);
+        launch_fft_cols(s, g + iz * np, g + iz * np, nu, nv, 1, 0, +1, 1.0f, twnv);
+    }
+    SDE_CHECK(hipGetLastError());
+    const CoordParams cp = make_cp(shift, uscale, 0.0f, vscale, 0.0f, uorigin, vorigin, nu, nv, supp, supp, osamp,
+                                   osamp);
+    return slowz_degrid(c, g, (float2 *)vis, wt, u, v, w, nvis, cp, make_pp(nz, cellz, refz), dcf, dcf);
+}
+
 int64_t sde_grddeh23_device(void *vis, float *wt, const float *u, const float *v, const float *w, int64_t nvis,
                             float uscale, float uoffset, float vscale, float voffset, int uorigin, int vorigin,
                             const void *gvis, int nu, int nv, const float *intfnu, int suppu, int osampu,

@@ -442,3 +442,36 @@ def vistoimg_slowz(vis, wt, u, v, w, nx, cellsize, nz, cellz, refz, ny=None, psf
         lib().sde_error_clear()
         raise SdeError("sde_vistoimg_slowz_device: " + msg)
     return cube.numpy(), g.numpy(), sw.value
+
+
+def vismodel_slowz(cube, wt, u, v, w, cellsize, cellz, refz, shift=None, cftype="SF", vis=None):
+    """Prediction from a w-plane image cube (NZ, NY, NX) (vis/visdegri.f:141-160
+    on the planes of IMGGRIDC 'CORRECT' and IMGFFT): each plane corrected
+    and transformed to its grid, then GRDDEH23.  Everything after the copy
+    of the inputs runs on the device.  Returns (vis, wt-with-flags)."""
+    from . import DeviceArray
+    cube = np.ascontiguousarray(cube, np.float32)
+    nz, ny, nx = cube.shape
+    geo = Plan(nx, ny, cellsize, "SF", weighting=0, fov=1.0)  # the geometry only
+    nu, nv, fnu, uorigin, vorigin = geo.nu, geo.nv, geo.fnu, geo.uorigin, geo.vorigin
+    uscale, vscale = geo.uscale, geo.vscale
+    geo.close()
+    n = len(u)
+    vis0 = np.zeros(n, np.complex64) if vis is None else np.ascontiguousarray(vis, np.complex64)
+    d = [DeviceArray.from_numpy(np.ascontiguousarray(a, t)) for a, t in
+         ((vis0, np.complex64), (wt, np.float32), (u, np.float32), (v, np.float32), (w, np.float32))]
+    dimg = DeviceArray.from_numpy(cube)
+    g = DeviceArray((nz, nv, nu), np.complex64)
+    sh = np.ascontiguousarray(np.eye(3).reshape(-1) if shift is None else shift, np.float64).reshape(-1)
+    nbad = lib().sde_imgtovis_slowz_device(C.c_void_p(dimg.ptr), C.c_int(nx), C.c_int(ny), C.c_int(fnu),
+                                           C.c_int(nv), C.c_char_p(cftype.encode()), C.c_int(nz), C.c_float(cellz),
+                                           C.c_float(refz), C.c_void_p(d[0].ptr), C.c_void_p(d[1].ptr),
+                                           C.c_void_p(d[2].ptr), C.c_void_p(d[3].ptr), C.c_void_p(d[4].ptr),
+                                           C.c_int64(n), C.c_float(uscale), C.c_float(vscale), C.c_int(uorigin),
+                                           C.c_int(vorigin), sh.ctypes.data_as(C.POINTER(C.c_double)),
+                                           C.c_void_p(g.ptr))
+    if nbad < 0:
+        msg = lib().sde_last_error().decode(errors="replace")
+        lib().sde_error_clear()
+        raise SdeError("sde_imgtovis_slowz_device: " + msg)
+    return d[0].numpy(), d[1].numpy()

@@ -248,3 +248,40 @@ def test_vistoimg_slowz_box_cube():
                           fp64=True)
         i32 = po.pixcor2d(po.fftr2d_c2r(g32[iz], nx, nx), po.grdccf("BOX", ccflen), True)
         assert_image(cube[iz], i32, i64, "box plane %d" % (iz + 1))
+
+
+@pytest.mark.parametrize("shifted", [False, True])
+def test_vismodel_slowz_matches_the_oracle(shifted):
+    """Prediction from an image cube: per plane PIXCOR2D 'CORRECT' and FFTR2D
+    (image -> grid), then GRDDEH23; the oracle's composition in FP64 is the
+    target, under the reference-relative rule of predictions (the correction
+    amplifies edge pixels before an FP32 transform)."""
+    from sde_amd import imaging, synth
+    from tests.tolerance import assert_vis_rel_ref
+    d = synth.small_field()
+    nx, cell = d["nx"], d["cellsize"]
+    nz, cellz, refz = 3, 0.05, 2.0
+    rng = np.random.default_rng(48)
+    cube = np.zeros((nz, nx, nx), np.float32)
+    for iz in range(nz):
+        ix, iy = rng.integers(nx // 4, 3 * nx // 4, (2, 10))
+        cube[iz, iy, ix] = rng.uniform(0.1, 1.0, 10)
+    sh = po.crdshift(10.0, 30.0, 10.0005, 30.0004) if shifted else IDENT
+    wt = d["wt"].copy()
+    wt[::31] = -1.0
+    print("vismodel slowz shifted=%s n=%d nz=%d" % (shifted, len(wt), nz), flush=True)
+    vg, wg = imaging.vismodel_slowz(cube, wt, d["u"], d["v"], d["w"], cell, cellz, refz, shift=sh)
+    nu, nv, vo, us, vs = _geometry(nx, cell)
+    fnu = po.fftpwr2(nx)
+    ccflen = 1 + fnu // 2
+    cf, supp, osamp = po.grdcf("SF")
+    cf64, _, _ = po.grdcf("SF", fp64=True)
+    g64 = np.stack([po.fftr2d_r2c(po.pixcor2d(cube[iz], po.grdccf("SF", ccflen, fp64=True), True, fp64=True), fnu,
+                                  nv, fp64=True) for iz in range(nz)])
+    g32 = np.stack([po.fftr2d_r2c(po.pixcor2d(cube[iz], po.grdccf("SF", ccflen), True), fnu, nv)
+                    for iz in range(nz)])
+    v64, w64, _ = po.grddeh23_64(wt, d["u"], d["v"], d["w"], us, vs, 1, vo, g64, cf64, supp, osamp, cellz, refz, sh)
+    v32, w32, _ = po.grddeh23(wt, d["u"], d["v"], d["w"], us, vs, 1, vo, g32, cf, supp, osamp, cellz, refz, sh)
+    np.testing.assert_array_equal(wg, w32)
+    good = wg > 0
+    assert_vis_rel_ref(vg[good], v32[good], v64[good], "slowz predict")
