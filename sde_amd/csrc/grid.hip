@@ -588,6 +588,10 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
     const int tcol = (wi.tile % tp.ntu) * T - cp.suppu, trow = (wi.tile / tp.ntu) * T;
     const bool shifted = cp.doshift != 0;
 
+    // the weighting sums, loaded once (uniform)
+    double4 wsum = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (WGT) wsum = make_double4(wp.wsums[0], wp.wsums[1], wp.wsums[2], wp.wsums[3]);
+
     const int len = wi.end - wi.begin;
     const int r0 = wi.begin + (int)(((int64_t)len * wave) / W);
     const int r1 = wi.begin + (int)(((int64_t)len * (wave + 1)) / W);
@@ -599,28 +603,36 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
         StageHot st0 = ZP == 2 ? zp.sorted[min(r0 + lane, r1 - 1)] : hot[sval_index(v0)];
         uint64_t v1 = svals[min(r0 + 64 + lane, r1 - 1)];
         for (int b = r0; b < r1; b += 64) {
-            const StageHot st1 = ZP == 2 ? zp.sorted[min(b + 64 + lane, r1 - 1)] : hot[sval_index(v1)];
-            const uint64_t v2 = svals[min(b + 128 + lane, r1 - 1)];
-
             // this lane's record of the batch: the staged gridding record
             // with its final weight
             const int p = b + lane;
             const bool valid = p < r1;
             const uint32_t i0 = sval_index(v0);
             float wti = sval_wt(v0);
-            // the cold record where this run or this point needs it (rare in the common case)
+            // this batch's own loads (the cold record where this run or this
+            // point needs it -- rare in the common case -- and the density
+            // weight) go out before the next batch's prefetch: loads complete
+            // in order, so waiting for them then leaves the prefetch in flight
+            // through this batch's build and tap loop
             StageCold sc;
             sc.wcell = kNoCell;
             sc.q2 = 0.0f;
             sc.rot = make_float2(1.0f, 0.0f);
             const bool need_cold = (cold_all && (WGT || (DEG && shifted))) || (WGT && (st0.gpk & kStageOwnCell));
             if (need_cold) sc = cold[i0];
+            uint32_t wc = kNoCell;
+            double gcell = 0.0;
             if (WGT) {
-                uint32_t wc = sc.wcell;
+                wc = sc.wcell;
                 if (!cold_all && !(st0.gpk & kStageOwnCell))
                     wc = (uint32_t)((trow + (int)((st0.gpk >> 8) & 0x1f)) * cp.nu + tcol + (int)(st0.gpk & 0x1f));
-                wti = robust_apply(wti, wc, (st0.gpk & kStageHalf) != 0, sc.q2, WGT == 2, wp.gwt, wp.wsums);
+                if (wc != kNoCell) gcell = wp.gwt[wc];
             }
+            __builtin_amdgcn_sched_barrier(0);
+            const StageHot st1 = ZP == 2 ? zp.sorted[min(b + 64 + lane, r1 - 1)] : hot[sval_index(v1)];
+            const uint64_t v2 = svals[min(b + 128 + lane, r1 - 1)];
+            __builtin_amdgcn_sched_barrier(0);
+            if (WGT) wti = robust_apply_g(wti, wc, gcell, (st0.gpk & kStageHalf) != 0, sc.q2, WGT == 2, wsum);
             GridRec mine;
             mine.pk = st0.gpk & ~kStageFlags;
             mine.wt = 0.0f;

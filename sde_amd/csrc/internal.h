@@ -411,15 +411,23 @@ __device__ __forceinline__ RobustCell robust_cell(const CoordParams &wcp, float 
 // weighting sums wsums = {sum WT, sum GWT^2, F2, D2} (grd/grdruw2d.f:
 // 225-240; GRDUWT2D is F2 = 1, D2 = 0): -1 off the density grid or on an
 // empty cell.
-__device__ __forceinline__ float robust_apply(float wti, uint32_t cell, bool half, float q2, bool az,
-                                              const double *gwt, const double *wsums) {
+// The same with the cell's density gcell = gwt[cell] and the weighting sums
+// ws[0..3] already loaded.
+__device__ __forceinline__ float robust_apply_g(float wti, uint32_t cell, double gcell, bool half, float q2, bool az,
+                                                const double4 &ws) {
     if (cell == kNoCell) return -1.0f;
-    const float f2 = az ? (float)((double)q2 / (wsums[1] / wsums[0])) : (float)wsums[2];
-    const float d2 = (float)wsums[3];
-    const float g = (float)gwt[cell];
+    const float f2 = az ? (float)((double)q2 / (ws.y / ws.x)) : (float)ws.z;
+    const float d2 = (float)ws.w;
+    const float g = (float)gcell;
     if (!(g > 0.0f)) return -1.0f;
     if (half) return 0.5f * wti / (g * f2 + d2);
     return wti / (g * f2 + d2);
+}
+
+__device__ __forceinline__ float robust_apply(float wti, uint32_t cell, bool half, float q2, bool az,
+                                              const double *gwt, const double *wsums) {
+    const double4 ws = make_double4(wsums[0], wsums[1], wsums[2], wsums[3]);
+    return robust_apply_g(wti, cell, cell == kNoCell ? 0.0 : gwt[cell], half, q2, az, ws);
 }
 
 // Both steps; *off reports a point off the density grid.
