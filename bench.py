@@ -48,26 +48,31 @@ REGION_KERNELS = {
 
 
 def alg_bytes(region, nvis, nu, nv, nx, ny, nitems, tile):
-    """Algorithmic HBM bytes of one launch of a timed region: every input
-    byte read once and every output byte written once (per-visibility
-    figures times the visibilities, plus the per-cell and per-tile arrays)."""
+    """Algorithmic HBM bytes of one launch of a timed region.  The regions
+    that are one of the path's operations take that operation's figure --
+    every input byte read once, every output byte written once:
+      gridding (dirty + PSF fused): u, v, w, vis, wt read, 24 B/vis, and
+        both grids written once;
+      degridding: u, v, w, wt read and vis written, 24 B/vis, and the grid
+        read once;
+      weighting: the density grid written and read (the per-visibility part
+        is the prep pass's);
+      FFT + correction: the grid read once and the image written once.
+    The passes between them (prep, sort, reduce) take the bytes of the
+    formats they hand on."""
     g = nu * nv * 8                      # one complex FP32 grid
     n2 = nx * ny * 4                     # one FP32 image
     part = nitems * 3 * (tile + 6) ** 2 * 4   # FP32 partial tiles (dirty re/im + PSF)
-    win = nitems * (tile + 8) ** 2 * 8        # degridding windows
     table = {
-        # vis, wt, u, v, w in; key, 32-byte staging record and the prediction WT out
-        "prep": 24 * nvis + 40 * nvis,
+        # vis, wt, u, v, w in; key, value (index, weight), 16-byte hot staging record and the
+        # prediction WT out
+        "prep": 24 * nvis + 32 * nvis,
         # keys and values in; sorted keys and values out
         "cell_sort": 12 * nvis + 12 * nvis,
-        # sorted keys and values (index, weight) in; the density grid (FP64) written once
-        "wt_density": 12 * nvis + nu * nv * 8,
-        # sorted values (index, weight), the 32-byte staging record and the density grid in; the
-        # 16-byte degridding half and the partial tiles out
-        "grid_tiles": 8 * nvis + 32 * nvis + nu * nv * 8 + 16 * nvis + part,
+        "wt_density": 2 * nu * nv * 8,
+        "grid_tiles": 24 * nvis + 2 * g,
         "grid_reduce": part + 2 * g,
-        # the 16-byte degridding half in, the predicted visibility out, each window read once
-        "degrid": 16 * nvis + 8 * nvis + win,
+        "degrid": 24 * nvis + g,
         "fft_cols": 2 * 2 * g,
         "fft_rows_psf": g + n2,
         "fft_rows_dirty": g + n2,
