@@ -490,6 +490,16 @@ __host__ __device__ inline int cf_row_stride(int osamp) {
     return n0 + (((4 - n0) % 32) + 32) % 32;
 }
 
+__host__ __device__ inline int cf_table_floats(int suppu, int osampu, int suppv, int osampv, int same_uv) {
+    return (2 * suppu + 1) * cf_row_stride(osampu) + (same_uv ? 0 : (2 * suppv + 1) * cf_row_stride(osampv));
+}
+
+// Record table of the tap loop: per record the kernel values of its taps
+// along u (up to kMaxTaps) and along v, rows of kRecTaps floats (odd, so the
+// 64 rows written by the 64 lanes spread over the banks).
+constexpr int kMaxTaps = 7;
+constexpr int kRecTaps = 2 * kMaxTaps + 1;
+
 // One work item of the gridding: a contiguous run of cell-sorted slots of
 // one tile, accumulated on the tile padded by kHalo, held in LDS in FP64.
 //
@@ -508,7 +518,12 @@ __host__ __device__ inline int cf_row_stride(int osamp) {
 // with readlane, consecutive points of one cell (the slots are cell-sorted)
 // accumulate in FP64 registers -- the product of two REAL factors is exact
 // in DOUBLE, so one fused multiply-add per channel is the whole update --
-// and are added into the LDS tile when the cell changes.
+// and are added into the LDS tile when the cell changes.  The tap loop is
+// the kernel's cost, so it is unrolled over the batch's 64 records and
+// carries nothing per record but the update: the kernel values come from a
+// per-wave LDS table filled by the lanes once per batch (read at fixed
+// offsets, no address arithmetic), the cell changes from a ballot (a scalar
+// bit test).
 //
 // Occupancy: 5 waves per SIMD (at most 96 VGPRs, no spills), so the
 // gathers of one wave are covered by the tap loops of the others.
@@ -535,6 +550,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
     double *tsu = smem_d + NCH * NC;             // sum of the taps of each phase
     double *tsv = same_uv ? tsu : tsu + cp.osampu + 1;
     float *cfl = (float *)(tsu + (cp.osampu + 1) + (same_uv ? 0 : cp.osampv + 1));  // one row per tap
+    float *rtab = cfl + cf_table_floats(cp.suppu, cp.osampu, cp.suppv, cp.osampv, same_uv);  // [W][64][kRecTaps]
     const int item = blockIdx.x;
     if (item >= *nitems) {
         if (threadIdx.x == 0) {
@@ -580,8 +596,10 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
     const int ou = lane % ntapu, ov = lane / ntapu;
     const bool active = ov < ntapv;
     const int lane_off = (ov - cp.suppv + kHalo) * P + (ou - cp.suppu + kHalo);
-    const float *rowu = cflu + (active ? ou * rsu : 0);
-    const float *rowv = cflv + (active ? ov * rsv : 0);
+    // this wave's record table: per record, kMaxTaps u values then kMaxTaps v values
+    float *rt = rtab + (size_t)wave * 64 * kRecTaps;
+    const float *ru = rt + (active ? ou : 0);
+    const float *rv = rt + kMaxTaps + (active ? ov : 0);
 
     // a point without a cold record has its density cell at its gridding cell:
     // row (tv*T + lv), column (tu*T + lu - SUPPU) of the NU x NV density grid
@@ -658,14 +676,38 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
             if (ZP == 1 && valid) zp.sorted[p] = st0;
             if (DEG && valid) rec_d[p] = degrid_rec(st0, sc.rot, i0);
 
-            // grid the batch; the table entries of the next record are
-            // read while this one accumulates (two records per turn, so the
-            // entries in flight never need a register copy).  Lanes past the
-            // taps compute on row 0 and never flush.
+            // grid the batch.  Each lane first looks up its record's
+            // kernel values (SUPPU*2+1 along u, SUPPV*2+1 along v) into its
+            // row of the wave's LDS record table, and the wave marks where
+            // cell runs start (a cell different from the record before it;
+            // the batch's first against the run open before the batch).
+            // The tap loop is unrolled over the 64 records: each record's
+            // two kernel values are LDS reads at a fixed offset, its value
+            // a readlane broadcast, and the cell test a scalar bit test.
+            // Lanes past the taps compute on row 0 and never flush.
             const int nb = __builtin_amdgcn_readfirstlane(min(64, r1 - b));
-            auto step = [&](uint32_t pk, float ta, float tb, int j) {
-                const int cell = (int)(pk & 0xffff);
-                if (cell != cur) {
+            {
+                const int pu = (int)((mine.pk >> 16) & 0xff), pv = (int)(mine.pk >> 24);
+                float *my = rt + lane * kRecTaps;
+                for (int k = 0; k < ntapu; k++) my[k] = cflu[k * rsu + pu];
+                for (int k = 0; k < ntapv; k++) my[kMaxTaps + k] = cflv[k * rsv + pv];
+            }
+            const int mycell = (int)(mine.pk & 0xffff);
+            const int before = __shfl_up(mycell, 1);
+            const int first = __builtin_amdgcn_readfirstlane(mycell);
+            const bool head = lane == 0 ? first != cur : mycell != before;
+            const uint64_t heads = __ballot(head);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // the next record's two kernel values are read while this one accumulates
+            float ta = ru[0], tb = rv[0];
+#pragma unroll
+            for (int j = 0; j < 64; j++) {
+                if (j < nb) {
+                const int jn = j + 1 < 64 ? j + 1 : 63;
+                const float tan = ru[jn * kRecTaps], tbn = rv[jn * kRecTaps];
+                if ((heads >> j) & 1ull) {
                     if (cur >= 0 && active) {
                         const int a = (cur >> 8) * P + (cur & 0xff) + lane_off;
                         if (CHM & 1) {
@@ -674,10 +716,12 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
                         }
                         if (CHM & 2) atomicAdd(&S[PSF * NC + a], s2);
                     }
-                    cur = cell;
+                    cur = __builtin_amdgcn_readlane(mycell, j);
                     s0 = s1 = s2 = 0.0;
                 }
                 const double cw = (double)(ta * tb);
+                ta = tan;
+                tb = tbn;
                 if (CHM & 1) {
                     const float re = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.re), j));
                     const float im = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.im), j));
@@ -688,21 +732,12 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
                     const float wt = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.wt), j));
                     s2 = fma(cw, (double)wt, s2);
                 }
-            };
-            // an odd batch takes one more lane: a copy of the last record
-            // with zero weight (same cell, adds nothing)
-            const int nb2 = (nb + 1) & ~1;
-            uint32_t pk0 = (uint32_t)__builtin_amdgcn_readlane((int)mine.pk, 0);
-            float ta0 = rowu[(pk0 >> 16) & 0xff], tb0 = rowv[pk0 >> 24];
-            for (int j = 0; j < nb2; j += 2) {
-                const uint32_t pk1 = (uint32_t)__builtin_amdgcn_readlane((int)mine.pk, j + 1);
-                const float ta1 = rowu[(pk1 >> 16) & 0xff], tb1 = rowv[pk1 >> 24];
-                step(pk0, ta0, tb0, j);
-                pk0 = (uint32_t)__builtin_amdgcn_readlane((int)mine.pk, min(j + 2, 63));
-                ta0 = rowu[(pk0 >> 16) & 0xff];
-                tb0 = rowv[pk0 >> 24];
-                step(pk1, ta1, tb1, j + 1);
+                }
             }
+            // the table is rewritten by the next batch
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             v0 = v1;
             v1 = v2;
             st0 = st1;
@@ -1158,10 +1193,10 @@ static void launch_tiles_t(hipStream_t s, const SortBufs &b, const TileParams &t
     constexpr int P = T + 2 * kHalo;
     constexpr int NCH = ((CHM & 1) ? 2 : 0) + ((CHM & 2) ? 1 : 0);
     const int same_uv = cfu == cfv && cp.suppu == cp.suppv && cp.osampu == cp.osampv;
-    const size_t tables = (size_t)(2 * cp.suppu + 1) * cf_row_stride(cp.osampu) +
-                          (same_uv ? 0 : (size_t)(2 * cp.suppv + 1) * cf_row_stride(cp.osampv));
+    const size_t tables = (size_t)cf_table_floats(cp.suppu, cp.osampu, cp.suppv, cp.osampv, same_uv);
     const size_t sums = (size_t)cp.osampu + 1 + (same_uv ? 0 : (size_t)cp.osampv + 1);
-    const size_t lds = sizeof(double) * ((size_t)NCH * P * P + sums) + sizeof(float) * tables;
+    const size_t lds = sizeof(double) * ((size_t)NCH * P * P + sums) +
+                       sizeof(float) * (tables + (size_t)W * 64 * kRecTaps);
     hipLaunchKernelGGL((k_grid_tiles<T, CHM, W, DEG, WGT, ZP>), dim3(b.max_items), dim3(64 * W), lds, s, b.svals,
                        b.hot, b.cold, b.cold_all, b.items, b.nitems, cfu, cfv, same_uv, cp, tp, wp, rec_d, item_sums,
                        partial, zp);
