@@ -112,7 +112,8 @@ __global__ void k_fft_cols(const float2 *srcs, float2 *dsts, int nu, int nv, siz
     extern __shared__ float2 xs[];
     const float2 *grid = srcs + stride * blockIdx.y;
     float2 *out = dsts + stride * blockIdx.y;
-    const int u0 = blockIdx.x * cb;
+    // adjacent column groups (which share each row's cache lines) on one XCD
+    const int u0 = xcd_swizzle(blockIdx.x, gridDim.x) * cb;
     const int icen = (nv + 1) / 2 - 1;  // 0-based centre row
     const int nthr = blockDim.x;
     for (int idx = threadIdx.x; idx < cb * nv; idx += nthr) {

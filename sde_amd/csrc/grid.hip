@@ -812,8 +812,12 @@ __global__ __launch_bounds__(256) void k_grid_reduce(
     const float *__restrict__ partial, const int *__restrict__ item_start, TileParams tp,
     int chmask, CoordParams cp, const double *__restrict__ sums, float2 *__restrict__ dirty,
     float2 *__restrict__ psf) {
-    const int iu = blockIdx.x * blockDim.x + threadIdx.x;  // 0-based output column
-    const int iv = blockIdx.y;                              // 0-based row
+    // the rows of one band of tiles (which read the same partial tiles) on one
+    // XCD, the bands dealt round the XCDs (the work per band varies)
+    const int lin = xcd_swizzle_chunks(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y,
+                                       gridDim.x * tp.T);
+    const int iu = (lin % gridDim.x) * blockDim.x + threadIdx.x;  // 0-based output column
+    const int iv = lin / gridDim.x;                                // 0-based row
     if (iu >= cp.nu) return;
     const int nch = ((chmask & 1) ? 2 : 0) + ((chmask & 2) ? 1 : 0);
     double dr = 0.0, di = 0.0, pr = 0.0;

@@ -133,6 +133,28 @@ size_t sort_temp_bytes(int64_t n, int nbits);  // sort.hip
 hipError_t launch_radix_sort(hipStream_t s, void *temp, size_t temp_bytes, const uint32_t *keys_in,
                              uint32_t *keys_out, const uint64_t *vals_in, uint64_t *vals_out, int64_t n,
                              int nbits);  // sort.hip
+// The dispatcher deals workgroups to the 8 XCDs round-robin (workgroup b to
+// XCD b mod 8), and each XCD has its own L2.  xcd_swizzle maps the
+// workgroup's linear id b (of n) to the logical one that keeps consecutive
+// logical workgroups on one XCD: XCD x takes a contiguous block of logical
+// ids (the first n mod 8 XCDs one more), so neighbours that read the same
+// cache lines (adjacent columns, adjacent rows) share that XCD's L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_swizzle(int b, int n) {
+    const int x = b % kXcds, k = b / kXcds;
+    const int q = n / kXcds, r = n % kXcds;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+// The same in chunks of c logical ids: chunk j goes to XCD j mod 8, so the
+// XCDs share out work that varies over the ids (a tail of fewer than 8*c
+// ids keeps its order).
+__device__ __forceinline__ int xcd_swizzle_chunks(int b, int n, int c) {
+    const int full = (n / (kXcds * c)) * (kXcds * c);
+    if (b >= full) return b;
+    const int x = b % kXcds, k = b / kXcds;
+    return (k / c) * (kXcds * c) + x * c + (k % c);
+}
+
 __device__ __forceinline__ uint32_t sval_index(uint64_t v) { return (uint32_t)v; }
 __device__ __forceinline__ float sval_wt(uint64_t v) { return __uint_as_float((uint32_t)(v >> 32)); }
 
