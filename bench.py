@@ -38,7 +38,7 @@ REGION_KERNELS = {
     "prep": ["k_prep<2, true>"],
     "cell_sort": ["rocprim::*"],
     # the density runs also write the tile ranges; the work items follow in the same region
-    "wt_density": ["k_density_runs", "k_items_scan", "k_grid_items"],
+    "wt_density": ["k_density_runs", "k_items_scan"],
     "wt_finish": ["k_wt_sumsq", "k_wt_finish"],
     "grid_tiles": ["k_grid_tiles<16, 3, 4, true, 1, 0>", "k_grid_totals"],
     "grid_reduce": ["k_grid_reduce"],
@@ -74,8 +74,8 @@ def alg_bytes(region, nvis, nu, nv, nx, ny, nitems, tile):
         "grid_reduce": part + 2 * g,
         "degrid": 24 * nvis + g,
         "fft_cols": 2 * 2 * g,
-        "fft_rows_psf": g + n2,
-        "fft_rows_dirty": g + n2,
+        "fft_rows": 2 * (g + n2),
+        "peak_scale": 2 * 2 * n2,
         "fft_rows_model": n2 + g,
         "fft_cols_model": 2 * g,
     }

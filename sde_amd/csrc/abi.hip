@@ -390,9 +390,10 @@ int run_sort(hipStream_t s, const float2 *vis, const float *wt, const float *u, 
 int run_grid(hipStream_t s, const CoordParams &cp, const TileParams &tp, int chmask, bool deg,
              const WeightParams &wp, const float *cfu, const float *cfv, const SortBufs &sb, DegridRec *rec_d,
              float *partial, double *item_sums, double *sums, float2 *dirty, float2 *psf, const MarkB &mb,
-             const MarkE &me, const ZPlanes *zp = nullptr) {
+             const MarkE &me, const ZPlanes *zp = nullptr, int *peak_reset = nullptr) {
     mb("grid_tiles");
-    if (launch_grid_tiles(s, sb, tp, chmask, deg, wp, cfu, cfv, cp, rec_d, item_sums, sums, partial, zp)) {
+    if (launch_grid_tiles(s, sb, tp, chmask, deg, wp, cfu, cfv, cp, rec_d, item_sums, sums, partial, zp,
+                          peak_reset)) {
         me();
         set_plan_error("unsupported gridding run (T=%d, channels %d, degridding %d)", tp.T, chmask, (int)deg);
         return -1;
@@ -1463,8 +1464,9 @@ int plan_image(sde_plan *p, const WeightParams &wp) {
     auto me = [p]() { plan_mark_end(p); };
     // the gridding run also writes the degridding halves of a prediction on these
     // visibilities (model VIS/WT are complete after sde_plan_vismodel)
+    // the gridding's totals also reset the peak key of the row passes below
     if (run_grid(s, p->cp, p->tp, 3, true, wp, p->d_cf, p->d_cf, p->sb, p->d_recd, p->d_partial, p->d_isums,
-                 p->d_sums, gd, gp, mb, me))
+                 p->d_sums, gd, gp, mb, me, nullptr, p->d_peak))
         return -1;
     p->filled = true;
     p->imaged = true;
@@ -1474,18 +1476,15 @@ int plan_image(sde_plan *p, const WeightParams &wp) {
     launch_fft_cols(s, p->d_grids, p->d_gwork, p->nu, p->nv, 2, ng, -1, 1.0f, twnv);
     plan_mark_end(p);
     const float2 *cd = p->d_gwork, *cpsf = p->d_gwork + ng;
-    SDE_CHECK(hipMemsetAsync(p->d_peak, 0x80, sizeof(int), s));
     const float scale = 1.0f / ((float)p->fnu * (float)p->nv);
-    plan_mark_begin(p, "fft_rows_psf");
-    launch_fft_rows_c2r(s, cpsf, p->nu, p->nv, p->fnu, p->cfg.nx, p->cfg.ny, twfnu, p->d_ccf, p->ccflen, 1, scale,
-                        p->d_psf, p->d_peak, nullptr);
+    // both row passes in one launch (the PSF's tracks its peak), then both
+    // images divided by that peak (task/uvmap.f:434-440,480-482)
+    plan_mark_begin(p, "fft_rows");
+    launch_fft_rows_c2r_pair(s, cpsf, p->d_psf, p->d_peak, cd, p->d_dirty, p->nu, p->nv, p->fnu, p->cfg.nx,
+                             p->cfg.ny, twfnu, p->d_ccf, p->ccflen, 1, scale);
     plan_mark_end(p);
-    plan_mark_begin(p, "fft_rows_dirty");
-    launch_fft_rows_c2r(s, cd, p->nu, p->nv, p->fnu, p->cfg.nx, p->cfg.ny, twfnu, p->d_ccf, p->ccflen, 1, scale,
-                        p->d_dirty, nullptr, p->d_peak);
-    plan_mark_end(p);
-    plan_mark_begin(p, "psf_scale");
-    launch_scale_by_peak(s, p->d_psf, (size_t)p->cfg.nx * p->cfg.ny, p->d_peak);
+    plan_mark_begin(p, "peak_scale");
+    launch_scale_by_peak(s, p->d_psf, (size_t)p->cfg.nx * p->cfg.ny, p->d_peak, p->d_dirty);
     plan_mark_end(p);
     SDE_CHECK(hipGetLastError());
     return 0;

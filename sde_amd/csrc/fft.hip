@@ -156,13 +156,19 @@ __device__ __forceinline__ int icf(float s, int i1, float cen) {
 }
 
 // Grid rows -> image rows (DIR < 0).
-__global__ void k_fft_rows_c2r(const float2 *__restrict__ grid, int nu, int nv, int fnu, int nx,
+// blockIdx.y = 1 transforms grid_b into image_b (no peak) in the same launch.
+__global__ void k_fft_rows_c2r(const float2 *__restrict__ grid_a, int nu, int nv, int fnu, int nx,
                                int ny, int ibegy, const float2 *__restrict__ tw,
                                const float *__restrict__ ccf, int correct, CorrParams cpar,
-                               float scale, float *__restrict__ image, int *__restrict__ peak_key,
-                               const int *__restrict__ div_key) {
+                               float scale, float *__restrict__ image_a, int *__restrict__ peak_a,
+                               const int *__restrict__ div_key, const float2 *__restrict__ grid_b,
+                               float *__restrict__ image_b) {
     extern __shared__ float2 zs[];
     __shared__ float red[1024];
+    const bool second = blockIdx.y == 1;
+    const float2 *grid = second ? grid_b : grid_a;
+    float *image = second ? image_b : image_a;
+    int *peak_key = second ? nullptr : peak_a;
     const int iy = blockIdx.x;       // 0-based image row
     const int rv = iy + ibegy - 1;   // 0-based grid row
     const int h = fnu / 2;
@@ -257,7 +263,10 @@ __global__ void k_fft_rows_r2c(const float *__restrict__ image, int nx, int ny, 
     }
 }
 
-__global__ void k_scale_by_peak(float *__restrict__ image, size_t n, const int *__restrict__ key) {
+// blockIdx.y = 1 scales image_b.
+__global__ void k_scale_by_peak(float *__restrict__ image_a, float *__restrict__ image_b, size_t n,
+                                const int *__restrict__ key) {
+    float *image = blockIdx.y == 1 ? image_b : image_a;
     const float inv = 1.0f / key_float(*key);
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         image[i] = inv * image[i] + 0.0f;
@@ -341,7 +350,18 @@ void launch_fft_rows_c2r(hipStream_t s, const float2 *grid, int nu, int nv, int 
     CorrParams cp = make_corr(nx, ny, ccflen, (float)((nx + 1) / 2), (float)((ny + 1) / 2));
     hipLaunchKernelGGL(k_fft_rows_c2r, dim3(ny), dim3(row_threads(h)), sizeof(float2) * h, s, grid,
                        nu, nv, fnu, nx, ny, ibegy, tw_fnu, ccf, correct, cp, scale, image, peak_key,
-                       div_key);
+                       div_key, (const float2 *)nullptr, (float *)nullptr);
+}
+
+void launch_fft_rows_c2r_pair(hipStream_t s, const float2 *grid_a, float *image_a, int *peak_a,
+                              const float2 *grid_b, float *image_b, int nu, int nv, int fnu, int nx, int ny,
+                              const float2 *tw_fnu, const float *ccf, int ccflen, int correct, float scale) {
+    const int h = fnu / 2;
+    const int ibegy = (nv + 1) / 2 - (ny + 1) / 2 + 1;
+    CorrParams cp = make_corr(nx, ny, ccflen, (float)((nx + 1) / 2), (float)((ny + 1) / 2));
+    hipLaunchKernelGGL(k_fft_rows_c2r, dim3(ny, 2), dim3(row_threads(h)), sizeof(float2) * h, s, grid_a, nu, nv,
+                       fnu, nx, ny, ibegy, tw_fnu, ccf, correct, cp, scale, image_a, peak_a, (const int *)nullptr,
+                       grid_b, image_b);
 }
 
 void launch_fft_rows_r2c(hipStream_t s, const float *image, int nx, int ny, int fnu, int nv,
@@ -354,10 +374,11 @@ void launch_fft_rows_r2c(hipStream_t s, const float *image, int nx, int ny, int 
                        nx, ny, fnu, nu, ibegy, tw_fnu, ccf, correct, cp, grid);
 }
 
-void launch_scale_by_peak(hipStream_t s, float *image, size_t n, const int *peak_key) {
+void launch_scale_by_peak(hipStream_t s, float *image, size_t n, const int *peak_key, float *image_b) {
     size_t nb = (n + 255) / 256;
     if (nb > 65536) nb = 65536;
-    hipLaunchKernelGGL(k_scale_by_peak, dim3((unsigned)nb), dim3(256), 0, s, image, n, peak_key);
+    hipLaunchKernelGGL(k_scale_by_peak, dim3((unsigned)nb, image_b ? 2 : 1), dim3(256), 0, s, image, image_b, n,
+                       peak_key);
 }
 
 void launch_copy(hipStream_t s, const float4 *src, float4 *dst, size_t n) {

@@ -6,11 +6,12 @@
 //                      FP64 cell of GRDH2D and its drop test give the sort
 //                      key (tile-major cell); everything of the gridding and
 //                      degridding records that does not depend on the
-//                      density weights goes into a 32-byte staging record.
+//                      density weights goes into a 16-byte hot staging
+//                      record (and, where needed, a 16-byte cold one).
 //   2. radix sort      (key, index|weight) pairs, stable (sort.hip); the
 //                      density runs or k_tile_bounds give the tile ranges,
-//                      k_items_scan/k_grid_items the work items (a tile's
-//                      slots cut into chunks).
+//                      k_items_scan the work items (a tile's slots cut into
+//                      chunks).
 //   3. k_grid_tiles    one workgroup per work item over a padded tile
 //                      (T+6)^2 held in LDS in FP64.  Each wave gathers the
 //                      staging records of a contiguous run of cell-sorted
@@ -100,10 +101,11 @@ __device__ __forceinline__ int cell_key(const TileParams &tp, int cu, int cv) {
 
 // Prep pass, one thread per visibility in the caller's order: the cell key
 // of GRDH2D's DOUBLE coordinate step and drop test (grd/grdh2d.f:115-145)
-// and the 32-byte staging record -- the gridding record's cell, kernel
-// phases and rotated visibility (grd/grdh2d.f:124-160), the density cell of
-// GRDRUW2D / GRDUWT2D (grd/grdruw2d.f:197-229) when DM > 0, the degridding
-// record of GRDDEH2D (grd/grddeh2d.f:92-118) when DEG -- and optionally
+// and the staging records -- the gridding record's cell, kernel phases and
+// rotated visibility (grd/grdh2d.f:124-160) and the degridding record of
+// GRDDEH2D (grd/grddeh2d.f:92-118) when DEG in the hot record; the density
+// cell of GRDRUW2D / GRDUWT2D (grd/grdruw2d.f:197-229) when DM > 0 and the
+// shift rotation in the cold record, written where needed -- and optionally
 //   DM = 1: the count of points off the density grid;
 //   DM = 2: also the density weight of every point whose REAL density cell
 //           (grd/grdruw2d.f:139-160) is not its DOUBLE gridding cell or
@@ -273,9 +275,11 @@ __global__ void k_tile_bounds(const uint32_t *__restrict__ skeys, int64_t n, uin
         for (int x = t + 1; x <= ntiles; x++) tile_start[x] = (int)n;
 }
 
-// Work items per tile (chunks of at most `chunk` records) and their scan.
+// Work items per tile (chunks of at most `chunk` records): their scan, and
+// each thread writes the items of its own tiles.
 __global__ __launch_bounds__(1024) void k_items_scan(const int *__restrict__ tile_start, int ntiles, int chunk,
-                                                     int *__restrict__ item_start, int *__restrict__ nitems) {
+                                                     int *__restrict__ item_start, int *__restrict__ nitems,
+                                                     WorkItem *__restrict__ items) {
     __shared__ int s[1024];
     const int t = threadIdx.x;
     const int pt = (ntiles + 1023) / 1024;
@@ -293,7 +297,15 @@ __global__ __launch_bounds__(1024) void k_items_scan(const int *__restrict__ til
     int run = t > 0 ? s[t - 1] : 0;
     for (int k = lo; k < hi; k++) {
         item_start[k] = run;
-        run += (tile_start[k + 1] - tile_start[k] + chunk - 1) / chunk;
+        const int b = tile_start[k], e = tile_start[k + 1];
+        for (int q = b; q < e; q += chunk, run++) {
+            WorkItem wi;
+            wi.tile = k;
+            wi.begin = q;
+            wi.end = min(q + chunk, e);
+            wi.pad = 0;
+            items[run] = wi;
+        }
     }
     if (t == 1023) {
         item_start[ntiles] = s[1023];
@@ -410,23 +422,6 @@ __global__ __launch_bounds__(kThreads) void k_density_runs(
     if (threadIdx.x == 0) blk[blockIdx.x] = dsum;
 }
 
-// Work items of every tile: chunks of at most `chunk` records.
-__global__ void k_grid_items(const int *__restrict__ tile_start, const int *__restrict__ item_start, int ntiles,
-                             int chunk, WorkItem *__restrict__ items) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntiles) return;
-    const int b = tile_start[t], e = tile_start[t + 1];
-    int k = item_start[t];
-    for (int s = b; s < e; s += chunk, k++) {
-        WorkItem wi;
-        wi.tile = t;
-        wi.begin = s;
-        wi.end = min(s + chunk, e);
-        wi.pad = 0;
-        items[k] = wi;
-    }
-}
-
 // The degridding half of a sorted point from its staged records.
 __device__ __forceinline__ DegridRec degrid_rec(const StageHot &sr, float2 rot, uint32_t i) {
     DegridRec d;
@@ -454,8 +449,11 @@ __global__ __launch_bounds__(kThreads) void k_degrid_gather(const uint64_t *__re
     }
 }
 
+// Also resets the image peak key (*peak_reset, when given) for the row
+// passes that follow.
 __global__ __launch_bounds__(1024) void k_grid_totals(const double *__restrict__ blk_sums, int nblk,
-                                                      double scale_num, double *__restrict__ sums) {
+                                                      double scale_num, double *__restrict__ sums,
+                                                      int *__restrict__ peak_reset) {
     __shared__ double ds[1024], dl[1024];
     const int t = threadIdx.x;
     double a = 0.0, b = 0.0;
@@ -477,6 +475,7 @@ __global__ __launch_bounds__(1024) void k_grid_totals(const double *__restrict__
         sums[0] = ds[0];
         sums[1] = dl[0];
         sums[2] = dl[0] != 0.0 ? scale_num / dl[0] : 0.0;
+        if (peak_reset) *peak_reset = (int)0x80808080u;
     }
 }
 
@@ -1156,9 +1155,7 @@ int launch_sort(hipStream_t s, SortBufs &b, int64_t n, const TileParams &tp, boo
 
 void launch_items(hipStream_t s, SortBufs &b, const TileParams &tp) {
     hipLaunchKernelGGL(k_items_scan, dim3(1), dim3(1024), 0, s, b.tile_start, tp.ntiles, tp.chunk, b.item_start,
-                       b.nitems);
-    hipLaunchKernelGGL(k_grid_items, dim3((tp.ntiles + 255) / 256), dim3(256), 0, s, b.tile_start, b.item_start,
-                       tp.ntiles, tp.chunk, b.items);
+                       b.nitems, b.items);
 }
 
 void launch_density_runs(hipStream_t s, const SortBufs &b, int64_t n, const TileParams &tp,
@@ -1235,7 +1232,8 @@ static int launch_tiles_w(hipStream_t s, const SortBufs &b, const TileParams &tp
 
 int launch_grid_tiles(hipStream_t s, const SortBufs &b, const TileParams &tp, int chmask, bool deg,
                       const WeightParams &wp, const float *cfu, const float *cfv, const CoordParams &cp,
-                      DegridRec *rec_d, double *item_sums, double *sums, float *partial, const ZPlanes *zpp) {
+                      DegridRec *rec_d, double *item_sums, double *sums, float *partial, const ZPlanes *zpp,
+                      int *peak_reset) {
     if (b.max_items <= 0) return 0;
     const int W = tp.waves == 8 ? 8 : 4;
     const ZPlanes zp = zpp ? *zpp : ZPlanes();
@@ -1246,7 +1244,7 @@ int launch_grid_tiles(hipStream_t s, const SortBufs &b, const TileParams &tp, in
     else if (tp.T == 32 && W == 8) rc = launch_tiles_w<32, 8>(s, b, tp, chmask, deg, wp, cfu, cfv, cp, rec_d, item_sums, partial, zp);
     if (rc) return rc;
     const double num = (double)((float)(cp.nu - 1) * (float)cp.nv);
-    hipLaunchKernelGGL(k_grid_totals, dim3(1), dim3(1024), 0, s, item_sums, b.max_items, num, sums);
+    hipLaunchKernelGGL(k_grid_totals, dim3(1), dim3(1024), 0, s, item_sums, b.max_items, num, sums, peak_reset);
     return 0;
 }
 
@@ -1263,7 +1261,7 @@ void launch_prep_slowz(hipStream_t s, const float2 *vis, const float *wt, const 
     if (n > 0)
         hipLaunchKernelGGL(k_prep_slowz, dim3(nb), dim3(kThreads), 0, s, vis, wt, u, v, w, n, cp, tp,
                            (uint32_t)sort_nkeys(tp), psf, pp.twopi, b.keys, b.vals, b.hot, blk);
-    hipLaunchKernelGGL(k_grid_totals, dim3(1), dim3(1024), 0, s, blk, n > 0 ? nb : 0, 0.0, sumwt);
+    hipLaunchKernelGGL(k_grid_totals, dim3(1), dim3(1024), 0, s, blk, n > 0 ? nb : 0, 0.0, sumwt, (int *)nullptr);
 }
 
 void launch_box_grid_slowz(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,

@@ -258,7 +258,7 @@ struct ZPlanes {
 int launch_grid_tiles(hipStream_t s, const SortBufs &b, const TileParams &tp, int chmask, bool deg,
                       const WeightParams &wp, const float *cfu, const float *cfv, const CoordParams &cp,
                       DegridRec *rec_d, double *item_sums, double *sums, float *partial,
-                      const ZPlanes *zp = nullptr);
+                      const ZPlanes *zp = nullptr, int *peak_reset = nullptr);
 void launch_grid_reduce(hipStream_t s, const float *partial, const int *item_start, const TileParams &tp,
                         int chmask, const CoordParams &cp, const double *sums, float2 *dirty, float2 *psf);
 size_t grid_partial_floats(const TileParams &tp, int chmask, int max_items);
@@ -287,7 +287,13 @@ void launch_fft_rows_c2r(hipStream_t s, const float2 *grid, int nu, int nv, int 
 void launch_fft_rows_r2c(hipStream_t s, const float *image, int nx, int ny, int fnu, int nv,
                          const float2 *tw_fnu, const float *ccf, int ccflen, int correct,
                          float2 *grid);
-void launch_scale_by_peak(hipStream_t s, float *image, size_t n, const int *peak_key);
+// image (and image_b when given) times 1/peak
+void launch_scale_by_peak(hipStream_t s, float *image, size_t n, const int *peak_key, float *image_b = nullptr);
+// Two row passes in one launch: grid_a -> image_a with its peak tracked in
+// peak_a, grid_b -> image_b.
+void launch_fft_rows_c2r_pair(hipStream_t s, const float2 *grid_a, float *image_a, int *peak_a,
+                              const float2 *grid_b, float *image_b, int nu, int nv, int fnu, int nx, int ny,
+                              const float2 *tw_fnu, const float *ccf, int ccflen, int correct, float scale);
 void launch_pixcor(hipStream_t s, const float *in, float *out, int nx, int ny, int cplx,
                    const float *cf, int cflen, int correct, float cenx, float ceny);
 
