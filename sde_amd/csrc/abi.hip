@@ -1410,7 +1410,9 @@ int plan_uvmap(sde_plan *p, bool reuse) {
             wp = make_weight_params(p->wcp, p->wmode == 3, p->cfg.robust, 4.0f * atanf(1.0f), p->d_gwt, p->d_wsums);
         return plan_image(p, wp);
     }
-    SDE_CHECK(hipMemsetAsync(p->d_cnt, 0, sizeof(unsigned long long) * 4, s));
+    // the counters of the weighting (points off the density grid); natural
+    // weights touch none of them, and the plan zeroes them at creation
+    if (weighting) SDE_CHECK(hipMemsetAsync(p->d_cnt, 0, sizeof(unsigned long long) * 4, s));
     // every imaging job sorts its visibilities afresh (the sort is part of the job)
     // the density grid comes from the sorted runs when its cell is the
     // gridding cell (WTFOV = 1: the same scales); else from atomics
@@ -1573,6 +1575,7 @@ int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **out) {
         return -1;
     }
     if (hipMemsetAsync(p->sb.tile_start, 0, sizeof(int) * (p->tp.ntiles + 1), p->stream) != hipSuccess ||
+        hipMemsetAsync(p->d_cnt, 0, sizeof(unsigned long long) * 4, p->stream) != hipSuccess ||
         hipMemcpyAsync(p->d_cf, cf, sizeof(float) * p->ncf, hipMemcpyHostToDevice, p->stream) != hipSuccess ||
         hipMemcpyAsync(p->d_ccf, ccf.data(), sizeof(float) * p->ccflen, hipMemcpyHostToDevice, p->stream) !=
             hipSuccess ||
