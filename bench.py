@@ -42,7 +42,8 @@ REGION_KERNELS = {
     "wt_finish": ["k_wt_sumsq", "k_wt_finish"],
     "grid_tiles": ["k_grid_tiles<16, 3, 4, true, 1, 0>", "k_grid_totals"],
     "grid_reduce": ["k_grid_reduce"],
-    "degrid": ["k_degrid_tiles<16, 3>"],
+    # the tile degridder writes its results into output bins; the scatter puts them in input order
+    "degrid": ["k_degrid_tiles<16, 3, true>", "k_degrid_scatter"],
     "degrid_axis": ["k_degrid_list"],
 }
 

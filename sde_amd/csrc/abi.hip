@@ -1206,6 +1206,7 @@ struct sde_plan {
     int *d_peak = nullptr;
     double *d_gwt = nullptr, *d_wblk = nullptr, *d_wsums = nullptr;
     unsigned long long *d_cnt = nullptr;
+    int *d_dbins = nullptr;  // fill counters of the degridder's output bins
     // profiling
     bool profile = false;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
@@ -1368,9 +1369,12 @@ int plan_vismodel(sde_plan *p, const float *img) {
                        p->d_cf, p->d_mvis, p->d_mwt, p->d_cnt + 1, p->d_list, p->d_nlist);
     plan_mark_end(p, p->side);
     SDE_CHECK(hipEventRecord(p->ev_join, p->side));
+    // the sort's unsorted keys and values (dead until the next prep pass) hold the output bins
+    const DegridBins bins = degrid_bins_for(p->nvis, reinterpret_cast<float2 *>(p->sb.vals), p->sb.keys, p->d_dbins);
     plan_mark_begin(p, "degrid");
+    SDE_CHECK(hipMemsetAsync(p->d_dbins, 0, sizeof(int) * (size_t)bins.nbins, s));
     const int rc = launch_degrid_tiles(s, p->d_recd, p->sb.items, p->sb.nitems, p->sb.max_items, p->tp, p->d_mgrid,
-                                       p->d_cf, p->d_cf, p->cp, p->d_mvis, p->d_mwt, p->d_cnt + 1);
+                                       p->d_cf, p->d_cf, p->cp, p->d_mvis, p->d_mwt, p->d_cnt + 1, &bins);
     plan_mark_end(p);
     SDE_CHECK(hipStreamWaitEvent(s, p->ev_join, 0));
     if (rc) {
@@ -1561,6 +1565,7 @@ int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **out) {
              plan_alloc(p, &p->d_mgrid, ng) || plan_alloc(p, &p->d_dirty, npix) || plan_alloc(p, &p->d_psf, npix) ||
              plan_alloc(p, &p->d_img_in, npix) || plan_alloc(p, &p->d_peak, 4) || plan_alloc(p, &p->d_gwt, ng) ||
              plan_alloc(p, &p->d_wblk, 4096) || plan_alloc(p, &p->d_wsums, 4) || plan_alloc(p, &p->d_cnt, 4) ||
+             plan_alloc(p, &p->d_dbins, (size_t)kMaxDegridBins) ||
              plan_alloc(p, &p->d_wsq, kWtSqBlocks) ||
              plan_alloc(p, &p->sb.tile_start, p->tp.ntiles + 1) || plan_alloc(p, &p->sb.item_start, p->tp.ntiles + 1) ||
              plan_alloc(p, &p->sb.nitems, 1) || plan_alloc(p, &p->d_nlist, kPrepBlocks) ||
@@ -1597,7 +1602,7 @@ int sde_plan_destroy(sde_plan *p) {
                     p->sb.keys, p->sb.skeys, p->sb.vals, p->sb.svals, p->sb.hot, p->sb.cold, p->sb.temp, p->sb.tile_start,
                     p->sb.item_start, p->sb.nitems, p->sb.items, p->d_recd, p->d_list, p->d_nlist,
                     p->d_partial, p->d_isums, p->d_sums, p->d_grids, p->d_gwork, p->d_mgrid, p->d_dirty, p->d_psf, p->d_img_in,
-                    p->d_peak, p->d_gwt, p->d_wblk, p->d_wsums, p->d_cnt, p->d_wsq};
+                    p->d_peak, p->d_gwt, p->d_wblk, p->d_wsums, p->d_cnt, p->d_wsq, p->d_dbins};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &kv : p->tw) (void)hipFree(kv.second);

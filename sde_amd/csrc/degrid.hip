@@ -307,7 +307,11 @@ __global__ __launch_bounds__(256) void k_degrid_list(
 // staged in LDS, the 2*S+1 u weights of the point are held in registers and
 // the gather runs in the reference's loop order (OFFV outer, OFFU inner) in
 // REAL arithmetic, so a given grid degrids to the reference's values.
-template <int T, int S>
+// BIN: the item's records are counted per output bin first, one run per bin
+// is reserved with a global atomic, and each result goes to its bin's run
+// (a contiguous stretch of the bin); k_degrid_scatter then puts them in
+// input order.
+template <int T, int S, bool BIN>
 __global__ __launch_bounds__(256) void k_degrid_tiles(const DegridRec *__restrict__ rec,
                                                       const WorkItem *__restrict__ items,
                                                       const int *__restrict__ nitems, TileParams tp,
@@ -315,13 +319,31 @@ __global__ __launch_bounds__(256) void k_degrid_tiles(const DegridRec *__restric
                                                       const float *__restrict__ cfu_g,
                                                       const float *__restrict__ cfv_g, CoordParams cp,
                                                       float2 *__restrict__ vis_out, float *__restrict__ wt_out,
-                                                      unsigned long long *__restrict__ nbad) {
+                                                      unsigned long long *__restrict__ nbad, DegridBins bins) {
     constexpr int PL = T + 2 * kHaloD;
+    constexpr int NB = BIN ? kMaxDegridBins : 1;
     __shared__ float2 g[PL * PL];
+    __shared__ int bcnt[NB], bbase[NB];
     extern __shared__ float cfl[];
     const int item = blockIdx.x;
     if (item >= *nitems) return;
     const WorkItem wi = items[item];
+    if (BIN) {
+        for (int k = threadIdx.x; k < bins.nbins; k += blockDim.x) bcnt[k] = 0;
+        __syncthreads();
+        for (int r = wi.begin + threadIdx.x; r < wi.end; r += blockDim.x) {
+            const uint32_t id = rec[r].idx;
+            if (id != kNoDegrid) atomicAdd(&bcnt[(id & 0x7fffffffu) >> bins.shift], 1);
+        }
+        __syncthreads();
+        // one run per bin; the counters restart as the ranks within the runs
+        for (int k = threadIdx.x; k < bins.nbins; k += blockDim.x) {
+            const int c = bcnt[k];
+            if (c) bbase[k] = (k << bins.shift) + atomicAdd(&bins.count[k], c);
+            bcnt[k] = 0;
+        }
+        // the barrier after the tile staging orders these before the main loop
+    }
     const int tu = wi.tile % tp.ntu, tv = wi.tile / tp.ntu;
     const int col0 = tu * T - S - kHaloD, row0 = tv * T - kHaloD;
     for (int k = threadIdx.x; k < PL * PL; k += blockDim.x) {
@@ -369,8 +391,32 @@ __global__ __launch_bounds__(256) void k_degrid_tiles(const DegridRec *__restric
         }
         if (d.idx & 0x80000000u) vi = -vi;
         // GRDDEH2D always rotates: by (1, 0) when there is no shift, exactly
-        vis_out[idx] = make_float2(vr * d.rr - vi * d.ri, vr * d.ri + vi * d.rr);
+        const float2 out = make_float2(vr * d.rr - vi * d.ri, vr * d.ri + vi * d.rr);
+        if (BIN) {
+            const int b = (int)(idx >> bins.shift);
+            const int slot = bbase[b] + atomicAdd(&bcnt[b], 1);
+            bins.vis[slot] = out;
+            bins.idx[slot] = idx;
+        } else {
+            vis_out[idx] = out;
+        }
     }
+}
+
+// Second pass of the binned degridder: the records of bin b move to
+// vis_out[idx].  Workgroup ids are dealt so that the `pieces` workgroups of
+// one bin are consecutive on one XCD and each XCD takes its bins in order:
+// the writes in flight stay in a narrow window of the output per XCD.
+__global__ __launch_bounds__(256) void k_degrid_scatter(const float2 *__restrict__ bvis,
+                                                        const uint32_t *__restrict__ bidx,
+                                                        const int *__restrict__ count, int shift, int pieces,
+                                                        float2 *__restrict__ vis_out) {
+    const int l = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int b = l / pieces, piece = l - b * pieces;
+    const int per = (1 << shift) / pieces;
+    const int lo = piece * per, hi = min(lo + per, count[b]);
+    const size_t base = (size_t)b << shift;
+    for (int k = lo + threadIdx.x; k < hi; k += blockDim.x) vis_out[bidx[base + k]] = bvis[base + k];
 }
 
 constexpr int kWtThreads = 256;
@@ -588,24 +634,44 @@ template <int T, int S>
 static void launch_degrid_tiles_t(hipStream_t s, const DegridRec *rec, const WorkItem *items, const int *nitems,
                                   int max_items, const TileParams &tp, const float2 *grid, const float *cfu,
                                   const float *cfv, const CoordParams &cp, float2 *vis_out, float *wt_out,
-                                  unsigned long long *nbad) {
+                                  unsigned long long *nbad, const DegridBins *bins) {
     const int ncfu = cp.osampu * (2 * S + 1) + 2;
     const int ncfv = cp.osampv * (2 * S + 1) + 2;
     const size_t lds = sizeof(float) * (ncfu + ncfv);
-    hipLaunchKernelGGL((k_degrid_tiles<T, S>), dim3(max_items), dim3(256), lds, s, rec, items, nitems, tp, grid, cfu,
-                       cfv, cp, vis_out, wt_out, nbad);
+    if (!bins) {
+        hipLaunchKernelGGL((k_degrid_tiles<T, S, false>), dim3(max_items), dim3(256), lds, s, rec, items, nitems, tp,
+                           grid, cfu, cfv, cp, vis_out, wt_out, nbad, DegridBins{});
+        return;
+    }
+    hipLaunchKernelGGL((k_degrid_tiles<T, S, true>), dim3(max_items), dim3(256), lds, s, rec, items, nitems, tp,
+                       grid, cfu, cfv, cp, vis_out, wt_out, nbad, *bins);
+    const int pieces = std::max(1, (1 << bins->shift) / kDegridScatterPiece);
+    hipLaunchKernelGGL(k_degrid_scatter, dim3(bins->nbins * pieces), dim3(std::min(256, kDegridScatterPiece)), 0, s,
+                       bins->vis, bins->idx, bins->count, bins->shift, pieces, vis_out);
+}
+
+DegridBins degrid_bins_for(int64_t n, float2 *vis, uint32_t *idx, int *count) {
+    DegridBins b;
+    b.vis = vis;
+    b.idx = idx;
+    b.count = count;
+    b.shift = kDegridBinShift;
+    while (((n + (1ll << b.shift) - 1) >> b.shift) > kMaxDegridBins) b.shift++;
+    b.nbins = (int)std::max<int64_t>(1, (n + (1ll << b.shift) - 1) >> b.shift);
+    return b;
 }
 
 int launch_degrid_tiles(hipStream_t s, const DegridRec *rec, const WorkItem *items, const int *nitems,
                         int max_items, const TileParams &tp, const float2 *grid, const float *cfu,
                         const float *cfv, const CoordParams &cp, float2 *vis_out, float *wt_out,
-                        unsigned long long *nbad) {
+                        unsigned long long *nbad, const DegridBins *bins) {
     if (max_items <= 0) return 0;
     if (cp.suppu != cp.suppv) return -1;
+    if (bins && (bins->nbins < 1 || bins->nbins > kMaxDegridBins)) return -1;
 #define SDE_DEG_CASE(TT, SS)                                                                                    \
     if (tp.T == TT && cp.suppu == SS) {                                                                         \
         launch_degrid_tiles_t<TT, SS>(s, rec, items, nitems, max_items, tp, grid, cfu, cfv, cp, vis_out, wt_out, \
-                                      nbad);                                                                    \
+                                      nbad, bins);                                                              \
         return 0;                                                                                               \
     }
     SDE_DEG_CASE(16, 1)

@@ -492,12 +492,37 @@ void launch_degrid_list(hipStream_t s, const float2 *grid, const float *u, const
                         const float *wt_in, const float2 *vis_in, int64_t nvis, const CoordParams &cp,
                         const float *cfu, const float *cfv, float2 *vis_out, float *wt_out,
                         unsigned long long *nbad, const int *list, const int *counts);
+// Output bins of the tile degridder.  Bin b takes the results for the
+// original indices [b << shift, (b + 1) << shift) at slots from b << shift
+// of vis/idx (a bin never holds more points than its index range, so n
+// slots hold every bin); count[b] is the bin's fill, zero before the
+// degridder.  A second pass moves each bin to the output in input order,
+// 256 points per workgroup and the bins in order on each XCD, so the
+// scattered 8-byte writes in flight fall in a few narrow output windows
+// (about 0.5 MB per XCD) instead of across the whole output: they hit few
+// DRAM rows at a time.  (The L2 does not merge them: each still leaves as
+// a 32-byte sector.)
+struct DegridBins {
+    float2 *vis = nullptr;
+    uint32_t *idx = nullptr;
+    int *count = nullptr;
+    int shift = 0, nbins = 0;
+};
+constexpr int kMaxDegridBins = 1024;
+constexpr int kDegridBinShift = 17;       // 2^17 points (1 MB of output) per bin
+constexpr int kDegridScatterPiece = 256;  // points per workgroup of the second pass
+// Bins for n points: 2^18-point windows, wider when n needs more than
+// kMaxDegridBins of them.
+DegridBins degrid_bins_for(int64_t n, float2 *vis, uint32_t *idx, int *count);
+
 // Tile path: one thread per record of the degridding halves (points near
 // the u = 0 axis, or not in the cell sort, go to the list for launch_degrid).
+// With bins (count zeroed by the caller) the results reach vis_out through
+// them; without, each thread writes its point's result in place.
 int launch_degrid_tiles(hipStream_t s, const DegridRec *rec, const WorkItem *items, const int *nitems,
                         int max_items, const TileParams &tp, const float2 *grid, const float *cfu,
                         const float *cfv, const CoordParams &cp, float2 *vis_out, float *wt_out,
-                        unsigned long long *nbad);
+                        unsigned long long *nbad, const DegridBins *bins = nullptr);
 void launch_wt_density(hipStream_t s, const float *wt, const float *u, const float *v,
                        const float *w, int64_t nvis, const CoordParams &cp, double *gwt,
                        double *blk, int nblk);

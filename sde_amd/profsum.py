@@ -26,7 +26,24 @@ import sys
 def short_name(name):
     name = re.sub(r"\(anonymous namespace\)::", "", name)
     name = re.sub(r"^void ", "", name)
+    if name.startswith("rocprim::"):  # rocprim::<ns>::detail::trampoline_kernel<...>
+        return "rocprim::" + re.sub(r"<.*", "", name).split("::")[-1]
     return name.split("(")[0].strip()
+
+
+def labels(names):
+    """Row label of every full kernel name: its short name, or, where several
+    kernels share one short name (rocPRIM's histogram and onesweep kernels
+    are all one trampoline template), the short name with "[k of n]" in the
+    order of the sorted full names, the same in every database of a run."""
+    groups = collections.defaultdict(list)
+    for n in sorted(set(names)):
+        groups[short_name(n)].append(n)
+    out = {}
+    for s, full in groups.items():
+        for k, n in enumerate(full):
+            out[n] = s if len(full) == 1 else "%s [%d of %d]" % (s, k + 1, len(full))
+    return out
 
 
 def _db(folder):
@@ -39,14 +56,17 @@ def _db(folder):
 def trace(folder):
     c = sqlite3.connect(_db(folder))
     rows = c.execute("select name, count(*), avg(end - start) from kernels group by name").fetchall()
-    return {short_name(r[0]): (r[1], r[2] / 1e6) for r in rows}
+    lab = labels([r[0] for r in rows])
+    return {lab[r[0]]: (r[1], r[2] / 1e6) for r in rows}
 
 
 def counters(folder):
     c = sqlite3.connect(_db(folder))
     acc = collections.defaultdict(lambda: collections.defaultdict(list))
-    for name, ctr, val in c.execute("select kernel_name, counter_name, value from counters_collection"):
-        acc[short_name(name)][ctr].append(val)
+    rows = c.execute("select kernel_name, counter_name, value from counters_collection").fetchall()
+    lab = labels([r[0] for r in rows])
+    for name, ctr, val in rows:
+        acc[lab[name]][ctr].append(val)
     return {k: {ctr: sum(v) / len(v) for ctr, v in d.items()} for k, d in acc.items()}
 
 
