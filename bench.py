@@ -518,7 +518,7 @@ def main():
                                   % ("robust NORM R=0" if args.weighting == "robust" else "natural weights"),
                       "nvis": nvis, "nx": nx, "cellsize_arcsec": round(cell, 4), "grid": [plan.nv, plan.nu],
                       "tile": args.tile, "waves": args.waves, "chunk": args.chunk,
-                      "arithmetic": "FP32 visibilities, grids and images; FP64 coordinates and tile accumulation"},
+                      "arithmetic": "FP32 visibilities, grids and images; FP64 coordinates and tile sums; cell runs as FP32 partials of at most 8 points summed in FP64"},
            "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 2),
                           "p99": round(float(np.percentile(lat, 99)) * 1e3, 2)},
            "roofline": roof, "kernels": kernels,

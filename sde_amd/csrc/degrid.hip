@@ -407,14 +407,13 @@ __global__ __launch_bounds__(256) void k_degrid_tiles(const DegridRec *__restric
 // vis_out[idx].  Workgroup ids are dealt so that the `pieces` workgroups of
 // one bin are consecutive on one XCD and each XCD takes its bins in order:
 // the writes in flight stay in a narrow window of the output per XCD.
-__global__ __launch_bounds__(256) void k_degrid_scatter(const float2 *__restrict__ bvis,
-                                                        const uint32_t *__restrict__ bidx,
-                                                        const int *__restrict__ count, int shift, int pieces,
-                                                        float2 *__restrict__ vis_out) {
+__global__ __launch_bounds__(1024) void k_degrid_scatter(const float2 *__restrict__ bvis,
+                                                         const uint32_t *__restrict__ bidx,
+                                                         const int *__restrict__ count, int shift, int pshift,
+                                                         float2 *__restrict__ vis_out) {
     const int l = xcd_swizzle(blockIdx.x, gridDim.x);
-    const int b = l / pieces, piece = l - b * pieces;
-    const int per = (1 << shift) / pieces;
-    const int lo = piece * per, hi = min(lo + per, count[b]);
+    const int b = l >> pshift, piece = l & ((1 << pshift) - 1);
+    const int lo = piece << (shift - pshift), hi = min(lo + (1 << (shift - pshift)), count[b]);
     const size_t base = (size_t)b << shift;
     for (int k = lo + threadIdx.x; k < hi; k += blockDim.x) vis_out[bidx[base + k]] = bvis[base + k];
 }
@@ -645,9 +644,10 @@ static void launch_degrid_tiles_t(hipStream_t s, const DegridRec *rec, const Wor
     }
     hipLaunchKernelGGL((k_degrid_tiles<T, S, true>), dim3(max_items), dim3(256), lds, s, rec, items, nitems, tp,
                        grid, cfu, cfv, cp, vis_out, wt_out, nbad, *bins);
-    const int pieces = std::max(1, (1 << bins->shift) / kDegridScatterPiece);
-    hipLaunchKernelGGL(k_degrid_scatter, dim3(bins->nbins * pieces), dim3(std::min(256, kDegridScatterPiece)), 0, s,
-                       bins->vis, bins->idx, bins->count, bins->shift, pieces, vis_out);
+    // one point per thread: a bin of 2^shift points in 2^pshift workgroups
+    const int pshift = bins->shift - kDegridScatterLog;
+    hipLaunchKernelGGL(k_degrid_scatter, dim3(bins->nbins << pshift), dim3(1 << kDegridScatterLog), 0, s, bins->vis,
+                       bins->idx, bins->count, bins->shift, pshift, vis_out);
 }
 
 DegridBins degrid_bins_for(int64_t n, float2 *vis, uint32_t *idx, int *count) {

@@ -18,8 +18,9 @@
 //                      slots, builds the records in registers (final weight,
 //                      WT*FVIS; the degridding halves go out coalesced) and
 //                      grids them with lane = kernel tap: consecutive points
-//                      of one cell accumulate in FP64 registers and are added
-//                      into LDS when the cell changes.  No global atomics.
+//                      of one cell accumulate in registers (FP32 partials of
+//                      at most 8 points summed in FP64) and are added into
+//                      LDS when the cell changes.  No global atomics.
 //   4. k_grid_reduce   one thread per output cell: FP64 sum of the padded
 //                      tiles covering it, the Hermitian fold of the u = 0
 //                      column, the shift left by SUPPU and the normalisation.
@@ -515,9 +516,11 @@ constexpr int kRecTaps = 2 * kMaxTaps + 1;
 // are in flight while a batch is built (one slot per lane) and gridded.
 // Gridding: each lane is one kernel tap; the batch's records are broadcast
 // with readlane, consecutive points of one cell (the slots are cell-sorted)
-// accumulate in FP64 registers -- the product of two REAL factors is exact
-// in DOUBLE, so one fused multiply-add per channel is the whole update --
-// and are added into the LDS tile when the cell changes.  The tap loop is
+// accumulate in registers -- an FP32 fused multiply-add per channel into a
+// partial of at most kRunPart records, each partial added into the run's
+// FP64 sum (FP64 arithmetic runs at half the FP32 rate, so the per-record
+// update stays FP32) -- and the run is added into the LDS tile when the
+// cell changes.  The tap loop is
 // the kernel's cost, so it is unrolled over the batch's 64 records and
 // carries nothing per record but the update: the kernel values come from a
 // per-wave LDS table filled by the lanes once per batch (read at fixed
@@ -531,6 +534,10 @@ constexpr int kRecTaps = 2 * kMaxTaps + 1;
 // they are built (grd/grdh23d.f:154-156); ZP = 1 also writes the gathered
 // staging records in sorted order, ZP = 2 reads them from there instead of
 // gathering.
+// Records per FP32 partial of a run: each partial's rounding error stays
+// within about kRunPart * 2^-24 of its size.
+constexpr int kRunPart = 8;
+
 template <int T, int CHM, int W, bool DEG, int WGT, int ZP>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) void k_grid_tiles(
     const uint64_t *__restrict__ svals, const StageHot *__restrict__ hot, const StageCold *__restrict__ cold,
@@ -614,6 +621,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
     const int r1 = wi.begin + (int)(((int64_t)len * (wave + 1)) / W);
     double sw = 0.0, ls = 0.0;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;  // the run's current partial (at most kRunPart records)
     int cur = -1;
     if (r0 < r1) {  // wave-uniform; lanes past the end load the wave's last slot
         uint64_t v0 = svals[min(r0 + lane, r1 - 1)];
@@ -710,29 +718,40 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
                     if (cur >= 0 && active) {
                         const int a = (cur >> 8) * P + (cur & 0xff) + lane_off;
                         if (CHM & 1) {
-                            atomicAdd(&S[a], s0);
-                            atomicAdd(&S[NC + a], s1);
+                            atomicAdd(&S[a], s0 + (double)f0);
+                            atomicAdd(&S[NC + a], s1 + (double)f1);
                         }
-                        if (CHM & 2) atomicAdd(&S[PSF * NC + a], s2);
+                        if (CHM & 2) atomicAdd(&S[PSF * NC + a], s2 + (double)f2);
                     }
                     cur = __builtin_amdgcn_readlane(mycell, j);
                     s0 = s1 = s2 = 0.0;
+                    f0 = f1 = f2 = 0.0f;
                 }
-                const double cw = (double)(ta * tb);
+                const float cw = ta * tb;
                 ta = tan;
                 tb = tbn;
                 if (CHM & 1) {
                     const float re = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.re), j));
                     const float im = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.im), j));
-                    s0 = fma(cw, (double)re, s0);
-                    s1 = fma(cw, (double)im, s1);
+                    f0 = fmaf(cw, re, f0);
+                    f1 = fmaf(cw, im, f1);
                 }
                 if (CHM & 2) {
                     const float wt = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.wt), j));
-                    s2 = fma(cw, (double)wt, s2);
+                    f2 = fmaf(cw, wt, f2);
+                }
+                if ((j & (kRunPart - 1)) == kRunPart - 1) {
+                    s0 += (double)f0;
+                    s1 += (double)f1;
+                    s2 += (double)f2;
+                    f0 = f1 = f2 = 0.0f;
                 }
                 }
             }
+            s0 += (double)f0;
+            s1 += (double)f1;
+            s2 += (double)f2;
+            f0 = f1 = f2 = 0.0f;
             // the table is rewritten by the next batch
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();

@@ -510,7 +510,7 @@ struct DegridBins {
 };
 constexpr int kMaxDegridBins = 1024;
 constexpr int kDegridBinShift = 17;       // 2^17 points (1 MB of output) per bin
-constexpr int kDegridScatterPiece = 256;  // points per workgroup of the second pass
+constexpr int kDegridScatterLog = 8;  // log2 of the points (one per thread) per workgroup of the second pass
 // Bins for n points: 2^18-point windows, wider when n needs more than
 // kMaxDegridBins of them.
 DegridBins degrid_bins_for(int64_t n, float2 *vis, uint32_t *idx, int *count);
