@@ -2,6 +2,7 @@
 // (GRDUWT2D grd/grduwt2d.f:77-167, GRDRUW2D grd/grdruw2d.f:87-249) and the
 // synthetic-track generator used for the large workloads.
 #include <algorithm>
+#include <climits>
 
 #include "internal.h"
 
@@ -170,7 +171,8 @@ __global__ __launch_bounds__(256) void k_slowz_keys(const float *__restrict__ u,
 // makes the 64 points of a workgroup neighbours on the grid, so their taps
 // share cache lines instead of each fetching its own from HBM.
 // NZC > 0 fixes the number of planes at compile time (the plane loops
-// unroll and a tap's loads issue together); 0 takes it from pp.
+// unroll and a tap's loads issue together); 0 takes it from pp.  Used for
+// plane counts other than 4, 8 and 16 (k_degrid_slowz_win takes those).
 template <int NZC>
 __global__ __launch_bounds__(64) void k_degrid_slowz(
     const float2 *__restrict__ grid, const float *__restrict__ u, const float *__restrict__ v,
@@ -274,6 +276,192 @@ __global__ __launch_bounds__(64) void k_degrid_slowz(
                 const float uvwt = cfu[delui + cp.osampu * offu + cfoffu - 1] * cv;
                 sumwt = sumwt + uvwt;
                 taps_scaled(cp.uorigin + ucen + offu, vint, uvwt);
+            }
+        }
+    }
+    if (sumwt > 0.0f) {
+        cdiv_by_real(vr, vi, sumwt);
+        wti = fabsf(wti);
+    } else {
+        atomicAdd(nbad, 1ull);
+        vr = vi = 0.0f;
+    }
+    if (cp.uscale * ul > 0.0f) vi = -vi;
+    vis_out[i] = make_float2(vr * rr - vi * ri, vr * ri + vi * rr);
+    wt_out[i] = wti;
+}
+
+// GRDDEH23 as k_degrid_slowz, 256 points per workgroup and NZC (4, 8 or 16)
+// planes, each point's plane rotations CROTZ in registers.  The points off
+// the u = 0 axis are neighbours in the coarse cell order: when the box of
+// their taps, all planes, fits kSlowzWin entries, the workgroup stages it
+// in LDS once and their gathers read it there; the others (near the axis,
+// or a box too wide where points are sparse) gather from the planes in
+// memory.  The same values in the same order: the results are those of
+// k_degrid_slowz.
+constexpr int kSlowzWin = 6144;  // float2 of the LDS window (48 KB)
+template <int NZC>
+__global__ __launch_bounds__(256) void k_degrid_slowz_win(
+    const float2 *__restrict__ grid, const float *__restrict__ u, const float *__restrict__ v,
+    const float *__restrict__ w, const float *__restrict__ wt_in, const float2 *vis_in, int64_t nvis,
+    CoordParams cp, const float *__restrict__ cfu, const float *__restrict__ cfv, PlaneParams pp,
+    const uint64_t *__restrict__ order, float2 *vis_out, float *__restrict__ wt_out,
+    unsigned long long *__restrict__ nbad) {
+    __shared__ float2 win[kSlowzWin];
+    __shared__ int box[4][4];
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int cfoffu = (cp.suppu + 1) * cp.osampu + 1;
+    const int cfoffv = (cp.suppv + 1) * cp.osampv + 1;
+    int64_t i = 0;
+    bool live = false;  // the point still gathers
+    float wti = 0.0f, ul = 0.0f, rr = 1.0f, ri = 0.0f, wz = 0.0f;
+    int nuc = 0, nvc = 0, delui = 0, delvi = 0;
+    if (j < nvis) {
+        i = order ? (int64_t)(uint32_t)order[j] : j;
+        wti = wt_in[i];
+        if (wti <= 0.0f) {
+            wt_out[i] = wti;
+            if (vis_in != vis_out) vis_out[i] = vis_in[i];
+        } else {
+            wti = -wti;
+            const float uf = u[i], vf = v[i], wf = w[i];
+            const double ug = uf, vg = vf, wg = wf;
+            ul = (float)(cp.shift[0] * ug + cp.shift[1] * vg + cp.shift[2] * wg);
+            const float vl = (float)(cp.shift[3] * ug + cp.shift[4] * vg + cp.shift[5] * wg);
+            const float wl = (float)(cp.shift[6] * ug + cp.shift[7] * vg + cp.shift[8] * wg);
+            const float phase = pp.twopi * (wl - wf);
+            rr = cosf(phase);
+            ri = sinf(phase);
+            float ucell = cp.uscale * ul + cp.uoffset, vcell = cp.vscale * vl + cp.voffset;
+            wz = wl;
+            if (ucell > 0.0f) {
+                ucell = ucell + (float)cp.uorigin;
+                vcell = vcell + (float)cp.vorigin;
+            } else {
+                ucell = -ucell + (float)cp.uorigin;
+                vcell = -vcell + (float)cp.vorigin;
+                wz = -wz;
+            }
+            nuc = nint_f(ucell);
+            nvc = nint_f(vcell);
+            if (nuc + cp.suppu > cp.nu || nvc + cp.suppv > cp.nv || nvc - cp.suppv < 1) {
+                vis_out[i] = make_float2(0.0f, 0.0f);
+                wt_out[i] = wti;
+            } else {
+                delui = nint_f((float)cp.osampu * ((float)nuc - ucell));
+                delvi = nint_f((float)cp.osampv * ((float)nvc - vcell));
+                live = true;
+            }
+        }
+    }
+    // the box of the interior points' cells, then of their taps
+    const bool inner = live && nuc > cp.suppu;
+    int umin = inner ? nuc : INT_MAX, umax = inner ? nuc : INT_MIN;
+    int vmin = inner ? nvc : INT_MAX, vmax = inner ? nvc : INT_MIN;
+    for (int o = 32; o > 0; o >>= 1) {
+        umin = min(umin, __shfl_xor(umin, o));
+        umax = max(umax, __shfl_xor(umax, o));
+        vmin = min(vmin, __shfl_xor(vmin, o));
+        vmax = max(vmax, __shfl_xor(vmax, o));
+    }
+    if (lane == 0) {
+        box[wave][0] = umin;
+        box[wave][1] = umax;
+        box[wave][2] = vmin;
+        box[wave][3] = vmax;
+    }
+    __syncthreads();
+    for (int k = 0; k < 4; k++) {
+        umin = min(umin, box[k][0]);
+        umax = max(umax, box[k][1]);
+        vmin = min(vmin, box[k][2]);
+        vmax = max(vmax, box[k][3]);
+    }
+    // interior taps stay on the grid: u in [nuc - SUPPU, nuc + SUPPU] within
+    // [1, NU] (nuc > SUPPU, nuc + SUPPU <= NU), v likewise by the edge test
+    const int u0 = umin - cp.suppu, v0 = vmin - cp.suppv;
+    const int wu = umax - umin + 1 + 2 * cp.suppu, wv = vmax - vmin + 1 + 2 * cp.suppv;
+    const bool staged = umin <= umax && (int64_t)wu * wv * NZC <= kSlowzWin;
+    if (staged) {
+        const int n = wu * wv * NZC;
+        for (int k = threadIdx.x; k < n; k += 256) {
+            const int cell = k / NZC, iz = k - cell * NZC;
+            const int r = cell / wu, c = cell - r * wu;
+            win[k] = grid[((size_t)(v0 + r - 1) * cp.nu + (u0 + c - 1)) * NZC + iz];
+        }
+    }
+    __syncthreads();
+    if (!live) return;
+    float2 cz[NZC];
+#pragma unroll
+    for (int iz = 0; iz < NZC; iz++) {
+        const float ph = -(pp.twopi * wz * pp.dtor * pp.cellz * ((float)(iz + 1) - pp.refz));
+        cz[iz] = make_float2(cosf(ph), sinf(ph));
+    }
+    const int ucen = nuc - cp.uorigin, vcen = nvc - cp.vorigin;
+    float vr = 0.0f, vi = 0.0f, sumwt = 0.0f;
+    // (G * UVWT) * CROTZ of every plane of one tap
+    auto taps_scaled = [&](const float2 *g, float uvwt) {
+#pragma unroll
+        for (int iz = 0; iz < NZC; iz++) {
+            const float2 gv = g[iz];
+            const float ar = gv.x * uvwt, ai = gv.y * uvwt;
+            vr = vr + (ar * cz[iz].x - ai * cz[iz].y);
+            vi = vi + (ar * cz[iz].y + ai * cz[iz].x);
+        }
+    };
+    auto at = [&](int ui, int vj) { return grid + ((size_t)(vj - 1) * cp.nu + (ui - 1)) * NZC; };
+    if (nuc <= cp.suppu) {
+        for (int offu = -cp.suppu; offu <= cp.suppu; offu++) {
+            const float cu = cfu[delui + cp.osampu * offu + cfoffu - 1];
+            if (ucen != -offu) {
+                const int usign = (ucen + offu) >= 0 ? 1 : -1;
+                const int uint_ = cp.uorigin + usign * (ucen + offu);
+                for (int offv = -cp.suppv; offv <= cp.suppv; offv++) {
+                    const int vint = cp.vorigin + usign * (vcen + offv);
+                    const float uvwt = cu * cfv[delvi + cp.osampv * offv + cfoffv - 1];
+                    sumwt = sumwt + uvwt;
+                    const bool in = vint >= 1 && vint <= cp.nv && uint_ >= 1 && uint_ <= cp.nu;
+                    const float2 *g = in ? at(uint_, vint) : grid;
+#pragma unroll
+                    for (int iz = 0; iz < NZC; iz++) {
+                        float2 gv = in ? g[iz] : make_float2(0.0f, 0.0f);
+                        if (usign < 0) gv.y = -gv.y;
+                        const float2 c = cz[iz];
+                        // (FVIS * CROTZ) * UVWT
+                        const float pr = gv.x * c.x - gv.y * c.y, pi = gv.x * c.y + gv.y * c.x;
+                        vr = vr + pr * uvwt;
+                        vi = vi + pi * uvwt;
+                    }
+                }
+            } else {
+                for (int offv = -cp.suppv; offv <= cp.suppv; offv++) {
+                    const int vint = cp.vorigin + (vcen + offv);
+                    const float uvwt = cu * cfv[delvi + cp.osampv * offv + cfoffv - 1];
+                    sumwt = sumwt + uvwt;
+                    taps_scaled(at(cp.uorigin, vint), uvwt);
+                }
+            }
+        }
+    } else if (staged) {
+        for (int offv = -cp.suppv; offv <= cp.suppv; offv++) {
+            const float cv = cfv[delvi + cp.osampv * offv + cfoffv - 1];
+            const float2 *row = win + ((size_t)(nvc + offv - v0) * wu + (nuc - u0)) * NZC;
+            for (int offu = -cp.suppu; offu <= cp.suppu; offu++) {
+                const float uvwt = cfu[delui + cp.osampu * offu + cfoffu - 1] * cv;
+                sumwt = sumwt + uvwt;
+                taps_scaled(row + offu * NZC, uvwt);
+            }
+        }
+    } else {
+        for (int offv = -cp.suppv; offv <= cp.suppv; offv++) {
+            const float cv = cfv[delvi + cp.osampv * offv + cfoffv - 1];
+            const int vint = cp.vorigin + vcen + offv;
+            for (int offu = -cp.suppu; offu <= cp.suppu; offu++) {
+                const float uvwt = cfu[delui + cp.osampu * offu + cfoffu - 1] * cv;
+                sumwt = sumwt + uvwt;
+                taps_scaled(at(cp.uorigin + ucen + offu, vint), uvwt);
             }
         }
     }
@@ -608,16 +796,21 @@ void launch_degrid_slowz(hipStream_t s, const float2 *grid, float2 *grid_inner, 
     const size_t ncell = (size_t)cp.nu * cp.nv;
     hipLaunchKernelGGL(k_planes_inner, dim3((unsigned)std::min<size_t>((ncell + 255) / 256, 8192)), dim3(256), 0, s,
                        grid, grid_inner, ncell, pp.nz);
+    if (pp.nz == 4 || pp.nz == 8 || pp.nz == 16) {
+        const unsigned nbw = (unsigned)((nvis + 255) / 256);
+#define SDE_DSW(NZC)                                                                                             \
+    hipLaunchKernelGGL(k_degrid_slowz_win<NZC>, dim3(nbw), dim3(256), 0, s, grid_inner, u, v, w, wt_in, vis_in, nvis, \
+                       cp, cfu, cfv, pp, order, vis_out, wt_out, nbad)
+        if (pp.nz == 4) SDE_DSW(4);
+        else if (pp.nz == 8) SDE_DSW(8);
+        else SDE_DSW(16);
+#undef SDE_DSW
+        return;
+    }
     const int64_t nb = (nvis + 63) / 64;
     const size_t lds = sizeof(float2) * 64 * (size_t)pp.nz;
-#define SDE_DSZ(NZC)                                                                                          \
-    hipLaunchKernelGGL(k_degrid_slowz<NZC>, dim3((unsigned)nb), dim3(64), lds, s, grid_inner, u, v, w, wt_in, \
-                       vis_in, nvis, cp, cfu, cfv, pp, order, vis_out, wt_out, nbad)
-    if (pp.nz == 4) SDE_DSZ(4);
-    else if (pp.nz == 8) SDE_DSZ(8);
-    else if (pp.nz == 16) SDE_DSZ(16);
-    else SDE_DSZ(0);
-#undef SDE_DSZ
+    hipLaunchKernelGGL(k_degrid_slowz<0>, dim3((unsigned)nb), dim3(64), lds, s, grid_inner, u, v, w, wt_in, vis_in,
+                       nvis, cp, cfu, cfv, pp, order, vis_out, wt_out, nbad);
 }
 
 void launch_degrid_list(hipStream_t s, const float2 *grid, const float *u, const float *v, const float *w,
