@@ -565,18 +565,19 @@ long long slowz_degrid(Ctx &c, const float2 *dg, float2 *dvis, float *dwt, const
     uint32_t *keys = ctx_buf<uint32_t>(c, "keys", nn), *skeys = ctx_buf<uint32_t>(c, "skeys", nn);
     uint64_t *vals = ctx_buf<uint64_t>(c, "vals", nn), *svals = ctx_buf<uint64_t>(c, "svals", nn);
     char *temp = ctx_buf<char>(c, "sort_temp", temp_bytes > 0 ? temp_bytes : 1);
-    if (!cnt || !inner || !keys || !skeys || !vals || !svals || !temp || temp_bytes == 0) {
+    float4 *pts = ctx_buf<float4>(c, "slowz_pts", nn);
+    if (!cnt || !inner || !keys || !skeys || !vals || !svals || !temp || temp_bytes == 0 || !pts) {
         set_plan_error("device allocation failed");
         return -1;
     }
     SDE_CHECK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * 2, s));
     // points in coarse cell order: neighbouring points share the planes' cache lines
-    launch_slowz_keys(s, du, dv, dw, dwt, n, cp, keys, vals);
+    launch_slowz_keys(s, du, dv, dw, dwt, n, cp, keys, vals, pts);
     if (n > 0 && launch_radix_sort(s, temp, temp_bytes, keys, skeys, vals, svals, n, nbits) != hipSuccess) {
         set_plan_error("cell sort failed");
         return -1;
     }
-    launch_degrid_slowz(s, dg, inner, du, dv, dw, dwt, dvis, n, cp, dcfu, dcfv, pp, svals, dvis, dwt, cnt);
+    launch_degrid_slowz(s, dg, inner, pts, dvis, n, cp, dcfu, dcfv, pp, svals, dvis, dwt, cnt);
     SDE_CHECK(hipGetLastError());
     unsigned long long nbad = 0;
     SDE_CHECK(hipMemcpyAsync(&nbad, cnt, sizeof nbad, hipMemcpyDeviceToHost, s));

@@ -128,15 +128,20 @@ __global__ __launch_bounds__(256) void k_planes_inner(const float2 *__restrict__
 
 // Coarse cell keys of the w-plane degridding: the point's REAL cell
 // (grd/grddeh23.f:92-114) in blocks of 8 x 8 cells, row-major; points with
-// WT <= 0 or off the grid take the last key.  vals = index.
+// WT <= 0 or off the grid take the last key.  vals = index; pts = the
+// point's (U, V, W, WT) in one 16-byte record, so the degridder's gather in
+// key order reads one cache line per point instead of four.
 __global__ __launch_bounds__(256) void k_slowz_keys(const float *__restrict__ u, const float *__restrict__ v,
                                                     const float *__restrict__ w, const float *__restrict__ wt,
                                                     int64_t n, CoordParams cp, int nbu, int nbv,
-                                                    uint32_t *__restrict__ keys, uint64_t *__restrict__ vals) {
+                                                    uint32_t *__restrict__ keys, uint64_t *__restrict__ vals,
+                                                    float4 *__restrict__ pts) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
         uint32_t key = (uint32_t)nbu * (uint32_t)nbv;
-        if (wt[i] > 0.0f) {
-            const double ug = u[i], vg = v[i], wg = w[i];
+        const float4 q = make_float4(u[i], v[i], w[i], wt[i]);
+        pts[i] = q;
+        if (q.w > 0.0f) {
+            const double ug = q.x, vg = q.y, wg = q.z;
             const float ul = (float)(cp.shift[0] * ug + cp.shift[1] * vg + cp.shift[2] * wg);
             const float vl = (float)(cp.shift[3] * ug + cp.shift[4] * vg + cp.shift[5] * wg);
             float ucell = cp.uscale * ul + cp.uoffset, vcell = cp.vscale * vl + cp.voffset;
@@ -175,8 +180,7 @@ __global__ __launch_bounds__(256) void k_slowz_keys(const float *__restrict__ u,
 // plane counts other than 4, 8 and 16 (k_degrid_slowz_win takes those).
 template <int NZC>
 __global__ __launch_bounds__(64) void k_degrid_slowz(
-    const float2 *__restrict__ grid, const float *__restrict__ u, const float *__restrict__ v,
-    const float *__restrict__ w, const float *__restrict__ wt_in, const float2 *vis_in, int64_t nvis,
+    const float2 *__restrict__ grid, const float4 *__restrict__ pts, const float2 *vis_in, int64_t nvis,
     CoordParams cp, const float *__restrict__ cfu, const float *__restrict__ cfv, PlaneParams pp,
     const uint64_t *__restrict__ order, float2 *vis_out, float *__restrict__ wt_out,
     unsigned long long *__restrict__ nbad) {
@@ -184,7 +188,8 @@ __global__ __launch_bounds__(64) void k_degrid_slowz(
     const int64_t j = (int64_t)blockIdx.x * 64 + threadIdx.x;
     if (j >= nvis) return;
     const int64_t i = order ? (int64_t)(uint32_t)order[j] : j;
-    float wti = wt_in[i];
+    const float4 q = pts[i];
+    float wti = q.w;
     if (wti <= 0.0f) {
         wt_out[i] = wti;
         if (vis_in != vis_out) vis_out[i] = vis_in[i];
@@ -194,7 +199,7 @@ __global__ __launch_bounds__(64) void k_degrid_slowz(
     const int cfoffv = (cp.suppv + 1) * cp.osampv + 1;
     const int nz = NZC > 0 ? NZC : pp.nz;
     wti = -wti;
-    const float uf = u[i], vf = v[i], wf = w[i];
+    const float uf = q.x, vf = q.y, wf = q.z;
     const double ug = uf, vg = vf, wg = wf;
     const float ul = (float)(cp.shift[0] * ug + cp.shift[1] * vg + cp.shift[2] * wg);
     const float vl = (float)(cp.shift[3] * ug + cp.shift[4] * vg + cp.shift[5] * wg);
@@ -302,8 +307,7 @@ __global__ __launch_bounds__(64) void k_degrid_slowz(
 constexpr int kSlowzWin = 6144;  // float2 of the LDS window (48 KB)
 template <int NZC>
 __global__ __launch_bounds__(256) void k_degrid_slowz_win(
-    const float2 *__restrict__ grid, const float *__restrict__ u, const float *__restrict__ v,
-    const float *__restrict__ w, const float *__restrict__ wt_in, const float2 *vis_in, int64_t nvis,
+    const float2 *__restrict__ grid, const float4 *__restrict__ pts, const float2 *vis_in, int64_t nvis,
     CoordParams cp, const float *__restrict__ cfu, const float *__restrict__ cfv, PlaneParams pp,
     const uint64_t *__restrict__ order, float2 *vis_out, float *__restrict__ wt_out,
     unsigned long long *__restrict__ nbad) {
@@ -319,13 +323,14 @@ __global__ __launch_bounds__(256) void k_degrid_slowz_win(
     int nuc = 0, nvc = 0, delui = 0, delvi = 0;
     if (j < nvis) {
         i = order ? (int64_t)(uint32_t)order[j] : j;
-        wti = wt_in[i];
+        const float4 q = pts[i];
+        wti = q.w;
         if (wti <= 0.0f) {
             wt_out[i] = wti;
             if (vis_in != vis_out) vis_out[i] = vis_in[i];
         } else {
             wti = -wti;
-            const float uf = u[i], vf = v[i], wf = w[i];
+            const float uf = q.x, vf = q.y, wf = q.z;
             const double ug = uf, vg = vf, wg = wf;
             ul = (float)(cp.shift[0] * ug + cp.shift[1] * vg + cp.shift[2] * wg);
             const float vl = (float)(cp.shift[3] * ug + cp.shift[4] * vg + cp.shift[5] * wg);
@@ -782,14 +787,14 @@ int slowz_key_bits(const CoordParams &cp) {
 }
 
 void launch_slowz_keys(hipStream_t s, const float *u, const float *v, const float *w, const float *wt, int64_t n,
-                       const CoordParams &cp, uint32_t *keys, uint64_t *vals) {
+                       const CoordParams &cp, uint32_t *keys, uint64_t *vals, float4 *pts) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_slowz_keys, dim3(prep_blocks(n)), dim3(256), 0, s, u, v, w, wt, n, cp, (cp.nu + 7) / 8,
-                       (cp.nv + 7) / 8, keys, vals);
+                       (cp.nv + 7) / 8, keys, vals, pts);
 }
 
-void launch_degrid_slowz(hipStream_t s, const float2 *grid, float2 *grid_inner, const float *u, const float *v,
-                         const float *w, const float *wt_in, const float2 *vis_in, int64_t nvis,
+void launch_degrid_slowz(hipStream_t s, const float2 *grid, float2 *grid_inner, const float4 *pts,
+                         const float2 *vis_in, int64_t nvis,
                          const CoordParams &cp, const float *cfu, const float *cfv, const PlaneParams &pp,
                          const uint64_t *order, float2 *vis_out, float *wt_out, unsigned long long *nbad) {
     if (nvis <= 0) return;
@@ -799,7 +804,7 @@ void launch_degrid_slowz(hipStream_t s, const float2 *grid, float2 *grid_inner, 
     if (pp.nz == 4 || pp.nz == 8 || pp.nz == 16) {
         const unsigned nbw = (unsigned)((nvis + 255) / 256);
 #define SDE_DSW(NZC)                                                                                             \
-    hipLaunchKernelGGL(k_degrid_slowz_win<NZC>, dim3(nbw), dim3(256), 0, s, grid_inner, u, v, w, wt_in, vis_in, nvis, \
+    hipLaunchKernelGGL(k_degrid_slowz_win<NZC>, dim3(nbw), dim3(256), 0, s, grid_inner, pts, vis_in, nvis,          \
                        cp, cfu, cfv, pp, order, vis_out, wt_out, nbad)
         if (pp.nz == 4) SDE_DSW(4);
         else if (pp.nz == 8) SDE_DSW(8);
@@ -809,8 +814,8 @@ void launch_degrid_slowz(hipStream_t s, const float2 *grid, float2 *grid_inner, 
     }
     const int64_t nb = (nvis + 63) / 64;
     const size_t lds = sizeof(float2) * 64 * (size_t)pp.nz;
-    hipLaunchKernelGGL(k_degrid_slowz<0>, dim3((unsigned)nb), dim3(64), lds, s, grid_inner, u, v, w, wt_in, vis_in,
-                       nvis, cp, cfu, cfv, pp, order, vis_out, wt_out, nbad);
+    hipLaunchKernelGGL(k_degrid_slowz<0>, dim3((unsigned)nb), dim3(64), lds, s, grid_inner, pts, vis_in, nvis, cp,
+                       cfu, cfv, pp, order, vis_out, wt_out, nbad);
 }
 
 void launch_degrid_list(hipStream_t s, const float2 *grid, const float *u, const float *v, const float *w,

@@ -558,11 +558,12 @@ void launch_prep_slowz(hipStream_t s, const float2 *vis, const float *wt, const 
 // REAL arithmetic (grd/grddeh23.f:86-210); grid holds NZ planes of NU x NV,
 // re-laid cell-major into grid_inner (NU*NV*NZ float2 of scratch) first.
 // The points are taken in the order of `order` (index in the low 32 bits;
-// null: the caller's order).
-void launch_degrid_slowz(hipStream_t s, const float2 *grid, float2 *grid_inner, const float *u, const float *v,
-                         const float *w, const float *wt_in, const float2 *vis_in, int64_t nvis,
-                         const CoordParams &cp, const float *cfu, const float *cfv, const PlaneParams &pp,
-                         const uint64_t *order, float2 *vis_out, float *wt_out, unsigned long long *nbad);
+// null: the caller's order); pts holds each point's (U, V, W, WT) as
+// written by launch_slowz_keys.
+void launch_degrid_slowz(hipStream_t s, const float2 *grid, float2 *grid_inner, const float4 *pts,
+                         const float2 *vis_in, int64_t nvis, const CoordParams &cp, const float *cfu,
+                         const float *cfv, const PlaneParams &pp, const uint64_t *order, float2 *vis_out,
+                         float *wt_out, unsigned long long *nbad);
 // GRDQH23D (BOX w planes): acc (2*NU*NV*NZ doubles) and sums[0] (SUMWT)
 // zeroed by the caller; the fold writes the NZ planes of gvis.
 void launch_box_grid_slowz(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
@@ -573,7 +574,7 @@ void launch_box_fold_slowz(hipStream_t s, const double *acc, const CoordParams &
 // Coarse (8 x 8 cell) keys of the points for that order, with vals = index;
 // sort over slowz_key_bits(cp) bits.
 void launch_slowz_keys(hipStream_t s, const float *u, const float *v, const float *w, const float *wt, int64_t n,
-                       const CoordParams &cp, uint32_t *keys, uint64_t *vals);
+                       const CoordParams &cp, uint32_t *keys, uint64_t *vals, float4 *pts);
 int slowz_key_bits(const CoordParams &cp);
 
 // ---- plain copy of n float4 (bandwidth probe, fft.hip) ----
