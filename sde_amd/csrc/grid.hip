@@ -341,14 +341,29 @@ __global__ __launch_bounds__(kThreads) void k_density_runs(
     const int tt = tp.T * tp.T;
     double dsum = 0.0, unused = 0.0;
     const int64_t stride = (int64_t)gridDim.x * kThreads * 4;
+    // the next iteration's slots are loaded while this one's are summed
+    uint4 k4n = make_uint4(0u, 0u, 0u, 0u);
+    ulonglong2 van = make_ulonglong2(0ull, 0ull), vbn = make_ulonglong2(0ull, 0ull);
+    {
+        const int64_t q = (int64_t)blockIdx.x * kThreads * 4 + 4 * (int64_t)threadIdx.x;
+        if (q + 3 < n) {
+            k4n = *reinterpret_cast<const uint4 *>(skeys + q);
+            van = *reinterpret_cast<const ulonglong2 *>(svals + q);
+            vbn = *reinterpret_cast<const ulonglong2 *>(svals + q + 2);
+        }
+    }
     for (int64_t base = (int64_t)blockIdx.x * kThreads * 4; base < n; base += stride) {
         const int64_t q0 = base + 4 * (int64_t)threadIdx.x;
+        const uint4 k4 = k4n;
+        const ulonglong2 va = van, vb = vbn;
+        if (q0 + stride + 3 < n) {
+            k4n = *reinterpret_cast<const uint4 *>(skeys + q0 + stride);
+            van = *reinterpret_cast<const ulonglong2 *>(svals + q0 + stride);
+            vbn = *reinterpret_cast<const ulonglong2 *>(svals + q0 + stride + 2);
+        }
         uint32_t kk[4];
         double xx[4];
         if (q0 + 3 < n) {
-            const uint4 k4 = *reinterpret_cast<const uint4 *>(skeys + q0);
-            const ulonglong2 va = *reinterpret_cast<const ulonglong2 *>(svals + q0);
-            const ulonglong2 vb = *reinterpret_cast<const ulonglong2 *>(svals + q0 + 2);
             const uint32_t kr[4] = {k4.x, k4.y, k4.z, k4.w};
             const uint64_t vr[4] = {va.x, va.y, vb.x, vb.y};
 #pragma unroll
