@@ -1208,6 +1208,10 @@ struct sde_plan {
     double *d_gwt = nullptr, *d_wblk = nullptr, *d_wsums = nullptr;
     unsigned long long *d_cnt = nullptr;
     int *d_dbins = nullptr;  // fill counters of the degridder's output bins
+    int *d_slot = nullptr;   // each sorted record's slot in the output bins
+    size_t slot_cap = 0;
+    unsigned sort_gen = 0;   // bumped by every cell sort
+    unsigned slot_gen = 0;   // the sort d_slot (and the bins' idx and counts) belong to
     // profiling
     bool profile = false;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
@@ -1318,6 +1322,7 @@ int plan_reserve_items(sde_plan *p) {
 // per-point degridding outputs into the model arrays.
 int plan_sort(sde_plan *p, const DensityOut *dens, bool bounds = true) {
     p->sorted_ok = false;
+    p->sort_gen++;
     auto mb = [p](const char *nm) { plan_mark_begin(p, nm); };
     auto me = [p]() { plan_mark_end(p); };
     DegridOut deg;
@@ -1371,9 +1376,18 @@ int plan_vismodel(sde_plan *p, const float *img) {
     plan_mark_end(p, p->side);
     SDE_CHECK(hipEventRecord(p->ev_join, p->side));
     // the sort's unsorted keys and values (dead until the next prep pass) hold the output bins
-    const DegridBins bins = degrid_bins_for(p->nvis, reinterpret_cast<float2 *>(p->sb.vals), p->sb.keys, p->d_dbins);
+    DegridBins bins = degrid_bins_for(p->nvis, reinterpret_cast<float2 *>(p->sb.vals), p->sb.keys, p->d_dbins);
+    // the slots of the first prediction on a cell sort serve the later ones (a major cycle)
+    if ((size_t)p->nvis > p->slot_cap) {
+        p->slot_cap = 0;
+        p->slot_gen = 0;
+        if (plan_alloc(p, &p->d_slot, (size_t)p->nvis)) return -1;
+        p->slot_cap = (size_t)p->nvis;
+    }
+    bins.slot = p->d_slot;
+    bins.reuse = p->slot_gen != 0 && p->slot_gen == p->sort_gen;
     plan_mark_begin(p, "degrid");
-    SDE_CHECK(hipMemsetAsync(p->d_dbins, 0, sizeof(int) * (size_t)bins.nbins, s));
+    if (!bins.reuse) SDE_CHECK(hipMemsetAsync(p->d_dbins, 0, sizeof(int) * (size_t)bins.nbins, s));
     const int rc = launch_degrid_tiles(s, p->d_recd, p->sb.items, p->sb.nitems, p->sb.max_items, p->tp, p->d_mgrid,
                                        p->d_cf, p->d_cf, p->cp, p->d_mvis, p->d_mwt, p->d_cnt + 1, &bins);
     plan_mark_end(p);
@@ -1382,6 +1396,7 @@ int plan_vismodel(sde_plan *p, const float *img) {
         set_plan_error("unsupported degridding tile T=%d", p->tp.T);
         return -1;
     }
+    p->slot_gen = p->sort_gen;
     SDE_CHECK(hipGetLastError());
     p->predicted = true;
     return 0;
@@ -1603,7 +1618,7 @@ int sde_plan_destroy(sde_plan *p) {
                     p->sb.keys, p->sb.skeys, p->sb.vals, p->sb.svals, p->sb.hot, p->sb.cold, p->sb.temp, p->sb.tile_start,
                     p->sb.item_start, p->sb.nitems, p->sb.items, p->d_recd, p->d_list, p->d_nlist,
                     p->d_partial, p->d_isums, p->d_sums, p->d_grids, p->d_gwork, p->d_mgrid, p->d_dirty, p->d_psf, p->d_img_in,
-                    p->d_peak, p->d_gwt, p->d_wblk, p->d_wsums, p->d_cnt, p->d_wsq, p->d_dbins};
+                    p->d_peak, p->d_gwt, p->d_wblk, p->d_wsums, p->d_cnt, p->d_wsq, p->d_dbins, p->d_slot};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &kv : p->tw) (void)hipFree(kv.second);

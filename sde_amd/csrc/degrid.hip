@@ -521,7 +521,7 @@ __global__ __launch_bounds__(256) void k_degrid_tiles(const DegridRec *__restric
     const int item = blockIdx.x;
     if (item >= *nitems) return;
     const WorkItem wi = items[item];
-    if (BIN) {
+    if (BIN && !bins.reuse) {
         for (int k = threadIdx.x; k < bins.nbins; k += blockDim.x) bcnt[k] = 0;
         __syncthreads();
         for (int r = wi.begin + threadIdx.x; r < wi.end; r += blockDim.x) {
@@ -586,10 +586,16 @@ __global__ __launch_bounds__(256) void k_degrid_tiles(const DegridRec *__restric
         // GRDDEH2D always rotates: by (1, 0) when there is no shift, exactly
         const float2 out = make_float2(vr * d.rr - vi * d.ri, vr * d.ri + vi * d.rr);
         if (BIN) {
-            const int b = (int)(idx >> bins.shift);
-            const int slot = bbase[b] + atomicAdd(&bcnt[b], 1);
+            int slot;
+            if (bins.reuse) {
+                slot = bins.slot[r];
+            } else {
+                const int b = (int)(idx >> bins.shift);
+                slot = bbase[b] + atomicAdd(&bcnt[b], 1);
+                bins.idx[slot] = idx;
+                if (bins.slot) bins.slot[r] = slot;
+            }
             bins.vis[slot] = out;
-            bins.idx[slot] = idx;
         } else {
             vis_out[idx] = out;
         }

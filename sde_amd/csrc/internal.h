@@ -507,6 +507,11 @@ struct DegridBins {
     uint32_t *idx = nullptr;
     int *count = nullptr;
     int shift = 0, nbins = 0;
+    // slot[r]: where sorted record r's result goes, written by a first pass
+    // (reuse = 0) and followed by later passes on the same cell sort
+    // (reuse = 1), which then neither count nor rank: idx and count stand
+    int *slot = nullptr;
+    int reuse = 0;
 };
 constexpr int kMaxDegridBins = 1024;
 constexpr int kDegridBinShift = 17;       // 2^17 points (1 MB of output) per bin

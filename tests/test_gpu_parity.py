@@ -316,27 +316,35 @@ def test_plan_uvmap_and_vismodel_small_field(small):
     print("C1 vismodel rel err gpu %.2e ref32 %.2e" % (e, e32))
 
 
-def test_plan_vismodel_many_output_bins():
-    """Predictions of 758,160 points (351 baselines x 360 times x 6
-    channels): the tile degridder's results reach the output through six
-    bins of 2^17 indices, hundreds of scatter workgroups per bin dealt over
-    the XCDs.  Every point is checked against the oracle, so a lost, doubled
-    or misplaced write fails."""
+def six_channel_track(seed=9):
+    """351 baselines x 360 times x 6 channels (758,160 points) with random
+    weights, 2000 flagged, and a 512^2 point-source image."""
     enu = synth.y_array()
     ntime = 360
     dha = 10.0 / 86400.0 * 2 * math.pi * 1.00273790935
     freqs = np.linspace(1.2e9, 1.6e9, 6)
     u, v, w = synth.track(enu, synth.LAT_Y, synth.DEC_Y, -0.5 * (ntime - 1) * dha, dha, ntime, freqs)
     n = len(u)
-    assert n == 351 * 360 * 6 and n > 5 * 2 ** 17
     rng = np.random.default_rng(5)
     wt = rng.uniform(0.5, 2.0, n).astype(np.float32)
     wt[rng.choice(n, 2000, replace=False)] = 0.0   # flagged points keep their inputs
     nx, cell = 512, synth.cell_for(u, v)
-    flux, l, m = synth.point_sky(30, nx // 4 * cell / 206264.8, seed=9)
+    flux, l, m = synth.point_sky(30, nx // 4 * cell / 206264.8, seed=seed)
     image = np.zeros((nx, nx), np.float32)
     for f, a, b in zip(flux, l, m):
         image[nx // 2 + int(round(b * 206264.8 / cell)), nx // 2 - int(round(a * 206264.8 / cell))] += f
+    return u, v, w, wt, nx, cell, image
+
+
+def test_plan_vismodel_many_output_bins():
+    """Predictions of 758,160 points (351 baselines x 360 times x 6
+    channels): the tile degridder's results reach the output through six
+    bins of 2^17 indices, hundreds of scatter workgroups per bin dealt over
+    the XCDs.  Every point is checked against the oracle, so a lost, doubled
+    or misplaced write fails."""
+    u, v, w, wt, nx, cell, image = six_channel_track()
+    n = len(u)
+    assert n == 351 * 360 * 6 and n > 5 * 2 ** 17
     mv, mw = imaging.vismodel(image, wt, u, v, w, cell)
     cfg = po.make_cfg(nx, cellsize=cell)
     v32, w32, _ = po.vismodel(cfg, image, wt, u, v, w)
@@ -344,6 +352,37 @@ def test_plan_vismodel_many_output_bins():
     np.testing.assert_array_equal(mw, w32)
     e, e32 = assert_vis_rel_ref(mv, v32, v64, "vismodel 758k")
     print("vismodel 758k rel err gpu %.2e ref32 %.2e" % (e, e32))
+
+
+def test_plan_vismodel_again_on_the_same_sort():
+    """Later predictions on a standing cell sort write through the output
+    bin slots that the first one placed (no count, no ranking): each equals
+    a fresh plan's prediction of the same image bit for bit, and predicting
+    the first image again gives the first result."""
+    u, v, w, wt, nx, cell, img1 = six_channel_track(seed=9)
+    img2 = six_channel_track(seed=19)[-1]
+    vis0 = np.zeros(len(u), np.complex64)
+    a = imaging.Plan(nx, None, cell)
+    b = imaging.Plan(nx, None, cell)
+    try:
+        a.set_vis(vis0, wt, u, v, w)
+        a.vismodel(img1)
+        m1, w1 = a.model()
+        a.vismodel(img2)
+        m2, w2 = a.model()
+        a.vismodel(img1)
+        m1b, w1b = a.model()
+        b.set_vis(vis0, wt, u, v, w)
+        b.vismodel(img2)
+        f2, fw2 = b.model()
+    finally:
+        a.close()
+        b.close()
+    assert not np.array_equal(m1, m2)
+    np.testing.assert_array_equal(m2, f2)
+    np.testing.assert_array_equal(w2, fw2)
+    np.testing.assert_array_equal(m1b, m1)
+    np.testing.assert_array_equal(w1b, w1)
 
 
 @pytest.mark.parametrize("rmode,robust,ds,fov,shifted", [
