@@ -142,6 +142,46 @@ def test_grddeh23_matches_reference_arithmetic(shifted, nz, cellz, refz):
         assert np.array_equal(va[good], vr[good])
 
 
+@pytest.mark.parametrize("nz", [8, 16])
+def test_grddeh23_dense_points_staged_window(nz):
+    """Dense points (200,000 within a few tens of cells of the centre of a
+    257 x 512 grid): the 256 points of a degridding workgroup lie in one or
+    two coarse cells, so the box of their taps over all planes fits the LDS
+    window and their gathers read it there; 5000 points near the u = 0 axis
+    gather from the planes in memory.  Checked as the sparse case above."""
+    nu, nv, vo, cellz, refz = 257, 512, 256, 2.0, 4.5
+    rng = np.random.default_rng(47)
+    n = 200_000
+    u = rng.normal(0.0, 20.0, n)
+    v = rng.normal(0.0, 20.0, n)
+    u[:5000] = rng.uniform(-3.5, 3.5, 5000)
+    w = rng.uniform(-8, 8, n).astype(np.float32)
+    u, v = u.astype(np.float32), v.astype(np.float32)
+    wt = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    wt[::23] = -1.0
+    g64 = rng.standard_normal((nz, nv, nu)) + 1j * rng.standard_normal((nz, nv, nu))
+    g32 = g64.astype(np.complex64)
+    cf, supp, osamp = po.grdcf("SF")
+    cf64, _, _ = po.grdcf("SF", fp64=True)
+    vin = np.full(n, 3 + 4j, np.complex64)
+    print("grddeh23 dense n=%d nz=%d" % (n, nz), flush=True)
+    vg, wg = grd.grddeh23(vin, wt, u, v, w, 1.0, 0.0, 1.0, 0.0, 1, vo, g32, cf, supp, osamp, cellz, refz)
+    vr, wr, _ = po.grddeh23(wt, u, v, w, 1.0, 1.0, 1, vo, g32, cf, supp, osamp, cellz, refz, IDENT)
+    v64, _, _ = po.grddeh23_64(wt, u, v, w, 1.0, 1.0, 1, vo, g32.astype(np.complex128), cf64, supp, osamp, cellz,
+                               refz, IDENT)
+    flagged = wt <= 0
+    assert np.all(vg[flagged] == vin[flagged])
+    np.testing.assert_array_equal(wg, wr)
+    good = wg > 0
+    assert np.sum(good) > 0.9 * n
+    assert_close_rel(vg[good], v64[good], "dense degrid vs FP64")
+    scale = np.abs(v64[good]).max()
+    eg = np.abs(vg[good] - v64[good]).max() / scale
+    er = np.abs(vr[good] - v64[good]).max() / scale
+    print("dense nz=%d rel err gpu %.2e ref32 %.2e" % (nz, eg, er))
+    assert eg <= 2 * er + 1e-7
+
+
 def test_device_entry_points_match_the_drop_ins():
     """sde_grdh23d_device / sde_grddeh23_device on device arrays give the
     drop-ins' results bit for bit (the same kernels on the same inputs)."""
