@@ -47,7 +47,11 @@ void grdh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis,
              float *gridfnu, int *suppu, int *osampu, float *gridfnv, int *suppv,
              int *osampv, double *shift, float *sumwt);
 
-/* Nearest-cell (BOX) gridding.  Replaces GRDQH2D, source/grd/grdqh2d.f:7-9. */
+/* Nearest-cell (BOX) gridding: the cell NINT(UCELL)+1, NINT(VCELL) of every
+ * point, the u-origin column folded, the grid shifted left by one column
+ * and normalised by (NU-1)*NV/LSUMWT; SUMWT counts every point with WT > 0.
+ * The sums run over the cell-sorted points in a fixed order.  Replaces
+ * GRDQH2D, source/grd/grdqh2d.f:7-9 (args :51-59). */
 void grdqh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis,
               float *uscale, float *uoffset, float *vscale, float *voffset,
               int *uorigin, int *vorigin, float *gvis, int *psf, int *nu, int *nv,
@@ -111,6 +115,15 @@ void grdruw2d_(float *wt, float *u, float *v, float *w, int *nvis, float *uscale
                double *shift, const char *rmode, float *robust, float *ds,
                size_t rmode_len);
 
+/* Selection by time range and uv radius: every point with WT > 0 whose TIME
+ * lies outside TIMERANG (any time when TIMERANG = (0, 0)) or whose
+ * U**2+V**2 lies outside [UVLIMITS(1)**2, UVLIMITS(2)**2] gets WT = -|WT|,
+ * in place; NSEL counts the others.  NEWWT is not written, as in the
+ * reference, whose caller VISSEL passes WT for both (vis/vissel.f).
+ * Replaces VISSELPI, source/vis/visselpi.f:7-8 (args :33-35). */
+void visselpi_(float *wt, int *nvis, float *time, float *u, float *v, float *w,
+               float *timerang, float *uvlimits, float *newwt, int *nsel);
+
 /* Convolution function table (BOX | SF | SZE), OSAMP = 128, unit sum.
  * Replaces GRDCF, source/grd/grdcf.f:7. */
 void grdcf_(const char *cfcode, float *cf, int *cfsupp, int *cfosamp, size_t len);
@@ -160,7 +173,8 @@ typedef struct sde_plan sde_plan;
 typedef struct {
     int nx, ny;            /* image size (pixels) */
     float cellsize;        /* arcsec, both axes (IMGMAKE: DELT = -cell/3600 on RA) */
-    char cftype[4];        /* "SF" or "SZE" (BOX: use grdqh2d_) */
+    char cftype[4];        /* "SF", "SZE" or "BOX" (VISGRID's dispatch: BOX grids
+                              nearest-cell as GRDQH2D, vis/visgrid.f:154-165) */
     double shift[9];       /* CRDSHIFT matrix, column-major (identity: no shift) */
     int weighting;         /* 0 natural, 1 density weighting (GRDUWT2D / GRDRUW2D) */
     float fov;             /* WTFOV of the density grid */
@@ -170,6 +184,12 @@ typedef struct {
     int tile;              /* gridding tile edge in cells: 0 = tuned default, else 16 or 32 */
     int chunk;             /* visibilities per work item: 0 = tuned default */
     int waves;             /* waves per gridding workgroup: 0 = tuned default, else 4 or 8 */
+    int select;            /* 1: uvmap selects by time and uv radius first (VISSEL,
+                              task/uvmap.f:158-168) when the plan holds TIME
+                              (sde_plan_set_time); without TIME it warns and selects
+                              nothing, as the task does */
+    float uvlimits[2];     /* UVLIMITS: allowed uv radius range (wavelengths) */
+    float timerange[2];    /* TIMERANG: allowed time range; (0, 0) = any time */
 } sde_imaging_cfg;
 
 /* Per-visibility accounting of the last run. */
@@ -182,6 +202,8 @@ typedef struct {
     double lsumwt;         /* sum of all convolution taps (normalisation) */
     float maxpsf;          /* PSF peak before normalisation */
     int nitems;            /* gridding work items of the last run */
+    int64_t nselected;     /* kept by the selection of the last uvmap (VISSEL's NSEL;
+                              -1 when it selected nothing) */
 } sde_run_info;
 
 int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **plan);
@@ -194,8 +216,12 @@ int sde_plan_destroy(sde_plan *plan);
  * complex (2*nvis floats). */
 int sde_plan_set_vis(sde_plan *plan, int64_t nvis, const float *vis, const float *wt,
                      const float *u, const float *v, const float *w, int on_device);
-/* Weighting, PSF and dirty image (both divided by the PSF peak), all on the
- * device (task/uvmap.f:276-282,428-482). */
+/* The time of each resident visibility (nvis floats, host or device
+ * source; synchronous), for the selection of uvmap.  No counterpart in the
+ * reference's pixel-level routines: the task reads 'Vis/TIME' (task/uvmap.f:158). */
+int sde_plan_set_time(sde_plan *plan, const float *time, int on_device);
+/* Selection (with cfg.select), weighting, PSF and dirty image (both divided
+ * by the PSF peak), all on the device (task/uvmap.f:158-168,276-282,428-482). */
 int sde_plan_uvmap(sde_plan *plan);
 /* Major cycle (the CLEAN and mosaic tasks' repeated VISTOIMG/IMGTOVIS,
  * e.g. task/mosclean.f): sde_plan_subtract_model replaces the resident

@@ -62,6 +62,17 @@ int r32_grdqh2d(const float *vis, const float *wt, const float *u, const float *
                 const float *w, int nvis, float uscale, float uoffset, float vscale,
                 float voffset, int uorigin, int vorigin, float *gvis, int psf, int nu,
                 int nv, const double *shift, float *sumwt);
+/* FP64 GRDQH2D: the REAL cell step, values in double; *lsumwt = the gridded
+ * weights. */
+int o64_grdqh2d(const float *vis, const double *wt, const float *u, const float *v,
+                const float *w, int nvis, float uscale, float uoffset, float vscale,
+                float voffset, int uorigin, int vorigin, double *gvis, int psf, int nu,
+                int nv, const double *shift, double *sumwt, double *lsumwt);
+
+/* ---- selection (vis/visselpi.f): WT negated in place, NEWWT untouched ---- */
+void r32_visselpi(float *wt, int nvis, const float *time, const float *u, const float *v,
+                  const float *w, const float *timerang, const float *uvlimits, float *newwt,
+                  int *nsel);
 
 /* ---- degridding (grd/grddeh2d.f) ---- */
 void r32_grddeh2d(float *vis, float *wt, const float *u, const float *v, const float *w,

@@ -482,7 +482,7 @@ int o64_uvmap(const or_uvmap_cfg *cfg, const float *vis, const float *wt, const 
     double cf[1025], *ccf, *wts, *grid, maxpsf = 0.0, lsum;
     int supp, osamp, rc = 0;
     size_t npix = (size_t)cfg->nx * cfg->ny;
-    if (strncmp(cfg->cftype, "BOX", 3) == 0) return 2; /* BOX is checked through the FP32 restatement */
+    const int box = strncmp(cfg->cftype, "BOX", 3) == 0;  /* VISGRID's BOX dispatch, vis/visgrid.f:154-158 */
     make_geom64(cfg, &g);
     if (o64_grdcf(cfg->cftype, cf, &supp, &osamp)) return 2;
     ccf = (double *)malloc(sizeof(double) * g.ccflen);
@@ -502,8 +502,12 @@ int o64_uvmap(const or_uvmap_cfg *cfg, const float *vis, const float *wt, const 
     for (int pass = 0; pass < 2 && rc == 0; pass++) {
         double *img = pass == 0 ? psf : dirty;
         double sw;
-        rc = o64_grdh2d(vis, wts, u, v, w, nvis, g.uscale, 0.0f, g.vscale, 0.0f, g.uorigin, g.vorigin, grid, pass == 0,
-                        g.nu, g.nv, cf, supp, osamp, g.shift, &sw, &lsum);
+        if (box)
+            rc = o64_grdqh2d(vis, wts, u, v, w, nvis, g.uscale, 0.0f, g.vscale, 0.0f, g.uorigin, g.vorigin, grid,
+                             pass == 0, g.nu, g.nv, g.shift, &sw, &lsum);
+        else
+            rc = o64_grdh2d(vis, wts, u, v, w, nvis, g.uscale, 0.0f, g.vscale, 0.0f, g.uorigin, g.vorigin, grid,
+                            pass == 0, g.nu, g.nv, cf, supp, osamp, g.shift, &sw, &lsum);
         if (rc) break;
         if (pass == 0) *sumwt = sw;
         o64_fftr2d(img, grid, cfg->nx, cfg->ny, g.fnu, g.nv, -1);
@@ -527,7 +531,6 @@ int o64_vismodel(const or_uvmap_cfg *cfg, const double *image, double *vis, floa
     double cf[1025], *ccf, *img, *grid;
     int supp, osamp;
     size_t npix = (size_t)cfg->nx * cfg->ny;
-    if (strncmp(cfg->cftype, "BOX", 3) == 0) return 2;
     make_geom64(cfg, &g);
     if (o64_grdcf(cfg->cftype, cf, &supp, &osamp)) return 2;
     ccf = (double *)malloc(sizeof(double) * g.ccflen);
@@ -832,5 +835,65 @@ int o64_grdqh23d(const float *vis, const double *wt, const float *u, const float
             }
             GRE3(gvis, nu, nv, nu, vg, iz) = GIM3(gvis, nu, nv, nu, vg, iz) = 0.0;
         }
+    return 0;
+}
+
+/* GRDQH2D (grd/grdqh2d.f:85-147) in FP64: the reference's REAL cell step
+ * (ULOCAL, VLOCAL, WLOCAL rounded to REAL, NINT(UCELL)+1, NINT(VCELL)), so the
+ * same cell takes every point; the phase rotation, the products, the grid,
+ * the fold of the u-origin column and the normalisation (NU-1)*NV/LSUMWT in
+ * double.  SUMWT counts every point with WT > 0, LSUMWT the gridded ones;
+ * *lsumwt returns the latter.  Returns 1 when nothing was gridded. */
+int o64_grdqh2d(const float *vis, const double *wt, const float *u, const float *v,
+                const float *w, int nvis, float uscale, float uoffset, float vscale,
+                float voffset, int uorigin, int vorigin, double *gvis, int psf, int nu,
+                int nv, const double *shift, double *sumwt, double *lsumwt) {
+    const double twopi = (double)(8.0f * atanf(1.0f));
+    double ls = 0.0;
+    memset(gvis, 0, sizeof(double) * 2 * (size_t)nu * nv);
+    *sumwt = 0.0;
+    for (int i = 0; i < nvis; i++) {
+        float ul, vl, wl, ucell, vcell;
+        if (wt[i] <= 0.0) continue;
+        local_uvw3(shift, u[i], v[i], w[i], &ul, &vl, &wl);
+        const double phase = twopi * ((double)wl - (double)w[i]);
+        const double c = cos(phase), s = -sin(phase);
+        double fr = vis[2 * i] * c - vis[2 * i + 1] * s;
+        double fi = vis[2 * i] * s + vis[2 * i + 1] * c;
+        ucell = uscale * ul + uoffset;
+        vcell = vscale * vl + voffset;
+        *sumwt += wt[i];
+        if (ucell > 0.0f) {
+            fi = -fi;
+            ucell = ucell + (float)uorigin;
+            vcell = vcell + (float)vorigin;
+        } else {
+            ucell = -ucell + (float)uorigin;
+            vcell = -vcell + (float)vorigin;
+        }
+        if (psf) { fr = 1.0; fi = 0.0; }
+        const int ugrid = nintf(ucell) + 1, vgrid = nintf(vcell);
+        if (ugrid < 1 || ugrid > nu || vgrid < 1 || vgrid > nv) continue;
+        GRE(gvis, nu, ugrid, vgrid) += wt[i] * fr;
+        GIM(gvis, nu, ugrid, vgrid) += wt[i] * fi;
+        ls += wt[i];
+    }
+    const int uz = uorigin + 1, vz = vorigin;
+    for (int offv = 0; offv <= nv / 2 - 1; offv++) {
+        GRE(gvis, nu, uz, vz + offv) += GRE(gvis, nu, uz, vz - offv);
+        GIM(gvis, nu, uz, vz + offv) -= GIM(gvis, nu, uz, vz - offv);
+        GRE(gvis, nu, uz, vz - offv) = GRE(gvis, nu, uz, vz + offv);
+        GIM(gvis, nu, uz, vz - offv) = -GIM(gvis, nu, uz, vz + offv);
+    }
+    *lsumwt = ls;
+    if (ls == 0.0) return 1;
+    const double rnorm = (double)(nu - 1) * (double)nv / ls;
+    for (int vg = 1; vg <= nv; vg++) {
+        for (int ug = 1; ug <= nu - 1; ug++) {
+            GRE(gvis, nu, ug, vg) = rnorm * GRE(gvis, nu, ug + 1, vg);
+            GIM(gvis, nu, ug, vg) = rnorm * GIM(gvis, nu, ug + 1, vg);
+        }
+        GRE(gvis, nu, nu, vg) = GIM(gvis, nu, nu, vg) = 0.0;
+    }
     return 0;
 }

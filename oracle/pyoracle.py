@@ -87,6 +87,9 @@ for name, res, args in [
                              _d, _d]),
     ("r32_grdqh2d", C.c_int, [_f, _f, _f, _f, _f, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
                               C.c_int, C.c_int, _f, C.c_int, C.c_int, C.c_int, _d, _f]),
+    ("o64_grdqh2d", C.c_int, [_f, _d, _f, _f, _f, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
+                              C.c_int, C.c_int, _d, C.c_int, C.c_int, C.c_int, _d, _d, _d]),
+    ("r32_visselpi", None, [_f, C.c_int, _f, _f, _f, _f, _f, _f, _f, _i]),
     ("r32_grddeh2d", None, [_f, _f, _f, _f, _f, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
                             C.c_int, C.c_int, _f, C.c_int, C.c_int, _f, C.c_int, C.c_int, _f, C.c_int,
                             C.c_int, _d, _i]),
@@ -225,6 +228,31 @@ def grdqh2d(vis, wt, u, v, w, uscale, vscale, uorigin, vorigin, psf, nu, nv, shi
                          uorigin, vorigin, _pf(g), int(psf), nu, nv, _pd(np.ascontiguousarray(shift, np.float64)),
                          _pf(sw))
     return g, float(sw[0]), rc
+
+
+def grdqh2d64(vis, wt, u, v, w, uscale, vscale, uorigin, vorigin, psf, nu, nv, shift, uoffset=0.0, voffset=0.0):
+    """FP64 GRDQH2D -> (GVIS (NV, NU) complex128, SUMWT, LSUMWT, rc)."""
+    vis, u, v, w = c64(vis), f32(u), f32(v), f32(w)
+    wt = np.ascontiguousarray(wt, np.float64)
+    g = np.zeros((nv, nu), np.complex128)
+    sw, ls = C.c_double(), C.c_double()
+    rc = lib.o64_grdqh2d(_pf(vis), _pd(wt), _pf(u), _pf(v), _pf(w), len(u), uscale, uoffset, vscale, voffset,
+                         uorigin, vorigin, _pd(g), int(psf), nu, nv, _pd(np.ascontiguousarray(shift, np.float64)),
+                         C.byref(sw), C.byref(ls))
+    return g, sw.value, ls.value, rc
+
+
+def visselpi(wt, time, u, v, w, timerang, uvlimits):
+    """VISSELPI restated (vis/visselpi.f:51-72) -> (WT with the deselected
+    points negated, NSEL).  The input array is not changed."""
+    wt = f32(wt).copy()
+    time, u, v, w = f32(time), f32(u), f32(v), f32(w)
+    tr = f32(np.asarray(timerang, np.float32).reshape(-1))
+    uvl = f32(np.asarray(uvlimits, np.float32).reshape(-1))
+    nsel = C.c_int()
+    lib.r32_visselpi(_pf(wt), len(wt), _pf(time), _pf(u), _pf(v), _pf(w), _pf(tr), _pf(uvl), _pf(wt),
+                     C.byref(nsel))
+    return wt, nsel.value
 
 
 def grddeh2d(wt, u, v, w, uscale, vscale, uorigin, vorigin, grid, cf, supp, osamp, shift,

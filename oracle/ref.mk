@@ -25,7 +25,8 @@ SRC = $(REF)/grd/grdh2d.f $(REF)/grd/grddeh2d.f $(REF)/grd/grdqh2d.f $(REF)/grd/
       $(REF)/grd/grdsf.f $(REF)/grd/grdsze.f $(REF)/grd/grdccf.f $(REF)/grd/grduwt2d.f \
       $(REF)/grd/grdruw2d.f $(REF)/pix/pixcor2d.f $(REF)/pix/pixcorx2.f $(REF)/fft/fftr2d.f \
       $(REF)/fft/fftr1d2.f $(REF)/fft/fftr1d.f $(REF)/fft/fftr.f $(REF)/fft/fftx1d.f \
-      $(REF)/fft/fftx.f $(REF)/fft/fftpwr2.f $(REF)/fft/fftrnp2.f $(REF)/fft/fftxnp2.f
+      $(REF)/fft/fftx.f $(REF)/fft/fftpwr2.f $(REF)/fft/fftrnp2.f $(REF)/fft/fftxnp2.f \
+      $(REF)/vis/visselpi.f
 
 all: $(OUT)/libsderef.so $(OUT)/libsderef_O0.so
 

@@ -1302,3 +1302,40 @@ int r32_grdqh23d(const float *vis, const float *wt, const float *u, const float 
         }
     return 0;
 }
+
+/* ---- selection (vis/visselpi.f:51-72) ----
+ * A point with WT > 0 outside the time range [TMIN, TMAX] (the whole line
+ * when TIMERANG is (0, 0)) or whose uv radius squared U**2+V**2 (REAL, no
+ * w) lies outside [UVLIMITS(1)**2, UVLIMITS(2)**2] gets WT = SIGN(WT, -1.0);
+ * NSEL counts the rest.  The routine writes WT in place and never NEWWT
+ * (its caller VISSEL passes the same array for both, vis/vissel.f). */
+void r32_visselpi(float *wt, int nvis, const float *time, const float *u, const float *v,
+                  const float *w, const float *timerang, const float *uvlimits, float *newwt,
+                  int *nsel) {
+    float tmin, tmax;
+    (void)w;
+    (void)newwt;
+    *nsel = 0;
+    if (timerang[0] == 0.0f && timerang[1] == 0.0f) {
+        tmin = -1e20f;
+        tmax = 1e20f;
+    } else {
+        tmin = timerang[0];
+        tmax = timerang[1];
+    }
+    const float rsqmin = uvlimits[0] * uvlimits[0];
+    const float rsqmax = uvlimits[1] * uvlimits[1];
+    for (int i = 0; i < nvis; i++) {
+        if (wt[i] <= 0.0f) continue;
+        if (time[i] < tmin || time[i] > tmax) {
+            wt[i] = -fabsf(wt[i]);
+            continue;
+        }
+        const float rsq = u[i] * u[i] + v[i] * v[i];
+        if (rsq < rsqmin || rsq > rsqmax) {
+            wt[i] = -fabsf(wt[i]);
+            continue;
+        }
+        *nsel = *nsel + 1;
+    }
+}

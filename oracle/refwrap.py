@@ -116,6 +116,19 @@ def grdqh2d(vis, wt, u, v, w, uscale, vscale, uorigin, vorigin, psf, nu, nv, shi
     return g, sw.value
 
 
+def visselpi(wt, time, u, v, w, timerang, uvlimits, variant=""):
+    """VISSELPI (vis/visselpi.f:7-8) as VISSEL calls it (vis/vissel.f), with
+    WT passed for both WT and NEWWT -> (WT, NSEL).  The input is not changed."""
+    wt = _f32(wt).copy()
+    time, u, v, w = _f32(time), _f32(u), _f32(v), _f32(w)
+    tr = _f32(np.asarray(timerang, np.float32).reshape(-1))
+    uvl = _f32(np.asarray(uvlimits, np.float32).reshape(-1))
+    nsel = C.c_int(-1)
+    lib(variant).visselpi_(_p(wt), _i(len(wt)), _p(time), _p(u), _p(v), _p(w), _p(tr), _p(uvl), _p(wt),
+                           C.byref(nsel))
+    return wt, nsel.value
+
+
 def grddeh2d(wt, u, v, w, uscale, vscale, uorigin, vorigin, grid, cf, supp, osamp, shift, uoffset=0.0,
              voffset=0.0, variant=""):
     """GRDDEH2D (grd/grddeh2d.f:7-9) -> (VIS complex64, WT with its sign flag)."""

@@ -364,13 +364,14 @@ using MarkE = std::function<void()>;
 // outputs), the radix sort, tile ranges and work items.
 int run_sort(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v, const float *w,
              int64_t nvis, const CoordParams &cp, const TileParams &tp, SortBufs &sb, const DensityOut *dens,
-             const DegridOut *deg, const MarkB &mb, const MarkE &me, bool bounds = true) {
+             const DegridOut *deg, const MarkB &mb, const MarkE &me, bool bounds = true,
+             const SelParams *sel = nullptr) {
     // every point needs its cold staging record when the weights need the
     // density cell (density scale not the gridding scale) or the AZ-NORM
     // factor, or the prediction needs the shift rotation
     sb.cold_all = cp.doshift || (dens && (dens->az.on || !dens->gwt));
     mb("prep");
-    launch_prep(s, vis, wt, u, v, w, nvis, cp, tp, sb, dens, deg);
+    launch_prep(s, vis, wt, u, v, w, nvis, cp, tp, sb, dens, deg, sel);
     me();
     mb("cell_sort");
     const int rc = launch_sort(s, sb, nvis, tp, bounds);
@@ -714,27 +715,49 @@ void grdh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, flo
     if (sums[1] == 0.0) report_error(kErrBdArg, R, "No weight!");
 }
 
+// GRDQH2D: the inputs copied to the device, then the cell sort of the BOX
+// cells and the run sums per work item (deterministic), the fold of the
+// u-origin column, the shift by one column and the normalisation.
 void grdqh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, float *uscale, float *uoffset,
               float *vscale, float *voffset, int *uorigin, int *vorigin, float *gvis, int *psf, int *nu, int *nv,
               double *shift, float *sumwt) {
     if (sysi_.error) return;
     const char *R = "GRDQH2D";
-    if (!check_kernel_args(R, *nvis, *nu, *nv, 0, 0, 1, 1)) return;
+    if (!check_kernel_args(R, *nvis, *nu, *nv, 0, 0, 1, 1, *uorigin, *vorigin)) return;
     std::lock_guard<std::mutex> lk(g_ctx_mu);
     Ctx &c = g_ctx;
     if (ctx_init(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
     hipStream_t s = c.stream;
     const int64_t n = *nvis;
     const size_t nn = (size_t)(n > 0 ? n : 1);
-    const CoordParams cp = make_cp(shift, *uscale, *uoffset, *vscale, *voffset, *uorigin, *vorigin, *nu, *nv, 0, 0,
+    // SUPPU = 1 carries GRDQH2D's one-column shift and u-origin fold into the reduce
+    const CoordParams cp = make_cp(shift, *uscale, *uoffset, *vscale, *voffset, *uorigin, *vorigin, *nu, *nv, 1, 0,
                                    1, 1);
+    const TileParams tp = make_tp(*nu, *nv, default_tile(*nu, *nv), default_chunk(n));
+    const int chmask = *psf ? 2 : 1;
+    SortBufs sb;
+    sb.max_items = max_items_for(tp, n);
+    sb.temp_bytes = sort_temp_bytes(n > 0 ? n : 1, sort_nbits(tp));
     float2 *dvis = ctx_buf<float2>(c, "vis", nn);
     float *dwt = ctx_buf<float>(c, "wt", nn), *du = ctx_buf<float>(c, "u", nn);
     float *dv = ctx_buf<float>(c, "v", nn), *dw = ctx_buf<float>(c, "w", nn);
-    double *acc = ctx_buf<double>(c, "boxacc", 2 * (size_t)*nu * *nv);
-    double *sums = ctx_buf<double>(c, "sums", 4);
+    sb.keys = ctx_buf<uint32_t>(c, "keys", nn);
+    sb.skeys = ctx_buf<uint32_t>(c, "skeys", nn);
+    sb.vals = ctx_buf<uint64_t>(c, "vals", nn);
+    sb.svals = ctx_buf<uint64_t>(c, "svals", nn);
+    sb.hot = ctx_buf<StageHot>(c, "stage_hot", nn);
+    sb.temp = ctx_buf<char>(c, "sort_temp", sb.temp_bytes > 0 ? sb.temp_bytes : 1);
+    sb.tile_start = ctx_buf<int>(c, "tile_start", tp.ntiles + 1);
+    sb.item_start = ctx_buf<int>(c, "item_start", tp.ntiles + 1);
+    sb.nitems = ctx_buf<int>(c, "nitems", 1);
+    sb.items = ctx_buf<WorkItem>(c, "items", sb.max_items);
+    float *partial = ctx_buf<float>(c, "partial", grid_partial_floats(tp, chmask, sb.max_items));
+    double *blk = ctx_buf<double>(c, "blk", 2 * (size_t)sb.max_items), *dsums = ctx_buf<double>(c, "sums", 4);
+    double *pblk = ctx_buf<double>(c, "prep_blk", 2 * (size_t)prep_blocks(n > 0 ? n : 1));
     float2 *dg = ctx_buf<float2>(c, "grid", (size_t)*nu * *nv);
-    if (!dvis || !dwt || !du || !dv || !dw || !acc || !sums || !dg) {
+    if (!dvis || !dwt || !du || !dv || !dw || !sb.keys || !sb.skeys || !sb.vals || !sb.svals || !sb.hot ||
+        !sb.temp || sb.temp_bytes == 0 || !sb.tile_start || !sb.item_start || !sb.nitems || !sb.items || !partial ||
+        !blk || !dsums || !pblk || !dg) {
         report_error(kErrNoMem, R, "device allocation failed");
         return;
     }
@@ -745,17 +768,67 @@ void grdqh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, fl
         SDE_F(hipMemcpyAsync(dv, v, sizeof(float) * n, hipMemcpyHostToDevice, s));
         SDE_F(hipMemcpyAsync(dw, w, sizeof(float) * n, hipMemcpyHostToDevice, s));
     }
-    SDE_F(hipMemsetAsync(acc, 0, sizeof(double) * 2 * (size_t)*nu * *nv, s));
-    SDE_F(hipMemsetAsync(sums, 0, sizeof(double) * 4, s));
-    launch_box_grid(s, dvis, dwt, du, dv, dw, n, cp, *psf, acc, sums);
-    launch_box_fold(s, acc, cp, sums, dg);
+    launch_prep_box(s, dvis, dwt, du, dv, dw, n, cp, tp, sb, pblk);
+    if (launch_sort(s, sb, n, tp, true) || launch_box_tiles(s, sb, tp, chmask, blk, partial)) {
+        report_error(kErrFatal, R, "cell sort failed");
+        return;
+    }
+    launch_box_totals(s, pblk, n > 0 ? prep_blocks(n) : 0, blk, sb.max_items, cp, dsums, nullptr);
+    launch_grid_reduce(s, partial, sb.item_start, tp, chmask, cp, dsums, chmask == 1 ? dg : nullptr,
+                       chmask == 2 ? dg : nullptr);
     SDE_F(hipGetLastError());
     double hs[2];
     SDE_F(hipMemcpyAsync(gvis, dg, sizeof(float2) * (size_t)*nu * *nv, hipMemcpyDeviceToHost, s));
-    SDE_F(hipMemcpyAsync(hs, sums, sizeof hs, hipMemcpyDeviceToHost, s));
+    SDE_F(hipMemcpyAsync(hs, dsums, sizeof hs, hipMemcpyDeviceToHost, s));
     SDE_F(hipStreamSynchronize(s));
     *sumwt = (float)hs[0];
     if (hs[1] == 0.0) report_error(kErrBdArg, R, "No weight!");
+}
+
+// VISSELPI (vis/visselpi.f:51-72) on a device copy: WT negated in place for
+// the deselected points and copied back; NEWWT is not written, as in the
+// reference (its caller VISSEL passes WT for both, vis/vissel.f).
+void visselpi_(float *wt, int *nvis, float *time, float *u, float *v, float *w, float *timerang, float *uvlimits,
+               float *newwt, int *nsel) {
+    (void)w;
+    (void)newwt;
+    if (sysi_.error) return;
+    const char *R = "VISSELPI";
+    if (*nvis < 0) {
+        report_error(kErrBdArg, R, "bad NVIS");
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    Ctx &c = g_ctx;
+    if (ctx_init(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
+    hipStream_t s = c.stream;
+    const int64_t n = *nvis;
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    float *dwt = ctx_buf<float>(c, "wt", nn), *dt = ctx_buf<float>(c, "time", nn);
+    float *du = ctx_buf<float>(c, "u", nn), *dv = ctx_buf<float>(c, "v", nn);
+    unsigned long long *cnt = ctx_buf<unsigned long long>(c, "counters", 2);
+    if (!dwt || !dt || !du || !dv || !cnt) {
+        report_error(kErrNoMem, R, "device allocation failed");
+        return;
+    }
+    SelParams sp = make_sel_params(timerang, uvlimits);
+    sp.time = dt;
+    sp.wt = dwt;
+    sp.nsel = cnt;
+    SDE_F(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), s));
+    if (n > 0) {
+        SDE_F(hipMemcpyAsync(dwt, wt, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(dt, time, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(du, u, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        SDE_F(hipMemcpyAsync(dv, v, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        launch_visselpi(s, sp, du, dv, n);
+        SDE_F(hipGetLastError());
+        SDE_F(hipMemcpyAsync(wt, dwt, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    }
+    unsigned long long ns = 0;
+    SDE_F(hipMemcpyAsync(&ns, cnt, sizeof ns, hipMemcpyDeviceToHost, s));
+    SDE_F(hipStreamSynchronize(s));
+    *nsel = (int)ns;
 }
 
 void grddeh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, float *uscale, float *uoffset,
@@ -1178,6 +1251,11 @@ struct sde_plan {
     CoordParams cp, wcp;
     TileParams tp;
     int wmode = 0;
+    bool box = false;        // BOX convolution: GRDQH2D's nearest-cell gridding (VISGRID's dispatch)
+    bool selected = false;   // the last uvmap ran the selection (count in d_cnt[3])
+    float *d_time = nullptr; // TIME of the resident visibilities (selection)
+    int64_t time_n = -1;     // visibilities TIME was set for (-1: none)
+    size_t time_cap = 0;
     std::map<int, float2 *> tw;
     float *d_cf = nullptr, *d_ccf = nullptr;
     int64_t nvis = 0, cap = 0;
@@ -1278,6 +1356,7 @@ int plan_alloc(sde_plan *p, T **ptr, size_t count) {
 }
 
 int plan_reserve_vis(sde_plan *p, int64_t n) {
+    p->time_n = -1;  // TIME belongs to the visibilities handed over before
     p->filled = false;
     p->weighted = false;
     p->sorted_ok = false;
@@ -1290,6 +1369,11 @@ int plan_reserve_vis(sde_plan *p, int64_t n) {
         plan_alloc(p, &p->sb.vals, c) || plan_alloc(p, &p->sb.svals, c) || plan_alloc(p, &p->sb.hot, c) || plan_alloc(p, &p->sb.cold, c) || plan_alloc(p, &p->d_recd, c) ||
         plan_alloc(p, &p->d_list, (size_t)list_capacity(n)))
         return -1;
+    // the degridder's output-bin slots (one per sorted record)
+    p->slot_cap = 0;
+    p->slot_gen = 0;
+    if (plan_alloc(p, &p->d_slot, c)) return -1;
+    p->slot_cap = c;
     const size_t tb = sort_temp_bytes(n, sort_nbits(p->tp));
     if (tb == 0) {
         set_plan_error("radix sort size query failed");
@@ -1320,7 +1404,7 @@ int plan_reserve_items(sde_plan *p) {
 
 // Cell sort of the resident visibilities; the prep pass also writes the
 // per-point degridding outputs into the model arrays.
-int plan_sort(sde_plan *p, const DensityOut *dens, bool bounds = true) {
+int plan_sort(sde_plan *p, const DensityOut *dens, bool bounds = true, const SelParams *sel = nullptr) {
     p->sorted_ok = false;
     p->sort_gen++;
     auto mb = [p](const char *nm) { plan_mark_begin(p, nm); };
@@ -1332,7 +1416,23 @@ int plan_sort(sde_plan *p, const DensityOut *dens, bool bounds = true) {
     deg.nlist = p->d_nlist;
     p->filled = false;
     return run_sort(p->stream, p->d_vis, p->d_wt, p->d_u, p->d_v, p->d_w, p->nvis, p->cp, p->tp, p->sb, dens, &deg,
-                    mb, me, bounds);
+                    mb, me, bounds, sel);
+}
+
+// The selection of the uvmap task (task/uvmap.f:158-168): VISSEL when the
+// visibilities have TIME, else the task's warning and no selection.  The
+// count of kept points goes to d_cnt[3].
+bool plan_selection(sde_plan *p, SelParams *sp) {
+    if (!p->cfg.select) return false;
+    if (p->time_n != p->nvis || !p->d_time) {
+        report_warning("No TIME information");
+        return false;
+    }
+    *sp = make_sel_params(p->cfg.timerange, p->cfg.uvlimits);
+    sp->time = p->d_time;
+    sp->wt = p->d_wt;
+    sp->nsel = p->d_cnt + 3;
+    return true;
 }
 
 // uv cell and coordinate parameters of an image cell (dx, dy in degrees,
@@ -1377,13 +1477,8 @@ int plan_vismodel(sde_plan *p, const float *img) {
     SDE_CHECK(hipEventRecord(p->ev_join, p->side));
     // the sort's unsorted keys and values (dead until the next prep pass) hold the output bins
     DegridBins bins = degrid_bins_for(p->nvis, reinterpret_cast<float2 *>(p->sb.vals), p->sb.keys, p->d_dbins);
-    // the slots of the first prediction on a cell sort serve the later ones (a major cycle)
-    if ((size_t)p->nvis > p->slot_cap) {
-        p->slot_cap = 0;
-        p->slot_gen = 0;
-        if (plan_alloc(p, &p->d_slot, (size_t)p->nvis)) return -1;
-        p->slot_cap = (size_t)p->nvis;
-    }
+    // the slots of the first prediction on a cell sort serve the later ones (a
+    // major cycle); plan_reserve_vis holds room for every resident visibility
     bins.slot = p->d_slot;
     bins.reuse = p->slot_gen != 0 && p->slot_gen == p->sort_gen;
     plan_mark_begin(p, "degrid");
@@ -1403,6 +1498,80 @@ int plan_vismodel(sde_plan *p, const float *img) {
 }
 
 int plan_image(sde_plan *p, const WeightParams &wp);
+int plan_transform(sde_plan *p);
+
+// The density weights of the plan in the caller's order, WT -> d_wteff
+// (GRDUWT2D / GRDRUW2D as the drop-ins run them); points off the density
+// grid are counted in d_cnt[0].
+int plan_weights_caller_order(sde_plan *p) {
+    hipStream_t s = p->stream;
+    const int64_t n = p->nvis;
+    const size_t ng = (size_t)p->nu * p->nv;
+    const int nblk = wt_blocks(n);
+    SDE_CHECK(hipMemsetAsync(p->d_gwt, 0, sizeof(double) * ng, s));
+    SDE_CHECK(hipMemcpyAsync(p->d_wteff, p->d_wt, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+    plan_mark_begin(p, "wt_density");
+    launch_wt_density(s, p->d_wt, p->d_u, p->d_v, p->d_w, n, p->wcp, p->d_gwt, p->d_wblk, nblk);
+    plan_mark_end(p);
+    const float rob = p->wmode == 3 ? (p->cfg.robust[0] + p->cfg.robust[1]) / 2.0f : p->cfg.robust[0];
+    const float t = 5.0f * powf(10.0f, -rob);
+    plan_mark_begin(p, "wt_finish");
+    launch_wt_finish(s, p->d_gwt, (int)ng, p->d_wblk, nblk, p->wmode, t * t, rob * rob,
+                     2.0f * (p->cfg.ds * p->cfg.ds), p->d_wsq, p->d_wsums);
+    launch_wt_apply(s, p->d_wt, p->d_u, p->d_v, p->d_w, n, p->wcp, p->d_gwt, p->d_wsums, p->wmode == 3,
+                    p->cfg.robust, 4.0f * atanf(1.0f), p->d_wteff, p->d_cnt);
+    plan_mark_end(p);
+    SDE_CHECK(hipGetLastError());
+    return 0;
+}
+
+// uvmap with the BOX convolution (VISGRID's dispatch to GRDQH2D,
+// vis/visgrid.f:154-158): selection, weights in the caller's order, the
+// cell sort of the BOX cells, the run sums per work item, the fold and
+// shift by one column, then the transforms of every uvmap.  A residual
+// uvmap repeats the whole sequence (the selection is idempotent and the
+// weights come out the same).
+int plan_uvmap_box(sde_plan *p) {
+    hipStream_t s = p->stream;
+    const int64_t n = p->nvis;
+    const size_t ng = (size_t)p->nu * p->nv;
+    SDE_CHECK(hipMemsetAsync(p->d_cnt, 0, sizeof(unsigned long long) * 4, s));
+    SelParams sp;
+    p->selected = plan_selection(p, &sp);
+    if (p->selected) {
+        plan_mark_begin(p, "select");
+        launch_visselpi(s, sp, p->d_u, p->d_v, n);
+        plan_mark_end(p);
+    }
+    const bool weighting = p->cfg.weighting != 0;
+    if (weighting && plan_weights_caller_order(p)) return -1;
+    const float *wt = weighting ? p->d_wteff : p->d_wt;
+    // the BOX sort replaces the cell sort: a prediction sorts afresh
+    p->sorted_ok = false;
+    p->filled = false;
+    p->sort_gen++;
+    plan_mark_begin(p, "prep");
+    launch_prep_box(s, p->d_vis, wt, p->d_u, p->d_v, p->d_w, n, p->cp, p->tp, p->sb, p->d_wblk);
+    plan_mark_end(p);
+    plan_mark_begin(p, "cell_sort");
+    const int rc = launch_sort(s, p->sb, n, p->tp, true);
+    plan_mark_end(p);
+    plan_mark_begin(p, "grid_tiles");
+    const int rc2 = rc ? rc : launch_box_tiles(s, p->sb, p->tp, 3, p->d_isums, p->d_partial);
+    launch_box_totals(s, p->d_wblk, prep_blocks(n), p->d_isums, p->sb.max_items, p->cp, p->d_sums, p->d_peak);
+    plan_mark_end(p);
+    if (rc || rc2) {
+        set_plan_error("BOX gridding failed (cell sort or tile %d)", p->tp.T);
+        return -1;
+    }
+    plan_mark_begin(p, "grid_reduce");
+    launch_grid_reduce(s, p->d_partial, p->sb.item_start, p->tp, 3, p->cp, p->d_sums, p->d_grids, p->d_grids + ng);
+    plan_mark_end(p);
+    SDE_CHECK(hipGetLastError());
+    p->weighted = weighting;
+    p->imaged = true;
+    return plan_transform(p);
+}
 
 // uvmap; with reuse (residual imaging) the cell sort, work items, staged
 // coordinate parts and density weights of the last uvmap stand and only the
@@ -1413,6 +1582,7 @@ int plan_uvmap(sde_plan *p, bool reuse) {
         set_plan_error("sde_plan_uvmap: no visibilities");
         return -1;
     }
+    if (p->box) return plan_uvmap_box(p);
     hipStream_t s = p->stream;
     const int64_t n = p->nvis;
     const size_t ng = (size_t)p->nu * p->nv;
@@ -1430,9 +1600,13 @@ int plan_uvmap(sde_plan *p, bool reuse) {
             wp = make_weight_params(p->wcp, p->wmode == 3, p->cfg.robust, 4.0f * atanf(1.0f), p->d_gwt, p->d_wsums);
         return plan_image(p, wp);
     }
-    // the counters of the weighting (points off the density grid); natural
-    // weights touch none of them, and the plan zeroes them at creation
+    // the counters of the weighting (points off the density grid) and of the
+    // selection; natural weights touch none of the first, and the plan
+    // zeroes them at creation
+    SelParams sp;
+    p->selected = plan_selection(p, &sp);
     if (weighting) SDE_CHECK(hipMemsetAsync(p->d_cnt, 0, sizeof(unsigned long long) * 4, s));
+    else if (p->selected) SDE_CHECK(hipMemsetAsync(p->d_cnt + 3, 0, sizeof(unsigned long long), s));
     // every imaging job sorts its visibilities afresh (the sort is part of the job)
     // the density grid comes from the sorted runs when its cell is the
     // gridding cell (WTFOV = 1: the same scales); else from atomics
@@ -1447,7 +1621,7 @@ int plan_uvmap(sde_plan *p, bool reuse) {
         dens.ndrop = p->d_cnt;
     }
     // with the density from the sorted runs, that pass also writes the tile ranges
-    if (plan_sort(p, weighting ? &dens : nullptr, !runs)) return -1;
+    if (plan_sort(p, weighting ? &dens : nullptr, !runs, p->selected ? &sp : nullptr)) return -1;
     if (weighting) {
         int nwblk;
         if (runs) {
@@ -1492,6 +1666,14 @@ int plan_image(sde_plan *p, const WeightParams &wp) {
         return -1;
     p->filled = true;
     p->imaged = true;
+    return plan_transform(p);
+}
+
+// The column and row transforms of the gridded PSF and dirty planes,
+// correction, and division by the PSF peak (task/uvmap.f:424-491).
+int plan_transform(sde_plan *p) {
+    hipStream_t s = p->stream;
+    const size_t ng = (size_t)p->nu * p->nv;
     float2 *twnv = p->tw[p->nv], *twfnu = p->tw[p->fnu];
     plan_mark_begin(p, "fft_cols");
     // out of place: the gridded planes stay readable (sde_plan_get_grids)
@@ -1530,11 +1712,12 @@ int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **out) {
     p->cfg = *cfg;
     std::string cft = fstr(cfg->cftype, strnlen(cfg->cftype, sizeof cfg->cftype));
     float cf[1 + 2 * 4 * 128];
-    if (fstarts(cft, "BOX") || make_cf(cft, cf, &p->supp, &p->osamp)) {
-        set_plan_error("sde_plan_create: convolution function must be SF or SZE");
+    if (make_cf(cft, cf, &p->supp, &p->osamp)) {
+        set_plan_error("sde_plan_create: convolution function must be SF, SZE or BOX");
         delete p;
         return -1;
     }
+    p->box = fstarts(cft, "BOX");
     p->ncf = 1 + 2 * (p->supp + 1) * p->osamp;
     const float dx = -cfg->cellsize / 3600.0f, dy = cfg->cellsize / 3600.0f;
     p->fnu = pow2_at_least(cfg->nx);
@@ -1618,7 +1801,8 @@ int sde_plan_destroy(sde_plan *p) {
                     p->sb.keys, p->sb.skeys, p->sb.vals, p->sb.svals, p->sb.hot, p->sb.cold, p->sb.temp, p->sb.tile_start,
                     p->sb.item_start, p->sb.nitems, p->sb.items, p->d_recd, p->d_list, p->d_nlist,
                     p->d_partial, p->d_isums, p->d_sums, p->d_grids, p->d_gwork, p->d_mgrid, p->d_dirty, p->d_psf, p->d_img_in,
-                    p->d_peak, p->d_gwt, p->d_wblk, p->d_wsums, p->d_cnt, p->d_wsq, p->d_dbins, p->d_slot};
+                    p->d_peak, p->d_gwt, p->d_wblk, p->d_wsums, p->d_cnt, p->d_wsq, p->d_dbins, p->d_slot,
+                    p->d_time};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &kv : p->tw) (void)hipFree(kv.second);
@@ -1651,6 +1835,22 @@ int sde_plan_set_vis(sde_plan *p, int64_t nvis, const float *vis, const float *w
     if (p->cfg.chunk <= 0) p->tp.chunk = default_chunk(nvis);
     if (plan_reserve_items(p)) return -1;
     if (on_device != 2) SDE_CHECK(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+int sde_plan_set_time(sde_plan *p, const float *time, int on_device) {
+    if (!p || (!time && p->nvis > 0)) return -1;
+    if (p->nvis > 0) {
+        if ((size_t)p->nvis > p->time_cap) {
+            p->time_cap = 0;
+            if (plan_alloc(p, &p->d_time, (size_t)p->nvis)) return -1;
+            p->time_cap = (size_t)p->nvis;
+        }
+        SDE_CHECK(hipMemcpyAsync(p->d_time, time, sizeof(float) * p->nvis,
+                                 on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, p->stream));
+        SDE_CHECK(hipStreamSynchronize(p->stream));
+    }
+    p->time_n = p->nvis;
     return 0;
 }
 
@@ -1772,7 +1972,7 @@ int64_t sde_plan_estimate_bytes(const sde_imaging_cfg *cfg, int64_t nvis) {
     if (nvis > 0) {
         const size_t n = (size_t)nvis;
         b += n * (sizeof(float2) * 2 + sizeof(float) * 6 + sizeof(uint32_t) * 2 + sizeof(uint64_t) * 2 +
-                  sizeof(StageHot) + sizeof(StageCold) + sizeof(DegridRec));
+                  sizeof(StageHot) + sizeof(StageCold) + sizeof(DegridRec) + sizeof(int));
         b += sizeof(int) * (size_t)list_capacity(nvis) + sort_temp_bytes(nvis, sort_nbits(tp));
     }
     const size_t mi = (size_t)max_items_for(tp, nvis);
@@ -2049,7 +2249,7 @@ int sde_plan_info(sde_plan *p, sde_run_info *info) {
     if (!p || !info) return -1;
     double sums[3] = {0, 0, 0};
     int key = 0, nitems = 0;
-    unsigned long long cnt[2] = {0, 0};
+    unsigned long long cnt[4] = {0, 0, 0, 0};
     SDE_CHECK(hipMemcpyAsync(sums, p->d_sums, sizeof sums, hipMemcpyDeviceToHost, p->stream));
     SDE_CHECK(hipMemcpyAsync(&key, p->d_peak, sizeof key, hipMemcpyDeviceToHost, p->stream));
     SDE_CHECK(hipMemcpyAsync(&nitems, p->sb.nitems, sizeof nitems, hipMemcpyDeviceToHost, p->stream));
@@ -2069,6 +2269,7 @@ int sde_plan_info(sde_plan *p, sde_run_info *info) {
     info->ndropped = (int64_t)cnt[0];
     info->nbad = (int64_t)cnt[1];
     info->ngridded = nsorted;
+    info->nselected = p->selected ? (int64_t)cnt[3] : -1;
     return 0;
 }
 

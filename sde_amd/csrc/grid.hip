@@ -115,16 +115,20 @@ __device__ __forceinline__ int cell_key(const TileParams &tp, int cu, int cv) {
 //   DEG: GRDDEH2D's per-point outputs (grd/grddeh2d.f:88-118) -- WT, VIS of
 //        flagged and off-grid points -- and the list of points the tile
 //        path cannot take.
+// With sel.on, VISSELPI's selection comes first (vis/visselpi.f:58-70): a
+// deselected point's WT is negated in the weights (sel.wt, the array wt
+// reads) and the point is flagged for everything after.
 template <int DM, bool DEG>
 __global__ __launch_bounds__(kThreads) void k_prep(
-    const float2 *__restrict__ vis, const float *__restrict__ wt, const float *__restrict__ u,
+    const float2 *__restrict__ vis, const float *wt, const float *__restrict__ u,
     const float *__restrict__ v, const float *__restrict__ w, int64_t n, CoordParams cp, TileParams tp,
     uint32_t sentinel, uint32_t *__restrict__ keys, uint64_t *__restrict__ vals, StageHot *__restrict__ hot,
     StageCold *__restrict__ cold, int cold_all, CoordParams wcp, AzParams az,
     double *__restrict__ gwt, double *__restrict__ dblk, unsigned long long *__restrict__ ndrop,
     float2 *__restrict__ vis_out, float *__restrict__ wt_out, int *__restrict__ list, int *__restrict__ nlist,
-    int seg_cap) {
+    int seg_cap, SelParams sel) {
     double dsum = 0.0, ndr = 0.0;
+    unsigned nkept = 0;
     __shared__ int s_nlist;
     if (DEG) {
         if (threadIdx.x == 0) s_nlist = 0;
@@ -138,8 +142,16 @@ __global__ __launch_bounds__(kThreads) void k_prep(
         const int64_t i = base + threadIdx.x;
         bool to_list = false;
         if (i < n) {
-            const float wti = wt[i];
+            float wti = wt[i];
             const float uf = u[i], vf = v[i], wf = w[i];
+            if (sel.on && wti > 0.0f) {
+                if (vissel_out(sel, sel.time[i], uf, vf)) {
+                    wti = -fabsf(wti);
+                    sel.wt[i] = wti;
+                } else {
+                    nkept++;
+                }
+            }
             const float2 vv = vis[i];
             StageHot sr;
             sr.gpk = 0u;
@@ -216,6 +228,10 @@ __global__ __launch_bounds__(kThreads) void k_prep(
     if (DEG) {
         __syncthreads();
         if (threadIdx.x == 0) nlist[blockIdx.x] = s_nlist;
+    }
+    if (sel.on) {
+        for (int d = 32; d > 0; d >>= 1) nkept += __shfl_down(nkept, d);
+        if ((threadIdx.x & 63) == 0 && nkept) atomicAdd(sel.nsel, (unsigned long long)nkept);
     }
     if (DM > 0) {
         block_sum2(dsum, ndr);
@@ -882,72 +898,6 @@ __global__ __launch_bounds__(256) void k_grid_reduce(
     if (chmask & 2) psf[o] = make_float2((float)pr, 0.0f);
 }
 
-// ---- BOX (nearest cell), GRDQH2D ----
-__global__ __launch_bounds__(kThreads) void k_box_grid(
-    const float2 *__restrict__ vis, const float *__restrict__ wt, const float *__restrict__ u,
-    const float *__restrict__ v, const float *__restrict__ w, int64_t nvis, CoordParams cp,
-    int psf, double *__restrict__ acc, double *__restrict__ blk) {
-    double sw = 0.0, ls = 0.0;
-    const float twopi = 8.0f * atanf(1.0f);
-    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < nvis;
-         i += (int64_t)gridDim.x * kThreads) {
-        const float wti = wt[i];
-        if (wti <= 0.0f) continue;
-        const float ul = (float)(cp.shift[0] * u[i] + cp.shift[1] * v[i] + cp.shift[2] * w[i]);
-        const float vl = (float)(cp.shift[3] * u[i] + cp.shift[4] * v[i] + cp.shift[5] * w[i]);
-        const float wl = (float)(cp.shift[6] * u[i] + cp.shift[7] * v[i] + cp.shift[8] * w[i]);
-        const float phase = twopi * (wl - w[i]);
-        const float cr = cosf(phase), ci = -sinf(phase);
-        const float2 vv = vis[i];
-        float fr = vv.x * cr - vv.y * ci, fi = vv.x * ci + vv.y * cr;
-        float ucell = cp.uscale * ul + cp.uoffset, vcell = cp.vscale * vl + cp.voffset;
-        sw += (double)wti;
-        if (ucell > 0.0f) {
-            fi = -fi;
-            ucell = ucell + (float)cp.uorigin;
-            vcell = vcell + (float)cp.vorigin;
-        } else {
-            ucell = -ucell + (float)cp.uorigin;
-            vcell = -vcell + (float)cp.vorigin;
-        }
-        if (psf) { fr = 1.0f; fi = 0.0f; }
-        const int ug = nint_f(ucell) + 1, vg = nint_f(vcell);
-        if (ug < 1 || ug > cp.nu || vg < 1 || vg > cp.nv) continue;
-        const size_t o = 2 * ((size_t)(vg - 1) * cp.nu + ug - 1);
-        atomicAdd(&acc[o], (double)(wti * fr));
-        atomicAdd(&acc[o + 1], (double)(wti * fi));
-        ls += (double)wti;
-    }
-    block_sum2(sw, ls);
-    if (threadIdx.x == 0) {
-        atomicAdd(&blk[0], sw);
-        atomicAdd(&blk[1], ls);
-    }
-}
-
-__global__ void k_box_fold(const double *__restrict__ acc, CoordParams cp,
-                           const double *__restrict__ sums, float2 *__restrict__ g) {
-    const int iu = blockIdx.x * blockDim.x + threadIdx.x;
-    const int iv = blockIdx.y;
-    if (iu >= cp.nu) return;
-    float2 out = make_float2(0.0f, 0.0f);
-    const double ls = sums[1];
-    if (iu + 1 <= cp.nu - 1 && ls != 0.0) {
-        const int col = iu + 2, row = iv + 1;  // 1-based pre-shift
-        const int uz = cp.uorigin + 1, vz = cp.vorigin;
-        double re = acc[2 * ((size_t)(row - 1) * cp.nu + col - 1)];
-        double im = acc[2 * ((size_t)(row - 1) * cp.nu + col - 1) + 1];
-        if (col == uz && row >= vz - (cp.nv / 2 - 1) && row <= vz + cp.nv / 2 - 1) {
-            const int mr = 2 * vz - row;
-            re += acc[2 * ((size_t)(mr - 1) * cp.nu + col - 1)];
-            im -= acc[2 * ((size_t)(mr - 1) * cp.nu + col - 1) + 1];
-        }
-        const double scale = (double)((float)(cp.nu - 1) * (float)cp.nv) / ls;
-        out = make_float2((float)(scale * re), (float)(scale * im));
-    }
-    g[(size_t)iv * cp.nu + iu] = out;
-}
-
 // ---- w planes (GRDH23D) ----
 // REAL <- DOUBLE PRECISION SHIFT times REAL coordinates: the products and
 // sums in DOUBLE, rounded once (grd/grdh23d.f:100-105).
@@ -1026,6 +976,203 @@ __global__ __launch_bounds__(kThreads) void k_prep_slowz(
         blk[2 * blockIdx.x] = sw;
         blk[2 * blockIdx.x + 1] = 0.0;
     }
+}
+
+// ---- BOX (nearest cell), GRDQH2D, on the cell sort ----
+// Prep pass, one thread per visibility (grd/grdqh2d.f:85-114): the REAL
+// coordinate step (ULOCAL, VLOCAL, WLOCAL rounded from DOUBLE products),
+// FVIS = VIS * CMPLX(COS(PHASE), -SIN(PHASE)), conjugated for u > 0, the cell
+// UGRID = NINT(UCELL)+1, VGRID = NINT(VCELL) and its bounds test.  The key
+// is the pre-shift cell (UGRID-1, VGRID-1); the hot record holds the local
+// cell and FVIS (the PSF's FVIS = 1 is applied by the tile pass).  SUMWT
+// (every point with WT > 0, dropped or not) per block.
+__global__ __launch_bounds__(kThreads) void k_prep_box(
+    const float2 *__restrict__ vis, const float *__restrict__ wt, const float *__restrict__ u,
+    const float *__restrict__ v, const float *__restrict__ w, int64_t n, CoordParams cp, TileParams tp,
+    uint32_t sentinel, uint32_t *__restrict__ keys, uint64_t *__restrict__ vals, StageHot *__restrict__ hot,
+    double *__restrict__ blk) {
+    const float twopi = 8.0f * atanf(1.0f);
+    double sw = 0.0, zero = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        const float wti = wt[i];
+        StageHot sr;
+        sr.gpk = 0u;
+        sr.fr = 0.0f;
+        sr.fi = 0.0f;
+        sr.dpk = 0u;
+        uint32_t key = sentinel;
+        if (wti > 0.0f) {
+            sw += (double)wti;
+            const float wf = w[i];
+            const float3 l = local_uvw(cp, u[i], v[i], wf);
+            const float phase = twopi * (l.z - wf);
+            const float cr = cosf(phase), ci = -sinf(phase);
+            const float2 vv = vis[i];
+            float fr = vv.x * cr - vv.y * ci, fi = vv.x * ci + vv.y * cr;
+            float ucell = cp.uscale * l.x + cp.uoffset, vcell = cp.vscale * l.y + cp.voffset;
+            if (ucell > 0.0f) {
+                fi = -fi;
+                ucell = ucell + (float)cp.uorigin;
+                vcell = vcell + (float)cp.vorigin;
+            } else {
+                ucell = -ucell + (float)cp.uorigin;
+                vcell = -vcell + (float)cp.vorigin;
+            }
+            const int ug = nint_f(ucell) + 1, vg = nint_f(vcell);
+            if (ug >= 1 && ug <= cp.nu && vg >= 1 && vg <= cp.nv) {
+                key = (uint32_t)cell_key(tp, ug - 1, vg - 1);
+                const uint32_t local = key & (uint32_t)(tp.T * tp.T - 1);
+                sr.gpk = (local % (uint32_t)tp.T) | ((local / (uint32_t)tp.T) << 8);
+                sr.fr = fr;
+                sr.fi = fi;
+            }
+        }
+        hot[i] = sr;
+        keys[i] = key;
+        vals[i] = (uint64_t)(uint32_t)i | ((uint64_t)__float_as_uint(wti) << 32);
+    }
+    block_sum2(sw, zero);
+    if (threadIdx.x == 0) blk[blockIdx.x] = sw;
+}
+
+// One work item (cell-sorted slots of one tile) of the BOX gridding: each of
+// the W waves takes a contiguous share of the slots, 64 per batch, and sums
+// the runs of equal cells with a segmented scan across the lanes in FP64;
+// the last lane of each run adds it into the wave's own LDS tile (no two
+// lanes of a batch share a cell, the batches of a wave follow in order).
+// The wave tiles are added in wave order and written as the item's partial
+// tile, in the layout of the convolutional gridder's partial tiles (padded
+// by kHalo, the margin left zero).  Every sum has a fixed order, so the
+// grid does not depend on scheduling.  Channels: dirty UVWT*FVIS
+// (grd/grdqh2d.f:124) and PSF UVWT; LSUMWT of the item to item_sums.
+template <int T, int CHM>
+__global__ __launch_bounds__(256) void k_box_tiles(const uint64_t *__restrict__ svals,
+                                                   const StageHot *__restrict__ hot,
+                                                   const WorkItem *__restrict__ items, const int *__restrict__ nitems,
+                                                   double *__restrict__ item_sums, float *__restrict__ partial) {
+    constexpr int W = 4, NC = T * T, P = T + 2 * kHalo, NCP = P * P;
+    constexpr int NCH = ((CHM & 1) ? 2 : 0) + ((CHM & 2) ? 1 : 0);
+    constexpr int PSF = (CHM & 1) ? 2 : 0;
+    __shared__ double S[W * NCH * NC];
+    __shared__ double s_red[W];
+    const int item = blockIdx.x;
+    if (item >= *nitems) {
+        if (threadIdx.x == 0) {
+            item_sums[2 * item] = 0.0;
+            item_sums[2 * item + 1] = 0.0;
+        }
+        return;
+    }
+    for (int k = threadIdx.x; k < W * NCH * NC; k += 256) S[k] = 0.0;
+    __syncthreads();
+    const WorkItem wi = items[item];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double *Sw = S + wave * NCH * NC;
+    const int len = wi.end - wi.begin;
+    const int r0 = wi.begin + (int)(((int64_t)len * wave) / W);
+    const int r1 = wi.begin + (int)(((int64_t)len * (wave + 1)) / W);
+    double ls = 0.0;
+    for (int b = r0; b < r1; b += 64) {
+        const int p = b + lane;
+        const bool valid = p < r1;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        int cell = -1;
+        if (valid) {
+            const uint64_t sv = svals[p];
+            const float wti = sval_wt(sv);
+            const StageHot h = hot[sval_index(sv)];
+            cell = (int)((h.gpk >> 8) & 0xffu) * T + (int)(h.gpk & 0xffu);
+            ls += (double)wti;
+            if (CHM & 1) {
+                a0 = (double)wti * (double)h.fr;
+                a1 = (double)wti * (double)h.fi;
+            }
+            if (CHM & 2) a2 = (double)wti;
+        }
+        const int prev = __shfl_up(cell, 1);
+        const bool head = lane == 0 || cell != prev;
+        const unsigned long long heads = __ballot(head);
+        const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+        const int seg = 63 - __clzll((long long)(heads & upto));
+        for (int d = 1; d < 64; d <<= 1) {
+            const double y0 = __shfl_up(a0, d), y1 = __shfl_up(a1, d), y2 = __shfl_up(a2, d);
+            if (lane - d >= seg) {
+                a0 += y0;
+                a1 += y1;
+                a2 += y2;
+            }
+        }
+        const int next = __shfl_down(cell, 1);
+        if (valid && (lane == 63 || next != cell)) {
+            if (CHM & 1) {
+                Sw[cell] += a0;
+                Sw[NC + cell] += a1;
+            }
+            if (CHM & 2) Sw[PSF * NC + cell] += a2;
+        }
+    }
+    for (int d = 32; d > 0; d >>= 1) ls += __shfl_down(ls, d);
+    if (lane == 0) s_red[wave] = ls;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < W; k++) t += s_red[k];
+        item_sums[2 * item] = 0.0;
+        item_sums[2 * item + 1] = t;
+    }
+    float *out = partial + (size_t)item * NCH * NCP;
+    for (int k = threadIdx.x; k < NCH * NCP; k += 256) {
+        const int ch = k / NCP, q = k - ch * NCP;
+        const int r = q / P - kHalo, c = q % P - kHalo;
+        double t = 0.0;
+        if (r >= 0 && r < T && c >= 0 && c < T)
+            for (int x = 0; x < W; x++) t += S[(x * NCH + ch) * NC + r * T + c];
+        out[k] = (float)t;
+    }
+}
+
+// SUMWT from the prep blocks, LSUMWT from the items and the normalisation
+// (NU-1)*NV/LSUMWT of GRDQH2D (grd/grdqh2d.f:135-141), in a fixed order.
+__global__ __launch_bounds__(1024) void k_box_totals(const double *__restrict__ blk, int nblk,
+                                                     const double *__restrict__ item_sums, int nitems, double num,
+                                                     double *__restrict__ sums, int *__restrict__ peak_reset) {
+    __shared__ double ds[1024], dl[1024];
+    const int t = threadIdx.x;
+    double a = 0.0, b = 0.0;
+    for (int k = t; k < nblk; k += 1024) a += blk[k];
+    for (int k = t; k < nitems; k += 1024) b += item_sums[2 * k + 1];
+    ds[t] = a;
+    dl[t] = b;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (t < s) {
+            ds[t] += ds[t + s];
+            dl[t] += dl[t + s];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        sums[0] = ds[0];
+        sums[1] = dl[0];
+        sums[2] = dl[0] != 0.0 ? num / dl[0] : 0.0;
+        if (peak_reset) *peak_reset = (int)0x80808080u;
+    }
+}
+
+// VISSELPI over device arrays (vis/visselpi.f:51-72), one thread per point.
+__global__ __launch_bounds__(kThreads) void k_visselpi(SelParams sp, const float *__restrict__ u,
+                                                       const float *__restrict__ v, int64_t n) {
+    unsigned kept = 0;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        const float wti = sp.wt[i];
+        if (!(wti > 0.0f)) continue;
+        if (vissel_out(sp, sp.time[i], u[i], v[i]))
+            sp.wt[i] = -fabsf(wti);
+        else
+            kept++;
+    }
+    for (int d = 32; d > 0; d >>= 1) kept += __shfl_down(kept, d);
+    if ((threadIdx.x & 63) == 0 && kept) atomicAdd(sp.nsel, (unsigned long long)kept);
 }
 
 // ---- BOX (nearest cell) w planes, GRDQH23D ----
@@ -1144,8 +1291,9 @@ int64_t list_capacity(int64_t n) { return (int64_t)list_seg_cap(n > 0 ? n : 1) *
 
 void launch_prep(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
                  const float *w, int64_t n, const CoordParams &cp, const TileParams &tp, SortBufs &b,
-                 const DensityOut *dens, const DegridOut *deg) {
+                 const DensityOut *dens, const DegridOut *deg, const SelParams *sel) {
     if (n <= 0) return;
+    const SelParams sp = sel ? *sel : SelParams();
     const int nb = prep_blocks(n);
     const uint32_t sentinel = (uint32_t)sort_nkeys(tp);
     CoordParams wcp = dens ? dens->wcp : cp;
@@ -1158,7 +1306,7 @@ void launch_prep(hipStream_t s, const float2 *vis, const float *wt, const float 
 #define SDE_PREP(DM, DG)                                                                                         \
     hipLaunchKernelGGL((k_prep<DM, DG>), dim3(nb), dim3(kThreads), 0, s, vis, wt, u, v, w, n, cp, tp, sentinel, \
                        b.keys, b.vals, b.hot, b.cold, b.cold_all, wcp, az, gwt, dblk, ndrop, vo, wo, list, nlist,    \
-                       list_seg_cap(n))
+                       list_seg_cap(n), sp)
     const int dm = dens ? (dens->gwt ? 2 : 1) : 0;
     if (deg) {
         if (dm == 2) SDE_PREP(2, true);
@@ -1311,13 +1459,55 @@ void launch_box_fold_slowz(hipStream_t s, const double *acc, const CoordParams &
     hipLaunchKernelGGL(k_box_fold_slowz, grid, dim3(256), 0, s, acc, cp, sums, gvis);
 }
 
-void launch_box_grid(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
-                     const float *w, int64_t nvis, const CoordParams &cp, int psf, double *acc, double *sums) {
-    hipLaunchKernelGGL(k_box_grid, dim3(blocks_for(nvis)), dim3(kThreads), 0, s, vis, wt, u, v, w, nvis, cp, psf,
-                       acc, sums);
+void launch_prep_box(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
+                     const float *w, int64_t n, const CoordParams &cp, const TileParams &tp, SortBufs &b,
+                     double *blk) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_prep_box, dim3(prep_blocks(n)), dim3(kThreads), 0, s, vis, wt, u, v, w, n, cp, tp,
+                       (uint32_t)sort_nkeys(tp), b.keys, b.vals, b.hot, blk);
 }
 
-void launch_box_fold(hipStream_t s, const double *acc, const CoordParams &cp, const double *sums, float2 *gvis) {
-    dim3 grid((cp.nu + 255) / 256, cp.nv);
-    hipLaunchKernelGGL(k_box_fold, grid, dim3(256), 0, s, acc, cp, sums, gvis);
+int launch_box_tiles(hipStream_t s, const SortBufs &b, const TileParams &tp, int chmask, double *item_sums,
+                     float *partial) {
+    if (b.max_items <= 0) return 0;
+#define SDE_BOX(TT, CH)                                                                                       \
+    if (tp.T == TT && chmask == CH) {                                                                         \
+        hipLaunchKernelGGL((k_box_tiles<TT, CH>), dim3(b.max_items), dim3(256), 0, s, b.svals, b.hot, b.items,  \
+                           b.nitems, item_sums, partial);                                                     \
+        return 0;                                                                                             \
+    }
+    SDE_BOX(16, 1)
+    SDE_BOX(16, 2)
+    SDE_BOX(16, 3)
+    SDE_BOX(32, 1)
+    SDE_BOX(32, 2)
+    SDE_BOX(32, 3)
+#undef SDE_BOX
+    return -1;
+}
+
+void launch_box_totals(hipStream_t s, const double *blk, int nblk, const double *item_sums, int nitems,
+                       const CoordParams &cp, double *sums, int *peak_reset) {
+    const double num = (double)((float)(cp.nu - 1) * (float)cp.nv);
+    hipLaunchKernelGGL(k_box_totals, dim3(1), dim3(1024), 0, s, blk, nblk, item_sums, nitems, num, sums, peak_reset);
+}
+
+SelParams make_sel_params(const float *timerang, const float *uvlimits) {
+    SelParams sp;
+    sp.on = 1;
+    if (timerang[0] == 0.0f && timerang[1] == 0.0f) {
+        sp.tmin = -1e20f;
+        sp.tmax = 1e20f;
+    } else {
+        sp.tmin = timerang[0];
+        sp.tmax = timerang[1];
+    }
+    sp.rsqmin = uvlimits[0] * uvlimits[0];
+    sp.rsqmax = uvlimits[1] * uvlimits[1];
+    return sp;
+}
+
+void launch_visselpi(hipStream_t s, const SelParams &sp, const float *u, const float *v, int64_t n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_visselpi, dim3(prep_blocks(n)), dim3(kThreads), 0, s, sp, u, v, n);
 }

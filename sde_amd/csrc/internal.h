@@ -200,12 +200,35 @@ int prep_blocks(int64_t n);
 int run_blocks(int64_t n);
 int gather_blocks(int64_t n);
 
+// Selection of VISSELPI (vis/visselpi.f:51-72, called by task/uvmap.f:162
+// through VISSEL before the weighting): a point with WT > 0 whose TIME lies
+// outside [tmin, tmax] (the whole line for TIMERANG = (0, 0)) or whose uv
+// radius squared U**2+V**2 (REAL; w is not used) lies outside
+// [UVLIMITS(1)**2, UVLIMITS(2)**2] gets WT = SIGN(WT, -1.0), written back in
+// place; nsel counts the others.
+struct SelParams {
+    int on = 0;
+    float tmin = 0.0f, tmax = 0.0f, rsqmin = 0.0f, rsqmax = 0.0f;
+    const float *time = nullptr;
+    float *wt = nullptr;                  // the weights, negated in place where deselected
+    unsigned long long *nsel = nullptr;   // selected points, added to
+};
+SelParams make_sel_params(const float *timerang, const float *uvlimits);
+__device__ __forceinline__ bool vissel_out(const SelParams &sp, float t, float u, float v) {
+    if (t < sp.tmin || t > sp.tmax) return true;
+    const float rsq = u * u + v * v;
+    return rsq < sp.rsqmin || rsq > sp.rsqmax;
+}
+// VISSELPI as a pass of its own over device arrays (the visselpi_ drop-in).
+void launch_visselpi(hipStream_t s, const SelParams &sp, const float *u, const float *v, int64_t n);
+
 // Keys and staging records; with dens the exception weights (and their
 // partial sums in dens->blk[0 .. prep_blocks)); with deg the per-point
-// degridding outputs.
+// degridding outputs; with sel the selection of VISSELPI first (the
+// weights written back where deselected).
 void launch_prep(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
                  const float *w, int64_t n, const CoordParams &cp, const TileParams &tp, SortBufs &b,
-                 const DensityOut *dens, const DegridOut *deg);
+                 const DensityOut *dens, const DegridOut *deg, const SelParams *sel = nullptr);
 // Radix sort; with bounds also the tile ranges and work items (else the
 // density runs write the ranges and launch_items follows them).
 int launch_sort(hipStream_t s, SortBufs &b, int64_t n, const TileParams &tp, bool bounds = true);
@@ -263,12 +286,24 @@ void launch_grid_reduce(hipStream_t s, const float *partial, const int *item_sta
                         int chmask, const CoordParams &cp, const double *sums, float2 *dirty, float2 *psf);
 size_t grid_partial_floats(const TileParams &tp, int chmask, int max_items);
 
-// ---- BOX gridding (grid.hip) ----
-void launch_box_grid(hipStream_t s, const float2 *vis, const float *wt, const float *u,
-                     const float *v, const float *w, int64_t nvis, const CoordParams &cp,
-                     int psf, double *acc, double *sums);
-void launch_box_fold(hipStream_t s, const double *acc, const CoordParams &cp,
-                     const double *sums, float2 *gvis);
+// ---- BOX gridding, GRDQH2D, on the cell sort (grid.hip) ----
+// Prep pass: GRDQH2D's REAL cell step (grd/grdqh2d.f:85-114); the key is
+// cell_key of the pre-shift cell (UGRID-1, VGRID-1), the hot record holds
+// the local cell and FVIS; SUMWT per block (every point with WT > 0) to
+// blk[0 .. prep_blocks(n)).
+void launch_prep_box(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
+                     const float *w, int64_t n, const CoordParams &cp, const TileParams &tp, SortBufs &b,
+                     double *blk);
+// One workgroup per work item: the runs of equal cells summed in FP64 in a
+// fixed order into the item's partial tile (the tile gridder's layout, so
+// launch_grid_reduce with SUPPU = 1 folds the u-origin column, shifts by
+// one column and normalises); LSUMWT per item to item_sums.
+int launch_box_tiles(hipStream_t s, const SortBufs &b, const TileParams &tp, int chmask, double *item_sums,
+                     float *partial);
+// sums[0] = SUMWT (prep blocks), sums[1] = LSUMWT (items), sums[2] =
+// (NU-1)*NV/LSUMWT; resets *peak_reset when given.
+void launch_box_totals(hipStream_t s, const double *blk, int nblk, const double *item_sums, int nitems,
+                       const CoordParams &cp, double *sums, int *peak_reset);
 
 // ---- FFT (fft.hip) ----
 // twiddle table: tw[k] = exp(+2 pi i k / n), k < n, as float2

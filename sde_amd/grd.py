@@ -118,6 +118,21 @@ def grdqh2d(vis, wt, u, v, w, uscale, uoffset, vscale, voffset, uorigin, vorigin
     return g, sumwt.value
 
 
+def visselpi(wt, time, u, v, w, timerang, uvlimits):
+    """VISSELPI (vis/visselpi.f:7-75) as VISSEL calls it, WT for both WT and
+    NEWWT (vis/vissel.f): points outside the time range or the uv-radius
+    range get WT = -|WT|.  Returns (WT as a new array, NSEL)."""
+    wt = _f32(wt).copy()
+    time, u, v, w = _f32(time), _f32(u), _f32(v), _f32(w)
+    tr = _f32(np.asarray(timerang, np.float32).reshape(-1))
+    uvl = _f32(np.asarray(uvlimits, np.float32).reshape(-1))
+    nsel = C.c_int(-1)
+    lib().visselpi_(_pf(wt), _ri(len(wt)), _pf(time), _pf(u), _pf(v), _pf(w), _pf(tr), _pf(uvl), _pf(wt),
+                    C.byref(nsel))
+    check_error("VISSELPI")
+    return wt, nsel.value
+
+
 def grddeh2d(vis, wt, u, v, w, uscale, uoffset, vscale, voffset, uorigin, vorigin, gvis, intfnu, suppu,
              osampu, intfnv=None, suppv=None, osampv=None, shift=IDENTITY):
     """GRDDEH2D (grd/grddeh2d.f:7-204): degrid.  VIS is written for WT > 0;

@@ -23,12 +23,13 @@ class ImagingCfg(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("cellsize", C.c_float), ("cftype", C.c_char * 4),
                 ("shift", C.c_double * 9), ("weighting", C.c_int), ("fov", C.c_float), ("rmode", C.c_char * 8),
                 ("robust", C.c_float * 3), ("ds", C.c_float), ("tile", C.c_int), ("chunk", C.c_int),
-                ("waves", C.c_int)]
+                ("waves", C.c_int), ("select", C.c_int), ("uvlimits", C.c_float * 2), ("timerange", C.c_float * 2)]
 
 
 class RunInfo(C.Structure):
     _fields_ = [("nvis", C.c_int64), ("ngridded", C.c_int64), ("ndropped", C.c_int64), ("nbad", C.c_int64),
-                ("sumwt", C.c_double), ("lsumwt", C.c_double), ("maxpsf", C.c_float), ("nitems", C.c_int)]
+                ("sumwt", C.c_double), ("lsumwt", C.c_double), ("maxpsf", C.c_float), ("nitems", C.c_int),
+                ("nselected", C.c_int64)]
 
 
 def crdshift(ra_vis, dec_vis, ra_img, dec_img, rota_vis=0.0):
@@ -74,9 +75,10 @@ class Plan:
     """Device-resident imaging plan (image size, cell, kernel, weighting)."""
 
     def __init__(self, nx, ny=None, cellsize=1.0, cftype="SF", shift=None, weighting=0, fov=1.0, rmode="NONE",
-                 robust=(0.0, 0.0, 0.0), ds=0.0, tile=0, chunk=0, waves=0):
+                 robust=(0.0, 0.0, 0.0), ds=0.0, tile=0, chunk=0, waves=0, uvlimits=None, timerange=(0.0, 0.0)):
         L = lib()
-        cfg = _cfg(nx, ny, cellsize, cftype, shift, weighting, fov, rmode, robust, ds, tile, chunk, waves)
+        cfg = _cfg(nx, ny, cellsize, cftype, shift, weighting, fov, rmode, robust, ds, tile, chunk, waves,
+                   uvlimits, timerange)
         self.cfg = cfg
         self._p = C.c_void_p()
         if L.sde_plan_create(C.byref(cfg), C.byref(self._p)) != 0:
@@ -167,6 +169,14 @@ class Plan:
         nb = len(enu) * (len(enu) - 1) // 2
         full = nb * ntime * len(freqs)
         self.nvis = nkeep if 0 < nkeep < full else full
+
+    def set_time(self, time):
+        """The time of each resident visibility, for the selection of uvmap
+        (the plan's uvlimits / timerange, VISSEL in task/uvmap.f:158-168)."""
+        t = np.ascontiguousarray(time, np.float32)
+        if len(t) != self.nvis:
+            raise ValueError("set_time: one time per resident visibility")
+        self._ok(lib().sde_plan_set_time(self._p, t.ctypes.data_as(C.POINTER(C.c_float)), 0), "sde_plan_set_time")
 
     def uvmap(self):
         self._ok(lib().sde_plan_uvmap(self._p), "sde_plan_uvmap")
@@ -289,7 +299,7 @@ class Plan:
 
 
 def _cfg(nx, ny=None, cellsize=1.0, cftype="SF", shift=None, weighting=0, fov=1.0, rmode="NONE",
-         robust=(0.0, 0.0, 0.0), ds=0.0, tile=0, chunk=0, waves=0):
+         robust=(0.0, 0.0, 0.0), ds=0.0, tile=0, chunk=0, waves=0, uvlimits=None, timerange=(0.0, 0.0)):
     cfg = ImagingCfg()
     cfg.nx, cfg.ny = int(nx), int(nx if ny is None else ny)
     cfg.cellsize = float(cellsize)
@@ -302,6 +312,10 @@ def _cfg(nx, ny=None, cellsize=1.0, cftype="SF", shift=None, weighting=0, fov=1.
     cfg.robust = (C.c_float * 3)(*[float(x) for x in robust])
     cfg.ds = float(ds)
     cfg.tile, cfg.chunk, cfg.waves = int(tile), int(chunk), int(waves)
+    # selection (VISSEL) only with uv limits given
+    cfg.select = 0 if uvlimits is None else 1
+    cfg.uvlimits = (C.c_float * 2)(*[float(x) for x in (uvlimits or (0.0, 0.0))])
+    cfg.timerange = (C.c_float * 2)(*[float(x) for x in timerange])
     return cfg
 
 
@@ -322,12 +336,16 @@ def device_memory():
 
 
 def uvmap(vis, wt, u, v, w, nx, cellsize, ny=None, fov=0.0, rmode="NONE", robust=(0.0, 0.0, 0.0), ds=0.0,
-          cftype="SF", shift=None, **tune):
-    """Dirty image and PSF (each divided by the PSF peak) and run info."""
+          cftype="SF", shift=None, time=None, uvlimits=None, timerange=(0.0, 0.0), **tune):
+    """Dirty image and PSF (each divided by the PSF peak) and run info.  With
+    time and uvlimits, the selection of the uvmap task first (VISSEL:
+    time range and uv-radius limits, task/uvmap.f:158-168)."""
     p = Plan(nx, ny, cellsize, cftype, shift, weighting=1 if fov != 0.0 else 0, fov=fov, rmode=rmode,
-             robust=robust, ds=ds, **tune)
+             robust=robust, ds=ds, uvlimits=uvlimits, timerange=timerange, **tune)
     try:
         p.set_vis(vis, wt, u, v, w)
+        if time is not None:
+            p.set_time(time)
         p.uvmap()
         dirty, psf = p.images()
         info = p.info()

@@ -603,3 +603,77 @@ def test_slowz_box_gridding_focuses_a_plane_and_matches_grdqh2d():
     g64, _, _ = po.grdqh23d(np.ones(len(u)), wt, u, v, w, 1.0, 1.0, 1, vo, True, nu, nv, nz, cellz, refz, ident,
                             fp64=True)
     assert np.abs(g64[1] - p2).max() < 1e-6 * np.abs(p2).max()
+
+
+def test_visselpi_hand_counted():
+    """VISSELPI on hand-built points: radius sqrt(U**2+V**2) (w ignored) in
+    [UVLIMITS(1), UVLIMITS(2)] and TIME in TIMERANG keep their weight; the
+    others get -|WT|; flagged points stay as they are and are not counted."""
+    u = np.float32([3.0, 0.0, 30.0, 6.0, 0.0, 3.0, 3.0, 8.0])
+    v = np.float32([4.0, 10.0, 40.0, 8.0, 1.0, 4.0, 4.0, 6.0])
+    w = np.float32([1e6, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0])
+    t = np.float32([1.0, 1.0, 1.0, 1.0, 1.0, 0.5, 2.5, 1.0])
+    wt = np.float32([1.0, 2.0, 1.0, 1.0, 1.0, 1.0, 1.0, -3.0])
+    # radii 5, 10, 50, 10, 1, 5, 5, 10 ; limits [5, 10] ; time [1, 2]
+    out, nsel = po.visselpi(wt, t, u, v, w, (1.0, 2.0), (5.0, 10.0))
+    assert list(out) == [1.0, 2.0, -1.0, 1.0, -1.0, -1.0, -1.0, -3.0]
+    assert nsel == 3
+    # TIMERANG (0, 0) accepts every time
+    out, nsel = po.visselpi(wt, t, u, v, w, (0.0, 0.0), (0.0, 1e10))
+    assert list(out) == list(wt) and nsel == 7
+
+
+def test_grdqh2d64_single_point_and_fold():
+    """FP64 GRDQH2D: one visibility lands in cell (NINT(UCELL)+1, NINT(VCELL)),
+    shifted left by one column and scaled by (NU-1)*NV/WT; on the u-origin
+    column the fold makes the column Hermitian."""
+    nu, nv, vz = 17, 32, 16
+    ident = np.eye(3).reshape(-1)
+    vis = np.complex64([0.3 - 0.7j])
+    g, sw, ls, rc = po.grdqh2d64(vis, [2.0], [-4.2], [3.4], [0.0], 1.0, 1.0, 1, vz, False, nu, nv, ident)
+    assert rc == 0 and sw == 2.0 and ls == 2.0
+    # UCELL = -4.2 -> 4.2 + 1 -> UGRID 6, after the shift column 5; VCELL = -3.4 + 16 -> 13
+    nz = np.argwhere(np.abs(g) > 0)
+    assert nz.tolist() == [[12, 4]]
+    scale = (nu - 1) * nv / 2.0
+    assert g[12, 4] == pytest.approx(scale * 2.0 * complex(vis[0]), rel=1e-15)
+    # u > 0: conjugated
+    g, _, _, _ = po.grdqh2d64(vis, [1.0], [4.2], [3.4], [0.0], 1.0, 1.0, 1, vz, False, nu, nv, ident)
+    assert g[18, 4] == pytest.approx(complex(np.conj(vis[0])) * (nu - 1) * nv, rel=1e-15)
+    # a point on the u-origin column (UGRID = UORIGIN + 1 = 2 -> column 1 after the shift)
+    g, _, _, _ = po.grdqh2d64(vis, [1.0], [0.2], [3.0], [0.0], 1.0, 1.0, 1, vz, False, nu, nv, ident)
+    col = g[:, 0]
+    for off in range(nv // 2):
+        assert col[vz - 1 + off] == pytest.approx(np.conj(col[vz - 1 - off]), abs=1e-12)
+    assert np.abs(col).max() > 0
+
+
+def test_grdqh2d64_against_fp32_and_planes():
+    """FP64 GRDQH2D against the REAL restatement (FP32 round-off) and against
+    the FP64 GRDQH23D with one plane at REFZ = 1 (zero plane phase), which
+    reduces to GRDQH2D up to the order of its products."""
+    rng = np.random.default_rng(3)
+    n = 3000
+    nu, nv, vz = 33, 64, 32
+    u = rng.uniform(-30, 30, n).astype(np.float32)
+    v = rng.uniform(-30, 30, n).astype(np.float32)
+    u[:100] = rng.uniform(-1.0, 1.0, 100).astype(np.float32)
+    w = rng.uniform(-20, 20, n).astype(np.float32)
+    vis = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    wt = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    wt[::11] = -1.0
+    s = po.crdshift(10.0, 30.0, 10.01, 30.02)
+    for psf in (False, True):
+        g64, sw64, ls64, rc = po.grdqh2d64(vis, wt, u, v, w, -0.9, 0.9, 1, vz, psf, nu, nv, s)
+        g32, sw32, rc32 = po.grdqh2d(vis, wt, u, v, w, -0.9, 0.9, 1, vz, psf, nu, nv, s)
+        assert rc == 0 and rc32 == 0
+        assert sw32 == pytest.approx(sw64, rel=1e-5)  # REAL running sum of 2700 weights
+        scale = np.abs(g64).max()
+        assert np.abs(g32 - g64).max() <= 2e-5 * scale
+        g3, sw3, rc3 = po.grdqh23d(vis, wt, u, v, w, -0.9, 0.9, 1, vz, psf, nu, nv, 1, 1.0, 1.0, s, fp64=True)
+        assert rc3 == 0
+        if psf:
+            # GRDQH23D's PSF grids FVIS = WT and normalises by SUMWT (all points), GRDQH2D by LSUMWT
+            assert np.abs(g3[0] * (sw64 / ls64) - g64).max() <= 1e-12 * scale
+        else:
+            assert np.abs(g3[0] * (sw64 / ls64) - g64).max() <= 1e-12 * scale

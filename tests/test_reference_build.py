@@ -228,3 +228,31 @@ def test_vismodel_sequence():
     ok = wt > 0
     assert _bits_equal(a_wt, b_wt)
     assert _bits_equal(a_vis[ok], b_vis[ok])
+
+
+@pytest.mark.parametrize("case", ["radius", "time", "both", "open"])
+def test_visselpi(case):
+    """VISSELPI (vis/visselpi.f:51-72): the weights it negates and NSEL, bit
+    for bit, for uv-radius limits, a time range, both, and TIMERANG = (0, 0)
+    (every time accepted)."""
+    rng = np.random.default_rng(21)
+    n = 20000
+    u = rng.uniform(-3000, 3000, n).astype(np.float32)
+    v = rng.uniform(-3000, 3000, n).astype(np.float32)
+    w = rng.uniform(-500, 500, n).astype(np.float32)
+    time = rng.uniform(0.0, 0.5, n).astype(np.float32)
+    wt = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    wt[::13] = -1.0
+    wt[::31] = 0.0
+    # radii exactly at the limits (REAL U**2+V**2 against UVLIMITS**2)
+    u[:4] = np.float32([300.0, 2500.0, 0.0, 2500.0])
+    v[:4] = np.float32([0.0, 0.0, 300.0, 1e-3])
+    tr = {"radius": (0.0, 0.0), "time": (0.1, 0.3), "both": (0.05, 0.45), "open": (0.0, 0.0)}[case]
+    uvl = {"radius": (300.0, 2500.0), "time": (0.0, 1e10), "both": (800.0, 2000.0), "open": (0.0, 1e10)}[case]
+    w1, n1 = rw.visselpi(wt, time, u, v, w, tr, uvl)
+    w2, n2 = po.visselpi(wt, time, u, v, w, tr, uvl)
+    assert n1 == n2
+    assert _bits_equal(w1, w2)
+    assert n1 == int(np.sum(w1 > 0))
+    if case == "open":
+        assert _bits_equal(w1, wt)
