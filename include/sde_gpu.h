@@ -372,6 +372,32 @@ int64_t sde_grddeh23_device(void *vis, float *wt, const float *u, const float *v
                             int osampu, const float *intfnv, int suppv, int osampv, int nz,
                             float cellz, float refz, const double *shift);
 
+/* ------------------------------------------------------------------ */
+/* Part 4: gridding that allocates nothing                              */
+/* ------------------------------------------------------------------ */
+
+/* GRDH2D on device-resident arrays (arguments as sde_grdh23d_device's
+ * without the planes: VIS, WT, U, V, W, GVIS (NU*NV complex) and the kernel
+ * tables in device memory, SHIFT on the host), run on the library's stream
+ * and synchronised before return; SUMWT to *sumwt.  It calls no hipMalloc
+ * or hipFree: the cell sort, staging records, work items, partial tiles and
+ * sums live in one static device array of sde_noalloc_scratch_bytes()
+ * (128 MiB) per GPU, held until the process ends.  A call needing more
+ * (about 56 bytes per visibility plus 3.9 KB per 16 x 16 tile of the grid
+ * plus the sort's temporary storage, e.g. at most about 1.5e6 visibilities
+ * on a 1025 x 2048 grid) returns -1 with sde_last_error() set and touches
+ * nothing.  Calls that overlap on one GPU race on the shared array.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that
+ * forbid allocation).  Returns 0, or -1 (also for "No weight!"). */
+int sde_grdh2d_device_noalloc(const void *vis, const float *wt, const float *u, const float *v,
+                              const float *w, int64_t nvis, float uscale, float uoffset,
+                              float vscale, float voffset, int uorigin, int vorigin, void *gvis,
+                              int psf, int nu, int nv, const float *gridfnu, int suppu,
+                              int osampu, const float *gridfnv, int suppv, int osampv,
+                              const double *shift, double *sumwt);
+size_t sde_noalloc_scratch_bytes(void);
+
 #ifdef __cplusplus
 }
 #endif

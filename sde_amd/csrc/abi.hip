@@ -2408,4 +2408,147 @@ int sde_plan_synth(sde_plan *p, int nant, const double *enu, double lat, double 
     return plan_reserve_items(p);
 }
 
+
+// ---------------------------------------------------------------------------
+// Gridding that allocates nothing: sde_grdh2d_device_noalloc.  No counterpart
+// in the reference; for callers that cannot allocate device memory (code
+// under graph capture, tight serving loops, harnesses that forbid it).
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+// The scratch of sde_grdh2d_device_noalloc: the cell sort (keys, values and
+// their sorted copies, the sort's temporary storage), the staging records,
+// the tile ranges and work items, the partial tiles and the sums of one
+// call, carved in 256-byte aligned pieces.  Every GPU the library runs on
+// holds its own copy until the process ends.  Calls on one GPU share it and
+// race when they overlap; each call rewrites every part it reads.
+constexpr size_t kNoAllocScratchBytes = (size_t)128 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+unsigned char *g_scratch_ptr[kMaxScratchDevices];  // device address of `scratch`, per device
+std::mutex g_scratch_mu;
+
+// The device address of the scratch array on device dev (current): looked
+// up once per device.  The array's name in host code is a host-side stand-in,
+// never passed to a kernel.
+int scratch_address(int dev, unsigned char **out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (dev < 0 || dev >= kMaxScratchDevices) {
+        set_plan_error("device %d outside the library's %d scratch slots", dev, kMaxScratchDevices);
+        return -1;
+    }
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        SDE_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            set_plan_error("hipGetSymbolAddress returned no address for the scratch array");
+            return -1;
+        }
+        g_scratch_ptr[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+// Carves 256-byte aligned pieces from [base, base + cap); with a null base it
+// only counts (the size a call needs).
+struct ScratchCarve {
+    unsigned char *base;
+    size_t off = 0;
+    template <typename T>
+    T *take(size_t count) {
+        const size_t bytes = ((count * sizeof(T) + 255) / 256) * 256;
+        T *p = base ? reinterpret_cast<T *>(base + off) : nullptr;
+        off += bytes;
+        return p;
+    }
+};
+
+// The buffers of one gridding call (run_sort + run_grid, no weighting, no
+// degridding halves) carved from base; returns the bytes taken.
+size_t carve_grid_scratch(unsigned char *base, int64_t n, const TileParams &tp, int chmask, SortBufs &sb,
+                          float **partial, double **item_sums, double **sums) {
+    ScratchCarve c{base};
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    sb.keys = c.take<uint32_t>(nn);
+    sb.skeys = c.take<uint32_t>(nn);
+    sb.vals = c.take<uint64_t>(nn);
+    sb.svals = c.take<uint64_t>(nn);
+    sb.hot = c.take<StageHot>(nn);
+    sb.cold = c.take<StageCold>(nn);
+    sb.temp = c.take<char>(sb.temp_bytes);
+    sb.tile_start = c.take<int>((size_t)tp.ntiles + 1);
+    sb.item_start = c.take<int>((size_t)tp.ntiles + 1);
+    sb.nitems = c.take<int>(1);
+    sb.items = c.take<WorkItem>((size_t)sb.max_items);
+    *partial = c.take<float>(grid_partial_floats(tp, chmask, sb.max_items));
+    *item_sums = c.take<double>(2 * (size_t)sb.max_items);
+    *sums = c.take<double>(4);
+    return c.off;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sde_grdh2d_device_noalloc(const void *vis, const float *wt, const float *u, const float *v, const float *w,
+                              int64_t nvis, float uscale, float uoffset, float vscale, float voffset, int uorigin,
+                              int vorigin, void *gvis, int psf, int nu, int nv, const float *gridfnu, int suppu,
+                              int osampu, const float *gridfnv, int suppv, int osampv, const double *shift,
+                              double *sumwt) {
+    const char *R = "GRDH2D";
+    if (nvis > INT32_MAX || !check_kernel_args(R, (int)nvis, nu, nv, suppu, suppv, osampu, osampv, uorigin, vorigin))
+        return -1;
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    Ctx &c = g_ctx;
+    if (ctx_init(c)) return -1;
+    int dev = 0;
+    SDE_CHECK(hipGetDevice(&dev));
+    unsigned char *base = nullptr;
+    if (scratch_address(dev, &base)) return -1;
+    hipStream_t s = c.stream;
+    const int64_t n = nvis;
+    const CoordParams cp = make_cp(shift, uscale, uoffset, vscale, voffset, uorigin, vorigin, nu, nv, suppu, suppv,
+                                   osampu, osampv);
+    const TileParams tp = make_tp(nu, nv, default_tile(nu, nv), default_chunk(n));
+    const int chmask = psf ? 2 : 1;
+    SortBufs sb;
+    sb.max_items = max_items_for(tp, n);
+    sb.temp_bytes = sort_temp_bytes(n > 0 ? n : 1, sort_nbits(tp));
+    if (sb.temp_bytes == 0) {
+        set_plan_error("GRDH2D: radix sort size query failed");
+        return -1;
+    }
+    float *partial = nullptr;
+    double *isums = nullptr, *dsums = nullptr;
+    const size_t need = carve_grid_scratch(nullptr, n, tp, chmask, sb, &partial, &isums, &dsums);
+    if (need > kNoAllocScratchBytes) {
+        set_plan_error("GRDH2D: %lld visibilities on a %d x %d grid need %zu bytes of scratch; the no-allocation "
+                       "entry point holds %zu", (long long)n, nu, nv, need, kNoAllocScratchBytes);
+        return -1;
+    }
+    carve_grid_scratch(base, n, tp, chmask, sb, &partial, &isums, &dsums);
+    auto nb = [](const char *) {};
+    auto ne = []() {};
+    float2 *dg = static_cast<float2 *>(gvis);
+    if (run_sort(s, static_cast<const float2 *>(vis), wt, u, v, w, n, cp, tp, sb, nullptr, nullptr, nb, ne) ||
+        run_grid(s, cp, tp, chmask, false, WeightParams(), gridfnu, gridfnv, sb, nullptr, partial, isums, dsums,
+                 chmask == 1 ? dg : nullptr, chmask == 2 ? dg : nullptr, nb, ne))
+        return -1;
+    double sums[2];
+    SDE_CHECK(hipMemcpyAsync(sums, dsums, sizeof sums, hipMemcpyDeviceToHost, s));
+    SDE_CHECK(hipStreamSynchronize(s));
+    if (sumwt) *sumwt = sums[0];
+    if (sums[1] == 0.0) {
+        set_plan_error("GRDH2D: No weight!");
+        return -1;
+    }
+    return 0;
+}
+
+size_t sde_noalloc_scratch_bytes(void) { return kNoAllocScratchBytes; }
+
 }  // extern "C"
