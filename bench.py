@@ -103,38 +103,122 @@ def read_traffic(region, workload_tag):
     return round(total / anchor["dispatches"])
 
 
-def cpu_baseline(vis, wt, u, v, w, nx, cell, nfull):
-    """The reference path on one host core, on two bounded samples of the
-    same workload; the per-visibility and fixed costs fitted from the two
-    are extrapolated to the full job.  Runs the reference's own routines
-    compiled from its Fortran sources (oracle/_ref, -O2) in task/uvmap.f's
-    and task/vismodel.f's call sequences when that build is present, else
-    the C restatement (oracle/ref32.c)."""
-    from oracle import pyoracle as po
-    from oracle import refwrap as rw
-    kind = "reference" if rw.available() else "port"
-    t = []
-    sizes = (4_000_000, 32_000_000)
-    for n in sizes:
-        t0 = time.perf_counter()
-        if kind == "reference":
-            dirty, psf, sw, _ = rw.uvmap(nx, cell, vis[:n], wt[:n], u[:n], v[:n], w[:n], fov=1.0, rmode="NORM",
-                                         robust=(0.0, 0.0, 0.0))
-            rw.vismodel(nx, cell, dirty, wt[:n], u[:n], v[:n], w[:n])
-        else:
-            cfg = po.make_cfg(nx, cellsize=cell, fov=1.0, rmode="NORM", robust=(0.0, 0.0, 0.0))
-            dirty, psf, sw, rc, _ = po.uvmap(cfg, vis[:n], wt[:n], u[:n], v[:n], w[:n])
-            po.vismodel(cfg, dirty, wt[:n], u[:n], v[:n], w[:n])
-        t.append(time.perf_counter() - t0)
+def read_lds(region, workload_tag):
+    """LDS bank-conflict figures of a region's kernels from the committed
+    rocprofv3 summary (its LDS table): conflict cycles per LDS instruction
+    and the bank-conflict share of the LDS-active cycles, per kernel."""
+    path = os.path.join(ROOT, "profiles", "kernels_%s.md" % workload_tag)
+    names = REGION_KERNELS.get(region)
+    if not names or not os.path.exists(path):
+        return None
+    out, in_lds = {}, False
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("| kernel | lds_instr"):
+                in_lds = True
+                continue
+            p = [x.strip() for x in line.strip().strip("|").split("|")]
+            if not in_lds or len(p) != 6 or p[0].startswith("---"):
+                continue
+            if any(p[0] == k or (k.endswith("*") and p[0].startswith(k[:-1])) for k in names):
+                out[p[0]] = {"conflict_cycles_per_lds_instr": float(p[3]), "bank_conflict_share": float(p[4]),
+                             "addr_conflict_share": float(p[5])}
+    return out or None
+
+
+def host_info():
+    """The host the CPU baseline runs on: core counts and the CPU model."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "cpu_model": model}
+
+
+def _fit(sizes, t, nfull):
     per_vis = (t[1] - t[0]) / (sizes[1] - sizes[0])
     fixed = max(t[0] - per_vis * sizes[0], 0.0)
-    t_full = fixed + per_vis * nfull
-    what = ("the reference's GRDRUW2D, GRDH2D x2, FFTR2D, PIXCOR2D, GRDDEH2D compiled from source (flang -O2, "
-            "oracle/_ref)" if kind == "reference" else "oracle/ref32.c")
-    return {"value": 2.0 * nfull / t_full, "unit": "visibilities/s", "cores": 1, "kind": kind,
-            "sample": "%s: uvmap+vismodel (robust NORM, 4096^2) on the first %d and %d visibilities of the "
-                      "workload: %.2f s and %.2f s; fitted %.3f us/visibility + %.2f s fixed, extrapolated to %d "
-                      "visibilities" % (what, sizes[0], sizes[1], t[0], t[1], per_vis * 1e6, fixed, nfull)}
+    return per_vis, fixed, fixed + per_vis * nfull
+
+
+def cpu_baseline(vis, wt, u, v, w, nx, cell, nfull):
+    """The reference path on the host's cores, each variant on two bounded
+    samples of the same workload, the per-visibility and fixed costs fitted
+    from the two and extrapolated to the full job:
+      1. one core, the reference's own routines compiled from its Fortran
+         sources without optimisation, as its build flags (oracle/_ref
+         libsderef_O0.so; var/comopt.LINUX: FFLAGS "-c ") -- or, without
+         the reference build, the C restatement at -O2;
+      2. one core, the same at -O2 (oracle/_ref libsderef.so);
+      3. all cores (the box's share, at most 16): the C restatement with the
+         visibilities partitioned over OpenMP threads, per-thread density
+         grids and gridded planes summed, degridding per share, the
+         transforms on one core (oracle/par32.c).
+    Each runs the uvmap + vismodel job in task/uvmap.f's and
+    task/vismodel.f's call sequences.  The reported baseline is the fastest
+    variant; all are listed."""
+    from oracle import pyoracle as po
+    from oracle import refwrap as rw
+    variants = []
+
+    def run_ref(variant, sizes, label):
+        t = []
+        for n in sizes:
+            t0 = time.perf_counter()
+            dirty, psf, sw, _ = rw.uvmap(nx, cell, vis[:n], wt[:n], u[:n], v[:n], w[:n], fov=1.0, rmode="NORM",
+                                         robust=(0.0, 0.0, 0.0), variant=variant)
+            rw.vismodel(nx, cell, dirty, wt[:n], u[:n], v[:n], w[:n], variant=variant)
+            t.append(time.perf_counter() - t0)
+        return t
+
+    def run_port(nthreads, sizes):
+        cfg = po.make_cfg(nx, cellsize=cell, fov=1.0, rmode="NORM", robust=(0.0, 0.0, 0.0))
+        t = []
+        for n in sizes:
+            t0 = time.perf_counter()
+            if nthreads == 0:
+                dirty, psf, sw, rc, _ = po.uvmap(cfg, vis[:n], wt[:n], u[:n], v[:n], w[:n])
+                po.vismodel(cfg, dirty, wt[:n], u[:n], v[:n], w[:n])
+            else:
+                po.par_uvmap_vismodel(nthreads, cfg, vis[:n], wt[:n], u[:n], v[:n], w[:n])
+            t.append(time.perf_counter() - t0)
+        return t
+
+    def add(kind, cores, flags, sizes, t, what):
+        per_vis, fixed, t_full = _fit(sizes, t, nfull)
+        variants.append({"value": 2.0 * nfull / t_full, "unit": "visibilities/s", "cores": cores, "kind": kind,
+                         "flags": flags,
+                         "sample": "%s: uvmap+vismodel (robust NORM, 4096^2) on the first %d and %d visibilities "
+                                   "of the workload: %.2f s and %.2f s; fitted %.4f us/visibility + %.2f s fixed, "
+                                   "extrapolated to %d visibilities" % (what, sizes[0], sizes[1], t[0], t[1],
+                                                                       per_vis * 1e6, fixed, nfull)})
+        print("# cpu %s %s x%d: %.3g vis/s" % (kind, flags, cores, variants[-1]["value"]), file=sys.stderr,
+              flush=True)
+
+    if rw.available("_O0"):
+        sz = (2_000_000, 8_000_000)
+        add("reference", 1, "-O0", sz, run_ref("_O0", sz, "O0"),
+            "the reference's GRDRUW2D, GRDH2D x2, FFTR2D, PIXCOR2D, GRDDEH2D compiled from source (flang -O0)")
+    if rw.available():
+        sz = (4_000_000, 32_000_000)
+        add("reference", 1, "-O2", sz, run_ref("", sz, "O2"),
+            "the reference's GRDRUW2D, GRDH2D x2, FFTR2D, PIXCOR2D, GRDDEH2D compiled from source (flang -O2)")
+    else:
+        sz = (4_000_000, 32_000_000)
+        add("port", 1, "-O2", sz, run_port(0, sz), "oracle/ref32.c")
+    nt = min(len(os.sched_getaffinity(0)), 16)
+    sz = (8_000_000, 32_000_000)
+    add("port", nt, "-O2 -fopenmp", sz, run_port(nt, sz),
+        "oracle/par32.c on %d threads (visibilities partitioned, grids summed, transforms on one core)" % nt)
+    best = dict(max(variants, key=lambda x: x["value"]))
+    best["variants"] = variants
+    best["host"] = host_info()
+    return best
 
 
 def accuracy(plan, vis, wt, u, v, w, nx, cell):
@@ -506,7 +590,10 @@ def main():
     roof = {"kernel": kname, "bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
             "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4) if achieved else None,
             "traffic": read_traffic(kname, "large_field"), "avg_ms": round(avg_s * 1e3, 3),
-            "alg_bytes": ab}
+            "alg_bytes": ab,
+            "traffic_counts": "FETCH_SIZE + WRITE_SIZE per launch from profiles/kernels_large_field.md, calibrated "
+                              "on a device copy: traffic between L2 and the fabric, Infinity Cache hits included",
+            "lds": read_lds(kname, "large_field")}
     kernels = {k: {"ms_per_launch": round(v[0] / v[1], 3), "launches": v[1]} for k, v in timings.items()}
 
     out = {"metric": METRIC, "value": round(value, 1), "unit": "visibilities/s", "n_gpus": 1, "steps": args.steps,

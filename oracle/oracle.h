@@ -166,6 +166,41 @@ typedef struct {
     double ra_vis, dec_vis, ra_img, dec_img, rota_vis;
 } or_uvmap_cfg;
 
+/* Geometry of a uvmap (IMGMAKE, FFTCONJA, VISGRID, IMGGRIDC). */
+typedef struct {
+    int fnu, nu, nv, uorigin, vorigin, cflen, supp, osamp, ccflen;
+    float udelt, vdelt, uscale, vscale;
+    double shift[9];
+} geom32;
+int r32_make_geom(const or_uvmap_cfg *cfg, geom32 *g);
+
+/* Parts of GRDH2D and GRDRUW2D over a range of visibilities (r32_grdh2d and
+ * r32_grdruw2d are these over all of them), for the parallel CPU port. */
+double r32_grdh2d_accum(const float *vis, const float *wt, const float *u, const float *v,
+                        const float *w, int nvis, float uscale, float uoffset, float vscale,
+                        float voffset, int uorigin, int vorigin, float *gvis, int psf, int nu,
+                        int nv, const float *gridfnu, int suppu, int osampu,
+                        const float *gridfnv, int suppv, int osampv, const double *shift,
+                        float *sumwt);
+int r32_grdh2d_finish(float *gvis, double lsumwt, int nu, int nv, int uorigin, int vorigin, int suppu);
+double r32_grdruw2d_density(const float *wt, const float *u, const float *v, const float *w,
+                            int nvis, float uscale, float uoffset, float vscale, float voffset,
+                            int uorigin, int vorigin, float *gwt, int nu, int nv, const double *shift);
+int r32_grdruw2d_weights(const float *wt, const float *u, const float *v, const float *w,
+                         int nvis, float uscale, float uoffset, float vscale, float voffset,
+                         int uorigin, int vorigin, float *newwt, const float *gwt, int nu, int nv,
+                         const double *shift, const char *rmode, const float *robust, float ds,
+                         double sumwt, int *ndrop);
+
+/* The uvmap + vismodel job of the CPU baseline on nthreads cores, the
+ * visibilities partitioned: per-thread density grids and gridded planes
+ * summed, weights and degridding per partition, the transforms on one
+ * core.  dirty/psf as r32_uvmap's; mvis/mwt the prediction from the dirty
+ * image.  Returns 0 or an error code (par32.c, OpenMP). */
+int p32_uvmap_vismodel(int nthreads, const or_uvmap_cfg *cfg, const float *vis, const float *wt,
+                       const float *u, const float *v, const float *w, int nvis, float *dirty,
+                       float *psf, float *mvis, float *mwt);
+
 /* Dirty image and PSF (both divided by the PSF peak), FP32 restatement.
  * wt is copied; the reweighted weights are returned through newwt if non-NULL.
  * Returns 0 or an error code (1 no weight, 2 bad config). */

@@ -536,3 +536,33 @@ def dft_image(vis, wt, u, v, nx, ny, dx_rad, dy_rad, psf=False, cenx=None, ceny=
     lib.o64_dft_image(_pf(vis), _pd(wt), _pf(u), _pf(v), len(u), nx, ny, dx_rad, dy_rad, cenx, ceny, int(psf),
                       _pd(img))
     return img
+
+
+_PAR = os.path.join(_HERE, "build", "liboracle_par.so")
+_plib = None
+
+
+def par_lib():
+    """oracle/build/liboracle_par.so (OpenMP), loaded on first use."""
+    global _plib
+    if _plib is None:
+        if not os.path.exists(_PAR):
+            build()
+        _plib = C.CDLL(_PAR)
+        _plib.p32_uvmap_vismodel.restype = C.c_int
+        _plib.p32_uvmap_vismodel.argtypes = [C.c_int, C.POINTER(UvmapCfg), _f, _f, _f, _f, _f, C.c_int, _f, _f,
+                                             _f, _f]
+    return _plib
+
+
+def par_uvmap_vismodel(nthreads, cfg, vis, wt, u, v, w):
+    """The uvmap + vismodel job on nthreads cores, visibilities partitioned
+    (oracle/par32.c) -> (dirty, psf, model vis, model wt, rc)."""
+    vis, wt, u, v, w = c64(vis), f32(wt), f32(u), f32(v), f32(w)
+    dirty = np.zeros((cfg.ny, cfg.nx), np.float32)
+    psf = np.zeros_like(dirty)
+    mvis = np.zeros(len(u), np.complex64)
+    mwt = np.zeros(len(u), np.float32)
+    rc = par_lib().p32_uvmap_vismodel(int(nthreads), C.byref(cfg), _pf(vis), _pf(wt), _pf(u), _pf(v), _pf(w), len(u),
+                                      _pf(dirty), _pf(psf), _pf(mvis), _pf(mwt))
+    return dirty, psf, mvis, mwt, rc

@@ -217,19 +217,21 @@ static int is_shift(const double *s) {
 }
 
 /* grd/grdh2d.f:87-227 */
-int r32_grdh2d(const float *vis, const float *wt, const float *u, const float *v,
-               const float *w, int nvis, float uscale, float uoffset, float vscale,
-               float voffset, int uorigin, int vorigin, float *gvis, int psf, int nu,
-               int nv, const float *gridfnu, int suppu, int osampu,
-               const float *gridfnv, int suppv, int osampv, const double *shift,
-               float *sumwt) {
+/* The loop over the visibilities of GRDH2D (grd/grdh2d.f:109-191), adding
+ * into gvis (not zeroed here) and into *sumwt; returns the LSUMWT of these
+ * visibilities.  r32_grdh2d is r32_grdh2d_accum over all of them into a
+ * zeroed grid, then r32_grdh2d_finish. */
+double r32_grdh2d_accum(const float *vis, const float *wt, const float *u, const float *v,
+                        const float *w, int nvis, float uscale, float uoffset, float vscale,
+                        float voffset, int uorigin, int vorigin, float *gvis, int psf, int nu,
+                        int nv, const float *gridfnu, int suppu, int osampu,
+                        const float *gridfnv, int suppv, int osampv, const double *shift,
+                        float *sumwt) {
     const double twopi = (double)(8.0f * atanf(1.0f));
     const int cfoffset = (suppu + 1) * osampu + 1;
     const int doshift = is_shift(shift);
     double lsumwt = 0.0;
     int ucen, vcen;
-    memset(gvis, 0, sizeof(float) * 2 * (size_t)nu * nv);
-    *sumwt = 0.0f;
     for (int i = 0; i < nvis; i++) {
         float fr, fi;
         double ucell, vcell;
@@ -284,6 +286,14 @@ int r32_grdh2d(const float *vis, const float *wt, const float *u, const float *v
             }
         }
     }
+    return lsumwt;
+}
+
+/* The fold of the u = 0 column, the shift left by SUPPU and the
+ * normalisation (NU-1)*NV/LSUMWT of GRDH2D (grd/grdh2d.f:193-227).
+ * Returns 1 for LSUMWT = 0 (no weight). */
+int r32_grdh2d_finish(float *gvis, double lsumwt, int nu, int nv, int uorigin, int vorigin, int suppu) {
+    int ucen, vcen;
     if (lsumwt == 0.0) return 1;
     ucen = uorigin + suppu;
     vcen = vorigin;
@@ -310,6 +320,20 @@ int r32_grdh2d(const float *vis, const float *wt, const float *u, const float *v
         for (int ug = nu - suppu + 1; ug <= nu; ug++) GRE(gvis, nu, ug, vg) = GIM(gvis, nu, ug, vg) = 0.0f;
     }
     return 0;
+}
+
+int r32_grdh2d(const float *vis, const float *wt, const float *u, const float *v,
+               const float *w, int nvis, float uscale, float uoffset, float vscale,
+               float voffset, int uorigin, int vorigin, float *gvis, int psf, int nu,
+               int nv, const float *gridfnu, int suppu, int osampu,
+               const float *gridfnv, int suppv, int osampv, const double *shift,
+               float *sumwt) {
+    memset(gvis, 0, sizeof(float) * 2 * (size_t)nu * nv);
+    *sumwt = 0.0f;
+    const double lsumwt = r32_grdh2d_accum(vis, wt, u, v, w, nvis, uscale, uoffset, vscale, voffset, uorigin,
+                                           vorigin, gvis, psf, nu, nv, gridfnu, suppu, osampu, gridfnv, suppv,
+                                           osampv, shift, sumwt);
+    return r32_grdh2d_finish(gvis, lsumwt, nu, nv, uorigin, vorigin, suppu);
 }
 
 /* grd/grdqh2d.f:70-147 (BOX, nearest cell; REAL coordinates) */
@@ -548,17 +572,39 @@ void r32_grduwt2d(const float *wt, const float *u, const float *v, const float *
 }
 
 /* grd/grdruw2d.f:87-249 */
-int r32_grdruw2d(const float *wt, const float *u, const float *v, const float *w,
-                 int nvis, float uscale, float uoffset, float vscale, float voffset,
-                 int uorigin, int vorigin, float *newwt, float *gwt, int nu, int nv,
-                 const double *shift, const char *rmode, const float *robust, float ds,
-                 int *ndrop) {
+/* The density loop of GRDRUW2D (grd/grdruw2d.f:139-172) over these
+ * visibilities, adding into gwt (not zeroed here); returns their SUMWT. */
+double r32_grdruw2d_density(const float *wt, const float *u, const float *v, const float *w,
+                            int nvis, float uscale, float uoffset, float vscale, float voffset,
+                            int uorigin, int vorigin, float *gwt, int nu, int nv, const double *shift) {
+    const int doshift = is_shift(shift);
+    int uc, vc;
+    float ul, vl;
+    double sumwt = 0.0;
+    for (int i = 0; i < nvis; i++) {
+        if (wt[i] <= 0.0f) continue;
+        wcell(shift, doshift, u[i], v[i], w[i], uscale, uoffset, vscale, voffset, uorigin, vorigin, &uc, &vc, &ul, &vl);
+        if (uc < 1 || uc > nu || vc < 1 || vc > nv) continue;
+        gwt[(size_t)(vc - 1) * nu + uc - 1] = gwt[(size_t)(vc - 1) * nu + uc - 1] + wt[i];
+        sumwt = sumwt + (double)wt[i];
+    }
+    return sumwt;
+}
+
+/* The rest of GRDRUW2D (grd/grdruw2d.f:184-249) given the density grid and
+ * SUMWT: the robustness factors, then the new weights of these
+ * visibilities; *ndrop counts the points off the density grid.  Returns 1
+ * for an unknown RMODE. */
+int r32_grdruw2d_weights(const float *wt, const float *u, const float *v, const float *w,
+                         int nvis, float uscale, float uoffset, float vscale, float voffset,
+                         int uorigin, int vorigin, float *newwt, const float *gwt, int nu, int nv,
+                         const double *shift, const char *rmode, const float *robust, float ds,
+                         double sumwt, int *ndrop) {
     const int doshift = is_shift(shift);
     int uc, vc, doaz = strncmp(rmode, "AZ", 2) == 0;
     float ul, vl, rob, rpa = 0.0f, rbscale = 0.0f, f2, d2;
     const float pi = 4.0f * atanf(1.0f);
-    double sumwt = 0.0, sumlocwt = 0.0;
-    memset(gwt, 0, sizeof(float) * (size_t)nu * nv);
+    double sumlocwt = 0.0;
     if (doaz) {
         rob = (robust[0] + robust[1]) / 2.0f;
         rpa = -robust[2] + 90.0f - 90.0f;
@@ -568,13 +614,6 @@ int r32_grdruw2d(const float *wt, const float *u, const float *v, const float *w
         rbscale = (robust[0] - robust[1]) / (pi / 2.0f);
     } else {
         rob = robust[0];
-    }
-    for (int i = 0; i < nvis; i++) {
-        if (wt[i] <= 0.0f) continue;
-        wcell(shift, doshift, u[i], v[i], w[i], uscale, uoffset, vscale, voffset, uorigin, vorigin, &uc, &vc, &ul, &vl);
-        if (uc < 1 || uc > nu || vc < 1 || vc > nv) continue;
-        gwt[(size_t)(vc - 1) * nu + uc - 1] = gwt[(size_t)(vc - 1) * nu + uc - 1] + wt[i];
-        sumwt = sumwt + (double)wt[i];
     }
     if (strcmp(rmode, "NORM") == 0 || strcmp(rmode, "AZ-NORM") == 0) {
         for (size_t k = 0; k < (size_t)nu * nv; k++)
@@ -617,6 +656,18 @@ int r32_grdruw2d(const float *wt, const float *u, const float *v, const float *w
         }
     }
     return 0;
+}
+
+int r32_grdruw2d(const float *wt, const float *u, const float *v, const float *w,
+                 int nvis, float uscale, float uoffset, float vscale, float voffset,
+                 int uorigin, int vorigin, float *newwt, float *gwt, int nu, int nv,
+                 const double *shift, const char *rmode, const float *robust, float ds,
+                 int *ndrop) {
+    memset(gwt, 0, sizeof(float) * (size_t)nu * nv);
+    const double sumwt = r32_grdruw2d_density(wt, u, v, w, nvis, uscale, uoffset, vscale, voffset, uorigin, vorigin,
+                                              gwt, nu, nv, shift);
+    return r32_grdruw2d_weights(wt, u, v, w, nvis, uscale, uoffset, vscale, voffset, uorigin, vorigin, newwt, gwt,
+                                nu, nv, shift, rmode, robust, ds, sumwt, ndrop);
 }
 
 /* pix/pixcor2d.f:44-68 */
@@ -918,15 +969,10 @@ void r32_fftr2d(float *r, float *x, int nx, int ny, int fnu, int nv, int dir) {
 
 /* ---- pipelines ---- */
 
-typedef struct {
-    int fnu, nu, nv, uorigin, vorigin, cflen, supp, osamp, ccflen;
-    float udelt, vdelt, uscale, vscale;
-    double shift[9];
-} geom32;
 
 /* img/imgmake.f:85-104, fft/fftconja.f:156-168, vis/visgrid.f:97-153,
  * img/imggridc.f:45-75. */
-static int make_geom(const or_uvmap_cfg *cfg, geom32 *g) {
+int r32_make_geom(const or_uvmap_cfg *cfg, geom32 *g) {
     float dx = -cfg->cellsize / 3600.0f, dy = cfg->cellsize / 3600.0f;
     g->fnu = or_fftpwr2(cfg->nx);
     g->nu = g->fnu / 2 + 1;
@@ -953,7 +999,7 @@ int r32_uvmap(const or_uvmap_cfg *cfg, const float *vis, const float *wt, const 
     float cf[1025], *ccf, *wts, *grid, maxpsf;
     int supp, osamp, rc = 0, ndrop;
     size_t npix = (size_t)cfg->nx * cfg->ny;
-    make_geom(cfg, &g);
+    r32_make_geom(cfg, &g);
     if (r32_grdcf(cfg->cftype, cf, &supp, &osamp)) return 2;
     ccf = (float *)malloc(sizeof(float) * g.ccflen);
     r32_grdccf(cfg->cftype, ccf, g.ccflen);
@@ -1006,7 +1052,7 @@ int r32_vismodel(const or_uvmap_cfg *cfg, const float *image, float *vis, float 
     float cf[1025], *ccf, *img, *grid;
     int supp, osamp;
     size_t npix = (size_t)cfg->nx * cfg->ny;
-    make_geom(cfg, &g);
+    r32_make_geom(cfg, &g);
     if (r32_grdcf(cfg->cftype, cf, &supp, &osamp)) return 2;
     ccf = (float *)malloc(sizeof(float) * g.ccflen);
     r32_grdccf(cfg->cftype, ccf, g.ccflen);

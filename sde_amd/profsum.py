@@ -2,10 +2,12 @@
 
     python -m sde_amd.profsum OUT "<command line>" > profiles/<name>.md
 
-OUT holds three rocprofv3 output folders of the same command:
+OUT holds three rocprofv3 output folders of the same command, and a fourth
+for the LDS counters when present:
     OUT/trace  --kernel-trace --stats
     OUT/fetch  --pmc FETCH_SIZE
     OUT/write  --pmc WRITE_SIZE
+    OUT/lds    --pmc SQ_INSTS_LDS SQ_LDS_BANK_CONFLICT SQ_LDS_IDX_ACTIVE SQ_LDS_ADDR_CONFLICT
 each written with ``-o run`` (rocpd SQLite databases, run_results.db).
 
 FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are calibrated against the
@@ -13,7 +15,16 @@ plain device copy that ``bench.py --copy-probe`` launches (k_copy, 1 GiB
 read + 1 GiB written): factor = 1 GiB / counter reading of that copy.  The
 table gives, per kernel, the dispatch count, the average duration, the
 calibrated read and written megabytes per dispatch and the resulting
-bandwidth.  bench.py reads the "hbm_bytes" column for its roofline.
+bandwidth, and that bandwidth as a fraction of MI355X's 8 TB/s HBM peak.
+bench.py reads the "hbm_bytes" column for its roofline.  The counters
+count the traffic between L2 and the fabric, so bytes served by the
+Infinity Cache (MALL) are included: a kernel can show more than the copy's
+rate.
+
+The LDS table gives, per kernel, the LDS instructions per dispatch, the
+cycles lost to bank conflicts per LDS instruction and the shares of the
+LDS-active cycles (SQ_LDS_IDX_ACTIVE) spent in bank and in address
+conflicts.
 """
 import collections
 import glob
@@ -83,13 +94,29 @@ def summarize(out, command):
     lines.append("Copy calibration (k_copy, 1 GiB read + 1 GiB written per dispatch): FETCH_SIZE %.0f KiB, "
                  "WRITE_SIZE %.0f KiB; factors fetch x%.3f, write x%.3f." % (cf or 0, cw or 0, kf, kw))
     lines.append("")
-    lines.append("| kernel | dispatches | avg_ms | read_MB | written_MB | hbm_bytes | GB/s |")
-    lines.append("|---|---|---|---|---|---|---|")
-    for k, (n, avg) in sorted(t.items(), key=lambda kv: -kv[1][0] * kv[1][1]):
+    lines.append("| kernel | dispatches | avg_ms | read_MB | written_MB | hbm_bytes | GB/s | frac_8TBs |")
+    lines.append("|---|---|---|---|---|---|---|---|")
+    order = sorted(t.items(), key=lambda kv: -kv[1][0] * kv[1][1])
+    for k, (n, avg) in order:
         rd = f.get(k, {}).get("FETCH_SIZE", 0.0) * kf * 1024.0
         wr = w.get(k, {}).get("WRITE_SIZE", 0.0) * kw * 1024.0
         gbs = (rd + wr) / (avg * 1e-3) / 1e9 if avg > 0 else 0.0
-        lines.append("| %s | %d | %.4f | %.1f | %.1f | %.0f | %.0f |" % (k, n, avg, rd / 1e6, wr / 1e6, rd + wr, gbs))
+        lines.append("| %s | %d | %.4f | %.1f | %.1f | %.0f | %.0f | %.3f |"
+                     % (k, n, avg, rd / 1e6, wr / 1e6, rd + wr, gbs, gbs / 8000.0))
+    lds_dir = os.path.join(out, "lds")
+    if os.path.isdir(lds_dir):
+        l = counters(lds_dir)
+        lines += ["", "LDS counters (per dispatch):", "",
+                  "| kernel | lds_instr | bank_conflict_cycles | conflict_cycles_per_instr | bank_share | addr_share |",
+                  "|---|---|---|---|---|---|"]
+        for k, _ in order:
+            c = l.get(k, {})
+            ni, bc = c.get("SQ_INSTS_LDS", 0.0), c.get("SQ_LDS_BANK_CONFLICT", 0.0)
+            act, ac = c.get("SQ_LDS_IDX_ACTIVE", 0.0), c.get("SQ_LDS_ADDR_CONFLICT", 0.0)
+            if ni <= 0:
+                continue
+            lines.append("| %s | %.0f | %.0f | %.3f | %.3f | %.3f |"
+                         % (k, ni, bc, bc / ni, bc / act if act else 0.0, ac / act if act else 0.0))
     return "\n".join(lines) + "\n"
 
 
@@ -101,7 +128,7 @@ def read_table(path):
     with open(path) as fh:
         for line in fh:
             p = [x.strip() for x in line.strip().strip("|").split("|")]
-            if len(p) != 7 or p[0] in ("kernel", "---"):
+            if len(p) < 7 or p[0] in ("kernel", "---") or p[0].startswith("---"):
                 continue
             try:
                 out[p[0]] = {"dispatches": int(p[1]), "avg_ms": float(p[2]), "read_MB": float(p[3]),

@@ -677,3 +677,37 @@ def test_grdqh2d64_against_fp32_and_planes():
             assert np.abs(g3[0] * (sw64 / ls64) - g64).max() <= 1e-12 * scale
         else:
             assert np.abs(g3[0] * (sw64 / ls64) - g64).max() <= 1e-12 * scale
+
+
+def test_parallel_cpu_port_matches_serial():
+    """The all-cores CPU baseline (oracle/par32.c: visibilities partitioned,
+    per-thread grids summed) against the serial restatement of the same
+    uvmap + vismodel job: images and predictions agree to FP32 round-off,
+    the flags of the prediction exactly; one thread is the serial order."""
+    from sde_amd import synth
+    d = synth.small_field()
+    n = 40000
+    vis, wt, u, v, w = d["vis"][:n], d["wt"][:n].copy(), d["u"][:n], d["v"][:n], d["w"][:n]
+    wt[::13] = -1.0
+    cfg = po.make_cfg(128, cellsize=2 * d["cellsize"], fov=1.0, rmode="NORM")
+    d32, p32, _, rc, _ = po.uvmap(cfg, vis, wt, u, v, w)
+    m32, mw32, _ = po.vismodel(cfg, d32, wt, u, v, w)
+    assert rc == 0
+    one = po.par_uvmap_vismodel(1, cfg, vis, wt, u, v, w)
+    assert one[4] == 0
+    np.testing.assert_array_equal(one[0], d32)
+    np.testing.assert_array_equal(one[1], p32)
+    np.testing.assert_array_equal(one[2], m32)
+    # more threads change the order of the REAL sums: against FP64 the images
+    # stay within twice the serial restatement's error; the prediction from
+    # the parallel dirty image is the serial prediction of that image exactly
+    d64, p64, _, rc64, _ = po.uvmap(cfg, vis, wt, u, v, w, fp64=True)
+    pk = np.abs(d64).max()
+    for nt in (3, 4):
+        dp, pp, mp, mwp, rcp = po.par_uvmap_vismodel(nt, cfg, vis, wt, u, v, w)
+        assert rcp == 0
+        assert np.abs(dp - d64).max() <= 2 * np.abs(d32 - d64).max() + 1e-5 * pk
+        assert np.abs(pp - p64).max() <= 2 * np.abs(p32 - p64).max() + 1e-5
+        ms, mws, _ = po.vismodel(cfg, dp, wt, u, v, w)
+        np.testing.assert_array_equal(mp, ms)
+        np.testing.assert_array_equal(mwp, mws)
