@@ -227,7 +227,8 @@ def accuracy(plan, vis, wt, u, v, w, nx, cell):
     plan's gridded planes taken through FFTR2D + PIXCOR2D in REAL arithmetic
     (oracle/ref32.c) and in DOUBLE (oracle/oracle64.c).  Tolerance: the
     inner half of the field within 1e-5 of the peak, the whole field within
-    that floor + 1e-5 of the peak."""
+    that floor + 1e-5 of the peak; whether the whole field is within 1e-5
+    of the peak outright is reported beside it."""
     from oracle import pyoracle as po
     dirty, psf = plan.images()
     cfg = po.make_cfg(nx, cellsize=cell, fov=1.0, rmode="NORM", robust=(0.0, 0.0, 0.0))
@@ -255,6 +256,8 @@ def accuracy(plan, vis, wt, u, v, w, nx, cell):
              "inner_max_err_rel_peak": float(e[nx // 2 - q:nx // 2 + q, nx // 2 - q:nx // 2 + q].max() / peak),
              "ref32_transform_max_err_rel_peak": floor}
         r["within_tolerance"] = bool(r["inner_max_err_rel_peak"] <= 1e-5 and r["max_err_rel_peak"] <= floor + 1e-5)
+        # the metric's bound read literally: the whole field within 1e-5 of the peak
+        r["whole_field_within_1e-5"] = bool(r["max_err_rel_peak"] <= 1e-5)
         ok = ok and r["within_tolerance"]
         out[name] = r
     out["within_tolerance"] = ok
@@ -605,7 +608,7 @@ def main():
                                   % ("robust NORM R=0" if args.weighting == "robust" else "natural weights"),
                       "nvis": nvis, "nx": nx, "cellsize_arcsec": round(cell, 4), "grid": [plan.nv, plan.nu],
                       "tile": args.tile, "waves": args.waves, "chunk": args.chunk,
-                      "arithmetic": "FP32 visibilities, grids and images; FP64 coordinates and tile sums; cell runs as FP32 partials of at most 8 points summed in FP64"},
+                      "arithmetic": "FP32 visibilities, grids and images in HBM; FP64 coordinates and tile sums; cell runs as FP32 partials of at most 8 points summed in FP64; FFT butterflies in FP64"},
            "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 2),
                           "p99": round(float(np.percentile(lat, 99)) * 1e3, 2)},
            "roofline": roof, "kernels": kernels,

@@ -229,18 +229,18 @@ float conj_delt(int n, float delt_deg) {
 
 double ref_twopi() { return (double)(8.0f * atanf(1.0f)); }
 
-// exp(+2 pi i k / n) in double, exact at the quadrant points.
-std::vector<float2> twiddles(int n) {
-    std::vector<float2> t(n);
+// exp(+2 pi i k / n) in long double rounded to double, exact at the quadrant points.
+std::vector<double2> twiddles(int n) {
+    std::vector<double2> t(n);
     for (int k = 0; k < n; k++) {
-        float2 z;
+        double2 z;
         if (n % 4 == 0 && k % (n / 4) == 0) {
             const int q = k / (n / 4);
-            const float c[4] = {1.0f, 0.0f, -1.0f, 0.0f}, s[4] = {0.0f, 1.0f, 0.0f, -1.0f};
-            z = make_float2(c[q], s[q]);
+            const double c[4] = {1.0, 0.0, -1.0, 0.0}, s[4] = {0.0, 1.0, 0.0, -1.0};
+            z = make_double2(c[q], s[q]);
         } else {
             const long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)n;
-            z = make_float2((float)cosl(a), (float)sinl(a));
+            z = make_double2((double)cosl(a), (double)sinl(a));
         }
         t[k] = z;
     }
@@ -261,7 +261,7 @@ struct Ctx {
     int ndev = 0;
     hipStream_t stream = nullptr;
     std::map<std::string, DevBuf> bufs;
-    std::map<int, float2 *> tw;
+    std::map<int, double2 *> tw;
 };
 
 std::mutex g_ctx_mu;
@@ -295,16 +295,16 @@ T *ctx_buf(Ctx &c, const char *name, size_t count) {
     return static_cast<T *>(b.p);
 }
 
-int get_twiddles(std::map<int, float2 *> &cache, int n, hipStream_t s, float2 **out) {
+int get_twiddles(std::map<int, double2 *> &cache, int n, hipStream_t s, double2 **out) {
     auto it = cache.find(n);
     if (it != cache.end()) {
         *out = it->second;
         return 0;
     }
-    std::vector<float2> t = twiddles(n);
-    float2 *d = nullptr;
-    SDE_CHECK(hipMalloc(&d, sizeof(float2) * n));
-    SDE_CHECK(hipMemcpyAsync(d, t.data(), sizeof(float2) * n, hipMemcpyHostToDevice, s));
+    std::vector<double2> t = twiddles(n);
+    double2 *d = nullptr;
+    SDE_CHECK(hipMalloc(&d, sizeof(double2) * n));
+    SDE_CHECK(hipMemcpyAsync(d, t.data(), sizeof(double2) * n, hipMemcpyHostToDevice, s));
     SDE_CHECK(hipStreamSynchronize(s));
     cache[n] = d;
     *out = d;
@@ -1186,7 +1186,7 @@ void fftr2d_(float *rarray, float *xarray, float *work, float *work2, float *rwo
     const size_t ng = (size_t)nu * *nv, npix = (size_t)*nx * *ny;
     float2 *dg = ctx_buf<float2>(c, "fft_grid", ng);
     float *dimg = ctx_buf<float>(c, "fft_img", npix);
-    float2 *tw_nv = nullptr, *tw_fnu = nullptr;
+    double2 *tw_nv = nullptr, *tw_fnu = nullptr;
     if (!dg || !dimg || get_twiddles(c.tw, *nv, s, &tw_nv) || get_twiddles(c.tw, *fnu, s, &tw_fnu)) {
         report_error(kErrNoMem, R, "device allocation failed");
         return;
@@ -1223,7 +1223,7 @@ void fftx1d_(float *xarray, float *xwork, float *xwork2, int *nx, int *dir) {
     if (ctx_init(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
     hipStream_t s = c.stream;
     float2 *dg = ctx_buf<float2>(c, "fft_grid", *nx);
-    float2 *tw = nullptr;
+    double2 *tw = nullptr;
     if (!dg || get_twiddles(c.tw, *nx, s, &tw)) {
         report_error(kErrNoMem, R, "device allocation failed");
         return;
@@ -1256,7 +1256,7 @@ struct sde_plan {
     float *d_time = nullptr; // TIME of the resident visibilities (selection)
     int64_t time_n = -1;     // visibilities TIME was set for (-1: none)
     size_t time_cap = 0;
-    std::map<int, float2 *> tw;
+    std::map<int, double2 *> tw;
     float *d_cf = nullptr, *d_ccf = nullptr;
     int64_t nvis = 0, cap = 0;
     float2 *d_vis = nullptr;
@@ -1451,7 +1451,7 @@ void plan_set_geometry(sde_plan *p, float dx, float dy) {
 
 int plan_vismodel(sde_plan *p, const float *img) {
     hipStream_t s = p->stream;
-    float2 *twnv = p->tw[p->nv], *twfnu = p->tw[p->fnu];
+    double2 *twnv = p->tw[p->nv], *twfnu = p->tw[p->fnu];
     plan_mark_begin(p, "fft_rows_model");
     launch_fft_rows_r2c(s, img, p->cfg.nx, p->cfg.ny, p->fnu, p->nv, twfnu, p->d_ccf, p->ccflen, 1, p->d_mgrid);
     plan_mark_end(p);
@@ -1674,7 +1674,7 @@ int plan_image(sde_plan *p, const WeightParams &wp) {
 int plan_transform(sde_plan *p) {
     hipStream_t s = p->stream;
     const size_t ng = (size_t)p->nu * p->nv;
-    float2 *twnv = p->tw[p->nv], *twfnu = p->tw[p->fnu];
+    double2 *twnv = p->tw[p->nv], *twfnu = p->tw[p->fnu];
     plan_mark_begin(p, "fft_cols");
     // out of place: the gridded planes stay readable (sde_plan_get_grids)
     launch_fft_cols(s, p->d_grids, p->d_gwork, p->nu, p->nv, 2, ng, -1, 1.0f, twnv);
@@ -1773,7 +1773,7 @@ int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **out) {
         sde_plan_destroy(p);
         return -1;
     }
-    float2 *t1, *t2;
+    double2 *t1, *t2;
     if (get_twiddles(p->tw, p->nv, p->stream, &t1) || get_twiddles(p->tw, p->fnu, p->stream, &t2)) {
         sde_plan_destroy(p);
         return -1;
@@ -1954,7 +1954,7 @@ int64_t sde_plan_device_bytes(sde_plan *p) {
     if (!p) return -1;
     size_t b = 0;
     for (const auto &kv : p->alloc_bytes) b += kv.second;
-    for (const auto &kv : p->tw) b += sizeof(float2) * (size_t)kv.first;
+    for (const auto &kv : p->tw) b += sizeof(double2) * (size_t)kv.first;
     return (int64_t)b;
 }
 
@@ -1968,7 +1968,7 @@ int64_t sde_plan_estimate_bytes(const sde_imaging_cfg *cfg, int64_t nvis) {
                                   cfg->chunk > 0 ? cfg->chunk : default_chunk(nvis), cfg->waves ? cfg->waves : 4);
     size_t b = sizeof(float) * (1 + 2 * 4 * 128 + (size_t)ccflen) + sizeof(float2) * 5 * ng +
                sizeof(float) * 3 * npix + sizeof(double) * ng + sizeof(double) * (4096 + kWtSqBlocks + 8) +
-               sizeof(int) * (2 * ((size_t)tp.ntiles + 1) + 1 + kPrepBlocks + 4) + sizeof(float2) * (size_t)(nv + fnu);
+               sizeof(int) * (2 * ((size_t)tp.ntiles + 1) + 1 + kPrepBlocks + 4) + sizeof(double2) * (size_t)(nv + fnu);
     if (nvis > 0) {
         const size_t n = (size_t)nvis;
         b += n * (sizeof(float2) * 2 + sizeof(float) * 6 + sizeof(uint32_t) * 2 + sizeof(uint64_t) * 2 +
@@ -2045,7 +2045,7 @@ int sde_vistoimg_slowz_device(const void *vis, const float *wt, const float *u, 
     const int ncf = 1 + 2 * (supp + 1) * osamp;
     float *dcf = ctx_buf<float>(c, "cfz", ncf), *dccf = ctx_buf<float>(c, "ccfz", ccflen);
     float2 *work = ctx_buf<float2>(c, "planework", (size_t)nu * nv);
-    float2 *twnv = nullptr, *twfnu = nullptr;
+    double2 *twnv = nullptr, *twfnu = nullptr;
     if (!dcf || !dccf || !work || get_twiddles(c.tw, nv, s, &twnv) || get_twiddles(c.tw, fnu, s, &twfnu)) {
         set_plan_error("VISTOIMG: device allocation failed");
         return -1;
@@ -2111,7 +2111,7 @@ int64_t sde_imgtovis_slowz_device(const float *images, int nx, int ny, int fnu, 
     hipStream_t s = c.stream;
     const int ncf = 1 + 2 * (supp + 1) * osamp;
     float *dcf = ctx_buf<float>(c, "cfz", ncf), *dccf = ctx_buf<float>(c, "ccfz", ccflen);
-    float2 *twnv = nullptr, *twfnu = nullptr;
+    double2 *twnv = nullptr, *twfnu = nullptr;
     if (!dcf || !dccf || get_twiddles(c.tw, nv, s, &twnv) || get_twiddles(c.tw, fnu, s, &twfnu)) {
         set_plan_error("VISDEGRI: device allocation failed");
         return -1;

@@ -4,10 +4,17 @@
 // (PIXCOR2D, pix/pixcor2d.f:44-68) and the uvmap peak normalisation
 // (task/uvmap.f:434-440,480-482).
 //
-// Transforms are LDS-resident in-place Stockham FFTs: radix-4 stages (one
-// radix-2 lead stage when log2 N is odd), each stage reading its operands to
-// registers, a barrier, and writing them back in autosorted order.  Twiddles
-// come from a per-size table exp(+2 pi i k / N) built in double on the host.
+// Transforms are LDS-resident in-place Stockham FFTs computed in FP64: the
+// operands are FP32 in HBM (as the reference's COMPLEX arrays), widened on
+// the way into LDS and rounded once on the way out, so the transform adds
+// no FP32 round-off of its own (the grid correction amplifies the field
+// edge by up to ~1e5, where FP32 butterflies would dominate the error).
+// Radix-8 stages (a radix-2 or radix-4 lead stage for the remaining
+// factor), each stage reading its operands to registers, a barrier, and
+// writing them back in autosorted order; the LDS index is padded by one
+// element in eight, so the stride-8 writes of the first stage and the
+// contiguous accesses of the others fall on distinct banks.  Twiddles come
+// from a per-size table exp(+2 pi i k / N) built in long double on the host.
 //   * column pass: C2C of length NV on CB adjacent u columns per workgroup;
 //     the centring (element (NV+1)/2 <-> index 0) is folded into the LDS
 //     load/store index.
@@ -15,87 +22,148 @@
 //     through one FNU/2-point complex transform; centring, crop to NX,
 //     1/(FNU*NV), grid correction and peak tracking / division fused in.
 //   * row pass, DIR > 0: image row (corrected, padded, centred) -> half row.
+#include <mutex>
+
 #include "internal.h"
 
 namespace {
 
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+// a * (sg * i)
+__device__ __forceinline__ double2 cmuli(double2 a, double sg) { return make_double2(-sg * a.y, sg * a.x); }
+
+// LDS index of element i: one pad element after every eight
+__host__ __device__ __forceinline__ int lpad(int i) { return i + (i >> 3); }
+__host__ __device__ __forceinline__ int lds_len(int n) { return lpad(n - 1) + 1; }
+
+// 4-point DFT in place, exp(sg*i*2pi/4) = sg*i
+__device__ __forceinline__ void dft4(double2 &a0, double2 &a1, double2 &a2, double2 &a3, double sg) {
+    const double2 s0 = cadd(a0, a2), s1 = csub(a0, a2);
+    const double2 s2 = cadd(a1, a3), s3 = cmuli(csub(a1, a3), sg);
+    a0 = cadd(s0, s2);
+    a2 = csub(s0, s2);
+    a1 = cadd(s1, s3);
+    a3 = csub(s1, s3);
 }
 
-// In-place Stockham FFT of n = 2^log2n points in LDS, run by nthr threads
-// (thread index tid).  sign = +1: exp(+i), sign = -1: exp(-i); unnormalised.
-// tw is the table exp(+2 pi i k / (n*tws)).  Requires n <= 16 * nthr * 4.
-__device__ void lds_fft(float2 *x, int n, int log2n, int tid, int nthr, const float2 *__restrict__ tw,
+// 8-point DFT in place: A[k] = sum_j a[j] exp(sg*i*2pi*j*k/8)
+__device__ __forceinline__ void dft8(double2 *a, double sg) {
+    constexpr double h = 0.70710678118654752440;
+    double2 t[4], u[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        t[j] = cadd(a[j], a[j + 4]);
+        u[j] = csub(a[j], a[j + 4]);
+    }
+    // u[j] * exp(sg*i*pi*j/4)
+    u[1] = make_double2(h * (u[1].x - sg * u[1].y), h * (u[1].y + sg * u[1].x));
+    u[2] = cmuli(u[2], sg);
+    u[3] = make_double2(h * (-u[3].x - sg * u[3].y), h * (-u[3].y + sg * u[3].x));
+    dft4(t[0], t[1], t[2], t[3], sg);
+    dft4(u[0], u[1], u[2], u[3], sg);
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        a[2 * m] = t[m];
+        a[2 * m + 1] = u[m];
+    }
+}
+
+// twiddle exp(sg*2 pi i m / tabn) from the table of exp(+2 pi i k / tabn)
+__device__ __forceinline__ double2 twid(const double2 *__restrict__ tw, int m, double sg) {
+    const double2 w = tw[m];
+    return make_double2(w.x, sg * w.y);
+}
+
+// In-place Stockham FFT of n = 2^log2n points in LDS (element i at lpad(i)),
+// run by nthr threads (thread index tid).  sign = +1: exp(+i), sign = -1:
+// exp(-i); unnormalised.  tw is the table exp(+2 pi i k / (n*tws)).
+// Requires n <= 16 * nthr.
+__device__ __forceinline__ void lds_fft(double2 *x, int n, int log2n, int tid, int nthr, const double2 *__restrict__ tw,
                         int tws, int sign) {
-    constexpr int JM = 4;  // radix-4 butterflies per thread per stage (n <= 16 * nthr)
-    const float sg = (float)sign;
+    constexpr int JM = 4;  // butterflies per thread in a radix-2 or radix-4 lead stage
+    constexpr int J8 = 2;  // radix-8 butterflies per thread per stage
+    const double sg = (double)sign;
     const int tabn = n * tws;
     int ns = 1;
-    if (log2n & 1) {
-        // radix-2 stage, Ns = 1: butterflies (j, j + n/2) -> (2j, 2j + 1)
-        float2 a[2 * JM], b[2 * JM];
+    const int lead = log2n % 3;
+    if (lead == 1 && n >= 2) {
+        // radix-2 stage, Ns = 1: (j, j + n/2) -> (2j, 2j + 1)
+        const int q = n / 2;
+        double2 a[2 * JM], b[2 * JM];
 #pragma unroll
         for (int c = 0; c < 2 * JM; c++) {
             const int j = tid + c * nthr;
-            if (j < n / 2) {
-                a[c] = x[j];
-                b[c] = x[j + n / 2];
+            if (j < q) {
+                a[c] = x[lpad(j)];
+                b[c] = x[lpad(j + q)];
             }
         }
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < 2 * JM; c++) {
             const int j = tid + c * nthr;
-            if (j < n / 2) {
-                x[2 * j] = make_float2(a[c].x + b[c].x, a[c].y + b[c].y);
-                x[2 * j + 1] = make_float2(a[c].x - b[c].x, a[c].y - b[c].y);
+            if (j < q) {
+                x[lpad(2 * j)] = cadd(a[c], b[c]);
+                x[lpad(2 * j + 1)] = csub(a[c], b[c]);
             }
         }
         __syncthreads();
         ns = 2;
-    }
-    for (; ns < n; ns *= 4) {
-        float2 v0[JM], v1[JM], v2[JM], v3[JM];
+    } else if (lead == 2) {
+        // radix-4 stage, Ns = 1: (j, j + q, j + 2q, j + 3q) -> 4j .. 4j + 3
         const int q = n / 4;
-        const int step = tabn / (ns * 4);
+        double2 v[JM][4];
 #pragma unroll
         for (int c = 0; c < JM; c++) {
             const int j = tid + c * nthr;
             if (j < q) {
-                const int k = j & (ns - 1);
-                float2 a0 = x[j], a1 = x[j + q], a2 = x[j + 2 * q], a3 = x[j + 3 * q];
-                if (ns > 1) {
-                    float2 w1 = tw[k * step], w2 = tw[2 * k * step], w3 = tw[3 * k * step];
-                    w1.y *= sg;
-                    w2.y *= sg;
-                    w3.y *= sg;
-                    a1 = cmul(a1, w1);
-                    a2 = cmul(a2, w2);
-                    a3 = cmul(a3, w3);
-                }
-                const float2 b0 = make_float2(a0.x + a2.x, a0.y + a2.y);
-                const float2 b1 = make_float2(a0.x - a2.x, a0.y - a2.y);
-                const float2 b2 = make_float2(a1.x + a3.x, a1.y + a3.y);
-                const float2 d = make_float2(a1.x - a3.x, a1.y - a3.y);
-                const float2 b3 = make_float2(-sg * d.y, sg * d.x);  // d * (sign * i)
-                v0[c] = make_float2(b0.x + b2.x, b0.y + b2.y);
-                v2[c] = make_float2(b0.x - b2.x, b0.y - b2.y);
-                v1[c] = make_float2(b1.x + b3.x, b1.y + b3.y);
-                v3[c] = make_float2(b1.x - b3.x, b1.y - b3.y);
+#pragma unroll
+                for (int r = 0; r < 4; r++) v[c][r] = x[lpad(j + r * q)];
+                dft4(v[c][0], v[c][1], v[c][2], v[c][3], sg);
             }
         }
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < JM; c++) {
             const int j = tid + c * nthr;
+            if (j < q)
+#pragma unroll
+                for (int r = 0; r < 4; r++) x[lpad(4 * j + r)] = v[c][r];
+        }
+        __syncthreads();
+        ns = 4;
+    }
+    for (; ns < n; ns *= 8) {
+        const int q = n / 8;
+        const int step = tabn / (ns * 8);
+        double2 v[J8][8];
+#pragma unroll
+        for (int c = 0; c < J8; c++) {
+            const int j = tid + c * nthr;
             if (j < q) {
                 const int k = j & (ns - 1);
-                const int d0 = (j - k) * 4 + k;
-                x[d0] = v0[c];
-                x[d0 + ns] = v1[c];
-                x[d0 + 2 * ns] = v2[c];
-                x[d0 + 3 * ns] = v3[c];
+#pragma unroll
+                for (int r = 0; r < 8; r++) v[c][r] = x[lpad(j + r * q)];
+                if (ns > 1) {
+#pragma unroll
+                    for (int r = 1; r < 8; r++) v[c][r] = cmul(v[c][r], twid(tw, r * k * step, sg));
+                }
+                dft8(v[c], sg);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < J8; c++) {
+            const int j = tid + c * nthr;
+            if (j < q) {
+                const int k = j & (ns - 1);
+                const int d0 = (j - k) * 8 + k;
+#pragma unroll
+                for (int r = 0; r < 8; r++) x[lpad(d0 + r * ns)] = v[c][r];
             }
         }
         __syncthreads();
@@ -108,33 +176,36 @@ __device__ __forceinline__ int ilog2(int n) { return 31 - __clz(n); }
 // src and dst may be the same array (in place): a block reads all of its
 // columns into LDS before it writes any.
 __global__ void k_fft_cols(const float2 *srcs, float2 *dsts, int nu, int nv, size_t stride, int cb,
-                           int tpc, int dir, float scale, const float2 *__restrict__ tw) {
-    extern __shared__ float2 xs[];
+                           int tpc, int dir, float scale, const double2 *__restrict__ tw) {
+    extern __shared__ double2 xs[];
     const float2 *grid = srcs + stride * blockIdx.y;
     float2 *out = dsts + stride * blockIdx.y;
     // adjacent column groups (which share each row's cache lines) on one XCD
     const int u0 = xcd_swizzle(blockIdx.x, gridDim.x) * cb;
     const int icen = (nv + 1) / 2 - 1;  // 0-based centre row
     const int nthr = blockDim.x;
+    const int ln = lds_len(nv);
     for (int idx = threadIdx.x; idx < cb * nv; idx += nthr) {
         const int vrow = idx / cb, c = idx - vrow * cb;
         const int u = u0 + c;
         int p = vrow - icen;
         p += p < 0 ? nv : 0;
-        xs[c * nv + p] = u < nu ? grid[(size_t)vrow * nu + u] : make_float2(0.0f, 0.0f);
+        const float2 g = u < nu ? grid[(size_t)vrow * nu + u] : make_float2(0.0f, 0.0f);
+        xs[c * ln + lpad(p)] = make_double2((double)g.x, (double)g.y);
     }
     __syncthreads();
     const int c = threadIdx.x / tpc;
-    lds_fft(xs + c * nv, nv, ilog2(nv), threadIdx.x - c * tpc, tpc, tw, 1, dir < 0 ? +1 : -1);
+    lds_fft(xs + c * ln, nv, ilog2(nv), threadIdx.x - c * tpc, tpc, tw, 1, dir < 0 ? +1 : -1);
+    const double sc = (double)scale;
     for (int idx = threadIdx.x; idx < cb * nv; idx += nthr) {
         const int vrow = idx / cb, cc = idx - vrow * cb;
         const int u = u0 + cc;
         int p = vrow - icen;
         p += p < 0 ? nv : 0;
         if (u < nu) {
-            float2 r = xs[cc * nv + p];
-            if (scale != 1.0f) r = make_float2(r.x * scale, r.y * scale);
-            out[(size_t)vrow * nu + u] = r;
+            double2 r = xs[cc * ln + lpad(p)];
+            if (scale != 1.0f) r = make_double2(r.x * sc, r.y * sc);
+            out[(size_t)vrow * nu + u] = make_float2((float)r.x, (float)r.y);
         }
     }
 }
@@ -158,12 +229,12 @@ __device__ __forceinline__ int icf(float s, int i1, float cen) {
 // Grid rows -> image rows (DIR < 0).
 // blockIdx.y = 1 transforms grid_b into image_b (no peak) in the same launch.
 __global__ void k_fft_rows_c2r(const float2 *__restrict__ grid_a, int nu, int nv, int fnu, int nx,
-                               int ny, int ibegy, const float2 *__restrict__ tw,
+                               int ny, int ibegy, const double2 *__restrict__ tw,
                                const float *__restrict__ ccf, int correct, CorrParams cpar,
                                float scale, float *__restrict__ image_a, int *__restrict__ peak_a,
                                const int *__restrict__ div_key, const float2 *__restrict__ grid_b,
                                float *__restrict__ image_b) {
-    extern __shared__ float2 zs[];
+    extern __shared__ double2 zs[];
     __shared__ float red[1024];
     const bool second = blockIdx.y == 1;
     const float2 *grid = second ? grid_b : grid_a;
@@ -174,17 +245,17 @@ __global__ void k_fft_rows_c2r(const float2 *__restrict__ grid_a, int nu, int nv
     const int h = fnu / 2;
     const float2 *row = grid + (size_t)rv * nu;
     for (int k = threadIdx.x; k < h; k += blockDim.x) {
-        float2 a = row[k], b = row[h - k];
+        const float2 af = row[k], bf = row[h - k];
+        double2 a = make_double2((double)af.x, (double)af.y), b = make_double2((double)bf.x, (double)bf.y);
         if (k == 0) {
-            a.y = 0.0f;
-            b.y = 0.0f;
+            a.y = 0.0;
+            b.y = 0.0;
         }
         b.y = -b.y;  // conj(X[H-k])
-        const float2 s = make_float2(a.x + b.x, a.y + b.y);
-        const float2 d = make_float2(a.x - b.x, a.y - b.y);
-        const float2 e = tw[k];
-        const float2 ed = cmul(e, d);
-        zs[k] = make_float2(s.x - ed.y, s.y + ed.x);  // s + i*e*d
+        const double2 s = cadd(a, b);
+        const double2 d = csub(a, b);
+        const double2 ed = cmul(tw[k], d);
+        zs[lpad(k)] = make_double2(s.x - ed.y, s.y + ed.x);  // s + i*e*d
     }
     __syncthreads();
     lds_fft(zs, h, ilog2(h), threadIdx.x, blockDim.x, tw, 2, +1);
@@ -197,8 +268,8 @@ __global__ void k_fft_rows_c2r(const float2 *__restrict__ grid_a, int nu, int nv
     for (int ix = threadIdx.x; ix < nx; ix += blockDim.x) {
         int p = ioff + ix + 1 - icen;
         p += p < 0 ? fnu : 0;
-        const float2 z = zs[p >> 1];
-        float val = ((p & 1) ? z.y : z.x) * scale;
+        const double2 z = zs[lpad(p >> 1)];
+        float val = (float)(((p & 1) ? z.y : z.x) * (double)scale);
         if (correct) val = val / (ccf[icf(cpar.sx, ix + 1, cpar.cenx)] * cy);
         if (div_key) val = inv * val;
         image[(size_t)iy * nx + ix] = val;
@@ -217,10 +288,10 @@ __global__ void k_fft_rows_c2r(const float2 *__restrict__ grid_a, int nu, int nv
 
 // Image rows -> grid rows (DIR > 0); rows outside the image are zeroed.
 __global__ void k_fft_rows_r2c(const float *__restrict__ image, int nx, int ny, int fnu, int nu,
-                               int ibegy, const float2 *__restrict__ tw,
+                               int ibegy, const double2 *__restrict__ tw,
                                const float *__restrict__ ccf, int correct, CorrParams cpar,
                                float2 *__restrict__ grid) {
-    extern __shared__ float2 zs[];
+    extern __shared__ double2 zs[];
     const int rv = blockIdx.x;
     const int iy = rv - (ibegy - 1);
     float2 *row = grid + (size_t)rv * nu;
@@ -229,8 +300,9 @@ __global__ void k_fft_rows_r2c(const float *__restrict__ image, int nx, int ny, 
         for (int k = threadIdx.x; k < nu; k += blockDim.x) row[k] = make_float2(0.0f, 0.0f);
         return;
     }
-    float *zr = reinterpret_cast<float *>(zs);
-    for (int k = threadIdx.x; k < fnu; k += blockDim.x) zr[k] = 0.0f;
+    // real element p of the FNU-point row is component p & 1 of complex element p >> 1
+    double *zr = reinterpret_cast<double *>(zs);
+    for (int k = threadIdx.x; k < lds_len(h); k += blockDim.x) zs[k] = make_double2(0.0, 0.0);
     __syncthreads();
     const int icen = (fnu + 1) / 2, ioff = (fnu - nx) / 2;
     const int icy = correct ? icf(cpar.sy, iy + 1, cpar.ceny) : 0;
@@ -240,26 +312,26 @@ __global__ void k_fft_rows_r2c(const float *__restrict__ image, int nx, int ny, 
         if (correct) val = val / (ccf[icf(cpar.sx, ix + 1, cpar.cenx)] * cy);
         int p = ioff + ix + 1 - icen;
         p += p < 0 ? fnu : 0;
-        zr[p] = val;
+        zr[2 * lpad(p >> 1) + (p & 1)] = (double)val;
     }
     __syncthreads();
     lds_fft(zs, h, ilog2(h), threadIdx.x, blockDim.x, tw, 2, -1);
     for (int k = threadIdx.x; k <= h; k += blockDim.x) {
-        float2 x;
+        double2 x;
         if (k == 0 || k == h) {
-            const float2 z0 = zs[0];
-            x = make_float2(k == 0 ? z0.x + z0.y : z0.x - z0.y, 0.0f);
+            const double2 z0 = zs[0];
+            x = make_double2(k == 0 ? z0.x + z0.y : z0.x - z0.y, 0.0);
         } else {
-            const float2 zk = zs[k], zc = zs[h - k];
-            const float2 e = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y - zc.y));  // (Z + conj Zc)/2
-            const float2 dd = make_float2(0.5f * (zk.x - zc.x), 0.5f * (zk.y + zc.y)); // (Z - conj Zc)/2
-            const float2 o = make_float2(dd.y, -dd.x);                                 // dd / i
-            float2 t = tw[k];
+            const double2 zk = zs[lpad(k)], zc = zs[lpad(h - k)];
+            const double2 e = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y - zc.y));   // (Z + conj Zc)/2
+            const double2 dd = make_double2(0.5 * (zk.x - zc.x), 0.5 * (zk.y + zc.y));  // (Z - conj Zc)/2
+            const double2 o = make_double2(dd.y, -dd.x);                                // dd / i
+            double2 t = tw[k];
             t.y = -t.y;  // exp(-2 pi i k / fnu)
-            const float2 to = cmul(t, o);
-            x = make_float2(e.x + to.x, e.y + to.y);
+            const double2 to = cmul(t, o);
+            x = make_double2(e.x + to.x, e.y + to.y);
         }
-        row[k] = x;
+        row[k] = make_float2((float)x.x, (float)x.y);
     }
 }
 
@@ -323,54 +395,93 @@ int row_threads(int h) {
     return t;
 }
 
+// Dynamic LDS above 64 KB needs the kernels' opt-in: done once (for every
+// host thread, before any of them launches), up to the device's limit less
+// each kernel's static LDS.  Returns the dynamic LDS every FFT kernel may use.
+size_t fft_lds_limit() {
+    static std::once_flag once;
+    static size_t lim = 0;
+    std::call_once(once, [] {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+            v = 65536;
+        size_t m = (size_t)v;
+        const void *fns[3] = {(const void *)k_fft_cols, (const void *)k_fft_rows_c2r, (const void *)k_fft_rows_r2c};
+        for (const void *f : fns) {
+            hipFuncAttributes a;
+            size_t st = 0;
+            if (hipFuncGetAttributes(&a, f) == hipSuccess) st = a.sharedSizeBytes;
+            size_t dyn = (size_t)v > st ? (size_t)v - st : 0;
+            if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) != hipSuccess) {
+                (void)hipGetLastError();  // not opted in: the default limit
+                dyn = 65536 > st ? 65536 - st : 0;
+            }
+            if (dyn < m) m = dyn;
+        }
+        lim = m;
+    });
+    return lim;
+}
+
 }  // namespace
 
-int fft_supported(int n) { return n >= 2 && n <= 8192 && (n & (n - 1)) == 0; }
+int fft_supported(int n) {
+    if (!(n >= 2 && n <= 8192 && (n & (n - 1)) == 0)) return 0;
+    // the largest transform a workgroup holds: one column or half row of n points in FP64
+    return sizeof(double2) * (size_t)lds_len(n) <= fft_lds_limit();
+}
 
 void launch_fft_cols(hipStream_t s, const float2 *src, float2 *dst, int nu, int nv, int ngrids,
-                     size_t grid_stride, int dir, float scale, const float2 *tw_nv) {
+                     size_t grid_stride, int dir, float scale, const double2 *tw_nv) {
+    // 8 points per butterfly, at most 4 butterflies per thread and stage
     int tpc = nv / 16;
     if (tpc < 64) tpc = 64;
-    int cb = 65536 / (nv * (int)sizeof(float2));
+    // columns per workgroup within 74 KB of LDS (two workgroups per CU at NV = 4096)
+    int cb = 75776 / (lds_len(nv) * (int)sizeof(double2));
     if (cb < 1) cb = 1;
     if (cb > 8) cb = 8;
     while (cb > 1 && cb * tpc > 1024) cb /= 2;
-    const size_t lds = sizeof(float2) * (size_t)cb * nv;
+    const size_t lds = sizeof(double2) * (size_t)cb * lds_len(nv);
+    (void)fft_lds_limit();
     dim3 g((nu + cb - 1) / cb, ngrids);
     hipLaunchKernelGGL(k_fft_cols, g, dim3(cb * tpc), lds, s, src, dst, nu, nv, grid_stride, cb, tpc,
                        dir, scale, tw_nv);
 }
 
 void launch_fft_rows_c2r(hipStream_t s, const float2 *grid, int nu, int nv, int fnu, int nx,
-                         int ny, const float2 *tw_fnu, const float *ccf, int ccflen,
+                         int ny, const double2 *tw_fnu, const float *ccf, int ccflen,
                          int correct, float scale, float *image, int *peak_key,
                          const int *div_key) {
     const int h = fnu / 2;
     const int ibegy = (nv + 1) / 2 - (ny + 1) / 2 + 1;
     CorrParams cp = make_corr(nx, ny, ccflen, (float)((nx + 1) / 2), (float)((ny + 1) / 2));
-    hipLaunchKernelGGL(k_fft_rows_c2r, dim3(ny), dim3(row_threads(h)), sizeof(float2) * h, s, grid,
+    (void)fft_lds_limit();
+    hipLaunchKernelGGL(k_fft_rows_c2r, dim3(ny), dim3(row_threads(h)), sizeof(double2) * lds_len(h), s, grid,
                        nu, nv, fnu, nx, ny, ibegy, tw_fnu, ccf, correct, cp, scale, image, peak_key,
                        div_key, (const float2 *)nullptr, (float *)nullptr);
 }
 
 void launch_fft_rows_c2r_pair(hipStream_t s, const float2 *grid_a, float *image_a, int *peak_a,
                               const float2 *grid_b, float *image_b, int nu, int nv, int fnu, int nx, int ny,
-                              const float2 *tw_fnu, const float *ccf, int ccflen, int correct, float scale) {
+                              const double2 *tw_fnu, const float *ccf, int ccflen, int correct, float scale) {
     const int h = fnu / 2;
     const int ibegy = (nv + 1) / 2 - (ny + 1) / 2 + 1;
     CorrParams cp = make_corr(nx, ny, ccflen, (float)((nx + 1) / 2), (float)((ny + 1) / 2));
-    hipLaunchKernelGGL(k_fft_rows_c2r, dim3(ny, 2), dim3(row_threads(h)), sizeof(float2) * h, s, grid_a, nu, nv,
+    (void)fft_lds_limit();
+    hipLaunchKernelGGL(k_fft_rows_c2r, dim3(ny, 2), dim3(row_threads(h)), sizeof(double2) * lds_len(h), s, grid_a, nu, nv,
                        fnu, nx, ny, ibegy, tw_fnu, ccf, correct, cp, scale, image_a, peak_a, (const int *)nullptr,
                        grid_b, image_b);
 }
 
 void launch_fft_rows_r2c(hipStream_t s, const float *image, int nx, int ny, int fnu, int nv,
-                         const float2 *tw_fnu, const float *ccf, int ccflen, int correct,
+                         const double2 *tw_fnu, const float *ccf, int ccflen, int correct,
                          float2 *grid) {
     const int h = fnu / 2, nu = h + 1;
     const int ibegy = (nv + 1) / 2 - (ny + 1) / 2 + 1;
     CorrParams cp = make_corr(nx, ny, ccflen, (float)((nx + 1) / 2), (float)((ny + 1) / 2));
-    hipLaunchKernelGGL(k_fft_rows_r2c, dim3(nv), dim3(row_threads(h)), sizeof(float2) * h, s, image,
+    (void)fft_lds_limit();
+    hipLaunchKernelGGL(k_fft_rows_r2c, dim3(nv), dim3(row_threads(h)), sizeof(double2) * lds_len(h), s, image,
                        nx, ny, fnu, nu, ibegy, tw_fnu, ccf, correct, cp, grid);
 }
 

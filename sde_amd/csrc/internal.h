@@ -306,21 +306,21 @@ void launch_box_totals(hipStream_t s, const double *blk, int nblk, const double 
                        const CoordParams &cp, double *sums, int *peak_reset);
 
 // ---- FFT (fft.hip) ----
-// twiddle table: tw[k] = exp(+2 pi i k / n), k < n, as float2
+// twiddle table: tw[k] = exp(+2 pi i k / n), k < n, as double2
 int fft_supported(int n);
 // scale multiplies the column output (1 in the fused pipeline, 1/NV for FFTR2D);
 // dst may equal src (in place)
 void launch_fft_cols(hipStream_t s, const float2 *src, float2 *dst, int nu, int nv, int ngrids,
-                     size_t grid_stride, int dir, float scale, const float2 *tw_nv);
+                     size_t grid_stride, int dir, float scale, const double2 *tw_nv);
 // DIR < 0 rows: grid rows -> image (NX,NY) with centring, crop, correction,
 // scale, optional peak tracking and optional division by a device peak.
 void launch_fft_rows_c2r(hipStream_t s, const float2 *grid, int nu, int nv, int fnu, int nx,
-                         int ny, const float2 *tw_fnu, const float *ccf, int ccflen,
+                         int ny, const double2 *tw_fnu, const float *ccf, int ccflen,
                          int correct, float scale, float *image, int *peak_key,
                          const int *div_key);
 // DIR > 0 rows: image -> grid rows (zero rows outside the image), optional correction.
 void launch_fft_rows_r2c(hipStream_t s, const float *image, int nx, int ny, int fnu, int nv,
-                         const float2 *tw_fnu, const float *ccf, int ccflen, int correct,
+                         const double2 *tw_fnu, const float *ccf, int ccflen, int correct,
                          float2 *grid);
 // image (and image_b when given) times 1/peak
 void launch_scale_by_peak(hipStream_t s, float *image, size_t n, const int *peak_key, float *image_b = nullptr);
@@ -328,7 +328,7 @@ void launch_scale_by_peak(hipStream_t s, float *image, size_t n, const int *peak
 // peak_a, grid_b -> image_b.
 void launch_fft_rows_c2r_pair(hipStream_t s, const float2 *grid_a, float *image_a, int *peak_a,
                               const float2 *grid_b, float *image_b, int nu, int nv, int fnu, int nx, int ny,
-                              const float2 *tw_fnu, const float *ccf, int ccflen, int correct, float scale);
+                              const double2 *tw_fnu, const float *ccf, int ccflen, int correct, float scale);
 void launch_pixcor(hipStream_t s, const float *in, float *out, int nx, int ny, int cplx,
                    const float *cf, int cflen, int correct, float cenx, float ceny);
 
