@@ -36,7 +36,7 @@ PEAK_HBM_GBS = 8000.0
 # a name ending in "*" matches every kernel with that prefix).
 REGION_KERNELS = {
     "prep": ["k_prep<2, true>"],
-    "cell_sort": ["rocprim::*"],
+    "cell_sort": ["k_sort_init", "k_sort_hist", "k_sort_scan", "k_sort_pass"],
     # the density runs also write the tile ranges; the work items follow in the same region
     "wt_density": ["k_density_runs", "k_items_scan"],
     "wt_finish": ["k_wt_sumsq", "k_wt_finish"],

@@ -398,6 +398,14 @@ int sde_grdh2d_device_noalloc(const void *vis, const float *wt, const float *u, 
                               const double *shift, double *sumwt);
 size_t sde_noalloc_scratch_bytes(void);
 
+/* The library's stable key/value radix sort on device arrays (bits
+ * [0, nbits) of the keys order them; 1 <= nbits <= 32, 0 <= n < 2^31), as the cell sort
+ * runs it; synchronous.  Returns 0, or -1 with sde_last_error() set.  No
+ * counterpart in the reference (its gridding does not sort); for tests and
+ * callers that sort their own keys. */
+int sde_sort_pairs_device(const uint32_t *keys_in, uint32_t *keys_out, const uint64_t *vals_in,
+                          uint64_t *vals_out, int64_t n, int nbits);
+
 #ifdef __cplusplus
 }
 #endif

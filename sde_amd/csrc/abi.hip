@@ -2551,4 +2551,25 @@ int sde_grdh2d_device_noalloc(const void *vis, const float *wt, const float *u, 
 
 size_t sde_noalloc_scratch_bytes(void) { return kNoAllocScratchBytes; }
 
+int sde_sort_pairs_device(const uint32_t *keys_in, uint32_t *keys_out, const uint64_t *vals_in, uint64_t *vals_out,
+                          int64_t n, int nbits) {
+    if (n < 0 || n >= (int64_t)1 << 31 || nbits < 1 || nbits > 32) {
+        set_plan_error("sde_sort_pairs_device: need 0 <= n < 2^31 and 1 <= nbits <= 32");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    Ctx &c = g_ctx;
+    if (ctx_init(c)) return -1;
+    hipStream_t s = c.stream;
+    const size_t tb = sort_temp_bytes(n, nbits);
+    char *temp = ctx_buf<char>(c, "sort_temp", tb);
+    if (!temp) {
+        set_plan_error("sde_sort_pairs_device: device allocation failed");
+        return -1;
+    }
+    SDE_CHECK(launch_radix_sort(s, temp, tb, keys_in, keys_out, vals_in, vals_out, n, nbits));
+    SDE_CHECK(hipStreamSynchronize(s));
+    return 0;
+}
+
 }  // extern "C"

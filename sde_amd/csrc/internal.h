@@ -130,6 +130,7 @@ struct SortBufs {
 int64_t sort_nkeys(const TileParams &tp);
 int sort_nbits(const TileParams &tp);
 size_t sort_temp_bytes(int64_t n, int nbits);  // sort.hip
+// The error word the last sort on temp left (nonzero: a look-back wait timed out).
 hipError_t launch_radix_sort(hipStream_t s, void *temp, size_t temp_bytes, const uint32_t *keys_in,
                              uint32_t *keys_out, const uint64_t *vals_in, uint64_t *vals_out, int64_t n,
                              int nbits);  // sort.hip
