@@ -9,7 +9,9 @@
 //
 // Least-significant-digit first, 8 bits a pass.  A pass is a reduce-then-
 // scan over tiles of kTile keys, with no waits between workgroups:
-//   k_sort_count       each tile's digit counts (one workgroup per tile);
+//   k_sort_count       each tile's digit counts (one workgroup per tile; LDS
+//                      atomics into 16 sub-histograms, so that runs of one
+//                      digit in neighbouring lanes spread over banks);
 //   k_sort_chunk       per chunk of kChunkTiles tiles: each tile's counts
 //                      replaced by their exclusive prefix within the chunk,
 //                      and the chunk's totals;
@@ -17,11 +19,17 @@
 //                      slot of each chunk's first key of every digit (the
 //                      digit's start over all keys plus the earlier chunks);
 //   k_sort_scatter     one workgroup per tile: the keys ranked stably in the
-//                      tile (per wave: the lanes of equal digit found with
-//                      eight ballots and a running count per wave; across
-//                      waves: LDS prefix counts in input order), staged in LDS
-//                      in digit order and written out as one contiguous run
-//                      per digit from the tile's slot of that digit.
+//                      tile (per wave: the lanes of equal digit found as the
+//                      OR of their lane bits in an LDS mask per digit -- the
+//                      same mask whatever order the ORs land in -- and a
+//                      running count per wave; across waves: LDS prefix counts
+//                      in input order), staged in LDS in digit order and
+//                      written out as one contiguous run per digit from the
+//                      tile's slot of that digit.  Workgroups are dealt to the
+//                      XCDs in turn, so workgroup b runs tile
+//                      xcd_tile(b): each XCD takes a contiguous range of
+//                      tiles, whose runs of one digit are neighbours in the
+//                      output and fill whole cache lines in that XCD's L2.
 // Every pass's counts come from kernels, so the sort needs no memset and
 // can be captured in a graph and replayed.  The counts are 32-bit: n < 2^31.
 #include <algorithm>
@@ -41,6 +49,8 @@ constexpr int kCountThr = 256;          // threads of a count workgroup
 constexpr int kCountKeys = kTile / kCountThr;
 constexpr int kGroupTiles = 32;         // tiles per thread group of k_sort_chunk
 constexpr int kChunkTiles = 4 * kGroupTiles;
+constexpr int kXcd = 8;                 // XCDs of the MI355X
+static_assert(kWaves * kBins <= kTile, "the match masks fit in the value staging area");
 
 struct SortTemp {
     uint32_t *keys;   // n: intermediate keys
@@ -77,42 +87,64 @@ size_t temp_bytes_for(int64_t n) {
 
 __device__ __forceinline__ unsigned digit_of(uint32_t key, int shift, unsigned mask) { return (key >> shift) & mask; }
 
-// The lanes of this wave whose digit equals this lane's, among the lanes in m.
-__device__ __forceinline__ unsigned long long match_digit(unsigned d, int bits, unsigned long long m) {
-#pragma unroll
-    for (int b = 0; b < kBits; b++) {
-        if (b < bits) {
-            const unsigned long long bb = __ballot((d >> b) & 1u);
-            m &= ((d >> b) & 1u) ? bb : ~bb;
-        }
-    }
-    return m;
+// The tile a scatter workgroup sorts: workgroups are dealt to the kXcd XCDs
+// in turn (b % kXcd), and XCD x takes the contiguous range of tiles
+// [x*q + min(x, r), ...) of q or q + 1 tiles, in the order its workgroups
+// start.  A bijection of [0, T) whatever the XCD count; only the speed
+// depends on it.
+__device__ __forceinline__ int xcd_tile(int b, int T) {
+    const int q = T / kXcd, r = T % kXcd, x = b % kXcd, j = b / kXcd;
+    return x * q + min(x, r) + j;
 }
 
+template <bool VEC>
 __global__ __launch_bounds__(kCountThr) void k_sort_count(const uint32_t *__restrict__ keys, int64_t n, int shift,
                                                          int bits, unsigned *__restrict__ tcount) {
-    __shared__ unsigned h[kBins];
-    h[threadIdx.x] = 0u;
+    constexpr int kSub = 16;  // sub-histograms: (wave, lane & 3)
+    __shared__ unsigned h[kSub][kBins];
+    for (int k = threadIdx.x; k < kSub * kBins; k += kCountThr) (&h[0][0])[k] = 0u;
     __syncthreads();
     const unsigned mask = (1u << bits) - 1u;
-    const int lane = threadIdx.x & 63;
-    const unsigned long long below = (1ull << lane) - 1ull;
+    const int lane = threadIdx.x & 63, sub = (threadIdx.x >> 6) * 4 + (lane & 3);
     const int64_t base = (int64_t)blockIdx.x * kTile;
     uint32_t key[kCountKeys];
+    bool valid[kCountKeys];
+    if (VEC) {  // keys 16-byte aligned: four keys a load
 #pragma unroll
-    for (int r = 0; r < kCountKeys; r++) {
-        const int64_t i = base + (int64_t)r * kCountThr + threadIdx.x;
-        key[r] = i < n ? keys[i] : 0u;
+        for (int r = 0; r < kCountKeys / 4; r++) {
+            const int64_t i = base + ((int64_t)r * kCountThr + threadIdx.x) * 4;
+            if (i + 3 < n) {
+                const uint4 x = *reinterpret_cast<const uint4 *>(keys + i);
+                key[4 * r] = x.x;
+                key[4 * r + 1] = x.y;
+                key[4 * r + 2] = x.z;
+                key[4 * r + 3] = x.w;
+#pragma unroll
+                for (int e = 0; e < 4; e++) valid[4 * r + e] = true;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    valid[4 * r + e] = i + e < n;
+                    key[4 * r + e] = valid[4 * r + e] ? keys[i + e] : 0u;
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < kCountKeys; r++) {
+            const int64_t i = base + (int64_t)r * kCountThr + threadIdx.x;
+            valid[r] = i < n;
+            key[r] = valid[r] ? keys[i] : 0u;
+        }
     }
 #pragma unroll
-    for (int r = 0; r < kCountKeys; r++) {
-        const bool valid = base + (int64_t)r * kCountThr + threadIdx.x < n;
-        const unsigned d = digit_of(key[r], shift, mask);
-        const unsigned long long m = match_digit(d, bits, __ballot(valid));
-        if (valid && (m & below) == 0ull) atomicAdd(&h[d], (unsigned)__popcll(m));
-    }
+    for (int r = 0; r < kCountKeys; r++)
+        if (valid[r]) atomicAdd(&h[sub][digit_of(key[r], shift, mask)], 1u);
     __syncthreads();
-    tcount[(size_t)blockIdx.x * kBins + threadIdx.x] = h[threadIdx.x];
+    unsigned s = 0;
+#pragma unroll
+    for (int k = 0; k < kSub; k++) s += h[k][threadIdx.x];
+    tcount[(size_t)blockIdx.x * kBins + threadIdx.x] = s;
 }
 
 // Thread (group g, digit d) holds the counts of kGroupTiles tiles of the chunk.
@@ -214,10 +246,14 @@ __global__ __launch_bounds__(kThr) void k_sort_scatter(const uint32_t *__restric
     const unsigned mask = (1u << bits) - 1u;
     for (int k = threadIdx.x; k < kWaves * kBins; k += kThr) wcnt[k] = 0;
     __syncthreads();
-    const int tile = blockIdx.x;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int64_t seg = (int64_t)tile * kTile + wave * kSeg;
     const unsigned long long below = (1ull << lane) - 1ull;
     unsigned short *wc = wcnt + wave * kBins;
+    // the wave's match masks of the 256 digits, in the value staging area
+    // (unused until the staging below)
+    unsigned long long *mk = reinterpret_cast<unsigned long long *>(sval) + wave * kBins;
+    for (int k = lane; k < kBins; k += 64) mk[k] = 0ull;
 
     uint32_t key[kRounds];
     uint64_t val[kRounds];
@@ -227,19 +263,24 @@ __global__ __launch_bounds__(kThr) void k_sort_scatter(const uint32_t *__restric
         key[r] = i < n ? keys_in[i] : 0u;
         val[r] = i < n ? vals_in[i] : 0ull;
     }
-    // rank within the wave: the lanes of equal digit in this round, after the
-    // wave's keys of that digit in earlier rounds (every lane of a digit reads
-    // the running count before its first lane adds the round's count; the
-    // LDS accesses of one wave complete in order)
+    // rank within the wave: the lanes of equal digit in this round (the OR of
+    // their lane bits), after the wave's keys of that digit in earlier
+    // rounds.  Every lane of a digit reads the mask and the running count
+    // before its first lane clears the one and adds to the other: the LDS
+    // accesses of one wave complete in program order.
     unsigned info[kRounds];  // digit | rank in the wave << 16; digit kBins: past the end
 #pragma unroll
     for (int r = 0; r < kRounds; r++) {
         const bool valid = seg + r * 64 + lane < n;
         const unsigned d = digit_of(key[r], shift, mask);
-        const unsigned long long m = match_digit(d, bits, __ballot(valid));
+        if (valid) atomicOr(&mk[d], 1ull << lane);
+        const unsigned long long m = valid ? mk[d] : 0ull;
         const unsigned rank = (unsigned)__popcll(m & below);
         const unsigned prior = valid ? wc[d] : 0u;
-        if (valid && rank == 0) wc[d] = (unsigned short)(prior + (unsigned)__popcll(m));
+        if (valid && rank == 0) {
+            mk[d] = 0ull;
+            wc[d] = (unsigned short)(prior + (unsigned)__popcll(m));
+        }
         info[r] = valid ? (d | ((prior + rank) << 16)) : (unsigned)kBins;
     }
     __syncthreads();
@@ -249,6 +290,7 @@ __global__ __launch_bounds__(kThr) void k_sort_scatter(const uint32_t *__restric
     if (threadIdx.x < kBins) {
         const int d = threadIdx.x;
         unsigned run = 0;
+#pragma unroll
         for (int w = 0; w < kWaves; w++) {
             const unsigned c = wcnt[w * kBins + d];
             wcnt[w * kBins + d] = (unsigned short)run;
@@ -325,7 +367,10 @@ hipError_t launch_radix_sort(hipStream_t s, void *temp, size_t temp_bytes, const
         uint32_t *ko = to_out ? keys_out : t.keys;
         uint64_t *vo = to_out ? vals_out : t.vals;
         const int shift = p * kBits, bits = std::min(kBits, nbits - shift);
-        hipLaunchKernelGGL(k_sort_count, dim3(tiles), dim3(kCountThr), 0, s, ki, n, shift, bits, t.tcount);
+        if (reinterpret_cast<uintptr_t>(ki) % 16 == 0)
+            hipLaunchKernelGGL(k_sort_count<true>, dim3(tiles), dim3(kCountThr), 0, s, ki, n, shift, bits, t.tcount);
+        else
+            hipLaunchKernelGGL(k_sort_count<false>, dim3(tiles), dim3(kCountThr), 0, s, ki, n, shift, bits, t.tcount);
         hipLaunchKernelGGL(k_sort_chunk, dim3(chunks), dim3(1024), 0, s, t.tcount, tiles, t.csum);
         hipLaunchKernelGGL(k_sort_chunk_scan, dim3(1), dim3(1024), 0, s, t.csum, chunks);
         hipLaunchKernelGGL(k_sort_scatter, dim3(tiles), dim3(kThr), 0, s, ki, vi, ko, vo, n, shift, bits, t.tcount,

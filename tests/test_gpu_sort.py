@@ -3,8 +3,8 @@ gridding, weighting and degridding pass runs on its cell keys) against
 numpy's stable argsort of the same keys: keys and values equal, bit for
 bit, over tile-sized edges (one key, one tile of 4096 keys, one more),
 keys of one value, sorted and reversed runs, few distinct keys, every
-width of the key from 8 to 32 bits, flag bits above the key width that
-ride along, and a cell-sort-sized input of 2e7 keys."""
+width of the key from 1 to 32 bits, flag bits above the key width that
+ride along, keys not aligned to 16 bytes, and an input of 2e7 keys."""
 import ctypes as C
 
 import numpy as np
@@ -16,15 +16,18 @@ sde = pytest.importorskip("sde_amd")
 from sde_amd import DeviceArray  # noqa: E402
 
 
-def _sort(keys, vals, nbits):
+def _sort(keys, vals, nbits, offset=0):
+    """offset: the keys start that many elements into their device array
+    (offset 1: not 16-byte aligned)."""
     L = sde.lib()
     n = len(keys)
-    dk = DeviceArray.from_numpy(keys) if n else DeviceArray((1,), np.uint32)
+    dk = DeviceArray.from_numpy(np.concatenate([np.zeros(offset, np.uint32), keys])) if n else \
+        DeviceArray((1,), np.uint32)
     dv = DeviceArray.from_numpy(vals) if n else DeviceArray((1,), np.uint64)
     ko = DeviceArray((max(n, 1),), np.uint32)
     vo = DeviceArray((max(n, 1),), np.uint64)
-    rc = L.sde_sort_pairs_device(C.c_void_p(dk.ptr), C.c_void_p(ko.ptr), C.c_void_p(dv.ptr), C.c_void_p(vo.ptr),
-                                 C.c_int64(n), C.c_int(nbits))
+    rc = L.sde_sort_pairs_device(C.c_void_p(dk.ptr + 4 * offset), C.c_void_p(ko.ptr), C.c_void_p(dv.ptr),
+                                 C.c_void_p(vo.ptr), C.c_int64(n), C.c_int(nbits))
     if rc != 0:
         msg = sde.last_error()
         sde.clear_error()
@@ -32,12 +35,12 @@ def _sort(keys, vals, nbits):
     return ko.numpy()[:n], vo.numpy()[:n]
 
 
-def _check(keys, nbits, seed=0, label=""):
+def _check(keys, nbits, seed=0, label="", offset=0):
     keys = np.ascontiguousarray(keys, np.uint32)
     n = len(keys)
     rng = np.random.default_rng(seed)
     vals = (np.arange(n, dtype=np.uint64) | (rng.integers(0, 1 << 31, n, dtype=np.uint64) << np.uint64(32)))
-    ko, vo = _sort(keys, vals, nbits)
+    ko, vo = _sort(keys, vals, nbits, offset)
     mask = np.uint32(0xFFFFFFFF) if nbits == 32 else np.uint32((1 << nbits) - 1)
     order = np.argsort(keys & mask, kind="stable")
     np.testing.assert_array_equal(ko, keys[order], err_msg=label + " keys")
@@ -87,13 +90,20 @@ def test_key_patterns(kind):
 
 
 def test_cell_sort_size():
-    """2e7 keys of a 4096^2 field's tiles (the headline job's count), skewed to
-    the inner cells as a uv coverage is."""
+    """2e7 keys over the 24-bit cell keys of a 4096^2 field (4883 tiles, not a
+    multiple of the XCD count), skewed to the inner cells as a uv coverage is."""
     rng = np.random.default_rng(77)
     n = 20_000_000
     r = np.abs(rng.standard_normal(n)) * 0.25
     keys = np.minimum((r * (1 << 24)).astype(np.int64), (1 << 24) - 1).astype(np.uint32)
     _check(keys, 24, label="cell sort size")
+
+
+@pytest.mark.parametrize("n", [5000, 300_001])
+def test_unaligned_keys(n):
+    """Keys that do not start on 16 bytes are counted one load a key."""
+    rng = np.random.default_rng(n + 3)
+    _check(rng.integers(0, 1 << 24, n, dtype=np.uint32), 24, label="unaligned", offset=1)
 
 
 def test_bad_arguments():
