@@ -316,6 +316,44 @@ int sde_plan_synth(sde_plan *plan, int nant, const double *enu, double lat, doub
                    double sigma, uint64_t seed, int64_t nkeep);
 
 /* ------------------------------------------------------------------ */
+/* Part 2b: batches of fields, one launch per stage                     */
+/* ------------------------------------------------------------------ */
+
+/* Many fields (mosaic pointings, channels: each its own visibility set,
+ * image size, cellsize and shift) imaged together.  Replaces the
+ * reference's loop of the single-field pipeline over a work queue of
+ * images (pvm/vistoimgh.f:128-199; task/mosclean.f loops the same way
+ * over pointings): the fields' visibilities are concatenated in HBM, each
+ * stage (prep, cell sort, tile gridding, per-field totals and folds, the
+ * transforms of every field of one image size, correction and PSF-peak
+ * division) is one launch over all fields, and the sequence is captured
+ * in a HIP graph at the first uvmap and replayed after.  Every field's
+ * dirty image and PSF are bit for bit a plan's of the same configuration
+ * (same tile, waves and chunk) on the same visibilities.  Natural weights,
+ * no selection, SF or SZE; every field shares field 0's cftype, tile,
+ * waves and chunk.  Returns 0, or -1 with sde_last_error() set. */
+typedef struct sde_batch sde_batch;
+int sde_batch_create(const sde_imaging_cfg *cfgs, int nfields, sde_batch **batch);
+int sde_batch_destroy(sde_batch *batch);
+/* The visibilities of every field, concatenated in field order (nvis[f]
+ * points of field f; vis complex), host (on_device = 0) or device (1)
+ * arrays; synchronous.  Drops the captured graph. */
+int sde_batch_set_vis(sde_batch *batch, const int64_t *nvis, const float *vis, const float *wt,
+                      const float *u, const float *v, const float *w, int on_device);
+/* Dirty image and PSF of every field, issued on the batch's stream (the
+ * graph's replay, or the launches themselves with use_graph off). */
+int sde_batch_uvmap(sde_batch *batch);
+int sde_batch_use_graph(sde_batch *batch, int on);
+int sde_batch_sync(sde_batch *batch);
+/* Field f's images (NX*NY floats each; either may be NULL), synchronous;
+ * -1 with GRDH2D's "No weight!" when nothing of the field was gridded. */
+int sde_batch_get_images(sde_batch *batch, int field, float *dirty, float *psf);
+int sde_batch_device_images(sde_batch *batch, int field, float **dirty, float **psf);
+/* SUMWT, LSUMWT, the PSF peak, work items and gridded points of field f. */
+int sde_batch_info(sde_batch *batch, int field, sde_run_info *info);
+int sde_batch_geometry(sde_batch *batch, int *nfields, int *ntiles, int *nclasses, size_t *bytes_held);
+
+/* ------------------------------------------------------------------ */
 /* Part 3: w planes (SlowZ) on device-resident arrays                   */
 /* ------------------------------------------------------------------ */
 

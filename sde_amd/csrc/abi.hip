@@ -2573,3 +2573,470 @@ int sde_sort_pairs_device(const uint32_t *keys_in, uint32_t *keys_out, const uin
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batch of fields: many images of natural-weighted visibility sets (mosaic
+// pointings, channels) through one launch per stage.  The reference images
+// many fields by looping its single-field pipeline over a work queue
+// (pvm/vistoimgh.f:128-199); here the fields' visibilities are concatenated,
+// every stage kernel reads a table of the fields, and the whole sequence is
+// captured once in a HIP graph and replayed.
+// ---------------------------------------------------------------------------
+struct sde_batch {
+    int nf = 0;
+    std::vector<sde_imaging_cfg> cfg;
+    std::vector<FieldDesc> fd;  // host copy of the field table
+    // fields of one image size share FFT launches: their grids and images
+    // lie at a fixed stride in the size class's buffers
+    struct SizeClass {
+        int nx = 0, ny = 0, fnu = 0, nu = 0, nv = 0, ccflen = 0, count = 0;
+        float2 *grids = nullptr, *gwork = nullptr;  // per field: dirty, PSF planes (NU x NV each)
+        float *images = nullptr;                    // per field: dirty, PSF images (NX x NY each)
+        float *ccf = nullptr;
+        int *peaks = nullptr;
+        double2 *twnv = nullptr, *twfnu = nullptr;
+    };
+    std::vector<SizeClass> cls;
+    std::vector<int> fcls, fslot;  // size class and slot of each field
+    std::map<int, double2 *> tw;
+    hipStream_t stream = nullptr;
+    int supp = 0, osamp = 0, ncf = 0;
+    float *d_cf = nullptr;
+    TileParams tp;  // T, chunk and waves of every field; ntiles over all fields
+    CoordParams cp0;
+    int max_nu = 0, max_nv = 0;
+    FieldDesc *d_fields = nullptr;
+    double *d_sums = nullptr;  // 4 per field
+    int64_t n = 0, cap = 0;
+    float2 *d_vis = nullptr;
+    float *d_wt = nullptr, *d_u = nullptr, *d_v = nullptr, *d_w = nullptr;
+    SortBufs sb{};
+    int nblocks = 0;
+    double *d_isums = nullptr;
+    float *d_partial = nullptr;
+    size_t items_cap = 0, partial_cap = 0;
+    bool use_graph = true, imaged = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::map<void *, size_t> alloc_bytes;
+};
+
+namespace {
+
+template <typename T>
+int batch_alloc(sde_batch *b, T **ptr, size_t count) {
+    if (*ptr) {
+        b->alloc_bytes.erase((void *)*ptr);
+        (void)hipFree(*ptr);
+    }
+    *ptr = nullptr;
+    const size_t bytes = sizeof(T) * (count > 0 ? count : 1);
+    SDE_CHECK(hipMalloc((void **)ptr, bytes));
+    b->alloc_bytes[(void *)*ptr] = bytes;
+    return 0;
+}
+
+void batch_drop_graph(sde_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// The uvmap of every field, launched on the batch's stream (or recorded
+// into a graph when the stream is capturing): prep, sort, tile gridding,
+// per-field totals and folds, then per image size the column and row
+// transforms with correction and the division by each field's PSF peak.
+int batch_issue(sde_batch *b) {
+    hipStream_t s = b->stream;
+    launch_prep_fields(s, b->d_vis, b->d_wt, b->d_u, b->d_v, b->d_w, b->d_fields, b->nf, b->nblocks, b->tp.T,
+                       (uint32_t)sort_nkeys(b->tp), b->sb);
+    if (launch_sort(s, b->sb, b->n, b->tp, true)) {
+        set_plan_error("sde_batch_uvmap: cell sort failed");
+        return -1;
+    }
+    if (launch_grid_tiles_fields(s, b->sb, b->tp, b->d_cf, b->cp0, b->d_isums, b->d_partial, b->d_fields, b->nf)) {
+        set_plan_error("sde_batch_uvmap: unsupported gridding run (T=%d)", b->tp.T);
+        return -1;
+    }
+    launch_grid_reduce_fields(s, b->d_partial, b->sb.item_start, b->d_fields, b->nf, b->tp.T, b->max_nu, b->max_nv);
+    for (const auto &c : b->cls) {
+        const size_t ng = (size_t)c.nu * c.nv, npix = (size_t)c.nx * c.ny;
+        launch_fft_cols(s, c.grids, c.gwork, c.nu, c.nv, 2 * c.count, ng, -1, 1.0f, c.twnv);
+        launch_fft_rows_c2r_fields(s, c.gwork + ng, c.gwork, 2 * ng, c.images + npix, c.images, 2 * npix, c.peaks,
+                                   c.count, c.nu, c.nv, c.fnu, c.nx, c.ny, c.twfnu, c.ccf, c.ccflen, 1,
+                                   1.0f / ((float)c.fnu * (float)c.nv));
+        launch_scale_by_peak_fields(s, c.images + npix, c.images, 2 * npix, npix, c.peaks, c.count);
+    }
+    SDE_CHECK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sde_batch_destroy(sde_batch *b) {
+    if (!b) return 0;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_drop_graph(b);
+    for (auto &kv : b->alloc_bytes) (void)hipFree(kv.first);
+    for (auto &kv : b->tw) (void)hipFree(kv.second);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return 0;
+}
+
+int sde_batch_create(const sde_imaging_cfg *cfgs, int nfields, sde_batch **out) {
+    if (!out) return -1;
+    *out = nullptr;
+    if (!cfgs || nfields < 1 || nfields > 65535) {
+        set_plan_error("sde_batch_create: need 1 to 65535 fields");
+        return -1;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_ctx_mu);
+        if (ctx_init(g_ctx)) return -1;
+    }
+    const sde_imaging_cfg &c0 = cfgs[0];
+    const std::string cft0 = fstr(c0.cftype, strnlen(c0.cftype, sizeof c0.cftype));
+    const int T = c0.tile ? c0.tile : 16;
+    const int waves = c0.waves ? c0.waves : 4, chunk = c0.chunk > 0 ? c0.chunk : 2048;
+    if (T != 16 && T != 32) {
+        set_plan_error("sde_batch_create: unsupported tile %d (16 or 32)", T);
+        return -1;
+    }
+    if (waves != 4 && waves != 8) {
+        set_plan_error("sde_batch_create: unsupported waves %d (4 or 8)", waves);
+        return -1;
+    }
+    for (int f = 0; f < nfields; f++) {
+        const sde_imaging_cfg &c = cfgs[f];
+        const std::string cft = fstr(c.cftype, strnlen(c.cftype, sizeof c.cftype));
+        if (c.nx < 2 || c.ny < 2 || c.nx > 8192 || c.ny > 8192 || !(c.cellsize > 0.0f)) {
+            set_plan_error("sde_batch_create: field %d: bad image size or cellsize", f);
+            return -1;
+        }
+        if (c.weighting != 0 || c.select != 0) {
+            set_plan_error("sde_batch_create: field %d: a batch images natural weights without selection "
+                           "(use a plan per field for weighting or selection)", f);
+            return -1;
+        }
+        if (cft != cft0 || fstarts(cft, "BOX")) {
+            set_plan_error("sde_batch_create: field %d: every field needs the same SF or SZE convolution function", f);
+            return -1;
+        }
+        if ((c.tile && c.tile != T) || (c.waves && c.waves != waves) || (c.chunk > 0 && c.chunk != chunk)) {
+            set_plan_error("sde_batch_create: field %d: tile, waves and chunk must match field 0's", f);
+            return -1;
+        }
+    }
+    sde_batch *b = new sde_batch();
+    b->nf = nfields;
+    b->cfg.assign(cfgs, cfgs + nfields);
+    float cf[1 + 2 * 4 * 128];
+    if (make_cf(cft0, cf, &b->supp, &b->osamp)) {
+        set_plan_error("sde_batch_create: convolution function must be SF or SZE");
+        delete b;
+        return -1;
+    }
+    b->ncf = 1 + 2 * (b->supp + 1) * b->osamp;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_plan_error("sde_batch_create: stream creation failed");
+        delete b;
+        return -1;
+    }
+    // per-field geometry, as a plan of that configuration has it
+    b->fd.resize(nfields);
+    b->fcls.resize(nfields);
+    b->fslot.resize(nfields);
+    int tiles = 0;
+    for (int f = 0; f < nfields; f++) {
+        const sde_imaging_cfg &c = cfgs[f];
+        sde_plan g;
+        g.cfg = c;
+        g.supp = b->supp;
+        g.osamp = b->osamp;
+        g.fnu = pow2_at_least(c.nx);
+        g.nu = g.fnu / 2 + 1;
+        g.nv = pow2_at_least(c.ny);
+        g.uorigin = 1;
+        g.vorigin = (g.nv + 1) / 2;
+        plan_set_geometry(&g, -c.cellsize / 3600.0f, c.cellsize / 3600.0f);
+        const TileParams ftp = make_tp(g.nu, g.nv, T, chunk, waves);
+        FieldDesc &d = b->fd[f];
+        d.cp = g.cp;
+        d.tile_begin = tiles;
+        d.ntu = ftp.ntu;
+        d.ntv = ftp.ntv;
+        tiles += ftp.ntiles;
+        b->max_nu = std::max(b->max_nu, g.nu);
+        b->max_nv = std::max(b->max_nv, g.nv);
+        int k = 0;
+        while (k < (int)b->cls.size() && !(b->cls[k].nx == c.nx && b->cls[k].ny == c.ny)) k++;
+        if (k == (int)b->cls.size()) {
+            sde_batch::SizeClass sc;
+            sc.nx = c.nx;
+            sc.ny = c.ny;
+            sc.fnu = g.fnu;
+            sc.nu = g.nu;
+            sc.nv = g.nv;
+            sc.ccflen = 1 + pow2_at_least(c.nx > c.ny ? c.nx : c.ny) / 2;
+            b->cls.push_back(sc);
+        }
+        b->fcls[f] = k;
+        b->fslot[f] = b->cls[k].count++;
+    }
+    b->cp0 = b->fd[0].cp;
+    b->tp = make_tp(1, 1, T, chunk, waves);
+    b->tp.ntiles = tiles;
+    b->tp.ntu = tiles;
+    b->tp.ntv = 1;
+    if (sort_nbits(b->tp) > 31) {
+        set_plan_error("sde_batch_create: %d tiles over all fields exceed the 31-bit cell key", tiles);
+        sde_batch_destroy(b);
+        return -1;
+    }
+    int rc = batch_alloc(b, &b->d_cf, b->ncf) || batch_alloc(b, &b->d_fields, nfields) ||
+             batch_alloc(b, &b->d_sums, 4 * (size_t)nfields) || batch_alloc(b, &b->sb.tile_start, tiles + 1) ||
+             batch_alloc(b, &b->sb.item_start, tiles + 1) || batch_alloc(b, &b->sb.nitems, 1);
+    for (auto &c : b->cls) {
+        if (rc) break;
+        // (fft_supported also makes the transforms' LDS opt-in, before any capture)
+        if (!fft_supported(c.nv) || !fft_supported(c.fnu / 2)) {
+            set_plan_error("sde_batch_create: unsupported transform size %d x %d", c.nx, c.ny);
+            rc = -1;
+            break;
+        }
+        const size_t ng = (size_t)c.nu * c.nv, npix = (size_t)c.nx * c.ny;
+        rc = batch_alloc(b, &c.grids, 2 * ng * c.count) || batch_alloc(b, &c.gwork, 2 * ng * c.count) ||
+             batch_alloc(b, &c.images, 2 * npix * c.count) || batch_alloc(b, &c.peaks, (size_t)c.count) ||
+             batch_alloc(b, &c.ccf, (size_t)c.ccflen) || get_twiddles(b->tw, c.nv, b->stream, &c.twnv) ||
+             get_twiddles(b->tw, c.fnu, b->stream, &c.twfnu);
+        if (rc) break;
+        std::vector<float> ccf(c.ccflen);
+        make_ccf(cft0, ccf.data(), c.ccflen);
+        if (hipMemcpyAsync(c.ccf, ccf.data(), sizeof(float) * c.ccflen, hipMemcpyHostToDevice, b->stream) !=
+            hipSuccess)
+            rc = -1;
+        if (hipStreamSynchronize(b->stream) != hipSuccess) rc = -1;
+    }
+    if (rc) {
+        sde_batch_destroy(b);
+        return -1;
+    }
+    for (int f = 0; f < nfields; f++) {
+        const auto &c = b->cls[b->fcls[f]];
+        const size_t ng = (size_t)c.nu * c.nv;
+        FieldDesc &d = b->fd[f];
+        d.dirty = c.grids + 2 * ng * b->fslot[f];
+        d.psf = d.dirty + ng;
+        d.sums = b->d_sums + 4 * f;
+        d.peak = c.peaks + b->fslot[f];
+    }
+    if (hipMemcpyAsync(b->d_cf, cf, sizeof(float) * b->ncf, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess) {
+        set_plan_error("sde_batch_create: table upload failed");
+        sde_batch_destroy(b);
+        return -1;
+    }
+    *out = b;
+    return 0;
+}
+
+int sde_batch_set_vis(sde_batch *b, const int64_t *nvis, const float *vis, const float *wt, const float *u,
+                      const float *v, const float *w, int on_device) {
+    if (!b || !nvis) return -1;
+    int64_t n = 0, blocks = 0;
+    for (int f = 0; f < b->nf; f++) {
+        if (nvis[f] < 0) {
+            set_plan_error("sde_batch_set_vis: field %d: negative visibility count", f);
+            return -1;
+        }
+        n += nvis[f];
+        blocks += (nvis[f] + kPrepFieldThreads - 1) / kPrepFieldThreads;
+    }
+    if (n >= (int64_t)1 << 31 || blocks >= (int64_t)1 << 31) {
+        set_plan_error("sde_batch_set_vis: %lld visibilities exceed the batch's 2^31", (long long)n);
+        return -1;
+    }
+    SDE_CHECK(hipStreamSynchronize(b->stream));
+    batch_drop_graph(b);
+    b->imaged = false;
+    if (n > b->cap) {
+        const size_t c = (size_t)n;
+        if (batch_alloc(b, &b->d_vis, c) || batch_alloc(b, &b->d_wt, c) || batch_alloc(b, &b->d_u, c) ||
+            batch_alloc(b, &b->d_v, c) || batch_alloc(b, &b->d_w, c) || batch_alloc(b, &b->sb.keys, c) ||
+            batch_alloc(b, &b->sb.skeys, c) || batch_alloc(b, &b->sb.vals, c) || batch_alloc(b, &b->sb.svals, c) ||
+            batch_alloc(b, &b->sb.hot, c))
+            return -1;
+        const size_t tb = sort_temp_bytes(n, sort_nbits(b->tp));
+        char *temp = static_cast<char *>(b->sb.temp);
+        if (tb == 0 || batch_alloc(b, &temp, tb)) {
+            set_plan_error("sde_batch_set_vis: sort scratch allocation failed");
+            return -1;
+        }
+        b->sb.temp = temp;
+        b->sb.temp_bytes = tb;
+        b->cap = n;
+    }
+    b->n = n;
+    const int mi = max_items_for(b->tp, n);
+    if ((size_t)mi > b->items_cap) {
+        if (batch_alloc(b, &b->sb.items, (size_t)mi) || batch_alloc(b, &b->d_isums, 2 * (size_t)mi)) return -1;
+        b->items_cap = mi;
+    }
+    b->sb.max_items = mi;
+    const size_t pf = grid_partial_floats(b->tp, 3, mi);
+    if (pf > b->partial_cap) {
+        if (batch_alloc(b, &b->d_partial, pf)) return -1;
+        b->partial_cap = pf;
+    }
+    int64_t off = 0;
+    int blk = 0;
+    for (int f = 0; f < b->nf; f++) {
+        FieldDesc &d = b->fd[f];
+        d.vis_begin = off;
+        d.nvis = nvis[f];
+        d.block_begin = blk;
+        off += nvis[f];
+        blk += (int)((nvis[f] + kPrepFieldThreads - 1) / kPrepFieldThreads);
+    }
+    b->nblocks = blk;
+    const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (n > 0) {
+        SDE_CHECK(hipMemcpyAsync(b->d_vis, vis, sizeof(float2) * n, kind, b->stream));
+        SDE_CHECK(hipMemcpyAsync(b->d_wt, wt, sizeof(float) * n, kind, b->stream));
+        SDE_CHECK(hipMemcpyAsync(b->d_u, u, sizeof(float) * n, kind, b->stream));
+        SDE_CHECK(hipMemcpyAsync(b->d_v, v, sizeof(float) * n, kind, b->stream));
+        SDE_CHECK(hipMemcpyAsync(b->d_w, w, sizeof(float) * n, kind, b->stream));
+    }
+    SDE_CHECK(hipMemcpyAsync(b->d_fields, b->fd.data(), sizeof(FieldDesc) * b->nf, hipMemcpyHostToDevice, b->stream));
+    SDE_CHECK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int sde_batch_use_graph(sde_batch *b, int on) {
+    if (!b) return -1;
+    SDE_CHECK(hipStreamSynchronize(b->stream));
+    if (!on) batch_drop_graph(b);
+    b->use_graph = on != 0;
+    return 0;
+}
+
+int sde_batch_uvmap(sde_batch *b) {
+    if (!b) return -1;
+    if (b->n <= 0) {
+        set_plan_error("sde_batch_uvmap: no visibilities");
+        return -1;
+    }
+    if (!b->use_graph) {
+        if (batch_issue(b)) return -1;
+        b->imaged = true;
+        return 0;
+    }
+    if (!b->exec) {
+        SDE_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = batch_issue(b);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (rc || e != hipSuccess || !g) {
+            if (g) (void)hipGraphDestroy(g);
+            if (!rc) set_plan_error("sde_batch_uvmap: graph capture failed: %s", hipGetErrorString(e));
+            return -1;
+        }
+        b->graph = g;
+        if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) {
+            batch_drop_graph(b);
+            set_plan_error("sde_batch_uvmap: graph instantiation failed");
+            return -1;
+        }
+    }
+    SDE_CHECK(hipGraphLaunch(b->exec, b->stream));
+    b->imaged = true;
+    return 0;
+}
+
+int sde_batch_sync(sde_batch *b) {
+    if (!b) return -1;
+    SDE_CHECK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int sde_batch_get_images(sde_batch *b, int field, float *dirty, float *psf) {
+    if (!b || field < 0 || field >= b->nf) {
+        set_plan_error("sde_batch_get_images: no field %d", field);
+        return -1;
+    }
+    const auto &c = b->cls[b->fcls[field]];
+    const size_t npix = (size_t)c.nx * c.ny;
+    const float *base = c.images + 2 * npix * b->fslot[field];
+    double sums[2] = {0.0, 0.0};
+    if (dirty) SDE_CHECK(hipMemcpyAsync(dirty, base, sizeof(float) * npix, hipMemcpyDeviceToHost, b->stream));
+    if (psf) SDE_CHECK(hipMemcpyAsync(psf, base + npix, sizeof(float) * npix, hipMemcpyDeviceToHost, b->stream));
+    if (b->imaged) SDE_CHECK(hipMemcpyAsync(sums, b->fd[field].sums, sizeof sums, hipMemcpyDeviceToHost, b->stream));
+    SDE_CHECK(hipStreamSynchronize(b->stream));
+    if (b->imaged && sums[1] == 0.0) {
+        set_plan_error("GRDH2D: No weight! (field %d)", field);
+        return -1;
+    }
+    return 0;
+}
+
+int sde_batch_device_images(sde_batch *b, int field, float **dirty, float **psf) {
+    if (!b || field < 0 || field >= b->nf) {
+        set_plan_error("sde_batch_device_images: no field %d", field);
+        return -1;
+    }
+    const auto &c = b->cls[b->fcls[field]];
+    const size_t npix = (size_t)c.nx * c.ny;
+    float *base = c.images + 2 * npix * b->fslot[field];
+    if (dirty) *dirty = base;
+    if (psf) *psf = base + npix;
+    return 0;
+}
+
+int sde_batch_info(sde_batch *b, int field, sde_run_info *info) {
+    if (!b || !info || field < 0 || field >= b->nf) {
+        set_plan_error("sde_batch_info: no field %d", field);
+        return -1;
+    }
+    const FieldDesc &d = b->fd[field];
+    double sums[3] = {0, 0, 0};
+    int key = 0, i0 = 0, i1 = 0, s0 = 0, s1 = 0;
+    SDE_CHECK(hipMemcpyAsync(sums, d.sums, sizeof sums, hipMemcpyDeviceToHost, b->stream));
+    SDE_CHECK(hipMemcpyAsync(&key, d.peak, sizeof key, hipMemcpyDeviceToHost, b->stream));
+    const int t0 = d.tile_begin, t1 = d.tile_begin + d.ntu * d.ntv;
+    SDE_CHECK(hipMemcpyAsync(&i0, b->sb.item_start + t0, sizeof i0, hipMemcpyDeviceToHost, b->stream));
+    SDE_CHECK(hipMemcpyAsync(&i1, b->sb.item_start + t1, sizeof i1, hipMemcpyDeviceToHost, b->stream));
+    SDE_CHECK(hipMemcpyAsync(&s0, b->sb.tile_start + t0, sizeof s0, hipMemcpyDeviceToHost, b->stream));
+    SDE_CHECK(hipMemcpyAsync(&s1, b->sb.tile_start + t1, sizeof s1, hipMemcpyDeviceToHost, b->stream));
+    SDE_CHECK(hipStreamSynchronize(b->stream));
+    const int bits = key >= 0 ? key : key ^ 0x7fffffff;
+    float peak;
+    memcpy(&peak, &bits, sizeof peak);
+    memset(info, 0, sizeof *info);
+    info->nvis = d.nvis;
+    info->sumwt = sums[0];
+    info->lsumwt = sums[1];
+    info->maxpsf = peak;
+    info->nitems = i1 - i0;
+    info->ngridded = s1 - s0;
+    info->nselected = -1;
+    return 0;
+}
+
+int sde_batch_geometry(sde_batch *b, int *nfields, int *ntiles, int *nclasses, size_t *bytes_held) {
+    if (!b) return -1;
+    if (nfields) *nfields = b->nf;
+    if (ntiles) *ntiles = b->tp.ntiles;
+    if (nclasses) *nclasses = (int)b->cls.size();
+    if (bytes_held) {
+        size_t t = 0;
+        for (auto &kv : b->alloc_bytes) t += kv.second;
+        *bytes_held = t;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -227,19 +227,22 @@ __device__ __forceinline__ int icf(float s, int i1, float cen) {
 }
 
 // Grid rows -> image rows (DIR < 0).
-// blockIdx.y = 1 transforms grid_b into image_b (no peak) in the same launch.
+// blockIdx.y = 2k + 1 transforms grid_b into image_b (no peak) in the same
+// launch; k > 0: the k-th of several fields, gstride and istride apart,
+// with their peaks in peak_a[k].
 __global__ void k_fft_rows_c2r(const float2 *__restrict__ grid_a, int nu, int nv, int fnu, int nx,
                                int ny, int ibegy, const double2 *__restrict__ tw,
                                const float *__restrict__ ccf, int correct, CorrParams cpar,
                                float scale, float *__restrict__ image_a, int *__restrict__ peak_a,
                                const int *__restrict__ div_key, const float2 *__restrict__ grid_b,
-                               float *__restrict__ image_b) {
+                               float *__restrict__ image_b, size_t gstride, size_t istride) {
     extern __shared__ double2 zs[];
     __shared__ float red[1024];
-    const bool second = blockIdx.y == 1;
-    const float2 *grid = second ? grid_b : grid_a;
-    float *image = second ? image_b : image_a;
-    int *peak_key = second ? nullptr : peak_a;
+    const bool second = (blockIdx.y & 1) != 0;
+    const int field = blockIdx.y >> 1;
+    const float2 *grid = (second ? grid_b : grid_a) + gstride * field;
+    float *image = (second ? image_b : image_a) + istride * field;
+    int *peak_key = second || !peak_a ? nullptr : peak_a + field;
     const int iy = blockIdx.x;       // 0-based image row
     const int rv = iy + ibegy - 1;   // 0-based grid row
     const int h = fnu / 2;
@@ -335,11 +338,13 @@ __global__ void k_fft_rows_r2c(const float *__restrict__ image, int nx, int ny, 
     }
 }
 
-// blockIdx.y = 1 scales image_b.
+// blockIdx.y = 2k + 1 scales image_b; k > 0: the k-th of several fields,
+// istride apart, by the peak key[k].
 __global__ void k_scale_by_peak(float *__restrict__ image_a, float *__restrict__ image_b, size_t n,
-                                const int *__restrict__ key) {
-    float *image = blockIdx.y == 1 ? image_b : image_a;
-    const float inv = 1.0f / key_float(*key);
+                                const int *__restrict__ key, size_t istride) {
+    const int field = blockIdx.y >> 1;
+    float *image = ((blockIdx.y & 1) ? image_b : image_a) + istride * field;
+    const float inv = 1.0f / key_float(key[field]);
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         image[i] = inv * image[i] + 0.0f;
 }
@@ -459,7 +464,7 @@ void launch_fft_rows_c2r(hipStream_t s, const float2 *grid, int nu, int nv, int 
     (void)fft_lds_limit();
     hipLaunchKernelGGL(k_fft_rows_c2r, dim3(ny), dim3(row_threads(h)), sizeof(double2) * lds_len(h), s, grid,
                        nu, nv, fnu, nx, ny, ibegy, tw_fnu, ccf, correct, cp, scale, image, peak_key,
-                       div_key, (const float2 *)nullptr, (float *)nullptr);
+                       div_key, (const float2 *)nullptr, (float *)nullptr, (size_t)0, (size_t)0);
 }
 
 void launch_fft_rows_c2r_pair(hipStream_t s, const float2 *grid_a, float *image_a, int *peak_a,
@@ -471,7 +476,21 @@ void launch_fft_rows_c2r_pair(hipStream_t s, const float2 *grid_a, float *image_
     (void)fft_lds_limit();
     hipLaunchKernelGGL(k_fft_rows_c2r, dim3(ny, 2), dim3(row_threads(h)), sizeof(double2) * lds_len(h), s, grid_a, nu, nv,
                        fnu, nx, ny, ibegy, tw_fnu, ccf, correct, cp, scale, image_a, peak_a, (const int *)nullptr,
-                       grid_b, image_b);
+                       grid_b, image_b, (size_t)0, (size_t)0);
+}
+
+void launch_fft_rows_c2r_fields(hipStream_t s, const float2 *grid_psf, const float2 *grid_dirty, size_t gstride,
+                                float *image_psf, float *image_dirty, size_t istride, int *peaks, int nf, int nu,
+                                int nv, int fnu, int nx, int ny, const double2 *tw_fnu, const float *ccf, int ccflen,
+                                int correct, float scale) {
+    if (nf <= 0) return;
+    const int h = fnu / 2;
+    const int ibegy = (nv + 1) / 2 - (ny + 1) / 2 + 1;
+    CorrParams cp = make_corr(nx, ny, ccflen, (float)((nx + 1) / 2), (float)((ny + 1) / 2));
+    (void)fft_lds_limit();
+    hipLaunchKernelGGL(k_fft_rows_c2r, dim3(ny, 2 * nf), dim3(row_threads(h)), sizeof(double2) * lds_len(h), s,
+                       grid_psf, nu, nv, fnu, nx, ny, ibegy, tw_fnu, ccf, correct, cp, scale, image_psf, peaks,
+                       (const int *)nullptr, grid_dirty, image_dirty, gstride, istride);
 }
 
 void launch_fft_rows_r2c(hipStream_t s, const float *image, int nx, int ny, int fnu, int nv,
@@ -489,7 +508,16 @@ void launch_scale_by_peak(hipStream_t s, float *image, size_t n, const int *peak
     size_t nb = (n + 255) / 256;
     if (nb > 65536) nb = 65536;
     hipLaunchKernelGGL(k_scale_by_peak, dim3((unsigned)nb, image_b ? 2 : 1), dim3(256), 0, s, image, image_b, n,
-                       peak_key);
+                       peak_key, (size_t)0);
+}
+
+void launch_scale_by_peak_fields(hipStream_t s, float *image_psf, float *image_dirty, size_t istride, size_t n,
+                                 const int *peaks, int nf) {
+    if (nf <= 0) return;
+    size_t nb = (n + 255) / 256;
+    if (nb > 4096) nb = 4096;
+    hipLaunchKernelGGL(k_scale_by_peak, dim3((unsigned)nb, 2 * nf), dim3(256), 0, s, image_psf, image_dirty, n, peaks,
+                       istride);
 }
 
 void launch_copy(hipStream_t s, const float4 *src, float4 *dst, size_t n) {

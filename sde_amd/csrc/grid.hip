@@ -100,6 +100,28 @@ __device__ __forceinline__ int cell_key(const TileParams &tp, int cu, int cv) {
     return ((tv * tp.ntu + tu) * tp.T + (cv - tv * tp.T)) * tp.T + (cu - tu * tp.T);
 }
 
+// GRDH2D's DOUBLE coordinate step and drop test for one point with WT > 0
+// (grd/grdh2d.f:115-145) and, when it lands on the grid, its cell key and
+// the gridding part of its hot record: the local cell in the tile, the
+// kernel phases and FVIS (grd/grdh2d.f:124-160).  Shared by every prep pass
+// that builds tile-gridding records, so they stage the same bits.
+__device__ __forceinline__ bool stage_cell(const CoordParams &cp, const TileParams &tp, int poffu, int poffv,
+                                           float uf, float vf, float wf, float2 vv, float2 rot, GridCell &c,
+                                           uint32_t &key, StageHot &sr) {
+    const bool in = grid_cell(cp, uf, vf, wf, vv, rot, true, c);
+    if (in) {
+        key = (uint32_t)cell_key(tp, c.nuc + cp.suppu - 1, c.nvc - 1);
+        const uint32_t local = key & (uint32_t)(tp.T * tp.T - 1);
+        const int pu = nint_d((double)cp.osampu * ((double)c.nuc - c.ucell)) + poffu;
+        const int pv = nint_d((double)cp.osampv * ((double)c.nvc - c.vcell)) + poffv;
+        sr.gpk = (local % (uint32_t)tp.T) | ((local / (uint32_t)tp.T) << 8) | ((uint32_t)pu << 16) |
+                 ((uint32_t)pv << 24);
+        sr.fr = c.fr;
+        sr.fi = c.fi;
+    }
+    return in;
+}
+
 // Prep pass, one thread per visibility in the caller's order: the cell key
 // of GRDH2D's DOUBLE coordinate step and drop test (grd/grdh2d.f:115-145)
 // and the staging records -- the gridding record's cell, kernel phases and
@@ -166,17 +188,7 @@ __global__ __launch_bounds__(kThreads) void k_prep(
             if (wti > 0.0f) {
                 sc.rot = shift_phase(cp, uf, vf, wf);
                 GridCell c;
-                const bool in = grid_cell(cp, uf, vf, wf, vv, sc.rot, true, c);
-                if (in) {
-                    key = (uint32_t)cell_key(tp, c.nuc + cp.suppu - 1, c.nvc - 1);
-                    const uint32_t local = key & (uint32_t)(tp.T * tp.T - 1);
-                    const int pu = nint_d((double)cp.osampu * ((double)c.nuc - c.ucell)) + poffu;
-                    const int pv = nint_d((double)cp.osampv * ((double)c.nvc - c.vcell)) + poffv;
-                    sr.gpk = (local % (uint32_t)tp.T) | ((local / (uint32_t)tp.T) << 8) | ((uint32_t)pu << 16) |
-                             ((uint32_t)pv << 24);
-                    sr.fr = c.fr;
-                    sr.fi = c.fi;
-                }
+                const bool in = stage_cell(cp, tp, poffu, poffv, uf, vf, wf, vv, sc.rot, c, key, sr);
                 if (DM > 0) {
                     const RobustCell rc = robust_cell(wcp, uf, vf, wf, az);
                     sc.wcell = rc.cell;
@@ -857,24 +869,20 @@ __device__ __forceinline__ Sum3 gather_cell(const float *__restrict__ partial,
     return s;
 }
 
-__global__ __launch_bounds__(256) void k_grid_reduce(
-    const float *__restrict__ partial, const int *__restrict__ item_start, TileParams tp,
-    int chmask, CoordParams cp, const double *__restrict__ sums, float2 *__restrict__ dirty,
-    float2 *__restrict__ psf) {
-    // the rows of one band of tiles (which read the same partial tiles) on one
-    // XCD, the bands dealt round the XCDs (the work per band varies)
-    const int lin = xcd_swizzle_chunks(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y,
-                                       gridDim.x * tp.T);
-    const int iu = (lin % gridDim.x) * blockDim.x + threadIdx.x;  // 0-based output column
-    const int iv = lin / gridDim.x;                                // 0-based row
-    if (iu >= cp.nu) return;
+// Output cell (iu, iv) (0-based) of the NU x NV grid from the partial
+// tiles: FP64 sum of the padded tiles covering it, the Hermitian fold of the
+// u = 0 column, the shift left by SUPPU, the normalisation sums[2].
+__device__ __forceinline__ void reduce_cell(const float *__restrict__ partial, const int *__restrict__ item_start,
+                                            int T, int ntu, int ntv, int chmask, const CoordParams &cp,
+                                            const double *__restrict__ sums, int iu, int iv,
+                                            float2 *__restrict__ dirty, float2 *__restrict__ psf) {
     const int nch = ((chmask & 1) ? 2 : 0) + ((chmask & 2) ? 1 : 0);
     double dr = 0.0, di = 0.0, pr = 0.0;
     if (iu + 1 <= cp.nu - cp.suppu) {
         const int pc = iu + 1 + cp.suppu;  // 1-based pre-shift column
         const int vrow = iv + 1;
         const int c = cp.uorigin + cp.suppu, r = cp.vorigin;
-        Sum3 s = gather_cell(partial, item_start, pc - 1, vrow - 1, tp.T, tp.ntu, tp.ntv, nch, chmask);
+        Sum3 s = gather_cell(partial, item_start, pc - 1, vrow - 1, T, ntu, ntv, nch, chmask);
         const int offv = vrow - r;
         const bool fold70 = pc > c && pc <= c + cp.suppu && offv >= -cp.nv / 2 + 1 && offv <= cp.nv / 2 - 1;
         const bool fold80 = pc == c && vrow >= r - (cp.nv / 2 - 1) && vrow <= r + cp.nv / 2 - 1;
@@ -882,7 +890,7 @@ __global__ __launch_bounds__(256) void k_grid_reduce(
             const int mc = fold70 ? 2 * c - pc : c;
             const int mr = 2 * r - vrow;
             if (mc >= 1 && mc <= cp.nu && mr >= 1 && mr <= cp.nv) {
-                Sum3 m = gather_cell(partial, item_start, mc - 1, mr - 1, tp.T, tp.ntu, tp.ntv, nch, chmask);
+                Sum3 m = gather_cell(partial, item_start, mc - 1, mr - 1, T, ntu, ntv, nch, chmask);
                 s.re += m.re;
                 s.im -= m.im;
                 s.p += m.p;
@@ -896,6 +904,134 @@ __global__ __launch_bounds__(256) void k_grid_reduce(
     const size_t o = (size_t)iv * cp.nu + iu;
     if (chmask & 1) dirty[o] = make_float2((float)dr, (float)di);
     if (chmask & 2) psf[o] = make_float2((float)pr, 0.0f);
+}
+
+__global__ __launch_bounds__(256) void k_grid_reduce(
+    const float *__restrict__ partial, const int *__restrict__ item_start, TileParams tp,
+    int chmask, CoordParams cp, const double *__restrict__ sums, float2 *__restrict__ dirty,
+    float2 *__restrict__ psf) {
+    // the rows of one band of tiles (which read the same partial tiles) on one
+    // XCD, the bands dealt round the XCDs (the work per band varies)
+    const int lin = xcd_swizzle_chunks(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y,
+                                       gridDim.x * tp.T);
+    const int iu = (lin % gridDim.x) * blockDim.x + threadIdx.x;  // 0-based output column
+    const int iv = lin / gridDim.x;                                // 0-based row
+    if (iu >= cp.nu) return;
+    reduce_cell(partial, item_start, tp.T, tp.ntu, tp.ntv, chmask, cp, sums, iu, iv, dirty, psf);
+}
+
+// ---- many fields in one launch per stage ----
+// Field f's visibilities are [vis_begin, vis_begin + nvis) of the
+// concatenated arrays, prepared by prep blocks [block_begin, ...), and its
+// tiles are [tile_begin, tile_begin + ntu*ntv) of one tile numbering over
+// all fields: the cell key of a point is tile_begin*T*T plus its key in its
+// own field, so one sort orders every field's points by field, then by
+// tile and cell, and the points of one field keep their order among
+// themselves.  The gridding work items then run unchanged; the totals,
+// the fold and the transforms run per field.
+
+// The field of prep block b: the last f with fields[f].block_begin <= b.
+__device__ __forceinline__ int field_of_block(const FieldDesc *__restrict__ fields, int nfields, int b) {
+    int lo = 0, hi = nfields - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (fields[mid].block_begin <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Prep pass of many fields with natural weights (no density cell, no
+// selection, no degridding outputs): one thread per visibility, blocks
+// dealt to fields; the key and the gridding part of the hot record as
+// k_prep stages them, the key moved to the field's tiles.
+__global__ __launch_bounds__(kPrepFieldThreads) void k_prep_fields(
+    const float2 *__restrict__ vis, const float *__restrict__ wt, const float *__restrict__ u,
+    const float *__restrict__ v, const float *__restrict__ w, const FieldDesc *__restrict__ fields, int nfields,
+    int T, uint32_t sentinel, uint32_t *__restrict__ keys, uint64_t *__restrict__ vals,
+    StageHot *__restrict__ hot) {
+    __shared__ int s_f;
+    if (threadIdx.x == 0) s_f = field_of_block(fields, nfields, blockIdx.x);
+    __syncthreads();
+    const FieldDesc &fd = fields[s_f];
+    const CoordParams cp = fd.cp;
+    TileParams tp;
+    tp.T = T;
+    tp.ntu = fd.ntu;
+    tp.ntv = fd.ntv;
+    tp.ntiles = fd.ntu * fd.ntv;
+    tp.chunk = 0;
+    tp.waves = 0;
+    const int64_t k = (int64_t)(blockIdx.x - fd.block_begin) * kPrepFieldThreads + threadIdx.x;
+    if (k >= fd.nvis) return;
+    const int64_t i = fd.vis_begin + k;
+    const int poffu = (cp.osampu + 1) / 2, poffv = (cp.osampv + 1) / 2;
+    const float wti = wt[i];
+    StageHot sr;
+    sr.gpk = 0u;
+    sr.fr = 0.0f;
+    sr.fi = 0.0f;
+    sr.dpk = 0u;
+    uint32_t key = sentinel;
+    if (wti > 0.0f) {
+        const float uf = u[i], vf = v[i], wf = w[i];
+        const float2 rot = shift_phase(cp, uf, vf, wf);
+        GridCell c;
+        if (stage_cell(cp, tp, poffu, poffv, uf, vf, wf, vis[i], rot, c, key, sr))
+            key += (uint32_t)fd.tile_begin * (uint32_t)(T * T);
+    }
+    hot[i] = sr;
+    keys[i] = key;
+    vals[i] = (uint64_t)(uint32_t)i | ((uint64_t)__float_as_uint(wti) << 32);
+}
+
+// SUMWT, LSUMWT and the normalisation of each field (one workgroup per
+// field) from its work items' sums, summed as k_grid_totals sums a single
+// field's: thread t takes items t, t + 1024, ... of the field in order, then
+// a tree over the threads.  The field's PSF peak key is reset.
+__global__ __launch_bounds__(1024) void k_field_totals(const double *__restrict__ item_sums,
+                                                       const int *__restrict__ item_start,
+                                                       const FieldDesc *__restrict__ fields) {
+    __shared__ double ds[1024], dl[1024];
+    const FieldDesc &fd = fields[blockIdx.x];
+    const int i0 = item_start[fd.tile_begin], i1 = item_start[fd.tile_begin + fd.ntu * fd.ntv];
+    const int t = threadIdx.x;
+    double a = 0.0, b = 0.0;
+    for (int k = i0 + t; k < i1; k += 1024) {
+        a += item_sums[2 * k];
+        b += item_sums[2 * k + 1];
+    }
+    ds[t] = a;
+    dl[t] = b;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (t < s) {
+            ds[t] += ds[t + s];
+            dl[t] += dl[t + s];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double num = (double)((float)(fd.cp.nu - 1) * (float)fd.cp.nv);
+        fd.sums[0] = ds[0];
+        fd.sums[1] = dl[0];
+        fd.sums[2] = dl[0] != 0.0 ? num / dl[0] : 0.0;
+        *fd.peak = (int)0x80808080u;
+    }
+}
+
+// The grids of every field in one launch: blockIdx.z = field, blocks past
+// the field's own NU x NV return.
+__global__ __launch_bounds__(256) void k_grid_reduce_fields(const float *__restrict__ partial,
+                                                            const int *__restrict__ item_start,
+                                                            const FieldDesc *__restrict__ fields, int T,
+                                                            int chmask) {
+    const FieldDesc &fd = fields[blockIdx.z];
+    const int iu = blockIdx.x * blockDim.x + threadIdx.x, iv = blockIdx.y;
+    if (iu >= fd.cp.nu || iv >= fd.cp.nv) return;
+    const CoordParams cp = fd.cp;
+    reduce_cell(partial, item_start + fd.tile_begin, T, fd.ntu, fd.ntv, chmask, cp, fd.sums, iu, iv, fd.dirty,
+                fd.psf);
 }
 
 // ---- w planes (GRDH23D) ----
@@ -1403,6 +1539,8 @@ static int launch_tiles_w(hipStream_t s, const SortBufs &b, const TileParams &tp
         launch_tiles_t<T, 2, W, false, 0, 0>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial, zp);
     else if (chmask == 3 && deg && wgt == 0)
         launch_tiles_t<T, 3, W, true, 0, 0>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial, zp);
+    else if (chmask == 3 && !deg && wgt == 0)
+        launch_tiles_t<T, 3, W, false, 0, 0>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial, zp);
     else if (chmask == 3 && deg && wgt == 1)
         launch_tiles_t<T, 3, W, true, 1, 0>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial, zp);
     else if (chmask == 3 && deg && wgt == 2)
@@ -1434,6 +1572,55 @@ void launch_grid_reduce(hipStream_t s, const float *partial, const int *item_sta
                         int chmask, const CoordParams &cp, const double *sums, float2 *dirty, float2 *psf) {
     dim3 grid((cp.nu + 255) / 256, cp.nv);
     hipLaunchKernelGGL(k_grid_reduce, grid, dim3(256), 0, s, partial, item_start, tp, chmask, cp, sums, dirty, psf);
+}
+
+void launch_prep_fields(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
+                        const float *w, const FieldDesc *fields, int nfields, int nblocks, int T, uint32_t sentinel,
+                        SortBufs &b) {
+    if (nblocks <= 0 || nfields <= 0) return;
+    hipLaunchKernelGGL(k_prep_fields, dim3(nblocks), dim3(kPrepFieldThreads), 0, s, vis, wt, u, v, w, fields, nfields,
+                       T, sentinel, b.keys, b.vals, b.hot);
+}
+
+int launch_grid_tiles_fields(hipStream_t s, const SortBufs &b, const TileParams &tp, const float *cf,
+                             const CoordParams &cp_any, double *item_sums, float *partial, const FieldDesc *fields,
+                             int nfields) {
+    if (b.max_items <= 0 || nfields <= 0) return 0;
+    // natural weights, no degridding halves: the work items read only the
+    // kernel support and sampling of the geometry, the same for every field
+    const int W = tp.waves == 8 ? 8 : 4;
+    const WeightParams wp{};
+    const ZPlanes zp{};
+    int rc = -1;
+    if (tp.T == 16 && W == 4) rc = launch_tiles_w<16, 4>(s, b, tp, 3, false, wp, cf, cf, cp_any, nullptr, item_sums, partial, zp);
+    else if (tp.T == 16 && W == 8) rc = launch_tiles_w<16, 8>(s, b, tp, 3, false, wp, cf, cf, cp_any, nullptr, item_sums, partial, zp);
+    else if (tp.T == 32 && W == 4) rc = launch_tiles_w<32, 4>(s, b, tp, 3, false, wp, cf, cf, cp_any, nullptr, item_sums, partial, zp);
+    else if (tp.T == 32 && W == 8) rc = launch_tiles_w<32, 8>(s, b, tp, 3, false, wp, cf, cf, cp_any, nullptr, item_sums, partial, zp);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_field_totals, dim3(nfields), dim3(1024), 0, s, item_sums, b.item_start, fields);
+    return 0;
+}
+
+void launch_grid_reduce_fields(hipStream_t s, const float *partial, const int *item_start, const FieldDesc *fields,
+                               int nfields, int T, int max_nu, int max_nv) {
+    if (nfields <= 0) return;
+    dim3 grid((max_nu + 255) / 256, max_nv, nfields);
+    hipLaunchKernelGGL(k_grid_reduce_fields, grid, dim3(256), 0, s, partial, item_start, fields, T, 3);
+}
+
+namespace {
+__global__ void k_zero(uint32_t *__restrict__ p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+}  // namespace
+
+void launch_zero(hipStream_t s, void *p, size_t n) {
+    const size_t words = n / 4;
+    if (words == 0) return;
+    size_t nb = (words + 255) / 256;
+    if (nb > 4096) nb = 4096;
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)nb), dim3(256), 0, s, static_cast<uint32_t *>(p), words);
 }
 
 void launch_prep_slowz(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,

@@ -287,6 +287,36 @@ void launch_grid_reduce(hipStream_t s, const float *partial, const int *item_sta
                         int chmask, const CoordParams &cp, const double *sums, float2 *dirty, float2 *psf);
 size_t grid_partial_floats(const TileParams &tp, int chmask, int max_items);
 
+// ---- many fields, one launch per stage (grid.hip, fft.hip) ----
+// One field of a batch: its geometry, where its visibilities, prep blocks
+// and tiles lie in the batch's concatenated arrays and single tile
+// numbering, and where its results go.
+struct FieldDesc {
+    CoordParams cp;
+    int64_t vis_begin = 0, nvis = 0;
+    int block_begin = 0;         // first prep block (kPrepFieldThreads points a block)
+    int tile_begin = 0;          // first tile in the batch's numbering
+    int ntu = 0, ntv = 0;        // the field's tiles
+    float2 *dirty = nullptr, *psf = nullptr;  // gridded planes, NU x NV each
+    double *sums = nullptr;      // SUMWT, LSUMWT, normalisation
+    int *peak = nullptr;         // PSF peak key
+};
+constexpr int kPrepFieldThreads = 256;
+// Keys and hot records of every field (natural weights); one launch.
+void launch_prep_fields(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
+                        const float *w, const FieldDesc *fields, int nfields, int nblocks, int T, uint32_t sentinel,
+                        SortBufs &b);
+// The work items of every field gridded (dirty and PSF, natural weights),
+// then each field's totals (and its peak key reset).
+int launch_grid_tiles_fields(hipStream_t s, const SortBufs &b, const TileParams &tp, const float *cf,
+                             const CoordParams &cp_any, double *item_sums, float *partial, const FieldDesc *fields,
+                             int nfields);
+// Every field's dirty and PSF grids (max_nu x max_nv covers every field).
+void launch_grid_reduce_fields(hipStream_t s, const float *partial, const int *item_start, const FieldDesc *fields,
+                               int nfields, int T, int max_nu, int max_nv);
+// Zero n bytes (a multiple of 4) with a kernel (graph-safe).
+void launch_zero(hipStream_t s, void *p, size_t n);
+
 // ---- BOX gridding, GRDQH2D, on the cell sort (grid.hip) ----
 // Prep pass: GRDQH2D's REAL cell step (grd/grdqh2d.f:85-114); the key is
 // cell_key of the pre-shift cell (UGRID-1, VGRID-1), the hot record holds
@@ -325,6 +355,16 @@ void launch_fft_rows_r2c(hipStream_t s, const float *image, int nx, int ny, int 
                          float2 *grid);
 // image (and image_b when given) times 1/peak
 void launch_scale_by_peak(hipStream_t s, float *image, size_t n, const int *peak_key, float *image_b = nullptr);
+// Row passes of nf fields of one size in one launch: field k's PSF plane
+// grid_psf + k*gstride -> image_psf + k*istride with its peak in peaks[k],
+// and its dirty plane likewise (no peak).
+void launch_fft_rows_c2r_fields(hipStream_t s, const float2 *grid_psf, const float2 *grid_dirty, size_t gstride,
+                                float *image_psf, float *image_dirty, size_t istride, int *peaks, int nf, int nu,
+                                int nv, int fnu, int nx, int ny, const double2 *tw_fnu, const float *ccf, int ccflen,
+                                int correct, float scale);
+// Both images of nf fields times 1/peak of their field.
+void launch_scale_by_peak_fields(hipStream_t s, float *image_psf, float *image_dirty, size_t istride, size_t n,
+                                 const int *peaks, int nf);
 // Two row passes in one launch: grid_a -> image_a with its peak tracked in
 // peak_a, grid_b -> image_b.
 void launch_fft_rows_c2r_pair(hipStream_t s, const float2 *grid_a, float *image_a, int *peak_a,
