@@ -488,7 +488,7 @@ int slowz_grid(Ctx &c, const float2 *dvis, const float *dwt, const float *du, co
     sb.temp = ctx_buf<char>(c, "sort_temp", sb.temp_bytes > 0 ? sb.temp_bytes : 1);
     sb.tile_start = ctx_buf<int>(c, "tile_start", tp.ntiles + 1);
     sb.item_start = ctx_buf<int>(c, "item_start", tp.ntiles + 1);
-    sb.nitems = ctx_buf<int>(c, "nitems", 1);
+    sb.nitems = ctx_buf<int>(c, "nitems", item_scan_ints(tp));
     sb.items = ctx_buf<WorkItem>(c, "items", sb.max_items);
     float *partial = ctx_buf<float>(c, "partial", grid_partial_floats(tp, 1, sb.max_items));
     double *blk = ctx_buf<double>(c, "blk", 2 * (size_t)sb.max_items), *dsums = ctx_buf<double>(c, "sums", 4);
@@ -670,7 +670,7 @@ void grdh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, flo
     sb.temp = ctx_buf<char>(c, "sort_temp", sb.temp_bytes > 0 ? sb.temp_bytes : 1);
     sb.tile_start = ctx_buf<int>(c, "tile_start", tp.ntiles + 1);
     sb.item_start = ctx_buf<int>(c, "item_start", tp.ntiles + 1);
-    sb.nitems = ctx_buf<int>(c, "nitems", 1);
+    sb.nitems = ctx_buf<int>(c, "nitems", item_scan_ints(tp));
     sb.items = ctx_buf<WorkItem>(c, "items", sb.max_items);
     float *partial = ctx_buf<float>(c, "partial", grid_partial_floats(tp, chmask, sb.max_items));
     double *blk = ctx_buf<double>(c, "blk", 2 * (size_t)sb.max_items), *dsums = ctx_buf<double>(c, "sums", 4);
@@ -749,7 +749,7 @@ void grdqh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, fl
     sb.temp = ctx_buf<char>(c, "sort_temp", sb.temp_bytes > 0 ? sb.temp_bytes : 1);
     sb.tile_start = ctx_buf<int>(c, "tile_start", tp.ntiles + 1);
     sb.item_start = ctx_buf<int>(c, "item_start", tp.ntiles + 1);
-    sb.nitems = ctx_buf<int>(c, "nitems", 1);
+    sb.nitems = ctx_buf<int>(c, "nitems", item_scan_ints(tp));
     sb.items = ctx_buf<WorkItem>(c, "items", sb.max_items);
     float *partial = ctx_buf<float>(c, "partial", grid_partial_floats(tp, chmask, sb.max_items));
     double *blk = ctx_buf<double>(c, "blk", 2 * (size_t)sb.max_items), *dsums = ctx_buf<double>(c, "sums", 4);
@@ -1767,7 +1767,7 @@ int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **out) {
              plan_alloc(p, &p->d_dbins, (size_t)kMaxDegridBins) ||
              plan_alloc(p, &p->d_wsq, kWtSqBlocks) ||
              plan_alloc(p, &p->sb.tile_start, p->tp.ntiles + 1) || plan_alloc(p, &p->sb.item_start, p->tp.ntiles + 1) ||
-             plan_alloc(p, &p->sb.nitems, 1) || plan_alloc(p, &p->d_nlist, kPrepBlocks) ||
+             plan_alloc(p, &p->sb.nitems, item_scan_ints(p->tp)) || plan_alloc(p, &p->d_nlist, kPrepBlocks) ||
              plan_alloc(p, &p->d_sums, 4);
     if (rc) {
         sde_plan_destroy(p);
@@ -1968,7 +1968,7 @@ int64_t sde_plan_estimate_bytes(const sde_imaging_cfg *cfg, int64_t nvis) {
                                   cfg->chunk > 0 ? cfg->chunk : default_chunk(nvis), cfg->waves ? cfg->waves : 4);
     size_t b = sizeof(float) * (1 + 2 * 4 * 128 + (size_t)ccflen) + sizeof(float2) * 5 * ng +
                sizeof(float) * 3 * npix + sizeof(double) * ng + sizeof(double) * (4096 + kWtSqBlocks + 8) +
-               sizeof(int) * (2 * ((size_t)tp.ntiles + 1) + 1 + kPrepBlocks + 4) + sizeof(double2) * (size_t)(nv + fnu);
+               sizeof(int) * (2 * ((size_t)tp.ntiles + 1) + (size_t)item_scan_ints(tp) + kPrepBlocks + 4) + sizeof(double2) * (size_t)(nv + fnu);
     if (nvis > 0) {
         const size_t n = (size_t)nvis;
         b += n * (sizeof(float2) * 2 + sizeof(float) * 6 + sizeof(uint32_t) * 2 + sizeof(uint64_t) * 2 +
@@ -2482,7 +2482,7 @@ size_t carve_grid_scratch(unsigned char *base, int64_t n, const TileParams &tp, 
     sb.temp = c.take<char>(sb.temp_bytes);
     sb.tile_start = c.take<int>((size_t)tp.ntiles + 1);
     sb.item_start = c.take<int>((size_t)tp.ntiles + 1);
-    sb.nitems = c.take<int>(1);
+    sb.nitems = c.take<int>((size_t)item_scan_ints(tp));
     sb.items = c.take<WorkItem>((size_t)sb.max_items);
     *partial = c.take<float>(grid_partial_floats(tp, chmask, sb.max_items));
     *item_sums = c.take<double>(2 * (size_t)sb.max_items);
@@ -2799,7 +2799,7 @@ int sde_batch_create(const sde_imaging_cfg *cfgs, int nfields, sde_batch **out) 
     }
     int rc = batch_alloc(b, &b->d_cf, b->ncf) || batch_alloc(b, &b->d_fields, nfields) ||
              batch_alloc(b, &b->d_sums, 4 * (size_t)nfields) || batch_alloc(b, &b->sb.tile_start, tiles + 1) ||
-             batch_alloc(b, &b->sb.item_start, tiles + 1) || batch_alloc(b, &b->sb.nitems, 1);
+             batch_alloc(b, &b->sb.item_start, tiles + 1) || batch_alloc(b, &b->sb.nitems, item_scan_ints(b->tp));
     for (auto &c : b->cls) {
         if (rc) break;
         // (fft_supported also makes the transforms' LDS opt-in, before any capture)

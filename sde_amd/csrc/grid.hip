@@ -10,8 +10,8 @@
 //                      record (and, where needed, a 16-byte cold one).
 //   2. radix sort      (key, index|weight) pairs, stable (sort.hip); the
 //                      density runs or k_tile_bounds give the tile ranges,
-//                      k_items_scan the work items (a tile's slots cut into
-//                      chunks).
+//                      k_items_count/_blocks/_write the work items (a
+//                      tile's slots cut into chunks).
 //   3. k_grid_tiles    one workgroup per work item over a padded tile
 //                      (T+6)^2 held in LDS in FP64.  Each wave gathers the
 //                      staging records of a contiguous run of cell-sorted
@@ -291,54 +291,99 @@ __device__ __forceinline__ int tile_of_key(uint32_t k, uint32_t nkeys, int tt, i
 }
 
 // First sorted slot of every tile (tile_start[ntiles] = number of sorted
-// points): each slot where the tile changes writes the starts of the tiles
-// it closes and opens, so every entry is written once.
+// points): tile_start[x] is the first slot whose tile is x or later -- one
+// thread per tile, a binary search over the sorted keys (their tiles do not
+// decrease; the sentinel's tile is ntiles).
 __global__ void k_tile_bounds(const uint32_t *__restrict__ skeys, int64_t n, uint32_t nkeys, int tt, int ntiles,
                               int *__restrict__ tile_start) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const int t = tile_of_key(skeys[p], nkeys, tt, ntiles);
-    const int tprev = p == 0 ? -1 : tile_of_key(skeys[p - 1], nkeys, tt, ntiles);
-    for (int x = tprev + 1; x <= t; x++) tile_start[x] = (int)p;
-    if (p == n - 1)
-        for (int x = t + 1; x <= ntiles; x++) tile_start[x] = (int)n;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x > ntiles) return;
+    int64_t lo = 0, hi = n;  // first p in [lo, hi) with tile(p) >= x
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) / 2;
+        if (tile_of_key(skeys[mid], nkeys, tt, ntiles) < x) lo = mid + 1;
+        else hi = mid;
+    }
+    tile_start[x] = (int)lo;
 }
 
-// Work items per tile (chunks of at most `chunk` records): their scan, and
-// each thread writes the items of its own tiles.
-__global__ __launch_bounds__(1024) void k_items_scan(const int *__restrict__ tile_start, int ntiles, int chunk,
-                                                     int *__restrict__ item_start, int *__restrict__ nitems,
-                                                     WorkItem *__restrict__ items) {
-    __shared__ int s[1024];
-    const int t = threadIdx.x;
-    const int pt = (ntiles + 1023) / 1024;
-    const int lo = min(t * pt, ntiles), hi = min(lo + pt, ntiles);
-    int its = 0;
-    for (int k = lo; k < hi; k++) its += (tile_start[k + 1] - tile_start[k] + chunk - 1) / chunk;
-    s[t] = its;
+// Work items per tile (chunks of at most `chunk` records), in three
+// launches over blocks of kItemTiles tiles: each block's item count into
+// nitems[1 + b]; one workgroup turns those into block offsets (and writes
+// the total to nitems[0] and item_start[ntiles]); each block then scans its
+// tiles' counts from its offset, writes item_start and its tiles' items.
+constexpr int kItemTiles = 1024;
+
+__device__ __forceinline__ int tile_items(const int *__restrict__ tile_start, int k, int ntiles, int chunk) {
+    return k < ntiles ? (tile_start[k + 1] - tile_start[k] + chunk - 1) / chunk : 0;
+}
+
+// Exclusive scan of x over the 1024 threads of the workgroup; *total gets the sum.
+__device__ __forceinline__ int block_excl_scan(int x, int *total) {
+    __shared__ int wsum[16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int incl = x;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) wsum[wave] = incl;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int x = t >= off ? s[t - off] : 0;
-        __syncthreads();
-        s[t] += x;
-        __syncthreads();
-    }
-    int run = t > 0 ? s[t - 1] : 0;
-    for (int k = lo; k < hi; k++) {
-        item_start[k] = run;
-        const int b = tile_start[k], e = tile_start[k + 1];
-        for (int q = b; q < e; q += chunk, run++) {
-            WorkItem wi;
-            wi.tile = k;
-            wi.begin = q;
-            wi.end = min(q + chunk, e);
-            wi.pad = 0;
-            items[run] = wi;
+    if (threadIdx.x < 64) {
+        int w = lane < 16 ? wsum[lane] : 0, wi = w;
+        for (int off = 1; off < 16; off <<= 1) {
+            const int y = __shfl_up(wi, off);
+            if (lane >= off) wi += y;
         }
+        if (lane < 16) wsum[lane] = wi - w;  // exclusive over waves
+        if (lane == 15) total[0] = wi;
     }
-    if (t == 1023) {
-        item_start[ntiles] = s[1023];
-        *nitems = s[1023];
+    __syncthreads();
+    return wsum[wave] + incl - x;
+}
+
+__global__ __launch_bounds__(1024) void k_items_count(const int *__restrict__ tile_start, int ntiles, int chunk,
+                                                      int *__restrict__ nitems) {
+    __shared__ int tot;
+    const int k = blockIdx.x * kItemTiles + threadIdx.x;
+    (void)block_excl_scan(tile_items(tile_start, k, ntiles, chunk), &tot);
+    if (threadIdx.x == 0) nitems[1 + blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_items_blocks(int nblocks, int ntiles, int *__restrict__ item_start,
+                                                       int *__restrict__ nitems) {
+    __shared__ int tot;
+    int carry = 0;
+    for (int b0 = 0; b0 < nblocks; b0 += 1024) {
+        const int b = b0 + threadIdx.x;
+        const int x = b < nblocks ? nitems[1 + b] : 0;
+        const int e = block_excl_scan(x, &tot);
+        if (b < nblocks) nitems[1 + b] = carry + e;
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        nitems[0] = carry;
+        item_start[ntiles] = carry;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_items_write(const int *__restrict__ tile_start, int ntiles, int chunk,
+                                                      const int *__restrict__ nitems, int *__restrict__ item_start,
+                                                      WorkItem *__restrict__ items) {
+    __shared__ int tot;
+    const int k = blockIdx.x * kItemTiles + threadIdx.x;
+    int run = nitems[1 + blockIdx.x] + block_excl_scan(tile_items(tile_start, k, ntiles, chunk), &tot);
+    if (k >= ntiles) return;
+    item_start[k] = run;
+    const int b = tile_start[k], e = tile_start[k + 1];
+    for (int q = b; q < e; q += chunk, run++) {
+        WorkItem wi;
+        wi.tile = k;
+        wi.begin = q;
+        wi.end = min(q + chunk, e);
+        wi.pad = 0;
+        items[run] = wi;
     }
 }
 
@@ -1402,6 +1447,7 @@ int blocks_for(int64_t n) {
 }  // namespace
 
 int64_t sort_nkeys(const TileParams &tp) { return (int64_t)tp.ntiles * tp.T * tp.T; }
+int item_scan_ints(const TileParams &tp) { return 1 + (tp.ntiles + kItemTiles - 1) / kItemTiles; }
 // bits that order every key and the sentinel nkeys
 int sort_nbits(const TileParams &tp) {
     const int64_t top = sort_nkeys(tp);
@@ -1462,8 +1508,8 @@ int launch_sort(hipStream_t s, SortBufs &b, int64_t n, const TileParams &tp, boo
             hipSuccess)
             return -1;
         if (bounds)
-            hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
-                               b.skeys, n, (uint32_t)sort_nkeys(tp), tp.T * tp.T, tp.ntiles, b.tile_start);
+            hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)((tp.ntiles + kThreads) / kThreads)), dim3(kThreads), 0,
+                               s, b.skeys, n, (uint32_t)sort_nkeys(tp), tp.T * tp.T, tp.ntiles, b.tile_start);
     } else if (hipMemsetAsync(b.tile_start, 0, sizeof(int) * (tp.ntiles + 1), s) != hipSuccess) {
         return -1;
     }
@@ -1472,8 +1518,12 @@ int launch_sort(hipStream_t s, SortBufs &b, int64_t n, const TileParams &tp, boo
 }
 
 void launch_items(hipStream_t s, SortBufs &b, const TileParams &tp) {
-    hipLaunchKernelGGL(k_items_scan, dim3(1), dim3(1024), 0, s, b.tile_start, tp.ntiles, tp.chunk, b.item_start,
-                       b.nitems, b.items);
+    const int nb = (tp.ntiles + kItemTiles - 1) / kItemTiles;
+    if (nb > 0) hipLaunchKernelGGL(k_items_count, dim3(nb), dim3(1024), 0, s, b.tile_start, tp.ntiles, tp.chunk, b.nitems);
+    hipLaunchKernelGGL(k_items_blocks, dim3(1), dim3(1024), 0, s, nb, tp.ntiles, b.item_start, b.nitems);
+    if (nb > 0)
+        hipLaunchKernelGGL(k_items_write, dim3(nb), dim3(1024), 0, s, b.tile_start, tp.ntiles, tp.chunk, b.nitems,
+                           b.item_start, b.items);
 }
 
 void launch_density_runs(hipStream_t s, const SortBufs &b, int64_t n, const TileParams &tp,

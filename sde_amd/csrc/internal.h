@@ -123,11 +123,16 @@ struct SortBufs {
     int cold_all = 0;            // every point has a cold record (else flagged points only)
     void *temp = nullptr;        // radix-sort scratch
     size_t temp_bytes = 0;
-    int *tile_start = nullptr, *item_start = nullptr, *nitems = nullptr;
+    int *tile_start = nullptr, *item_start = nullptr;
+    int *nitems = nullptr;       // item_scan_ints(tp): nitems[0] the work-item count
+
     WorkItem *items = nullptr;
     int max_items = 0;
 };
 int64_t sort_nkeys(const TileParams &tp);
+// ints of SortBufs::nitems: the work-item count, then the item scan's
+// per-block offsets
+int item_scan_ints(const TileParams &tp);
 int sort_nbits(const TileParams &tp);
 size_t sort_temp_bytes(int64_t n, int nbits);  // sort.hip
 // The error word the last sort on temp left (nonzero: a look-back wait timed out).
