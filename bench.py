@@ -36,9 +36,9 @@ PEAK_HBM_GBS = 8000.0
 # a name ending in "*" matches every kernel with that prefix).
 REGION_KERNELS = {
     "prep": ["k_prep<2, true>"],
-    "cell_sort": ["k_sort_init", "k_sort_hist", "k_sort_scan", "k_sort_pass"],
+    "cell_sort": ["k_sort_count*", "k_sort_chunk", "k_sort_chunk_scan", "k_sort_scatter"],
     # the density runs also write the tile ranges; the work items follow in the same region
-    "wt_density": ["k_density_runs", "k_items_scan"],
+    "wt_density": ["k_density_runs", "k_items_count", "k_items_blocks", "k_items_write"],
     "wt_finish": ["k_wt_sumsq", "k_wt_finish"],
     "grid_tiles": ["k_grid_tiles<16, 3, 4, true, 1, 0>", "k_grid_totals"],
     "grid_reduce": ["k_grid_reduce"],
@@ -286,6 +286,21 @@ def bench_mosaic(args):
         batch.uvmap_fields(fb)
     t_eager = min(batch.uvmap_fields(fb) for _ in range(args.steps))
     geo = fb.geometry()
+    if args.fields_only:  # the field batch alone (for profiling its kernels)
+        fb.close()
+        if args.copy_probe:
+            from sde_amd import imaging
+            probe = imaging.Plan(256, None, 1.0)
+            probe.copy_probe(1 << 30, reps=3)
+            probe.close()
+        return {"metric": "mosaic gridded visibilities/sec (dirty + PSF, 64 pointings)",
+                "value": round(nvis / t_graph, 1), "unit": "visibilities/s", "n_gpus": 1, "steps": args.steps,
+                "warmup": args.warmup, "ms_per_step": round(1e3 * t_graph, 3), "higher_is_better": True,
+                "dtype": "f32", "data": "synthetic (seeded)",
+                "config": {"workload": "batched mosaic (field batch only): 64 pointings, log-uniform 2e4-5e5 vis, "
+                                       "512^2/1024^2, natural weights", "nvis": nvis,
+                           "field_batch": dict(field_tune, **geo)},
+                "field_batch_graph_ms": round(1e3 * t_graph, 3), "field_batch_eager_ms": round(1e3 * t_eager, 3)}
     # the same jobs as plans of the batch's launch shape: equal images
     same_plans = batch.make_plans(jobs, **field_tune)
     batch.uvmap_sequential(same_plans)
@@ -548,6 +563,7 @@ def main():
     ap.add_argument("--workload", choices=("large_field", "mosaic", "predict", "served", "major_cycle", "wplanes"),
                     default="large_field",
                     help="large_field: the headline job (default); the others are the batched and served forms")
+    ap.add_argument("--fields-only", action="store_true", help="mosaic: time the field batch only")
     ap.add_argument("--jobs", type=int, default=100, help="served: number of requests")
     ap.add_argument("--rate", type=float, default=400.0, help="served: request rate (1/s)")
     ap.add_argument("--inflight", type=int, default=16, help="served: most jobs in flight")
