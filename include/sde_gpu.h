@@ -436,6 +436,16 @@ int sde_grdh2d_device_noalloc(const void *vis, const float *wt, const float *u, 
                               const double *shift, double *sumwt);
 size_t sde_noalloc_scratch_bytes(void);
 
+/* Test and tuning controls of the w-plane degridder (GRDDEH23 and the cube
+ * predictions; no counterpart in the reference): path 0 (default) the
+ * LDS-window kernel for 4, 8 or 16 planes, 1 the same kernel with no window
+ * staged (every tap gathered from the planes in memory), 2 the per-point
+ * kernel in the reference's loop order for any plane count -- the same
+ * values on every path; and the workgroups of the last call that staged
+ * their window. */
+int sde_slowz_degrid_path(int path);
+long long sde_slowz_staged_blocks(void);
+
 /* The library's stable key/value radix sort on device arrays (bits
  * [0, nbits) of the keys order them; 1 <= nbits <= 32, 0 <= n < 2^31), as the cell sort
  * runs it; synchronous.  Returns 0, or -1 with sde_last_error() set.  No

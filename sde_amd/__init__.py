@@ -58,6 +58,7 @@ def _declare(L):
     L.sde_grdh2d_device_noalloc.restype = _C.c_int
     L.sde_noalloc_scratch_bytes.restype = _C.c_size_t
     L.sde_sort_pairs_device.restype = _C.c_int
+    L.sde_slowz_staged_blocks.restype = _C.c_longlong
     L.sde_host_alloc.restype = _C.c_void_p
     L.sde_host_alloc.argtypes = [_C.c_size_t]
     L.sde_host_free.argtypes = [_C.c_void_p]

@@ -552,6 +552,12 @@ int slowz_box_grid(Ctx &c, const float2 *dvis, const float *dwt, const float *du
     return 0;
 }
 
+// The w-plane degridder's path (launch_degrid_slowz) and the workgroups that
+// staged their window in the last call: test and tuning controls, set and
+// read under the context mutex the drop-ins hold.
+int g_slowz_path = 0;
+long long g_slowz_staged = 0;
+
 // GRDDEH23 on device arrays (in place on vis/wt), synchronised; returns the
 // number of points without kernel support, or -1.
 long long slowz_degrid(Ctx &c, const float2 *dg, float2 *dvis, float *dwt, const float *du, const float *dv,
@@ -578,12 +584,13 @@ long long slowz_degrid(Ctx &c, const float2 *dg, float2 *dvis, float *dwt, const
         set_plan_error("cell sort failed");
         return -1;
     }
-    launch_degrid_slowz(s, dg, inner, pts, dvis, n, cp, dcfu, dcfv, pp, svals, dvis, dwt, cnt);
+    launch_degrid_slowz(s, dg, inner, pts, dvis, n, cp, dcfu, dcfv, pp, svals, dvis, dwt, cnt, g_slowz_path);
     SDE_CHECK(hipGetLastError());
-    unsigned long long nbad = 0;
-    SDE_CHECK(hipMemcpyAsync(&nbad, cnt, sizeof nbad, hipMemcpyDeviceToHost, s));
+    unsigned long long got[2] = {0, 0};
+    SDE_CHECK(hipMemcpyAsync(got, cnt, sizeof got, hipMemcpyDeviceToHost, s));
     SDE_CHECK(hipStreamSynchronize(s));
-    return (long long)nbad;
+    g_slowz_staged = (long long)got[1];
+    return (long long)got[0];
 }
 
 }  // namespace
@@ -2550,6 +2557,21 @@ int sde_grdh2d_device_noalloc(const void *vis, const float *wt, const float *u, 
 }
 
 size_t sde_noalloc_scratch_bytes(void) { return kNoAllocScratchBytes; }
+
+int sde_slowz_degrid_path(int path) {
+    if (path < 0 || path > 2) {
+        set_plan_error("sde_slowz_degrid_path: path 0, 1 or 2");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    g_slowz_path = path;
+    return 0;
+}
+
+long long sde_slowz_staged_blocks(void) {
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    return g_slowz_staged;
+}
 
 int sde_sort_pairs_device(const uint32_t *keys_in, uint32_t *keys_out, const uint64_t *vals_in, uint64_t *vals_out,
                           int64_t n, int nbits) {

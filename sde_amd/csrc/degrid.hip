@@ -310,7 +310,7 @@ __global__ __launch_bounds__(256) void k_degrid_slowz_win(
     const float2 *__restrict__ grid, const float4 *__restrict__ pts, const float2 *vis_in, int64_t nvis,
     CoordParams cp, const float *__restrict__ cfu, const float *__restrict__ cfv, PlaneParams pp,
     const uint64_t *__restrict__ order, float2 *vis_out, float *__restrict__ wt_out,
-    unsigned long long *__restrict__ nbad) {
+    unsigned long long *__restrict__ nbad, int win_cap) {
     __shared__ float2 win[kSlowzWin];
     __shared__ int box[4][4];
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -387,8 +387,9 @@ __global__ __launch_bounds__(256) void k_degrid_slowz_win(
     // [1, NU] (nuc > SUPPU, nuc + SUPPU <= NU), v likewise by the edge test
     const int u0 = umin - cp.suppu, v0 = vmin - cp.suppv;
     const int wu = umax - umin + 1 + 2 * cp.suppu, wv = vmax - vmin + 1 + 2 * cp.suppv;
-    const bool staged = umin <= umax && (int64_t)wu * wv * NZC <= kSlowzWin;
+    const bool staged = umin <= umax && (int64_t)wu * wv * NZC <= win_cap;
     if (staged) {
+        if (threadIdx.x == 0) atomicAdd(&nbad[1], 1ull);  // workgroups that staged their window
         const int n = wu * wv * NZC;
         for (int k = threadIdx.x; k < n; k += 256) {
             const int cell = k / NZC, iz = k - cell * NZC;
@@ -802,16 +803,18 @@ void launch_slowz_keys(hipStream_t s, const float *u, const float *v, const floa
 void launch_degrid_slowz(hipStream_t s, const float2 *grid, float2 *grid_inner, const float4 *pts,
                          const float2 *vis_in, int64_t nvis,
                          const CoordParams &cp, const float *cfu, const float *cfv, const PlaneParams &pp,
-                         const uint64_t *order, float2 *vis_out, float *wt_out, unsigned long long *nbad) {
+                         const uint64_t *order, float2 *vis_out, float *wt_out, unsigned long long *nbad,
+                         int path) {
     if (nvis <= 0) return;
     const size_t ncell = (size_t)cp.nu * cp.nv;
     hipLaunchKernelGGL(k_planes_inner, dim3((unsigned)std::min<size_t>((ncell + 255) / 256, 8192)), dim3(256), 0, s,
                        grid, grid_inner, ncell, pp.nz);
-    if (pp.nz == 4 || pp.nz == 8 || pp.nz == 16) {
+    if ((pp.nz == 4 || pp.nz == 8 || pp.nz == 16) && path != 2) {
         const unsigned nbw = (unsigned)((nvis + 255) / 256);
+        const int cap = path == 1 ? 0 : kSlowzWin;
 #define SDE_DSW(NZC)                                                                                             \
     hipLaunchKernelGGL(k_degrid_slowz_win<NZC>, dim3(nbw), dim3(256), 0, s, grid_inner, pts, vis_in, nvis,          \
-                       cp, cfu, cfv, pp, order, vis_out, wt_out, nbad)
+                       cp, cfu, cfv, pp, order, vis_out, wt_out, nbad, cap)
         if (pp.nz == 4) SDE_DSW(4);
         else if (pp.nz == 8) SDE_DSW(8);
         else SDE_DSW(16);

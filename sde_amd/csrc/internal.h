@@ -645,11 +645,15 @@ void launch_prep_slowz(hipStream_t s, const float2 *vis, const float *wt, const 
 // re-laid cell-major into grid_inner (NU*NV*NZ float2 of scratch) first.
 // The points are taken in the order of `order` (index in the low 32 bits;
 // null: the caller's order); pts holds each point's (U, V, W, WT) as
-// written by launch_slowz_keys.
+// written by launch_slowz_keys.  nbad[0] counts the points without kernel
+// support, nbad[1] the workgroups that staged their tap window in LDS.
+// path 0: the window kernel for 4, 8 or 16 planes, else the per-point one;
+// 1: the window kernel with no window staged; 2: the per-point kernel for
+// every plane count (all three give the same values).
 void launch_degrid_slowz(hipStream_t s, const float2 *grid, float2 *grid_inner, const float4 *pts,
                          const float2 *vis_in, int64_t nvis, const CoordParams &cp, const float *cfu,
                          const float *cfv, const PlaneParams &pp, const uint64_t *order, float2 *vis_out,
-                         float *wt_out, unsigned long long *nbad);
+                         float *wt_out, unsigned long long *nbad, int path = 0);
 // GRDQH23D (BOX w planes): acc (2*NU*NV*NZ doubles) and sums[0] (SUMWT)
 // zeroed by the caller; the fold writes the NZ planes of gvis.
 void launch_box_grid_slowz(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,

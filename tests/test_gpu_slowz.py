@@ -325,3 +325,41 @@ def test_vismodel_slowz_matches_the_oracle(shifted):
     np.testing.assert_array_equal(wg, w32)
     good = wg > 0
     assert_vis_rel_ref(vg[good], v32[good], v64[good], "slowz predict")
+
+
+@pytest.mark.parametrize("nz", [8, 16])
+def test_grddeh23_window_paths_identical(nz):
+    """The dense points of the test above through the three paths of the
+    w-plane degridder: the LDS-window kernel (some workgroups stage their
+    window), the same kernel with no window staged, and the per-point kernel
+    in the reference's loop order -- VIS and WT the same bit for bit."""
+    nu, nv, vo, cellz, refz = 257, 512, 256, 2.0, 4.5
+    rng = np.random.default_rng(48)
+    n = 150_000
+    u = rng.normal(0.0, 20.0, n)
+    v = rng.normal(0.0, 20.0, n)
+    u[:4000] = rng.uniform(-3.5, 3.5, 4000)
+    u, v = u.astype(np.float32), v.astype(np.float32)
+    w = rng.uniform(-8, 8, n).astype(np.float32)
+    wt = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    wt[::29] = -1.0
+    g32 = (rng.standard_normal((nz, nv, nu)) + 1j * rng.standard_normal((nz, nv, nu))).astype(np.complex64)
+    cf, supp, osamp = po.grdcf("SF")
+    vin = np.full(n, 1 - 2j, np.complex64)
+    L = sde.lib()
+    out, staged = [], []
+    try:
+        for path in (0, 1, 2):
+            assert L.sde_slowz_degrid_path(path) == 0
+            print("grddeh23 nz=%d path %d" % (nz, path), flush=True)
+            out.append(grd.grddeh23(vin, wt, u, v, w, 1.0, 0.0, 1.0, 0.0, 1, vo, g32, cf, supp, osamp, cellz, refz))
+            staged.append(L.sde_slowz_staged_blocks())
+    finally:
+        L.sde_slowz_degrid_path(0)
+    print("staged workgroups per path:", staged)
+    assert staged[0] > 0 and staged[1] == 0 and staged[2] == 0
+    for k in (1, 2):
+        np.testing.assert_array_equal(out[k][1], out[0][1], err_msg="WT path %d" % k)
+        np.testing.assert_array_equal(out[k][0], out[0][0], err_msg="VIS path %d" % k)
+    assert L.sde_slowz_degrid_path(3) != 0
+    sde.clear_error()
