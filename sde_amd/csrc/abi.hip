@@ -1281,6 +1281,7 @@ struct sde_plan {
     bool predicted = false;  // model VIS/WT of the resident visibilities are complete
     bool imaged = false;     // the images hold a uvmap's output (sums valid)
     DegridRec *d_recd = nullptr;
+    float *d_swt = nullptr;  // GRDDEH2D's SUMWT per kernel phase pair (launch_degrid_sumwt)
     double *d_wsq = nullptr;
     int *d_list = nullptr, *d_nlist = nullptr;
     float *d_partial = nullptr;
@@ -1491,7 +1492,8 @@ int plan_vismodel(sde_plan *p, const float *img) {
     plan_mark_begin(p, "degrid");
     if (!bins.reuse) SDE_CHECK(hipMemsetAsync(p->d_dbins, 0, sizeof(int) * (size_t)bins.nbins, s));
     const int rc = launch_degrid_tiles(s, p->d_recd, p->sb.items, p->sb.nitems, p->sb.max_items, p->tp, p->d_mgrid,
-                                       p->d_cf, p->d_cf, p->cp, p->d_mvis, p->d_mwt, p->d_cnt + 1, &bins);
+                                       p->d_cf, p->d_cf, p->cp, p->d_swt, p->d_mvis, p->d_mwt, p->d_cnt + 1,
+                                       &bins);
     plan_mark_end(p);
     SDE_CHECK(hipStreamWaitEvent(s, p->ev_join, 0));
     if (rc) {
@@ -1775,7 +1777,7 @@ int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **out) {
              plan_alloc(p, &p->d_wsq, kWtSqBlocks) ||
              plan_alloc(p, &p->sb.tile_start, p->tp.ntiles + 1) || plan_alloc(p, &p->sb.item_start, p->tp.ntiles + 1) ||
              plan_alloc(p, &p->sb.nitems, item_scan_ints(p->tp)) || plan_alloc(p, &p->d_nlist, kPrepBlocks) ||
-             plan_alloc(p, &p->d_sums, 4);
+             plan_alloc(p, &p->d_sums, 4) || plan_alloc(p, &p->d_swt, degrid_sumwt_floats(p->cp));
     if (rc) {
         sde_plan_destroy(p);
         return -1;
@@ -1790,6 +1792,7 @@ int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **out) {
         hipMemcpyAsync(p->d_cf, cf, sizeof(float) * p->ncf, hipMemcpyHostToDevice, p->stream) != hipSuccess ||
         hipMemcpyAsync(p->d_ccf, ccf.data(), sizeof(float) * p->ccflen, hipMemcpyHostToDevice, p->stream) !=
             hipSuccess ||
+        (launch_degrid_sumwt(p->stream, p->d_cf, p->d_cf, p->cp, p->d_swt), hipGetLastError()) != hipSuccess ||
         hipStreamSynchronize(p->stream) != hipSuccess) {
         set_plan_error("sde_plan_create: table upload failed");
         sde_plan_destroy(p);
@@ -1973,7 +1976,11 @@ int64_t sde_plan_estimate_bytes(const sde_imaging_cfg *cfg, int64_t nvis) {
     const size_t ng = (size_t)nu * nv, npix = (size_t)cfg->nx * cfg->ny;
     const TileParams tp = make_tp(nu, nv, cfg->tile ? cfg->tile : default_tile(nu, nv),
                                   cfg->chunk > 0 ? cfg->chunk : default_chunk(nvis), cfg->waves ? cfg->waves : 4);
-    size_t b = sizeof(float) * (1 + 2 * 4 * 128 + (size_t)ccflen) + sizeof(float2) * 5 * ng +
+    float cft[1 + 2 * 4 * 128];
+    int supp = 0, osamp = 0;
+    if (make_cf(fstr(cfg->cftype, strnlen(cfg->cftype, sizeof cfg->cftype)), cft, &supp, &osamp)) return -1;
+    const size_t nswt = (size_t)(osamp + 2) * (osamp + 2);
+    size_t b = sizeof(float) * (1 + 2 * 4 * 128 + (size_t)ccflen + nswt) + sizeof(float2) * 5 * ng +
                sizeof(float) * 3 * npix + sizeof(double) * ng + sizeof(double) * (4096 + kWtSqBlocks + 8) +
                sizeof(int) * (2 * ((size_t)tp.ntiles + 1) + (size_t)item_scan_ints(tp) + kPrepBlocks + 4) + sizeof(double2) * (size_t)(nv + fnu);
     if (nvis > 0) {

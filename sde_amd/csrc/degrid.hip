@@ -505,15 +505,17 @@ __global__ __launch_bounds__(256) void k_degrid_list(
 // is reserved with a global atomic, and each result goes to its bin's run
 // (a contiguous stretch of the bin); k_degrid_scatter then puts them in
 // input order.
-template <int T, int S, bool BIN>
+template <int T, int S, bool BIN, bool R8>
 __global__ __launch_bounds__(256) void k_degrid_tiles(const DegridRec *__restrict__ rec,
                                                       const WorkItem *__restrict__ items,
                                                       const int *__restrict__ nitems, TileParams tp,
                                                       const float2 *__restrict__ grid,
                                                       const float *__restrict__ cfu_g,
                                                       const float *__restrict__ cfv_g, CoordParams cp,
+                                                      const float *__restrict__ swt,
                                                       float2 *__restrict__ vis_out, float *__restrict__ wt_out,
                                                       unsigned long long *__restrict__ nbad, DegridBins bins) {
+    const DegridRec8 *__restrict__ rec8 = reinterpret_cast<const DegridRec8 *>(rec);
     constexpr int PL = T + 2 * kHaloD;
     constexpr int NB = BIN ? kMaxDegridBins : 1;
     __shared__ float2 g[PL * PL];
@@ -526,7 +528,7 @@ __global__ __launch_bounds__(256) void k_degrid_tiles(const DegridRec *__restric
         for (int k = threadIdx.x; k < bins.nbins; k += blockDim.x) bcnt[k] = 0;
         __syncthreads();
         for (int r = wi.begin + threadIdx.x; r < wi.end; r += blockDim.x) {
-            const uint32_t id = rec[r].idx;
+            const uint32_t id = R8 ? rec8[r].idx : rec[r].idx;
             if (id != kNoDegrid) atomicAdd(&bcnt[(id & 0x7fffffffu) >> bins.shift], 1);
         }
         __syncthreads();
@@ -554,14 +556,25 @@ __global__ __launch_bounds__(256) void k_degrid_tiles(const DegridRec *__restric
     for (int k = threadIdx.x; k < ncfv; k += blockDim.x) cflv[k] = cfv_g[cp.osampv - poffv + k];
     __syncthreads();
     for (int r = wi.begin + threadIdx.x; r < wi.end; r += blockDim.x) {
-        const DegridRec d = rec[r];
+        DegridRec d;
+        if (R8) {
+            const DegridRec8 d8 = rec8[r];
+            d.idx = d8.idx;
+            d.pk = d8.pk;
+            d.rr = 1.0f;
+            d.ri = 0.0f;
+        } else {
+            d = rec[r];
+        }
         if (d.idx == kNoDegrid) continue;
         const int lu = d.pk & 0xff, lv = (d.pk >> 8) & 0xff;
         const int pu = (d.pk >> 16) & 0xff, pv = d.pk >> 24;
+        // SUMWT depends on the phases alone: the table holds it, summed in this loop's order
+        const float sumwt = swt[pu * (cp.osampv + 2) + pv];
         float cu[2 * S + 1];
 #pragma unroll
         for (int k = 0; k < 2 * S + 1; k++) cu[k] = cflu[pu + cp.osampu * k];
-        float vr = 0.0f, vi = 0.0f, sumwt = 0.0f;
+        float vr = 0.0f, vi = 0.0f;
 #pragma unroll
         for (int offv = -S; offv <= S; offv++) {
             const float cv = cflv[pv + cp.osampv * (offv + S)];
@@ -572,7 +585,6 @@ __global__ __launch_bounds__(256) void k_degrid_tiles(const DegridRec *__restric
                 const float2 gg = grow[offu];
                 vr = vr + gg.x * uvwt;
                 vi = vi + gg.y * uvwt;
-                sumwt = sumwt + uvwt;
             }
         }
         const uint32_t idx = d.idx & 0x7fffffffu;
@@ -601,6 +613,26 @@ __global__ __launch_bounds__(256) void k_degrid_tiles(const DegridRec *__restric
             vis_out[idx] = out;
         }
     }
+}
+
+// The sum of the kernel weights of every phase pair, as k_degrid_tiles's
+// loop would form it (v taps outer, u taps inner, REAL sums).
+__global__ __launch_bounds__(256) void k_degrid_sumwt(const float *__restrict__ cfu_g, const float *__restrict__ cfv_g,
+                                                      CoordParams cp, float *__restrict__ swt) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int npv = cp.osampv + 2;
+    if (k >= (cp.osampu + 2) * npv) return;
+    const int pu = k / npv, pv = k - pu * npv;
+    const int poffu = (cp.osampu + 1) / 2, poffv = (cp.osampv + 1) / 2;
+    float sumwt = 0.0f;
+    for (int offv = -cp.suppv; offv <= cp.suppv; offv++) {
+        const float cv = cfv_g[cp.osampv - poffv + pv + cp.osampv * (offv + cp.suppv)];
+        for (int offu = -cp.suppu; offu <= cp.suppu; offu++) {
+            const float uvwt = cfu_g[cp.osampu - poffu + pu + cp.osampu * (offu + cp.suppu)] * cv;
+            sumwt = sumwt + uvwt;
+        }
+    }
+    swt[k] = sumwt;
 }
 
 // Second pass of the binned degridder: the records of bin b move to
@@ -839,18 +871,23 @@ void launch_degrid_list(hipStream_t s, const float2 *grid, const float *u, const
 template <int T, int S>
 static void launch_degrid_tiles_t(hipStream_t s, const DegridRec *rec, const WorkItem *items, const int *nitems,
                                   int max_items, const TileParams &tp, const float2 *grid, const float *cfu,
-                                  const float *cfv, const CoordParams &cp, float2 *vis_out, float *wt_out,
-                                  unsigned long long *nbad, const DegridBins *bins) {
+                                  const float *cfv, const CoordParams &cp, const float *swt, float2 *vis_out,
+                                  float *wt_out, unsigned long long *nbad, const DegridBins *bins) {
     const int ncfu = cp.osampu * (2 * S + 1) + 2;
     const int ncfv = cp.osampv * (2 * S + 1) + 2;
     const size_t lds = sizeof(float) * (ncfu + ncfv);
+    const bool r8 = cp.doshift == 0;
+#define SDE_DT(BINV, R8V, B)                                                                                     \
+    hipLaunchKernelGGL((k_degrid_tiles<T, S, BINV, R8V>), dim3(max_items), dim3(256), lds, s, rec, items, nitems, \
+                       tp, grid, cfu, cfv, cp, swt, vis_out, wt_out, nbad, B)
     if (!bins) {
-        hipLaunchKernelGGL((k_degrid_tiles<T, S, false>), dim3(max_items), dim3(256), lds, s, rec, items, nitems, tp,
-                           grid, cfu, cfv, cp, vis_out, wt_out, nbad, DegridBins{});
+        if (r8) SDE_DT(false, true, DegridBins{});
+        else SDE_DT(false, false, DegridBins{});
         return;
     }
-    hipLaunchKernelGGL((k_degrid_tiles<T, S, true>), dim3(max_items), dim3(256), lds, s, rec, items, nitems, tp,
-                       grid, cfu, cfv, cp, vis_out, wt_out, nbad, *bins);
+    if (r8) SDE_DT(true, true, *bins);
+    else SDE_DT(true, false, *bins);
+#undef SDE_DT
     // one point per thread: a bin of 2^shift points in 2^pshift workgroups
     const int pshift = bins->shift - kDegridScatterLog;
     hipLaunchKernelGGL(k_degrid_scatter, dim3(bins->nbins << pshift), dim3(1 << kDegridScatterLog), 0, s, bins->vis,
@@ -868,17 +905,24 @@ DegridBins degrid_bins_for(int64_t n, float2 *vis, uint32_t *idx, int *count) {
     return b;
 }
 
+size_t degrid_sumwt_floats(const CoordParams &cp) { return (size_t)(cp.osampu + 2) * (cp.osampv + 2); }
+
+void launch_degrid_sumwt(hipStream_t s, const float *cfu, const float *cfv, const CoordParams &cp, float *swt) {
+    const int n = (int)degrid_sumwt_floats(cp);
+    hipLaunchKernelGGL(k_degrid_sumwt, dim3((n + 255) / 256), dim3(256), 0, s, cfu, cfv, cp, swt);
+}
+
 int launch_degrid_tiles(hipStream_t s, const DegridRec *rec, const WorkItem *items, const int *nitems,
                         int max_items, const TileParams &tp, const float2 *grid, const float *cfu,
-                        const float *cfv, const CoordParams &cp, float2 *vis_out, float *wt_out,
-                        unsigned long long *nbad, const DegridBins *bins) {
+                        const float *cfv, const CoordParams &cp, const float *swt, float2 *vis_out,
+                        float *wt_out, unsigned long long *nbad, const DegridBins *bins) {
     if (max_items <= 0) return 0;
     if (cp.suppu != cp.suppv) return -1;
     if (bins && (bins->nbins < 1 || bins->nbins > kMaxDegridBins)) return -1;
 #define SDE_DEG_CASE(TT, SS)                                                                                    \
     if (tp.T == TT && cp.suppu == SS) {                                                                         \
-        launch_degrid_tiles_t<TT, SS>(s, rec, items, nitems, max_items, tp, grid, cfu, cfv, cp, vis_out, wt_out, \
-                                      nbad, bins);                                                              \
+        launch_degrid_tiles_t<TT, SS>(s, rec, items, nitems, max_items, tp, grid, cfu, cfv, cp, swt, vis_out,    \
+                                      wt_out, nbad, bins);                                                      \
         return 0;                                                                                               \
     }
     SDE_DEG_CASE(16, 1)

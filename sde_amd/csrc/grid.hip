@@ -521,10 +521,18 @@ __device__ __forceinline__ DegridRec degrid_rec(const StageHot &sr, float2 rot, 
     return d;
 }
 
+__device__ __forceinline__ DegridRec8 degrid_rec8(const StageHot &sr, uint32_t i) {
+    DegridRec8 d;
+    d.idx = (sr.gpk & kStageDegTile) ? (i | ((sr.gpk & kStageConj) ? 0x80000000u : 0u)) : kNoDegrid;
+    d.pk = sr.dpk;
+    return d;
+}
+
 // Degridding halves in sorted order, one thread per sorted slot, from the
 // staged records of the slot's visibility (one 16-byte gather per point,
-// the rotation only with a shift; coalesced writes).  Used when the
-// records were not built by a gridding run.
+// the rotation only with a shift; coalesced writes of 16-byte records with
+// a shift, else 8-byte ones).  Used when the records were not built by a
+// gridding run.
 __global__ __launch_bounds__(kThreads) void k_degrid_gather(const uint64_t *__restrict__ svals,
                                                              const StageHot *__restrict__ hot,
                                                              const StageCold *__restrict__ cold, int doshift,
@@ -534,7 +542,8 @@ __global__ __launch_bounds__(kThreads) void k_degrid_gather(const uint64_t *__re
     for (int64_t p = (int64_t)blockIdx.x * kThreads + threadIdx.x; p < nvalid; p += (int64_t)gridDim.x * kThreads) {
         const uint32_t i = sval_index(svals[p]);
         const StageHot h = hot[i];
-        rec_d[p] = degrid_rec(h, doshift ? cold[i].rot : make_float2(1.0f, 0.0f), i);
+        if (doshift) rec_d[p] = degrid_rec(h, cold[i].rot, i);
+        else reinterpret_cast<DegridRec8 *>(rec_d)[p] = degrid_rec8(h, i);
     }
 }
 
@@ -769,7 +778,10 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
                 mine.im = PSF_ONLY ? 0.0f : (float)((double)wti * (double)fi);
             }
             if (ZP == 1 && valid) zp.sorted[p] = st0;
-            if (DEG && valid) rec_d[p] = degrid_rec(st0, sc.rot, i0);
+            if (DEG && valid) {
+                if (shifted) rec_d[p] = degrid_rec(st0, sc.rot, i0);
+                else reinterpret_cast<DegridRec8 *>(rec_d)[p] = degrid_rec8(st0, i0);
+            }
 
             // grid the batch.  Each lane first looks up its record's
             // kernel values (SUPPU*2+1 along u, SUPPV*2+1 along v) into its

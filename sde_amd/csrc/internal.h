@@ -59,6 +59,12 @@ struct __align__(16) DegridRec {
     float rr, ri;
 };
 constexpr uint32_t kNoDegrid = 0xffffffffu;
+// Without a shift the rotation is (1, 0) for every point and the record is
+// its first half (8 bytes): idx and pk.  The buffer holding 16-byte records
+// holds these at the same slot index, twice as densely.
+struct __align__(8) DegridRec8 {
+    uint32_t idx, pk;
+};
 
 struct __align__(16) WorkItem {
     int tile, begin, end, pad;
@@ -601,14 +607,23 @@ constexpr int kDegridScatterLog = 8;  // log2 of the points (one per thread) per
 // kMaxDegridBins of them.
 DegridBins degrid_bins_for(int64_t n, float2 *vis, uint32_t *idx, int *count);
 
+// GRDDEH2D's sum of the kernel weights of each (u phase, v phase) pair, in
+// its loop order off the axis (v taps outer, u taps inner;
+// grd/grddeh2d.f:163-171): swt[pu*(OSAMPV+2) + pv], (OSAMPU+2)*(OSAMPV+2)
+// floats, pu and pv the phases as the degridding records pack them (0 to
+// OSAMP + 1: NINT of OSAMP*(NUC-UCELL) reaches +-(OSAMP+1)/2 for odd OSAMP).
+size_t degrid_sumwt_floats(const CoordParams &cp);
+void launch_degrid_sumwt(hipStream_t s, const float *cfu, const float *cfv, const CoordParams &cp, float *swt);
 // Tile path: one thread per record of the degridding halves (points near
 // the u = 0 axis, or not in the cell sort, go to the list for launch_degrid).
+// The records are 16-byte DegridRec with a shift (cp.doshift), else 8-byte
+// DegridRec8; swt is the table of launch_degrid_sumwt for these kernels.
 // With bins (count zeroed by the caller) the results reach vis_out through
 // them; without, each thread writes its point's result in place.
 int launch_degrid_tiles(hipStream_t s, const DegridRec *rec, const WorkItem *items, const int *nitems,
                         int max_items, const TileParams &tp, const float2 *grid, const float *cfu,
-                        const float *cfv, const CoordParams &cp, float2 *vis_out, float *wt_out,
-                        unsigned long long *nbad, const DegridBins *bins = nullptr);
+                        const float *cfv, const CoordParams &cp, const float *swt, float2 *vis_out,
+                        float *wt_out, unsigned long long *nbad, const DegridBins *bins = nullptr);
 void launch_wt_density(hipStream_t s, const float *wt, const float *u, const float *v,
                        const float *w, int64_t nvis, const CoordParams &cp, double *gwt,
                        double *blk, int nblk);
