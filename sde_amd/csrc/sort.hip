@@ -101,8 +101,9 @@ template <bool VEC>
 __global__ __launch_bounds__(kCountThr) void k_sort_count(const uint32_t *__restrict__ keys, int64_t n, int shift,
                                                          int bits, unsigned *__restrict__ tcount) {
     constexpr int kSub = 16;  // sub-histograms: (wave, lane & 3)
-    __shared__ unsigned h[kSub][kBins];
-    for (int k = threadIdx.x; k < kSub * kBins; k += kCountThr) (&h[0][0])[k] = 0u;
+    constexpr int kStride = kBins + 1;  // one word apart on the banks for each sub-histogram
+    __shared__ unsigned h[kSub][kStride];
+    for (int k = threadIdx.x; k < kSub * kStride; k += kCountThr) (&h[0][0])[k] = 0u;
     __syncthreads();
     const unsigned mask = (1u << bits) - 1u;
     const int lane = threadIdx.x & 63, sub = (threadIdx.x >> 6) * 4 + (lane & 3);
