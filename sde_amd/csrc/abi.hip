@@ -1201,14 +1201,14 @@ void fftr2d_(float *rarray, float *xarray, float *work, float *work2, float *rwo
     if (*dir < 0) {
         SDE_F(hipMemcpyAsync(dg, xarray, sizeof(float2) * ng, hipMemcpyHostToDevice, s));
         launch_fft_cols(s, dg, dg, nu, *nv, 1, 0, -1, 1.0f / (float)*nv, tw_nv);
-        launch_fft_rows_c2r(s, dg, nu, *nv, *fnu, *nx, *ny, tw_fnu, nullptr, 1, 0, 1.0f / (float)*fnu, dimg,
+        launch_fft_rows_c2r(s, dg, nu, *nv, *fnu, *nx, *ny, tw_fnu, nullptr, 1.0f / (float)*fnu, dimg,
                             nullptr, nullptr);
         SDE_F(hipGetLastError());
         SDE_F(hipMemcpyAsync(rarray, dimg, sizeof(float) * npix, hipMemcpyDeviceToHost, s));
         SDE_F(hipMemcpyAsync(xarray, dg, sizeof(float2) * ng, hipMemcpyDeviceToHost, s));
     } else {
         SDE_F(hipMemcpyAsync(dimg, rarray, sizeof(float) * npix, hipMemcpyHostToDevice, s));
-        launch_fft_rows_r2c(s, dimg, *nx, *ny, *fnu, *nv, tw_fnu, nullptr, 1, 0, dg);
+        launch_fft_rows_r2c(s, dimg, *nx, *ny, *fnu, *nv, tw_fnu, nullptr, dg);
         launch_fft_cols(s, dg, dg, nu, *nv, 1, 0, +1, 1.0f, tw_nv);
         SDE_F(hipGetLastError());
         SDE_F(hipMemcpyAsync(xarray, dg, sizeof(float2) * ng, hipMemcpyDeviceToHost, s));
@@ -1282,6 +1282,7 @@ struct sde_plan {
     bool imaged = false;     // the images hold a uvmap's output (sums valid)
     DegridRec *d_recd = nullptr;
     float *d_swt = nullptr;  // GRDDEH2D's SUMWT per kernel phase pair (launch_degrid_sumwt)
+    float *d_rc = nullptr;   // reciprocal correction factors of the row passes (launch_corr_recip)
     double *d_wsq = nullptr;
     int *d_list = nullptr, *d_nlist = nullptr;
     float *d_partial = nullptr;
@@ -1461,7 +1462,7 @@ int plan_vismodel(sde_plan *p, const float *img) {
     hipStream_t s = p->stream;
     double2 *twnv = p->tw[p->nv], *twfnu = p->tw[p->fnu];
     plan_mark_begin(p, "fft_rows_model");
-    launch_fft_rows_r2c(s, img, p->cfg.nx, p->cfg.ny, p->fnu, p->nv, twfnu, p->d_ccf, p->ccflen, 1, p->d_mgrid);
+    launch_fft_rows_r2c(s, img, p->cfg.nx, p->cfg.ny, p->fnu, p->nv, twfnu, p->d_rc, p->d_mgrid);
     plan_mark_end(p);
     plan_mark_begin(p, "fft_cols_model");
     launch_fft_cols(s, p->d_mgrid, p->d_mgrid, p->nu, p->nv, 1, 0, +1, 1.0f, twnv);
@@ -1694,7 +1695,7 @@ int plan_transform(sde_plan *p) {
     // images divided by that peak (task/uvmap.f:434-440,480-482)
     plan_mark_begin(p, "fft_rows");
     launch_fft_rows_c2r_pair(s, cpsf, p->d_psf, p->d_peak, cd, p->d_dirty, p->nu, p->nv, p->fnu, p->cfg.nx,
-                             p->cfg.ny, twfnu, p->d_ccf, p->ccflen, 1, scale);
+                             p->cfg.ny, twfnu, p->d_rc, scale);
     plan_mark_end(p);
     plan_mark_begin(p, "peak_scale");
     launch_scale_by_peak(s, p->d_psf, (size_t)p->cfg.nx * p->cfg.ny, p->d_peak, p->d_dirty);
@@ -1777,7 +1778,8 @@ int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **out) {
              plan_alloc(p, &p->d_wsq, kWtSqBlocks) ||
              plan_alloc(p, &p->sb.tile_start, p->tp.ntiles + 1) || plan_alloc(p, &p->sb.item_start, p->tp.ntiles + 1) ||
              plan_alloc(p, &p->sb.nitems, item_scan_ints(p->tp)) || plan_alloc(p, &p->d_nlist, kPrepBlocks) ||
-             plan_alloc(p, &p->d_sums, 4) || plan_alloc(p, &p->d_swt, degrid_sumwt_floats(p->cp));
+             plan_alloc(p, &p->d_sums, 4) || plan_alloc(p, &p->d_swt, degrid_sumwt_floats(p->cp)) ||
+             plan_alloc(p, &p->d_rc, (size_t)cfg->nx + cfg->ny);
     if (rc) {
         sde_plan_destroy(p);
         return -1;
@@ -1793,6 +1795,8 @@ int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **out) {
         hipMemcpyAsync(p->d_ccf, ccf.data(), sizeof(float) * p->ccflen, hipMemcpyHostToDevice, p->stream) !=
             hipSuccess ||
         (launch_degrid_sumwt(p->stream, p->d_cf, p->d_cf, p->cp, p->d_swt), hipGetLastError()) != hipSuccess ||
+        (launch_corr_recip(p->stream, p->d_ccf, p->ccflen, cfg->nx, cfg->ny, p->d_rc), hipGetLastError()) !=
+            hipSuccess ||
         hipStreamSynchronize(p->stream) != hipSuccess) {
         set_plan_error("sde_plan_create: table upload failed");
         sde_plan_destroy(p);
@@ -1980,7 +1984,8 @@ int64_t sde_plan_estimate_bytes(const sde_imaging_cfg *cfg, int64_t nvis) {
     int supp = 0, osamp = 0;
     if (make_cf(fstr(cfg->cftype, strnlen(cfg->cftype, sizeof cfg->cftype)), cft, &supp, &osamp)) return -1;
     const size_t nswt = (size_t)(osamp + 2) * (osamp + 2);
-    size_t b = sizeof(float) * (1 + 2 * 4 * 128 + (size_t)ccflen + nswt) + sizeof(float2) * 5 * ng +
+    size_t b = sizeof(float) * (1 + 2 * 4 * 128 + (size_t)ccflen + nswt + (size_t)cfg->nx + cfg->ny) +
+               sizeof(float2) * 5 * ng +
                sizeof(float) * 3 * npix + sizeof(double) * ng + sizeof(double) * (4096 + kWtSqBlocks + 8) +
                sizeof(int) * (2 * ((size_t)tp.ntiles + 1) + (size_t)item_scan_ints(tp) + kPrepBlocks + 4) + sizeof(double2) * (size_t)(nv + fnu);
     if (nvis > 0) {
@@ -2058,14 +2063,16 @@ int sde_vistoimg_slowz_device(const void *vis, const float *wt, const float *u, 
     hipStream_t s = c.stream;
     const int ncf = 1 + 2 * (supp + 1) * osamp;
     float *dcf = ctx_buf<float>(c, "cfz", ncf), *dccf = ctx_buf<float>(c, "ccfz", ccflen);
+    float *drc = ctx_buf<float>(c, "corr_recip", (size_t)nx + ny);
     float2 *work = ctx_buf<float2>(c, "planework", (size_t)nu * nv);
     double2 *twnv = nullptr, *twfnu = nullptr;
-    if (!dcf || !dccf || !work || get_twiddles(c.tw, nv, s, &twnv) || get_twiddles(c.tw, fnu, s, &twfnu)) {
+    if (!dcf || !dccf || !drc || !work || get_twiddles(c.tw, nv, s, &twnv) || get_twiddles(c.tw, fnu, s, &twfnu)) {
         set_plan_error("VISTOIMG: device allocation failed");
         return -1;
     }
     SDE_CHECK(hipMemcpyAsync(dcf, cf.data(), sizeof(float) * ncf, hipMemcpyHostToDevice, s));
     SDE_CHECK(hipMemcpyAsync(dccf, ccf.data(), sizeof(float) * ccflen, hipMemcpyHostToDevice, s));
+    launch_corr_recip(s, dccf, ccflen, nx, ny, drc);
     const CoordParams cp = make_cp(shift, uscale, 0.0f, vscale, 0.0f, uorigin, vorigin, nu, nv, supp, supp, osamp,
                                    osamp);
     double out[2];
@@ -2086,8 +2093,7 @@ int sde_vistoimg_slowz_device(const void *vis, const float *wt, const float *u, 
     const size_t np = (size_t)nu * nv, npix = (size_t)nx * ny;
     for (int iz = 0; iz < nz; iz++) {
         launch_fft_cols(s, g + iz * np, work, nu, nv, 1, 0, -1, 1.0f, twnv);
-        launch_fft_rows_c2r(s, work, nu, nv, fnu, nx, ny, twfnu, dccf, ccflen, 1, scale, images + iz * npix, nullptr,
-                            nullptr);
+        launch_fft_rows_c2r(s, work, nu, nv, fnu, nx, ny, twfnu, drc, scale, images + iz * npix, nullptr, nullptr);
     }
     SDE_CHECK(hipGetLastError());
     SDE_CHECK(hipStreamSynchronize(s));
@@ -2125,18 +2131,20 @@ int64_t sde_imgtovis_slowz_device(const float *images, int nx, int ny, int fnu, 
     hipStream_t s = c.stream;
     const int ncf = 1 + 2 * (supp + 1) * osamp;
     float *dcf = ctx_buf<float>(c, "cfz", ncf), *dccf = ctx_buf<float>(c, "ccfz", ccflen);
+    float *drc = ctx_buf<float>(c, "corr_recip", (size_t)nx + ny);
     double2 *twnv = nullptr, *twfnu = nullptr;
-    if (!dcf || !dccf || get_twiddles(c.tw, nv, s, &twnv) || get_twiddles(c.tw, fnu, s, &twfnu)) {
+    if (!dcf || !dccf || !drc || get_twiddles(c.tw, nv, s, &twnv) || get_twiddles(c.tw, fnu, s, &twfnu)) {
         set_plan_error("VISDEGRI: device allocation failed");
         return -1;
     }
     SDE_CHECK(hipMemcpyAsync(dcf, cf.data(), sizeof(float) * ncf, hipMemcpyHostToDevice, s));
     SDE_CHECK(hipMemcpyAsync(dccf, ccf.data(), sizeof(float) * ccflen, hipMemcpyHostToDevice, s));
+    launch_corr_recip(s, dccf, ccflen, nx, ny, drc);
     // IMGGRIDC 'CORRECT' and IMGFFT one plane at a time (img/imggridc.f:103-109, arr/arrfftr.f:143-151)
     float2 *g = (float2 *)gvis;
     const size_t np = (size_t)nu * nv, npix = (size_t)nx * ny;
     for (int iz = 0; iz < nz; iz++) {
-        launch_fft_rows_r2c(s, images + iz * npix, nx, ny, fnu, nv, twfnu, dccf, ccflen, 1, g + iz * np);
+        launch_fft_rows_r2c(s, images + iz * npix, nx, ny, fnu, nv, twfnu, drc, g + iz * np);
         launch_fft_cols(s, g + iz * np, g + iz * np, nu, nv, 1, 0, +1, 1.0f, twnv);
     }
     SDE_CHECK(hipGetLastError());
@@ -2622,6 +2630,7 @@ struct sde_batch {
         float2 *grids = nullptr, *gwork = nullptr;  // per field: dirty, PSF planes (NU x NV each)
         float *images = nullptr;                    // per field: dirty, PSF images (NX x NY each)
         float *ccf = nullptr;
+        float *rc = nullptr;  // reciprocal correction factors (launch_corr_recip)
         int *peaks = nullptr;
         double2 *twnv = nullptr, *twfnu = nullptr;
     };
@@ -2633,7 +2642,8 @@ struct sde_batch {
     float *d_cf = nullptr;
     TileParams tp;  // T, chunk and waves of every field; ntiles over all fields
     CoordParams cp0;
-    int max_nu = 0, max_nv = 0;
+    int rblocks = 0;  // reduce blocks over all fields
+    int max_nu = 0;   // the widest field's NU
     FieldDesc *d_fields = nullptr;
     double *d_sums = nullptr;  // 4 per field
     int64_t n = 0, cap = 0;
@@ -2688,12 +2698,12 @@ int batch_issue(sde_batch *b) {
         set_plan_error("sde_batch_uvmap: unsupported gridding run (T=%d)", b->tp.T);
         return -1;
     }
-    launch_grid_reduce_fields(s, b->d_partial, b->sb.item_start, b->d_fields, b->nf, b->tp.T, b->max_nu, b->max_nv);
+    launch_grid_reduce_fields(s, b->d_partial, b->sb.item_start, b->d_fields, b->nf, b->rblocks, b->tp.T, b->max_nu);
     for (const auto &c : b->cls) {
         const size_t ng = (size_t)c.nu * c.nv, npix = (size_t)c.nx * c.ny;
         launch_fft_cols(s, c.grids, c.gwork, c.nu, c.nv, 2 * c.count, ng, -1, 1.0f, c.twnv);
         launch_fft_rows_c2r_fields(s, c.gwork + ng, c.gwork, 2 * ng, c.images + npix, c.images, 2 * npix, c.peaks,
-                                   c.count, c.nu, c.nv, c.fnu, c.nx, c.ny, c.twfnu, c.ccf, c.ccflen, 1,
+                                   c.count, c.nu, c.nv, c.fnu, c.nx, c.ny, c.twfnu, c.rc,
                                    1.0f / ((float)c.fnu * (float)c.nv));
         launch_scale_by_peak_fields(s, c.images + npix, c.images, 2 * npix, npix, c.peaks, c.count);
     }
@@ -2799,8 +2809,9 @@ int sde_batch_create(const sde_imaging_cfg *cfgs, int nfields, sde_batch **out) 
         d.ntu = ftp.ntu;
         d.ntv = ftp.ntv;
         tiles += ftp.ntiles;
+        d.rblock_begin = b->rblocks;
+        b->rblocks += (int)(((int64_t)g.nu * g.nv + 255) / 256);
         b->max_nu = std::max(b->max_nu, g.nu);
-        b->max_nv = std::max(b->max_nv, g.nv);
         int k = 0;
         while (k < (int)b->cls.size() && !(b->cls[k].nx == c.nx && b->cls[k].ny == c.ny)) k++;
         if (k == (int)b->cls.size()) {
@@ -2840,7 +2851,8 @@ int sde_batch_create(const sde_imaging_cfg *cfgs, int nfields, sde_batch **out) 
         const size_t ng = (size_t)c.nu * c.nv, npix = (size_t)c.nx * c.ny;
         rc = batch_alloc(b, &c.grids, 2 * ng * c.count) || batch_alloc(b, &c.gwork, 2 * ng * c.count) ||
              batch_alloc(b, &c.images, 2 * npix * c.count) || batch_alloc(b, &c.peaks, (size_t)c.count) ||
-             batch_alloc(b, &c.ccf, (size_t)c.ccflen) || get_twiddles(b->tw, c.nv, b->stream, &c.twnv) ||
+             batch_alloc(b, &c.ccf, (size_t)c.ccflen) || batch_alloc(b, &c.rc, (size_t)c.nx + c.ny) ||
+             get_twiddles(b->tw, c.nv, b->stream, &c.twnv) ||
              get_twiddles(b->tw, c.fnu, b->stream, &c.twfnu);
         if (rc) break;
         std::vector<float> ccf(c.ccflen);
@@ -2848,6 +2860,8 @@ int sde_batch_create(const sde_imaging_cfg *cfgs, int nfields, sde_batch **out) 
         if (hipMemcpyAsync(c.ccf, ccf.data(), sizeof(float) * c.ccflen, hipMemcpyHostToDevice, b->stream) !=
             hipSuccess)
             rc = -1;
+        launch_corr_recip(b->stream, c.ccf, c.ccflen, c.nx, c.ny, c.rc);
+        if (hipGetLastError() != hipSuccess) rc = -1;
         if (hipStreamSynchronize(b->stream) != hipSuccess) rc = -1;
     }
     if (rc) {

@@ -232,12 +232,12 @@ __device__ __forceinline__ int icf(float s, int i1, float cen) {
 // with their peaks in peak_a[k].
 __global__ void k_fft_rows_c2r(const float2 *__restrict__ grid_a, int nu, int nv, int fnu, int nx,
                                int ny, int ibegy, const double2 *__restrict__ tw,
-                               const float *__restrict__ ccf, int correct, CorrParams cpar,
-                               float scale, float *__restrict__ image_a, int *__restrict__ peak_a,
+                               const float *__restrict__ rc, float scale, float *__restrict__ image_a,
+                               int *__restrict__ peak_a,
                                const int *__restrict__ div_key, const float2 *__restrict__ grid_b,
                                float *__restrict__ image_b, size_t gstride, size_t istride) {
     extern __shared__ double2 zs[];
-    __shared__ float red[1024];
+    __shared__ float red[16];
     const bool second = (blockIdx.y & 1) != 0;
     const int field = blockIdx.y >> 1;
     const float2 *grid = (second ? grid_b : grid_a) + gstride * field;
@@ -266,34 +266,35 @@ __global__ void k_fft_rows_c2r(const float2 *__restrict__ grid_a, int nu, int nv
     float inv = 1.0f;
     if (div_key) inv = 1.0f / key_float(*div_key);
     float mx = -INFINITY;
-    const int icy = correct ? icf(cpar.sy, iy + 1, cpar.ceny) : 0;
-    const float cy = correct ? ccf[icy] : 1.0f;
+    const float ry = rc ? rc[nx + iy] : 1.0f;
     for (int ix = threadIdx.x; ix < nx; ix += blockDim.x) {
         int p = ioff + ix + 1 - icen;
         p += p < 0 ? fnu : 0;
         const double2 z = zs[lpad(p >> 1)];
         float val = (float)(((p & 1) ? z.y : z.x) * (double)scale);
-        if (correct) val = val / (ccf[icf(cpar.sx, ix + 1, cpar.cenx)] * cy);
+        if (rc) val = val * (rc[ix] * ry);
         if (div_key) val = inv * val;
         image[(size_t)iy * nx + ix] = val;
         mx = fmaxf(mx, val);
     }
     if (peak_key) {
-        red[threadIdx.x] = mx;
+        // the row's maximum: within each wave, then over the waves
+        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+        const int nw = (blockDim.x + 63) / 64;
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
         __syncthreads();
-        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
-            __syncthreads();
+        if (threadIdx.x == 0) {
+            float m = red[0];
+            for (int w = 1; w < nw; w++) m = fmaxf(m, red[w]);
+            atomicMax(peak_key, float_key(m));
         }
-        if (threadIdx.x == 0) atomicMax(peak_key, float_key(red[0]));
     }
 }
 
 // Image rows -> grid rows (DIR > 0); rows outside the image are zeroed.
 __global__ void k_fft_rows_r2c(const float *__restrict__ image, int nx, int ny, int fnu, int nu,
                                int ibegy, const double2 *__restrict__ tw,
-                               const float *__restrict__ ccf, int correct, CorrParams cpar,
-                               float2 *__restrict__ grid) {
+                               const float *__restrict__ rc, float2 *__restrict__ grid) {
     extern __shared__ double2 zs[];
     const int rv = blockIdx.x;
     const int iy = rv - (ibegy - 1);
@@ -308,11 +309,10 @@ __global__ void k_fft_rows_r2c(const float *__restrict__ image, int nx, int ny, 
     for (int k = threadIdx.x; k < lds_len(h); k += blockDim.x) zs[k] = make_double2(0.0, 0.0);
     __syncthreads();
     const int icen = (fnu + 1) / 2, ioff = (fnu - nx) / 2;
-    const int icy = correct ? icf(cpar.sy, iy + 1, cpar.ceny) : 0;
-    const float cy = correct ? ccf[icy] : 1.0f;
+    const float ry = rc ? rc[nx + iy] : 1.0f;
     for (int ix = threadIdx.x; ix < nx; ix += blockDim.x) {
         float val = image[(size_t)iy * nx + ix];
-        if (correct) val = val / (ccf[icf(cpar.sx, ix + 1, cpar.cenx)] * cy);
+        if (rc) val = val * (rc[ix] * ry);
         int p = ioff + ix + 1 - icen;
         p += p < 0 ? fnu : 0;
         zr[2 * lpad(p >> 1) + (p & 1)] = (double)val;
@@ -371,6 +371,15 @@ __global__ void k_pixcor(const float *__restrict__ in, float *__restrict__ out, 
     } else {
         out[k] = correct ? in[k] / c : c * in[k];
     }
+}
+
+// The reciprocal correction factors of the fused row passes: rc[ix] =
+// 1/CCF(x index of pixel ix) for ix < NX, then rc[NX + iy] = 1/CCF(y index
+// of row iy), with PIXCOR2D's index arithmetic (pix/pixcor2d.f:44-68).
+__global__ void k_corr_recip(const float *__restrict__ cf, int nx, int ny, CorrParams cpar, float *__restrict__ rc) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nx) rc[k] = 1.0f / cf[icf(cpar.sx, k + 1, cpar.cenx)];
+    else if (k < nx + ny) rc[k] = 1.0f / cf[icf(cpar.sy, k - nx + 1, cpar.ceny)];
 }
 
 __global__ void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n) {
@@ -457,54 +466,53 @@ void launch_fft_cols(hipStream_t s, const float2 *src, float2 *dst, int nu, int 
                        dir, scale, tw_nv);
 }
 
+void launch_corr_recip(hipStream_t s, const float *ccf, int ccflen, int nx, int ny, float *rc) {
+    const CorrParams cp = make_corr(nx, ny, ccflen, (float)((nx + 1) / 2), (float)((ny + 1) / 2));
+    hipLaunchKernelGGL(k_corr_recip, dim3((nx + ny + 255) / 256), dim3(256), 0, s, ccf, nx, ny, cp, rc);
+}
+
 void launch_fft_rows_c2r(hipStream_t s, const float2 *grid, int nu, int nv, int fnu, int nx,
-                         int ny, const double2 *tw_fnu, const float *ccf, int ccflen,
-                         int correct, float scale, float *image, int *peak_key,
+                         int ny, const double2 *tw_fnu, const float *rc, float scale, float *image, int *peak_key,
                          const int *div_key) {
     const int h = fnu / 2;
     const int ibegy = (nv + 1) / 2 - (ny + 1) / 2 + 1;
-    CorrParams cp = make_corr(nx, ny, ccflen, (float)((nx + 1) / 2), (float)((ny + 1) / 2));
     (void)fft_lds_limit();
     hipLaunchKernelGGL(k_fft_rows_c2r, dim3(ny), dim3(row_threads(h)), sizeof(double2) * lds_len(h), s, grid,
-                       nu, nv, fnu, nx, ny, ibegy, tw_fnu, ccf, correct, cp, scale, image, peak_key,
+                       nu, nv, fnu, nx, ny, ibegy, tw_fnu, rc, scale, image, peak_key,
                        div_key, (const float2 *)nullptr, (float *)nullptr, (size_t)0, (size_t)0);
 }
 
 void launch_fft_rows_c2r_pair(hipStream_t s, const float2 *grid_a, float *image_a, int *peak_a,
                               const float2 *grid_b, float *image_b, int nu, int nv, int fnu, int nx, int ny,
-                              const double2 *tw_fnu, const float *ccf, int ccflen, int correct, float scale) {
+                              const double2 *tw_fnu, const float *rc, float scale) {
     const int h = fnu / 2;
     const int ibegy = (nv + 1) / 2 - (ny + 1) / 2 + 1;
-    CorrParams cp = make_corr(nx, ny, ccflen, (float)((nx + 1) / 2), (float)((ny + 1) / 2));
     (void)fft_lds_limit();
     hipLaunchKernelGGL(k_fft_rows_c2r, dim3(ny, 2), dim3(row_threads(h)), sizeof(double2) * lds_len(h), s, grid_a, nu, nv,
-                       fnu, nx, ny, ibegy, tw_fnu, ccf, correct, cp, scale, image_a, peak_a, (const int *)nullptr,
+                       fnu, nx, ny, ibegy, tw_fnu, rc, scale, image_a, peak_a, (const int *)nullptr,
                        grid_b, image_b, (size_t)0, (size_t)0);
 }
 
 void launch_fft_rows_c2r_fields(hipStream_t s, const float2 *grid_psf, const float2 *grid_dirty, size_t gstride,
                                 float *image_psf, float *image_dirty, size_t istride, int *peaks, int nf, int nu,
-                                int nv, int fnu, int nx, int ny, const double2 *tw_fnu, const float *ccf, int ccflen,
-                                int correct, float scale) {
+                                int nv, int fnu, int nx, int ny, const double2 *tw_fnu, const float *rc,
+                                float scale) {
     if (nf <= 0) return;
     const int h = fnu / 2;
     const int ibegy = (nv + 1) / 2 - (ny + 1) / 2 + 1;
-    CorrParams cp = make_corr(nx, ny, ccflen, (float)((nx + 1) / 2), (float)((ny + 1) / 2));
     (void)fft_lds_limit();
     hipLaunchKernelGGL(k_fft_rows_c2r, dim3(ny, 2 * nf), dim3(row_threads(h)), sizeof(double2) * lds_len(h), s,
-                       grid_psf, nu, nv, fnu, nx, ny, ibegy, tw_fnu, ccf, correct, cp, scale, image_psf, peaks,
+                       grid_psf, nu, nv, fnu, nx, ny, ibegy, tw_fnu, rc, scale, image_psf, peaks,
                        (const int *)nullptr, grid_dirty, image_dirty, gstride, istride);
 }
 
 void launch_fft_rows_r2c(hipStream_t s, const float *image, int nx, int ny, int fnu, int nv,
-                         const double2 *tw_fnu, const float *ccf, int ccflen, int correct,
-                         float2 *grid) {
+                         const double2 *tw_fnu, const float *rc, float2 *grid) {
     const int h = fnu / 2, nu = h + 1;
     const int ibegy = (nv + 1) / 2 - (ny + 1) / 2 + 1;
-    CorrParams cp = make_corr(nx, ny, ccflen, (float)((nx + 1) / 2), (float)((ny + 1) / 2));
     (void)fft_lds_limit();
     hipLaunchKernelGGL(k_fft_rows_r2c, dim3(nv), dim3(row_threads(h)), sizeof(double2) * lds_len(h), s, image,
-                       nx, ny, fnu, nu, ibegy, tw_fnu, ccf, correct, cp, grid);
+                       nx, ny, fnu, nu, ibegy, tw_fnu, rc, grid);
 }
 
 void launch_scale_by_peak(hipStream_t s, float *image, size_t n, const int *peak_key, float *image_b) {

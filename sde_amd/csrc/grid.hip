@@ -967,13 +967,13 @@ __global__ __launch_bounds__(256) void k_grid_reduce(
     const float *__restrict__ partial, const int *__restrict__ item_start, TileParams tp,
     int chmask, CoordParams cp, const double *__restrict__ sums, float2 *__restrict__ dirty,
     float2 *__restrict__ psf) {
-    // the rows of one band of tiles (which read the same partial tiles) on one
-    // XCD, the bands dealt round the XCDs (the work per band varies)
-    const int lin = xcd_swizzle_chunks(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y,
-                                       gridDim.x * tp.T);
-    const int iu = (lin % gridDim.x) * blockDim.x + threadIdx.x;  // 0-based output column
-    const int iv = lin / gridDim.x;                                // 0-based row
-    if (iu >= cp.nu) return;
+    // 256 consecutive cells (row-major) per workgroup; the workgroups of one
+    // band of T rows (which read the same partial tiles) on one XCD, the bands
+    // dealt round the XCDs (the work per band varies)
+    const int lin = xcd_swizzle_chunks(blockIdx.x, gridDim.x, (tp.T * cp.nu + 255) / 256);
+    const int64_t c = (int64_t)lin * 256 + threadIdx.x;
+    if (c >= (int64_t)cp.nu * cp.nv) return;
+    const int iv = (int)(c / cp.nu), iu = (int)(c - (int64_t)iv * cp.nu);  // 0-based row and column
     reduce_cell(partial, item_start, tp.T, tp.ntu, tp.ntv, chmask, cp, sums, iu, iv, dirty, psf);
 }
 
@@ -1077,16 +1077,34 @@ __global__ __launch_bounds__(1024) void k_field_totals(const double *__restrict_
     }
 }
 
-// The grids of every field in one launch: blockIdx.z = field, blocks past
-// the field's own NU x NV return.
+// The field of reduce block b: the last f with fields[f].rblock_begin <= b.
+__device__ __forceinline__ int field_of_rblock(const FieldDesc *__restrict__ fields, int nfields, int b) {
+    int lo = 0, hi = nfields - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (fields[mid].rblock_begin <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// The grids of every field in one launch: 256 consecutive cells (row-major)
+// of one field per workgroup, the workgroups dealt to fields in order and
+// in bands of `band` consecutive workgroups to the XCDs (a band of tile rows
+// reads the same partial tiles).
 __global__ __launch_bounds__(256) void k_grid_reduce_fields(const float *__restrict__ partial,
                                                             const int *__restrict__ item_start,
-                                                            const FieldDesc *__restrict__ fields, int T,
-                                                            int chmask) {
-    const FieldDesc &fd = fields[blockIdx.z];
-    const int iu = blockIdx.x * blockDim.x + threadIdx.x, iv = blockIdx.y;
-    if (iu >= fd.cp.nu || iv >= fd.cp.nv) return;
+                                                            const FieldDesc *__restrict__ fields, int nfields,
+                                                            int T, int chmask, int band) {
+    __shared__ int s_f;
+    const int b = xcd_swizzle_chunks(blockIdx.x, gridDim.x, band);
+    if (threadIdx.x == 0) s_f = field_of_rblock(fields, nfields, b);
+    __syncthreads();
+    const FieldDesc &fd = fields[s_f];
     const CoordParams cp = fd.cp;
+    const int64_t c = (int64_t)(b - fd.rblock_begin) * 256 + threadIdx.x;
+    if (c >= (int64_t)cp.nu * cp.nv) return;
+    const int iv = (int)(c / cp.nu), iu = (int)(c - (int64_t)iv * cp.nu);
     reduce_cell(partial, item_start + fd.tile_begin, T, fd.ntu, fd.ntv, chmask, cp, fd.sums, iu, iv, fd.dirty,
                 fd.psf);
 }
@@ -1632,7 +1650,7 @@ int launch_grid_tiles(hipStream_t s, const SortBufs &b, const TileParams &tp, in
 
 void launch_grid_reduce(hipStream_t s, const float *partial, const int *item_start, const TileParams &tp,
                         int chmask, const CoordParams &cp, const double *sums, float2 *dirty, float2 *psf) {
-    dim3 grid((cp.nu + 255) / 256, cp.nv);
+    dim3 grid((unsigned)(((int64_t)cp.nu * cp.nv + 255) / 256));
     hipLaunchKernelGGL(k_grid_reduce, grid, dim3(256), 0, s, partial, item_start, tp, chmask, cp, sums, dirty, psf);
 }
 
@@ -1664,10 +1682,10 @@ int launch_grid_tiles_fields(hipStream_t s, const SortBufs &b, const TileParams 
 }
 
 void launch_grid_reduce_fields(hipStream_t s, const float *partial, const int *item_start, const FieldDesc *fields,
-                               int nfields, int T, int max_nu, int max_nv) {
-    if (nfields <= 0) return;
-    dim3 grid((max_nu + 255) / 256, max_nv, nfields);
-    hipLaunchKernelGGL(k_grid_reduce_fields, grid, dim3(256), 0, s, partial, item_start, fields, T, 3);
+                               int nfields, int nblocks, int T, int max_nu) {
+    if (nfields <= 0 || nblocks <= 0) return;
+    hipLaunchKernelGGL(k_grid_reduce_fields, dim3(nblocks), dim3(256), 0, s, partial, item_start, fields, nfields, T,
+                       3, (T * max_nu + 255) / 256);
 }
 
 namespace {

@@ -306,6 +306,7 @@ struct FieldDesc {
     CoordParams cp;
     int64_t vis_begin = 0, nvis = 0;
     int block_begin = 0;         // first prep block (kPrepFieldThreads points a block)
+    int rblock_begin = 0;        // first reduce block (256 grid cells a block)
     int tile_begin = 0;          // first tile in the batch's numbering
     int ntu = 0, ntv = 0;        // the field's tiles
     float2 *dirty = nullptr, *psf = nullptr;  // gridded planes, NU x NV each
@@ -322,9 +323,10 @@ void launch_prep_fields(hipStream_t s, const float2 *vis, const float *wt, const
 int launch_grid_tiles_fields(hipStream_t s, const SortBufs &b, const TileParams &tp, const float *cf,
                              const CoordParams &cp_any, double *item_sums, float *partial, const FieldDesc *fields,
                              int nfields);
-// Every field's dirty and PSF grids (max_nu x max_nv covers every field).
+// Every field's dirty and PSF grids (nblocks: the reduce blocks of all
+// fields; max_nu: the widest field's NU, for the XCD bands).
 void launch_grid_reduce_fields(hipStream_t s, const float *partial, const int *item_start, const FieldDesc *fields,
-                               int nfields, int T, int max_nu, int max_nv);
+                               int nfields, int nblocks, int T, int max_nu);
 // Zero n bytes (a multiple of 4) with a kernel (graph-safe).
 void launch_zero(hipStream_t s, void *p, size_t n);
 
@@ -354,16 +356,19 @@ int fft_supported(int n);
 // dst may equal src (in place)
 void launch_fft_cols(hipStream_t s, const float2 *src, float2 *dst, int nu, int nv, int ngrids,
                      size_t grid_stride, int dir, float scale, const double2 *tw_nv);
+// The correction of the fused row passes: rc[ix] = 1/CCF of pixel column
+// ix (ix < NX), then rc[NX + iy] = 1/CCF of row iy (NX + NY floats), by
+// PIXCOR2D's indexing of the GRDCCF table ccf (ccflen entries).  A pass
+// given rc multiplies each pixel by rc[ix]*rc[NX+iy]; null: no correction.
+void launch_corr_recip(hipStream_t s, const float *ccf, int ccflen, int nx, int ny, float *rc);
 // DIR < 0 rows: grid rows -> image (NX,NY) with centring, crop, correction,
 // scale, optional peak tracking and optional division by a device peak.
 void launch_fft_rows_c2r(hipStream_t s, const float2 *grid, int nu, int nv, int fnu, int nx,
-                         int ny, const double2 *tw_fnu, const float *ccf, int ccflen,
-                         int correct, float scale, float *image, int *peak_key,
+                         int ny, const double2 *tw_fnu, const float *rc, float scale, float *image, int *peak_key,
                          const int *div_key);
 // DIR > 0 rows: image -> grid rows (zero rows outside the image), optional correction.
 void launch_fft_rows_r2c(hipStream_t s, const float *image, int nx, int ny, int fnu, int nv,
-                         const double2 *tw_fnu, const float *ccf, int ccflen, int correct,
-                         float2 *grid);
+                         const double2 *tw_fnu, const float *rc, float2 *grid);
 // image (and image_b when given) times 1/peak
 void launch_scale_by_peak(hipStream_t s, float *image, size_t n, const int *peak_key, float *image_b = nullptr);
 // Row passes of nf fields of one size in one launch: field k's PSF plane
@@ -371,8 +376,8 @@ void launch_scale_by_peak(hipStream_t s, float *image, size_t n, const int *peak
 // and its dirty plane likewise (no peak).
 void launch_fft_rows_c2r_fields(hipStream_t s, const float2 *grid_psf, const float2 *grid_dirty, size_t gstride,
                                 float *image_psf, float *image_dirty, size_t istride, int *peaks, int nf, int nu,
-                                int nv, int fnu, int nx, int ny, const double2 *tw_fnu, const float *ccf, int ccflen,
-                                int correct, float scale);
+                                int nv, int fnu, int nx, int ny, const double2 *tw_fnu, const float *rc,
+                                float scale);
 // Both images of nf fields times 1/peak of their field.
 void launch_scale_by_peak_fields(hipStream_t s, float *image_psf, float *image_dirty, size_t istride, size_t n,
                                  const int *peaks, int nf);
@@ -380,7 +385,7 @@ void launch_scale_by_peak_fields(hipStream_t s, float *image_psf, float *image_d
 // peak_a, grid_b -> image_b.
 void launch_fft_rows_c2r_pair(hipStream_t s, const float2 *grid_a, float *image_a, int *peak_a,
                               const float2 *grid_b, float *image_b, int nu, int nv, int fnu, int nx, int ny,
-                              const double2 *tw_fnu, const float *ccf, int ccflen, int correct, float scale);
+                              const double2 *tw_fnu, const float *rc, float scale);
 void launch_pixcor(hipStream_t s, const float *in, float *out, int nx, int ny, int cplx,
                    const float *cf, int cflen, int correct, float cenx, float ceny);
 
