@@ -1691,14 +1691,14 @@ int plan_transform(sde_plan *p) {
     plan_mark_end(p);
     const float2 *cd = p->d_gwork, *cpsf = p->d_gwork + ng;
     const float scale = 1.0f / ((float)p->fnu * (float)p->nv);
-    // both row passes in one launch (the PSF's tracks its peak), then both
-    // images divided by that peak (task/uvmap.f:434-440,480-482)
+    // the PSF's row pass (tracking its peak), the dirty image's divided by that
+    // peak as it is written, then the PSF divided (task/uvmap.f:434-440,480-482)
     plan_mark_begin(p, "fft_rows");
     launch_fft_rows_c2r_pair(s, cpsf, p->d_psf, p->d_peak, cd, p->d_dirty, p->nu, p->nv, p->fnu, p->cfg.nx,
                              p->cfg.ny, twfnu, p->d_rc, scale);
     plan_mark_end(p);
     plan_mark_begin(p, "peak_scale");
-    launch_scale_by_peak(s, p->d_psf, (size_t)p->cfg.nx * p->cfg.ny, p->d_peak, p->d_dirty);
+    launch_scale_by_peak(s, p->d_psf, (size_t)p->cfg.nx * p->cfg.ny, p->d_peak);
     plan_mark_end(p);
     SDE_CHECK(hipGetLastError());
     return 0;
@@ -2705,7 +2705,7 @@ int batch_issue(sde_batch *b) {
         launch_fft_rows_c2r_fields(s, c.gwork + ng, c.gwork, 2 * ng, c.images + npix, c.images, 2 * npix, c.peaks,
                                    c.count, c.nu, c.nv, c.fnu, c.nx, c.ny, c.twfnu, c.rc,
                                    1.0f / ((float)c.fnu * (float)c.nv));
-        launch_scale_by_peak_fields(s, c.images + npix, c.images, 2 * npix, npix, c.peaks, c.count);
+        launch_scale_by_peak_fields(s, c.images + npix, 2 * npix, npix, c.peaks, c.count);
     }
     SDE_CHECK(hipGetLastError());
     return 0;

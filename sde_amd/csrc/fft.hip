@@ -226,20 +226,21 @@ __device__ __forceinline__ int icf(float s, int i1, float cen) {
     return abs((int)roundf(s * ((float)i1 - cen)));  // 0-based table index
 }
 
-// Grid rows -> image rows (DIR < 0).
-// blockIdx.y = 2k + 1 transforms grid_b into image_b (no peak) in the same
-// launch; k > 0: the k-th of several fields, gstride and istride apart,
-// with their peaks in peak_a[k].
+// Grid rows -> image rows (DIR < 0) of one plane per field (planes = 1,
+// blockIdx.y = field k) or two (planes = 2: blockIdx.y = 2k + 1 transforms
+// grid_b into image_b, no peak); field k's planes lie k*gstride and its
+// images k*istride on, its peak in peak_a[k], the peak it is divided by in
+// div_key[k].
 __global__ void k_fft_rows_c2r(const float2 *__restrict__ grid_a, int nu, int nv, int fnu, int nx,
                                int ny, int ibegy, const double2 *__restrict__ tw,
                                const float *__restrict__ rc, float scale, float *__restrict__ image_a,
                                int *__restrict__ peak_a,
                                const int *__restrict__ div_key, const float2 *__restrict__ grid_b,
-                               float *__restrict__ image_b, size_t gstride, size_t istride) {
+                               float *__restrict__ image_b, size_t gstride, size_t istride, int planes) {
     extern __shared__ double2 zs[];
     __shared__ float red[16];
-    const bool second = (blockIdx.y & 1) != 0;
-    const int field = blockIdx.y >> 1;
+    const bool second = planes == 2 && (blockIdx.y & 1) != 0;
+    const int field = planes == 2 ? blockIdx.y >> 1 : blockIdx.y;
     const float2 *grid = (second ? grid_b : grid_a) + gstride * field;
     float *image = (second ? image_b : image_a) + istride * field;
     int *peak_key = second || !peak_a ? nullptr : peak_a + field;
@@ -264,7 +265,7 @@ __global__ void k_fft_rows_c2r(const float2 *__restrict__ grid_a, int nu, int nv
     lds_fft(zs, h, ilog2(h), threadIdx.x, blockDim.x, tw, 2, +1);
     const int icen = (fnu + 1) / 2, ioff = (fnu - nx) / 2;
     float inv = 1.0f;
-    if (div_key) inv = 1.0f / key_float(*div_key);
+    if (div_key) inv = 1.0f / key_float(div_key[field]);
     float mx = -INFINITY;
     const float ry = rc ? rc[nx + iy] : 1.0f;
     for (int ix = threadIdx.x; ix < nx; ix += blockDim.x) {
@@ -273,7 +274,7 @@ __global__ void k_fft_rows_c2r(const float2 *__restrict__ grid_a, int nu, int nv
         const double2 z = zs[lpad(p >> 1)];
         float val = (float)(((p & 1) ? z.y : z.x) * (double)scale);
         if (rc) val = val * (rc[ix] * ry);
-        if (div_key) val = inv * val;
+        if (div_key) val = inv * val + 0.0f;  // as k_scale_by_peak divides
         image[(size_t)iy * nx + ix] = val;
         mx = fmaxf(mx, val);
     }
@@ -338,12 +339,12 @@ __global__ void k_fft_rows_r2c(const float *__restrict__ image, int nx, int ny, 
     }
 }
 
-// blockIdx.y = 2k + 1 scales image_b; k > 0: the k-th of several fields,
-// istride apart, by the peak key[k].
+// Images times 1/peak: planes = 2, blockIdx.y = 2k + 1 scales image_b;
+// planes = 1, blockIdx.y = k; field k's images istride on, its peak key[k].
 __global__ void k_scale_by_peak(float *__restrict__ image_a, float *__restrict__ image_b, size_t n,
-                                const int *__restrict__ key, size_t istride) {
-    const int field = blockIdx.y >> 1;
-    float *image = ((blockIdx.y & 1) ? image_b : image_a) + istride * field;
+                                const int *__restrict__ key, size_t istride, int planes) {
+    const int field = planes == 2 ? blockIdx.y >> 1 : blockIdx.y;
+    float *image = ((planes == 2 && (blockIdx.y & 1)) ? image_b : image_a) + istride * field;
     const float inv = 1.0f / key_float(key[field]);
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         image[i] = inv * image[i] + 0.0f;
@@ -479,7 +480,7 @@ void launch_fft_rows_c2r(hipStream_t s, const float2 *grid, int nu, int nv, int 
     (void)fft_lds_limit();
     hipLaunchKernelGGL(k_fft_rows_c2r, dim3(ny), dim3(row_threads(h)), sizeof(double2) * lds_len(h), s, grid,
                        nu, nv, fnu, nx, ny, ibegy, tw_fnu, rc, scale, image, peak_key,
-                       div_key, (const float2 *)nullptr, (float *)nullptr, (size_t)0, (size_t)0);
+                       div_key, (const float2 *)nullptr, (float *)nullptr, (size_t)0, (size_t)0, 1);
 }
 
 void launch_fft_rows_c2r_pair(hipStream_t s, const float2 *grid_a, float *image_a, int *peak_a,
@@ -488,9 +489,13 @@ void launch_fft_rows_c2r_pair(hipStream_t s, const float2 *grid_a, float *image_
     const int h = fnu / 2;
     const int ibegy = (nv + 1) / 2 - (ny + 1) / 2 + 1;
     (void)fft_lds_limit();
-    hipLaunchKernelGGL(k_fft_rows_c2r, dim3(ny, 2), dim3(row_threads(h)), sizeof(double2) * lds_len(h), s, grid_a, nu, nv,
-                       fnu, nx, ny, ibegy, tw_fnu, rc, scale, image_a, peak_a, (const int *)nullptr,
-                       grid_b, image_b, (size_t)0, (size_t)0);
+    // grid_a with its peak, then grid_b divided by that peak
+    hipLaunchKernelGGL(k_fft_rows_c2r, dim3(ny), dim3(row_threads(h)), sizeof(double2) * lds_len(h), s, grid_a, nu,
+                       nv, fnu, nx, ny, ibegy, tw_fnu, rc, scale, image_a, peak_a, (const int *)nullptr,
+                       (const float2 *)nullptr, (float *)nullptr, (size_t)0, (size_t)0, 1);
+    hipLaunchKernelGGL(k_fft_rows_c2r, dim3(ny), dim3(row_threads(h)), sizeof(double2) * lds_len(h), s, grid_b, nu,
+                       nv, fnu, nx, ny, ibegy, tw_fnu, rc, scale, image_b, (int *)nullptr, peak_a,
+                       (const float2 *)nullptr, (float *)nullptr, (size_t)0, (size_t)0, 1);
 }
 
 void launch_fft_rows_c2r_fields(hipStream_t s, const float2 *grid_psf, const float2 *grid_dirty, size_t gstride,
@@ -501,9 +506,13 @@ void launch_fft_rows_c2r_fields(hipStream_t s, const float2 *grid_psf, const flo
     const int h = fnu / 2;
     const int ibegy = (nv + 1) / 2 - (ny + 1) / 2 + 1;
     (void)fft_lds_limit();
-    hipLaunchKernelGGL(k_fft_rows_c2r, dim3(ny, 2 * nf), dim3(row_threads(h)), sizeof(double2) * lds_len(h), s,
+    // every PSF with its peak, then every dirty plane divided by its field's peak
+    hipLaunchKernelGGL(k_fft_rows_c2r, dim3(ny, nf), dim3(row_threads(h)), sizeof(double2) * lds_len(h), s,
                        grid_psf, nu, nv, fnu, nx, ny, ibegy, tw_fnu, rc, scale, image_psf, peaks,
-                       (const int *)nullptr, grid_dirty, image_dirty, gstride, istride);
+                       (const int *)nullptr, (const float2 *)nullptr, (float *)nullptr, gstride, istride, 1);
+    hipLaunchKernelGGL(k_fft_rows_c2r, dim3(ny, nf), dim3(row_threads(h)), sizeof(double2) * lds_len(h), s,
+                       grid_dirty, nu, nv, fnu, nx, ny, ibegy, tw_fnu, rc, scale, image_dirty, (int *)nullptr, peaks,
+                       (const float2 *)nullptr, (float *)nullptr, gstride, istride, 1);
 }
 
 void launch_fft_rows_r2c(hipStream_t s, const float *image, int nx, int ny, int fnu, int nv,
@@ -519,16 +528,15 @@ void launch_scale_by_peak(hipStream_t s, float *image, size_t n, const int *peak
     size_t nb = (n + 255) / 256;
     if (nb > 65536) nb = 65536;
     hipLaunchKernelGGL(k_scale_by_peak, dim3((unsigned)nb, image_b ? 2 : 1), dim3(256), 0, s, image, image_b, n,
-                       peak_key, (size_t)0);
+                       peak_key, (size_t)0, image_b ? 2 : 1);
 }
 
-void launch_scale_by_peak_fields(hipStream_t s, float *image_psf, float *image_dirty, size_t istride, size_t n,
-                                 const int *peaks, int nf) {
+void launch_scale_by_peak_fields(hipStream_t s, float *image, size_t istride, size_t n, const int *peaks, int nf) {
     if (nf <= 0) return;
     size_t nb = (n + 255) / 256;
     if (nb > 4096) nb = 4096;
-    hipLaunchKernelGGL(k_scale_by_peak, dim3((unsigned)nb, 2 * nf), dim3(256), 0, s, image_psf, image_dirty, n, peaks,
-                       istride);
+    hipLaunchKernelGGL(k_scale_by_peak, dim3((unsigned)nb, nf), dim3(256), 0, s, image, (float *)nullptr, n, peaks,
+                       istride, 1);
 }
 
 void launch_copy(hipStream_t s, const float4 *src, float4 *dst, size_t n) {

@@ -371,18 +371,19 @@ void launch_fft_rows_r2c(hipStream_t s, const float *image, int nx, int ny, int 
                          const double2 *tw_fnu, const float *rc, float2 *grid);
 // image (and image_b when given) times 1/peak
 void launch_scale_by_peak(hipStream_t s, float *image, size_t n, const int *peak_key, float *image_b = nullptr);
-// Row passes of nf fields of one size in one launch: field k's PSF plane
-// grid_psf + k*gstride -> image_psf + k*istride with its peak in peaks[k],
-// and its dirty plane likewise (no peak).
+// Row passes of nf fields of one size, one launch per plane: field k's PSF
+// plane grid_psf + k*gstride -> image_psf + k*istride with its peak in
+// peaks[k], then its dirty plane likewise, divided by that peak (the PSF
+// images are divided by launch_scale_by_peak_fields after).
 void launch_fft_rows_c2r_fields(hipStream_t s, const float2 *grid_psf, const float2 *grid_dirty, size_t gstride,
                                 float *image_psf, float *image_dirty, size_t istride, int *peaks, int nf, int nu,
                                 int nv, int fnu, int nx, int ny, const double2 *tw_fnu, const float *rc,
                                 float scale);
-// Both images of nf fields times 1/peak of their field.
-void launch_scale_by_peak_fields(hipStream_t s, float *image_psf, float *image_dirty, size_t istride, size_t n,
-                                 const int *peaks, int nf);
-// Two row passes in one launch: grid_a -> image_a with its peak tracked in
-// peak_a, grid_b -> image_b.
+// One image of each of nf fields (istride apart) times 1/peak of its field.
+void launch_scale_by_peak_fields(hipStream_t s, float *image, size_t istride, size_t n, const int *peaks, int nf);
+// Two row passes: grid_a -> image_a with its peak tracked in peak_a, then
+// grid_b -> image_b divided by that peak (image_a is divided after, by
+// launch_scale_by_peak).
 void launch_fft_rows_c2r_pair(hipStream_t s, const float2 *grid_a, float *image_a, int *peak_a,
                               const float2 *grid_b, float *image_b, int nu, int nv, int fnu, int nx, int ny,
                               const double2 *tw_fnu, const float *rc, float scale);
