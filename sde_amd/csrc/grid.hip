@@ -144,7 +144,7 @@ template <int DM, bool DEG>
 __global__ __launch_bounds__(kThreads) void k_prep(
     const float2 *__restrict__ vis, const float *wt, const float *__restrict__ u,
     const float *__restrict__ v, const float *__restrict__ w, int64_t n, CoordParams cp, TileParams tp,
-    uint32_t sentinel, uint32_t *__restrict__ keys, uint64_t *__restrict__ vals, StageHot *__restrict__ hot,
+    uint32_t sentinel, uint32_t *__restrict__ keys, StageHot *__restrict__ hot,
     StageCold *__restrict__ cold, int cold_all, CoordParams wcp, AzParams az,
     double *__restrict__ gwt, double *__restrict__ dblk, unsigned long long *__restrict__ ndrop,
     float2 *__restrict__ vis_out, float *__restrict__ wt_out, int *__restrict__ list, int *__restrict__ nlist,
@@ -232,8 +232,7 @@ __global__ __launch_bounds__(kThreads) void k_prep(
             }
             hot[i] = sr;
             if (cold_all || (sr.gpk & kStageOwnCell)) cold[i] = sc;
-            keys[i] = key;
-            vals[i] = (uint64_t)(uint32_t)i | ((uint64_t)__float_as_uint(wti) << 32);
+            keys[i] = key;  // the sort forms the value from the index and wt[i] (= wti)
         }
         if (DEG) seg_append(to_list, (int)i, seg, &s_nlist);
     }
@@ -1005,8 +1004,7 @@ __device__ __forceinline__ int field_of_block(const FieldDesc *__restrict__ fiel
 __global__ __launch_bounds__(kPrepFieldThreads) void k_prep_fields(
     const float2 *__restrict__ vis, const float *__restrict__ wt, const float *__restrict__ u,
     const float *__restrict__ v, const float *__restrict__ w, const FieldDesc *__restrict__ fields, int nfields,
-    int T, uint32_t sentinel, uint32_t *__restrict__ keys, uint64_t *__restrict__ vals,
-    StageHot *__restrict__ hot) {
+    int T, uint32_t sentinel, uint32_t *__restrict__ keys, StageHot *__restrict__ hot) {
     __shared__ int s_f;
     if (threadIdx.x == 0) s_f = field_of_block(fields, nfields, blockIdx.x);
     __syncthreads();
@@ -1038,8 +1036,7 @@ __global__ __launch_bounds__(kPrepFieldThreads) void k_prep_fields(
             key += (uint32_t)fd.tile_begin * (uint32_t)(T * T);
     }
     hot[i] = sr;
-    keys[i] = key;
-    vals[i] = (uint64_t)(uint32_t)i | ((uint64_t)__float_as_uint(wti) << 32);
+    keys[i] = key;  // the sort forms the value from the index and wt[i]
 }
 
 // SUMWT, LSUMWT and the normalisation of each field (one workgroup per
@@ -1504,6 +1501,7 @@ int64_t list_capacity(int64_t n) { return (int64_t)list_seg_cap(n > 0 ? n : 1) *
 void launch_prep(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
                  const float *w, int64_t n, const CoordParams &cp, const TileParams &tp, SortBufs &b,
                  const DensityOut *dens, const DegridOut *deg, const SelParams *sel) {
+    b.vals_wt = wt;  // k_prep writes no values: the sort forms them
     if (n <= 0) return;
     const SelParams sp = sel ? *sel : SelParams();
     const int nb = prep_blocks(n);
@@ -1517,7 +1515,7 @@ void launch_prep(hipStream_t s, const float2 *vis, const float *wt, const float 
     int *list = deg ? deg->list : nullptr, *nlist = deg ? deg->nlist : nullptr;
 #define SDE_PREP(DM, DG)                                                                                         \
     hipLaunchKernelGGL((k_prep<DM, DG>), dim3(nb), dim3(kThreads), 0, s, vis, wt, u, v, w, n, cp, tp, sentinel, \
-                       b.keys, b.vals, b.hot, b.cold, b.cold_all, wcp, az, gwt, dblk, ndrop, vo, wo, list, nlist,    \
+                       b.keys, b.hot, b.cold, b.cold_all, wcp, az, gwt, dblk, ndrop, vo, wo, list, nlist,            \
                        list_seg_cap(n), sp)
     const int dm = dens ? (dens->gwt ? 2 : 1) : 0;
     if (deg) {
@@ -1534,7 +1532,8 @@ void launch_prep(hipStream_t s, const float2 *vis, const float *wt, const float 
 
 int launch_sort(hipStream_t s, SortBufs &b, int64_t n, const TileParams &tp, bool bounds) {
     if (n > 0) {
-        if (launch_radix_sort(s, b.temp, b.temp_bytes, b.keys, b.skeys, b.vals, b.svals, n, sort_nbits(tp)) !=
+        if (launch_radix_sort(s, b.temp, b.temp_bytes, b.keys, b.skeys, b.vals, b.svals, n, sort_nbits(tp),
+                              b.vals_wt) !=
             hipSuccess)
             return -1;
         if (bounds)
@@ -1657,9 +1656,10 @@ void launch_grid_reduce(hipStream_t s, const float *partial, const int *item_sta
 void launch_prep_fields(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
                         const float *w, const FieldDesc *fields, int nfields, int nblocks, int T, uint32_t sentinel,
                         SortBufs &b) {
+    b.vals_wt = wt;  // k_prep_fields writes no values: the sort forms them
     if (nblocks <= 0 || nfields <= 0) return;
     hipLaunchKernelGGL(k_prep_fields, dim3(nblocks), dim3(kPrepFieldThreads), 0, s, vis, wt, u, v, w, fields, nfields,
-                       T, sentinel, b.keys, b.vals, b.hot);
+                       T, sentinel, b.keys, b.hot);
 }
 
 int launch_grid_tiles_fields(hipStream_t s, const SortBufs &b, const TileParams &tp, const float *cf,
@@ -1706,6 +1706,7 @@ void launch_zero(hipStream_t s, void *p, size_t n) {
 void launch_prep_slowz(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
                        const float *w, int64_t n, const CoordParams &cp, const TileParams &tp, int psf,
                        const PlaneParams &pp, SortBufs &b, double *blk, double *sumwt) {
+    b.vals_wt = nullptr;  // k_prep_slowz writes the values
     const int nb = prep_blocks(n > 0 ? n : 1);
     if (n > 0)
         hipLaunchKernelGGL(k_prep_slowz, dim3(nb), dim3(kThreads), 0, s, vis, wt, u, v, w, n, cp, tp,
@@ -1729,6 +1730,7 @@ void launch_box_fold_slowz(hipStream_t s, const double *acc, const CoordParams &
 void launch_prep_box(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
                      const float *w, int64_t n, const CoordParams &cp, const TileParams &tp, SortBufs &b,
                      double *blk) {
+    b.vals_wt = nullptr;  // k_prep_box writes the values
     if (n <= 0) return;
     hipLaunchKernelGGL(k_prep_box, dim3(prep_blocks(n)), dim3(kThreads), 0, s, vis, wt, u, v, w, n, cp, tp,
                        (uint32_t)sort_nkeys(tp), b.keys, b.vals, b.hot, blk);

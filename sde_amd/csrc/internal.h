@@ -122,6 +122,9 @@ constexpr uint32_t kKeyFlag = 0x80000000u;
 struct SortBufs {
     uint32_t *keys = nullptr;    // per visibility, caller's order
     uint64_t *vals = nullptr;    // per visibility, caller's order
+    // set by a prep pass that writes no values: the sort's first pass forms
+    // each value as index | WT bits << 32 from these weights instead
+    const float *vals_wt = nullptr;
     uint32_t *skeys = nullptr;   // sorted keys
     uint64_t *svals = nullptr;   // sorted values
     StageHot *hot = nullptr;     // per visibility, caller's order
@@ -141,10 +144,12 @@ int64_t sort_nkeys(const TileParams &tp);
 int item_scan_ints(const TileParams &tp);
 int sort_nbits(const TileParams &tp);
 size_t sort_temp_bytes(int64_t n, int nbits);  // sort.hip
-// The error word the last sort on temp left (nonzero: a look-back wait timed out).
+// Stable key/value radix sort over key bits [0, nbits) (n < 2^31); with
+// vals_wt, vals_in is not read: the value of key i is i | vals_wt[i]'s bits
+// << 32.
 hipError_t launch_radix_sort(hipStream_t s, void *temp, size_t temp_bytes, const uint32_t *keys_in,
                              uint32_t *keys_out, const uint64_t *vals_in, uint64_t *vals_out, int64_t n,
-                             int nbits);  // sort.hip
+                             int nbits, const float *vals_wt = nullptr);  // sort.hip
 // The dispatcher deals workgroups to the 8 XCDs round-robin (workgroup b to
 // XCD b mod 8), and each XCD has its own L2.  xcd_swizzle maps the
 // workgroup's linear id b (of n) to the logical one that keeps consecutive

@@ -230,8 +230,10 @@ __global__ __launch_bounds__(1024) void k_sort_chunk_scan(unsigned *__restrict__
 // order of the keys, to keys_out/vals_out.  Wave w of the tile holds keys
 // [w*kSeg, (w+1)*kSeg) of it, round r of them at r*64 + lane: the tile's
 // input order is (wave, round, lane).
+template <bool WTV>
 __global__ __launch_bounds__(kThr) void k_sort_scatter(const uint32_t *__restrict__ keys_in,
                                                       const uint64_t *__restrict__ vals_in,
+                                                      const float *__restrict__ vals_wt,
                                                       uint32_t *__restrict__ keys_out,
                                                       uint64_t *__restrict__ vals_out, int64_t n, int shift, int bits,
                                                       const unsigned *__restrict__ tcount,
@@ -262,7 +264,8 @@ __global__ __launch_bounds__(kThr) void k_sort_scatter(const uint32_t *__restric
     for (int r = 0; r < kRounds; r++) {
         const int64_t i = seg + r * 64 + lane;
         key[r] = i < n ? keys_in[i] : 0u;
-        val[r] = i < n ? vals_in[i] : 0ull;
+        if (WTV) val[r] = i < n ? ((uint64_t)(uint32_t)i | ((uint64_t)__float_as_uint(vals_wt[i]) << 32)) : 0ull;
+        else val[r] = i < n ? vals_in[i] : 0ull;
     }
     // rank within the wave: the lanes of equal digit in this round (the OR of
     // their lane bits), after the wave's keys of that digit in earlier
@@ -354,7 +357,7 @@ size_t sort_temp_bytes(int64_t n, int nbits) {
 
 hipError_t launch_radix_sort(hipStream_t s, void *temp, size_t temp_bytes, const uint32_t *keys_in,
                              uint32_t *keys_out, const uint64_t *vals_in, uint64_t *vals_out, int64_t n,
-                             int nbits) {
+                             int nbits, const float *vals_wt) {
     if (n < 0 || n >= (int64_t)1 << 31 || nbits < 1 || nbits > 32) return hipErrorInvalidValue;
     if (n == 0) return hipSuccess;
     if (temp_bytes < temp_bytes_for(n)) return hipErrorInvalidValue;
@@ -374,8 +377,12 @@ hipError_t launch_radix_sort(hipStream_t s, void *temp, size_t temp_bytes, const
             hipLaunchKernelGGL(k_sort_count<false>, dim3(tiles), dim3(kCountThr), 0, s, ki, n, shift, bits, t.tcount);
         hipLaunchKernelGGL(k_sort_chunk, dim3(chunks), dim3(1024), 0, s, t.tcount, tiles, t.csum);
         hipLaunchKernelGGL(k_sort_chunk_scan, dim3(1), dim3(1024), 0, s, t.csum, chunks);
-        hipLaunchKernelGGL(k_sort_scatter, dim3(tiles), dim3(kThr), 0, s, ki, vi, ko, vo, n, shift, bits, t.tcount,
-                           t.csum);
+        if (p == 0 && vals_wt)  // the first pass forms the values
+            hipLaunchKernelGGL(k_sort_scatter<true>, dim3(tiles), dim3(kThr), 0, s, ki, vi, vals_wt, ko, vo, n, shift,
+                               bits, t.tcount, t.csum);
+        else
+            hipLaunchKernelGGL(k_sort_scatter<false>, dim3(tiles), dim3(kThr), 0, s, ki, vi, (const float *)nullptr, ko,
+                               vo, n, shift, bits, t.tcount, t.csum);
         ki = ko;
         vi = vo;
     }
