@@ -11,8 +11,10 @@ field of the same configuration, and against the oracle:
   * the 64-pointing mosaic (512^2 and 1024^2 pointings, 2e4-5e5
     visibilities each) equal to its plans field by field;
   * one field of a batch against the oracle's uvmap (tests/tolerance.py);
-  * the errors: weighting, BOX, mismatched tiles, no visibilities, a field
-    with nothing gridded ("No weight!").
+  * SZE fields, a non-square field, a field with no visibilities between
+    others, and visibilities handed over as device arrays;
+  * the errors: weighting, BOX, no visibilities, a field with nothing
+    gridded ("No weight!").
 """
 import numpy as np
 import pytest
@@ -178,5 +180,51 @@ def test_batch_errors():
             fb.images(1)
         d, _ = fb.images(0)
         assert np.isfinite(d).all()
+    finally:
+        fb.close()
+
+
+def test_batch_edge_fields_sze_nonsquare_device_input():
+    """SZE fields, a non-square 256 x 128 field, a field with no
+    visibilities between two others, and visibilities handed over as device
+    arrays: every field with points still equals its plan, and the empty
+    field reports GRDH2D's "No weight!"."""
+    import ctypes as C
+    base = _jobs(seed=21, npoint=3, sizes=(256,), nshift=1)
+    base[1] = dict(base[1], ny=128)
+    empty = dict(base[2], vis=base[2]["vis"][:0], wt=base[2]["wt"][:0], u=base[2]["u"][:0], v=base[2]["v"][:0],
+                 w=base[2]["w"][:0])
+    jobs = [base[0], empty, base[1], base[2]]
+    tune = dict(TUNE, cftype="SZE")
+    ref = []
+    for j in (base[0], base[1], base[2]):
+        p = imaging.Plan(j["nx"], j.get("ny"), j["cellsize"], shift=j.get("shift"), cftype="SZE", **TUNE)
+        try:
+            p.set_vis(j["vis"], j["wt"], j["u"], j["v"], j["w"])
+            p.uvmap()
+            d, q = p.images()
+            ref.append((d, q, p.info()))
+        finally:
+            p.close()
+    fb = batch.FieldBatch([dict(j) for j in jobs], **tune)
+    try:
+        # the same visibilities again, handed over from device arrays
+        vis = np.concatenate([np.asarray(j["vis"], np.complex64) for j in jobs])
+        arrs = [np.concatenate([np.asarray(j[k], np.float32) for j in jobs]) for k in ("wt", "u", "v", "w")]
+        dv = [sde.DeviceArray.from_numpy(a) for a in [vis] + arrs]
+        counts = (C.c_int64 * len(jobs))(*[len(j["u"]) for j in jobs])
+        assert sde.lib().sde_batch_set_vis(fb._b, counts, *[C.c_void_p(d.ptr) for d in dv], 1) == 0
+        fb.uvmap()
+        fb.sync()
+        for k, r in zip((0, 2, 3), ref):
+            d, q = fb.images(k)
+            assert d.shape == r[0].shape
+            np.testing.assert_array_equal(d, r[0], err_msg="field %d dirty" % k)
+            np.testing.assert_array_equal(q, r[1], err_msg="field %d psf" % k)
+            bi = fb.info(k)
+            assert bi["sumwt"] == r[2]["sumwt"] and bi["maxpsf"] == r[2]["maxpsf"]
+        with pytest.raises(sde.SdeError, match="No weight"):
+            fb.images(1)
+        assert fb.info(1)["nvis"] == 0
     finally:
         fb.close()
