@@ -140,6 +140,11 @@ __device__ __forceinline__ bool stage_cell(const CoordParams &cp, const TilePara
 // With sel.on, VISSELPI's selection comes first (vis/visselpi.f:58-70): a
 // deselected point's WT is negated in the weights (sel.wt, the array wt
 // reads) and the point is flagged for everything after.
+// A block takes chunks of `chunk` points (chunk c = blockIdx.x, + gridDim.x,
+// ...).  With tcount, a chunk is one tile of the cell sort and the block
+// also writes the counts of its keys' low digit (key & 0xff: the sort's
+// first pass) to tcount[c * kSortBins + digit], summed over one LDS
+// histogram per wave.
 template <int DM, bool DEG>
 __global__ __launch_bounds__(kThreads) void k_prep(
     const float2 *__restrict__ vis, const float *wt, const float *__restrict__ u,
@@ -148,93 +153,111 @@ __global__ __launch_bounds__(kThreads) void k_prep(
     StageCold *__restrict__ cold, int cold_all, CoordParams wcp, AzParams az,
     double *__restrict__ gwt, double *__restrict__ dblk, unsigned long long *__restrict__ ndrop,
     float2 *__restrict__ vis_out, float *__restrict__ wt_out, int *__restrict__ list, int *__restrict__ nlist,
-    int seg_cap, SelParams sel) {
+    int seg_cap, SelParams sel, int chunk, unsigned *__restrict__ tcount) {
+    static_assert(kThreads == kSortBins, "one thread per digit sums the tile counts");
+    constexpr int kHist = kThreads / 64;
     double dsum = 0.0, ndr = 0.0;
     unsigned nkept = 0;
     __shared__ int s_nlist;
-    if (DEG) {
-        if (threadIdx.x == 0) s_nlist = 0;
-        __syncthreads();
-    }
+    __shared__ unsigned s_hist[kHist][kSortBins + 1];
+    const bool count = tcount != nullptr;
+    if (count)
+        for (int k = 0; k < kHist; k++) s_hist[k][threadIdx.x] = 0u;
+    if (DEG && threadIdx.x == 0) s_nlist = 0;
+    if (DEG || count) __syncthreads();
     int *seg = DEG ? list + (size_t)blockIdx.x * seg_cap : nullptr;
     const int poffu = (cp.osampu + 1) / 2, poffv = (cp.osampv + 1) / 2;
-    const int64_t stride = (int64_t)gridDim.x * kThreads;
+    unsigned *hist = s_hist[threadIdx.x >> 6];
     // block-uniform loop bounds: every lane of a wave takes part in the list append
-    for (int64_t base = (int64_t)blockIdx.x * kThreads; base < n; base += stride) {
-        const int64_t i = base + threadIdx.x;
-        bool to_list = false;
-        if (i < n) {
-            float wti = wt[i];
-            const float uf = u[i], vf = v[i], wf = w[i];
-            if (sel.on && wti > 0.0f) {
-                if (vissel_out(sel, sel.time[i], uf, vf)) {
-                    wti = -fabsf(wti);
-                    sel.wt[i] = wti;
-                } else {
-                    nkept++;
+    for (int64_t c0 = (int64_t)blockIdx.x * chunk; c0 < n; c0 += (int64_t)gridDim.x * chunk) {
+        for (int sub = 0; sub < chunk && c0 + sub < n; sub += kThreads) {
+            const int64_t i = c0 + sub + threadIdx.x;
+            bool to_list = false;
+            if (i < n) {
+                float wti = wt[i];
+                const float uf = u[i], vf = v[i], wf = w[i];
+                if (sel.on && wti > 0.0f) {
+                    if (vissel_out(sel, sel.time[i], uf, vf)) {
+                        wti = -fabsf(wti);
+                        sel.wt[i] = wti;
+                    } else {
+                        nkept++;
+                    }
                 }
-            }
-            const float2 vv = vis[i];
-            StageHot sr;
-            sr.gpk = 0u;
-            sr.fr = 0.0f;
-            sr.fi = 0.0f;
-            sr.dpk = 0u;
-            StageCold sc;
-            sc.wcell = kNoCell;
-            sc.q2 = 0.0f;
-            sc.rot = make_float2(1.0f, 0.0f);
-            uint32_t key = sentinel;
-            if (wti > 0.0f) {
-                sc.rot = shift_phase(cp, uf, vf, wf);
-                GridCell c;
-                const bool in = stage_cell(cp, tp, poffu, poffv, uf, vf, wf, vv, sc.rot, c, key, sr);
-                if (DM > 0) {
-                    const RobustCell rc = robust_cell(wcp, uf, vf, wf, az);
-                    sc.wcell = rc.cell;
-                    sc.q2 = rc.q2;
-                    if (rc.half) sr.gpk |= kStageHalf;
-                    if (rc.cell == kNoCell) {
-                        ndr += 1.0;
-                        if (in) {
-                            sr.gpk |= kStageOwnCell;
-                            key |= kKeyFlag;
-                        }
-                    } else if (DM == 2) {
-                        if (!(in && rc.uc == c.nuc && rc.vc == c.nvc)) {
-                            atomicAdd(&gwt[rc.cell], (double)wti);
-                            dsum += (double)wti;
+                const float2 vv = vis[i];
+                StageHot sr;
+                sr.gpk = 0u;
+                sr.fr = 0.0f;
+                sr.fi = 0.0f;
+                sr.dpk = 0u;
+                StageCold sc;
+                sc.wcell = kNoCell;
+                sc.q2 = 0.0f;
+                sc.rot = make_float2(1.0f, 0.0f);
+                uint32_t key = sentinel;
+                if (wti > 0.0f) {
+                    sc.rot = shift_phase(cp, uf, vf, wf);
+                    GridCell c;
+                    const bool in = stage_cell(cp, tp, poffu, poffv, uf, vf, wf, vv, sc.rot, c, key, sr);
+                    if (DM > 0) {
+                        const RobustCell rc = robust_cell(wcp, uf, vf, wf, az);
+                        sc.wcell = rc.cell;
+                        sc.q2 = rc.q2;
+                        if (rc.half) sr.gpk |= kStageHalf;
+                        if (rc.cell == kNoCell) {
+                            ndr += 1.0;
                             if (in) {
-                                key |= kKeyFlag;
                                 sr.gpk |= kStageOwnCell;
+                                key |= kKeyFlag;
+                            }
+                        } else if (DM == 2) {
+                            if (!(in && rc.uc == c.nuc && rc.vc == c.nvc)) {
+                                atomicAdd(&gwt[rc.cell], (double)wti);
+                                dsum += (double)wti;
+                                if (in) {
+                                    key |= kKeyFlag;
+                                    sr.gpk |= kStageOwnCell;
+                                }
                             }
                         }
                     }
-                }
-                if (DEG) {
-                    DegridRec d;
-                    const int path = degrid_half(cp, tp, uf, vf, wf, in ? (int)(key & ~kKeyFlag) : -1, (uint32_t)i,
-                                                 sc.rot, d);
-                    if (path == kDegOff) {
-                        vis_out[i] = make_float2(0.0f, 0.0f);
-                        wt_out[i] = -wti;
-                    } else if (path == kDegList) {
-                        to_list = true;
-                    } else {
-                        wt_out[i] = wti;  // negated by the tile kernel if the point fails
-                        sr.dpk = d.pk;
-                        sr.gpk |= kStageDegTile | ((d.idx & 0x80000000u) ? kStageConj : 0u);
+                    if (DEG) {
+                        DegridRec d;
+                        const int path = degrid_half(cp, tp, uf, vf, wf, in ? (int)(key & ~kKeyFlag) : -1, (uint32_t)i,
+                                                     sc.rot, d);
+                        if (path == kDegOff) {
+                            vis_out[i] = make_float2(0.0f, 0.0f);
+                            wt_out[i] = -wti;
+                        } else if (path == kDegList) {
+                            to_list = true;
+                        } else {
+                            wt_out[i] = wti;  // negated by the tile kernel if the point fails
+                            sr.dpk = d.pk;
+                            sr.gpk |= kStageDegTile | ((d.idx & 0x80000000u) ? kStageConj : 0u);
+                        }
                     }
+                } else if (DEG) {
+                    wt_out[i] = wti;
+                    vis_out[i] = vv;
                 }
-            } else if (DEG) {
-                wt_out[i] = wti;
-                vis_out[i] = vv;
+                hot[i] = sr;
+                if (cold_all || (sr.gpk & kStageOwnCell)) cold[i] = sc;
+                keys[i] = key;  // the sort forms the value from the index and wt[i] (= wti)
+                if (count) atomicAdd(&hist[key & (kSortBins - 1)], 1u);
             }
-            hot[i] = sr;
-            if (cold_all || (sr.gpk & kStageOwnCell)) cold[i] = sc;
-            keys[i] = key;  // the sort forms the value from the index and wt[i] (= wti)
+            if (DEG) seg_append(to_list, (int)i, seg, &s_nlist);
         }
-        if (DEG) seg_append(to_list, (int)i, seg, &s_nlist);
+        if (count) {
+            __syncthreads();
+            unsigned t = 0;
+#pragma unroll
+            for (int k = 0; k < kHist; k++) {
+                t += s_hist[k][threadIdx.x];
+                s_hist[k][threadIdx.x] = 0u;  // this thread's own column: no wait needed before
+            }
+            tcount[(c0 / chunk) * kSortBins + threadIdx.x] = t;
+            __syncthreads();
+        }
     }
     if (DEG) {
         __syncthreads();
@@ -1491,10 +1514,15 @@ static int capped_blocks(int64_t n, int cap) {
 int prep_blocks(int64_t n) { return capped_blocks(n, kPrepBlocks); }
 int run_blocks(int64_t n) { return capped_blocks(n, kRunBlocks); }
 int gather_blocks(int64_t n) { return capped_blocks(n, kGatherBlocks); }
-// slots per prep block: the block's share of a block-uniform grid-stride loop
+// k_prep counts the sort's first digit when its blocks take whole sort
+// tiles and prep_blocks(n) is kPrepBlocks whether a step is one tile or
+// kThreads points
+constexpr int64_t kPrepTileMin = (int64_t)kPrepBlocks * kSortTileKeys;
+int prep_chunk(int64_t n) { return n >= kPrepTileMin ? kSortTileKeys : kThreads; }
+// slots per prep block: the points of its share of the chunks
 int list_seg_cap(int64_t n) {
-    const int64_t per_pass = (int64_t)prep_blocks(n) * kThreads;
-    return (int)(((n + per_pass - 1) / per_pass) * kThreads);
+    const int64_t chunk = prep_chunk(n), chunks = (n + chunk - 1) / chunk, nb = prep_blocks(n);
+    return (int)(((chunks + nb - 1) / nb) * chunk);
 }
 int64_t list_capacity(int64_t n) { return (int64_t)list_seg_cap(n > 0 ? n : 1) * prep_blocks(n > 0 ? n : 1); }
 
@@ -1502,7 +1530,13 @@ void launch_prep(hipStream_t s, const float2 *vis, const float *wt, const float 
                  const float *w, int64_t n, const CoordParams &cp, const TileParams &tp, SortBufs &b,
                  const DensityOut *dens, const DegridOut *deg, const SelParams *sel) {
     b.vals_wt = wt;  // k_prep writes no values: the sort forms them
+    const int chunk = prep_chunk(n);
+    // with whole sort tiles a block step, k_prep counts the sort's first digit
+    const size_t need = sort_temp_bytes(n, sort_nbits(tp));
+    b.counted0 = n > 0 && chunk == kSortTileKeys && sort_nbits(tp) >= 8 && b.temp != nullptr && need > 0 &&
+                 b.temp_bytes >= need;
     if (n <= 0) return;
+    unsigned *tcount = b.counted0 ? sort_tile_counts(b.temp, n) : nullptr;
     const SelParams sp = sel ? *sel : SelParams();
     const int nb = prep_blocks(n);
     const uint32_t sentinel = (uint32_t)sort_nkeys(tp);
@@ -1516,7 +1550,7 @@ void launch_prep(hipStream_t s, const float2 *vis, const float *wt, const float 
 #define SDE_PREP(DM, DG)                                                                                         \
     hipLaunchKernelGGL((k_prep<DM, DG>), dim3(nb), dim3(kThreads), 0, s, vis, wt, u, v, w, n, cp, tp, sentinel, \
                        b.keys, b.hot, b.cold, b.cold_all, wcp, az, gwt, dblk, ndrop, vo, wo, list, nlist,            \
-                       list_seg_cap(n), sp)
+                       list_seg_cap(n), sp, chunk, tcount)
     const int dm = dens ? (dens->gwt ? 2 : 1) : 0;
     if (deg) {
         if (dm == 2) SDE_PREP(2, true);
@@ -1533,7 +1567,7 @@ void launch_prep(hipStream_t s, const float2 *vis, const float *wt, const float 
 int launch_sort(hipStream_t s, SortBufs &b, int64_t n, const TileParams &tp, bool bounds) {
     if (n > 0) {
         if (launch_radix_sort(s, b.temp, b.temp_bytes, b.keys, b.skeys, b.vals, b.svals, n, sort_nbits(tp),
-                              b.vals_wt) !=
+                              b.vals_wt, b.counted0) !=
             hipSuccess)
             return -1;
         if (bounds)
@@ -1657,6 +1691,7 @@ void launch_prep_fields(hipStream_t s, const float2 *vis, const float *wt, const
                         const float *w, const FieldDesc *fields, int nfields, int nblocks, int T, uint32_t sentinel,
                         SortBufs &b) {
     b.vals_wt = wt;  // k_prep_fields writes no values: the sort forms them
+    b.counted0 = false;
     if (nblocks <= 0 || nfields <= 0) return;
     hipLaunchKernelGGL(k_prep_fields, dim3(nblocks), dim3(kPrepFieldThreads), 0, s, vis, wt, u, v, w, fields, nfields,
                        T, sentinel, b.keys, b.hot);
@@ -1707,6 +1742,7 @@ void launch_prep_slowz(hipStream_t s, const float2 *vis, const float *wt, const 
                        const float *w, int64_t n, const CoordParams &cp, const TileParams &tp, int psf,
                        const PlaneParams &pp, SortBufs &b, double *blk, double *sumwt) {
     b.vals_wt = nullptr;  // k_prep_slowz writes the values
+    b.counted0 = false;
     const int nb = prep_blocks(n > 0 ? n : 1);
     if (n > 0)
         hipLaunchKernelGGL(k_prep_slowz, dim3(nb), dim3(kThreads), 0, s, vis, wt, u, v, w, n, cp, tp,
@@ -1731,6 +1767,7 @@ void launch_prep_box(hipStream_t s, const float2 *vis, const float *wt, const fl
                      const float *w, int64_t n, const CoordParams &cp, const TileParams &tp, SortBufs &b,
                      double *blk) {
     b.vals_wt = nullptr;  // k_prep_box writes the values
+    b.counted0 = false;
     if (n <= 0) return;
     hipLaunchKernelGGL(k_prep_box, dim3(prep_blocks(n)), dim3(kThreads), 0, s, vis, wt, u, v, w, n, cp, tp,
                        (uint32_t)sort_nkeys(tp), b.keys, b.vals, b.hot, blk);

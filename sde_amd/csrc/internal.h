@@ -125,6 +125,9 @@ struct SortBufs {
     // set by a prep pass that writes no values: the sort's first pass forms
     // each value as index | WT bits << 32 from these weights instead
     const float *vals_wt = nullptr;
+    // set by a prep pass that also counted the low digit of every sort tile
+    // into the sort's scratch (sort_tile_counts): the first pass counts nothing
+    bool counted0 = false;
     uint32_t *skeys = nullptr;   // sorted keys
     uint64_t *svals = nullptr;   // sorted values
     StageHot *hot = nullptr;     // per visibility, caller's order
@@ -144,12 +147,20 @@ int64_t sort_nkeys(const TileParams &tp);
 int item_scan_ints(const TileParams &tp);
 int sort_nbits(const TileParams &tp);
 size_t sort_temp_bytes(int64_t n, int nbits);  // sort.hip
+// The sort works on tiles of kSortTileKeys keys ([t * kSortTileKeys, ...));
+// its first pass (digit = key & 0xff) reads the counts of tile t's digits
+// from sort_tile_counts(temp, n)[t * kSortBins ...].  A prep pass may write
+// them itself and hand the sort counted0 (needs nbits >= 8).
+constexpr int kSortTileKeys = 4096;
+constexpr int kSortBins = 256;
+unsigned *sort_tile_counts(void *temp, int64_t n);  // sort.hip
 // Stable key/value radix sort over key bits [0, nbits) (n < 2^31); with
 // vals_wt, vals_in is not read: the value of key i is i | vals_wt[i]'s bits
-// << 32.
+// << 32; with counted0 the first pass's tile counts are already in the
+// scratch (sort_tile_counts).
 hipError_t launch_radix_sort(hipStream_t s, void *temp, size_t temp_bytes, const uint32_t *keys_in,
                              uint32_t *keys_out, const uint64_t *vals_in, uint64_t *vals_out, int64_t n,
-                             int nbits, const float *vals_wt = nullptr);  // sort.hip
+                             int nbits, const float *vals_wt = nullptr, bool counted0 = false);  // sort.hip
 // The dispatcher deals workgroups to the 8 XCDs round-robin (workgroup b to
 // XCD b mod 8), and each XCD has its own L2.  xcd_swizzle maps the
 // workgroup's linear id b (of n) to the logical one that keeps consecutive
@@ -188,6 +199,10 @@ struct DegridOut {
 };
 int list_seg_cap(int64_t n);
 int64_t list_capacity(int64_t n);  // list_seg_cap(n) * prep_blocks(n)
+// Points per step of a k_prep block: whole sort tiles from kPrepTileMin
+// points on (the prep pass then counts the sort's first digit), where
+// prep_blocks(n) is kPrepBlocks either way; else kThreads.
+int prep_chunk(int64_t n);
 
 // Density grid built during the sort (robust/uniform weighting with the
 // density cell equal to the gridding cell, WTFOV = 1): the prep pass adds

@@ -32,6 +32,9 @@
 //                      output and fill whole cache lines in that XCD's L2.
 // Every pass's counts come from kernels, so the sort needs no memset and
 // can be captured in a graph and replayed.  The counts are 32-bit: n < 2^31.
+// The pass that writes the keys may count the first pass's digits itself,
+// tile by tile into the same scratch (sort_tile_counts); the sort then
+// starts at k_sort_chunk.
 #include <algorithm>
 
 #include "internal.h"
@@ -51,6 +54,7 @@ constexpr int kGroupTiles = 32;         // tiles per thread group of k_sort_chun
 constexpr int kChunkTiles = 4 * kGroupTiles;
 constexpr int kXcd = 8;                 // XCDs of the MI355X
 static_assert(kWaves * kBins <= kTile, "the match masks fit in the value staging area");
+static_assert(kTile == kSortTileKeys && kBins == kSortBins, "the tiles and digits a prep pass counts");
 
 struct SortTemp {
     uint32_t *keys;   // n: intermediate keys
@@ -350,6 +354,8 @@ __global__ __launch_bounds__(kThr) void k_sort_scatter(const uint32_t *__restric
 
 }  // namespace
 
+unsigned *sort_tile_counts(void *temp, int64_t n) { return carve(temp, n > 0 ? n : 1).tcount; }
+
 size_t sort_temp_bytes(int64_t n, int nbits) {
     if (n < 0 || n >= (int64_t)1 << 31 || nbits < 1 || nbits > 32) return 0;
     return temp_bytes_for(n > 0 ? n : 1);
@@ -357,8 +363,9 @@ size_t sort_temp_bytes(int64_t n, int nbits) {
 
 hipError_t launch_radix_sort(hipStream_t s, void *temp, size_t temp_bytes, const uint32_t *keys_in,
                              uint32_t *keys_out, const uint64_t *vals_in, uint64_t *vals_out, int64_t n,
-                             int nbits, const float *vals_wt) {
+                             int nbits, const float *vals_wt, bool counted0) {
     if (n < 0 || n >= (int64_t)1 << 31 || nbits < 1 || nbits > 32) return hipErrorInvalidValue;
+    if (counted0 && nbits < kBits) return hipErrorInvalidValue;
     if (n == 0) return hipSuccess;
     if (temp_bytes < temp_bytes_for(n)) return hipErrorInvalidValue;
     SortTemp t = carve(temp, n);
@@ -371,7 +378,9 @@ hipError_t launch_radix_sort(hipStream_t s, void *temp, size_t temp_bytes, const
         uint32_t *ko = to_out ? keys_out : t.keys;
         uint64_t *vo = to_out ? vals_out : t.vals;
         const int shift = p * kBits, bits = std::min(kBits, nbits - shift);
-        if (reinterpret_cast<uintptr_t>(ki) % 16 == 0)
+        if (p == 0 && counted0) {
+            // the prep pass counted this pass's digits
+        } else if (reinterpret_cast<uintptr_t>(ki) % 16 == 0)
             hipLaunchKernelGGL(k_sort_count<true>, dim3(tiles), dim3(kCountThr), 0, s, ki, n, shift, bits, t.tcount);
         else
             hipLaunchKernelGGL(k_sort_count<false>, dim3(tiles), dim3(kCountThr), 0, s, ki, n, shift, bits, t.tcount);

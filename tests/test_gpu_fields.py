@@ -13,6 +13,8 @@ field of the same configuration, and against the oracle:
   * one field of a batch against the oracle's uvmap (tests/tolerance.py);
   * SZE fields, a non-square field, a field with no visibilities between
     others, and visibilities handed over as device arrays;
+  * a field of 9e6 visibilities, where a plan's prep pass counts the cell
+    sort's first digit itself and the batch's does not;
   * the errors: weighting, BOX, no visibilities, a field with nothing
     gridded ("No weight!").
 """
@@ -226,5 +228,30 @@ def test_batch_edge_fields_sze_nonsquare_device_input():
         with pytest.raises(sde.SdeError, match="No weight"):
             fb.images(1)
         assert fb.info(1)["nvis"] == 0
+    finally:
+        fb.close()
+
+
+def test_plan_prep_counted_sort_equals_batch():
+    """From 2048 sort tiles of 4096 visibilities on, a plan's prep pass also
+    counts the first digit of the cell sort; the batch's prep pass never
+    does.  Their images agree bit for bit only if those counts are the
+    sort's own (a wrong count misplaces points and changes the sums)."""
+    rng = np.random.default_rng(91)
+    n = 2048 * 4096 + 12345  # a partial last tile
+    u = rng.normal(0.0, 800.0, n).astype(np.float32)
+    v = rng.normal(0.0, 800.0, n).astype(np.float32)
+    w = rng.normal(0.0, 50.0, n).astype(np.float32)
+    vis = (rng.normal(size=n) + 1j * rng.normal(size=n)).astype(np.complex64)
+    wt = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    wt[rng.integers(0, n, 1000)] = -1.0  # flagged points take the sentinel key
+    job = dict(vis=vis, wt=wt, u=u, v=v, w=w, nx=1024, cellsize=synth.cell_for(u, v), nvis=n)
+    ref = _plan_results([job])
+    fb = batch.FieldBatch([job], **TUNE)
+    try:
+        fb.uvmap()
+        fb.sync()
+        _check_equal(fb, ref, "9e6 visibilities")
+        assert fb.info(0)["ngridded"] > 0.99 * n - 1000
     finally:
         fb.close()
