@@ -795,9 +795,10 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
                     fr = a * cr - b2 * ci;
                     fi = a * ci + b2 * cr;
                 }
-                // the PSF grids VIS = 1
-                mine.re = PSF_ONLY ? wti : (float)((double)wti * (double)fr);
-                mine.im = PSF_ONLY ? 0.0f : (float)((double)wti * (double)fi);
+                // the weight is carried by the tap table's v values: a tap
+                // adds WT*CF to the PSF and WT*CF*FVIS to the dirty grid
+                mine.re = fr;
+                mine.im = fi;
             }
             if (ZP == 1 && valid) zp.sorted[p] = st0;
             if (DEG && valid) {
@@ -806,20 +807,22 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
             }
 
             // grid the batch.  Each lane first looks up its record's
-            // kernel values (SUPPU*2+1 along u, SUPPV*2+1 along v) into its
-            // row of the wave's LDS record table, and the wave marks where
-            // cell runs start (a cell different from the record before it;
-            // the batch's first against the run open before the batch).
-            // The tap loop is unrolled over the 64 records: each record's
-            // two kernel values are LDS reads at a fixed offset, its value
-            // a readlane broadcast, and the cell test a scalar bit test.
-            // Lanes past the taps compute on row 0 and never flush.
-            const int nb = __builtin_amdgcn_readfirstlane(min(64, r1 - b));
+            // kernel values (SUPPU*2+1 along u, SUPPV*2+1 along v, the v
+            // values times the record's weight) into its row of the wave's
+            // LDS record table, and the wave marks where cell runs start (a
+            // cell different from the record before it; the batch's first
+            // against the run open before the batch).  The tap loop is
+            // unrolled over all 64 records: each record's two table values
+            // are LDS reads at a fixed offset, its FVIS two readlane
+            // broadcasts, and the cell test a scalar bit test.  Slots past
+            // the wave's range repeat its last record with weight 0: they
+            // add zeros and start no run.  Lanes past the taps compute on
+            // row 0 and never flush.
             {
                 const int pu = (int)((mine.pk >> 16) & 0xff), pv = (int)(mine.pk >> 24);
                 float *my = rt + lane * kRecTaps;
                 for (int k = 0; k < ntapu; k++) my[k] = cflu[k * rsu + pu];
-                for (int k = 0; k < ntapv; k++) my[kMaxTaps + k] = cflv[k * rsv + pv];
+                for (int k = 0; k < ntapv; k++) my[kMaxTaps + k] = cflv[k * rsv + pv] * mine.wt;
             }
             const int mycell = (int)(mine.pk & 0xffff);
             const int before = __shfl_up(mycell, 1);
@@ -833,7 +836,6 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
             float ta = ru[0], tb = rv[0];
 #pragma unroll
             for (int j = 0; j < 64; j++) {
-                if (j < nb) {
                 const int jn = j + 1 < 64 ? j + 1 : 63;
                 const float tan = ru[jn * kRecTaps], tbn = rv[jn * kRecTaps];
                 if ((heads >> j) & 1ull) {
@@ -849,7 +851,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
                     s0 = s1 = s2 = 0.0;
                     f0 = f1 = f2 = 0.0f;
                 }
-                const float cw = ta * tb;
+                const float cw = ta * tb;  // WT * CF(u) * CF(v)
                 ta = tan;
                 tb = tbn;
                 if (CHM & 1) {
@@ -858,16 +860,12 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
                     f0 = fmaf(cw, re, f0);
                     f1 = fmaf(cw, im, f1);
                 }
-                if (CHM & 2) {
-                    const float wt = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.wt), j));
-                    f2 = fmaf(cw, wt, f2);
-                }
+                if (CHM & 2) f2 = f2 + cw;
                 if ((j & (kRunPart - 1)) == kRunPart - 1) {
                     s0 += (double)f0;
                     s1 += (double)f1;
                     s2 += (double)f2;
                     f0 = f1 = f2 = 0.0f;
-                }
                 }
             }
             s0 += (double)f0;

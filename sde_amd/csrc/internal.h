@@ -41,8 +41,9 @@ struct TileParams {
 // Gridding record of one sorted point, built and held in registers by the
 // gridding wave (16 bytes, one per lane):
 //   pk: local u | local v << 8 | u phase << 16 | v phase << 24
-//   wt: weight
-//   re, im: weight * visibility (float), after phase rotation and conjugation
+//   wt: weight (0 for a point that adds nothing)
+//   re, im: the visibility after phase rotation and conjugation (FVIS; the
+//           weight enters through the v kernel values of the tap table)
 struct __align__(16) GridRec {
     uint32_t pk;
     float wt, re, im;
