@@ -78,17 +78,39 @@ __device__ __forceinline__ double2 twid(const double2 *__restrict__ tw, int m, d
     return make_double2(w.x, sg * w.y);
 }
 
+// Radix-8 stages of a transform of up to 2^14 points (after the lead stage).
+constexpr int kMaxR8 = 4;
+
+// Ns of the first radix-8 stage: after a radix-2 (log2n = 1 mod 3) or
+// radix-4 (log2n = 2 mod 3) lead stage, or 1 without one.
+__device__ __forceinline__ int first_r8_ns(int log2n) {
+    const int lead = log2n % 3;
+    return lead == 1 ? 2 : (lead == 2 ? 4 : 1);
+}
+
+// v[r] *= w^r, r = 1..7, from w, w^2, w^4 (FP64 products)
+__device__ __forceinline__ void twiddle8(double2 *v, double2 w1, double2 w2, double2 w4) {
+    const double2 w3 = cmul(w1, w2);
+    v[1] = cmul(v[1], w1);
+    v[2] = cmul(v[2], w2);
+    v[3] = cmul(v[3], w3);
+    v[4] = cmul(v[4], w4);
+    v[5] = cmul(v[5], cmul(w1, w4));
+    v[6] = cmul(v[6], cmul(w2, w4));
+    v[7] = cmul(v[7], cmul(w3, w4));
+}
+
 // In-place Stockham FFT of n = 2^log2n points in LDS (element i at lpad(i)),
 // run by nthr threads (thread index tid).  sign = +1: exp(+i), sign = -1:
 // exp(-i); unnormalised.  tw is the table exp(+2 pi i k / (n*tws)).
-// Requires n <= 16 * nthr.
+// Requires n <= 8 * nthr and n <= 2^14 (every launch gives a transform at
+// least n/8 threads).
 __device__ __forceinline__ void lds_fft(double2 *x, int n, int log2n, int tid, int nthr, const double2 *__restrict__ tw,
                         int tws, int sign) {
-    constexpr int JM = 4;  // butterflies per thread in a radix-2 or radix-4 lead stage
-    constexpr int J8 = 2;  // radix-8 butterflies per thread per stage
+    constexpr int JM = 2;  // butterflies per thread in a radix-4 lead stage (twice that in a radix-2 one)
+    constexpr int J8 = 1;  // radix-8 butterflies per thread per stage
     const double sg = (double)sign;
     const int tabn = n * tws;
-    int ns = 1;
     const int lead = log2n % 3;
     if (lead == 1 && n >= 2) {
         // radix-2 stage, Ns = 1: (j, j + n/2) -> (2j, 2j + 1)
@@ -112,7 +134,6 @@ __device__ __forceinline__ void lds_fft(double2 *x, int n, int log2n, int tid, i
             }
         }
         __syncthreads();
-        ns = 2;
     } else if (lead == 2) {
         // radix-4 stage, Ns = 1: (j, j + q, j + 2q, j + 3q) -> 4j .. 4j + 3
         const int q = n / 4;
@@ -135,9 +156,11 @@ __device__ __forceinline__ void lds_fft(double2 *x, int n, int log2n, int tid, i
                 for (int r = 0; r < 4; r++) x[lpad(4 * j + r)] = v[c][r];
         }
         __syncthreads();
-        ns = 4;
     }
-    for (; ns < n; ns *= 8) {
+    int ns = first_r8_ns(log2n);
+#pragma unroll
+    for (int s = 0; s < kMaxR8; s++) {
+        if (ns >= n) break;
         const int q = n / 8;
         const int step = tabn / (ns * 8);
         double2 v[J8][8];
@@ -148,10 +171,9 @@ __device__ __forceinline__ void lds_fft(double2 *x, int n, int log2n, int tid, i
                 const int k = j & (ns - 1);
 #pragma unroll
                 for (int r = 0; r < 8; r++) v[c][r] = x[lpad(j + r * q)];
-                if (ns > 1) {
-#pragma unroll
-                    for (int r = 1; r < 8; r++) v[c][r] = cmul(v[c][r], twid(tw, r * k * step, sg));
-                }
+                // w^r for r = 1..7 from the table's w, w^2 and w^4 (three loads)
+                if (ns > 1)
+                    twiddle8(v[c], twid(tw, k * step, sg), twid(tw, 2 * k * step, sg), twid(tw, 4 * k * step, sg));
                 dft8(v[c], sg);
             }
         }
@@ -167,6 +189,7 @@ __device__ __forceinline__ void lds_fft(double2 *x, int n, int log2n, int tid, i
             }
         }
         __syncthreads();
+        ns *= 8;
     }
 }
 
@@ -185,6 +208,8 @@ __global__ void k_fft_cols(const float2 *srcs, float2 *dsts, int nu, int nv, siz
     const int icen = (nv + 1) / 2 - 1;  // 0-based centre row
     const int nthr = blockDim.x;
     const int ln = lds_len(nv);
+    const int c = threadIdx.x / tpc;
+    const int sign = dir < 0 ? +1 : -1;
     for (int idx = threadIdx.x; idx < cb * nv; idx += nthr) {
         const int vrow = idx / cb, c = idx - vrow * cb;
         const int u = u0 + c;
@@ -194,8 +219,7 @@ __global__ void k_fft_cols(const float2 *srcs, float2 *dsts, int nu, int nv, siz
         xs[c * ln + lpad(p)] = make_double2((double)g.x, (double)g.y);
     }
     __syncthreads();
-    const int c = threadIdx.x / tpc;
-    lds_fft(xs + c * ln, nv, ilog2(nv), threadIdx.x - c * tpc, tpc, tw, 1, dir < 0 ? +1 : -1);
+    lds_fft(xs + c * ln, nv, ilog2(nv), threadIdx.x - c * tpc, tpc, tw, 1, sign);
     const double sc = (double)scale;
     for (int idx = threadIdx.x; idx < cb * nv; idx += nthr) {
         const int vrow = idx / cb, cc = idx - vrow * cb;
