@@ -623,6 +623,11 @@ def main():
     total = time.perf_counter() - t0
     timings = plan.timings()
     info = plan.info()
+    # the gridding tile kernel's record gather alone, on the same sorted buffers
+    plan.reset_timings()
+    plan.gather_probe(5)
+    gp = plan.timings().get("gather_probe")
+    gather_ms = gp[0] / gp[1] if gp else None
     ms = total / args.steps * 1e3
     value = 2.0 * nvis * args.steps / total
 
@@ -640,6 +645,16 @@ def main():
             "traffic_counts": "FETCH_SIZE + WRITE_SIZE per launch from profiles/kernels_large_field.md, calibrated "
                               "on a device copy: traffic between L2 and the fabric, Infinity Cache hits included",
             "lds": read_lds(kname, "large_field")}
+    if kname == "grid_tiles" and gather_ms:
+        # a random 16-byte gather moves a whole line per point, so this kernel's
+        # ceiling is the gather's own rate, measured live on the same buffers
+        roof["gather_ceiling"] = {
+            "probe_ms": round(gather_ms, 3), "kernel_ms": round(avg_s * 1e3, 3),
+            "gathers_per_s_probe": round(nvis / (gather_ms * 1e-3), 1),
+            "gathers_per_s_kernel": round(nvis / avg_s, 1),
+            "frac": round(gather_ms / (avg_s * 1e3), 3),
+            "probe": "sde_plan_gather_probe: sorted values read in order, each point's 16-byte staging record "
+                     "gathered through its index, nothing gridded"}
     kernels = {k: {"ms_per_launch": round(v[0] / v[1], 3), "launches": v[1]} for k, v in timings.items()}
 
     out = {"metric": METRIC, "value": round(value, 1), "unit": "visibilities/s", "n_gpus": 1, "steps": args.steps,

@@ -302,6 +302,12 @@ int sde_plan_reset_timings(sde_plan *plan);
 /* Bandwidth probe: reps launches of a plain device copy of nbytes on the
  * plan's stream, timed as "copy_probe" (calibrates HBM counters). */
 int sde_plan_copy_probe(sde_plan *plan, int64_t nbytes, int reps);
+/* Gather probe (no counterpart in the reference; measurement only): reps
+ * launches of the gridding tile kernel's record access alone -- the sorted
+ * values read in order, each point's 16-byte staging record gathered through
+ * its index, nothing gridded -- on the plan's buffers, timed as
+ * "gather_probe".  Needs a cell sort (a uvmap run first). */
+int sde_plan_gather_probe(sde_plan *plan, int reps);
 
 /* Synthetic track: uvw of baselines between antennas (ENU metres) for hour
  * angles ha0 + k*dha (k < ntime, radians), declination dec, latitude lat,

@@ -2372,6 +2372,27 @@ int sde_plan_copy_probe(sde_plan *p, int64_t nbytes, int reps) {
     return 0;
 }
 
+int sde_plan_gather_probe(sde_plan *p, int reps) {
+    if (!p || reps < 1) return -1;
+    if (!p->sorted_ok) {
+        set_plan_error("gather probe: no cell sort (run a uvmap first)");
+        return -1;
+    }
+    void *blk = nullptr;
+    SDE_CHECK(hipMalloc(&blk, sizeof(float) * 1024));
+    SDE_CHECK(hipMemsetAsync(blk, 0, sizeof(float) * 1024, p->stream));
+    for (int r = 0; r < reps; r++) {
+        plan_mark_begin(p, "gather_probe");
+        launch_gather_probe(p->stream, p->sb, p->nvis, static_cast<float *>(blk));
+        plan_mark_end(p);
+    }
+    SDE_CHECK(hipGetLastError());
+    SDE_CHECK(hipStreamSynchronize(p->stream));
+    plan_collect(p);
+    (void)hipFree(blk);
+    return 0;
+}
+
 int sde_plan_reset_timings(sde_plan *p) {
     if (!p) return -1;
     SDE_CHECK(hipStreamSynchronize(p->stream));

@@ -912,6 +912,22 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
     for (int k = threadIdx.x; k < NCH * NC; k += 64 * W) out[k] = (float)S[k];
 }
 
+// The gridding tile kernel's record access alone: the sorted values read
+// in order and each point's 16-byte staging record gathered through its
+// index, nothing gridded (a measurement of that access's rate on the
+// plan's own buffers; the sum keeps the loads live).
+__global__ __launch_bounds__(256) void k_gather_probe(const uint64_t *__restrict__ svals,
+                                                      const StageHot *__restrict__ hot, int64_t n,
+                                                      float *__restrict__ blk) {
+    float acc = 0.0f;
+    for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < n; r += (int64_t)gridDim.x * 256) {
+        const StageHot h = hot[sval_index(svals[r])];
+        acc += h.fr + h.fi + __uint_as_float(h.gpk & 0x7fffffu) + __uint_as_float(h.dpk & 0x7fffffu);
+    }
+    for (int d = 32; d > 0; d >>= 1) acc += __shfl_down(acc, d);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&blk[blockIdx.x & 1023], acc);
+}
+
 struct Sum3 {
     double re, im, p;
 };
@@ -1591,6 +1607,10 @@ void launch_density_runs(hipStream_t s, const SortBufs &b, int64_t n, const Tile
                          const CoordParams &cp, double *gwt, double *blk) {
     hipLaunchKernelGGL(k_density_runs, dim3(run_blocks(n)), dim3(kThreads), 0, s, b.skeys, b.svals, n,
                        (uint32_t)sort_nkeys(tp), tp, cp.nu, cp.suppu, gwt, blk, b.tile_start);
+}
+
+void launch_gather_probe(hipStream_t s, const SortBufs &b, int64_t n, float *blk) {
+    hipLaunchKernelGGL(k_gather_probe, dim3(8192), dim3(256), 0, s, b.svals, b.hot, n, blk);
 }
 
 void launch_restage(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v, const float *w,

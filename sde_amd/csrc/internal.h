@@ -317,6 +317,9 @@ int launch_grid_tiles(hipStream_t s, const SortBufs &b, const TileParams &tp, in
                       const ZPlanes *zp = nullptr, int *peak_reset = nullptr);
 void launch_grid_reduce(hipStream_t s, const float *partial, const int *item_start, const TileParams &tp,
                         int chmask, const CoordParams &cp, const double *sums, float2 *dirty, float2 *psf);
+// The gridding tile kernel's record access alone over the n sorted points
+// (measurement; blk receives per-block sums, 1024 floats).
+void launch_gather_probe(hipStream_t s, const SortBufs &b, int64_t n, float *blk);
 size_t grid_partial_floats(const TileParams &tp, int chmask, int max_items);
 
 // ---- many fields, one launch per stage (grid.hip, fft.hip) ----

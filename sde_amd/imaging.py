@@ -271,6 +271,9 @@ class Plan:
     def copy_probe(self, nbytes, reps=5):
         self._ok(lib().sde_plan_copy_probe(self._p, C.c_int64(int(nbytes)), int(reps)), "copy_probe")
 
+    def gather_probe(self, reps=3):
+        self._ok(lib().sde_plan_gather_probe(self._p, int(reps)), "gather_probe")
+
     def device_vis(self):
         ptrs = [C.POINTER(C.c_float)() for _ in range(5)]
         self._ok(lib().sde_plan_device_vis(self._p, *[C.byref(x) for x in ptrs]), "device_vis")

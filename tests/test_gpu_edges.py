@@ -97,3 +97,26 @@ def test_full_field_dirty_image_is_linear(full_field):
     peak = np.abs(dirty).max()
     assert np.abs(dirty2 - 2 * dirty).max() <= 1e-6 * peak
     plan.set_vis(vis, wt, u, v, w)
+
+
+def test_gather_probe_needs_a_cell_sort_and_leaves_the_images():
+    """The measurement entry point (sde_plan_gather_probe) refuses a plan
+    with no cell sort, times its launches once there is one, and changes
+    nothing a later call reads."""
+    d = synth.small_field()
+    plan = imaging.Plan(d["nx"], d["nx"], d["cellsize"], "SF")
+    plan.set_vis(d["vis"], d["wt"], d["u"], d["v"], d["w"])
+    with pytest.raises(sde.SdeError, match="no cell sort"):
+        plan.gather_probe(1)
+    plan.profile(True)
+    plan.uvmap()
+    dirty0, psf0 = plan.images()
+    plan.reset_timings()
+    plan.gather_probe(2)
+    t = plan.timings()["gather_probe"]
+    assert t[1] == 2 and t[0] > 0.0
+    plan.uvmap()
+    dirty1, psf1 = plan.images()
+    np.testing.assert_array_equal(psf1, psf0)
+    np.testing.assert_array_equal(dirty1, dirty0)
+    plan.close()
