@@ -170,12 +170,37 @@ __global__ __launch_bounds__(kThreads) void k_prep(
     unsigned *hist = s_hist[threadIdx.x >> 6];
     // block-uniform loop bounds: every lane of a wave takes part in the list append
     for (int64_t c0 = (int64_t)blockIdx.x * chunk; c0 < n; c0 += (int64_t)gridDim.x * chunk) {
+        // each point's inputs are loaded one step ahead, so their latency
+        // overlaps the step before
+        float wtn = 0.0f, un = 0.0f, vn = 0.0f, wn = 0.0f;
+        float2 vvn = make_float2(0.0f, 0.0f);
+        {
+            const int64_t i = c0 + threadIdx.x;
+            if (i < n) {
+                wtn = wt[i];
+                un = u[i];
+                vn = v[i];
+                wn = w[i];
+                vvn = vis[i];
+            }
+        }
         for (int sub = 0; sub < chunk && c0 + sub < n; sub += kThreads) {
             const int64_t i = c0 + sub + threadIdx.x;
             bool to_list = false;
+            float wti = wtn;
+            const float uf = un, vf = vn, wf = wn;
+            const float2 vv = vvn;
+            {
+                const int64_t i2 = i + kThreads;
+                if (sub + kThreads < chunk && i2 < n) {
+                    wtn = wt[i2];
+                    un = u[i2];
+                    vn = v[i2];
+                    wn = w[i2];
+                    vvn = vis[i2];
+                }
+            }
             if (i < n) {
-                float wti = wt[i];
-                const float uf = u[i], vf = v[i], wf = w[i];
                 if (sel.on && wti > 0.0f) {
                     if (vissel_out(sel, sel.time[i], uf, vf)) {
                         wti = -fabsf(wti);
@@ -184,7 +209,6 @@ __global__ __launch_bounds__(kThreads) void k_prep(
                         nkept++;
                     }
                 }
-                const float2 vv = vis[i];
                 StageHot sr;
                 sr.gpk = 0u;
                 sr.fr = 0.0f;
