@@ -555,16 +555,31 @@ __global__ __launch_bounds__(256) void k_degrid_tiles(const DegridRec *__restric
     for (int k = threadIdx.x; k < ncfu; k += blockDim.x) cflu[k] = cfu_g[cp.osampu - poffu + k];
     for (int k = threadIdx.x; k < ncfv; k += blockDim.x) cflv[k] = cfv_g[cp.osampv - poffv + k];
     __syncthreads();
-    for (int r = wi.begin + threadIdx.x; r < wi.end; r += blockDim.x) {
-        DegridRec d;
+    // each record is loaded one step ahead of its gather
+    DegridRec dn;
+    if (wi.begin + (int)threadIdx.x < wi.end) {
         if (R8) {
-            const DegridRec8 d8 = rec8[r];
-            d.idx = d8.idx;
-            d.pk = d8.pk;
-            d.rr = 1.0f;
-            d.ri = 0.0f;
+            const DegridRec8 d8 = rec8[wi.begin + threadIdx.x];
+            dn.idx = d8.idx;
+            dn.pk = d8.pk;
+            dn.rr = 1.0f;
+            dn.ri = 0.0f;
         } else {
-            d = rec[r];
+            dn = rec[wi.begin + threadIdx.x];
+        }
+    }
+    for (int r = wi.begin + threadIdx.x; r < wi.end; r += blockDim.x) {
+        const DegridRec d = dn;
+        if (r + (int)blockDim.x < wi.end) {
+            if (R8) {
+                const DegridRec8 d8 = rec8[r + blockDim.x];
+                dn.idx = d8.idx;
+                dn.pk = d8.pk;
+                dn.rr = 1.0f;
+                dn.ri = 0.0f;
+            } else {
+                dn = rec[r + blockDim.x];
+            }
         }
         if (d.idx == kNoDegrid) continue;
         const int lu = d.pk & 0xff, lv = (d.pk >> 8) & 0xff;
