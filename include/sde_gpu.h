@@ -22,6 +22,14 @@
  * Part 2: a plan API that keeps visibilities, grids and images resident in
  * HBM across calls (imaging + prediction without host round trips), used by
  * the host drivers and the benchmark.
+ *
+ * Devices: a plan lives on the device current when it is created.  The
+ * drop-ins of Part 1 and the *_device entry points share the library's own
+ * stream and buffers, which live on the device current at the first such
+ * call.  A Part 1 drop-in runs there whatever device is current (the call
+ * makes it current and gives the caller's device back); a *_device entry
+ * point takes arrays on that device and fails, with sde_last_error() set,
+ * when another device is current.
  */
 #ifndef SDE_GPU_H
 #define SDE_GPU_H

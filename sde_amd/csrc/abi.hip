@@ -256,9 +256,12 @@ struct DevBuf {
     size_t cap = 0;
 };
 
+// The drop-ins' context: a stream and grow-only buffers on one device, the
+// device current at the library's first call that needs it.
 struct Ctx {
     bool ready = false;
     int ndev = 0;
+    int dev = 0;
     hipStream_t stream = nullptr;
     std::map<std::string, DevBuf> bufs;
     std::map<int, double2 *> tw;
@@ -275,9 +278,43 @@ int ctx_init(Ctx &c) {
         return -1;
     }
     c.ndev = n;
-    SDE_CHECK(hipSetDevice(0));
+    SDE_CHECK(hipGetDevice(&c.dev));
     SDE_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     c.ready = true;
+    return 0;
+}
+
+// A host-array drop-in runs on the context's device whatever device the
+// caller has current: enter() makes it current for the call, the destructor
+// gives the caller's back.
+struct CtxDevice {
+    int prev = -1;
+    bool switched = false;
+    int enter(Ctx &c) {
+        if (ctx_init(c)) return -1;
+        SDE_CHECK(hipGetDevice(&prev));
+        if (prev != c.dev) {
+            SDE_CHECK(hipSetDevice(c.dev));
+            switched = true;
+        }
+        return 0;
+    }
+    ~CtxDevice() {
+        if (switched) (void)hipSetDevice(prev);
+    }
+};
+
+// A device-array entry point runs on the context's stream: the caller's
+// arrays must live on the context's device, which must be current.
+int ctx_device_arrays(Ctx &c, const char *who) {
+    if (ctx_init(c)) return -1;
+    int d = -1;
+    SDE_CHECK(hipGetDevice(&d));
+    if (d != c.dev) {
+        set_plan_error("%s: the current device is %d; the library's drop-in context (its stream and buffers) is on "
+                       "device %d, the device current at its first call", who, d, c.dev);
+        return -1;
+    }
     return 0;
 }
 
@@ -652,7 +689,8 @@ void grdh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, flo
     if (!check_kernel_args(R, *nvis, *nu, *nv, *suppu, *suppv, *osampu, *osampv, *uorigin, *vorigin)) return;
     std::lock_guard<std::mutex> lk(g_ctx_mu);
     Ctx &c = g_ctx;
-    if (ctx_init(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
+    CtxDevice cdev;
+    if (cdev.enter(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
     hipStream_t s = c.stream;
     const int64_t n = *nvis;
     const int ncfu = 1 + 2 * (*suppu + 1) * *osampu, ncfv = 1 + 2 * (*suppv + 1) * *osampv;
@@ -733,7 +771,8 @@ void grdqh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, fl
     if (!check_kernel_args(R, *nvis, *nu, *nv, 0, 0, 1, 1, *uorigin, *vorigin)) return;
     std::lock_guard<std::mutex> lk(g_ctx_mu);
     Ctx &c = g_ctx;
-    if (ctx_init(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
+    CtxDevice cdev;
+    if (cdev.enter(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
     hipStream_t s = c.stream;
     const int64_t n = *nvis;
     const size_t nn = (size_t)(n > 0 ? n : 1);
@@ -807,7 +846,8 @@ void visselpi_(float *wt, int *nvis, float *time, float *u, float *v, float *w, 
     }
     std::lock_guard<std::mutex> lk(g_ctx_mu);
     Ctx &c = g_ctx;
-    if (ctx_init(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
+    CtxDevice cdev;
+    if (cdev.enter(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
     hipStream_t s = c.stream;
     const int64_t n = *nvis;
     const size_t nn = (size_t)(n > 0 ? n : 1);
@@ -846,7 +886,8 @@ void grddeh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, f
     if (!check_kernel_args(R, *nvis, *nu, *nv, *suppu, *suppv, *osampu, *osampv, *uorigin, *vorigin)) return;
     std::lock_guard<std::mutex> lk(g_ctx_mu);
     Ctx &c = g_ctx;
-    if (ctx_init(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
+    CtxDevice cdev;
+    if (cdev.enter(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
     hipStream_t s = c.stream;
     const int64_t n = *nvis;
     const size_t nn = (size_t)(n > 0 ? n : 1);
@@ -902,7 +943,8 @@ void grdh23d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, fl
         return;
     std::lock_guard<std::mutex> lk(g_ctx_mu);
     Ctx &c = g_ctx;
-    if (ctx_init(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
+    CtxDevice cdev;
+    if (cdev.enter(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
     hipStream_t s = c.stream;
     const int64_t n = *nvis;
     const int ncfu = 1 + 2 * (*suppu + 1) * *osampu, ncfv = 1 + 2 * (*suppv + 1) * *osampv;
@@ -948,7 +990,8 @@ void grdqh23d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, f
     if (!check_kernel_args(R, *nvis, *nu, *nv, 0, 0, 1, 1) || !check_planes(R, *nz)) return;
     std::lock_guard<std::mutex> lk(g_ctx_mu);
     Ctx &c = g_ctx;
-    if (ctx_init(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
+    CtxDevice cdev;
+    if (cdev.enter(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
     hipStream_t s = c.stream;
     const int64_t n = *nvis;
     const size_t nn = (size_t)(n > 0 ? n : 1), ng = (size_t)*nu * *nv * *nz;
@@ -992,7 +1035,8 @@ void grddeh23_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, f
         return;
     std::lock_guard<std::mutex> lk(g_ctx_mu);
     Ctx &c = g_ctx;
-    if (ctx_init(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
+    CtxDevice cdev;
+    if (cdev.enter(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
     hipStream_t s = c.stream;
     const int64_t n = *nvis;
     const size_t nn = (size_t)(n > 0 ? n : 1), ng = (size_t)*nu * *nv * *nz;
@@ -1042,7 +1086,8 @@ static void weight_common(const char *R, float *wt, float *u, float *v, float *w
                           float *gwt, int *nu, int *nv, double *shift, int mode, const float *robust, float ds) {
     std::lock_guard<std::mutex> lk(g_ctx_mu);
     Ctx &c = g_ctx;
-    if (ctx_init(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
+    CtxDevice cdev;
+    if (cdev.enter(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
     hipStream_t s = c.stream;
     const int64_t n = *nvis;
     const size_t nn = (size_t)(n > 0 ? n : 1), ncell = (size_t)*nu * *nv;
@@ -1146,7 +1191,8 @@ static void pixcor_common(const char *R, float *in, int *nx, int *ny, float *cf,
     }
     std::lock_guard<std::mutex> lk(g_ctx_mu);
     Ctx &c = g_ctx;
-    if (ctx_init(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
+    CtxDevice cdev;
+    if (cdev.enter(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
     hipStream_t s = c.stream;
     const size_t npix = (size_t)*nx * *ny * (cplx ? 2 : 1);
     float *din = ctx_buf<float>(c, "pix_in", npix), *dout = ctx_buf<float>(c, "pix_out", npix);
@@ -1187,7 +1233,8 @@ void fftr2d_(float *rarray, float *xarray, float *work, float *work2, float *rwo
     }
     std::lock_guard<std::mutex> lk(g_ctx_mu);
     Ctx &c = g_ctx;
-    if (ctx_init(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
+    CtxDevice cdev;
+    if (cdev.enter(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
     hipStream_t s = c.stream;
     const int nu = *fnu / 2 + 1;
     const size_t ng = (size_t)nu * *nv, npix = (size_t)*nx * *ny;
@@ -1227,7 +1274,8 @@ void fftx1d_(float *xarray, float *xwork, float *xwork2, int *nx, int *dir) {
     }
     std::lock_guard<std::mutex> lk(g_ctx_mu);
     Ctx &c = g_ctx;
-    if (ctx_init(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
+    CtxDevice cdev;
+    if (cdev.enter(c)) { report_error(kErrFatal, R, sde_last_error()); return; }
     hipStream_t s = c.stream;
     float2 *dg = ctx_buf<float2>(c, "fft_grid", *nx);
     double2 *tw = nullptr;
@@ -2018,7 +2066,7 @@ int sde_grdh23d_device(const void *vis, const float *wt, const float *u, const f
         return -1;
     std::lock_guard<std::mutex> lk(g_ctx_mu);
     Ctx &c = g_ctx;
-    if (ctx_init(c)) return -1;
+    if (ctx_device_arrays(c, "sde_grdh23d_device")) return -1;
     const CoordParams cp = make_cp(shift, uscale, uoffset, vscale, voffset, uorigin, vorigin, nu, nv, suppu, suppv,
                                    osampu, osampv);
     double out[2];
@@ -2059,7 +2107,7 @@ int sde_vistoimg_slowz_device(const void *vis, const float *wt, const float *u, 
     make_ccf(code, ccf.data(), ccflen);
     std::lock_guard<std::mutex> lk(g_ctx_mu);
     Ctx &c = g_ctx;
-    if (ctx_init(c)) return -1;
+    if (ctx_device_arrays(c, "sde_vistoimg_slowz_device")) return -1;
     hipStream_t s = c.stream;
     const int ncf = 1 + 2 * (supp + 1) * osamp;
     float *dcf = ctx_buf<float>(c, "cfz", ncf), *dccf = ctx_buf<float>(c, "ccfz", ccflen);
@@ -2127,7 +2175,7 @@ int64_t sde_imgtovis_slowz_device(const float *images, int nx, int ny, int fnu, 
     make_ccf(code, ccf.data(), ccflen);
     std::lock_guard<std::mutex> lk(g_ctx_mu);
     Ctx &c = g_ctx;
-    if (ctx_init(c)) return -1;
+    if (ctx_device_arrays(c, "sde_imgtovis_slowz_device")) return -1;
     hipStream_t s = c.stream;
     const int ncf = 1 + 2 * (supp + 1) * osamp;
     float *dcf = ctx_buf<float>(c, "cfz", ncf), *dccf = ctx_buf<float>(c, "ccfz", ccflen);
@@ -2164,7 +2212,7 @@ int64_t sde_grddeh23_device(void *vis, float *wt, const float *u, const float *v
         return -1;
     std::lock_guard<std::mutex> lk(g_ctx_mu);
     Ctx &c = g_ctx;
-    if (ctx_init(c)) return -1;
+    if (ctx_device_arrays(c, "sde_grddeh23_device")) return -1;
     const CoordParams cp = make_cp(shift, uscale, uoffset, vscale, voffset, uorigin, vorigin, nu, nv, suppu, suppv,
                                    osampu, osampv);
     return slowz_degrid(c, (const float2 *)gvis, (float2 *)vis, wt, u, v, w, nvis, cp, make_pp(nz, cellz, refz),
@@ -2547,11 +2595,10 @@ int sde_grdh2d_device_noalloc(const void *vis, const float *wt, const float *u, 
         return -1;
     std::lock_guard<std::mutex> lk(g_ctx_mu);
     Ctx &c = g_ctx;
-    if (ctx_init(c)) return -1;
-    int dev = 0;
-    SDE_CHECK(hipGetDevice(&dev));
+    if (ctx_device_arrays(c, "sde_grdh2d_device_noalloc")) return -1;
+    // the scratch array of the context's device, which is current
     unsigned char *base = nullptr;
-    if (scratch_address(dev, &base)) return -1;
+    if (scratch_address(c.dev, &base)) return -1;
     hipStream_t s = c.stream;
     const int64_t n = nvis;
     const CoordParams cp = make_cp(shift, uscale, uoffset, vscale, voffset, uorigin, vorigin, nu, nv, suppu, suppv,
@@ -2617,7 +2664,7 @@ int sde_sort_pairs_device(const uint32_t *keys_in, uint32_t *keys_out, const uin
     }
     std::lock_guard<std::mutex> lk(g_ctx_mu);
     Ctx &c = g_ctx;
-    if (ctx_init(c)) return -1;
+    if (ctx_device_arrays(c, "sde_sort_pairs_device")) return -1;
     hipStream_t s = c.stream;
     const size_t tb = sort_temp_bytes(n, nbits);
     char *temp = ctx_buf<char>(c, "sort_temp", tb);

@@ -691,7 +691,6 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
     constexpr int NC = P * P;
     constexpr int NCH = ((CHM & 1) ? 2 : 0) + ((CHM & 2) ? 1 : 0);
     constexpr int PSF = (CHM & 1) ? 2 : 0;
-    constexpr bool PSF_ONLY = CHM == 2;
     extern __shared__ double smem_d[];
     __shared__ double s_red[2 * W];
     double *S = smem_d;                          // [NCH][NC] FP64 sums

@@ -3,7 +3,7 @@
 // The cell sort orders the visibilities by tile-major cell key; the values
 // are the original index and the weight of each point.  The sort is
 // stable, so points of one cell keep their original order and the sorted
-// order -- and every result built from it -- is deterministic.  Only bits
+// order is deterministic (so is every sum taken over it in a fixed order).  Only bits
 // [0, nbits) order the keys; bits above ride along (bit 31 carries a flag
 // of the prep pass).
 //

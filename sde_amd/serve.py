@@ -18,8 +18,12 @@ kernels run side by side with those already in flight.  A job is done when
 its stream is (non-blocking query); its slot goes back to the pool.
 
 The plain form serves the same requests one at a time from the same pool.
-Results are identical in both forms: each job's buffers are its own and
-every kernel is deterministic.
+Each job's buffers are its own and its sorted order, work items and
+per-item sums are fixed, so a job gives the same images in both forms; the
+one order that is not fixed is the waves' FP64 additions into a tile's LDS
+cells in the gridding kernel, which can differ in the last FP64 bit
+between runs and is rounded away in the FP32 images (the tests compare
+both forms bit for bit).
 """
 import collections
 import time
