@@ -24,7 +24,7 @@ import time
 import numpy as np
 
 from . import SdeError, lib
-from .imaging import Plan, RunInfo, _cfg
+from .imaging import Plan, RunInfo, _cfg, _same_length
 
 
 def make_plans(jobs, **plan_args):
@@ -118,6 +118,7 @@ class FieldBatch:
             raise ValueError("FieldBatch: only %s apply to a batch (got %s)" % (sorted(allowed), sorted(bad)))
         cfgs = (type(_cfg(2)) * len(jobs))()
         for k, j in enumerate(jobs):
+            _same_length(j["vis"], j["wt"], j["u"], j["v"], j["w"])
             cfgs[k] = _cfg(j["nx"], j.get("ny"), j["cellsize"], shift=j.get("shift"), **plan_args)
         self._b = C.c_void_p()
         if L.sde_batch_create(cfgs, C.c_int(len(jobs)), C.byref(self._b)) != 0:

@@ -46,6 +46,15 @@ def _c64(a):
     return np.ascontiguousarray(a, dtype=np.complex64)
 
 
+def _nvis(*arrays):
+    """The common length of per-visibility arrays (the routines take one NVIS
+    for all of them)."""
+    n = len(arrays[0])
+    if any(len(a) != n for a in arrays[1:]):
+        raise ValueError("per-visibility arrays of different lengths: %s" % [len(a) for a in arrays])
+    return n
+
+
 def _shift(s):
     s = np.ascontiguousarray(np.asarray(s, np.float64).reshape(-1))
     if s.size != 9:
@@ -90,7 +99,7 @@ def grdh2d(vis, wt, u, v, w, uscale, uoffset, vscale, voffset, uorigin, vorigin,
 
     Returns (gvis (NV,NU) complex64, sumwt)."""
     vis, wt, u, v, w = _c64(vis), _f32(wt), _f32(u), _f32(v), _f32(w)
-    n = len(u)
+    n = _nvis(vis, wt, u, v, w)
     gridfnu = _f32(gridfnu)
     gridfnv = gridfnu if gridfnv is None else _f32(gridfnv)
     suppv = suppu if suppv is None else suppv
@@ -111,7 +120,8 @@ def grdqh2d(vis, wt, u, v, w, uscale, uoffset, vscale, voffset, uorigin, vorigin
     vis, wt, u, v, w = _c64(vis), _f32(wt), _f32(u), _f32(v), _f32(w)
     g = np.zeros((int(nv), int(nu)), np.complex64)
     sumwt = C.c_float(0.0)
-    lib().grdqh2d_(_pf(vis), _pf(wt), _pf(u), _pf(v), _pf(w), _ri(len(u)), _rf(uscale), _rf(uoffset),
+    n = _nvis(vis, wt, u, v, w)
+    lib().grdqh2d_(_pf(vis), _pf(wt), _pf(u), _pf(v), _pf(w), _ri(n), _rf(uscale), _rf(uoffset),
                    _rf(vscale), _rf(voffset), _ri(uorigin), _ri(vorigin), _pf(g), _ri(1 if psf else 0), _ri(nu),
                    _ri(nv), _pd(_shift(shift)), C.byref(sumwt))
     check_error("GRDQH2D")
@@ -127,7 +137,8 @@ def visselpi(wt, time, u, v, w, timerang, uvlimits):
     tr = _f32(np.asarray(timerang, np.float32).reshape(-1))
     uvl = _f32(np.asarray(uvlimits, np.float32).reshape(-1))
     nsel = C.c_int(-1)
-    lib().visselpi_(_pf(wt), _ri(len(wt)), _pf(time), _pf(u), _pf(v), _pf(w), _pf(tr), _pf(uvl), _pf(wt),
+    n = _nvis(wt, time, u, v, w)
+    lib().visselpi_(_pf(wt), _ri(n), _pf(time), _pf(u), _pf(v), _pf(w), _pf(tr), _pf(uvl), _pf(wt),
                     C.byref(nsel))
     check_error("VISSELPI")
     return wt, nsel.value
@@ -148,7 +159,8 @@ def grddeh2d(vis, wt, u, v, w, uscale, uoffset, vscale, voffset, uorigin, vorigi
     intfnv = intfnu if intfnv is None else _f32(intfnv)
     suppv = suppu if suppv is None else suppv
     osampv = osampu if osampv is None else osampv
-    lib().grddeh2d_(_pf(vis), _pf(wt), _pf(u), _pf(v), _pf(w), _ri(len(u)), _rf(uscale), _rf(uoffset),
+    n = _nvis(vis, wt, u, v, w)
+    lib().grddeh2d_(_pf(vis), _pf(wt), _pf(u), _pf(v), _pf(w), _ri(n), _rf(uscale), _rf(uoffset),
                     _rf(vscale), _rf(voffset), _ri(uorigin), _ri(vorigin), _pf(gvis), _ri(nu), _ri(nv),
                     _pf(intfnu), _ri(suppu), _ri(osampu), _pf(intfnv), _ri(suppv), _ri(osampv), _pd(_shift(shift)))
     check_error("GRDDEH2D")
@@ -168,7 +180,8 @@ def grdh23d(vis, wt, u, v, w, uscale, uoffset, vscale, voffset, uorigin, vorigin
     osampv = osampu if osampv is None else osampv
     g = np.zeros((int(nz), int(nv), int(nu)), np.complex64)
     sumwt = C.c_float(0.0)
-    lib().grdh23d_(_pf(vis), _pf(wt), _pf(u), _pf(v), _pf(w), _ri(len(u)), _rf(uscale), _rf(uoffset), _rf(vscale),
+    n = _nvis(vis, wt, u, v, w)
+    lib().grdh23d_(_pf(vis), _pf(wt), _pf(u), _pf(v), _pf(w), _ri(n), _rf(uscale), _rf(uoffset), _rf(vscale),
                    _rf(voffset), _ri(uorigin), _ri(vorigin), _pf(g), _ri(1 if psf else 0), _ri(nu), _ri(nv),
                    _pf(gridfnu), _ri(suppu), _ri(osampu), _pf(gridfnv), _ri(suppv), _ri(osampv), _ri(nz),
                    _rf(cellz), _rf(refz), _pd(_shift(shift)), C.byref(sumwt))
@@ -183,7 +196,8 @@ def grdqh23d(vis, wt, u, v, w, uscale, uoffset, vscale, voffset, uorigin, vorigi
     vis, wt, u, v, w = _c64(vis), _f32(wt), _f32(u), _f32(v), _f32(w)
     g = np.zeros((int(nz), int(nv), int(nu)), np.complex64)
     sumwt = C.c_float(0.0)
-    lib().grdqh23d_(_pf(vis), _pf(wt), _pf(u), _pf(v), _pf(w), _ri(len(u)), _rf(uscale), _rf(uoffset),
+    n = _nvis(vis, wt, u, v, w)
+    lib().grdqh23d_(_pf(vis), _pf(wt), _pf(u), _pf(v), _pf(w), _ri(n), _rf(uscale), _rf(uoffset),
                     _rf(vscale), _rf(voffset), _ri(uorigin), _ri(vorigin), _pf(g), _ri(1 if psf else 0), _ri(nu),
                     _ri(nv), _ri(nz), _rf(cellz), _rf(refz), _pd(_shift(shift)), C.byref(sumwt))
     check_error("GRDQH23D")
@@ -205,7 +219,8 @@ def grddeh23(vis, wt, u, v, w, uscale, uoffset, vscale, voffset, uorigin, vorigi
     intfnv = intfnu if intfnv is None else _f32(intfnv)
     suppv = suppu if suppv is None else suppv
     osampv = osampu if osampv is None else osampv
-    lib().grddeh23_(_pf(vis), _pf(wt), _pf(u), _pf(v), _pf(w), _ri(len(u)), _rf(uscale), _rf(uoffset),
+    n = _nvis(vis, wt, u, v, w)
+    lib().grddeh23_(_pf(vis), _pf(wt), _pf(u), _pf(v), _pf(w), _ri(n), _rf(uscale), _rf(uoffset),
                     _rf(vscale), _rf(voffset), _ri(uorigin), _ri(vorigin), _pf(gvis), _ri(nu), _ri(nv),
                     _pf(intfnu), _ri(suppu), _ri(osampu), _pf(intfnv), _ri(suppv), _ri(osampv), _ri(nz),
                     _rf(cellz), _rf(refz), _pd(_shift(shift)))
@@ -272,7 +287,8 @@ def grduwt2d(wt, u, v, w, uscale, uoffset, vscale, voffset, uorigin, vorigin, nu
     wt, u, v, w = _f32(wt), _f32(u), _f32(v), _f32(w)
     out = wt.copy() if newwt is None else _f32(newwt).copy()
     gwt = np.zeros((int(nv), int(nu)), np.float32)
-    lib().grduwt2d_(_pf(wt), _pf(u), _pf(v), _pf(w), _ri(len(u)), _rf(uscale), _rf(uoffset), _rf(vscale),
+    n = _nvis(wt, u, v, w)
+    lib().grduwt2d_(_pf(wt), _pf(u), _pf(v), _pf(w), _ri(n), _rf(uscale), _rf(uoffset), _rf(vscale),
                     _rf(voffset), _ri(uorigin), _ri(vorigin), _pf(out), _pf(gwt), _ri(nu), _ri(nv),
                     _pd(_shift(shift)))
     check_error("GRDUWT2D")
@@ -289,7 +305,8 @@ def grdruw2d(wt, u, v, w, uscale, uoffset, vscale, voffset, uorigin, vorigin, nu
     gwt = np.zeros((int(nv), int(nu)), np.float32)
     rob = _f32(np.resize(np.asarray(robust, np.float32).reshape(-1), 3))
     m = rmode.encode()
-    lib().grdruw2d_(_pf(wt), _pf(u), _pf(v), _pf(w), _ri(len(u)), _rf(uscale), _rf(uoffset), _rf(vscale),
+    n = _nvis(wt, u, v, w)
+    lib().grdruw2d_(_pf(wt), _pf(u), _pf(v), _pf(w), _ri(n), _rf(uscale), _rf(uoffset), _rf(vscale),
                     _rf(voffset), _ri(uorigin), _ri(vorigin), _pf(out), _pf(gwt), _ri(nu), _ri(nv),
                     _pd(_shift(shift)), C.c_char_p(m), _pf(rob), _rf(ds), C.c_size_t(len(m)))
     check_error("GRDRUW2D")

@@ -71,6 +71,13 @@ def crdshift(ra_vis, dec_vis, ra_img, dec_img, rota_vis=0.0):
     return np.ascontiguousarray(t.T.reshape(-1))  # column-major SHIFT(i,j) at [i + 3j]
 
 
+def _same_length(*arrays):
+    """The plan takes one visibility count for all its per-visibility arrays."""
+    n = len(arrays[0])
+    if any(len(a) != n for a in arrays[1:]):
+        raise ValueError("per-visibility arrays of different lengths: %s" % [len(a) for a in arrays])
+
+
 class Plan:
     """Device-resident imaging plan (image size, cell, kernel, weighting)."""
 
@@ -123,6 +130,7 @@ class Plan:
     def set_vis(self, vis, wt, u, v, w):
         vis = np.ascontiguousarray(vis, np.complex64)
         arrs = [np.ascontiguousarray(a, np.float32) for a in (wt, u, v, w)]
+        _same_length(vis, *arrs)
         self._keep = (vis, arrs)
         fp = C.POINTER(C.c_float)
         self._ok(lib().sde_plan_set_vis(self._p, C.c_int64(len(vis)), vis.ctypes.data_as(fp),
@@ -137,6 +145,7 @@ class Plan:
             raise TypeError("set_vis_async: complex64 vis and float32 wt, u, v, w")
         if not all(a.flags.c_contiguous for a in (vis, wt, u, v, w)):
             raise ValueError("set_vis_async: contiguous arrays")
+        _same_length(vis, wt, u, v, w)
         self._keep = (vis, wt, u, v, w)
         fp = C.POINTER(C.c_float)
         self._ok(lib().sde_plan_set_vis(self._p, C.c_int64(len(vis)), vis.ctypes.data_as(fp),

@@ -131,3 +131,17 @@ def test_w_plane_routines_reject_plane_counts_before_any_device_work(lib, nz):
     assert lib.sde_error_flag() == 1 and b"GRDDEH23" in lib.sde_last_error()
     assert np.all(vis == 1) and np.all(f == 1)
     lib.sde_error_clear()
+
+
+def test_python_mirror_rejects_arrays_of_different_lengths():
+    """The routines take one NVIS for every per-visibility array; the Python
+    mirror refuses arrays of different lengths before the call (no library
+    call is made, so this runs without a GPU)."""
+    from sde_amd import grd
+    from sde_amd.imaging import _same_length
+    a, b = np.zeros(5, np.float32), np.zeros(4, np.float32)
+    with pytest.raises(ValueError, match="different lengths"):
+        grd._nvis(a, a, b)
+    with pytest.raises(ValueError, match="different lengths"):
+        _same_length(np.zeros(5, np.complex64), a, a, b, a)
+    assert grd._nvis(a, a, a) == 5
