@@ -305,6 +305,11 @@ int sde_plan_device_vis(sde_plan *plan, float **vis, float **wt, float **u, floa
  * plan's stream; sde_plan_timings fills up to max entries of (name, total ms,
  * launches) for the runs since the last reset. */
 int sde_plan_profile(sde_plan *plan, int enable);
+/* Restrict the profiling events to the region called name (NULL or "": every
+ * region), so that a timed run carries only the events it needs: each event
+ * pair leaves the stream a gap of some microseconds.  Measurement only; no
+ * counterpart in the reference. */
+int sde_plan_profile_region(sde_plan *plan, const char *name);
 int sde_plan_timings(sde_plan *plan, int max, char (*names)[32], double *ms, int *count);
 int sde_plan_reset_timings(sde_plan *plan);
 /* Bandwidth probe: reps launches of a plain device copy of nbytes on the

@@ -1347,8 +1347,11 @@ struct sde_plan {
     size_t slot_cap = 0;
     unsigned sort_gen = 0;   // bumped by every cell sort
     unsigned slot_gen = 0;   // the sort d_slot (and the bins' idx and counts) belong to
-    // profiling
+    // profiling: events around every region, or around prof_only alone when
+    // it is set (each event pair costs the stream a short gap)
     bool profile = false;
+    std::string prof_only;
+    bool mark_open = false;  // the last plan_mark_begin recorded an event
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
     std::vector<hipEvent_t> pool;
     std::vector<std::string> order;
@@ -1369,14 +1372,17 @@ hipEvent_t pool_event(sde_plan *p) {
 }
 
 void plan_mark_begin(sde_plan *p, const char *name, hipStream_t s = nullptr) {
-    if (!p->profile) return;
+    p->mark_open = false;
+    if (!p->profile || (!p->prof_only.empty() && p->prof_only != name)) return;
     hipEvent_t a = pool_event(p), b = pool_event(p);
     (void)hipEventRecord(a, s ? s : p->stream);
     p->pending.push_back({name, {a, b}});
+    p->mark_open = true;
 }
 void plan_mark_end(sde_plan *p, hipStream_t s = nullptr) {
-    if (!p->profile || p->pending.empty()) return;
+    if (!p->profile || !p->mark_open || p->pending.empty()) return;
     (void)hipEventRecord(p->pending.back().second.second, s ? s : p->stream);
+    p->mark_open = false;
 }
 
 void plan_collect(sde_plan *p) {
@@ -2383,6 +2389,12 @@ int sde_plan_device_vis(sde_plan *p, float **vis, float **wt, float **u, float *
 int sde_plan_profile(sde_plan *p, int enable) {
     if (!p) return -1;
     p->profile = enable != 0;
+    return 0;
+}
+
+int sde_plan_profile_region(sde_plan *p, const char *name) {
+    if (!p) return -1;
+    p->prof_only = name ? name : "";
     return 0;
 }
 

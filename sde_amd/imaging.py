@@ -296,6 +296,10 @@ class Plan:
     def profile(self, on=True):
         self._ok(lib().sde_plan_profile(self._p, 1 if on else 0), "profile")
 
+    def profile_region(self, name=None):
+        """Profile only the region called name (None: every region)."""
+        self._ok(lib().sde_plan_profile_region(self._p, (name or "").encode()), "profile_region")
+
     def timings(self):
         n = 64
         names = (C.c_char * 32 * n)()
