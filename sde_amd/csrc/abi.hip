@@ -1827,7 +1827,7 @@ int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **out) {
     int rc = plan_alloc(p, &p->d_cf, p->ncf) || plan_alloc(p, &p->d_ccf, p->ccflen) || plan_alloc(p, &p->d_grids, 2 * ng) || plan_alloc(p, &p->d_gwork, 2 * ng) ||
              plan_alloc(p, &p->d_mgrid, ng) || plan_alloc(p, &p->d_dirty, npix) || plan_alloc(p, &p->d_psf, npix) ||
              plan_alloc(p, &p->d_img_in, npix) || plan_alloc(p, &p->d_peak, 4) || plan_alloc(p, &p->d_gwt, ng) ||
-             plan_alloc(p, &p->d_wblk, 4096) || plan_alloc(p, &p->d_wsums, 4) || plan_alloc(p, &p->d_cnt, 4) ||
+             plan_alloc(p, &p->d_wblk, kWeightSumBlocks) || plan_alloc(p, &p->d_wsums, 4) || plan_alloc(p, &p->d_cnt, 4) ||
              plan_alloc(p, &p->d_dbins, (size_t)kMaxDegridBins) ||
              plan_alloc(p, &p->d_wsq, kWtSqBlocks) ||
              plan_alloc(p, &p->sb.tile_start, p->tp.ntiles + 1) || plan_alloc(p, &p->sb.item_start, p->tp.ntiles + 1) ||

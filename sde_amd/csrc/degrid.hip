@@ -666,7 +666,6 @@ __global__ __launch_bounds__(1024) void k_degrid_scatter(const float2 *__restric
 }
 
 constexpr int kWtThreads = 256;
-constexpr int kMaxWtBlocks = 4096;
 
 __global__ __launch_bounds__(kWtThreads) void k_wt_density(
     const float *__restrict__ wt, const float *__restrict__ u, const float *__restrict__ v,

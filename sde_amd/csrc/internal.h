@@ -229,6 +229,11 @@ struct DensityOut {
 constexpr int kPrepBlocks = 2048;
 constexpr int kRunBlocks = 2048;
 constexpr int kGatherBlocks = 4096;
+// Per-block partial weight sums: the prep pass's kPrepBlocks and the density
+// runs' kRunBlocks after them, or the caller-order density pass's
+// kMaxWtBlocks; a plan holds room for the larger.
+constexpr int kMaxWtBlocks = 4096;
+constexpr int kWeightSumBlocks = kPrepBlocks + kRunBlocks > kMaxWtBlocks ? kPrepBlocks + kRunBlocks : kMaxWtBlocks;
 int prep_blocks(int64_t n);
 int run_blocks(int64_t n);
 int gather_blocks(int64_t n);
