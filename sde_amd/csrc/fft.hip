@@ -254,56 +254,65 @@ __device__ __forceinline__ int icf(float s, int i1, float cen) {
 // blockIdx.y = field k) or two (planes = 2: blockIdx.y = 2k + 1 transforms
 // grid_b into image_b, no peak); field k's planes lie k*gstride and its
 // images k*istride on, its peak in peak_a[k], the peak it is divided by in
-// div_key[k].
+// div_key[k].  A workgroup takes rpb consecutive image rows, blockDim.x /
+// rpb threads each (short rows share a workgroup, so that more of them are
+// in flight on a CU); a row past the image computes on its own LDS and
+// writes nothing, so every thread reaches every barrier.
 __global__ void k_fft_rows_c2r(const float2 *__restrict__ grid_a, int nu, int nv, int fnu, int nx,
                                int ny, int ibegy, const double2 *__restrict__ tw,
                                const float *__restrict__ rc, float scale, float *__restrict__ image_a,
                                int *__restrict__ peak_a,
                                const int *__restrict__ div_key, const float2 *__restrict__ grid_b,
-                               float *__restrict__ image_b, size_t gstride, size_t istride, int planes) {
-    extern __shared__ double2 zs[];
+                               float *__restrict__ image_b, size_t gstride, size_t istride, int planes, int rpb) {
+    extern __shared__ double2 zsh[];
     __shared__ float red[16];
     const bool second = planes == 2 && (blockIdx.y & 1) != 0;
     const int field = planes == 2 ? blockIdx.y >> 1 : blockIdx.y;
     const float2 *grid = (second ? grid_b : grid_a) + gstride * field;
     float *image = (second ? image_b : image_a) + istride * field;
     int *peak_key = second || !peak_a ? nullptr : peak_a + field;
-    const int iy = blockIdx.x;       // 0-based image row
-    const int rv = iy + ibegy - 1;   // 0-based grid row
     const int h = fnu / 2;
+    const int nthr = blockDim.x / rpb, sub = threadIdx.x / nthr, tid = threadIdx.x - sub * nthr;
+    double2 *zs = zsh + (size_t)sub * lds_len(h);
+    const int iy = blockIdx.x * rpb + sub;  // 0-based image row
+    const bool live = iy < ny;
+    const int rv = iy + ibegy - 1;          // 0-based grid row
     const float2 *row = grid + (size_t)rv * nu;
-    for (int k = threadIdx.x; k < h; k += blockDim.x) {
-        const float2 af = row[k], bf = row[h - k];
-        double2 a = make_double2((double)af.x, (double)af.y), b = make_double2((double)bf.x, (double)bf.y);
-        if (k == 0) {
-            a.y = 0.0;
-            b.y = 0.0;
+    if (live)
+        for (int k = tid; k < h; k += nthr) {
+            const float2 af = row[k], bf = row[h - k];
+            double2 a = make_double2((double)af.x, (double)af.y), b = make_double2((double)bf.x, (double)bf.y);
+            if (k == 0) {
+                a.y = 0.0;
+                b.y = 0.0;
+            }
+            b.y = -b.y;  // conj(X[H-k])
+            const double2 s = cadd(a, b);
+            const double2 d = csub(a, b);
+            const double2 ed = cmul(tw[k], d);
+            zs[lpad(k)] = make_double2(s.x - ed.y, s.y + ed.x);  // s + i*e*d
         }
-        b.y = -b.y;  // conj(X[H-k])
-        const double2 s = cadd(a, b);
-        const double2 d = csub(a, b);
-        const double2 ed = cmul(tw[k], d);
-        zs[lpad(k)] = make_double2(s.x - ed.y, s.y + ed.x);  // s + i*e*d
-    }
     __syncthreads();
-    lds_fft(zs, h, ilog2(h), threadIdx.x, blockDim.x, tw, 2, +1);
+    lds_fft(zs, h, ilog2(h), tid, nthr, tw, 2, +1);
     const int icen = (fnu + 1) / 2, ioff = (fnu - nx) / 2;
     float inv = 1.0f;
     if (div_key) inv = 1.0f / key_float(div_key[field]);
     float mx = -INFINITY;
-    const float ry = rc ? rc[nx + iy] : 1.0f;
-    for (int ix = threadIdx.x; ix < nx; ix += blockDim.x) {
-        int p = ioff + ix + 1 - icen;
-        p += p < 0 ? fnu : 0;
-        const double2 z = zs[lpad(p >> 1)];
-        float val = (float)(((p & 1) ? z.y : z.x) * (double)scale);
-        if (rc) val = val * (rc[ix] * ry);
-        if (div_key) val = inv * val + 0.0f;  // as k_scale_by_peak divides
-        image[(size_t)iy * nx + ix] = val;
-        mx = fmaxf(mx, val);
+    if (live) {
+        const float ry = rc ? rc[nx + iy] : 1.0f;
+        for (int ix = tid; ix < nx; ix += nthr) {
+            int p = ioff + ix + 1 - icen;
+            p += p < 0 ? fnu : 0;
+            const double2 z = zs[lpad(p >> 1)];
+            float val = (float)(((p & 1) ? z.y : z.x) * (double)scale);
+            if (rc) val = val * (rc[ix] * ry);
+            if (div_key) val = inv * val + 0.0f;  // as k_scale_by_peak divides
+            image[(size_t)iy * nx + ix] = val;
+            mx = fmaxf(mx, val);
+        }
     }
     if (peak_key) {
-        // the row's maximum: within each wave, then over the waves
+        // the rows' maximum: within each wave, then over the waves
         for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
         const int nw = (blockDim.x + 63) / 64;
         if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
@@ -316,35 +325,40 @@ __global__ void k_fft_rows_c2r(const float2 *__restrict__ grid_a, int nu, int nv
     }
 }
 
-// Image rows -> grid rows (DIR > 0); rows outside the image are zeroed.
-__global__ void k_fft_rows_r2c(const float *__restrict__ image, int nx, int ny, int fnu, int nu,
+// Image rows -> grid rows (DIR > 0); rows outside the image are zeroed.  A
+// workgroup takes rpb consecutive grid rows, as k_fft_rows_c2r.
+__global__ void k_fft_rows_r2c(const float *__restrict__ image, int nx, int ny, int fnu, int nu, int nv,
                                int ibegy, const double2 *__restrict__ tw,
-                               const float *__restrict__ rc, float2 *__restrict__ grid) {
-    extern __shared__ double2 zs[];
-    const int rv = blockIdx.x;
-    const int iy = rv - (ibegy - 1);
-    float2 *row = grid + (size_t)rv * nu;
+                               const float *__restrict__ rc, float2 *__restrict__ grid, int rpb) {
+    extern __shared__ double2 zsh[];
     const int h = fnu / 2;
-    if (iy < 0 || iy >= ny) {
-        for (int k = threadIdx.x; k < nu; k += blockDim.x) row[k] = make_float2(0.0f, 0.0f);
-        return;
-    }
+    const int nthr = blockDim.x / rpb, sub = threadIdx.x / nthr, tid = threadIdx.x - sub * nthr;
+    double2 *zs = zsh + (size_t)sub * lds_len(h);
+    const int rv = blockIdx.x * rpb + sub;
+    const int iy = rv - (ibegy - 1);
+    const bool live = rv < nv, inimg = live && iy >= 0 && iy < ny;
+    float2 *row = grid + (size_t)rv * nu;
+    if (live && !inimg)
+        for (int k = tid; k < nu; k += nthr) row[k] = make_float2(0.0f, 0.0f);
     // real element p of the FNU-point row is component p & 1 of complex element p >> 1
     double *zr = reinterpret_cast<double *>(zs);
-    for (int k = threadIdx.x; k < lds_len(h); k += blockDim.x) zs[k] = make_double2(0.0, 0.0);
+    for (int k = tid; k < lds_len(h); k += nthr) zs[k] = make_double2(0.0, 0.0);
     __syncthreads();
     const int icen = (fnu + 1) / 2, ioff = (fnu - nx) / 2;
-    const float ry = rc ? rc[nx + iy] : 1.0f;
-    for (int ix = threadIdx.x; ix < nx; ix += blockDim.x) {
-        float val = image[(size_t)iy * nx + ix];
-        if (rc) val = val * (rc[ix] * ry);
-        int p = ioff + ix + 1 - icen;
-        p += p < 0 ? fnu : 0;
-        zr[2 * lpad(p >> 1) + (p & 1)] = (double)val;
+    if (inimg) {
+        const float ry = rc ? rc[nx + iy] : 1.0f;
+        for (int ix = tid; ix < nx; ix += nthr) {
+            float val = image[(size_t)iy * nx + ix];
+            if (rc) val = val * (rc[ix] * ry);
+            int p = ioff + ix + 1 - icen;
+            p += p < 0 ? fnu : 0;
+            zr[2 * lpad(p >> 1) + (p & 1)] = (double)val;
+        }
     }
     __syncthreads();
-    lds_fft(zs, h, ilog2(h), threadIdx.x, blockDim.x, tw, 2, -1);
-    for (int k = threadIdx.x; k <= h; k += blockDim.x) {
+    lds_fft(zs, h, ilog2(h), tid, nthr, tw, 2, -1);
+    if (!inimg) return;  // after the transform's last barrier
+    for (int k = tid; k <= h; k += nthr) {
         double2 x;
         if (k == 0 || k == h) {
             const double2 z0 = zs[0];
@@ -430,11 +444,20 @@ CorrParams make_corr(int nx, int ny, int ccflen, float cenx, float ceny) {
 // one radix-8 butterfly per thread and stage: twice the waves of two a
 // thread, which the LDS latency of each stage needs (measured at FNU = 4096:
 // 0.26 -> 0.18 ms for the row pair)
+// rows per workgroup of the row passes: short rows four (or two) to a
+// workgroup of 256 threads
+int rows_per_block(int h);
+
 int row_threads(int h) {
     int t = h / 8;
     if (t < 64) t = 64;
     if (t > 1024) t = 1024;
     return t;
+}
+
+int rows_per_block(int h) {
+    const int t = row_threads(h);
+    return t >= 256 ? 1 : 256 / t;
 }
 
 // Dynamic LDS above 64 KB needs the kernels' opt-in: done once (for every
@@ -502,9 +525,11 @@ void launch_fft_rows_c2r(hipStream_t s, const float2 *grid, int nu, int nv, int 
     const int h = fnu / 2;
     const int ibegy = (nv + 1) / 2 - (ny + 1) / 2 + 1;
     (void)fft_lds_limit();
-    hipLaunchKernelGGL(k_fft_rows_c2r, dim3(ny), dim3(row_threads(h)), sizeof(double2) * lds_len(h), s, grid,
+    const int rpb = rows_per_block(h);
+    const size_t lds = sizeof(double2) * (size_t)rpb * lds_len(h);
+    hipLaunchKernelGGL(k_fft_rows_c2r, dim3((ny + rpb - 1) / rpb), dim3(rpb * row_threads(h)), lds, s, grid,
                        nu, nv, fnu, nx, ny, ibegy, tw_fnu, rc, scale, image, peak_key,
-                       div_key, (const float2 *)nullptr, (float *)nullptr, (size_t)0, (size_t)0, 1);
+                       div_key, (const float2 *)nullptr, (float *)nullptr, (size_t)0, (size_t)0, 1, rpb);
 }
 
 void launch_fft_rows_c2r_pair(hipStream_t s, const float2 *grid_a, float *image_a, int *peak_a,
@@ -513,13 +538,15 @@ void launch_fft_rows_c2r_pair(hipStream_t s, const float2 *grid_a, float *image_
     const int h = fnu / 2;
     const int ibegy = (nv + 1) / 2 - (ny + 1) / 2 + 1;
     (void)fft_lds_limit();
+    const int rpb = rows_per_block(h);
+    const size_t lds = sizeof(double2) * (size_t)rpb * lds_len(h);
     // grid_a with its peak, then grid_b divided by that peak
-    hipLaunchKernelGGL(k_fft_rows_c2r, dim3(ny), dim3(row_threads(h)), sizeof(double2) * lds_len(h), s, grid_a, nu,
+    hipLaunchKernelGGL(k_fft_rows_c2r, dim3((ny + rpb - 1) / rpb), dim3(rpb * row_threads(h)), lds, s, grid_a, nu,
                        nv, fnu, nx, ny, ibegy, tw_fnu, rc, scale, image_a, peak_a, (const int *)nullptr,
-                       (const float2 *)nullptr, (float *)nullptr, (size_t)0, (size_t)0, 1);
-    hipLaunchKernelGGL(k_fft_rows_c2r, dim3(ny), dim3(row_threads(h)), sizeof(double2) * lds_len(h), s, grid_b, nu,
+                       (const float2 *)nullptr, (float *)nullptr, (size_t)0, (size_t)0, 1, rpb);
+    hipLaunchKernelGGL(k_fft_rows_c2r, dim3((ny + rpb - 1) / rpb), dim3(rpb * row_threads(h)), lds, s, grid_b, nu,
                        nv, fnu, nx, ny, ibegy, tw_fnu, rc, scale, image_b, (int *)nullptr, peak_a,
-                       (const float2 *)nullptr, (float *)nullptr, (size_t)0, (size_t)0, 1);
+                       (const float2 *)nullptr, (float *)nullptr, (size_t)0, (size_t)0, 1, rpb);
 }
 
 void launch_fft_rows_c2r_fields(hipStream_t s, const float2 *grid_psf, const float2 *grid_dirty, size_t gstride,
@@ -530,13 +557,15 @@ void launch_fft_rows_c2r_fields(hipStream_t s, const float2 *grid_psf, const flo
     const int h = fnu / 2;
     const int ibegy = (nv + 1) / 2 - (ny + 1) / 2 + 1;
     (void)fft_lds_limit();
+    const int rpb = rows_per_block(h);
+    const size_t lds = sizeof(double2) * (size_t)rpb * lds_len(h);
     // every PSF with its peak, then every dirty plane divided by its field's peak
-    hipLaunchKernelGGL(k_fft_rows_c2r, dim3(ny, nf), dim3(row_threads(h)), sizeof(double2) * lds_len(h), s,
+    hipLaunchKernelGGL(k_fft_rows_c2r, dim3((ny + rpb - 1) / rpb, nf), dim3(rpb * row_threads(h)), lds, s,
                        grid_psf, nu, nv, fnu, nx, ny, ibegy, tw_fnu, rc, scale, image_psf, peaks,
-                       (const int *)nullptr, (const float2 *)nullptr, (float *)nullptr, gstride, istride, 1);
-    hipLaunchKernelGGL(k_fft_rows_c2r, dim3(ny, nf), dim3(row_threads(h)), sizeof(double2) * lds_len(h), s,
+                       (const int *)nullptr, (const float2 *)nullptr, (float *)nullptr, gstride, istride, 1, rpb);
+    hipLaunchKernelGGL(k_fft_rows_c2r, dim3((ny + rpb - 1) / rpb, nf), dim3(rpb * row_threads(h)), lds, s,
                        grid_dirty, nu, nv, fnu, nx, ny, ibegy, tw_fnu, rc, scale, image_dirty, (int *)nullptr, peaks,
-                       (const float2 *)nullptr, (float *)nullptr, gstride, istride, 1);
+                       (const float2 *)nullptr, (float *)nullptr, gstride, istride, 1, rpb);
 }
 
 void launch_fft_rows_r2c(hipStream_t s, const float *image, int nx, int ny, int fnu, int nv,
@@ -544,8 +573,10 @@ void launch_fft_rows_r2c(hipStream_t s, const float *image, int nx, int ny, int 
     const int h = fnu / 2, nu = h + 1;
     const int ibegy = (nv + 1) / 2 - (ny + 1) / 2 + 1;
     (void)fft_lds_limit();
-    hipLaunchKernelGGL(k_fft_rows_r2c, dim3(nv), dim3(row_threads(h)), sizeof(double2) * lds_len(h), s, image,
-                       nx, ny, fnu, nu, ibegy, tw_fnu, rc, grid);
+    const int rpb = rows_per_block(h);
+    const size_t lds = sizeof(double2) * (size_t)rpb * lds_len(h);
+    hipLaunchKernelGGL(k_fft_rows_r2c, dim3((nv + rpb - 1) / rpb), dim3(rpb * row_threads(h)), lds, s, image,
+                       nx, ny, fnu, nu, nv, ibegy, tw_fnu, rc, grid, rpb);
 }
 
 void launch_scale_by_peak(hipStream_t s, float *image, size_t n, const int *peak_key, float *image_b) {
