@@ -245,6 +245,14 @@ int sde_plan_vismodel(sde_plan *plan, const float *image);
 /* The same from an NX*NY image already in device memory (another plan's
  * image buffer or the caller's); nothing is copied. */
 int sde_plan_vismodel_device(sde_plan *plan, const float *device_image);
+/* Predict the plan's visibilities from the model grid of another plan of the
+ * same image size, cell and kernel (its last sde_plan_vismodel*): several
+ * pointings or channels of one model share one correction and transform,
+ * and only GRDDEH2D (grd/grddeh2d.f:7-204) runs here, on this plan's stream
+ * once the source's transform is complete.  The source's model must not
+ * change until this prediction has completed.  No counterpart in the
+ * reference, whose VISMODEL transforms the model on every call. */
+int sde_plan_vismodel_grid(sde_plan *plan, const sde_plan *source);
 /* Load an NX*NY model image into the plan's image buffer (host or device
  * source; synchronous) and return that buffer's device address, so several
  * plans can predict from one uploaded image. */

@@ -77,14 +77,17 @@ def uvmap_sequential(plans):
 
 
 def vismodel_batched(plans, image):
-    """Every plan (same image size) predicts from one model image, uploaded
-    once and read by all from device memory; issued back to back, then all
-    awaited.  Seconds, the upload included."""
+    """Every plan (same image size, cell and kernel) predicts from one model
+    image, uploaded once: the first plan corrects and transforms it into its
+    model grid and every plan degrids from that grid on its own stream
+    (sde_plan_vismodel_grid); issued back to back, then all awaited.
+    Seconds, the upload included."""
     t0 = time.perf_counter()
     plans[0].set_image(image)
     src = plans[0].device_image()
-    for p in plans:
-        p.vismodel_device(src)
+    plans[0].vismodel_device(src)
+    for p in plans[1:]:
+        p.vismodel_from(plans[0])
     for p in plans:
         p.sync()
     return time.perf_counter() - t0

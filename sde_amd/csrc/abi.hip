@@ -1301,6 +1301,7 @@ struct sde_plan {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;  // independent work beside the main sequence (joined into it)
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_model = nullptr;  // recorded when the model grid d_mgrid is complete
     int fnu = 0, nu = 0, nv = 0, uorigin = 1, vorigin = 1, ccflen = 0, supp = 0, osamp = 0, ncf = 0;
     float udelt = 0, vdelt = 0, uscale = 0, vscale = 0;
     CoordParams cp, wcp;
@@ -1512,6 +1513,10 @@ void plan_set_geometry(sde_plan *p, float dx, float dy) {
     p->wcp = make_cp(cfg->shift, wus, 0.0f, wvs, 0.0f, p->uorigin, p->vorigin, p->nu, p->nv, 0, 0, 1, 1);
 }
 
+int plan_degrid(sde_plan *p, const float2 *mgrid);
+
+// VISMODEL (task/vismodel.f:69-78): the model image corrected and transformed
+// into the plan's model grid (IMGGRIDC 'CORRECT', IMGFFT), then degridded.
 int plan_vismodel(sde_plan *p, const float *img) {
     hipStream_t s = p->stream;
     double2 *twnv = p->tw[p->nv], *twfnu = p->tw[p->fnu];
@@ -1521,6 +1526,14 @@ int plan_vismodel(sde_plan *p, const float *img) {
     plan_mark_begin(p, "fft_cols_model");
     launch_fft_cols(s, p->d_mgrid, p->d_mgrid, p->nu, p->nv, 1, 0, +1, 1.0f, twnv);
     plan_mark_end(p);
+    SDE_CHECK(hipEventRecord(p->ev_model, s));
+    return plan_degrid(p, p->d_mgrid);
+}
+
+// GRDDEH2D of the plan's visibilities from a model grid of the plan's
+// geometry (its own, or another plan's on the same grid).
+int plan_degrid(sde_plan *p, const float2 *mgrid) {
+    hipStream_t s = p->stream;
     if (p->nvis <= 0) return 0;
     SDE_CHECK(hipMemsetAsync(p->d_cnt + 1, 0, sizeof(unsigned long long), s));
     if (!p->filled) {
@@ -1534,7 +1547,7 @@ int plan_vismodel(sde_plan *p, const float *img) {
     SDE_CHECK(hipEventRecord(p->ev_fork, s));
     SDE_CHECK(hipStreamWaitEvent(p->side, p->ev_fork, 0));
     plan_mark_begin(p, "degrid_axis", p->side);
-    launch_degrid_list(p->side, p->d_mgrid, p->d_u, p->d_v, p->d_w, p->d_wt, p->d_vis, p->nvis, p->cp, p->d_cf,
+    launch_degrid_list(p->side, mgrid, p->d_u, p->d_v, p->d_w, p->d_wt, p->d_vis, p->nvis, p->cp, p->d_cf,
                        p->d_cf, p->d_mvis, p->d_mwt, p->d_cnt + 1, p->d_list, p->d_nlist);
     plan_mark_end(p, p->side);
     SDE_CHECK(hipEventRecord(p->ev_join, p->side));
@@ -1546,7 +1559,7 @@ int plan_vismodel(sde_plan *p, const float *img) {
     bins.reuse = p->slot_gen != 0 && p->slot_gen == p->sort_gen;
     plan_mark_begin(p, "degrid");
     if (!bins.reuse) SDE_CHECK(hipMemsetAsync(p->d_dbins, 0, sizeof(int) * (size_t)bins.nbins, s));
-    const int rc = launch_degrid_tiles(s, p->d_recd, p->sb.items, p->sb.nitems, p->sb.max_items, p->tp, p->d_mgrid,
+    const int rc = launch_degrid_tiles(s, p->d_recd, p->sb.items, p->sb.nitems, p->sb.max_items, p->tp, mgrid,
                                        p->d_cf, p->d_cf, p->cp, p->d_swt, p->d_mvis, p->d_mwt, p->d_cnt + 1,
                                        &bins);
     plan_mark_end(p);
@@ -1816,7 +1829,8 @@ int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **out) {
     if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&p->ev_model, hipEventDisableTiming) != hipSuccess) {
         set_plan_error("sde_plan_create: stream creation failed");
         delete p;
         return -1;
@@ -1877,6 +1891,7 @@ int sde_plan_destroy(sde_plan *p) {
     if (p->side) (void)hipStreamSynchronize(p->side);
     if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
     if (p->ev_join) (void)hipEventDestroy(p->ev_join);
+    if (p->ev_model) (void)hipEventDestroy(p->ev_model);
     if (p->side) (void)hipStreamDestroy(p->side);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
@@ -1953,6 +1968,20 @@ int sde_plan_vismodel(sde_plan *p, const float *image) {
 int sde_plan_vismodel_device(sde_plan *p, const float *device_image) {
     if (!p || !device_image) return -1;
     return plan_vismodel(p, device_image);
+}
+
+int sde_plan_vismodel_grid(sde_plan *p, const sde_plan *src) {
+    if (!p || !src) return -1;
+    if (src->nu != p->nu || src->nv != p->nv || src->fnu != p->fnu || src->cfg.nx != p->cfg.nx ||
+        src->cfg.ny != p->cfg.ny || src->cfg.cellsize != p->cfg.cellsize || src->ncf != p->ncf ||
+        strncmp(src->cfg.cftype, p->cfg.cftype, sizeof p->cfg.cftype) != 0) {
+        set_plan_error("sde_plan_vismodel_grid: the source plan's grid is not this plan's (image size, cell and "
+                       "kernel must agree)");
+        return -1;
+    }
+    // the source's last model grid, once its transform has run
+    SDE_CHECK(hipStreamWaitEvent(p->stream, src->ev_model, 0));
+    return plan_degrid(p, src->d_mgrid);
 }
 
 int sde_plan_set_image(sde_plan *p, const float *image, int on_device) {

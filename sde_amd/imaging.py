@@ -228,6 +228,11 @@ class Plan:
         from device_image() of this or another plan of the same image size)."""
         self._ok(lib().sde_plan_vismodel_device(self._p, C.c_void_p(device_image)), "sde_plan_vismodel_device")
 
+    def vismodel_from(self, source):
+        """Predict from source's model grid (a plan of the same image size,
+        cell and kernel that has predicted): one transform serves both."""
+        self._ok(lib().sde_plan_vismodel_grid(self._p, source._p), "sde_plan_vismodel_grid")
+
     def sync(self):
         self._ok(lib().sde_plan_sync(self._p), "sde_plan_sync")
 
