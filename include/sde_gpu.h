@@ -249,8 +249,10 @@ int sde_plan_vismodel_device(sde_plan *plan, const float *device_image);
  * same image size, cell and kernel (its last sde_plan_vismodel*): several
  * pointings or channels of one model share one correction and transform,
  * and only GRDDEH2D (grd/grddeh2d.f:7-204) runs here, on this plan's stream
- * once the source's transform is complete.  The source's model must not
- * change until this prediction has completed.  No counterpart in the
+ * once the source's transform is complete.  Fails when the source has not
+ * predicted since its creation or its last cell change, or lives on
+ * another device.  The source's model must not change until this
+ * prediction has completed.  No counterpart in the
  * reference, whose VISMODEL transforms the model on every call. */
 int sde_plan_vismodel_grid(sde_plan *plan, const sde_plan *source);
 /* Load an NX*NY model image into the plan's image buffer (host or device

@@ -1302,6 +1302,7 @@ struct sde_plan {
     hipStream_t side = nullptr;  // independent work beside the main sequence (joined into it)
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipEvent_t ev_model = nullptr;  // recorded when the model grid d_mgrid is complete
+    int dev = 0;                     // the device current at creation, which holds the plan's buffers
     int fnu = 0, nu = 0, nv = 0, uorigin = 1, vorigin = 1, ccflen = 0, supp = 0, osamp = 0, ncf = 0;
     float udelt = 0, vdelt = 0, uscale = 0, vscale = 0;
     CoordParams cp, wcp;
@@ -1328,6 +1329,7 @@ struct sde_plan {
     // the last uvmap stand for the resident uvw and weights (residual imaging)
     bool sorted_ok = false;
     bool predicted = false;  // model VIS/WT of the resident visibilities are complete
+    bool model_ready = false;  // d_mgrid holds the transform of a model image of the current geometry
     bool imaged = false;     // the images hold a uvmap's output (sums valid)
     DegridRec *d_recd = nullptr;
     float *d_swt = nullptr;  // GRDDEH2D's SUMWT per kernel phase pair (launch_degrid_sumwt)
@@ -1527,6 +1529,7 @@ int plan_vismodel(sde_plan *p, const float *img) {
     launch_fft_cols(s, p->d_mgrid, p->d_mgrid, p->nu, p->nv, 1, 0, +1, 1.0f, twnv);
     plan_mark_end(p);
     SDE_CHECK(hipEventRecord(p->ev_model, s));
+    p->model_ready = true;
     return plan_degrid(p, p->d_mgrid);
 }
 
@@ -1826,7 +1829,8 @@ int sde_plan_create(const sde_imaging_cfg *cfg, sde_plan **out) {
         return -1;
     }
     p->tp = make_tp(p->nu, p->nv, T, cfg->chunk > 0 ? cfg->chunk : 2048, cfg->waves ? cfg->waves : 4);
-    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess ||
+    if (hipGetDevice(&p->dev) != hipSuccess ||
+        hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming) != hipSuccess ||
@@ -1972,6 +1976,12 @@ int sde_plan_vismodel_device(sde_plan *p, const float *device_image) {
 
 int sde_plan_vismodel_grid(sde_plan *p, const sde_plan *src) {
     if (!p || !src) return -1;
+    if (!src->model_ready || src->dev != p->dev) {
+        set_plan_error(!src->model_ready ? "sde_plan_vismodel_grid: the source plan holds no transformed model of "
+                                           "its current geometry (run a prediction on it first)"
+                                         : "sde_plan_vismodel_grid: the source plan is on another device");
+        return -1;
+    }
     if (src->nu != p->nu || src->nv != p->nv || src->fnu != p->fnu || src->cfg.nx != p->cfg.nx ||
         src->cfg.ny != p->cfg.ny || src->cfg.cellsize != p->cfg.cellsize || src->ncf != p->ncf ||
         strncmp(src->cfg.cftype, p->cfg.cftype, sizeof p->cfg.cftype) != 0) {
@@ -2006,6 +2016,7 @@ int sde_plan_set_cell(sde_plan *p, float cellsize) {
     }
     p->cfg.cellsize = cellsize;
     plan_set_geometry(p, -cellsize / 3600.0f, cellsize / 3600.0f);
+    p->model_ready = false;
     p->filled = false;
     p->weighted = false;
     p->sorted_ok = false;

@@ -130,6 +130,28 @@ def test_predict_batch_equals_alone_oracle_and_dft():
     assert floor < 0.05
 
 
+def test_vismodel_from_needs_a_transformed_source():
+    img, cell, ptgs, _ = synth.predict_batch(npoint=2, nchan=1, ntime=20, nx=128, ncomp=20, seed=4)
+    jobs = [dict(nx=128, cellsize=cell, u=p["u"], v=p["v"], w=p["w"], wt=p["wt"]) for p in ptgs]
+    plans = batch.make_plans(jobs)
+    try:
+        with pytest.raises(sde.SdeError, match="no transformed model"):
+            plans[1].vismodel_from(plans[0])
+        plans[0].vismodel(img)
+        plans[1].vismodel_from(plans[0])
+        plans[1].sync()
+        shared = plans[1].model()
+        plans[1].vismodel(img)
+        np.testing.assert_array_equal(shared[0], plans[1].model()[0])
+        # a new cell invalidates the source's model grid
+        plans[0].set_cell(cell * 1.5)
+        with pytest.raises(sde.SdeError, match="no transformed model"):
+            plans[1].vismodel_from(plans[0])
+    finally:
+        for p in plans:
+            p.close()
+
+
 def _small_requests(n=10, seed=8):
     reqs = synth.served_requests(n, rate_hz=2000.0, seed=seed, sizes=(128, 256, 512), median=3e4, sigma=0.7,
                                  nvis_max=200_000)
