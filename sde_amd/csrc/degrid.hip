@@ -505,8 +505,10 @@ __global__ __launch_bounds__(256) void k_degrid_list(
 // is reserved with a global atomic, and each result goes to its bin's run
 // (a contiguous stretch of the bin); k_degrid_scatter then puts them in
 // input order.
+constexpr int kDegridThreads = 256;
+static_assert(kMaxDegridBins % kDegridThreads == 0, "bins per thread");
 template <int T, int S, bool BIN, bool R8>
-__global__ __launch_bounds__(256) void k_degrid_tiles(const DegridRec *__restrict__ rec,
+__global__ __launch_bounds__(kDegridThreads) void k_degrid_tiles(const DegridRec *__restrict__ rec,
                                                       const WorkItem *__restrict__ items,
                                                       const int *__restrict__ nitems, TileParams tp,
                                                       const float2 *__restrict__ grid,
@@ -518,42 +520,92 @@ __global__ __launch_bounds__(256) void k_degrid_tiles(const DegridRec *__restric
     const DegridRec8 *__restrict__ rec8 = reinterpret_cast<const DegridRec8 *>(rec);
     constexpr int PL = T + 2 * kHaloD;
     constexpr int NB = BIN ? kMaxDegridBins : 1;
+    constexpr int NT = kDegridThreads;
     __shared__ float2 g[PL * PL];
     __shared__ int bcnt[NB], bbase[NB];
     extern __shared__ float cfl[];
     const int item = blockIdx.x;
     if (item >= *nitems) return;
-    const WorkItem wi = items[item];
-    if (BIN && !bins.reuse) {
-        for (int k = threadIdx.x; k < bins.nbins; k += blockDim.x) bcnt[k] = 0;
-        __syncthreads();
-        for (int r = wi.begin + threadIdx.x; r < wi.end; r += blockDim.x) {
-            const uint32_t id = R8 ? rec8[r].idx : rec[r].idx;
-            if (id != kNoDegrid) atomicAdd(&bcnt[(id & 0x7fffffffu) >> bins.shift], 1);
-        }
-        __syncthreads();
-        // one run per bin; the counters restart as the ranks within the runs
-        for (int k = threadIdx.x; k < bins.nbins; k += blockDim.x) {
-            const int c = bcnt[k];
-            if (c) bbase[k] = (k << bins.shift) + atomicAdd(&bins.count[k], c);
-            bcnt[k] = 0;
-        }
-        // the barrier after the tile staging orders these before the main loop
-    }
-    const int tu = wi.tile % tp.ntu, tv = wi.tile / tp.ntu;
-    const int col0 = tu * T - S - kHaloD, row0 = tv * T - kHaloD;
-    for (int k = threadIdx.x; k < PL * PL; k += blockDim.x) {
-        const int r = k / PL, c = k - r * PL;
-        const int gc = col0 + c, gr = row0 + r;
-        g[k] = (gc >= 0 && gc < cp.nu && gr >= 0 && gr < cp.nv) ? grid[(size_t)gr * cp.nu + gc]
-                                                               : make_float2(0.0f, 0.0f);
-    }
+    // The prologue is a few rounds of independent loads (or atomics), each
+    // issued whole before any result is used: the workgroup waits on one
+    // memory latency per round rather than one per loop trip.
     const int poffu = (cp.osampu + 1) / 2, poffv = (cp.osampv + 1) / 2;
     const int ncfu = cp.osampu * (2 * S + 1) + 2;
     const int ncfv = cp.osampv * (2 * S + 1) + 2;
     float *cflu = cfl, *cflv = cfl + ncfu;
-    for (int k = threadIdx.x; k < ncfu; k += blockDim.x) cflu[k] = cfu_g[cp.osampu - poffu + k];
-    for (int k = threadIdx.x; k < ncfv; k += blockDim.x) cflv[k] = cfv_g[cp.osampv - poffv + k];
+    constexpr int CFU = 4;  // kernel-table entries per thread and round
+    float cfr[2 * CFU];
+    if (ncfu <= CFU * NT && ncfv <= CFU * NT) {
+#pragma unroll
+        for (int q = 0; q < CFU; q++) {
+            const int k = q * NT + threadIdx.x;
+            cfr[q] = k < ncfu ? cfu_g[cp.osampu - poffu + k] : 0.0f;
+            cfr[CFU + q] = k < ncfv ? cfv_g[cp.osampv - poffv + k] : 0.0f;
+        }
+    } else {
+        for (int k = threadIdx.x; k < ncfu; k += NT) cflu[k] = cfu_g[cp.osampu - poffu + k];
+        for (int k = threadIdx.x; k < ncfv; k += NT) cflv[k] = cfv_g[cp.osampv - poffv + k];
+    }
+    const WorkItem wi = items[item];
+    const int tu = wi.tile % tp.ntu, tv = wi.tile / tp.ntu;
+    const int col0 = tu * T - S - kHaloD, row0 = tv * T - kHaloD;
+    constexpr int GQ = (PL * PL + NT - 1) / NT;
+    float2 gr_[GQ];
+#pragma unroll
+    for (int q = 0; q < GQ; q++) {
+        const int k = q * NT + threadIdx.x;
+        const int r = k / PL, c = k - r * PL;
+        const int gc = col0 + c, gr = row0 + r;
+        gr_[q] = (k < PL * PL && gc >= 0 && gc < cp.nu && gr >= 0 && gr < cp.nv) ? grid[(size_t)gr * cp.nu + gc]
+                                                                                : make_float2(0.0f, 0.0f);
+    }
+    if (BIN && !bins.reuse) {
+        constexpr int BQ = BIN ? NB / NT : 1;
+#pragma unroll
+        for (int q = 0; q < BQ; q++) bcnt[q * NT + threadIdx.x] = 0;
+        __syncthreads();
+        constexpr int RQ = 8;  // records per thread and round of the count
+        for (int r0 = wi.begin; r0 < wi.end; r0 += RQ * NT) {
+            uint32_t id[RQ];
+#pragma unroll
+            for (int q = 0; q < RQ; q++) {
+                const int r = r0 + q * NT + threadIdx.x;
+                id[q] = r < wi.end ? (R8 ? rec8[r].idx : rec[r].idx) : kNoDegrid;
+            }
+#pragma unroll
+            for (int q = 0; q < RQ; q++)
+                if (id[q] != kNoDegrid) atomicAdd(&bcnt[(id[q] & 0x7fffffffu) >> bins.shift], 1);
+        }
+        __syncthreads();
+        // one run per bin; the counters restart as the ranks within the runs
+        int cnt[BQ], at[BQ];
+#pragma unroll
+        for (int q = 0; q < BQ; q++) {
+            const int k = q * NT + threadIdx.x;
+            cnt[q] = k < bins.nbins ? bcnt[k] : 0;
+            at[q] = cnt[q] ? atomicAdd(&bins.count[k], cnt[q]) : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < BQ; q++) {
+            const int k = q * NT + threadIdx.x;
+            if (cnt[q]) bbase[k] = (k << bins.shift) + at[q];
+            bcnt[k] = 0;
+        }
+        // the barrier after the tile staging orders these before the main loop
+    }
+#pragma unroll
+    for (int q = 0; q < GQ; q++) {
+        const int k = q * NT + threadIdx.x;
+        if (k < PL * PL) g[k] = gr_[q];
+    }
+    if (ncfu <= CFU * NT && ncfv <= CFU * NT) {
+#pragma unroll
+        for (int q = 0; q < CFU; q++) {
+            const int k = q * NT + threadIdx.x;
+            if (k < ncfu) cflu[k] = cfr[q];
+            if (k < ncfv) cflv[k] = cfr[CFU + q];
+        }
+    }
     __syncthreads();
     // each record is loaded one step ahead of its gather
     DegridRec dn;
@@ -892,8 +944,8 @@ static void launch_degrid_tiles_t(hipStream_t s, const DegridRec *rec, const Wor
     const size_t lds = sizeof(float) * (ncfu + ncfv);
     const bool r8 = cp.doshift == 0;
 #define SDE_DT(BINV, R8V, B)                                                                                     \
-    hipLaunchKernelGGL((k_degrid_tiles<T, S, BINV, R8V>), dim3(max_items), dim3(256), lds, s, rec, items, nitems, \
-                       tp, grid, cfu, cfv, cp, swt, vis_out, wt_out, nbad, B)
+    hipLaunchKernelGGL((k_degrid_tiles<T, S, BINV, R8V>), dim3(max_items), dim3(kDegridThreads), lds, s, rec,    \
+                       items, nitems, tp, grid, cfu, cfv, cp, swt, vis_out, wt_out, nbad, B)
     if (!bins) {
         if (r8) SDE_DT(false, true, DegridBins{});
         else SDE_DT(false, false, DegridBins{});
