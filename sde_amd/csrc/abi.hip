@@ -401,8 +401,7 @@ using MarkE = std::function<void()>;
 // outputs), the radix sort, tile ranges and work items.
 int run_sort(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v, const float *w,
              int64_t nvis, const CoordParams &cp, const TileParams &tp, SortBufs &sb, const DensityOut *dens,
-             const DegridOut *deg, const MarkB &mb, const MarkE &me, bool bounds = true,
-             const SelParams *sel = nullptr) {
+             const DegridOut *deg, const MarkB &mb, const MarkE &me, const SelParams *sel = nullptr) {
     // every point needs its cold staging record when the weights need the
     // density cell (density scale not the gridding scale) or the AZ-NORM
     // factor, or the prediction needs the shift rotation
@@ -411,7 +410,7 @@ int run_sort(hipStream_t s, const float2 *vis, const float *wt, const float *u, 
     launch_prep(s, vis, wt, u, v, w, nvis, cp, tp, sb, dens, deg, sel);
     me();
     mb("cell_sort");
-    const int rc = launch_sort(s, sb, nvis, tp, bounds);
+    const int rc = launch_sort(s, sb, nvis, tp);
     me();
     if (rc) {
         set_plan_error("cell sort failed");
@@ -537,7 +536,7 @@ int slowz_grid(Ctx &c, const float2 *dvis, const float *dwt, const float *du, co
         return -1;
     }
     launch_prep_slowz(s, dvis, dwt, du, dv, dw, n, cp, tp, psf, pp, sb, pblk, dsw);
-    if (launch_sort(s, sb, n, tp, true)) {
+    if (launch_sort(s, sb, n, tp)) {
         set_plan_error("cell sort failed");
         return -1;
     }
@@ -815,7 +814,7 @@ void grdqh2d_(float *vis, float *wt, float *u, float *v, float *w, int *nvis, fl
         SDE_F(hipMemcpyAsync(dw, w, sizeof(float) * n, hipMemcpyHostToDevice, s));
     }
     launch_prep_box(s, dvis, dwt, du, dv, dw, n, cp, tp, sb, pblk);
-    if (launch_sort(s, sb, n, tp, true) || launch_box_tiles(s, sb, tp, chmask, blk, partial)) {
+    if (launch_sort(s, sb, n, tp) || launch_box_tiles(s, sb, tp, chmask, blk, partial)) {
         report_error(kErrFatal, R, "cell sort failed");
         return;
     }
@@ -1470,7 +1469,7 @@ int plan_reserve_items(sde_plan *p) {
 
 // Cell sort of the resident visibilities; the prep pass also writes the
 // per-point degridding outputs into the model arrays.
-int plan_sort(sde_plan *p, const DensityOut *dens, bool bounds = true, const SelParams *sel = nullptr) {
+int plan_sort(sde_plan *p, const DensityOut *dens, const SelParams *sel = nullptr) {
     p->sorted_ok = false;
     p->sort_gen++;
     auto mb = [p](const char *nm) { plan_mark_begin(p, nm); };
@@ -1482,7 +1481,7 @@ int plan_sort(sde_plan *p, const DensityOut *dens, bool bounds = true, const Sel
     deg.nlist = p->d_nlist;
     p->filled = false;
     return run_sort(p->stream, p->d_vis, p->d_wt, p->d_u, p->d_v, p->d_w, p->nvis, p->cp, p->tp, p->sb, dens, &deg,
-                    mb, me, bounds, sel);
+                    mb, me, sel);
 }
 
 // The selection of the uvmap task (task/uvmap.f:158-168): VISSEL when the
@@ -1634,7 +1633,7 @@ int plan_uvmap_box(sde_plan *p) {
     launch_prep_box(s, p->d_vis, wt, p->d_u, p->d_v, p->d_w, n, p->cp, p->tp, p->sb, p->d_wblk);
     plan_mark_end(p);
     plan_mark_begin(p, "cell_sort");
-    const int rc = launch_sort(s, p->sb, n, p->tp, true);
+    const int rc = launch_sort(s, p->sb, n, p->tp);
     plan_mark_end(p);
     plan_mark_begin(p, "grid_tiles");
     const int rc2 = rc ? rc : launch_box_tiles(s, p->sb, p->tp, 3, p->d_isums, p->d_partial);
@@ -1700,15 +1699,13 @@ int plan_uvmap(sde_plan *p, bool reuse) {
         dens.blk = p->d_wblk;
         dens.ndrop = p->d_cnt;
     }
-    // with the density from the sorted runs, that pass also writes the tile ranges
-    if (plan_sort(p, weighting ? &dens : nullptr, !runs, p->selected ? &sp : nullptr)) return -1;
+    if (plan_sort(p, weighting ? &dens : nullptr, p->selected ? &sp : nullptr)) return -1;
     if (weighting) {
         int nwblk;
         if (runs) {
             const int np = prep_blocks(n);
             plan_mark_begin(p, "wt_density");
             launch_density_runs(s, p->sb, n, p->tp, p->cp, p->d_gwt, p->d_wblk + np);
-            launch_items(s, p->sb, p->tp);
             plan_mark_end(p);
             nwblk = np + run_blocks(n);
         } else {
@@ -2810,7 +2807,7 @@ int batch_issue(sde_batch *b) {
     hipStream_t s = b->stream;
     launch_prep_fields(s, b->d_vis, b->d_wt, b->d_u, b->d_v, b->d_w, b->d_fields, b->nf, b->nblocks, b->tp.T,
                        (uint32_t)sort_nkeys(b->tp), b->sb);
-    if (launch_sort(s, b->sb, b->n, b->tp, true)) {
+    if (launch_sort(s, b->sb, b->n, b->tp)) {
         set_plan_error("sde_batch_uvmap: cell sort failed");
         return -1;
     }

@@ -436,7 +436,7 @@ __global__ __launch_bounds__(1024) void k_items_write(const int *__restrict__ ti
 // Density grid from the sorted keys and weights (points whose density cell
 // is their gridding cell): a segmented sum over the runs of equal keys, one
 // FP64 atomic per run (per wave chunk).  Flagged points were added by the
-// prep pass.  The same pass writes the tile ranges (as k_tile_bounds).
+// prep pass.  The tile ranges come from k_tile_bounds.
 //
 // Each lane takes 4 consecutive slots (one 16-byte key load, two 16-byte
 // value loads): runs inside a lane are summed there; the run open at a
@@ -454,10 +454,8 @@ __device__ __forceinline__ void density_add(uint32_t key, double x, uint32_t nke
 
 __global__ __launch_bounds__(kThreads) void k_density_runs(
     const uint32_t *__restrict__ skeys, const uint64_t *__restrict__ svals, int64_t n, uint32_t nkeys,
-    TileParams tp, int nu, int suppu, double *__restrict__ gwt, double *__restrict__ blk,
-    int *__restrict__ tile_start) {
+    TileParams tp, int nu, int suppu, double *__restrict__ gwt, double *__restrict__ blk) {
     const int lane = threadIdx.x & 63;
-    const int tt = tp.T * tp.T;
     double dsum = 0.0, unused = 0.0;
     const int64_t stride = (int64_t)gridDim.x * kThreads * 4;
     // the next iteration's slots are loaded while this one's are summed
@@ -502,20 +500,6 @@ __global__ __launch_bounds__(kThreads) void k_density_runs(
         // key of the slot before this lane's first
         uint32_t prev = (uint32_t)__shfl_up((int)kk[3], 1);
         if (lane == 0) prev = (q0 > 0 && q0 - 1 < n) ? (skeys[q0 - 1] & ~kKeyFlag) : 0xffffffffu;
-        // tile ranges
-        uint32_t kp = q0 == 0 ? 0xffffffffu : prev;
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int64_t p = q0 + e;
-            if (p < n) {
-                const int t = tile_of_key(kk[e], nkeys, tt, tp.ntiles);
-                const int tprev = p == 0 ? -1 : tile_of_key(kp, nkeys, tt, tp.ntiles);
-                for (int q = tprev + 1; q <= t; q++) tile_start[q] = (int)p;
-                if (p == n - 1)
-                    for (int q = t + 1; q <= tp.ntiles; q++) tile_start[q] = (int)n;
-            }
-            kp = kk[e];
-        }
         // runs inside the lane
         double first = 0.0;
         int e0 = 0;
@@ -1612,19 +1596,18 @@ void launch_prep(hipStream_t s, const float2 *vis, const float *wt, const float 
 #undef SDE_PREP
 }
 
-int launch_sort(hipStream_t s, SortBufs &b, int64_t n, const TileParams &tp, bool bounds) {
+int launch_sort(hipStream_t s, SortBufs &b, int64_t n, const TileParams &tp) {
     if (n > 0) {
         if (launch_radix_sort(s, b.temp, b.temp_bytes, b.keys, b.skeys, b.vals, b.svals, n, sort_nbits(tp),
                               b.vals_wt, b.counted0) !=
             hipSuccess)
             return -1;
-        if (bounds)
-            hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)((tp.ntiles + kThreads) / kThreads)), dim3(kThreads), 0,
-                               s, b.skeys, n, (uint32_t)sort_nkeys(tp), tp.T * tp.T, tp.ntiles, b.tile_start);
+        hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)((tp.ntiles + kThreads) / kThreads)), dim3(kThreads), 0, s,
+                           b.skeys, n, (uint32_t)sort_nkeys(tp), tp.T * tp.T, tp.ntiles, b.tile_start);
     } else if (hipMemsetAsync(b.tile_start, 0, sizeof(int) * (tp.ntiles + 1), s) != hipSuccess) {
         return -1;
     }
-    if (bounds || n <= 0) launch_items(s, b, tp);
+    launch_items(s, b, tp);
     return 0;
 }
 
@@ -1640,7 +1623,7 @@ void launch_items(hipStream_t s, SortBufs &b, const TileParams &tp) {
 void launch_density_runs(hipStream_t s, const SortBufs &b, int64_t n, const TileParams &tp,
                          const CoordParams &cp, double *gwt, double *blk) {
     hipLaunchKernelGGL(k_density_runs, dim3(run_blocks(n)), dim3(kThreads), 0, s, b.skeys, b.svals, n,
-                       (uint32_t)sort_nkeys(tp), tp, cp.nu, cp.suppu, gwt, blk, b.tile_start);
+                       (uint32_t)sort_nkeys(tp), tp, cp.nu, cp.suppu, gwt, blk);
 }
 
 void launch_gather_probe(hipStream_t s, const SortBufs &b, int64_t n, float *blk) {

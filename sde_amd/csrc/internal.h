@@ -267,12 +267,11 @@ void launch_visselpi(hipStream_t s, const SelParams &sp, const float *u, const f
 void launch_prep(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,
                  const float *w, int64_t n, const CoordParams &cp, const TileParams &tp, SortBufs &b,
                  const DensityOut *dens, const DegridOut *deg, const SelParams *sel = nullptr);
-// Radix sort; with bounds also the tile ranges and work items (else the
-// density runs write the ranges and launch_items follows them).
-int launch_sort(hipStream_t s, SortBufs &b, int64_t n, const TileParams &tp, bool bounds = true);
+// Radix sort, then the tile ranges and the work items.
+int launch_sort(hipStream_t s, SortBufs &b, int64_t n, const TileParams &tp);
 void launch_items(hipStream_t s, SortBufs &b, const TileParams &tp);
-// Density runs over the sorted keys and weights, with the tile ranges:
-// partial sums to blk[0 .. run_blocks).
+// Density runs over the sorted keys and weights: partial sums to
+// blk[0 .. run_blocks).
 void launch_density_runs(hipStream_t s, const SortBufs &b, int64_t n, const TileParams &tp,
                          const CoordParams &cp, double *gwt, double *blk);
 
