@@ -236,7 +236,9 @@ int sde_plan_uvmap(sde_plan *plan);
  * visibilities by VIS - MODEL wherever the last sde_plan_vismodel predicted
  * them (model WT > 0); sde_plan_uvmap_residual images them on the standing
  * cell sort and density weights of the last sde_plan_uvmap (uvw and weights
- * unchanged), restaging only the visibility values. */
+ * unchanged), restaging only the visibility values.  On a standing sort the
+ * subtraction restages them in its own pass; a residual uvmap after
+ * sde_plan_device_vis restages them again (the caller may have written them). */
 int sde_plan_subtract_model(sde_plan *plan);
 int sde_plan_uvmap_residual(sde_plan *plan);
 /* Model prediction from an image (host NX*NY floats, or NULL for the plan's
@@ -308,7 +310,10 @@ int sde_plan_geometry(sde_plan *plan, int *nu, int *nv, int *fnu, int *uorigin,
                       int *vorigin, float *uscale, float *vscale);
 /* Copy the resident visibility arrays back to host memory (any may be NULL). */
 int sde_plan_get_vis(sde_plan *plan, float *vis, float *wt, float *u, float *v, float *w);
-/* Device pointers of the resident visibility arrays (for synthetic input). */
+/* Device pointers of the resident visibility arrays (for synthetic input).
+ * Writes through them are seen by the next sde_plan_uvmap, vismodel or
+ * uvmap_residual; a write after the call that follows this one needs
+ * sde_plan_device_vis to be called again first. */
 int sde_plan_device_vis(sde_plan *plan, float **vis, float **wt, float **u, float **v,
                         float **w);
 /* Per-kernel timing: enable = 1 records HIP events around every launch on the

@@ -1327,6 +1327,9 @@ struct sde_plan {
     // the cell sort, work items, staged coordinate parts and density weights of
     // the last uvmap stand for the resident uvw and weights (residual imaging)
     bool sorted_ok = false;
+    // with sorted_ok: the hot staging records hold the resident visibilities
+    // (staged by the uvmap or restaged since), so a residual uvmap restages nothing
+    bool hot_current = false;
     bool predicted = false;  // model VIS/WT of the resident visibilities are complete
     bool model_ready = false;  // d_mgrid holds the transform of a model image of the current geometry
     bool imaged = false;     // the images hold a uvmap's output (sums valid)
@@ -1425,6 +1428,7 @@ int plan_reserve_vis(sde_plan *p, int64_t n) {
     p->filled = false;
     p->weighted = false;
     p->sorted_ok = false;
+    p->hot_current = false;
     p->predicted = false;
     if (n <= p->cap) return 0;
     const size_t c = (size_t)n;
@@ -1672,9 +1676,12 @@ int plan_uvmap(sde_plan *p, bool reuse) {
             set_plan_error("sde_plan_uvmap_residual: no standing cell sort (run sde_plan_uvmap on these visibilities)");
             return -1;
         }
-        plan_mark_begin(p, "restage");
-        launch_restage(s, p->d_vis, p->d_wt, p->d_u, p->d_v, p->d_w, n, p->cp, p->sb.hot);
-        plan_mark_end(p);
+        if (!p->hot_current) {
+            plan_mark_begin(p, "restage");
+            launch_restage(s, p->d_vis, p->d_wt, p->d_u, p->d_v, p->d_w, n, p->cp, p->sb.hot);
+            plan_mark_end(p);
+            p->hot_current = true;
+        }
         if (weighting)
             wp = make_weight_params(p->wcp, p->wmode == 3, p->cfg.robust, 4.0f * atanf(1.0f), p->d_gwt, p->d_wsums);
         return plan_image(p, wp);
@@ -1724,6 +1731,7 @@ int plan_uvmap(sde_plan *p, bool reuse) {
     }
     p->weighted = weighting;
     p->sorted_ok = true;
+    p->hot_current = true;
     return plan_image(p, wp);
 }
 
@@ -1949,7 +1957,15 @@ int sde_plan_subtract_model(sde_plan *p) {
         return -1;
     }
     plan_mark_begin(p, "subtract");
-    launch_subtract(p->stream, p->d_vis, p->d_mvis, p->d_mwt, p->nvis);
+    if (p->sorted_ok) {
+        // on a standing cell sort the residual's staging records are written in the same pass
+        launch_subtract_restage(p->stream, p->d_vis, p->d_mvis, p->d_mwt, p->d_wt, p->d_u, p->d_v, p->d_w, p->nvis,
+                                p->cp, p->sb.hot);
+        p->hot_current = true;
+    } else {
+        launch_subtract(p->stream, p->d_vis, p->d_mvis, p->d_mwt, p->nvis);
+        p->hot_current = false;
+    }
     plan_mark_end(p);
     SDE_CHECK(hipGetLastError());
     return 0;
@@ -2415,6 +2431,7 @@ int sde_plan_geometry(sde_plan *p, int *nu, int *nv, int *fnu, int *uorigin, int
 
 int sde_plan_device_vis(sde_plan *p, float **vis, float **wt, float **u, float **v, float **w) {
     if (!p) return -1;
+    p->hot_current = false;  // the caller may write the visibilities through these
     *vis = reinterpret_cast<float *>(p->d_vis);
     *wt = p->d_wt;
     *u = p->d_u;

@@ -291,6 +291,10 @@ void launch_restage(hipStream_t s, const float2 *vis, const float *wt, const flo
                     int64_t n, const CoordParams &cp, StageHot *hot);
 // vis -= mvis where mwt > 0 (residual visibilities).
 void launch_subtract(hipStream_t s, float2 *vis, const float2 *mvis, const float *mwt, int64_t n);
+// launch_subtract and launch_restage of the result in one pass
+void launch_subtract_restage(hipStream_t s, float2 *vis, const float2 *mvis, const float *mwt, const float *wt,
+                             const float *u, const float *v, const float *w, int64_t n, const CoordParams &cp,
+                             StageHot *hot);
 
 // Degridding halves in sorted order from the staging records (when no
 // gridding run wrote them).

@@ -251,3 +251,36 @@ def test_major_cycle_residual_equals_fresh_uvmap(weighting):
     finally:
         a.close()
         b.close()
+
+
+@pytest.mark.parametrize("shifted,handed_out", [(False, True), (True, False), (True, True)])
+def test_major_cycle_restage_paths(shifted, handed_out):
+    """On a standing sort the subtraction writes the residual's staging
+    records in its own pass; once the visibilities' device pointers have been
+    handed out the residual uvmap restages them itself.  Either way, shifted
+    or not, the residual images equal a fresh uvmap of the residual."""
+    d = synth.small_field()
+    shift = imaging.crdshift(30.0, 40.0, 30.01, 40.005) if shifted else None
+    args = dict(weighting=1, fov=1.0, rmode="NORM", shift=shift)
+    a = imaging.Plan(d["nx"], None, d["cellsize"], **args)
+    b = imaging.Plan(d["nx"], None, d["cellsize"], **args)
+    try:
+        a.set_vis(d["vis"], d["wt"], d["u"], d["v"], d["w"])
+        a.uvmap()
+        dirty0, _ = a.images()
+        model = np.where(dirty0 > 0.3 * dirty0.max(), dirty0, 0).astype(np.float32)
+        a.vismodel(model)
+        a.subtract_model()
+        if handed_out:
+            a.device_vis()
+        a.uvmap_residual()
+        r_dirty, r_psf = a.images()
+        res = a.get_vis()[0]
+        b.set_vis(res, d["wt"], d["u"], d["v"], d["w"])
+        b.uvmap()
+        f_dirty, f_psf = b.images()
+        np.testing.assert_array_equal(r_dirty, f_dirty)
+        np.testing.assert_array_equal(r_psf, f_psf)
+    finally:
+        a.close()
+        b.close()
