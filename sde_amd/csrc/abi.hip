@@ -450,9 +450,6 @@ int run_grid(hipStream_t s, const CoordParams &cp, const TileParams &tp, int chm
 // ---------------------------------------------------------------------------
 namespace {
 
-void noop_begin(const char *) {}
-void noop_end() {}
-
 bool check_kernel_args(const char *routine, int nvis, int nu, int nv, int suppu, int suppv, int osampu,
                        int osampv, int uorigin = 1, int vorigin = 1) {
     char msg[160];
