@@ -390,11 +390,24 @@ __global__ __launch_bounds__(256) void k_degrid_slowz_win(
     const bool staged = umin <= umax && (int64_t)wu * wv * NZC <= win_cap;
     if (staged) {
         if (threadIdx.x == 0) atomicAdd(&nbad[1], 1ull);  // workgroups that staged their window
+        // rounds of WQ loads per thread, each round's loads all issued before
+        // any is stored: one memory latency per round, not per entry
+        constexpr int WQ = 8;
         const int n = wu * wv * NZC;
-        for (int k = threadIdx.x; k < n; k += 256) {
-            const int cell = k / NZC, iz = k - cell * NZC;
-            const int r = cell / wu, c = cell - r * wu;
-            win[k] = grid[((size_t)(v0 + r - 1) * cp.nu + (u0 + c - 1)) * NZC + iz];
+        for (int k0 = 0; k0 < n; k0 += WQ * 256) {
+            float2 t[WQ];
+#pragma unroll
+            for (int q = 0; q < WQ; q++) {
+                const int k = k0 + q * 256 + (int)threadIdx.x;
+                const int cell = k / NZC, iz = k - cell * NZC;
+                const int r = cell / wu, c = cell - r * wu;
+                t[q] = k < n ? grid[((size_t)(v0 + r - 1) * cp.nu + (u0 + c - 1)) * NZC + iz] : make_float2(0.0f, 0.0f);
+            }
+#pragma unroll
+            for (int q = 0; q < WQ; q++) {
+                const int k = k0 + q * 256 + (int)threadIdx.x;
+                if (k < n) win[k] = t[q];
+            }
         }
     }
     __syncthreads();
