@@ -95,9 +95,11 @@ __device__ __forceinline__ bool grid_cell(const CoordParams &cp, double ug, doub
     return true;
 }
 
+// cu, cv >= 0 (cells on the grid)
 __device__ __forceinline__ int cell_key(const TileParams &tp, int cu, int cv) {
-    const int tu = cu / tp.T, tv = cv / tp.T;
-    return ((tv * tp.ntu + tu) * tp.T + (cv - tv * tp.T)) * tp.T + (cu - tu * tp.T);
+    const int ts = tile_log2(tp.T), m = tp.T - 1;
+    const int tu = cu >> ts, tv = cv >> ts;
+    return ((((tv * tp.ntu + tu) << ts) | (cv & m)) << ts) | (cu & m);
 }
 
 // GRDH2D's DOUBLE coordinate step and drop test for one point with WT > 0
@@ -114,7 +116,7 @@ __device__ __forceinline__ bool stage_cell(const CoordParams &cp, const TilePara
         const uint32_t local = key & (uint32_t)(tp.T * tp.T - 1);
         const int pu = nint_d((double)cp.osampu * ((double)c.nuc - c.ucell)) + poffu;
         const int pv = nint_d((double)cp.osampv * ((double)c.nvc - c.vcell)) + poffv;
-        sr.gpk = (local % (uint32_t)tp.T) | ((local / (uint32_t)tp.T) << 8) | ((uint32_t)pu << 16) |
+        sr.gpk = (local & (uint32_t)(tp.T - 1)) | ((local >> tile_log2(tp.T)) << 8) | ((uint32_t)pu << 16) |
                  ((uint32_t)pv << 24);
         sr.fr = c.fr;
         sr.fi = c.fi;
@@ -369,7 +371,7 @@ __global__ __launch_bounds__(kThreads) void k_subtract(float2 *__restrict__ vis,
 
 __device__ __forceinline__ int tile_of_key(uint32_t k, uint32_t nkeys, int tt, int ntiles) {
     k &= ~kKeyFlag;
-    return k >= nkeys ? ntiles : (int)(k / (uint32_t)tt);
+    return k >= nkeys ? ntiles : (int)(k >> (31 - __clz(tt)));  // tt = T*T, a power of two
 }
 
 // First sorted slot of every tile (tile_start[ntiles] = number of sorted
@@ -482,8 +484,9 @@ __device__ __forceinline__ void density_add(uint32_t key, double x, uint32_t nke
                                             int suppu, double *__restrict__ gwt) {
     if (key >= nkeys || x == 0.0) return;
     const int tt = tp.T * tp.T;
-    const int tile = (int)(key / (uint32_t)tt), rem = (int)(key - (uint32_t)tile * tt);
-    const int lv = rem / tp.T, lu = rem - lv * tp.T;
+    const int ts = tile_log2(tp.T);
+    const int tile = (int)(key >> (2 * ts)), rem = (int)(key & (uint32_t)(tt - 1));
+    const int lv = rem >> ts, lu = rem & (tp.T - 1);
     const int cu = (tile % tp.ntu) * tp.T + lu, cv = (tile / tp.ntu) * tp.T + lv;
     atomicAdd(&gwt[(size_t)cv * nu + (cu - suppu)], x);
 }
@@ -1259,7 +1262,7 @@ __global__ __launch_bounds__(kThreads) void k_prep_slowz(
                 const uint32_t local = key & (uint32_t)(tp.T * tp.T - 1);
                 const int pu = nint_f((float)cp.osampu * ((float)nuc - ucell)) + poffu;
                 const int pv = nint_f((float)cp.osampv * ((float)nvc - vcell)) + poffv;
-                sr.gpk = (local % (uint32_t)tp.T) | ((local / (uint32_t)tp.T) << 8) | ((uint32_t)pu << 16) |
+                sr.gpk = (local & (uint32_t)(tp.T - 1)) | ((local >> tile_log2(tp.T)) << 8) | ((uint32_t)pu << 16) |
                          ((uint32_t)pv << 24);
                 sr.fr = fr;
                 sr.fi = fi;
@@ -1321,7 +1324,7 @@ __global__ __launch_bounds__(kThreads) void k_prep_box(
             if (ug >= 1 && ug <= cp.nu && vg >= 1 && vg <= cp.nv) {
                 key = (uint32_t)cell_key(tp, ug - 1, vg - 1);
                 const uint32_t local = key & (uint32_t)(tp.T * tp.T - 1);
-                sr.gpk = (local % (uint32_t)tp.T) | ((local / (uint32_t)tp.T) << 8);
+                sr.gpk = (local & (uint32_t)(tp.T - 1)) | ((local >> tile_log2(tp.T)) << 8);
                 sr.fr = fr;
                 sr.fi = fi;
             }

@@ -31,12 +31,17 @@ struct CoordParams {
 
 // Tiling of the pre-shift grid into T x T tiles (padded by kHalo in LDS).
 struct TileParams {
-    int T;          // tile edge
+    int T;          // tile edge: 16 or 32 (a power of two, see tile_log2)
     int ntu, ntv;   // tiles along u and v
     int ntiles;
     int chunk;      // visibilities per work item
     int waves;      // waves per gridding workgroup (4 or 8)
 };
+
+// Every tile edge is a power of two (the plan and batch constructors accept
+// 16 or 32, the drop-ins use 16): the tile coordinates of the non-negative
+// cell indices come by shifts and masks, not by integer division.
+__host__ __device__ __forceinline__ int tile_log2(int T) { return 31 - __builtin_clz((unsigned)T); }
 
 // Gridding record of one sorted point, built and held in registers by the
 // gridding wave (16 bytes, one per lane):
@@ -487,7 +492,7 @@ __device__ __forceinline__ int degrid_half(const CoordParams &cp, const TilePara
     const int nuc = (int)roundf(ucell), nvc = (int)roundf(vcell);
     if (nuc + cp.suppu > cp.nu || nvc + cp.suppv > cp.nv || nvc - cp.suppv < 1) return kDegOff;
     if (key < 0 || nuc <= cp.suppu) return kDegList;
-    const int tile = key / (tp.T * tp.T);
+    const int tile = key >> (2 * tile_log2(tp.T));
     const int tu = tile % tp.ntu, tv = tile / tp.ntu;
     const int lu = (nuc - 1) - (tu * tp.T - cp.suppu - kHaloD);
     const int lv = (nvc - 1) - (tv * tp.T - kHaloD);
