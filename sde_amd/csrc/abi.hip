@@ -523,7 +523,9 @@ int slowz_grid(Ctx &c, const float2 *dvis, const float *dwt, const float *du, co
     sb.item_start = ctx_buf<int>(c, "item_start", tp.ntiles + 1);
     sb.nitems = ctx_buf<int>(c, "nitems", item_scan_ints(tp));
     sb.items = ctx_buf<WorkItem>(c, "items", sb.max_items);
-    float *partial = ctx_buf<float>(c, "partial", grid_partial_floats(tp, 1, sb.max_items));
+    // several planes a pass with the 16-cell tiles and 4 waves of the drop-ins
+    const int npl_max = tp.T == 16 && tp.waves == 4 ? kMaxPlanesPerPass : 1;
+    float *partial = ctx_buf<float>(c, "partial", (size_t)npl_max * grid_partial_floats(tp, 1, sb.max_items));
     double *blk = ctx_buf<double>(c, "blk", 2 * (size_t)sb.max_items), *dsums = ctx_buf<double>(c, "sums", 4);
     double *pblk = ctx_buf<double>(c, "prep_blk", 2 * (size_t)prep_blocks(n > 0 ? n : 1));
     double *dsw = ctx_buf<double>(c, "sumwt", 4);
@@ -537,21 +539,32 @@ int slowz_grid(Ctx &c, const float2 *dvis, const float *dwt, const float *du, co
         set_plan_error("cell sort failed");
         return -1;
     }
-    auto nb = [](const char *) {};
-    auto ne = []() {};
-    // plane 1 gathers the staging records and keeps them in sorted order; the
-    // other planes read that copy
+    // the first pass gathers the staging records and keeps them in sorted
+    // order; the later passes read that copy.  A pass grids up to
+    // kMaxPlanesPerPass planes (4, then 2, then 1 as the planes run out), each
+    // plane's partial tiles one slice of every work item's.
     ZPlanes zp;
     zp.twopi = pp.twopi;
     zp.dtor = pp.dtor;
     zp.cellz = pp.cellz;
     zp.sorted = zsorted;
-    for (int iz = 1; iz <= pp.nz; iz++) {
+    const size_t plane_floats = grid_partial_floats(tp, 1, 1);  // one plane of one work item
+    for (int iz = 1; iz <= pp.nz;) {
+        const int left = pp.nz - iz + 1;
+        const int npl = left >= 4 && npl_max >= 4 ? 4 : (left >= 2 && npl_max >= 2 ? 2 : 1);
         zp.mode = iz == 1 ? 1 : 2;
-        zp.dz = (float)iz - pp.refz;
-        if (run_grid(s, cp, tp, 1, false, WeightParams(), dcfu, dcfv, sb, nullptr, partial, blk, dsums,
-                     dg + (size_t)(iz - 1) * np, nullptr, nb, ne, &zp))
+        zp.npl = npl;
+        for (int q = 0; q < npl; q++) zp.dz[q] = (float)(iz + q) - pp.refz;
+        if (launch_grid_tiles(s, sb, tp, 1, false, WeightParams(), dcfu, dcfv, cp, nullptr, blk, dsums, partial,
+                              &zp)) {
+            set_plan_error("unsupported w-plane gridding run (T=%d, %d planes a pass)", tp.T, npl);
             return -1;
+        }
+        for (int q = 0; q < npl; q++)
+            launch_grid_reduce(s, partial + (size_t)q * plane_floats, sb.item_start, tp, 1, cp, dsums,
+                               dg + (size_t)(iz - 1 + q) * np, nullptr, npl);
+        SDE_CHECK(hipGetLastError());
+        iz += npl;
     }
     double sums[3], sw[3];
     SDE_CHECK(hipMemcpyAsync(sums, dsums, sizeof sums, hipMemcpyDeviceToHost, s));

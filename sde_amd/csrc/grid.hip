@@ -699,12 +699,14 @@ constexpr int kRecTaps = 2 * kMaxTaps + 1;
 // ZP (w planes, GRDH23D): the records are rotated by the plane's phase as
 // they are built (grd/grdh23d.f:154-156); ZP = 1 also writes the gathered
 // staging records in sorted order, ZP = 2 reads them from there instead of
-// gathering.
+// gathering.  NPL planes per pass (w planes, dirty channel): each record's
+// kernel values, cell runs and table serve all of them, each plane keeps its
+// own run partials and LDS tile and adds in the single-plane order.
 // Records per FP32 partial of a run: each partial's rounding error stays
 // within about kRunPart * 2^-24 of its size.
 constexpr int kRunPart = 8;
 
-template <int T, int CHM, int W, bool DEG, int WGT, int ZP>
+template <int T, int CHM, int W, bool DEG, int WGT, int ZP, int NPL = 1>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) void k_grid_tiles(
     const uint64_t *__restrict__ svals, const StageHot *__restrict__ hot, const StageCold *__restrict__ cold,
     int cold_all, const WorkItem *__restrict__ items, const int *__restrict__ nitems, const float *__restrict__ cfu_g,
@@ -714,11 +716,13 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
     constexpr int NC = P * P;
     constexpr int NCH = ((CHM & 1) ? 2 : 0) + ((CHM & 2) ? 1 : 0);
     constexpr int PSF = (CHM & 1) ? 2 : 0;
+    constexpr int PL = NCH * NC;  // one plane's tile
+    static_assert(NPL == 1 || (ZP != 0 && CHM == 1 && !DEG && WGT == 0), "several planes: w planes, dirty only");
     extern __shared__ double smem_d[];
     __shared__ double s_red[2 * W];
-    double *S = smem_d;                          // [NCH][NC] FP64 sums
+    double *S = smem_d;                          // [NPL][NCH][NC] FP64 sums
     // with the same kernel on both axes (same_uv) one set of tables serves both
-    double *tsu = smem_d + NCH * NC;             // sum of the taps of each phase
+    double *tsu = smem_d + NPL * PL;             // sum of the taps of each phase
     double *tsv = same_uv ? tsu : tsu + cp.osampu + 1;
     float *cfl = (float *)(tsu + (cp.osampu + 1) + (same_uv ? 0 : cp.osampv + 1));  // one row per tap
     float *rtab = cfl + cf_table_floats(cp.suppu, cp.osampu, cp.suppv, cp.osampv, same_uv);  // [W][64][kRecTaps]
@@ -746,7 +750,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
             const int o = k / (cp.osampv + 1), ph = k - o * (cp.osampv + 1);
             cflv[o * rsv + ph] = cfv_g[basev + ph + cp.osampv * o];
         }
-    for (int k = threadIdx.x; k < NCH * NC; k += 64 * W) S[k] = 0.0;
+    for (int k = threadIdx.x; k < NPL * PL; k += 64 * W) S[k] = 0.0;
     __syncthreads();
     // the normalisation's sum over taps of one phase, in the reference's order
     for (int k = threadIdx.x; k <= cp.osampu; k += 64 * W) {
@@ -785,8 +789,14 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
     const int r0 = wi.begin + (int)(((int64_t)len * wave) / W);
     const int r1 = wi.begin + (int)(((int64_t)len * (wave + 1)) / W);
     double sw = 0.0, ls = 0.0;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;  // the run's current partial (at most kRunPart records)
+    // per plane: the run's FP64 sums and its current FP32 partial (at most kRunPart records)
+    double s0[NPL], s1[NPL], s2 = 0.0;
+    float f0[NPL], f1[NPL], f2 = 0.0f;
+#pragma unroll
+    for (int q = 0; q < NPL; q++) {
+        s0[q] = s1[q] = 0.0;
+        f0[q] = f1[q] = 0.0f;
+    }
     int cur = -1;
     if (r0 < r1) {  // wave-uniform; lanes past the end load the wave's last slot
         uint64_t v0 = svals[min(r0 + lane, r1 - 1)];
@@ -826,25 +836,29 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
             GridRec mine;
             mine.pk = st0.gpk & ~kStageFlags;
             mine.wt = 0.0f;
-            mine.re = 0.0f;
-            mine.im = 0.0f;
+            float mre[NPL], mim[NPL];  // FVIS (rotated to each plane)
+#pragma unroll
+            for (int q = 0; q < NPL; q++) mre[q] = mim[q] = 0.0f;
             if (wti > 0.0f && valid) {
                 sw += (double)wti;
                 ls += (double)wti * tsu[(mine.pk >> 16) & 0xff] * tsv[mine.pk >> 24];
                 mine.wt = wti;
-                float fr = st0.fr, fi = st0.fi;
-                if (ZP) {
-                    // ROTFVIS = FVIS * CMPLX(COS(PHASE), -SIN(PHASE))
-                    const float phase = -(zp.twopi * __uint_as_float(st0.dpk) * zp.dtor * zp.cellz * zp.dz);
-                    const float cr = cosf(phase), ci = -sinf(phase);
-                    const float a = fr, b2 = fi;
-                    fr = a * cr - b2 * ci;
-                    fi = a * ci + b2 * cr;
+#pragma unroll
+                for (int q = 0; q < NPL; q++) {
+                    float fr = st0.fr, fi = st0.fi;
+                    if (ZP) {
+                        // ROTFVIS = FVIS * CMPLX(COS(PHASE), -SIN(PHASE))
+                        const float phase = -(zp.twopi * __uint_as_float(st0.dpk) * zp.dtor * zp.cellz * zp.dz[q]);
+                        const float cr = cosf(phase), ci = -sinf(phase);
+                        const float a = fr, b2 = fi;
+                        fr = a * cr - b2 * ci;
+                        fi = a * ci + b2 * cr;
+                    }
+                    // the weight is carried by the tap table's v values: a tap
+                    // adds WT*CF to the PSF and WT*CF*FVIS to the dirty grid
+                    mre[q] = fr;
+                    mim[q] = fi;
                 }
-                // the weight is carried by the tap table's v values: a tap
-                // adds WT*CF to the PSF and WT*CF*FVIS to the dirty grid
-                mine.re = fr;
-                mine.im = fi;
             }
             if (ZP == 1 && valid) zp.sorted[p] = st0;
             if (DEG && valid) {
@@ -888,36 +902,55 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
                     if (cur >= 0 && active) {
                         const int a = (cur >> 8) * P + (cur & 0xff) + lane_off;
                         if (CHM & 1) {
-                            atomicAdd(&S[a], s0 + (double)f0);
-                            atomicAdd(&S[NC + a], s1 + (double)f1);
+#pragma unroll
+                            for (int q = 0; q < NPL; q++) {
+                                atomicAdd(&S[q * PL + a], s0[q] + (double)f0[q]);
+                                atomicAdd(&S[q * PL + NC + a], s1[q] + (double)f1[q]);
+                            }
                         }
                         if (CHM & 2) atomicAdd(&S[PSF * NC + a], s2 + (double)f2);
                     }
                     cur = __builtin_amdgcn_readlane(mycell, j);
-                    s0 = s1 = s2 = 0.0;
-                    f0 = f1 = f2 = 0.0f;
+#pragma unroll
+                    for (int q = 0; q < NPL; q++) {
+                        s0[q] = s1[q] = 0.0;
+                        f0[q] = f1[q] = 0.0f;
+                    }
+                    s2 = 0.0;
+                    f2 = 0.0f;
                 }
                 const float cw = ta * tb;  // WT * CF(u) * CF(v)
                 ta = tan;
                 tb = tbn;
                 if (CHM & 1) {
-                    const float re = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.re), j));
-                    const float im = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.im), j));
-                    f0 = fmaf(cw, re, f0);
-                    f1 = fmaf(cw, im, f1);
+#pragma unroll
+                    for (int q = 0; q < NPL; q++) {
+                        const float re = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mre[q]), j));
+                        const float im = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mim[q]), j));
+                        f0[q] = fmaf(cw, re, f0[q]);
+                        f1[q] = fmaf(cw, im, f1[q]);
+                    }
                 }
                 if (CHM & 2) f2 = f2 + cw;
                 if ((j & (kRunPart - 1)) == kRunPart - 1) {
-                    s0 += (double)f0;
-                    s1 += (double)f1;
+#pragma unroll
+                    for (int q = 0; q < NPL; q++) {
+                        s0[q] += (double)f0[q];
+                        s1[q] += (double)f1[q];
+                        f0[q] = f1[q] = 0.0f;
+                    }
                     s2 += (double)f2;
-                    f0 = f1 = f2 = 0.0f;
+                    f2 = 0.0f;
                 }
             }
-            s0 += (double)f0;
-            s1 += (double)f1;
+#pragma unroll
+            for (int q = 0; q < NPL; q++) {
+                s0[q] += (double)f0[q];
+                s1[q] += (double)f1[q];
+                f0[q] = f1[q] = 0.0f;
+            }
             s2 += (double)f2;
-            f0 = f1 = f2 = 0.0f;
+            f2 = 0.0f;
             // the table is rewritten by the next batch
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -930,8 +963,11 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
     if (cur >= 0 && active) {
         const int a = (cur >> 8) * P + (cur & 0xff) + lane_off;
         if (CHM & 1) {
-            atomicAdd(&S[a], s0);
-            atomicAdd(&S[NC + a], s1);
+#pragma unroll
+            for (int q = 0; q < NPL; q++) {
+                atomicAdd(&S[q * PL + a], s0[q]);
+                atomicAdd(&S[q * PL + NC + a], s1[q]);
+            }
         }
         if (CHM & 2) atomicAdd(&S[PSF * NC + a], s2);
     }
@@ -954,8 +990,8 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
         item_sums[2 * item] = a;
         item_sums[2 * item + 1] = b;
     }
-    float *out = partial + (size_t)item * NCH * NC;
-    for (int k = threadIdx.x; k < NCH * NC; k += 64 * W) out[k] = (float)S[k];
+    float *out = partial + (size_t)item * NPL * PL;
+    for (int k = threadIdx.x; k < NPL * PL; k += 64 * W) out[k] = (float)S[k];
 }
 
 // The gridding tile kernel's record access alone: the sorted values read
@@ -978,14 +1014,16 @@ struct Sum3 {
     double re, im, p;
 };
 
+// pstride: floats per work item in partial (its channels, of one or more planes)
 __device__ __forceinline__ Sum3 gather_cell(const float *__restrict__ partial,
                                             const int *__restrict__ item_start, int cu, int cv,
-                                            int T, int ntu, int ntv, int nch, int chmask) {
+                                            int T, int ntu, int ntv, int chmask, int pstride) {
     const int P = T + 2 * kHalo, NC = P * P;
     const int psf = (chmask & 1) ? 2 : 0;
     Sum3 s = {0.0, 0.0, 0.0};
-    const int tu0 = cu / T, tv0 = cv / T;
-    const int lu0 = cu - tu0 * T, lv0 = cv - tv0 * T;
+    const int ts = tile_log2(T);
+    const int tu0 = cu >> ts, tv0 = cv >> ts;  // cu, cv >= 0
+    const int lu0 = cu & (T - 1), lv0 = cv & (T - 1);
     for (int b = -1; b <= 1; b++) {
         const int tv = tv0 + b, lv = lv0 - b * T;
         if (tv < 0 || tv >= ntv || lv < -kHalo || lv >= T + kHalo) continue;
@@ -996,7 +1034,7 @@ __device__ __forceinline__ Sum3 gather_cell(const float *__restrict__ partial,
             const int i0 = item_start[t], i1 = item_start[t + 1];
             const int idx = (lv + kHalo) * P + (lu + kHalo);
             for (int it = i0; it < i1; it++) {
-                const float *pp = partial + (size_t)it * nch * NC + idx;
+                const float *pp = partial + (size_t)it * pstride + idx;
                 if (chmask & 1) {
                     s.re += (double)pp[0];
                     s.im += (double)pp[NC];
@@ -1012,16 +1050,15 @@ __device__ __forceinline__ Sum3 gather_cell(const float *__restrict__ partial,
 // tiles: FP64 sum of the padded tiles covering it, the Hermitian fold of the
 // u = 0 column, the shift left by SUPPU, the normalisation sums[2].
 __device__ __forceinline__ void reduce_cell(const float *__restrict__ partial, const int *__restrict__ item_start,
-                                            int T, int ntu, int ntv, int chmask, const CoordParams &cp,
+                                            int pstride, int T, int ntu, int ntv, int chmask, const CoordParams &cp,
                                             const double *__restrict__ sums, int iu, int iv,
                                             float2 *__restrict__ dirty, float2 *__restrict__ psf) {
-    const int nch = ((chmask & 1) ? 2 : 0) + ((chmask & 2) ? 1 : 0);
     double dr = 0.0, di = 0.0, pr = 0.0;
     if (iu + 1 <= cp.nu - cp.suppu) {
         const int pc = iu + 1 + cp.suppu;  // 1-based pre-shift column
         const int vrow = iv + 1;
         const int c = cp.uorigin + cp.suppu, r = cp.vorigin;
-        Sum3 s = gather_cell(partial, item_start, pc - 1, vrow - 1, T, ntu, ntv, nch, chmask);
+        Sum3 s = gather_cell(partial, item_start, pc - 1, vrow - 1, T, ntu, ntv, chmask, pstride);
         const int offv = vrow - r;
         const bool fold70 = pc > c && pc <= c + cp.suppu && offv >= -cp.nv / 2 + 1 && offv <= cp.nv / 2 - 1;
         const bool fold80 = pc == c && vrow >= r - (cp.nv / 2 - 1) && vrow <= r + cp.nv / 2 - 1;
@@ -1029,7 +1066,7 @@ __device__ __forceinline__ void reduce_cell(const float *__restrict__ partial, c
             const int mc = fold70 ? 2 * c - pc : c;
             const int mr = 2 * r - vrow;
             if (mc >= 1 && mc <= cp.nu && mr >= 1 && mr <= cp.nv) {
-                Sum3 m = gather_cell(partial, item_start, mc - 1, mr - 1, T, ntu, ntv, nch, chmask);
+                Sum3 m = gather_cell(partial, item_start, mc - 1, mr - 1, T, ntu, ntv, chmask, pstride);
                 s.re += m.re;
                 s.im -= m.im;
                 s.p += m.p;
@@ -1052,7 +1089,7 @@ __device__ __forceinline__ void reduce_cell(const float *__restrict__ partial, c
 // a few neighbouring tiles only; bands of tile rows are dealt round the
 // XCDs (the work per band varies with the uv density).
 __global__ __launch_bounds__(256) void k_grid_reduce(
-    const float *__restrict__ partial, const int *__restrict__ item_start, TileParams tp,
+    const float *__restrict__ partial, const int *__restrict__ item_start, int pstride, TileParams tp,
     int chmask, CoordParams cp, const double *__restrict__ sums, float2 *__restrict__ dirty,
     float2 *__restrict__ psf) {
     const int nbx = tp.ntu + 1;
@@ -1063,7 +1100,7 @@ __global__ __launch_bounds__(256) void k_grid_reduce(
         const int lv = k / T, lu = k - lv * T;
         const int iu = bx * T - cp.suppu + lu, iv = by * T + lv;
         if (iu < 0 || iu >= cp.nu || iv >= cp.nv) continue;
-        reduce_cell(partial, item_start, T, tp.ntu, tp.ntv, chmask, cp, sums, iu, iv, dirty, psf);
+        reduce_cell(partial, item_start, pstride, T, tp.ntu, tp.ntv, chmask, cp, sums, iu, iv, dirty, psf);
     }
 }
 
@@ -1196,8 +1233,9 @@ __global__ __launch_bounds__(256) void k_grid_reduce_fields(const float *__restr
     const int64_t c = (int64_t)(b - fd.rblock_begin) * 256 + threadIdx.x;
     if (c >= (int64_t)cp.nu * cp.nv) return;
     const int iv = (int)(c / cp.nu), iu = (int)(c - (int64_t)iv * cp.nu);
-    reduce_cell(partial, item_start + fd.tile_begin, T, fd.ntu, fd.ntv, chmask, cp, fd.sums, iu, iv, fd.dirty,
-                fd.psf);
+    const int nch = ((chmask & 1) ? 2 : 0) + ((chmask & 2) ? 1 : 0), P = T + 2 * kHalo;
+    reduce_cell(partial, item_start + fd.tile_begin, nch * P * P, T, fd.ntu, fd.ntv, chmask, cp, fd.sums, iu, iv,
+                fd.dirty, fd.psf);
 }
 
 // ---- w planes (GRDH23D) ----
@@ -1700,7 +1738,7 @@ size_t grid_partial_floats(const TileParams &tp, int chmask, int max_items) {
     return (size_t)max_items * nch * P * P;
 }
 
-template <int T, int CHM, int W, bool DEG, int WGT, int ZP>
+template <int T, int CHM, int W, bool DEG, int WGT, int ZP, int NPL = 1>
 static void launch_tiles_t(hipStream_t s, const SortBufs &b, const TileParams &tp, const WeightParams &wp,
                            const float *cfu, const float *cfv, const CoordParams &cp, DegridRec *rec_d,
                            double *item_sums, float *partial, const ZPlanes &zp) {
@@ -1709,9 +1747,9 @@ static void launch_tiles_t(hipStream_t s, const SortBufs &b, const TileParams &t
     const int same_uv = cfu == cfv && cp.suppu == cp.suppv && cp.osampu == cp.osampv;
     const size_t tables = (size_t)cf_table_floats(cp.suppu, cp.osampu, cp.suppv, cp.osampv, same_uv);
     const size_t sums = (size_t)cp.osampu + 1 + (same_uv ? 0 : (size_t)cp.osampv + 1);
-    const size_t lds = sizeof(double) * ((size_t)NCH * P * P + sums) +
+    const size_t lds = sizeof(double) * ((size_t)NPL * NCH * P * P + sums) +
                        sizeof(float) * (tables + (size_t)W * 64 * kRecTaps);
-    hipLaunchKernelGGL((k_grid_tiles<T, CHM, W, DEG, WGT, ZP>), dim3(b.max_items), dim3(64 * W), lds, s, b.svals,
+    hipLaunchKernelGGL((k_grid_tiles<T, CHM, W, DEG, WGT, ZP, NPL>), dim3(b.max_items), dim3(64 * W), lds, s, b.svals,
                        b.hot, b.cold, b.cold_all, b.items, b.nitems, cfu, cfv, same_uv, cp, tp, wp, rec_d, item_sums,
                        partial, zp);
 }
@@ -1724,7 +1762,23 @@ static int launch_tiles_w(hipStream_t s, const SortBufs &b, const TileParams &tp
     const int wgt = wp.on ? (wp.az.on ? 2 : 1) : 0;
     if (zp.mode != 0) {
         if (chmask != 1 || deg || wgt != 0 || !zp.sorted) return -1;
-        if (zp.mode == 1)
+        if (zp.npl != 1) {
+            // several planes per pass: the drop-ins' 16-cell tiles and 4 waves only
+            if constexpr (T == 16 && W == 4) {
+                if (zp.npl == 2 && zp.mode == 1)
+                    launch_tiles_t<T, 1, W, false, 0, 1, 2>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial, zp);
+                else if (zp.npl == 2)
+                    launch_tiles_t<T, 1, W, false, 0, 2, 2>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial, zp);
+                else if (zp.npl == 4 && zp.mode == 1)
+                    launch_tiles_t<T, 1, W, false, 0, 1, 4>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial, zp);
+                else if (zp.npl == 4)
+                    launch_tiles_t<T, 1, W, false, 0, 2, 4>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial, zp);
+                else
+                    return -1;
+            } else {
+                return -1;
+            }
+        } else if (zp.mode == 1)
             launch_tiles_t<T, 1, W, false, 0, 1>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial, zp);
         else
             launch_tiles_t<T, 1, W, false, 0, 2>(s, b, tp, wp, cfu, cfv, cp, rec_d, item_sums, partial, zp);
@@ -1764,10 +1818,14 @@ int launch_grid_tiles(hipStream_t s, const SortBufs &b, const TileParams &tp, in
 }
 
 void launch_grid_reduce(hipStream_t s, const float *partial, const int *item_start, const TileParams &tp,
-                        int chmask, const CoordParams &cp, const double *sums, float2 *dirty, float2 *psf) {
+                        int chmask, const CoordParams &cp, const double *sums, float2 *dirty, float2 *psf,
+                        int planes) {
     // every output column's pre-shift column (+ SUPPU) lies in tile columns [0, ntu]
     dim3 grid((unsigned)((tp.ntu + 1) * tp.ntv));
-    hipLaunchKernelGGL(k_grid_reduce, grid, dim3(256), 0, s, partial, item_start, tp, chmask, cp, sums, dirty, psf);
+    const int nch = ((chmask & 1) ? 2 : 0) + ((chmask & 2) ? 1 : 0);
+    const int P = tp.T + 2 * kHalo;
+    hipLaunchKernelGGL(k_grid_reduce, grid, dim3(256), 0, s, partial, item_start, planes * nch * P * P, tp, chmask,
+                       cp, sums, dirty, psf);
 }
 
 void launch_prep_fields(hipStream_t s, const float2 *vis, const float *wt, const float *u, const float *v,

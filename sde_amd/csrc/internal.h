@@ -311,9 +311,14 @@ void launch_degrid_gather(hipStream_t s, const SortBufs &b, int64_t n, const Coo
 // plane's phase -(TWOPI*WLOCAL*DTOR*CELLZ*DZ) while building it.  mode 1
 // gathers the records and writes them to `sorted` in sorted order; mode 2
 // (the following planes) reads them from there, coalesced.
+// w planes of one gridding pass: npl consecutive planes, plane q's offset
+// IZ - REFZ in dz[q] (as the host rounds it)
+constexpr int kMaxPlanesPerPass = 4;
 struct ZPlanes {
     int mode = 0;
-    float twopi = 0.0f, dtor = 0.0f, cellz = 0.0f, dz = 0.0f;
+    int npl = 1;
+    float twopi = 0.0f, dtor = 0.0f, cellz = 0.0f;
+    float dz[kMaxPlanesPerPass] = {0.0f, 0.0f, 0.0f, 0.0f};
     StageHot *sorted = nullptr;
 };
 
@@ -328,8 +333,11 @@ int launch_grid_tiles(hipStream_t s, const SortBufs &b, const TileParams &tp, in
                       const WeightParams &wp, const float *cfu, const float *cfv, const CoordParams &cp,
                       DegridRec *rec_d, double *item_sums, double *sums, float *partial,
                       const ZPlanes *zp = nullptr, int *peak_reset = nullptr);
+// planes: the partial tiles of each work item hold that many planes, this
+// plane's at partial (a pass of several w planes)
 void launch_grid_reduce(hipStream_t s, const float *partial, const int *item_start, const TileParams &tp,
-                        int chmask, const CoordParams &cp, const double *sums, float2 *dirty, float2 *psf);
+                        int chmask, const CoordParams &cp, const double *sums, float2 *dirty, float2 *psf,
+                        int planes = 1);
 // The gridding tile kernel's record access alone over the n sorted points
 // (measurement; blk receives per-block sums, 1024 floats).
 void launch_gather_probe(hipStream_t s, const SortBufs &b, int64_t n, float *blk);

@@ -2,12 +2,13 @@
 
     python -m sde_amd.profsum OUT "<command line>" > profiles/<name>.md
 
-OUT holds three rocprofv3 output folders of the same command, and a fourth
-for the LDS counters when present:
+OUT holds three rocprofv3 output folders of the same command, and others
+for the LDS and VALU counters when present:
     OUT/trace  --kernel-trace --stats
     OUT/fetch  --pmc FETCH_SIZE
     OUT/write  --pmc WRITE_SIZE
     OUT/lds    --pmc SQ_INSTS_LDS SQ_LDS_BANK_CONFLICT SQ_LDS_IDX_ACTIVE SQ_LDS_ADDR_CONFLICT
+    OUT/valu   --pmc VALUBusy SQ_INSTS_VALU SQ_INSTS_SALU GRBM_GUI_ACTIVE
 each written with ``-o run`` (rocpd SQLite databases, run_results.db).
 
 FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are calibrated against the
@@ -25,6 +26,13 @@ The LDS table gives, per kernel, the LDS instructions per dispatch, the
 cycles lost to bank conflicts per LDS instruction and the shares of the
 LDS-active cycles (SQ_LDS_IDX_ACTIVE) spent in bank and in address
 conflicts.
+
+The VALU table gives, per kernel, the vector and scalar ALU instructions
+per dispatch (wave instructions), VALUBusy (the share of the kernel's GPU
+time the CUs spend executing VALU instructions, in percent) and the VALU
+instructions issued per CU per cycle of the kernel's GPU time
+(SQ_INSTS_VALU / (CUs * GRBM_GUI_ACTIVE per XCD)); one wave64 FP32
+instruction per CU per cycle is the issue rate of the four SIMDs.
 """
 import collections
 import glob
@@ -81,6 +89,10 @@ def counters(folder):
     return {k: {ctr: sum(v) / len(v) for ctr, v in d.items()} for k, d in acc.items()}
 
 
+CUS = 256  # MI355X: 8 XCDs of 32 CUs
+XCDS = 8
+
+
 def summarize(out, command):
     t = trace(os.path.join(out, "trace"))
     f = counters(os.path.join(out, "fetch"))
@@ -117,6 +129,20 @@ def summarize(out, command):
                 continue
             lines.append("| %s | %.0f | %.0f | %.3f | %.3f | %.3f |"
                          % (k, ni, bc, bc / ni, bc / act if act else 0.0, ac / act if act else 0.0))
+    valu_dir = os.path.join(out, "valu")
+    if os.path.isdir(valu_dir):
+        v = counters(valu_dir)
+        lines += ["", "VALU counters (per dispatch):", "",
+                  "| kernel | valu_instr | salu_instr | valu_busy_pct | valu_per_cu_cycle |",
+                  "|---|---|---|---|---|"]
+        for k, _ in order:
+            c = v.get(k, {})
+            nv, ns = c.get("SQ_INSTS_VALU", 0.0), c.get("SQ_INSTS_SALU", 0.0)
+            busy, gui = c.get("VALUBusy", 0.0), c.get("GRBM_GUI_ACTIVE", 0.0)
+            if nv <= 0:
+                continue
+            ipc = nv / (CUS * gui / XCDS) if gui > 0 else 0.0
+            lines.append("| %s | %.0f | %.0f | %.1f | %.3f |" % (k, nv, ns, busy, ipc))
     return "\n".join(lines) + "\n"
 
 
@@ -125,8 +151,14 @@ def read_table(path):
     out = {}
     if not os.path.exists(path):
         return out
+    seen = False  # the first table only
     with open(path) as fh:
         for line in fh:
+            if not line.startswith("|"):
+                if seen:
+                    break
+                continue
+            seen = True
             p = [x.strip() for x in line.strip().strip("|").split("|")]
             if len(p) < 7 or p[0] in ("kernel", "---") or p[0].startswith("---"):
                 continue
