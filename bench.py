@@ -34,16 +34,18 @@ PEAK_HBM_GBS = 8000.0
 
 # Kernels launched inside each timed region of the plan (rocprofv3 names;
 # a name ending in "*" matches every kernel with that prefix).
+# kernel names as the profile summaries print them; "name*" matches a prefix
 REGION_KERNELS = {
     "prep": ["k_prep<2, true>"],
-    "cell_sort": ["k_sort_count*", "k_sort_chunk", "k_sort_chunk_scan", "k_sort_scatter"],
-    # the density runs also write the tile ranges; the work items follow in the same region
-    "wt_density": ["k_density_runs", "k_items_count", "k_items_blocks", "k_items_write"],
+    # the sort, then the tile ranges and the work items
+    "cell_sort": ["k_sort_count*", "k_sort_chunk", "k_sort_chunk_scan", "k_sort_scatter*", "k_tile_bounds",
+                  "k_items_count", "k_items_blocks", "k_items_write"],
+    "wt_density": ["k_density_runs"],
     "wt_finish": ["k_wt_sumsq", "k_wt_finish"],
-    "grid_tiles": ["k_grid_tiles<16, 3, 4, true, 1, 0>", "k_grid_totals"],
+    "grid_tiles": ["k_grid_tiles<16, 3, 4, true, 1, 0*", "k_grid_totals"],
     "grid_reduce": ["k_grid_reduce"],
     # the tile degridder writes its results into output bins; the scatter puts them in input order
-    "degrid": ["k_degrid_tiles<16, 3, true>", "k_degrid_scatter"],
+    "degrid": ["k_degrid_tiles<16, 3, true*", "k_degrid_scatter"],
     "degrid_axis": ["k_degrid_list"],
 }
 
@@ -123,6 +125,28 @@ def read_lds(region, workload_tag):
             if any(p[0] == k or (k.endswith("*") and p[0].startswith(k[:-1])) for k in names):
                 out[p[0]] = {"conflict_cycles_per_lds_instr": float(p[3]), "bank_conflict_share": float(p[4]),
                              "addr_conflict_share": float(p[5])}
+    return out or None
+
+
+def read_valu(region, workload_tag):
+    """VALU issue figures of a region's kernels from the committed rocprofv3
+    summary (its VALU table): VALUBusy in percent and VALU instructions per
+    CU per cycle (1 is the issue rate), per kernel."""
+    path = os.path.join(ROOT, "profiles", "kernels_%s.md" % workload_tag)
+    names = REGION_KERNELS.get(region)
+    if not names or not os.path.exists(path):
+        return None
+    out, in_valu = {}, False
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("| kernel | valu_instr"):
+                in_valu = True
+                continue
+            p = [x.strip() for x in line.strip().strip("|").split("|")]
+            if not in_valu or len(p) != 5 or p[0].startswith("---"):
+                continue
+            if any(p[0] == k or (k.endswith("*") and p[0].startswith(k[:-1])) for k in names):
+                out[p[0]] = {"valu_busy_pct": float(p[3]), "valu_per_cu_cycle": float(p[4])}
     return out or None
 
 
@@ -673,7 +697,7 @@ def main():
             "alg_bytes": ab,
             "traffic_counts": "FETCH_SIZE + WRITE_SIZE per launch from profiles/kernels_large_field.md, calibrated "
                               "on a device copy: traffic between L2 and the fabric, Infinity Cache hits included",
-            "lds": read_lds(kname, "large_field")}
+            "lds": read_lds(kname, "large_field"), "valu": read_valu(kname, "large_field")}
     if kname == "grid_tiles" and gather_ms:
         # a random 16-byte gather moves a whole line per point, so this kernel's
         # ceiling is the gather's own rate, measured live on the same buffers

@@ -145,3 +145,19 @@ def test_python_mirror_rejects_arrays_of_different_lengths():
     with pytest.raises(ValueError, match="different lengths"):
         _same_length(np.zeros(5, np.complex64), a, a, b, a)
     assert grd._nvis(a, a, a) == 5
+
+
+def test_bench_finds_every_region_in_the_committed_profile():
+    """The bench line's traffic, LDS and VALU figures come from the committed
+    kernel summary by kernel name: every timed region of the 4096^2 job must
+    find its kernels there (a renamed or re-templated kernel would otherwise
+    leave the figures null)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_module", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    for region in ("prep", "cell_sort", "wt_density", "grid_tiles", "grid_reduce", "degrid", "degrid_axis"):
+        assert bench.read_traffic(region, "large_field"), region
+    for region in ("prep", "grid_tiles", "degrid"):
+        assert bench.read_lds(region, "large_field"), region
+        assert bench.read_valu(region, "large_field"), region
