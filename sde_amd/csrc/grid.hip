@@ -8,10 +8,12 @@
 //                      degridding records that does not depend on the
 //                      density weights goes into a 16-byte hot staging
 //                      record (and, where needed, a 16-byte cold one).
-//   2. radix sort      (key, index|weight) pairs, stable (sort.hip); the
-//                      density runs or k_tile_bounds give the tile ranges,
-//                      k_items_count/_blocks/_write the work items (a
-//                      tile's slots cut into chunks).
+//   2. radix sort      (key, index|weight) pairs, stable (sort.hip);
+//                      k_tile_bounds gives the tile ranges (a binary search
+//                      per tile), k_items_count/_blocks/_write the work
+//                      items (a tile's slots cut into chunks).  With density
+//                      weighting, k_density_runs then sums the sorted
+//                      weights over the runs of equal cells.
 //   3. k_grid_tiles    one workgroup per work item over a padded tile
 //                      (T+6)^2 held in LDS in FP64.  Each wave gathers the
 //                      staging records of a contiguous run of cell-sorted
