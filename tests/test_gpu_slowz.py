@@ -43,10 +43,14 @@ def _shift(shifted):
     return np.asarray(po.crdshift(0.3, 0.6, 0.3002, 0.6001), np.float64).reshape(-1) if shifted else IDENT
 
 
+# the gridding takes up to four planes a pass: 3 planes are passes of 2 and 1,
+# 5 of 4 and 1, 7 of 4, 2 and 1 (the first pass gathers, the later ones read
+# its sorted copy)
+@pytest.mark.parametrize("nz", [3, 5, 7])
 @pytest.mark.parametrize("psf", [False, True])
 @pytest.mark.parametrize("shifted", [False, True])
-def test_grdh23d_matches_fp64(psf, shifted):
-    nu, nv, vo, nz, cellz, refz = 65, 128, 64, 5, 3.0, 2.5
+def test_grdh23d_matches_fp64(psf, shifted, nz):
+    nu, nv, vo, cellz, refz = 65, 128, 64, 3.0, 2.5
     vis, wt, u, v, w = _points(4000, 41, 40, 55, near_axis=400)
     cf, supp, osamp = po.grdcf("SF")
     cf64, _, _ = po.grdcf("SF", fp64=True)
