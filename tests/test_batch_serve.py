@@ -223,13 +223,14 @@ def test_uvmap_from_and_to_fits(tmp_path):
     assert info["sumwt"] > 0 and hd["CRVAL1"] == 10.0
 
 
-@pytest.mark.parametrize("weighting", [0, 1])
-def test_major_cycle_residual_equals_fresh_uvmap(weighting):
+# ABS needs R or DS nonzero (GRDRUW2D divides by GWT*R**2 + 2*DS**2)
+@pytest.mark.parametrize("weighting,rmode", [(0, "NORM"), (1, "NORM"), (1, "ABS"), (1, "AZ-NORM")])
+def test_major_cycle_residual_equals_fresh_uvmap(weighting, rmode):
     """Predict a model, subtract it, image the residual on the standing sort:
     identical to a fresh uvmap of the residual visibilities (same sort, same
     weights), and the device residuals equal VIS - MODEL computed on the host."""
     d = synth.small_field()
-    args = dict(weighting=weighting, fov=1.0, rmode="NORM")
+    args = dict(weighting=weighting, fov=1.0, rmode=rmode, robust=(0.5, 0.5, 0.0), ds=0.05)
     a = imaging.Plan(d["nx"], None, d["cellsize"], **args)
     b = imaging.Plan(d["nx"], None, d["cellsize"], **args)
     try:
