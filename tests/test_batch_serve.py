@@ -152,6 +152,43 @@ def test_vismodel_from_needs_a_transformed_source():
             p.close()
 
 
+@pytest.mark.parametrize("seed", range(12))
+def test_major_cycle_random(seed):
+    """Seeded sweep of two major cycles per configuration: weighting mode,
+    shift or none, a lower model threshold on the second cycle, and on some
+    seeds the visibilities' device pointers handed out before the second
+    residual (which then restages them): each residual image equals a fresh
+    uvmap of the residual visibilities bit for bit."""
+    d = synth.small_field(seed=100 + seed)
+    shift = imaging.crdshift(30.0, 40.0, 30.0 + 0.01 * (seed % 3), 40.0 - 0.005 * (seed % 2)) if seed % 2 else None
+    rmode = ["NONE", "NORM", "ABS", "AZ-NORM"][seed % 4]
+    args = dict(weighting=0 if rmode == "NONE" else 1, fov=1.0, rmode=rmode, shift=shift,
+                robust=(0.5, 0.5, 0.0), ds=0.05)
+    a = imaging.Plan(d["nx"], None, d["cellsize"], **args)
+    b = imaging.Plan(d["nx"], None, d["cellsize"], **args)
+    try:
+        a.set_vis(d["vis"], d["wt"], d["u"], d["v"], d["w"])
+        a.uvmap()
+        dirty0, _ = a.images()
+        for cycle in range(2):
+            model = np.where(dirty0 > (0.5 - 0.2 * cycle) * dirty0.max(), dirty0, 0).astype(np.float32)
+            a.vismodel(model)
+            a.subtract_model()
+            if seed % 3 == 0 and cycle == 1:
+                a.device_vis()
+            a.uvmap_residual()
+            r_dirty, r_psf = a.images()
+            b.set_vis(a.get_vis()[0], d["wt"], d["u"], d["v"], d["w"])
+            b.uvmap()
+            f_dirty, f_psf = b.images()
+            np.testing.assert_array_equal(r_dirty, f_dirty, err_msg="cycle %d" % cycle)
+            np.testing.assert_array_equal(r_psf, f_psf, err_msg="cycle %d" % cycle)
+            dirty0 = r_dirty
+    finally:
+        a.close()
+        b.close()
+
+
 def _small_requests(n=10, seed=8):
     reqs = synth.served_requests(n, rate_hz=2000.0, seed=seed, sizes=(128, 256, 512), median=3e4, sigma=0.7,
                                  nvis_max=200_000)

@@ -69,6 +69,31 @@ def test_grdh23d_matches_fp64(psf, shifted, nz):
     print("rel err gpu %.2e ref32 %.2e" % (np.abs(g - g64).max() / scale, np.abs(g32 - g64).max() / scale))
 
 
+@pytest.mark.parametrize("seed", range(36))
+def test_grdh23d_random_planes(seed):
+    """Seeded sweep: 1-9 planes (every split into passes of 4, 2 and 1),
+    PSF or dirty, shifted or not, random CELLZ and REFZ, against FP64."""
+    rng = np.random.default_rng(seed)
+    nz = int(1 + seed % 9)
+    psf, shifted = bool(seed % 2), seed % 3 == 0
+    nu, nv, vo = 65, 128, 64
+    cellz, refz = float(rng.uniform(0.5, 3.0)), float(rng.uniform(1.0, nz))
+    vis, wt, u, v, w = _points(int(rng.integers(1000, 6000)), 500 + seed, 40, 55, near_axis=300)
+    cf, supp, osamp = po.grdcf("SF")
+    cf64, _, _ = po.grdcf("SF", fp64=True)
+    sh = _shift(shifted)
+    print("grdh23d seed %d nz=%d psf=%s shifted=%s n=%d" % (seed, nz, psf, shifted, len(u)), flush=True)
+    g, sw = grd.grdh23d(vis, wt, u, v, w, 1.0, 0.0, 1.0, 0.0, 1, vo, psf, nu, nv, cf, supp, osamp, nz, cellz, refz,
+                        shift=sh)
+    g64, sw64, rc = po.grdh23d64(vis, wt, u, v, w, 1.0, 1.0, 1, vo, psf, nu, nv, cf64, supp, osamp, nz, cellz, refz,
+                                 sh)
+    assert rc == 0
+    assert sw == pytest.approx(sw64, rel=1e-6)
+    for iz in range(nz):
+        assert_close_rel(g[iz], g64[iz], "plane %d" % (iz + 1),
+                         rel=1e-5 * np.abs(g64).max() / max(np.abs(g64[iz]).max(), 1e-30))
+
+
 def test_grdh23d_larger_grid_and_drops():
     nu, nv, vo, nz = 257, 512, 256, 8
     vis, wt, u, v, w = _points(60000, 42, 270, 270, near_axis=2000)  # some footprints leave the grid

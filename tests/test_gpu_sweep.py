@@ -26,7 +26,7 @@ pytestmark = pytest.mark.gpu
 sde = pytest.importorskip("sde_amd")
 from sde_amd import grd  # noqa: E402
 
-SEEDS = list(range(24))
+SEEDS = list(range(120))
 
 
 def config(seed):
@@ -95,7 +95,7 @@ def test_grdh2d_and_grddeh2d_random_configuration(seed):
     assert_close_rel(vg[good], v64[good], "seed %d degrid vs FP64" % seed)
 
 
-@pytest.mark.parametrize("seed", SEEDS[:12])
+@pytest.mark.parametrize("seed", SEEDS[:48])
 def test_grdqh2d_random_configuration(seed):
     c = config(seed)
     args = (c["us"], c["uo"], c["vs"], c["vo"], 1, c["vorigin"])
