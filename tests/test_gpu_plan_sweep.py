@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as po
-from tests.tolerance import assert_image
+from tests.tolerance import assert_image, assert_vis_rel_ref
 
 pytestmark = pytest.mark.gpu
 
@@ -80,3 +80,40 @@ def test_plan_uvmap_random_configuration(seed):
     assert info["sumwt"] == pytest.approx(sw64, rel=1e-6)
     assert_image(dirty, d32, d64, "seed %d dirty" % seed)
     assert_image(psf, p32, p64, "seed %d psf" % seed)
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_plan_vismodel_random_configuration(seed):
+    """Prediction from a random model image on the same configurations: WT
+    (the success flags) identical to the REAL restatement of the reference,
+    VIS by the reference-relative rule of predictions; a second prediction
+    on the same cell sort (slots reused) identical to the first."""
+    c = plan_config(seed)
+    rng = np.random.default_rng(9000 + seed)
+    img = np.zeros((c["ny"], c["nx"]), np.float32)
+    k = int(rng.integers(5, 40))
+    img[rng.integers(c["ny"] // 4, 3 * c["ny"] // 4, k), rng.integers(c["nx"] // 4, 3 * c["nx"] // 4, k)] = \
+        rng.uniform(0.1, 1.0, k).astype(np.float32)
+    img += (0.01 * rng.standard_normal(img.shape)).astype(np.float32)
+    ra, dec = 0.5, 0.7
+    ra_img, dec_img = (ra + 2e-4, dec - 1e-4) if c["shifted"] else (ra, dec)
+    s = imaging.crdshift(ra, dec, ra_img, dec_img)
+    print("seed %d nx=%d ny=%d %s shifted=%s %s n=%d" % (seed, c["nx"], c["ny"], c["cftype"], c["shifted"],
+                                                         c["tune"], len(c["u"])), flush=True)
+    p = imaging.Plan(c["nx"], c["ny"], c["cell"], c["cftype"], s, **c["tune"])
+    try:
+        p.set_vis(np.zeros(len(c["u"]), np.complex64), c["wt"], c["u"], c["v"], c["w"])
+        p.vismodel(img)
+        mv, mw = p.model()
+        p.vismodel(img)
+        mv2, mw2 = p.model()
+    finally:
+        p.close()
+    np.testing.assert_array_equal(mv2, mv)
+    np.testing.assert_array_equal(mw2, mw)
+    cfg = po.make_cfg(c["nx"], c["ny"], cellsize=c["cell"], cftype=c["cftype"], ra=ra, dec=dec, ra_img=ra_img,
+                      dec_img=dec_img)
+    v32, w32, _ = po.vismodel(cfg, img, c["wt"], c["u"], c["v"], c["w"])
+    v64, _, _ = po.vismodel(cfg, img.astype(np.float64), c["wt"], c["u"], c["v"], c["w"], fp64=True)
+    np.testing.assert_array_equal(mw, w32)
+    assert_vis_rel_ref(mv, v32, v64, "seed %d" % seed)
