@@ -255,3 +255,45 @@ def test_plan_prep_counted_sort_equals_batch():
         assert fb.info(0)["ngridded"] > 0.99 * n - 1000
     finally:
         fb.close()
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_batch_random_composition(seed):
+    """Seeded sweep of batch compositions: 2-16 fields of 128^2 to 1024^2
+    (some non-square), uneven visibility counts, shifted fields, SF or SZE,
+    16- or 32-cell tiles, 4 or 8 waves, small or large work items; every
+    field bit for bit the plan's, eager and over two graph replays."""
+    rng = np.random.default_rng(4000 + seed)
+    sizes = tuple(sorted(set(int(x) for x in rng.choice([128, 256, 512, 1024], 2))))
+    jobs = synth.mosaic(npoint=int(rng.integers(2, 17)), nvis_range=(3e3, 8e4), sizes=sizes, seed=200 + seed)
+    for j in jobs:
+        if rng.random() < 0.3:
+            j["shift"] = imaging.crdshift(30.0, -20.0, 30.0 + float(rng.uniform(-1e-3, 1e-3)),
+                                          -20.0 + float(rng.uniform(-1e-3, 1e-3)))
+        if rng.random() < 0.2:
+            j["ny"] = j["nx"] // 2
+    tune = dict(tile=int(rng.choice([16, 32])), waves=int(rng.choice([4, 8])),
+                chunk=int(rng.choice([512, 2048, 8192])), cftype="SZE" if seed % 4 == 3 else "SF")
+    print("seed %d fields %d sizes %s %s" % (seed, len(jobs), sizes, tune), flush=True)
+    ref = []
+    for j in jobs:
+        p = imaging.Plan(j["nx"], j.get("ny"), j["cellsize"], shift=j.get("shift"), **tune)
+        try:
+            p.set_vis(j["vis"], j["wt"], j["u"], j["v"], j["w"])
+            p.uvmap()
+            d, q = p.images()
+            ref.append((d, q, p.info()))
+        finally:
+            p.close()
+    fb = batch.FieldBatch(jobs, graph=False, **tune)
+    try:
+        fb.uvmap()
+        fb.sync()
+        _check_equal(fb, ref, "eager")
+        fb.use_graph(True)
+        for rep in range(2):
+            fb.uvmap()
+            fb.sync()
+            _check_equal(fb, ref, "graph replay %d" % rep)
+    finally:
+        fb.close()
