@@ -695,7 +695,8 @@ constexpr int kRecTaps = 2 * kMaxTaps + 1;
 // offsets, no address arithmetic), the cell changes from a ballot (a scalar
 // bit test).
 //
-// Occupancy: 5 waves per SIMD (at most 96 VGPRs, no spills), so the
+// Occupancy: 5 waves per SIMD (at most 96 VGPRs; no spills, but for the
+// four-plane w-plane pass, which keeps two registers in scratch), so the
 // gathers of one wave are covered by the tap loops of the others.
 //
 // ZP (w planes, GRDH23D): the records are rotated by the plane's phase as
