@@ -626,9 +626,21 @@ __global__ __launch_bounds__(1024) void k_grid_totals(const double *__restrict__
     __shared__ double ds[1024], dl[1024];
     const int t = threadIdx.x;
     double a = 0.0, b = 0.0;
-    for (int k = t; k < nblk; k += 1024) {
-        a += blk_sums[2 * k];
-        b += blk_sums[2 * k + 1];
+    // rounds of eight independent loads a thread, added in the same order
+    constexpr int Q = 8;
+    for (int k0 = t; k0 < nblk; k0 += Q * 1024) {
+        double2 x[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int k = k0 + q * 1024;
+            x[q] = k < nblk ? reinterpret_cast<const double2 *>(blk_sums)[k] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            if (k0 + q * 1024 >= nblk) break;
+            a += x[q].x;
+            b += x[q].y;
+        }
     }
     ds[t] = a;
     dl[t] = b;
